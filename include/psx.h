@@ -1,0 +1,331 @@
+/*
+ * psx.h: C ABI of the MI355X (gfx950) mutation-placement and sequencing
+ * library. Plain pointers and sizes only; no exceptions cross this boundary.
+ *
+ * Every entry point returns a psx_status; on failure psx_last_error(ctx)
+ * holds the message (the Rcpp shim rethrows it as an R error, which is how
+ * the reference reports failures: src/seq_simulation.cpp:103,254-259,437-445,
+ * src/sequencers.cpp:86-101).
+ *
+ * One psx_ctx owns one HIP stream and is used by one host thread at a time;
+ * several contexts may run concurrently. Genomes, forests and mutation tables
+ * are opaque device-resident handles that stay in HBM across calls.
+ *
+ * Reference interfaces replaced (paths relative to the ProCESS sources):
+ *   psx_forest_upload      SamplesForest node table, src/forest.hpp:29-67,
+ *                          src/samples_forest.hpp:31-82
+ *   psx_place_mutations    MutationEngine::place_mutations,
+ *                          src/mutation_engine.cpp:1085-1113 (RACES call at
+ *                          :1097-1101), overloads src/mutation_engine.hpp:160-206
+ *   psx_simulate_seq       the ReadSimulator call inside simulate_seq,
+ *                          src/seq_simulation.cpp:354-429 (RACES call at
+ *                          :371-372, :413-414, :423-424)
+ *   psx_seq_table          get_result_dataframe / get_active_mutations /
+ *                          add_sample_statistics, src/seq_simulation.cpp:52-181
+ *   psx_insert_size_check  get_bin_dist, src/seq_simulation.cpp:431-451
+ */
+#ifndef PSX_H
+#define PSX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PSX_ABI_VERSION 1
+
+/* Reads whose template starts fall in one stratum of PSX_STRATUM reference
+ * positions are drawn from one counter range; part of the sampling model. */
+#define PSX_STRATUM 8192u
+/* Largest number of sequencing errors drawn in one read. */
+#define PSX_MAX_READ_ERRORS 16u
+#define PSX_MAX_CHROMOSOMES 64u
+#define PSX_NO_NODE 0xFFFFFFFFu
+#define PSX_NO_CAUSE 0xFFFFu
+
+typedef enum psx_status {
+    PSX_OK = 0,
+    PSX_EINVAL = 1,          /* invalid argument (message in psx_last_error) */
+    PSX_ENOMEM_DEVICE = 2,   /* device allocation failed */
+    PSX_EHIP = 3,            /* HIP runtime error */
+    PSX_ECANCELLED = 4,      /* cancelled between work chunks */
+    PSX_ENOMEM_HOST = 5,
+    PSX_EUNSUPPORTED = 6
+} psx_status;
+
+/* Mutation kinds and natures (the reference's "classes" column:
+ * src/phylogenetic_forest.cpp:342-344). */
+enum { PSX_SNV = 0, PSX_DEL = 1, PSX_INS = 2 };
+enum { PSX_PASSENGER = 0, PSX_DRIVER = 1, PSX_PRENEOPLASTIC = 2, PSX_GERMINAL = 3 };
+
+typedef struct psx_ctx psx_ctx;
+typedef struct psx_genome psx_genome;
+typedef struct psx_forest psx_forest;
+typedef struct psx_mutations psx_mutations;
+
+/* ------------------------------------------------------------------ context */
+
+int psx_abi_version(void);
+psx_status psx_ctx_create(int device, psx_ctx** out);
+void psx_ctx_destroy(psx_ctx* ctx);
+const char* psx_last_error(const psx_ctx* ctx);
+psx_status psx_ctx_synchronize(psx_ctx* ctx);
+/* Milliseconds of the dominant kernel(s) of the last sequencing call, timed
+ * with HIP events on the context stream. */
+psx_status psx_ctx_last_kernel_ms(const psx_ctx* ctx, double* ms);
+/* Free and total HBM of the context device, in bytes. */
+psx_status psx_ctx_mem_info(psx_ctx* ctx, uint64_t* free_bytes, uint64_t* total_bytes);
+
+/* Device buffers on the context device for callers that have no allocator of
+ * their own in this process (no reference counterpart; plumbing for the
+ * device-pointer entry points). Copies are synchronous. */
+psx_status psx_dev_alloc(psx_ctx* ctx, uint64_t bytes, void** out);
+void psx_dev_free(psx_ctx* ctx, void* p);
+psx_status psx_dev_upload(psx_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+psx_status psx_dev_download(psx_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+psx_status psx_dev_fill_u8(psx_ctx* ctx, void* dst, uint8_t value, uint64_t bytes);
+
+/* ------------------------------------------------------------------- genome */
+
+/* Synthetic reference: base i of chromosome c is i.i.d. uniform over ACGT,
+ * drawn from Philox(key = ref_seed). Stored 2-bit packed (A=0 C=1 G=2 T=3,
+ * 16 bases per uint32, base i in bits 2*(i%16)), each chromosome starting on a
+ * fresh word. Replaces the FASTA the reference downloads
+ * (src/mutation_engine.cpp:75-112), which needs network access. */
+psx_status psx_genome_synth(psx_ctx* ctx, uint32_t n_chr, const uint32_t* chr_len,
+                            uint64_t ref_seed, psx_genome** out);
+/* A caller-provided packed reference in the same layout. */
+psx_status psx_genome_upload(psx_ctx* ctx, uint32_t n_chr, const uint32_t* chr_len,
+                             const uint32_t* packed, psx_genome** out);
+/* Words of the packed layout: sum over chromosomes of ceil(len / 16). */
+uint64_t psx_genome_words(uint32_t n_chr, const uint32_t* chr_len);
+psx_status psx_genome_download(psx_ctx* ctx, const psx_genome* g, uint32_t* packed,
+                               uint64_t n_words);
+void psx_genome_free(psx_genome* g);
+
+/* ------------------------------------------------------------------- forest */
+
+/* One row per cell, as SamplesForest$get_nodes() (src/forest.hpp:29-67). */
+typedef struct psx_forest_desc {
+    uint32_t n_nodes;
+    const int64_t* cell_id;
+    const int64_t* parent;    /* ancestor cell id, -1 for a root */
+    const uint32_t* species;  /* species (mutant x epistate) index */
+    const int32_t* sample;    /* sample index of a sampled leaf, -1 otherwise */
+    const double* birth_time;
+    uint32_t n_samples;
+} psx_forest_desc;
+
+/* Flattens the forest into DFS pre-order (roots and children by ascending
+ * cell id) and uploads it. A node's subtree holds the leaves with ranks
+ * [leaf_lo, leaf_hi). */
+psx_status psx_forest_upload(psx_ctx* ctx, const psx_forest_desc* desc, psx_forest** out);
+psx_status psx_forest_info(const psx_forest* f, uint32_t* n_nodes, uint32_t* n_leaves,
+                           uint32_t* n_samples);
+/* Host copies of the flattened arrays (each n_nodes long, except
+ * sample_leaves which is n_leaves long and sample_begin n_samples + 1). */
+psx_status psx_forest_flat(const psx_forest* f, int64_t* dfs_cell_id, uint32_t* parent_dfs,
+                           uint32_t* leaf_lo, uint32_t* leaf_hi, uint32_t* sample_begin,
+                           uint32_t* sample_leaves);
+void psx_forest_free(psx_forest* f);
+
+/* ---------------------------------------------------------------- placement */
+
+typedef struct psx_driver {
+    uint32_t species;  /* the driver is placed on the edge into every node where this
+                          species first appears on a lineage */
+    uint32_t chr;
+    uint32_t pos;      /* 0-based */
+    uint8_t alt;       /* 0..3 */
+    int8_t allele;     /* copy index, -1 = uniform draw */
+    uint8_t pad[2];
+} psx_driver;
+
+typedef struct psx_germline {
+    uint32_t chr;
+    uint32_t pos;
+    uint8_t alt;
+    uint8_t allele;
+    uint8_t pad[2];
+} psx_germline;
+
+typedef struct psx_engine_params {
+    uint32_t n_species;
+    const double* snv_rate;     /* [n_species] SNVs per base per division */
+    const double* indel_rate;   /* [n_species] indels per base per division */
+    uint32_t n_sbs;             /* SBS signatures, 96 classes each, COSMIC order */
+    const double* sbs;          /* [n_sbs][96] */
+    uint32_t n_epochs;          /* timed exposures (MutationEngine$add_exposure) */
+    const double* epoch_start;  /* [n_epochs], ascending */
+    const double* exposure;     /* [n_epochs][n_sbs] */
+    uint32_t n_preneoplastic_snv;
+    uint32_t n_preneoplastic_indel;
+    uint32_t preneoplastic_sbs; /* signature index of the preneoplastic SNVs */
+    uint32_t indel_max_len;     /* indel lengths are geometric on [1, indel_max_len] */
+    double indel_len_p;
+    int32_t infinite_sites;     /* MutationEngine$infinite_sites_model */
+    uint32_t ploidy;            /* copies of every chromosome (2) */
+    uint32_t n_drivers;
+    const psx_driver* drivers;
+    uint32_t n_germline;
+    const psx_germline* germline;
+} psx_engine_params;
+
+/* One record per placed mutation; the table is sorted by SID
+ * (chr, pos, kind, length, alternative sequence), ties by creation order. */
+typedef struct psx_mutation {
+    uint32_t chr;
+    uint32_t pos;       /* 0-based */
+    uint32_t leaf_lo;   /* carried by leaves [leaf_lo, leaf_hi) */
+    uint32_t leaf_hi;
+    uint32_t node;      /* DFS index of the edge node, PSX_NO_NODE for preneoplastic/germline */
+    uint32_t seq;       /* packed bases: deleted reference bases (DEL), inserted bases (INS) */
+    uint16_t cause;     /* SBS signature index, PSX_NO_CAUSE if none */
+    uint8_t allele;
+    uint8_t kind;       /* PSX_SNV / PSX_DEL / PSX_INS */
+    uint8_t ref;        /* reference base at pos */
+    uint8_t alt;        /* SNV alternative base */
+    uint8_t len;        /* indel length */
+    uint8_t nature;     /* PSX_PASSENGER / PSX_DRIVER / PSX_PRENEOPLASTIC / PSX_GERMINAL */
+    uint8_t sid_head;   /* 1 on the first record of each SID */
+    uint8_t pad[3];
+} psx_mutation;
+
+psx_status psx_place_mutations(psx_ctx* ctx, const psx_genome* g, const psx_forest* f,
+                               const psx_engine_params* p, uint64_t seed, psx_mutations** out);
+/* Replay: a caller-provided (e.g. exported) table; it is sorted by SID here. */
+psx_status psx_mutations_upload(psx_ctx* ctx, const psx_forest* f, const psx_mutation* recs,
+                                uint64_t n, uint32_t ploidy, psx_mutations** out);
+psx_status psx_mutations_count(const psx_mutations* m, uint64_t* n);
+psx_status psx_mutations_download(psx_ctx* ctx, const psx_mutations* m, psx_mutation* recs,
+                                  uint64_t n);
+void psx_mutations_free(psx_mutations* m);
+/* Milliseconds of the placement kernels of the last psx_place_mutations. */
+psx_status psx_ctx_last_place_ms(const psx_ctx* ctx, double* ms);
+
+/* --------------------------------------------------------------- sequencing */
+
+typedef struct psx_seq_params {
+    double coverage;
+    uint32_t read_size;
+    uint32_t insert_size_mean;    /* 0 = single-end reads */
+    uint32_t insert_size_stddev;
+    double error_rate;            /* 0 = error-less Illumina sequencer */
+    double purity;                /* fraction of tumour cells in every tumour sample */
+    int32_t with_normal_sample;
+    int32_t preneoplastic_in_normal;
+    uint32_t n_chromosomes;       /* 0 = every chromosome of the genome */
+    const uint32_t* chromosomes;
+    uint32_t n_samples;           /* 0 = every forest sample, in index order */
+    const uint32_t* samples;
+} psx_seq_params;
+
+/* Validates the paired-end insert-size model Binomial(t, p) with
+ * p = 1 - sd^2/mean and t = floor(mean / p). */
+psx_status psx_insert_size_check(psx_ctx* ctx, uint32_t mean, uint32_t stddev);
+
+/* Number of output samples: requested samples, plus one normal sample. */
+uint32_t psx_seq_num_samples(const psx_forest* f, const psx_seq_params* p);
+/* Reads the call simulates (closed form of the sampling model). */
+psx_status psx_seq_num_reads(const psx_genome* g, const psx_forest* f, const psx_seq_params* p,
+                             uint64_t* n_reads);
+
+/* Count-only sequencing (write_SAM = FALSE). Output arrays are host memory
+ * of n_out_samples x n_mutations entries, sample-major: alt = reads showing
+ * the SID's alternative allele, depth = reads covering its position, present
+ * = 1 if some source genome of the sample carries the mutation. */
+psx_status psx_simulate_seq(psx_ctx* ctx, const psx_genome* g, const psx_forest* f,
+                            const psx_mutations* m, const psx_seq_params* p, uint64_t seed,
+                            int32_t* alt, int32_t* depth, uint8_t* present, uint64_t* n_reads);
+/* Same, writing device buffers (inputs and outputs stay in HBM). */
+psx_status psx_simulate_seq_device(psx_ctx* ctx, const psx_genome* g, const psx_forest* f,
+                                   const psx_mutations* m, const psx_seq_params* p,
+                                   uint64_t seed, int32_t* d_alt, int32_t* d_depth,
+                                   uint8_t* d_present, uint64_t* n_reads);
+
+/* Read materialisation (write_SAM = TRUE analogue): every read as a 16-byte
+ * header (position, source leaf, flags) plus ceil(read_size/16) words of
+ * 2-bit bases with the source's mutations and sequencing errors applied.
+ * d_reads holds psx_read_words(p) uint32 per read, reads ordered by
+ * (sample, chromosome, stratum, template, mate). */
+uint32_t psx_read_words(const psx_seq_params* p);
+psx_status psx_simulate_seq_reads(psx_ctx* ctx, const psx_genome* g, const psx_forest* f,
+                                  const psx_mutations* m, const psx_seq_params* p,
+                                  uint64_t seed, uint32_t* d_reads, uint64_t capacity_reads,
+                                  uint64_t* n_reads);
+
+/* ------------------------------------------------------------- batched mode */
+
+typedef struct psx_seq_job {
+    const psx_genome* genome;
+    const psx_forest* forest;
+    const psx_mutations* mutations;
+    psx_seq_params params;
+    uint64_t seed;
+    int32_t* d_alt;       /* device, psx_seq_num_samples x n_mutations */
+    int32_t* d_depth;
+    uint8_t* d_present;
+} psx_seq_job;
+
+typedef struct psx_launch_config {
+    uint32_t block_threads;    /* 64..1024, multiple of 64 */
+    uint32_t lds_records;      /* mutation records staged in LDS per block */
+    uint32_t strata_per_item;  /* strata processed by one work item */
+    uint32_t blocks_per_cu;    /* persistent blocks per compute unit */
+    int32_t use_graph;         /* capture the launch sequence in a hipGraph */
+} psx_launch_config;
+
+/* All jobs in one persistent launch pulling (job, sample, chromosome,
+ * strata) items from a device queue. cfg = NULL uses the tuned default. */
+psx_status psx_simulate_seq_batch(psx_ctx* ctx, const psx_seq_job* jobs, uint32_t n_jobs,
+                                  const psx_launch_config* cfg, uint64_t* n_reads);
+/* Times candidate launch configurations on the jobs and returns the fastest
+ * (results are identical for every configuration). */
+psx_status psx_autotune_seq(psx_ctx* ctx, const psx_seq_job* jobs, uint32_t n_jobs,
+                            psx_launch_config* best, double* best_ms);
+
+/* ------------------------------------------------------ result table (host) */
+
+/* get_result_dataframe semantics on the host arrays of psx_simulate_seq:
+ * rows are the SIDs (sid_head records) covered by >= 1 alternative read in
+ * some sample, or present in some sample when include_non_sequenced != 0.
+ * row_rec[n_rows] receives record indices; occ/cov/vaf are
+ * [n_out_samples][n_mutations]-capacity, filled [sample][row]. VAF is
+ * occ/cov when the sample holds the SID (NaN if cov == 0), else 0. */
+psx_status psx_seq_table(const psx_mutation* recs, uint64_t n_mutations,
+                         uint32_t n_out_samples, const int32_t* alt, const int32_t* depth,
+                         const uint8_t* present, int32_t include_non_sequenced,
+                         uint64_t* n_rows, uint64_t* row_rec, int32_t* occ, int32_t* cov,
+                         double* vaf);
+
+/* ------------------------------------------- synthetic forest (host input) */
+
+/* Birth-death genealogy standing in for the CPU spatial simulation, which
+ * stays outside the accelerated path. Species 0 founds the tumour; species
+ * s > 0 is introduced by converting one uniformly chosen species-0 cell when
+ * the population reaches intro_at[s] cells. Growth stops at n_cells; then
+ * n_samples disjoint samples of sample_size cells are drawn uniformly. */
+typedef struct psx_synth_params {
+    uint32_t n_species;
+    const double* growth;     /* [n_species] division rate */
+    const double* death;      /* [n_species] death rate */
+    const uint64_t* intro_at; /* [n_species], ignored for species 0 */
+    uint64_t n_cells;
+    uint32_t n_samples;
+    uint64_t sample_size;
+} psx_synth_params;
+
+typedef struct psx_synth_forest psx_synth_forest;
+psx_status psx_synth_forest_run(const psx_synth_params* p, uint64_t seed, psx_synth_forest** out);
+uint32_t psx_synth_forest_nodes(const psx_synth_forest* s);
+/* Arrays of psx_synth_forest_nodes() entries, in creation order. */
+psx_status psx_synth_forest_get(const psx_synth_forest* s, int64_t* cell_id, int64_t* parent,
+                                uint32_t* species, int32_t* sample, double* birth_time);
+void psx_synth_forest_free(psx_synth_forest* s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PSX_H */

@@ -1,0 +1,989 @@
+/*
+ * CPU oracle: TEST INFRASTRUCTURE ONLY (see oracle.h). Never linked into the
+ * product; loaded by tests/, the smoke check and bench.py's CPU baseline.
+ *
+ * This file is the executable statement of the sampling model. Each function
+ * names the reference code whose behaviour it restates.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+enum {
+    P_REF = 0x01, P_TEMPLATE = 0x10, P_ERROR = 0x11, P_EDGE = 0x20, P_SNV = 0x21,
+    P_SITE = 0x22, P_INDEL = 0x23, P_DRIVER = 0x25
+};
+
+#define SITE_TRIES (1u << 20)
+#define INDEL_TRIES (1u << 20)
+#define IS_ROUNDS 64
+#define TABLE_CAP (1u << 16)
+#define TWO32 4294967296.0
+
+static __thread char g_err[512];
+
+static int fail(const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return PSX_EINVAL;
+}
+
+const char* orc_last_error(void) { return g_err; }
+void orc_free(void* p) { free(p); }
+
+/* ------------------------------------------------------------ Philox4x32-10 */
+
+void orc_philox(const uint32_t ctr[4], uint64_t key, uint32_t out[4]) {
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    uint32_t k0 = (uint32_t)key, k1 = (uint32_t)(key >> 32);
+    for (int r = 0; r < 10; ++r) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c0 = n0;
+        c1 = (uint32_t)p1;
+        c2 = n2;
+        c3 = (uint32_t)p0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+static void philox4(uint64_t key, uint32_t a, uint32_t b, uint32_t c, uint32_t d, uint32_t o[4]) {
+    uint32_t ctr[4] = {a, b, c, d};
+    orc_philox(ctr, key, o);
+}
+
+static uint32_t mulhi32(uint32_t u, uint32_t n) { return (uint32_t)(((uint64_t)u * n) >> 32); }
+
+static uint64_t mulhi64(uint32_t hi, uint32_t lo, uint64_t n) {
+    unsigned __int128 u = ((unsigned __int128)(((uint64_t)hi << 32) | lo)) * n;
+    return (uint64_t)(u >> 64);
+}
+
+/* ------------------------------------------------------------ inversion tables */
+
+int orc_cdf_table(const double* pmf, uint32_t n, uint64_t* T) {
+    if (n == 0) return fail("empty distribution");
+    double total = 0.0;
+    for (uint32_t k = 0; k < n; ++k) {
+        if (!(pmf[k] >= 0.0)) return fail("negative or NaN probability");
+        total += pmf[k];
+    }
+    if (!(total > 0.0) || !isfinite(total)) return fail("distribution has no mass");
+    double cum = 0.0;
+    for (uint32_t k = 0; k + 1 < n; ++k) {
+        cum += pmf[k];
+        double x = cum / total;
+        T[k] = x >= 1.0 ? (uint64_t)1 << 32 : (uint64_t)(x * TWO32);
+    }
+    T[n - 1] = (uint64_t)1 << 32;
+    return 0;
+}
+
+/* Linear scan (the kernels use a binary search; both give the unique k). */
+static uint32_t pick(const uint64_t* T, uint32_t n, uint32_t u) {
+    uint32_t k = 0;
+    while (k + 1 < n && (uint64_t)u >= T[k]) ++k;
+    return k;
+}
+
+int orc_poisson_table(double lambda, uint32_t* k0, uint32_t* n, uint64_t* T, uint32_t cap) {
+    if (!(lambda >= 0.0) || !isfinite(lambda)) return fail("invalid Poisson mean %g", lambda);
+    if (lambda == 0.0) {
+        *k0 = 0; *n = 1; T[0] = (uint64_t)1 << 32;
+        return 0;
+    }
+    double sd = sqrt(lambda);
+    double lo = floor(lambda - 40.0 * sd - 10.0);
+    if (lo < 0.0) lo = 0.0;
+    double hi = ceil(lambda + 40.0 * sd + 10.0);
+    double cnt = hi - lo + 1.0;
+    if (cnt > (double)cap) return fail("Poisson mean %g too large", lambda);
+    uint32_t m = (uint32_t)cnt;
+    double* pmf = (double*)malloc(sizeof(double) * m);
+    double ll = log(lambda);
+    for (uint32_t i = 0; i < m; ++i) {
+        double k = lo + (double)i;
+        pmf[i] = exp(-lambda + k * ll - lgamma(k + 1.0));
+    }
+    int rc = orc_cdf_table(pmf, m, T);
+    free(pmf);
+    *k0 = (uint32_t)lo; *n = m;
+    return rc;
+}
+
+static int binomial_range(uint32_t t, double p, double lo, double hi, uint32_t cap, uint32_t* k0,
+                          uint32_t* n, uint64_t* T) {
+    double cnt = hi - lo + 1.0;
+    if (cnt > (double)cap) return fail("binomial table too large");
+    uint32_t m = (uint32_t)cnt;
+    double* pmf = (double*)malloc(sizeof(double) * m);
+    double lp = log(p), lq = log1p(-p), lgt = lgamma((double)t + 1.0);
+    for (uint32_t i = 0; i < m; ++i) {
+        double k = lo + (double)i;
+        pmf[i] = exp(lgt - lgamma(k + 1.0) - lgamma((double)t - k + 1.0) + k * lp + ((double)t - k) * lq);
+    }
+    int rc = orc_cdf_table(pmf, m, T);
+    free(pmf);
+    *k0 = (uint32_t)lo; *n = m;
+    return rc;
+}
+
+int orc_binomial_table(uint32_t t, double p, uint32_t* k0, uint32_t* n, uint64_t* T, uint32_t cap) {
+    if (!(p > 0.0) || p > 1.0) return fail("invalid binomial probability %g", p);
+    if (p == 1.0) {
+        *k0 = t; *n = 1; T[0] = (uint64_t)1 << 32;
+        return 0;
+    }
+    double m = (double)t * p, sd = sqrt((double)t * p * (1.0 - p));
+    double lo = floor(m - 40.0 * sd - 10.0), hi = ceil(m + 40.0 * sd + 10.0);
+    if (lo < 0.0) lo = 0.0;
+    if (hi > (double)t) hi = (double)t;
+    return binomial_range(t, p, lo, hi, cap, k0, n, T);
+}
+
+/* Number of sequencing errors in one read: Binomial(L, e) on
+ * [0, min(L, PSX_MAX_READ_ERRORS)]. */
+static int error_table(uint32_t L, double e, uint32_t* n, uint64_t* T) {
+    uint32_t k0;
+    double hi = L < PSX_MAX_READ_ERRORS ? (double)L : (double)PSX_MAX_READ_ERRORS;
+    return binomial_range(L, e, 0.0, hi, PSX_MAX_READ_ERRORS + 1, &k0, n, T);
+}
+
+/* get_bin_dist, src/seq_simulation.cpp:431-451: q = sd^2/mean, p = 1 - q,
+ * error when p < 0 (p == 0 is rejected as well: t would be infinite),
+ * t = (uint32)(mean / p). */
+static int insert_table(uint32_t mean, uint32_t sd, uint32_t* k0, uint32_t* n, uint64_t* T) {
+    double q = (double)(sd * sd) / (double)mean;
+    double p = 1.0 - q;
+    if (!(p > 0.0))
+        return fail("The insert size mean (%u) must be greater than or equal to its variance "
+                    "(%u*%u=%u).", mean, sd, sd, sd * sd);
+    uint32_t t = (uint32_t)((double)mean / p);
+    return orc_binomial_table(t, p, k0, n, T, 4096);
+}
+
+/* -------------------------------------------------------------------- genome */
+
+uint64_t orc_genome_words(uint32_t n_chr, const uint32_t* len) {
+    uint64_t w = 0;
+    for (uint32_t c = 0; c < n_chr; ++c) w += (len[c] + 15ull) / 16;
+    return w;
+}
+
+/* Synthetic reference (stands in for the FASTA of src/mutation_engine.cpp:75-112):
+ * word w of chromosome c is word (w % 4) of Philox(key = seed,
+ * counter = (w / 4, c, 0, REF << 24)); bits past the chromosome end are 0. */
+int orc_genome_synth(uint32_t n_chr, const uint32_t* len, uint64_t seed, uint32_t* packed) {
+    if (n_chr == 0 || n_chr > PSX_MAX_CHROMOSOMES) return fail("invalid number of chromosomes");
+    uint64_t off = 0;
+    for (uint32_t c = 0; c < n_chr; ++c) {
+        uint64_t words = (len[c] + 15ull) / 16;
+        for (uint64_t w = 0; w < words; ++w) {
+            uint32_t o[4];
+            philox4(seed, (uint32_t)(w / 4), c, 0, (uint32_t)P_REF << 24, o);
+            packed[off + w] = o[w % 4];
+        }
+        uint32_t rem = len[c] % 16;
+        if (rem) packed[off + words - 1] &= (1u << (2 * rem)) - 1u;
+        off += words;
+    }
+    return 0;
+}
+
+typedef struct genome_view {
+    uint32_t n_chr;
+    const uint32_t* len;
+    const uint32_t* packed;
+    uint64_t woff[PSX_MAX_CHROMOSOMES];
+    uint64_t boff[PSX_MAX_CHROMOSOMES + 1]; /* absolute base offset of each chromosome */
+} genome_view;
+
+static void gv_init(genome_view* g, uint32_t n_chr, const uint32_t* len, const uint32_t* packed) {
+    g->n_chr = n_chr; g->len = len; g->packed = packed;
+    uint64_t w = 0, b = 0;
+    for (uint32_t c = 0; c < n_chr; ++c) {
+        g->woff[c] = w; g->boff[c] = b;
+        w += (len[c] + 15ull) / 16;
+        b += len[c];
+    }
+    g->boff[n_chr] = b;
+}
+
+static uint32_t base_at(const genome_view* g, uint32_t c, uint32_t i) {
+    return (g->packed[g->woff[c] + i / 16] >> (2 * (i % 16))) & 3u;
+}
+
+static void locate(const genome_view* g, uint64_t x, uint32_t* c, uint32_t* pos) {
+    uint32_t k = 0;
+    while (x >= g->boff[k + 1]) ++k;
+    *c = k;
+    *pos = (uint32_t)(x - g->boff[k]);
+}
+
+/* -------------------------------------------------------------------- forest */
+
+typedef struct { int64_t id; uint32_t row; } id_row;
+
+static int cmp_id_row(const void* a, const void* b) {
+    const id_row* x = (const id_row*)a; const id_row* y = (const id_row*)b;
+    return x->id < y->id ? -1 : (x->id > y->id ? 1 : 0);
+}
+
+static int64_t find_row(const id_row* ix, uint32_t n, int64_t id) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        uint32_t mid = lo + (hi - lo) / 2;
+        if (ix[mid].id < id) lo = mid + 1; else hi = mid;
+    }
+    return (lo < n && ix[lo].id == id) ? (int64_t)ix[lo].row : -1;
+}
+
+void orc_forest_free(orc_forest* f) {
+    if (!f) return;
+    free(f->cell_id); free(f->parent); free(f->species); free(f->birth); free(f->leaf_lo);
+    free(f->leaf_hi); free(f->sample_begin); free(f->sample_leaves);
+    free(f);
+}
+
+/* SamplesForest node table (src/forest.hpp:29-67) in DFS pre-order: roots and
+ * children by ascending cell id; leaves are the childless nodes. */
+int orc_forest_flatten(const psx_forest_desc* d, orc_forest** out) {
+    uint32_t n = d->n_nodes;
+    if (n == 0) return fail("the forest has no nodes");
+    id_row* ix = (id_row*)malloc(sizeof(id_row) * n);
+    for (uint32_t i = 0; i < n; ++i) { ix[i].id = d->cell_id[i]; ix[i].row = i; }
+    qsort(ix, n, sizeof(id_row), cmp_id_row);
+    for (uint32_t i = 1; i < n; ++i)
+        if (ix[i].id == ix[i - 1].id) { free(ix); return fail("duplicated cell id %lld", (long long)ix[i].id); }
+    uint32_t* prow = (uint32_t*)malloc(sizeof(uint32_t) * n);
+    uint32_t* nchild = (uint32_t*)calloc(n + 1, sizeof(uint32_t));
+    for (uint32_t i = 0; i < n; ++i) {
+        if (d->parent[i] < 0) { prow[i] = PSX_NO_NODE; continue; }
+        int64_t r = find_row(ix, n, d->parent[i]);
+        if (r < 0) { free(ix); free(prow); free(nchild); return fail("unknown ancestor %lld", (long long)d->parent[i]); }
+        prow[i] = (uint32_t)r;
+        nchild[r + 1]++;
+    }
+    for (uint32_t i = 0; i < n; ++i) nchild[i + 1] += nchild[i];
+    uint32_t* child = (uint32_t*)malloc(sizeof(uint32_t) * (n ? n : 1));
+    uint32_t* fill = (uint32_t*)calloc(n, sizeof(uint32_t));
+    /* visiting rows in cell-id order leaves every child list sorted by id */
+    for (uint32_t k = 0; k < n; ++k) {
+        uint32_t i = ix[k].row;
+        if (prow[i] != PSX_NO_NODE) child[nchild[prow[i]] + fill[prow[i]]++] = i;
+    }
+    free(fill);
+    orc_forest* f = (orc_forest*)calloc(1, sizeof(orc_forest));
+    f->n_nodes = n;
+    f->cell_id = (int64_t*)malloc(sizeof(int64_t) * n);
+    f->parent = (uint32_t*)malloc(sizeof(uint32_t) * n);
+    f->species = (uint32_t*)malloc(sizeof(uint32_t) * n);
+    f->birth = (double*)malloc(sizeof(double) * n);
+    f->leaf_lo = (uint32_t*)malloc(sizeof(uint32_t) * n);
+    f->leaf_hi = (uint32_t*)malloc(sizeof(uint32_t) * n);
+    uint32_t* dfs_of_row = (uint32_t*)malloc(sizeof(uint32_t) * n);
+    uint32_t* row_of_dfs = (uint32_t*)malloc(sizeof(uint32_t) * n);
+    uint32_t* stack = (uint32_t*)malloc(sizeof(uint32_t) * n);
+    uint32_t top = 0, next = 0;
+    for (int64_t k = (int64_t)n - 1; k >= 0; --k) {
+        uint32_t i = ix[k].row;
+        if (prow[i] == PSX_NO_NODE) stack[top++] = i;
+    }
+    while (top) {
+        uint32_t i = stack[--top];
+        dfs_of_row[i] = next;
+        row_of_dfs[next++] = i;
+        for (uint32_t c = nchild[i + 1]; c > nchild[i]; --c) stack[top++] = child[c - 1];
+    }
+    free(stack); free(ix);
+    if (next != n) {
+        free(prow); free(nchild); free(child); free(dfs_of_row); free(row_of_dfs);
+        orc_forest_free(f);
+        return fail("the ancestor relation has a cycle");
+    }
+    /* subtree sizes by reverse pre-order, then leaf prefix counts */
+    uint32_t* size = (uint32_t*)malloc(sizeof(uint32_t) * n);
+    for (uint32_t v = 0; v < n; ++v) size[v] = 1;
+    for (int64_t v = (int64_t)n - 1; v >= 0; --v) {
+        uint32_t i = row_of_dfs[v];
+        if (prow[i] != PSX_NO_NODE) size[dfs_of_row[prow[i]]] += size[v];
+    }
+    uint32_t* leaves_before = (uint32_t*)malloc(sizeof(uint32_t) * (n + 1));
+    leaves_before[0] = 0;
+    for (uint32_t v = 0; v < n; ++v) {
+        uint32_t i = row_of_dfs[v];
+        leaves_before[v + 1] = leaves_before[v] + (nchild[i + 1] == nchild[i] ? 1u : 0u);
+    }
+    f->n_leaves = leaves_before[n];
+    f->n_samples = d->n_samples;
+    f->sample_begin = (uint32_t*)calloc(d->n_samples + 1, sizeof(uint32_t));
+    f->sample_leaves = (uint32_t*)malloc(sizeof(uint32_t) * (f->n_leaves ? f->n_leaves : 1));
+    int rc = 0;
+    for (uint32_t v = 0; v < n && !rc; ++v) {
+        uint32_t i = row_of_dfs[v];
+        f->cell_id[v] = d->cell_id[i];
+        f->parent[v] = prow[i] == PSX_NO_NODE ? PSX_NO_NODE : dfs_of_row[prow[i]];
+        f->species[v] = d->species[i];
+        f->birth[v] = d->birth_time[i];
+        f->leaf_lo[v] = leaves_before[v];
+        f->leaf_hi[v] = leaves_before[v + size[v]];
+        int leaf = nchild[i + 1] == nchild[i];
+        int32_t s = d->sample[i];
+        if (s >= 0 && (!leaf || (uint32_t)s >= d->n_samples))
+            rc = fail("cell %lld: invalid sample %d", (long long)d->cell_id[i], s);
+        else if (s >= 0)
+            f->sample_begin[s + 1]++;
+    }
+    for (uint32_t s = 0; s < d->n_samples; ++s) f->sample_begin[s + 1] += f->sample_begin[s];
+    uint32_t* fillc = (uint32_t*)calloc(d->n_samples + 1, sizeof(uint32_t));
+    for (uint32_t v = 0; v < n && !rc; ++v) {
+        int32_t s = d->sample[row_of_dfs[v]];
+        if (s >= 0) f->sample_leaves[f->sample_begin[s] + fillc[s]++] = leaves_before[v];
+    }
+    free(fillc); free(size); free(leaves_before); free(prow); free(nchild); free(child);
+    free(dfs_of_row); free(row_of_dfs);
+    if (rc) { orc_forest_free(f); return rc; }
+    *out = f;
+    return 0;
+}
+
+/* --------------------------------------------------------------- SID order */
+
+static uint64_t sid_key(const psx_mutation* r) {
+    uint64_t payload = r->kind == PSX_SNV ? r->alt : (r->kind == PSX_INS ? r->seq : 0u);
+    return ((uint64_t)r->chr << 58) | ((uint64_t)r->pos << 26) | ((uint64_t)r->kind << 24) |
+           ((uint64_t)(r->len & 15u) << 20) | (payload & 0xFFFFFu);
+}
+
+typedef struct { uint64_t key; uint64_t id; } key_id;
+
+static int cmp_key_id(const void* a, const void* b) {
+    const key_id* x = (const key_id*)a; const key_id* y = (const key_id*)b;
+    if (x->key != y->key) return x->key < y->key ? -1 : 1;
+    return x->id < y->id ? -1 : (x->id > y->id ? 1 : 0);
+}
+
+int orc_sort_table(psx_mutation* m, uint64_t n) {
+    key_id* k = (key_id*)malloc(sizeof(key_id) * (n ? n : 1));
+    for (uint64_t i = 0; i < n; ++i) { k[i].key = sid_key(&m[i]); k[i].id = i; }
+    qsort(k, n, sizeof(key_id), cmp_key_id);
+    psx_mutation* tmp = (psx_mutation*)malloc(sizeof(psx_mutation) * (n ? n : 1));
+    for (uint64_t i = 0; i < n; ++i) {
+        memcpy(&tmp[i], &m[k[i].id], sizeof(psx_mutation));
+        tmp[i].sid_head = (i == 0 || k[i].key != k[i - 1].key) ? 1 : 0;
+    }
+    memcpy(m, tmp, sizeof(psx_mutation) * n);
+    free(tmp); free(k);
+    return 0;
+}
+
+/* ----------------------------------------------------------------- placement */
+
+typedef struct { uint32_t k0, n; uint64_t* T; } table;
+
+enum { SLOT_DRIVER, SLOT_GERM, SLOT_PRE_SNV, SLOT_PRE_INDEL, SLOT_SNV, SLOT_INDEL };
+
+typedef struct slot {
+    uint8_t type;
+    uint32_t node;   /* DFS index or PSX_NO_NODE */
+    uint32_t index;  /* driver / germline index, or rank within the node */
+    uint32_t attempt;
+} slot;
+
+typedef struct place_ctx {
+    genome_view g;
+    const orc_forest* f;
+    const psx_engine_params* p;
+    uint64_t seed, G;
+    table* expo;     /* per epoch, over signatures */
+    table* cls;      /* per signature, over 96 classes */
+    table len_tab;   /* indel lengths, value = 1 + index */
+} place_ctx;
+
+static const uint32_t SUB_TARGET[6] = {0, 2, 3, 0, 1, 2}; /* C>A C>G C>T T>A T>C T>G */
+
+static uint32_t epoch_of(const psx_engine_params* p, double t) {
+    uint32_t e = 0;
+    for (uint32_t k = 0; k < p->n_epochs; ++k)
+        if (p->epoch_start[k] <= t) e = k;
+    return e;
+}
+
+/* SBS passenger: signature from the timed exposure at the cell's birth
+ * (MutationEngine$add_exposure, src/mutation_engine.cpp:734-739), class from
+ * the signature, site uniform over positions whose pyrimidine-centred
+ * trinucleotide matches the class (rejection on the reference). */
+static int draw_snv(const place_ctx* x, uint32_t v, uint32_t k, uint32_t a, int fixed_sig,
+                    psx_mutation* r) {
+    uint32_t w[4];
+    philox4(x->seed, k, v, a, (uint32_t)P_SNV << 24, w);
+    uint32_t sig;
+    if (fixed_sig >= 0) sig = (uint32_t)fixed_sig;
+    else {
+        const table* e = &x->expo[epoch_of(x->p, x->f->birth[v])];
+        sig = pick(e->T, e->n, w[0]);
+    }
+    const table* ct = &x->cls[sig];
+    uint32_t cls = pick(ct->T, ct->n, w[1]);
+    uint32_t allele = mulhi32(w[2], x->p->ploidy);
+    uint32_t sub = cls / 16, five = (cls >> 2) & 3u, three = cls & 3u;
+    uint32_t mid = sub < 3 ? 1u : 3u, tgt = SUB_TARGET[sub];
+    uint32_t ws[4] = {0, 0, 0, 0};
+    for (uint32_t t = 0; t < SITE_TRIES; ++t) {
+        if ((t & 1u) == 0) philox4(x->seed, k, v, a, ((uint32_t)P_SITE << 24) | (t >> 1), ws);
+        uint32_t hi = (t & 1u) ? ws[2] : ws[0], lo = (t & 1u) ? ws[3] : ws[1];
+        uint32_t c, pos;
+        locate(&x->g, mulhi64(hi, lo, x->G), &c, &pos);
+        if (pos < 1 || pos + 1 >= x->g.len[c]) continue;
+        uint32_t b5 = base_at(&x->g, c, pos - 1), m = base_at(&x->g, c, pos), b3 = base_at(&x->g, c, pos + 1);
+        uint32_t alt;
+        if (m == 1u || m == 3u) {
+            if (!(b5 == five && m == mid && b3 == three)) continue;
+            alt = tgt;
+        } else {
+            if (!(3u - b3 == five && 3u - m == mid && 3u - b5 == three)) continue;
+            alt = 3u - tgt;
+        }
+        r->chr = c; r->pos = pos; r->kind = PSX_SNV; r->ref = (uint8_t)m; r->alt = (uint8_t)alt;
+        r->len = 0; r->seq = 0; r->allele = (uint8_t)allele; r->cause = (uint16_t)sig;
+        return 0;
+    }
+    return fail("no reference site matches SBS class %u", cls);
+}
+
+/* Indel: deletion or insertion (top bit), geometric length, uniform position
+ * with room for the deleted bases, inserted bases from a second draw. */
+static int draw_indel(const place_ctx* x, uint32_t v, uint32_t k, uint32_t a, psx_mutation* r) {
+    for (uint32_t t = 0; t < INDEL_TRIES; ++t) {
+        uint32_t w[4];
+        philox4(x->seed, k, v, a, ((uint32_t)P_INDEL << 24) | t, w);
+        uint32_t kind = (w[0] >> 31) ? PSX_INS : PSX_DEL;
+        uint32_t allele = mulhi32(w[0] << 1, x->p->ploidy);
+        uint32_t l = 1 + pick(x->len_tab.T, x->len_tab.n, w[1]);
+        uint32_t c, pos;
+        locate(&x->g, mulhi64(w[2], w[3], x->G), &c, &pos);
+        if ((uint64_t)pos + l >= x->g.len[c]) continue;
+        uint32_t seq = 0;
+        if (kind == PSX_DEL) {
+            for (uint32_t i = 0; i < l; ++i) seq |= base_at(&x->g, c, pos + 1 + i) << (2 * i);
+        } else {
+            uint32_t s[4];
+            philox4(x->seed, k, v, a, ((uint32_t)P_INDEL << 24) | 0x400000u | t, s);
+            seq = s[0] & ((1u << (2 * l)) - 1u);
+        }
+        r->chr = c; r->pos = pos; r->kind = (uint8_t)kind; r->ref = (uint8_t)base_at(&x->g, c, pos);
+        r->alt = 0; r->len = (uint8_t)l; r->seq = seq; r->allele = (uint8_t)allele;
+        r->cause = PSX_NO_CAUSE;
+        return 0;
+    }
+    return fail("no room for an indel");
+}
+
+static int draw_slot(const place_ctx* x, const slot* s, psx_mutation* r) {
+    const orc_forest* f = x->f;
+    const psx_engine_params* p = x->p;
+    memset(r, 0, sizeof *r);
+    int rc = 0;
+    switch (s->type) {
+    case SLOT_DRIVER: {
+        const psx_driver* d = &p->drivers[s->index];
+        uint32_t allele;
+        if (d->allele >= 0) allele = (uint32_t)d->allele;
+        else {
+            uint32_t w[4];
+            philox4(x->seed, s->index, s->node, 0, (uint32_t)P_DRIVER << 24, w);
+            allele = mulhi32(w[0], p->ploidy);
+        }
+        r->chr = d->chr; r->pos = d->pos; r->kind = PSX_SNV;
+        r->ref = (uint8_t)base_at(&x->g, d->chr, d->pos); r->alt = d->alt;
+        r->allele = (uint8_t)allele; r->cause = PSX_NO_CAUSE; r->nature = PSX_DRIVER;
+        r->node = s->node; r->leaf_lo = f->leaf_lo[s->node]; r->leaf_hi = f->leaf_hi[s->node];
+        return 0;
+    }
+    case SLOT_GERM: {
+        const psx_germline* gm = &p->germline[s->index];
+        r->chr = gm->chr; r->pos = gm->pos; r->kind = PSX_SNV;
+        r->ref = (uint8_t)base_at(&x->g, gm->chr, gm->pos); r->alt = gm->alt;
+        r->allele = gm->allele; r->cause = PSX_NO_CAUSE; r->nature = PSX_GERMINAL;
+        r->node = PSX_NO_NODE; r->leaf_lo = 0; r->leaf_hi = f->n_leaves;
+        return 0;
+    }
+    case SLOT_PRE_SNV:
+        rc = draw_snv(x, PSX_NO_NODE, s->index, s->attempt, (int)p->preneoplastic_sbs, r);
+        r->nature = PSX_PRENEOPLASTIC; r->node = PSX_NO_NODE; r->leaf_lo = 0; r->leaf_hi = f->n_leaves;
+        return rc;
+    case SLOT_PRE_INDEL:
+        rc = draw_indel(x, PSX_NO_NODE, s->index, s->attempt, r);
+        r->nature = PSX_PRENEOPLASTIC; r->node = PSX_NO_NODE; r->leaf_lo = 0; r->leaf_hi = f->n_leaves;
+        return rc;
+    case SLOT_SNV:
+        rc = draw_snv(x, s->node, s->index, s->attempt, -1, r);
+        break;
+    default:
+        rc = draw_indel(x, s->node, s->index, s->attempt, r);
+        break;
+    }
+    r->nature = PSX_PASSENGER; r->node = s->node;
+    r->leaf_lo = f->leaf_lo[s->node]; r->leaf_hi = f->leaf_hi[s->node];
+    return rc;
+}
+
+typedef struct { uint32_t chr, start, end; uint64_t id; } footprint;
+
+static int cmp_fp(const void* a, const void* b) {
+    const footprint* x = (const footprint*)a; const footprint* y = (const footprint*)b;
+    if (x->chr != y->chr) return x->chr < y->chr ? -1 : 1;
+    if (x->start != y->start) return x->start < y->start ? -1 : 1;
+    return x->id < y->id ? -1 : (x->id > y->id ? 1 : 0);
+}
+
+static int touches(const footprint* a, const footprint* b) {
+    return a->chr == b->chr && (uint64_t)b->start <= (uint64_t)a->end + 1 &&
+           (uint64_t)a->start <= (uint64_t)b->end + 1;
+}
+
+static void free_tables(place_ctx* x) {
+    if (x->expo) for (uint32_t e = 0; e < x->p->n_epochs; ++e) free(x->expo[e].T);
+    if (x->cls) for (uint32_t s = 0; s < x->p->n_sbs; ++s) free(x->cls[s].T);
+    free(x->expo); free(x->cls); free(x->len_tab.T);
+}
+
+/* MutationEngine::place_mutations (src/mutation_engine.cpp:1085-1113): per
+ * non-root edge Poisson(rate(species) * ploidy * G) SNVs and indels, the
+ * preneoplastic SNVs/indels shared by every tumour cell, drivers on the edge
+ * where their species first appears, germline in every cell. With the
+ * infinite-sites model (src/mutation_engine.hpp:148-156) no two mutations
+ * have touching footprints: the later-created one is redrawn. */
+int orc_place_mutations(uint32_t n_chr, const uint32_t* len, const uint32_t* packed,
+                        const orc_forest* f, const psx_engine_params* p, uint64_t seed,
+                        psx_mutation** out, uint64_t* n_out) {
+    if (n_chr == 0 || n_chr > PSX_MAX_CHROMOSOMES) return fail("invalid number of chromosomes");
+    if (p->ploidy < 1 || p->ploidy > 255) return fail("invalid ploidy");
+    if (p->indel_max_len < 1 || p->indel_max_len > 10) return fail("indel_max_len must be in [1, 10]");
+    if (!(p->indel_len_p > 0.0 && p->indel_len_p <= 1.0)) return fail("invalid indel length parameter");
+    for (uint32_t v = 0; v < f->n_nodes; ++v)
+        if (f->species[v] >= p->n_species) return fail("node species %u has no rates", f->species[v]);
+    place_ctx x;
+    memset(&x, 0, sizeof x);
+    gv_init(&x.g, n_chr, len, packed);
+    x.f = f; x.p = p; x.seed = seed; x.G = x.g.boff[n_chr];
+    int need_snv = p->n_preneoplastic_snv > 0;
+    for (uint32_t s = 0; s < p->n_species; ++s) need_snv |= p->snv_rate[s] > 0.0;
+    if (need_snv && (p->n_sbs == 0 || p->n_epochs == 0)) return fail("SNVs need a signature and an exposure");
+    if (p->n_preneoplastic_snv > 0 && p->preneoplastic_sbs >= p->n_sbs) return fail("unknown preneoplastic signature");
+    int rc = 0;
+    x.expo = (table*)calloc(p->n_epochs ? p->n_epochs : 1, sizeof(table));
+    x.cls = (table*)calloc(p->n_sbs ? p->n_sbs : 1, sizeof(table));
+    for (uint32_t e = 0; e < p->n_epochs && !rc; ++e) {
+        x.expo[e].T = (uint64_t*)malloc(sizeof(uint64_t) * p->n_sbs);
+        x.expo[e].n = p->n_sbs;
+        rc = orc_cdf_table(p->exposure + (size_t)e * p->n_sbs, p->n_sbs, x.expo[e].T);
+    }
+    for (uint32_t s = 0; s < p->n_sbs && !rc; ++s) {
+        x.cls[s].T = (uint64_t*)malloc(sizeof(uint64_t) * 96);
+        x.cls[s].n = 96;
+        rc = orc_cdf_table(p->sbs + (size_t)s * 96, 96, x.cls[s].T);
+    }
+    double lp[10];
+    for (uint32_t l = 1; l <= p->indel_max_len; ++l)
+        lp[l - 1] = pow(1.0 - p->indel_len_p, (double)(l - 1)) * p->indel_len_p;
+    x.len_tab.T = (uint64_t*)malloc(sizeof(uint64_t) * 10);
+    x.len_tab.n = p->indel_max_len;
+    if (!rc) rc = orc_cdf_table(lp, p->indel_max_len, x.len_tab.T);
+    table* ts = (table*)calloc(p->n_species, sizeof(table));
+    table* ti = (table*)calloc(p->n_species, sizeof(table));
+    double allelic = (double)((uint64_t)p->ploidy * x.G);
+    for (uint32_t s = 0; s < p->n_species && !rc; ++s) {
+        ts[s].T = (uint64_t*)malloc(sizeof(uint64_t) * TABLE_CAP);
+        ti[s].T = (uint64_t*)malloc(sizeof(uint64_t) * TABLE_CAP);
+        rc = orc_poisson_table(p->snv_rate[s] * allelic, &ts[s].k0, &ts[s].n, ts[s].T, TABLE_CAP);
+        if (!rc) rc = orc_poisson_table(p->indel_rate[s] * allelic, &ti[s].k0, &ti[s].n, ti[s].T, TABLE_CAP);
+    }
+    for (uint32_t d = 0; d < p->n_drivers && !rc; ++d) {
+        const psx_driver* dr = &p->drivers[d];
+        if (dr->chr >= n_chr || dr->pos >= len[dr->chr] || dr->alt > 3 ||
+            (dr->allele >= 0 && (uint32_t)dr->allele >= p->ploidy) ||
+            dr->alt == base_at(&x.g, dr->chr, dr->pos))
+            rc = fail("invalid driver %u", d);
+    }
+    for (uint32_t gi = 0; gi < p->n_germline && !rc; ++gi) {
+        const psx_germline* gm = &p->germline[gi];
+        if (gm->chr >= n_chr || gm->pos >= len[gm->chr] || gm->alt > 3 || gm->allele >= p->ploidy ||
+            gm->alt == base_at(&x.g, gm->chr, gm->pos))
+            rc = fail("invalid germline mutation %u", gi);
+    }
+    if (rc) {
+        for (uint32_t s = 0; s < p->n_species; ++s) { free(ts[s].T); free(ti[s].T); }
+        free(ts); free(ti); free_tables(&x);
+        return rc;
+    }
+    /* creation order = mutation id */
+    uint32_t* cs = (uint32_t*)calloc(f->n_nodes, sizeof(uint32_t));
+    uint32_t* ci = (uint32_t*)calloc(f->n_nodes, sizeof(uint32_t));
+    uint64_t total = (uint64_t)p->n_germline + p->n_preneoplastic_snv + p->n_preneoplastic_indel;
+    uint64_t n_driver_slots = 0;
+    for (uint32_t d = 0; d < p->n_drivers; ++d)
+        for (uint32_t v = 0; v < f->n_nodes; ++v) {
+            uint32_t par = f->parent[v];
+            if (f->species[v] == p->drivers[d].species &&
+                (par == PSX_NO_NODE || f->species[par] != p->drivers[d].species))
+                ++n_driver_slots;
+        }
+    total += n_driver_slots;
+    for (uint32_t v = 0; v < f->n_nodes; ++v) {
+        if (f->parent[v] == PSX_NO_NODE) continue;
+        uint32_t w[4];
+        philox4(seed, v, 0, 0, (uint32_t)P_EDGE << 24, w);
+        uint32_t s = f->species[v];
+        cs[v] = ts[s].k0 + pick(ts[s].T, ts[s].n, w[0]);
+        ci[v] = ti[s].k0 + pick(ti[s].T, ti[s].n, w[1]);
+        total += (uint64_t)cs[v] + ci[v];
+    }
+    for (uint32_t s = 0; s < p->n_species; ++s) { free(ts[s].T); free(ti[s].T); }
+    free(ts); free(ti);
+    slot* sl = (slot*)malloc(sizeof(slot) * (total ? total : 1));
+    uint64_t id = 0;
+    for (uint32_t d = 0; d < p->n_drivers; ++d)
+        for (uint32_t v = 0; v < f->n_nodes; ++v) {
+            uint32_t par = f->parent[v];
+            if (f->species[v] == p->drivers[d].species &&
+                (par == PSX_NO_NODE || f->species[par] != p->drivers[d].species))
+                sl[id++] = (slot){SLOT_DRIVER, v, d, 0};
+        }
+    for (uint32_t gi = 0; gi < p->n_germline; ++gi) sl[id++] = (slot){SLOT_GERM, PSX_NO_NODE, gi, 0};
+    for (uint32_t k = 0; k < p->n_preneoplastic_snv; ++k) sl[id++] = (slot){SLOT_PRE_SNV, PSX_NO_NODE, k, 0};
+    for (uint32_t k = 0; k < p->n_preneoplastic_indel; ++k) sl[id++] = (slot){SLOT_PRE_INDEL, PSX_NO_NODE, k, 0};
+    for (uint32_t v = 0; v < f->n_nodes; ++v) {
+        for (uint32_t k = 0; k < cs[v]; ++k) sl[id++] = (slot){SLOT_SNV, v, k, 0};
+        for (uint32_t k = 0; k < ci[v]; ++k) sl[id++] = (slot){SLOT_INDEL, v, k, 0};
+    }
+    free(cs); free(ci);
+    psx_mutation* m = (psx_mutation*)calloc(total ? total : 1, sizeof(psx_mutation));
+    for (uint64_t i = 0; i < total && !rc; ++i) rc = draw_slot(&x, &sl[i], &m[i]);
+    if (!rc && p->infinite_sites) {
+        footprint* fp = (footprint*)malloc(sizeof(footprint) * (total ? total : 1));
+        uint8_t* loser = (uint8_t*)malloc(total ? total : 1);
+        int round;
+        for (round = 0; round < IS_ROUNDS && !rc; ++round) {
+            for (uint64_t i = 0; i < total; ++i) {
+                fp[i].chr = m[i].chr; fp[i].start = m[i].pos;
+                fp[i].end = m[i].pos + (m[i].kind == PSX_DEL ? m[i].len : 0u);
+                fp[i].id = i;
+            }
+            qsort(fp, total, sizeof(footprint), cmp_fp);
+            memset(loser, 0, total ? total : 1);
+            uint64_t n_lose = 0;
+            for (uint64_t j = 0; j < total; ++j) {
+                uint64_t idj = fp[j].id;
+                if (sl[idj].type == SLOT_DRIVER || sl[idj].type == SLOT_GERM) continue;
+                int lose = 0;
+                for (uint64_t i = j; i-- > 0 && !lose;) {
+                    if (fp[i].chr != fp[j].chr || fp[i].start + 12u < fp[j].start) break;
+                    if (fp[i].id < idj && touches(&fp[i], &fp[j])) lose = 1;
+                }
+                for (uint64_t i = j + 1; i < total && !lose; ++i) {
+                    if (fp[i].chr != fp[j].chr || (uint64_t)fp[i].start > (uint64_t)fp[j].end + 1) break;
+                    if (fp[i].id < idj && touches(&fp[i], &fp[j])) lose = 1;
+                }
+                if (lose) { loser[idj] = 1; ++n_lose; }
+            }
+            if (n_lose == 0) break;
+            for (uint64_t i = 0; i < total && !rc; ++i)
+                if (loser[i]) { sl[i].attempt++; rc = draw_slot(&x, &sl[i], &m[i]); }
+        }
+        if (!rc && round == IS_ROUNDS) rc = fail("infinite-sites placement did not converge");
+        free(fp); free(loser);
+    }
+    free(sl);
+    free_tables(&x);
+    if (rc) { free(m); return rc; }
+    orc_sort_table(m, total);
+    *out = m;
+    *n_out = total;
+    return 0;
+}
+
+/* ---------------------------------------------------------------- sequencing */
+
+typedef struct seq_ctx {
+    const orc_forest* f;
+    const psx_mutation* m;
+    uint64_t n_mut;
+    uint64_t chr_begin[PSX_MAX_CHROMOSOMES + 1];
+    uint32_t ploidy, L, pe, preneo_in_normal;
+    uint64_t seed;
+    double e;
+    uint32_t ins_k0, ins_n;
+    uint64_t ins_T[4096];
+    uint32_t err_n;
+    uint64_t err_T[PSX_MAX_READ_ERRORS + 1];
+} seq_ctx;
+
+static int normal_carries(const seq_ctx* s, const psx_mutation* r) {
+    return r->nature == PSX_GERMINAL || (r->nature == PSX_PRENEOPLASTIC && s->preneo_in_normal);
+}
+
+static int carried(const seq_ctx* s, const psx_mutation* r, uint32_t leaf, uint32_t allele) {
+    if (r->allele != allele) return 0;
+    if (leaf == s->f->n_leaves) return normal_carries(s, r);
+    return r->leaf_lo <= leaf && leaf < r->leaf_hi;
+}
+
+/* Error set of one read: K ~ Binomial(L, e) from word 0, then K distinct
+ * offsets, one word each (offset = floor(w L / 2^32), substitution index =
+ * floor((w & 0xFFFF) 3 / 2^16)); a repeated offset takes the next word. */
+static uint32_t error_set(const seq_ctx* s, uint32_t r, uint32_t j, uint32_t c2, uint32_t mate,
+                          uint32_t* off, uint32_t* sub) {
+    uint32_t w[4];
+    uint32_t base = ((uint32_t)P_ERROR << 24) | (mate << 16);
+    philox4(s->seed, r, j, c2, base, w);
+    uint32_t K = pick(s->err_T, s->err_n, w[0]);
+    uint32_t cnt = 0;
+    for (uint32_t t = 1; cnt < K && t < 1 + 4 * PSX_MAX_READ_ERRORS; ++t) {
+        if ((t & 3u) == 0) philox4(s->seed, r, j, c2, base | (t >> 2), w);
+        uint32_t word = w[t & 3u];
+        uint32_t o = mulhi32(word, s->L);
+        int dup = 0;
+        for (uint32_t q = 0; q < cnt; ++q) dup |= off[q] == o;
+        if (dup) continue;
+        off[cnt] = o;
+        sub[cnt] = ((word & 0xFFFFu) * 3u) >> 16;
+        ++cnt;
+    }
+    return cnt;
+}
+
+static void add32(int32_t* p, int32_t v) {
+#ifdef _OPENMP
+#pragma omp atomic
+#endif
+    *p += v;
+}
+
+/* One read [x, x + L) from source (leaf, allele): every mutation record at a
+ * covered position gains depth; each SID head gains an alternative read when
+ * the observed base equals its alternative (SNV; the true base is the first
+ * carried SNV at the position, else the reference, then the read's error, if
+ * any, substitutes it) or when the source carries the SID (indel). */
+static void seq_read(const seq_ctx* s, uint32_t c, uint32_t x, uint32_t leaf, uint32_t allele,
+                     uint32_t r, uint32_t j, uint32_t c2, uint32_t mate, int32_t* alt,
+                     int32_t* depth) {
+    uint64_t lo = s->chr_begin[c], hi = s->chr_begin[c + 1];
+    while (lo < hi) {
+        uint64_t mid = lo + (hi - lo) / 2;
+        if (s->m[mid].pos < x) lo = mid + 1; else hi = mid;
+    }
+    uint64_t i = lo, end = s->chr_begin[c + 1];
+    int have_err = 0;
+    uint32_t ne = 0, off[PSX_MAX_READ_ERRORS], sub[PSX_MAX_READ_ERRORS];
+    while (i < end && s->m[i].pos < x + s->L) {
+        uint32_t pos = s->m[i].pos;
+        uint64_t g0 = i;
+        while (i < end && s->m[i].pos == pos) ++i;
+        uint32_t tb = s->m[g0].ref;
+        for (uint64_t q = g0; q < i; ++q)
+            if (s->m[q].kind == PSX_SNV && carried(s, &s->m[q], leaf, allele)) { tb = s->m[q].alt; break; }
+        uint32_t ob = tb;
+        if (s->e > 0.0) {
+            if (!have_err) { ne = error_set(s, r, j, c2, mate, off, sub); have_err = 1; }
+            for (uint32_t q = 0; q < ne; ++q)
+                if (off[q] == pos - x) ob = sub[q] < tb ? sub[q] : sub[q] + 1;
+        }
+        for (uint64_t q = g0; q < i; ++q) {
+            add32(&depth[q], 1);
+            if (!s->m[q].sid_head) continue;
+            if (s->m[q].kind == PSX_SNV) {
+                if (ob == s->m[q].alt) add32(&alt[q], 1);
+            } else {
+                int car = 0;
+                for (uint64_t q2 = q; q2 < i && (q2 == q || !s->m[q2].sid_head); ++q2)
+                    car |= carried(s, &s->m[q2], leaf, allele);
+                if (car) add32(&alt[q], 1);
+            }
+        }
+    }
+}
+
+typedef struct { uint32_t so, c, j; } stratum_ref;
+
+/* The ReadSimulator call of simulate_seq (src/seq_simulation.cpp:354-429)
+ * restated as a count-only simulation. Per output sample and chromosome:
+ * T templates (round(C * len / L), halved for paired ends) with starts
+ * stratified over PSX_STRATUM-base strata of the W = len - L + 1 valid
+ * starts (stratum j gets floor(T e_j / W) - floor(T b_j / W)); each template
+ * draws its start, normal-vs-tumour source (probability 1 - purity), source
+ * (cell, copy) uniformly, and for paired ends an insert size from get_bin_dist
+ * (src/seq_simulation.cpp:431-451); the mate ends the fragment. */
+int orc_simulate_seq(uint32_t n_chr, const uint32_t* len, const orc_forest* f,
+                     const psx_mutation* m, uint64_t n_mut, uint32_t ploidy,
+                     const psx_seq_params* p, uint64_t seed, int32_t* alt, int32_t* depth,
+                     uint8_t* present, uint64_t* n_reads) {
+    if (!(p->coverage >= 0.0) || !isfinite(p->coverage)) return fail("invalid coverage");
+    if (p->read_size < 1) return fail("invalid read size");
+    if (!(p->purity >= 0.0 && p->purity <= 1.0)) return fail("The purity must be in [0,1]");
+    if (!(p->error_rate >= 0.0 && p->error_rate < 1.0))
+        return fail("The parameter \"error_rate\" must be a positive real number.");
+    if (ploidy < 1 || (uint64_t)f->n_leaves * ploidy >= ((uint64_t)1 << 32)) return fail("invalid ploidy");
+    seq_ctx* s = (seq_ctx*)calloc(1, sizeof(seq_ctx));
+    s->f = f; s->m = m; s->n_mut = n_mut; s->ploidy = ploidy; s->L = p->read_size;
+    s->pe = p->insert_size_mean > 0; s->preneo_in_normal = p->preneoplastic_in_normal != 0;
+    s->seed = seed; s->e = p->error_rate;
+    int rc = 0;
+    if (s->pe) rc = insert_table(p->insert_size_mean, p->insert_size_stddev, &s->ins_k0, &s->ins_n, s->ins_T);
+    if (!rc && s->e > 0.0) rc = error_table(s->L, s->e, &s->err_n, s->err_T);
+    /* chromosome set: sorted, unique (std::set semantics of get_relevant_chr_set) */
+    uint8_t sel[PSX_MAX_CHROMOSOMES];
+    memset(sel, 0, sizeof sel);
+    if (p->n_chromosomes == 0) memset(sel, 1, n_chr);
+    for (uint32_t k = 0; k < p->n_chromosomes && !rc; ++k) {
+        if (p->chromosomes[k] >= n_chr) rc = fail("unknown chromosome %u", p->chromosomes[k]);
+        else sel[p->chromosomes[k]] = 1;
+    }
+    uint32_t n_req = p->n_samples ? p->n_samples : f->n_samples;
+    uint32_t* req = (uint32_t*)malloc(sizeof(uint32_t) * (n_req + 1));
+    for (uint32_t k = 0; k < n_req && !rc; ++k) {
+        req[k] = p->n_samples ? p->samples[k] : k;
+        if (req[k] >= f->n_samples) rc = fail("unknown sample %u", req[k]);
+        else if (f->sample_begin[req[k] + 1] == f->sample_begin[req[k]]) rc = fail("sample %u has no cells", req[k]);
+    }
+    for (uint64_t i = 1; i < n_mut && !rc; ++i)
+        if (sid_key(&m[i]) < sid_key(&m[i - 1])) rc = fail("mutation table is not in SID order");
+    if (rc) { free(req); free(s); return rc; }
+    uint32_t n_out = n_req + (p->with_normal_sample ? 1u : 0u);
+    {
+        uint64_t i = 0;
+        for (uint32_t c = 0; c <= n_chr; ++c) {
+            while (i < n_mut && m[i].chr < c) ++i;
+            s->chr_begin[c] = i;
+        }
+        for (uint32_t c = n_chr + 1; c <= PSX_MAX_CHROMOSOMES; ++c) s->chr_begin[c] = n_mut;
+    }
+    memset(alt, 0, sizeof(int32_t) * n_mut * n_out);
+    memset(depth, 0, sizeof(int32_t) * n_mut * n_out);
+    uint64_t thr_normal = (uint64_t)((1.0 - p->purity) * TWO32);
+    /* present */
+    for (uint32_t so = 0; so < n_out; ++so) {
+        int is_normal = so == n_req;
+        const uint32_t* sl = is_normal ? NULL : f->sample_leaves + f->sample_begin[req[so]];
+        uint32_t ns = is_normal ? 0 : f->sample_begin[req[so] + 1] - f->sample_begin[req[so]];
+        for (uint64_t i = 0; i < n_mut; ++i) {
+            int pres = 0;
+            if (is_normal || p->purity < 1.0) pres = normal_carries(s, &m[i]);
+            if (!is_normal && !pres) {
+                uint32_t lo = 0, hi = ns;
+                while (lo < hi) {
+                    uint32_t mid = (lo + hi) / 2;
+                    if (sl[mid] < m[i].leaf_lo) lo = mid + 1; else hi = mid;
+                }
+                pres = lo < ns && sl[lo] < m[i].leaf_hi;
+            }
+            present[(uint64_t)so * n_mut + i] = (uint8_t)pres;
+        }
+    }
+    /* strata */
+    uint64_t n_strata = 0, reads = 0;
+    for (uint32_t so = 0; so < n_out; ++so)
+        for (uint32_t c = 0; c < n_chr; ++c)
+            if (sel[c] && len[c] >= s->L) n_strata += (len[c] - s->L + 1 + PSX_STRATUM - 1) / PSX_STRATUM;
+    stratum_ref* st = (stratum_ref*)malloc(sizeof(stratum_ref) * (n_strata ? n_strata : 1));
+    uint64_t q = 0;
+    for (uint32_t so = 0; so < n_out; ++so)
+        for (uint32_t c = 0; c < n_chr; ++c) {
+            if (!sel[c] || len[c] < s->L) continue;
+            uint32_t W = len[c] - s->L + 1;
+            uint32_t J = (W + PSX_STRATUM - 1) / PSX_STRATUM;
+            for (uint32_t j = 0; j < J; ++j) st[q++] = (stratum_ref){so, c, j};
+            uint64_t T = (uint64_t)llround(p->coverage * (double)len[c] / (double)(s->pe ? 2 * s->L : s->L));
+            reads += T * (s->pe ? 2u : 1u);
+        }
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 8)
+#endif
+    for (int64_t qi = 0; qi < (int64_t)n_strata; ++qi) {
+        uint32_t so = st[qi].so, c = st[qi].c, j = st[qi].j;
+        int is_normal = so == n_req;
+        const uint32_t* sl = is_normal ? NULL : f->sample_leaves + f->sample_begin[req[so]];
+        uint32_t ns = is_normal ? 0 : f->sample_begin[req[so] + 1] - f->sample_begin[req[so]];
+        uint32_t W = len[c] - s->L + 1;
+        uint64_t T = (uint64_t)llround(p->coverage * (double)len[c] / (double)(s->pe ? 2 * s->L : s->L));
+        uint32_t b = j * PSX_STRATUM, e = b + PSX_STRATUM < W ? b + PSX_STRATUM : W;
+        uint64_t nj = T * e / W - T * b / W;
+        uint32_t c2 = (c << 16) | so;
+        int32_t* a_out = alt + (uint64_t)so * n_mut;
+        int32_t* d_out = depth + (uint64_t)so * n_mut;
+        for (uint32_t r = 0; r < (uint32_t)nj; ++r) {
+            uint32_t w[4];
+            philox4(seed, r, j, c2, (uint32_t)P_TEMPLATE << 24, w);
+            uint32_t x = b + mulhi32(w[0], e - b);
+            uint32_t leaf, allele;
+            if (is_normal || (uint64_t)w[1] < thr_normal) {
+                leaf = f->n_leaves;
+                allele = mulhi32(w[2], ploidy);
+            } else {
+                uint32_t X = mulhi32(w[2], ns * ploidy);
+                leaf = sl[X / ploidy];
+                allele = X % ploidy;
+            }
+            seq_read(s, c, x, leaf, allele, r, j, c2, 0, a_out, d_out);
+            if (s->pe) {
+                uint32_t I = s->ins_k0 + pick(s->ins_T, s->ins_n, w[3]);
+                if (I < s->L) I = s->L;
+                uint64_t x2 = (uint64_t)x + I - s->L;
+                if (x2 > len[c] - s->L) x2 = len[c] - s->L;
+                seq_read(s, c, (uint32_t)x2, leaf, allele, r, j, c2, 1, a_out, d_out);
+            }
+        }
+    }
+    free(st); free(req); free(s);
+    *n_reads = reads;
+    return 0;
+}
+
+/* ------------------------------------------------------------- result table */
+
+/* get_active_mutations + add_sample_statistics (src/seq_simulation.cpp:92-167):
+ * a row per SID with an alternative read in some sample (or, when
+ * include_non_sequenced, held by some sample); VAF = occ/cov when the sample
+ * holds the SID, 0 otherwise (:123-130). */
+int orc_seq_table(const psx_mutation* recs, uint64_t n, uint32_t S, const int32_t* alt,
+                  const int32_t* depth, const uint8_t* present, int32_t include,
+                  uint64_t* n_rows, uint64_t* row_rec, int32_t* occ, int32_t* cov, double* vaf) {
+    uint64_t rows = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        if (!recs[i].sid_head) continue;
+        uint64_t g1 = i + 1;
+        while (g1 < n && !recs[g1].sid_head) ++g1;
+        int any_occ = 0, any_pres = 0;
+        for (uint32_t s = 0; s < S; ++s) {
+            any_occ |= alt[(uint64_t)s * n + i] > 0;
+            for (uint64_t q = i; q < g1; ++q) any_pres |= present[(uint64_t)s * n + q] != 0;
+        }
+        if (!(any_occ || (include && any_pres))) continue;
+        row_rec[rows] = i;
+        for (uint32_t s = 0; s < S; ++s) {
+            int32_t o = alt[(uint64_t)s * n + i], cv = depth[(uint64_t)s * n + i];
+            int in_data = o > 0;
+            for (uint64_t q = i; q < g1; ++q) in_data |= present[(uint64_t)s * n + q] != 0;
+            occ[(uint64_t)s * n + rows] = in_data ? o : 0;
+            cov[(uint64_t)s * n + rows] = cv;
+            vaf[(uint64_t)s * n + rows] = in_data ? (double)o / (double)cv : 0.0;
+        }
+        ++rows;
+    }
+    *n_rows = rows;
+    return 0;
+}

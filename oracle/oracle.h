@@ -1,0 +1,80 @@
+/*
+ * CPU oracle: TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C restatement of the sampling model of mutation placement and
+ * count-only sequencing. Only tests/, the smoke check and the CPU-baseline
+ * leg of bench.py may load it, and only as the checker: the product library
+ * never calls it. Structurally it is the naive form (one read at a time,
+ * binary search over the whole mutation table, sequential placement), written
+ * independently of the HIP kernels; it shares with them only the plain-data
+ * structs of include/psx.h and the published Philox4x32-10 algorithm.
+ *
+ * Reference semantics followed (ProCESS sources; the RACES library that holds
+ * the arithmetic is not available, so the sampling model is this project's
+ * own and is pinned by closed forms, invariants and brute force in tests/):
+ *   place_mutations      src/mutation_engine.cpp:1085-1113,
+ *                        src/mutation_engine.hpp:148-206
+ *   simulate_seq         src/seq_simulation.cpp:354-451,517-601,
+ *                        src/sequencing.cpp:193-209
+ *   result table         src/seq_simulation.cpp:52-181
+ */
+#ifndef PSX_ORACLE_H
+#define PSX_ORACLE_H
+
+#include "../include/psx.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_forest {
+    uint32_t n_nodes, n_leaves, n_samples;
+    int64_t* cell_id;      /* by DFS index */
+    uint32_t* parent;      /* DFS index of the parent, PSX_NO_NODE for roots */
+    uint32_t* species;
+    double* birth;
+    uint32_t* leaf_lo;
+    uint32_t* leaf_hi;
+    uint32_t* sample_begin;  /* n_samples + 1 */
+    uint32_t* sample_leaves; /* leaf ranks, ascending within each sample */
+} orc_forest;
+
+const char* orc_last_error(void);
+void orc_free(void* p);
+
+uint64_t orc_genome_words(uint32_t n_chr, const uint32_t* len);
+int orc_genome_synth(uint32_t n_chr, const uint32_t* len, uint64_t seed, uint32_t* packed);
+
+int orc_forest_flatten(const psx_forest_desc* d, orc_forest** out);
+void orc_forest_free(orc_forest* f);
+
+int orc_place_mutations(uint32_t n_chr, const uint32_t* len, const uint32_t* packed,
+                        const orc_forest* f, const psx_engine_params* p, uint64_t seed,
+                        psx_mutation** out, uint64_t* n_out);
+
+/* Sorts an arbitrary table into SID order and marks sid_head. */
+int orc_sort_table(psx_mutation* m, uint64_t n);
+
+int orc_simulate_seq(uint32_t n_chr, const uint32_t* len, const orc_forest* f,
+                     const psx_mutation* m, uint64_t n_mut, uint32_t ploidy,
+                     const psx_seq_params* p, uint64_t seed, int32_t* alt, int32_t* depth,
+                     uint8_t* present, uint64_t* n_reads);
+
+int orc_seq_table(const psx_mutation* recs, uint64_t n_mutations, uint32_t n_out_samples,
+                  const int32_t* alt, const int32_t* depth, const uint8_t* present,
+                  int32_t include_non_sequenced, uint64_t* n_rows, uint64_t* row_rec,
+                  int32_t* occ, int32_t* cov, double* vaf);
+
+/* Exposed for the known-answer tests. */
+void orc_philox(const uint32_t ctr[4], uint64_t key, uint32_t out[4]);
+/* Integer cumulative table of a pmf, as used by every inversion draw. */
+int orc_cdf_table(const double* pmf, uint32_t n, uint64_t* T);
+/* Inversion tables (value = k0 + index): Poisson(lambda), Binomial(t, p). */
+int orc_poisson_table(double lambda, uint32_t* k0, uint32_t* n, uint64_t* T, uint32_t cap);
+int orc_binomial_table(uint32_t t, double p, uint32_t* k0, uint32_t* n, uint64_t* T,
+                       uint32_t cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

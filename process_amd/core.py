@@ -1,0 +1,362 @@
+"""Thin owners of the libpsx device handles (context, genome, forest, mutation
+table) plus the array-level calls of the C ABI. The R-style API in
+process_amd.api is built on these."""
+from __future__ import annotations
+
+import ctypes as C
+import threading
+
+import numpy as np
+
+from . import _abi as A
+from ._lib import check, lib
+
+_default_ctx = None
+_ctx_lock = threading.Lock()
+
+
+class Context:
+    """One HIP stream on one device (psx_ctx)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        st = lib().psx_ctx_create(device, C.byref(h))
+        self.h = h
+        if st != 0:
+            try:
+                check(st, h)
+            finally:
+                lib().psx_ctx_destroy(h)
+                self.h = None
+        self.device = device
+
+    def close(self):
+        if self.h:
+            lib().psx_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def synchronize(self):
+        check(lib().psx_ctx_synchronize(self.h), self.h)
+
+    def last_kernel_ms(self) -> float:
+        v = C.c_double()
+        check(lib().psx_ctx_last_kernel_ms(self.h, C.byref(v)), self.h)
+        return v.value
+
+    def last_place_ms(self) -> float:
+        v = C.c_double()
+        check(lib().psx_ctx_last_place_ms(self.h, C.byref(v)), self.h)
+        return v.value
+
+    def mem_info(self) -> tuple[int, int]:
+        f, t = C.c_uint64(), C.c_uint64()
+        check(lib().psx_ctx_mem_info(self.h, C.byref(f), C.byref(t)), self.h)
+        return f.value, t.value
+
+
+class DeviceArray:
+    """A device buffer owned through libpsx (psx_dev_alloc), with numpy copies.
+
+    libpsx and the ROCm runtime bundled with torch are different HIP runtimes
+    and cannot share the GPU in one process, so device buffers for the
+    device-pointer entry points come from the library itself."""
+
+    def __init__(self, ctx: Context, shape, dtype):
+        self.ctx = ctx
+        self.shape = tuple(int(s) for s in np.atleast_1d(shape))
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = C.c_void_p()
+        check(lib().psx_dev_alloc(ctx.h, self.nbytes, C.byref(p)), ctx.h)
+        self.ptr = p.value
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.shape, self.dtype)
+        check(lib().psx_dev_download(self.ctx.h, out.ctypes.data_as(C.c_void_p), self.ptr, self.nbytes), self.ctx.h)
+        return out
+
+    def upload(self, a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a, self.dtype)
+        assert a.nbytes == self.nbytes
+        check(lib().psx_dev_upload(self.ctx.h, self.ptr, a.ctypes.data_as(C.c_void_p), self.nbytes), self.ctx.h)
+        return self
+
+    def fill(self, byte: int) -> "DeviceArray":
+        check(lib().psx_dev_fill_u8(self.ctx.h, self.ptr, byte, self.nbytes), self.ctx.h)
+        return self
+
+    def close(self):
+        if self.ptr:
+            lib().psx_dev_free(self.ctx.h, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def default_context() -> Context:
+    global _default_ctx
+    with _ctx_lock:
+        if _default_ctx is None:
+            _default_ctx = Context(0)
+        return _default_ctx
+
+
+class _Handle:
+    _free = None
+
+    def __init__(self, ctx: Context, h: C.c_void_p):
+        self.ctx = ctx
+        self.h = h
+
+    def close(self):
+        if self.h:
+            getattr(lib(), self._free)(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Genome(_Handle):
+    """Device-resident 2-bit packed reference (psx_genome)."""
+
+    _free = "psx_genome_free"
+
+    def __init__(self, ctx: Context, lengths, *, seed: int | None = None, packed=None):
+        L = np.ascontiguousarray(lengths, np.uint32)
+        h = C.c_void_p()
+        if packed is None:
+            check(lib().psx_genome_synth(ctx.h, len(L), A.ptr(L, C.c_uint32), C.c_uint64(seed or 0), C.byref(h)),
+                  ctx.h)
+        else:
+            pk = np.ascontiguousarray(packed, np.uint32)
+            check(lib().psx_genome_upload(ctx.h, len(L), A.ptr(L, C.c_uint32), A.ptr(pk, C.c_uint32), C.byref(h)),
+                  ctx.h)
+        super().__init__(ctx, h)
+        self.lengths = L
+        self.seed = seed
+
+    def download(self) -> np.ndarray:
+        n = int(lib().psx_genome_words(len(self.lengths), A.ptr(self.lengths, C.c_uint32)))
+        out = np.zeros(n, np.uint32)
+        check(lib().psx_genome_download(self.ctx.h, self.h, A.ptr(out, C.c_uint32), n), self.ctx.h)
+        return out
+
+
+class Forest(_Handle):
+    """Flattened, device-resident samples forest (psx_forest)."""
+
+    _free = "psx_forest_free"
+
+    def __init__(self, ctx: Context, cell_id, parent, species, sample, birth, n_samples: int):
+        desc, keep = A.forest_desc(cell_id, parent, species, sample, birth, n_samples)
+        h = C.c_void_p()
+        check(lib().psx_forest_upload(ctx.h, C.byref(desc), C.byref(h)), ctx.h)
+        del keep
+        super().__init__(ctx, h)
+        nn, nl, ns = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        check(lib().psx_forest_info(h, C.byref(nn), C.byref(nl), C.byref(ns)))
+        self.n_nodes, self.n_leaves, self.n_samples = nn.value, nl.value, ns.value
+
+    def flat(self) -> dict:
+        n, nl, ns = self.n_nodes, self.n_leaves, self.n_samples
+        out = dict(cell_id=np.zeros(n, np.int64), parent=np.zeros(n, np.uint32), leaf_lo=np.zeros(n, np.uint32),
+                   leaf_hi=np.zeros(n, np.uint32), sample_begin=np.zeros(ns + 1, np.uint32),
+                   sample_leaves=np.zeros(max(nl, 1), np.uint32))
+        check(lib().psx_forest_flat(self.h, A.ptr(out["cell_id"], C.c_int64), A.ptr(out["parent"], C.c_uint32),
+                                    A.ptr(out["leaf_lo"], C.c_uint32), A.ptr(out["leaf_hi"], C.c_uint32),
+                                    A.ptr(out["sample_begin"], C.c_uint32), A.ptr(out["sample_leaves"], C.c_uint32)))
+        out["sample_leaves"] = out["sample_leaves"][: int(out["sample_begin"][-1])]
+        return out
+
+
+class Mutations(_Handle):
+    """Device-resident mutation table in SID order (psx_mutations)."""
+
+    _free = "psx_mutations_free"
+
+    def __init__(self, ctx: Context, h: C.c_void_p, ploidy: int):
+        super().__init__(ctx, h)
+        n = C.c_uint64()
+        check(lib().psx_mutations_count(h, C.byref(n)))
+        self.n = n.value
+        self.ploidy = ploidy
+
+    @classmethod
+    def place(cls, ctx: Context, genome: Genome, forest: Forest, seed: int, **engine) -> "Mutations":
+        params, keep = A.engine_params(**engine)
+        h = C.c_void_p()
+        check(lib().psx_place_mutations(ctx.h, genome.h, forest.h, C.byref(params), C.c_uint64(seed), C.byref(h)),
+              ctx.h)
+        del keep
+        return cls(ctx, h, int(engine.get("ploidy", 2)))
+
+    @classmethod
+    def upload(cls, ctx: Context, forest: Forest, recs: np.ndarray, ploidy: int = 2) -> "Mutations":
+        r = np.ascontiguousarray(recs, A.MUTATION_DTYPE)
+        h = C.c_void_p()
+        check(lib().psx_mutations_upload(ctx.h, forest.h, r.ctypes.data_as(C.c_void_p), len(r), ploidy,
+                                         C.byref(h)), ctx.h)
+        return cls(ctx, h, ploidy)
+
+    def download(self) -> np.ndarray:
+        out = np.zeros(self.n, A.MUTATION_DTYPE)
+        check(lib().psx_mutations_download(self.ctx.h, self.h, out.ctypes.data_as(C.c_void_p), self.n), self.ctx.h)
+        return out
+
+
+def num_out_samples(forest: Forest, **seq) -> int:
+    p, keep = A.seq_params(**seq)
+    return int(lib().psx_seq_num_samples(forest.h, C.byref(p)))
+
+
+def simulate_seq_arrays(ctx: Context, genome: Genome, forest: Forest, muts: Mutations, seed: int, **seq):
+    """Count-only sequencing into host arrays: (alt, depth, present, n_reads),
+    each [n_out_samples, n_mutations]."""
+    p, keep = A.seq_params(**seq)
+    S = int(lib().psx_seq_num_samples(forest.h, C.byref(p)))
+    M = muts.n
+    alt = np.zeros(S * M, np.int32)
+    dep = np.zeros(S * M, np.int32)
+    pres = np.zeros(S * M, np.uint8)
+    nr = C.c_uint64()
+    check(lib().psx_simulate_seq(ctx.h, genome.h, forest.h, muts.h, C.byref(p), C.c_uint64(seed),
+                                 A.ptr(alt, C.c_int32), A.ptr(dep, C.c_int32), A.ptr(pres, C.c_uint8), C.byref(nr)),
+          ctx.h)
+    del keep
+    return alt.reshape(S, M), dep.reshape(S, M), pres.reshape(S, M), nr.value
+
+
+def num_reads(genome: Genome, forest: Forest, **seq) -> int:
+    p, keep = A.seq_params(**seq)
+    n = C.c_uint64()
+    check(lib().psx_seq_num_reads(genome.h, forest.h, C.byref(p), C.byref(n)))
+    return n.value
+
+
+class SeqJob:
+    """One job of a batched launch, with device-resident outputs."""
+
+    def __init__(self, ctx: Context, genome: Genome, forest: Forest, muts: Mutations, seed: int, **seq):
+        self.genome, self.forest, self.muts, self.seed, self.seq = genome, forest, muts, int(seed), seq
+        self.n_out = num_out_samples(forest, **seq)
+        shape = (self.n_out, max(muts.n, 1))
+        self.alt = DeviceArray(ctx, shape, np.int32)
+        self.depth = DeviceArray(ctx, shape, np.int32)
+        self.present = DeviceArray(ctx, shape, np.uint8)
+
+    def results(self):
+        M = self.muts.n
+        return (self.alt.numpy()[:, :M], self.depth.numpy()[:, :M], self.present.numpy()[:, :M])
+
+
+def _job_array(jobs):
+    arr = (A.SeqJob * len(jobs))()
+    keep = []
+    for i, j in enumerate(jobs):
+        p, k = A.seq_params(**j.seq)
+        keep.append(k)
+        arr[i] = A.SeqJob(j.genome.h, j.forest.h, j.muts.h, p, j.seed, j.alt.ptr, j.depth.ptr, j.present.ptr)
+    return arr, keep
+
+
+def simulate_seq_batch(ctx: Context, jobs: list, config: A.LaunchConfig | None = None) -> int:
+    """All jobs in one persistent launch; returns the simulated reads."""
+    arr, keep = _job_array(jobs)
+    nr = C.c_uint64()
+    check(lib().psx_simulate_seq_batch(ctx.h, arr, len(jobs), None if config is None else C.byref(config),
+                                       C.byref(nr)), ctx.h)
+    return nr.value
+
+
+def autotune(ctx: Context, jobs: list) -> tuple[A.LaunchConfig, float]:
+    arr, keep = _job_array(jobs)
+    best = A.LaunchConfig()
+    ms = C.c_double()
+    check(lib().psx_autotune_seq(ctx.h, arr, len(jobs), C.byref(best), C.byref(ms)), ctx.h)
+    return best, ms.value
+
+
+def simulate_seq_device(ctx: Context, genome: Genome, forest: Forest, muts: Mutations, seed: int, alt: DeviceArray,
+                        depth: DeviceArray, present: DeviceArray, **seq) -> int:
+    p, keep = A.seq_params(**seq)
+    nr = C.c_uint64()
+    check(lib().psx_simulate_seq_device(ctx.h, genome.h, forest.h, muts.h, C.byref(p), C.c_uint64(seed), alt.ptr,
+                                        depth.ptr, present.ptr, C.byref(nr)), ctx.h)
+    return nr.value
+
+
+def read_words(**seq) -> int:
+    p, keep = A.seq_params(**seq)
+    return int(lib().psx_read_words(C.byref(p)))
+
+
+def simulate_seq_reads(ctx: Context, genome: Genome, forest: Forest, muts: Mutations, seed: int,
+                       out: DeviceArray | None = None, **seq):
+    """Materialised reads into a device buffer of shape [n_reads, read_words]."""
+    n = num_reads(genome, forest, **seq)
+    rw = read_words(**seq)
+    if out is None:
+        out = DeviceArray(ctx, (max(n, 1), rw), np.uint32)
+    p, keep = A.seq_params(**seq)
+    got = C.c_uint64()
+    check(lib().psx_simulate_seq_reads(ctx.h, genome.h, forest.h, muts.h, C.byref(p), C.c_uint64(seed), out.ptr,
+                                       out.shape[0], C.byref(got)), ctx.h)
+    return out, got.value
+
+
+def seq_table(recs: np.ndarray, alt, depth, present, include_non_sequenced: bool = False):
+    """psx_seq_table: (row record indices, occ, cov, vaf) with [sample, row] arrays."""
+    r = np.ascontiguousarray(recs, A.MUTATION_DTYPE)
+    S, M = alt.shape
+    Mx = max(M, 1)
+    a = np.ascontiguousarray(alt, np.int32)
+    d = np.ascontiguousarray(depth, np.int32)
+    p = np.ascontiguousarray(present, np.uint8)
+    rows = np.zeros(Mx, np.uint64)
+    occ = np.zeros(S * Mx, np.int32)
+    cov = np.zeros(S * Mx, np.int32)
+    vaf = np.zeros(S * Mx, np.float64)
+    nr = C.c_uint64()
+    check(lib().psx_seq_table(r.ctypes.data_as(C.c_void_p), M, S, A.ptr(a, C.c_int32), A.ptr(d, C.c_int32),
+                              A.ptr(p, C.c_uint8), int(bool(include_non_sequenced)), C.byref(nr),
+                              A.ptr(rows, C.c_uint64), A.ptr(occ, C.c_int32), A.ptr(cov, C.c_int32),
+                              A.ptr(vaf, C.c_double)))
+    n = nr.value
+    return (rows[:n].copy(), occ.reshape(S, Mx)[:, :n].copy(), cov.reshape(S, Mx)[:, :n].copy(),
+            vaf.reshape(S, Mx)[:, :n].copy())
+
+
+def synth_forest(*, growth, death, intro_at, n_cells: int, n_samples: int, sample_size: int, seed: int) -> dict:
+    """Synthetic birth-death samples forest (host generator in libpsx)."""
+    g = np.ascontiguousarray(growth, np.float64)
+    d = np.ascontiguousarray(death, np.float64)
+    ia = np.ascontiguousarray(intro_at, np.uint64)
+    sp = A.SynthParams(len(g), A.ptr(g, C.c_double), A.ptr(d, C.c_double), A.ptr(ia, C.c_uint64), int(n_cells),
+                       int(n_samples), int(sample_size))
+    h = C.c_void_p()
+    check(lib().psx_synth_forest_run(C.byref(sp), C.c_uint64(seed), C.byref(h)))
+    try:
+        n = lib().psx_synth_forest_nodes(h)
+        out = dict(cell_id=np.zeros(n, np.int64), parent=np.zeros(n, np.int64), species=np.zeros(n, np.uint32),
+                   sample=np.zeros(n, np.int32), birth=np.zeros(n, np.float64))
+        check(lib().psx_synth_forest_get(h, A.ptr(out["cell_id"], C.c_int64), A.ptr(out["parent"], C.c_int64),
+                                         A.ptr(out["species"], C.c_uint32), A.ptr(out["sample"], C.c_int32),
+                                         A.ptr(out["birth"], C.c_double)))
+    finally:
+        lib().psx_synth_forest_free(h)
+    out["n_samples"] = int(n_samples)
+    return out

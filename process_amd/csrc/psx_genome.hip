@@ -1,0 +1,293 @@
+// Device-resident reference genome (2-bit packed) and the device layout of
+// mutation tables (SoA arrays plus a per-stratum index).
+#include <algorithm>
+#include <numeric>
+
+#include "psx_internal.h"
+
+namespace {
+
+struct genome_synth_args {
+    uint32_t* packed;
+    uint64_t seed;
+    uint32_t n_chr;
+    uint32_t len[PSX_MAX_CHROMOSOMES];
+    uint64_t woff[PSX_MAX_CHROMOSOMES];
+    uint64_t qoff[PSX_MAX_CHROMOSOMES + 1];  // first Philox block of each chromosome
+};
+
+// One thread per Philox block: 4 packed words = 64 bases. Word w of
+// chromosome c is word (w % 4) of Philox(seed, (w / 4, c, 0, REF << 24)).
+__global__ void __launch_bounds__(256) k_genome_synth(const genome_synth_args a) {
+    const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= a.qoff[a.n_chr]) return;
+    uint32_t c = 0;
+    while (q >= a.qoff[c + 1]) ++c;
+    const uint64_t lb = q - a.qoff[c];
+    const psx_u4 w = psx_draw(a.seed, (uint32_t)lb, c, 0u, PSX_P_REF << 24);
+    const uint64_t words = ((uint64_t)a.len[c] + 15u) / 16u;
+    const uint32_t rem = a.len[c] % 16u;
+    for (int i = 0; i < 4; ++i) {
+        const uint64_t wi = lb * 4u + (uint64_t)i;
+        if (wi >= words) break;
+        uint32_t v = psx_word(w, i);
+        if (wi + 1 == words && rem) v &= (1u << (2u * rem)) - 1u;
+        a.packed[a.woff[c] + wi] = v;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_mut_soa(const psx_mutation* __restrict__ recs, uint64_t n,
+                                                 uint32_t* __restrict__ pos, uint32_t* __restrict__ lo,
+                                                 uint32_t* __restrict__ hi, uint32_t* __restrict__ info) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const psx_mutation r = recs[i];
+    pos[i] = r.pos;
+    lo[i] = r.leaf_lo;
+    hi[i] = r.leaf_hi;
+    info[i] = uint32_t(r.allele) | (uint32_t(r.ref & 3u) << 8) | (uint32_t(r.alt & 3u) << 10) |
+              (uint32_t(r.kind & 3u) << 12) | (uint32_t(r.nature & 3u) << 14) |
+              (uint32_t(r.sid_head & 1u) << 16);
+}
+
+struct bucket_args {
+    const uint32_t* pos;
+    uint32_t* bucket;
+    uint32_t n_chr;
+    uint64_t chr_begin[PSX_MAX_CHROMOSOMES + 1];
+    uint64_t boff[PSX_MAX_CHROMOSOMES + 1];  // first bucket entry of each chromosome
+};
+
+// bucket[boff[c] + k] = first record of chromosome c with pos >= k * PSX_STRATUM
+// (as an absolute record index).
+__global__ void __launch_bounds__(256) k_mut_bucket(const bucket_args a) {
+    const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= a.boff[a.n_chr]) return;
+    uint32_t c = 0;
+    while (q >= a.boff[c + 1]) ++c;
+    const uint64_t k = q - a.boff[c];
+    const uint64_t target = k * PSX_STRATUM;
+    uint64_t lo = a.chr_begin[c], hi = a.chr_begin[c + 1];
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if ((uint64_t)a.pos[mid] < target) lo = mid + 1; else hi = mid;
+    }
+    a.bucket[q] = (uint32_t)lo;
+}
+
+}  // namespace
+
+static void genome_layout(psx_genome* g, uint32_t n_chr, const uint32_t* chr_len) {
+    if (n_chr == 0 || n_chr > PSX_MAX_CHROMOSOMES)
+        psx_throw(PSX_EINVAL, "the genome must have 1 to %u chromosomes", PSX_MAX_CHROMOSOMES);
+    g->n_chr = n_chr;
+    g->len.assign(chr_len, chr_len + n_chr);
+    g->woff.assign(n_chr, 0);
+    g->boff.assign(n_chr + 1, 0);
+    uint64_t w = 0;
+    for (uint32_t c = 0; c < n_chr; ++c) {
+        if (chr_len[c] < 3) psx_throw(PSX_EINVAL, "chromosome %u is shorter than 3 bases", c);
+        g->woff[c] = w;
+        g->boff[c + 1] = g->boff[c] + chr_len[c];
+        w += ((uint64_t)chr_len[c] + 15u) / 16u;
+    }
+    g->words = w;
+}
+
+extern "C" uint64_t psx_genome_words(uint32_t n_chr, const uint32_t* chr_len) {
+    uint64_t w = 0;
+    for (uint32_t c = 0; c < n_chr; ++c) w += ((uint64_t)chr_len[c] + 15u) / 16u;
+    return w;
+}
+
+extern "C" psx_status psx_genome_synth(psx_ctx* ctx, uint32_t n_chr, const uint32_t* chr_len,
+                                       uint64_t ref_seed, psx_genome** out) {
+    if (!ctx || !chr_len || !out) return PSX_EINVAL;
+    psx_genome* g = new psx_genome();
+    g->ctx = ctx;
+    psx_status st = psx_guard(ctx, [&] {
+        genome_layout(g, n_chr, chr_len);
+        PSX_HIP(hipSetDevice(ctx->device));
+        g->packed.alloc(g->words * 4u);
+        genome_synth_args a{};
+        a.packed = g->packed.as<uint32_t>();
+        a.seed = ref_seed;
+        a.n_chr = n_chr;
+        a.qoff[0] = 0;
+        for (uint32_t c = 0; c < n_chr; ++c) {
+            a.len[c] = chr_len[c];
+            a.woff[c] = g->woff[c];
+            a.qoff[c + 1] = a.qoff[c] + (((uint64_t)chr_len[c] + 15u) / 16u + 3u) / 4u;
+        }
+        const uint64_t blocks = (a.qoff[n_chr] + 255u) / 256u;
+        hipLaunchKernelGGL(k_genome_synth, dim3((uint32_t)blocks), dim3(256), 0, ctx->stream, a);
+        PSX_HIP(hipGetLastError());
+        PSX_HIP(hipStreamSynchronize(ctx->stream));
+    });
+    if (st != PSX_OK) { delete g; return st; }
+    *out = g;
+    return PSX_OK;
+}
+
+extern "C" psx_status psx_genome_upload(psx_ctx* ctx, uint32_t n_chr, const uint32_t* chr_len,
+                                        const uint32_t* packed, psx_genome** out) {
+    if (!ctx || !chr_len || !packed || !out) return PSX_EINVAL;
+    psx_genome* g = new psx_genome();
+    g->ctx = ctx;
+    psx_status st = psx_guard(ctx, [&] {
+        genome_layout(g, n_chr, chr_len);
+        PSX_HIP(hipSetDevice(ctx->device));
+        g->packed.alloc(g->words * 4u);
+        PSX_HIP(hipMemcpyAsync(g->packed.p, packed, g->words * 4u, hipMemcpyHostToDevice, ctx->stream));
+        PSX_HIP(hipStreamSynchronize(ctx->stream));
+    });
+    if (st != PSX_OK) { delete g; return st; }
+    *out = g;
+    return PSX_OK;
+}
+
+extern "C" psx_status psx_genome_download(psx_ctx* ctx, const psx_genome* g, uint32_t* packed,
+                                          uint64_t n_words) {
+    if (!ctx || !g || !packed) return PSX_EINVAL;
+    return psx_guard(ctx, [&] {
+        if (n_words < g->words) psx_throw(PSX_EINVAL, "buffer holds %llu words, need %llu",
+                                          (unsigned long long)n_words, (unsigned long long)g->words);
+        PSX_HIP(hipSetDevice(ctx->device));
+        PSX_HIP(hipMemcpyAsync(packed, g->packed.p, g->words * 4u, hipMemcpyDeviceToHost, ctx->stream));
+        PSX_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+extern "C" void psx_genome_free(psx_genome* g) {
+    if (!g) return;
+    if (g->ctx) (void)hipSetDevice(g->ctx->device);
+    delete g;
+}
+
+// d_recs must hold m->n records in SID order with sid_head set; fills the SoA
+// arrays, the chromosome ranges and the per-stratum index.
+void psx_mutations_finish(psx_ctx* ctx, psx_mutations* m, const std::vector<uint64_t>& chr_begin) {
+    const uint64_t n = m->n;
+    m->chr_begin = chr_begin;
+    m->chr_begin.resize(PSX_MAX_CHROMOSOMES + 1, n);
+    m->d_pos.alloc(n * 4u);
+    m->d_lo.alloc(n * 4u);
+    m->d_hi.alloc(n * 4u);
+    m->d_info.alloc(n * 4u);
+    if (n) {
+        hipLaunchKernelGGL(k_mut_soa, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, ctx->stream,
+                           m->d_recs.as<psx_mutation>(), n, m->d_pos.as<uint32_t>(),
+                           m->d_lo.as<uint32_t>(), m->d_hi.as<uint32_t>(), m->d_info.as<uint32_t>());
+        PSX_HIP(hipGetLastError());
+    }
+    psx_mutations_index(ctx, m, m->chr_len);
+}
+
+void psx_mutations_index(psx_ctx* ctx, psx_mutations* m, const std::vector<uint32_t>& chr_len) {
+    const uint32_t n_chr = (uint32_t)chr_len.size();
+    if (n_chr == 0 || n_chr > PSX_MAX_CHROMOSOMES) psx_throw(PSX_EINVAL, "invalid number of chromosomes");
+    m->chr_len = chr_len;
+    bucket_args a{};
+    a.pos = m->d_pos.as<uint32_t>();
+    a.n_chr = n_chr;
+    m->bucket_off.assign(n_chr + 1, 0);
+    for (uint32_t c = 0; c <= PSX_MAX_CHROMOSOMES; ++c) a.chr_begin[c] = m->chr_begin[c];
+    for (uint32_t c = 0; c < n_chr; ++c)
+        m->bucket_off[c + 1] = m->bucket_off[c] + (chr_len[c] + PSX_STRATUM - 1) / PSX_STRATUM + 1;
+    for (uint32_t c = 0; c <= n_chr; ++c) a.boff[c] = m->bucket_off[c];
+    m->d_bucket.alloc(m->bucket_off[n_chr] * 4u);
+    a.bucket = m->d_bucket.as<uint32_t>();
+    hipLaunchKernelGGL(k_mut_bucket, dim3((uint32_t)((m->bucket_off[n_chr] + 255) / 256)), dim3(256), 0,
+                       ctx->stream, a);
+    PSX_HIP(hipGetLastError());
+}
+
+// Host path (replay uploads): SID sort with ties in input order.
+void psx_mutations_build(psx_ctx* ctx, psx_mutations* m, std::vector<psx_mutation>& recs,
+                         const std::vector<uint32_t>& chr_len) {
+    const uint64_t n = recs.size();
+    std::vector<uint64_t> key(n), idx(n);
+    for (uint64_t i = 0; i < n; ++i) key[i] = psx_sid_key(recs[i]);
+    std::iota(idx.begin(), idx.end(), 0ull);
+    std::stable_sort(idx.begin(), idx.end(), [&](uint64_t a, uint64_t b) { return key[a] < key[b]; });
+    std::vector<psx_mutation> sorted(n);
+    for (uint64_t i = 0; i < n; ++i) {
+        sorted[i] = recs[idx[i]];
+        sorted[i].sid_head = (i == 0 || key[idx[i]] != key[idx[i - 1]]) ? 1 : 0;
+    }
+    recs.swap(sorted);
+    std::vector<uint64_t> chr_begin(PSX_MAX_CHROMOSOMES + 1, n);
+    {
+        uint64_t i = 0;
+        for (uint32_t c = 0; c <= PSX_MAX_CHROMOSOMES; ++c) {
+            while (i < n && recs[i].chr < c) ++i;
+            chr_begin[c] = i;
+        }
+    }
+    m->n = n;
+    m->chr_len = chr_len;
+    m->d_recs.alloc(n * sizeof(psx_mutation));
+    if (n)
+        PSX_HIP(hipMemcpyAsync(m->d_recs.p, recs.data(), n * sizeof(psx_mutation), hipMemcpyHostToDevice,
+                               ctx->stream));
+    psx_mutations_finish(ctx, m, chr_begin);
+    PSX_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+extern "C" psx_status psx_mutations_upload(psx_ctx* ctx, const psx_forest* f, const psx_mutation* recs,
+                                           uint64_t n, uint32_t ploidy, psx_mutations** out) {
+    if (!ctx || !f || (!recs && n) || !out) return PSX_EINVAL;
+    psx_mutations* m = new psx_mutations();
+    m->ctx = ctx;
+    psx_status st = psx_guard(ctx, [&] {
+        if (ploidy < 1 || ploidy > 255) psx_throw(PSX_EINVAL, "invalid ploidy %u", ploidy);
+        std::vector<psx_mutation> v(recs, recs + n);
+        uint32_t max_chr = 0;
+        for (const auto& r : v) {
+            if (r.chr >= PSX_MAX_CHROMOSOMES) psx_throw(PSX_EINVAL, "chromosome index %u out of range", r.chr);
+            if (r.allele >= ploidy) psx_throw(PSX_EINVAL, "allele %u out of range", r.allele);
+            if (r.kind > PSX_INS || r.nature > PSX_GERMINAL || r.ref > 3 || r.alt > 3 || r.len > 15)
+                psx_throw(PSX_EINVAL, "malformed mutation record");
+            if (r.leaf_lo > r.leaf_hi || r.leaf_hi > f->n_leaves)
+                psx_throw(PSX_EINVAL, "leaf range [%u, %u) outside the forest", r.leaf_lo, r.leaf_hi);
+            max_chr = std::max(max_chr, r.chr + 1);
+        }
+        m->ploidy = ploidy;
+        m->n_leaves = f->n_leaves;
+        // chromosome lengths are only known from the genome at sequencing time;
+        // index every chromosome up to the largest position seen
+        std::vector<uint32_t> chr_len(std::max(max_chr, 1u), 0);
+        for (const auto& r : v) chr_len[r.chr] = std::max(chr_len[r.chr], r.pos + 1);
+        PSX_HIP(hipSetDevice(ctx->device));
+        psx_mutations_build(ctx, m, v, chr_len);
+    });
+    if (st != PSX_OK) { delete m; return st; }
+    *out = m;
+    return PSX_OK;
+}
+
+extern "C" psx_status psx_mutations_count(const psx_mutations* m, uint64_t* n) {
+    if (!m || !n) return PSX_EINVAL;
+    *n = m->n;
+    return PSX_OK;
+}
+
+extern "C" psx_status psx_mutations_download(psx_ctx* ctx, const psx_mutations* m, psx_mutation* recs,
+                                             uint64_t n) {
+    if (!ctx || !m || (!recs && m->n)) return PSX_EINVAL;
+    return psx_guard(ctx, [&] {
+        if (n < m->n) psx_throw(PSX_EINVAL, "buffer holds %llu records, need %llu", (unsigned long long)n,
+                                (unsigned long long)m->n);
+        PSX_HIP(hipSetDevice(ctx->device));
+        if (m->n)
+            PSX_HIP(hipMemcpyAsync(recs, m->d_recs.p, m->n * sizeof(psx_mutation), hipMemcpyDeviceToHost,
+                                   ctx->stream));
+        PSX_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+extern "C" void psx_mutations_free(psx_mutations* m) {
+    if (!m) return;
+    if (m->ctx) (void)hipSetDevice(m->ctx->device);
+    delete m;
+}

@@ -1,0 +1,184 @@
+// Internal declarations shared by the host and device translation units of
+// libpsx. Not part of the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/psx.h"
+#include "psx_rng.h"
+
+// -------------------------------------------------------------- status/error
+
+struct psx_error : std::exception {
+    psx_status code;
+    std::string msg;
+    psx_error(psx_status c, std::string m) : code(c), msg(std::move(m)) {}
+    const char* what() const noexcept override { return msg.c_str(); }
+};
+
+[[noreturn]] void psx_throw(psx_status code, const char* fmt, ...);
+
+#define PSX_HIP(call)                                                                    \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            psx_throw(e_ == hipErrorOutOfMemory ? PSX_ENOMEM_DEVICE : PSX_EHIP,          \
+                      "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__,   \
+                      __LINE__);                                                         \
+    } while (0)
+
+// Runs fn, converting exceptions into a status and the context message.
+template <class F>
+psx_status psx_guard(psx_ctx* ctx, F&& fn);
+
+// -------------------------------------------------------------- device memory
+
+struct dbuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    dbuf() = default;
+    dbuf(const dbuf&) = delete;
+    dbuf& operator=(const dbuf&) = delete;
+    dbuf(dbuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+    dbuf& operator=(dbuf&& o) noexcept {
+        if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+        return *this;
+    }
+    ~dbuf() { release(); }
+    void alloc(size_t n) {
+        release();
+        if (n == 0) n = 256;
+        PSX_HIP(hipMalloc(&p, n));
+        bytes = n;
+    }
+    // Grows to at least n bytes, keeping nothing.
+    void reserve(size_t n) { if (n > bytes) alloc(n); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+// ------------------------------------------------------------------- handles
+
+struct psx_ctx {
+    int device = 0;
+    int n_cu = 256;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string err;
+    double last_kernel_ms = 0.0;
+    double last_place_ms = 0.0;
+    dbuf scratch;       // per-call tables and descriptors
+    dbuf queue;         // persistent-kernel work counters
+    bool tuned = false;
+    psx_launch_config tuned_cfg{};
+};
+
+struct psx_genome {
+    psx_ctx* ctx = nullptr;
+    uint32_t n_chr = 0;
+    std::vector<uint32_t> len;
+    std::vector<uint64_t> woff;   // first word of each chromosome
+    std::vector<uint64_t> boff;   // absolute base offset, n_chr + 1 entries
+    uint64_t words = 0;
+    dbuf packed;
+};
+
+struct psx_forest {
+    psx_ctx* ctx = nullptr;
+    uint32_t n_nodes = 0, n_leaves = 0, n_samples = 0;
+    // host copies of the flattened forest (DFS pre-order)
+    std::vector<int64_t> cell_id;
+    std::vector<uint32_t> parent, species, leaf_lo, leaf_hi;
+    std::vector<double> birth;
+    std::vector<uint32_t> sample_begin, sample_leaves;
+    // device copies
+    dbuf d_leaf_lo, d_leaf_hi, d_species, d_parent, d_sample_leaves;
+};
+
+// Mutation table in SID order: device SoA for the kernels, AoS for download.
+struct psx_mutations {
+    psx_ctx* ctx = nullptr;
+    uint64_t n = 0;
+    uint32_t ploidy = 2;
+    uint32_t n_leaves = 0;
+    std::vector<uint64_t> chr_begin;    // PSX_MAX_CHROMOSOMES + 1
+    std::vector<uint64_t> bucket_off;   // per chromosome offset into d_bucket
+    std::vector<uint32_t> chr_len;      // chromosome lengths of the genome used
+    dbuf d_recs;                        // psx_mutation[n]
+    dbuf d_pos, d_lo, d_hi, d_info;     // uint32[n]
+    dbuf d_bucket;                      // per chromosome: first record at or after k * PSX_STRATUM
+};
+
+// info word of a device mutation record
+#define PSX_INFO_ALLELE(i) ((i) & 0xFFu)
+#define PSX_INFO_REF(i) (((i) >> 8) & 3u)
+#define PSX_INFO_ALT(i) (((i) >> 10) & 3u)
+#define PSX_INFO_KIND(i) (((i) >> 12) & 3u)
+#define PSX_INFO_NATURE(i) (((i) >> 14) & 3u)
+#define PSX_INFO_HEAD(i) (((i) >> 16) & 1u)
+
+// ------------------------------------------------------------ host helpers
+
+// Integer inversion table of a pmf (psx_rng.h): T[k] = floor(cdf_k * 2^32),
+// T[n-1] = 2^32.
+void psx_cdf_table(const double* pmf, uint32_t n, uint64_t* T);
+struct psx_itable {
+    uint32_t k0 = 0;
+    std::vector<uint64_t> T;
+};
+psx_itable psx_poisson_table(double lambda);
+psx_itable psx_binomial_table(uint32_t t, double p, uint32_t cap);
+psx_itable psx_error_table(uint32_t L, double e);
+psx_itable psx_insert_table(uint32_t mean, uint32_t sd);
+
+uint64_t psx_sid_key(const psx_mutation& r);
+uint32_t psx_info_word(const psx_mutation& r);
+
+// Sorts recs into SID order, marks sid_head and builds the device table.
+void psx_mutations_build(psx_ctx* ctx, psx_mutations* m, std::vector<psx_mutation>& recs,
+                         const std::vector<uint32_t>& chr_len);
+// d_recs holds m->n records in SID order: builds the SoA arrays and the
+// per-stratum index for the chromosome lengths in m->chr_len.
+void psx_mutations_finish(psx_ctx* ctx, psx_mutations* m, const std::vector<uint64_t>& chr_begin);
+// Rebuilds the per-stratum index for other chromosome lengths.
+void psx_mutations_index(psx_ctx* ctx, psx_mutations* m, const std::vector<uint32_t>& chr_len);
+
+// Copies host bytes into the context scratch arena at a 256-byte aligned
+// offset and returns the device address.
+struct scratch_builder {
+    std::vector<uint8_t> host;
+    size_t put(const void* src, size_t bytes);
+    template <class T> size_t put_vec(const std::vector<T>& v) {
+        return put(v.data(), v.size() * sizeof(T));
+    }
+};
+void* psx_scratch_upload(psx_ctx* ctx, const scratch_builder& b);
+
+// Message of the last failed call made without a context (psx_last_error(NULL)).
+extern thread_local std::string psx_tls_err;
+
+template <class F>
+psx_status psx_guard(psx_ctx* ctx, F&& fn) {
+    std::string& err = ctx ? ctx->err : psx_tls_err;
+    try {
+        fn();
+        return PSX_OK;
+    } catch (const psx_error& e) {
+        err = e.msg;
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        err = "host allocation failed";
+        return PSX_ENOMEM_HOST;
+    } catch (const std::exception& e) {
+        err = e.what();
+        return PSX_EINVAL;
+    }
+}

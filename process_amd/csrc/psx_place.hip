@@ -1,0 +1,554 @@
+// MutationEngine::place_mutations on gfx950 (src/mutation_engine.cpp:1085-1113).
+//
+//   k_edge_counts   one thread per forest node: Poisson SNV / indel counts
+//   (scan)          exclusive sum of counts -> mutation ids per node
+//   k_place         one thread per mutation: signature, SBS class, allele and
+//                   a site whose trinucleotide matches the class (rejection on
+//                   the 2-bit reference in HBM), or an indel
+//   k_conflicts     infinite-sites model: after a footprint sort, a mutation
+//                   touching an earlier-created one is redrawn (next attempt)
+//   (sort + gather) SID order, sid_head flags, chromosome ranges
+//
+// Every draw is Philox(seed, counter) with the counter layout of the oracle,
+// so the table is identical to the CPU restatement's.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+
+#include "psx_internal.h"
+
+namespace {
+
+constexpr uint32_t SITE_TRIES = 1u << 20;
+constexpr uint32_t INDEL_TRIES = 1u << 20;
+constexpr int IS_ROUNDS = 64;
+
+enum : uint32_t { ST_OK = 0, ST_NO_SITE = 1, ST_NO_INDEL_ROOM = 2 };
+
+struct gdev {
+    const uint32_t* packed;
+    const uint32_t* len;
+    const uint64_t* woff;
+    const uint64_t* boff;  // n_chr + 1
+    uint64_t G;
+    uint32_t n_chr;
+};
+
+__device__ __forceinline__ uint32_t gbase(const gdev& g, uint32_t c, uint32_t i) {
+    return (g.packed[g.woff[c] + (i >> 4)] >> (2u * (i & 15u))) & 3u;
+}
+
+__device__ __forceinline__ void glocate(const gdev& g, uint64_t x, uint32_t& c, uint32_t& pos) {
+    uint32_t k = 0;
+    while (x >= g.boff[k + 1]) ++k;
+    c = k;
+    pos = (uint32_t)(x - g.boff[k]);
+}
+
+struct place_args {
+    gdev g;
+    uint64_t seed;
+    uint32_t ploidy;
+    uint32_t n_sbs;
+    const uint64_t* expo_T;    // [n_epochs][n_sbs]
+    const uint64_t* cls_T;     // [n_sbs][96]
+    const uint64_t* len_T;
+    uint32_t len_n;
+    const uint32_t* node_epoch;
+    const uint32_t* leaf_lo;
+    const uint32_t* leaf_hi;
+    uint32_t n_leaves;
+    uint32_t n_nodes;
+    // slot layout by id: drivers | germline | preneoplastic SNVs | preneoplastic indels | passengers
+    uint64_t n_drv, n_germ, n_pre_snv, n_pre_indel;
+    const uint32_t* drv_d;
+    const uint32_t* drv_v;
+    const psx_driver* drivers;
+    const psx_germline* germ;
+    uint32_t pre_sig;
+    const uint32_t* cnt_snv;
+    const uint64_t* node_off;
+    uint64_t M;
+    psx_mutation* recs;
+    uint32_t* attempt;
+    const uint8_t* redraw;  // nullptr = every id
+    uint32_t* status;
+};
+
+__device__ const uint32_t SUB_TARGET[6] = {0, 2, 3, 0, 1, 2};  // C>A C>G C>T T>A T>C T>G
+
+__device__ uint32_t draw_snv(const place_args& a, uint32_t v, uint32_t k, uint32_t att, int fixed_sig,
+                             psx_mutation& r) {
+    const psx_u4 w = psx_draw(a.seed, k, v, att, PSX_P_SNV << 24);
+    const uint32_t sig = fixed_sig >= 0 ? (uint32_t)fixed_sig
+                                        : psx_cdf_pick(a.expo_T + (size_t)a.node_epoch[v] * a.n_sbs, a.n_sbs, w.x);
+    const uint32_t cls = psx_cdf_pick(a.cls_T + (size_t)sig * 96u, 96u, w.y);
+    const uint32_t allele = psx_uniform32(w.z, a.ploidy);
+    const uint32_t sub = cls >> 4, five = (cls >> 2) & 3u, three = cls & 3u;
+    const uint32_t mid = sub < 3 ? 1u : 3u, tgt = SUB_TARGET[sub];
+    psx_u4 ws{0, 0, 0, 0};
+    for (uint32_t t = 0; t < SITE_TRIES; ++t) {
+        if ((t & 1u) == 0) ws = psx_draw(a.seed, k, v, att, (PSX_P_SITE << 24) | (t >> 1));
+        const uint32_t hi = (t & 1u) ? ws.z : ws.x, lo = (t & 1u) ? ws.w : ws.y;
+        uint32_t c, pos;
+        glocate(a.g, psx_uniform64(hi, lo, a.g.G), c, pos);
+        if (pos < 1 || pos + 1 >= a.g.len[c]) continue;
+        const uint32_t m = gbase(a.g, c, pos);
+        const uint32_t b5 = gbase(a.g, c, pos - 1), b3 = gbase(a.g, c, pos + 1);
+        uint32_t alt;
+        if (m == 1u || m == 3u) {
+            if (!(b5 == five && m == mid && b3 == three)) continue;
+            alt = tgt;
+        } else {
+            if (!(3u - b3 == five && 3u - m == mid && 3u - b5 == three)) continue;
+            alt = 3u - tgt;
+        }
+        r.chr = c; r.pos = pos; r.kind = PSX_SNV; r.ref = (uint8_t)m; r.alt = (uint8_t)alt;
+        r.len = 0; r.seq = 0; r.allele = (uint8_t)allele; r.cause = (uint16_t)sig;
+        return ST_OK;
+    }
+    return ST_NO_SITE;
+}
+
+__device__ uint32_t draw_indel(const place_args& a, uint32_t v, uint32_t k, uint32_t att, psx_mutation& r) {
+    for (uint32_t t = 0; t < INDEL_TRIES; ++t) {
+        const psx_u4 w = psx_draw(a.seed, k, v, att, (PSX_P_INDEL << 24) | t);
+        const uint32_t kind = (w.x >> 31) ? PSX_INS : PSX_DEL;
+        const uint32_t allele = psx_uniform32(w.x << 1, a.ploidy);
+        const uint32_t l = 1u + psx_cdf_pick(a.len_T, a.len_n, w.y);
+        uint32_t c, pos;
+        glocate(a.g, psx_uniform64(w.z, w.w, a.g.G), c, pos);
+        if ((uint64_t)pos + l >= a.g.len[c]) continue;
+        uint32_t seq = 0;
+        if (kind == PSX_DEL) {
+            for (uint32_t i = 0; i < l; ++i) seq |= gbase(a.g, c, pos + 1 + i) << (2u * i);
+        } else {
+            const psx_u4 s = psx_draw(a.seed, k, v, att, (PSX_P_INDEL << 24) | 0x400000u | t);
+            seq = s.x & ((1u << (2u * l)) - 1u);
+        }
+        r.chr = c; r.pos = pos; r.kind = (uint8_t)kind; r.ref = (uint8_t)gbase(a.g, c, pos);
+        r.alt = 0; r.len = (uint8_t)l; r.seq = seq; r.allele = (uint8_t)allele; r.cause = PSX_NO_CAUSE;
+        return ST_OK;
+    }
+    return ST_NO_INDEL_ROOM;
+}
+
+__device__ uint32_t place_one(const place_args& a, uint64_t id, uint32_t att, psx_mutation& r) {
+    r = psx_mutation{};
+    uint64_t i = id;
+    if (i < a.n_drv) {
+        const uint32_t d = a.drv_d[i], v = a.drv_v[i];
+        const psx_driver dr = a.drivers[d];
+        uint32_t allele;
+        if (dr.allele >= 0) allele = (uint32_t)dr.allele;
+        else allele = psx_uniform32(psx_draw(a.seed, d, v, 0u, PSX_P_DRIVER << 24).x, a.ploidy);
+        r.chr = dr.chr; r.pos = dr.pos; r.kind = PSX_SNV; r.ref = (uint8_t)gbase(a.g, dr.chr, dr.pos);
+        r.alt = dr.alt; r.allele = (uint8_t)allele; r.cause = PSX_NO_CAUSE; r.nature = PSX_DRIVER;
+        r.node = v; r.leaf_lo = a.leaf_lo[v]; r.leaf_hi = a.leaf_hi[v];
+        return ST_OK;
+    }
+    i -= a.n_drv;
+    if (i < a.n_germ) {
+        const psx_germline gm = a.germ[i];
+        r.chr = gm.chr; r.pos = gm.pos; r.kind = PSX_SNV; r.ref = (uint8_t)gbase(a.g, gm.chr, gm.pos);
+        r.alt = gm.alt; r.allele = gm.allele; r.cause = PSX_NO_CAUSE; r.nature = PSX_GERMINAL;
+        r.node = PSX_NO_NODE; r.leaf_lo = 0; r.leaf_hi = a.n_leaves;
+        return ST_OK;
+    }
+    i -= a.n_germ;
+    uint32_t st;
+    if (i < a.n_pre_snv) {
+        st = draw_snv(a, PSX_NO_NODE, (uint32_t)i, att, (int)a.pre_sig, r);
+        r.nature = PSX_PRENEOPLASTIC; r.node = PSX_NO_NODE; r.leaf_lo = 0; r.leaf_hi = a.n_leaves;
+        return st;
+    }
+    i -= a.n_pre_snv;
+    if (i < a.n_pre_indel) {
+        st = draw_indel(a, PSX_NO_NODE, (uint32_t)i, att, r);
+        r.nature = PSX_PRENEOPLASTIC; r.node = PSX_NO_NODE; r.leaf_lo = 0; r.leaf_hi = a.n_leaves;
+        return st;
+    }
+    i -= a.n_pre_indel;
+    // node of passenger i: last v with node_off[v] <= i
+    uint32_t lo = 0, hi = a.n_nodes;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (a.node_off[mid] <= i) lo = mid + 1; else hi = mid;
+    }
+    const uint32_t v = lo - 1;
+    const uint32_t q = (uint32_t)(i - a.node_off[v]);
+    if (q < a.cnt_snv[v]) st = draw_snv(a, v, q, att, -1, r);
+    else st = draw_indel(a, v, q - a.cnt_snv[v], att, r);
+    r.nature = PSX_PASSENGER; r.node = v; r.leaf_lo = a.leaf_lo[v]; r.leaf_hi = a.leaf_hi[v];
+    return st;
+}
+
+__global__ void __launch_bounds__(256) k_place(const place_args a) {
+    const uint64_t id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= a.M) return;
+    if (a.redraw && !a.redraw[id]) return;
+    uint32_t att = 0;
+    if (a.redraw) {
+        att = a.attempt[id] + 1;
+        a.attempt[id] = att;
+    }
+    psx_mutation r;
+    const uint32_t st = place_one(a, id, att, r);
+    if (st != ST_OK) atomicOr(a.status, st);
+    a.recs[id] = r;
+}
+
+struct edge_args {
+    uint64_t seed;
+    uint32_t n_nodes;
+    const uint32_t* parent;
+    const uint32_t* species;
+    const uint32_t* snv_k0;
+    const uint32_t* snv_off;
+    const uint32_t* snv_n;
+    const uint32_t* ind_k0;
+    const uint32_t* ind_off;
+    const uint32_t* ind_n;
+    const uint64_t* T;
+    uint32_t* cnt_snv;
+    uint64_t* cnt_tot;
+};
+
+// Passenger counts of the edge into node v (none for roots).
+__global__ void __launch_bounds__(256) k_edge_counts(const edge_args a) {
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= a.n_nodes) return;
+    if (a.parent[v] == PSX_NO_NODE) {
+        a.cnt_snv[v] = 0;
+        a.cnt_tot[v] = 0;
+        return;
+    }
+    const psx_u4 w = psx_draw(a.seed, v, 0u, 0u, PSX_P_EDGE << 24);
+    const uint32_t s = a.species[v];
+    const uint32_t ns = a.snv_k0[s] + psx_cdf_pick(a.T + a.snv_off[s], a.snv_n[s], w.x);
+    const uint32_t ni = a.ind_k0[s] + psx_cdf_pick(a.T + a.ind_off[s], a.ind_n[s], w.y);
+    a.cnt_snv[v] = ns;
+    a.cnt_tot[v] = (uint64_t)ns + ni;
+}
+
+__global__ void __launch_bounds__(256) k_footprint(const psx_mutation* __restrict__ recs, uint64_t M,
+                                                   uint64_t* __restrict__ fkey, uint32_t* __restrict__ ids) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M) return;
+    fkey[i] = ((uint64_t)recs[i].chr << 32) | recs[i].pos;
+    ids[i] = (uint32_t)i;
+}
+
+__device__ __forceinline__ uint32_t fp_end(const psx_mutation& r) {
+    return r.pos + (r.kind == PSX_DEL ? (uint32_t)r.len : 0u);
+}
+
+// Footprints [pos, pos + deleted length] touching (gap < 2) an earlier-created
+// mutation's footprint lose; drivers and germline mutations never lose.
+__global__ void __launch_bounds__(256) k_conflicts(const psx_mutation* __restrict__ recs,
+                                                   const uint64_t* __restrict__ fkey,
+                                                   const uint32_t* __restrict__ ids, uint64_t M,
+                                                   uint64_t n_fixed, uint8_t* __restrict__ loser,
+                                                   uint32_t* __restrict__ n_lose) {
+    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= M) return;
+    const uint32_t idj = ids[j];
+    loser[idj] = 0;
+    if (idj < n_fixed) return;
+    const uint64_t kj = fkey[j];
+    const uint32_t chr = (uint32_t)(kj >> 32), sj = (uint32_t)kj, ej = fp_end(recs[idj]);
+    bool lose = false;
+    for (uint64_t i = j; i-- > 0 && !lose;) {
+        const uint64_t ki = fkey[i];
+        const uint32_t si = (uint32_t)ki;
+        if ((uint32_t)(ki >> 32) != chr || si + 12u < sj) break;
+        const uint32_t idi = ids[i];
+        if (idi < idj && (uint64_t)sj <= (uint64_t)fp_end(recs[idi]) + 1u && (uint64_t)si <= (uint64_t)ej + 1u)
+            lose = true;
+    }
+    for (uint64_t i = j + 1; i < M && !lose; ++i) {
+        const uint64_t ki = fkey[i];
+        const uint32_t si = (uint32_t)ki;
+        if ((uint32_t)(ki >> 32) != chr || (uint64_t)si > (uint64_t)ej + 1u) break;
+        const uint32_t idi = ids[i];
+        if (idi < idj && (uint64_t)sj <= (uint64_t)fp_end(recs[idi]) + 1u) lose = true;
+    }
+    if (lose) {
+        loser[idj] = 1;
+        atomicAdd(n_lose, 1u);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_sid_key(const psx_mutation* __restrict__ recs, uint64_t M,
+                                                 uint64_t* __restrict__ key, uint32_t* __restrict__ ids) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M) return;
+    const psx_mutation r = recs[i];
+    const uint64_t payload = r.kind == PSX_SNV ? r.alt : (r.kind == PSX_INS ? r.seq : 0u);
+    key[i] = ((uint64_t)r.chr << 58) | ((uint64_t)r.pos << 26) | ((uint64_t)r.kind << 24) |
+             ((uint64_t)(r.len & 15u) << 20) | (payload & 0xFFFFFu);
+    ids[i] = (uint32_t)i;
+}
+
+__global__ void __launch_bounds__(256) k_gather(const psx_mutation* __restrict__ src,
+                                                const uint64_t* __restrict__ key,
+                                                const uint32_t* __restrict__ ids, uint64_t M,
+                                                psx_mutation* __restrict__ dst) {
+    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= M) return;
+    psx_mutation r = src[ids[j]];
+    r.sid_head = (j == 0 || key[j] != key[j - 1]) ? 1 : 0;
+    dst[j] = r;
+}
+
+__global__ void k_chr_begin(const uint64_t* __restrict__ key, uint64_t M, uint64_t* __restrict__ out) {
+    const uint32_t c = threadIdx.x;
+    if (c > PSX_MAX_CHROMOSOMES) return;
+    const uint64_t target = c >= 64 ? ~0ull : ((uint64_t)c << 58);
+    uint64_t lo = 0, hi = M;
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (key[mid] < target) lo = mid + 1; else hi = mid;
+    }
+    out[c] = c >= 64 ? M : lo;
+}
+
+inline uint32_t grid_of(uint64_t n) { return (uint32_t)((n + 255) / 256); }
+
+}  // namespace
+
+extern "C" psx_status psx_place_mutations(psx_ctx* ctx, const psx_genome* g, const psx_forest* f,
+                                          const psx_engine_params* p, uint64_t seed, psx_mutations** out) {
+    if (!ctx || !g || !f || !p || !out) return PSX_EINVAL;
+    psx_mutations* mt = new psx_mutations();
+    mt->ctx = ctx;
+    psx_status st = psx_guard(ctx, [&] {
+        PSX_HIP(hipSetDevice(ctx->device));
+        hipStream_t s = ctx->stream;
+        if (p->ploidy < 1 || p->ploidy > 255) psx_throw(PSX_EINVAL, "invalid ploidy");
+        if (p->indel_max_len < 1 || p->indel_max_len > 10) psx_throw(PSX_EINVAL, "indel_max_len must be in [1, 10]");
+        if (!(p->indel_len_p > 0.0 && p->indel_len_p <= 1.0)) psx_throw(PSX_EINVAL, "invalid indel length parameter");
+        for (uint32_t v = 0; v < f->n_nodes; ++v)
+            if (f->species[v] >= p->n_species) psx_throw(PSX_EINVAL, "node species %u has no rates", f->species[v]);
+        bool need_snv = p->n_preneoplastic_snv > 0;
+        for (uint32_t sp = 0; sp < p->n_species; ++sp) need_snv |= p->snv_rate[sp] > 0.0;
+        if (need_snv && (p->n_sbs == 0 || p->n_epochs == 0))
+            psx_throw(PSX_EINVAL, "SNVs need a signature and an exposure");
+        if (p->n_preneoplastic_snv > 0 && p->preneoplastic_sbs >= p->n_sbs)
+            psx_throw(PSX_EINVAL, "unknown preneoplastic signature");
+        const uint64_t G = g->boff[g->n_chr];
+        // ---- integer tables (host, from the double pmfs) -> one u64 array
+        std::vector<uint64_t> T;
+        auto append = [&](const uint64_t* src, size_t n) { size_t off = T.size(); T.insert(T.end(), src, src + n); return off; };
+        std::vector<uint64_t> tmp;
+        const size_t expo_off = T.size();
+        for (uint32_t e = 0; e < p->n_epochs; ++e) {
+            tmp.assign(p->n_sbs, 0);
+            psx_cdf_table(p->exposure + (size_t)e * p->n_sbs, p->n_sbs, tmp.data());
+            append(tmp.data(), tmp.size());
+        }
+        const size_t cls_off = T.size();
+        for (uint32_t sg = 0; sg < p->n_sbs; ++sg) {
+            tmp.assign(96, 0);
+            psx_cdf_table(p->sbs + (size_t)sg * 96, 96, tmp.data());
+            append(tmp.data(), tmp.size());
+        }
+        std::vector<double> lp(p->indel_max_len);
+        for (uint32_t l = 1; l <= p->indel_max_len; ++l)
+            lp[l - 1] = std::pow(1.0 - p->indel_len_p, (double)(l - 1)) * p->indel_len_p;
+        tmp.assign(p->indel_max_len, 0);
+        psx_cdf_table(lp.data(), p->indel_max_len, tmp.data());
+        const size_t len_off = append(tmp.data(), tmp.size());
+        const double allelic = (double)((uint64_t)p->ploidy * G);
+        std::vector<uint32_t> sk0(p->n_species), soff(p->n_species), sn(p->n_species);
+        std::vector<uint32_t> ik0(p->n_species), ioff(p->n_species), in_(p->n_species);
+        for (uint32_t sp = 0; sp < p->n_species; ++sp) {
+            psx_itable a = psx_poisson_table(p->snv_rate[sp] * allelic);
+            sk0[sp] = a.k0; sn[sp] = (uint32_t)a.T.size(); soff[sp] = (uint32_t)append(a.T.data(), a.T.size());
+            psx_itable b = psx_poisson_table(p->indel_rate[sp] * allelic);
+            ik0[sp] = b.k0; in_[sp] = (uint32_t)b.T.size(); ioff[sp] = (uint32_t)append(b.T.data(), b.T.size());
+        }
+        // ---- drivers, germline, node epochs (host)
+        auto base_at = [&](uint32_t c, uint32_t i) -> uint32_t {
+            uint32_t w = 0;
+            PSX_HIP(hipMemcpy(&w, g->packed.as<uint32_t>() + g->woff[c] + i / 16, 4, hipMemcpyDeviceToHost));
+            return (w >> (2 * (i % 16))) & 3u;
+        };
+        std::vector<uint32_t> drv_d, drv_v;
+        for (uint32_t d = 0; d < p->n_drivers; ++d) {
+            const psx_driver& dr = p->drivers[d];
+            if (dr.chr >= g->n_chr || dr.pos >= g->len[dr.chr] || dr.alt > 3 ||
+                (dr.allele >= 0 && (uint32_t)dr.allele >= p->ploidy) || dr.alt == base_at(dr.chr, dr.pos))
+                psx_throw(PSX_EINVAL, "invalid driver %u", d);
+            for (uint32_t v = 0; v < f->n_nodes; ++v) {
+                const uint32_t par = f->parent[v];
+                if (f->species[v] == dr.species && (par == PSX_NO_NODE || f->species[par] != dr.species)) {
+                    drv_d.push_back(d);
+                    drv_v.push_back(v);
+                }
+            }
+        }
+        for (uint32_t k = 0; k < p->n_germline; ++k) {
+            const psx_germline& gm = p->germline[k];
+            if (gm.chr >= g->n_chr || gm.pos >= g->len[gm.chr] || gm.alt > 3 || gm.allele >= p->ploidy ||
+                gm.alt == base_at(gm.chr, gm.pos))
+                psx_throw(PSX_EINVAL, "invalid germline mutation %u", k);
+        }
+        std::vector<uint32_t> epoch(f->n_nodes, 0);
+        for (uint32_t v = 0; v < f->n_nodes; ++v)
+            for (uint32_t e = 0; e < p->n_epochs; ++e)
+                if (p->epoch_start[e] <= f->birth[v]) epoch[v] = e;
+        // ---- descriptors into the scratch arena
+        scratch_builder sb;
+        const size_t o_T = sb.put_vec(T);
+        const size_t o_len = sb.put_vec(g->len);
+        const size_t o_woff = sb.put_vec(g->woff);
+        const size_t o_boff = sb.put_vec(g->boff);
+        const size_t o_sk0 = sb.put_vec(sk0), o_soff = sb.put_vec(soff), o_sn = sb.put_vec(sn);
+        const size_t o_ik0 = sb.put_vec(ik0), o_ioff = sb.put_vec(ioff), o_in = sb.put_vec(in_);
+        const size_t o_epoch = sb.put_vec(epoch);
+        const size_t o_drvd = sb.put_vec(drv_d), o_drvv = sb.put_vec(drv_v);
+        const size_t o_drivers = sb.put(p->drivers, sizeof(psx_driver) * p->n_drivers);
+        const size_t o_germ = sb.put(p->germline, sizeof(psx_germline) * p->n_germline);
+        const uint32_t zero = 0;
+        const size_t o_status = sb.put(&zero, 4);
+        const size_t o_nlose = sb.put(&zero, 4);
+        uint8_t* base = (uint8_t*)psx_scratch_upload(ctx, sb);
+        auto at = [&](size_t off) { return (void*)(base + off); };
+
+        PSX_HIP(hipEventRecord(ctx->ev0, s));
+        const uint32_t n = f->n_nodes;
+        dbuf d_cnt_snv, d_cnt_tot, d_off, d_tmp;
+        d_cnt_snv.alloc((size_t)n * 4);
+        d_cnt_tot.alloc((size_t)n * 8);
+        d_off.alloc((size_t)n * 8);
+        edge_args ea{};
+        ea.seed = seed; ea.n_nodes = n; ea.parent = f->d_parent.as<uint32_t>(); ea.species = f->d_species.as<uint32_t>();
+        ea.snv_k0 = (uint32_t*)at(o_sk0); ea.snv_off = (uint32_t*)at(o_soff); ea.snv_n = (uint32_t*)at(o_sn);
+        ea.ind_k0 = (uint32_t*)at(o_ik0); ea.ind_off = (uint32_t*)at(o_ioff); ea.ind_n = (uint32_t*)at(o_in);
+        ea.T = (uint64_t*)at(o_T); ea.cnt_snv = d_cnt_snv.as<uint32_t>(); ea.cnt_tot = d_cnt_tot.as<uint64_t>();
+        hipLaunchKernelGGL(k_edge_counts, dim3(grid_of(n)), dim3(256), 0, s, ea);
+        PSX_HIP(hipGetLastError());
+        size_t tmp_bytes = 0;
+        PSX_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, d_cnt_tot.as<uint64_t>(), d_off.as<uint64_t>(),
+                                                 (int)n, s));
+        d_tmp.alloc(tmp_bytes);
+        PSX_HIP(hipcub::DeviceScan::ExclusiveSum(d_tmp.p, tmp_bytes, d_cnt_tot.as<uint64_t>(), d_off.as<uint64_t>(),
+                                                 (int)n, s));
+        uint64_t last_off = 0, last_cnt = 0;
+        PSX_HIP(hipMemcpyAsync(&last_off, d_off.as<uint64_t>() + (n - 1), 8, hipMemcpyDeviceToHost, s));
+        PSX_HIP(hipMemcpyAsync(&last_cnt, d_cnt_tot.as<uint64_t>() + (n - 1), 8, hipMemcpyDeviceToHost, s));
+        PSX_HIP(hipStreamSynchronize(s));
+        const uint64_t n_pass = last_off + last_cnt;
+        const uint64_t n_fixed = drv_d.size() + p->n_germline;
+        const uint64_t M = n_fixed + p->n_preneoplastic_snv + p->n_preneoplastic_indel + n_pass;
+        if (M >= 0xFFFFFFFFull) psx_throw(PSX_EINVAL, "too many mutations (%llu)", (unsigned long long)M);
+
+        dbuf d_recs, d_attempt, d_redraw, d_key, d_key2, d_ids, d_ids2;
+        d_recs.alloc(M * sizeof(psx_mutation));
+        d_attempt.alloc(M * 4);
+        d_redraw.alloc(M);
+        d_key.alloc(M * 8); d_key2.alloc(M * 8);
+        d_ids.alloc(M * 4); d_ids2.alloc(M * 4);
+        PSX_HIP(hipMemsetAsync(d_attempt.p, 0, M * 4, s));
+        place_args pa{};
+        pa.g = gdev{g->packed.as<uint32_t>(), (uint32_t*)at(o_len), (uint64_t*)at(o_woff), (uint64_t*)at(o_boff), G,
+                    g->n_chr};
+        pa.seed = seed; pa.ploidy = p->ploidy; pa.n_sbs = p->n_sbs;
+        pa.expo_T = (uint64_t*)at(o_T) + expo_off; pa.cls_T = (uint64_t*)at(o_T) + cls_off;
+        pa.len_T = (uint64_t*)at(o_T) + len_off; pa.len_n = p->indel_max_len;
+        pa.node_epoch = (uint32_t*)at(o_epoch);
+        pa.leaf_lo = f->d_leaf_lo.as<uint32_t>(); pa.leaf_hi = f->d_leaf_hi.as<uint32_t>();
+        pa.n_leaves = f->n_leaves; pa.n_nodes = n;
+        pa.n_drv = drv_d.size(); pa.n_germ = p->n_germline;
+        pa.n_pre_snv = p->n_preneoplastic_snv; pa.n_pre_indel = p->n_preneoplastic_indel;
+        pa.drv_d = (uint32_t*)at(o_drvd); pa.drv_v = (uint32_t*)at(o_drvv);
+        pa.drivers = (psx_driver*)at(o_drivers); pa.germ = (psx_germline*)at(o_germ);
+        pa.pre_sig = p->preneoplastic_sbs;
+        pa.cnt_snv = d_cnt_snv.as<uint32_t>(); pa.node_off = d_off.as<uint64_t>();
+        pa.M = M; pa.recs = d_recs.as<psx_mutation>(); pa.attempt = d_attempt.as<uint32_t>();
+        pa.redraw = nullptr; pa.status = (uint32_t*)at(o_status);
+        if (M) {
+            hipLaunchKernelGGL(k_place, dim3(grid_of(M)), dim3(256), 0, s, pa);
+            PSX_HIP(hipGetLastError());
+        }
+        auto check_status = [&] {
+            uint32_t stv = 0;
+            PSX_HIP(hipMemcpyAsync(&stv, pa.status, 4, hipMemcpyDeviceToHost, s));
+            PSX_HIP(hipStreamSynchronize(s));
+            if (stv & ST_NO_SITE) psx_throw(PSX_EINVAL, "no reference site matches a drawn SBS class");
+            if (stv & ST_NO_INDEL_ROOM) psx_throw(PSX_EINVAL, "no room for an indel");
+        };
+        check_status();
+        size_t sort_bytes = 0, sort_bytes38 = 0;
+        PSX_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, d_key.as<uint64_t>(), d_key2.as<uint64_t>(),
+                                                   d_ids.as<uint32_t>(), d_ids2.as<uint32_t>(), (int)M, 0, 64, s));
+        PSX_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes38, d_key.as<uint64_t>(), d_key2.as<uint64_t>(),
+                                                   d_ids.as<uint32_t>(), d_ids2.as<uint32_t>(), (int)M, 0, 38, s));
+        sort_bytes = std::max(sort_bytes, sort_bytes38);
+        dbuf d_sort;
+        d_sort.alloc(sort_bytes);
+        if (p->infinite_sites && M > 1) {
+            int round = 0;
+            for (; round < IS_ROUNDS; ++round) {
+                hipLaunchKernelGGL(k_footprint, dim3(grid_of(M)), dim3(256), 0, s, pa.recs, M, d_key.as<uint64_t>(),
+                                   d_ids.as<uint32_t>());
+                PSX_HIP(hipGetLastError());
+                PSX_HIP(hipcub::DeviceRadixSort::SortPairs(d_sort.p, sort_bytes, d_key.as<uint64_t>(),
+                                                           d_key2.as<uint64_t>(), d_ids.as<uint32_t>(),
+                                                           d_ids2.as<uint32_t>(), (int)M, 0, 38, s));
+                PSX_HIP(hipMemsetAsync(at(o_nlose), 0, 4, s));
+                hipLaunchKernelGGL(k_conflicts, dim3(grid_of(M)), dim3(256), 0, s, pa.recs,
+                                   (const uint64_t*)d_key2.p, (const uint32_t*)d_ids2.p, M, n_fixed,
+                                   d_redraw.as<uint8_t>(), (uint32_t*)at(o_nlose));
+                PSX_HIP(hipGetLastError());
+                uint32_t n_lose = 0;
+                PSX_HIP(hipMemcpyAsync(&n_lose, at(o_nlose), 4, hipMemcpyDeviceToHost, s));
+                PSX_HIP(hipStreamSynchronize(s));
+                if (n_lose == 0) break;
+                place_args pr = pa;
+                pr.redraw = d_redraw.as<uint8_t>();
+                hipLaunchKernelGGL(k_place, dim3(grid_of(M)), dim3(256), 0, s, pr);
+                PSX_HIP(hipGetLastError());
+                check_status();
+            }
+            if (round == IS_ROUNDS) psx_throw(PSX_EINVAL, "infinite-sites placement did not converge");
+        }
+        // ---- SID order
+        mt->n = M;
+        mt->ploidy = p->ploidy;
+        mt->n_leaves = f->n_leaves;
+        mt->chr_len = g->len;
+        mt->d_recs.alloc(M * sizeof(psx_mutation));
+        std::vector<uint64_t> chr_begin(PSX_MAX_CHROMOSOMES + 1, M);
+        if (M) {
+            hipLaunchKernelGGL(k_sid_key, dim3(grid_of(M)), dim3(256), 0, s, pa.recs, M, d_key.as<uint64_t>(),
+                               d_ids.as<uint32_t>());
+            PSX_HIP(hipGetLastError());
+            PSX_HIP(hipcub::DeviceRadixSort::SortPairs(d_sort.p, sort_bytes, d_key.as<uint64_t>(),
+                                                       d_key2.as<uint64_t>(), d_ids.as<uint32_t>(),
+                                                       d_ids2.as<uint32_t>(), (int)M, 0, 64, s));
+            hipLaunchKernelGGL(k_gather, dim3(grid_of(M)), dim3(256), 0, s, pa.recs, (const uint64_t*)d_key2.p,
+                               (const uint32_t*)d_ids2.p, M, mt->d_recs.as<psx_mutation>());
+            PSX_HIP(hipGetLastError());
+            dbuf d_cb;
+            d_cb.alloc(8 * (PSX_MAX_CHROMOSOMES + 1));
+            hipLaunchKernelGGL(k_chr_begin, dim3(1), dim3(128), 0, s, (const uint64_t*)d_key2.p, M,
+                               d_cb.as<uint64_t>());
+            PSX_HIP(hipGetLastError());
+            PSX_HIP(hipMemcpyAsync(chr_begin.data(), d_cb.p, 8 * (PSX_MAX_CHROMOSOMES + 1), hipMemcpyDeviceToHost, s));
+            PSX_HIP(hipStreamSynchronize(s));
+        }
+        PSX_HIP(hipEventRecord(ctx->ev1, s));
+        PSX_HIP(hipStreamSynchronize(s));
+        psx_mutations_finish(ctx, mt, chr_begin);
+        PSX_HIP(hipStreamSynchronize(s));
+        float ms = 0.f;
+        PSX_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        ctx->last_place_ms = ms;
+    });
+    if (st != PSX_OK) { delete mt; return st; }
+    *out = mt;
+    return PSX_OK;
+}

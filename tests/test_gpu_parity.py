@@ -1,0 +1,219 @@
+"""Bit-exact parity of the gfx950 path with the CPU oracle (same seeds, same
+inputs): reference synthesis, forest flattening, mutation placement and
+count-only sequencing, including the LDS-overflow path, replayed tables,
+batched launches and read materialisation."""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+
+import cases
+from oracle import oracle as O
+from process_amd import _abi as A
+from process_amd import core
+
+pytestmark = pytest.mark.gpu
+
+LENS = np.array([40_000, 23_457], np.uint32)
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+class World:
+    pass
+
+
+def build_world(ctx, lens, genome_seed, forest_kw, engine_kw, place_seed):
+    w = World()
+    w.lens = np.asarray(lens, np.uint32)
+    w.packed = O.genome_synth(w.lens, genome_seed)
+    w.g = core.Genome(ctx, w.lens, seed=genome_seed)
+    fd = cases.forest(**forest_kw)
+    w.fd = fd
+    w.of = O.Forest(fd["cell_id"], fd["parent"], fd["species"], fd["sample"], fd["birth"], fd["n_samples"])
+    w.pf = core.Forest(ctx, fd["cell_id"], fd["parent"], fd["species"], fd["sample"], fd["birth"], fd["n_samples"])
+    w.eng = cases.engine(w.packed, w.lens, **engine_kw)
+    w.om = O.place_mutations(w.lens, w.packed, w.of, place_seed, **w.eng)
+    w.pm = core.Mutations.place(ctx, w.g, w.pf, place_seed, **w.eng)
+    return w
+
+
+@pytest.fixture(scope="module")
+def world(ctx):
+    say("[parity] building world: genome 40000+23457, forest 400 cells, place seed 12345")
+    return build_world(ctx, LENS, 77, dict(n_cells=400, n_samples=2, sample_size=80, seed=3), {}, 12345)
+
+
+def test_genome_bit_exact(world):
+    assert np.array_equal(world.g.download(), world.packed)
+
+
+def test_forest_flat_matches(world):
+    fl = world.pf.flat()
+    of = world.of
+    assert np.array_equal(fl["cell_id"], of.cell_id)
+    assert np.array_equal(fl["parent"], of.parent)
+    assert np.array_equal(fl["leaf_lo"], of.leaf_lo)
+    assert np.array_equal(fl["leaf_hi"], of.leaf_hi)
+    assert np.array_equal(fl["sample_begin"], of.sample_begin)
+    assert np.array_equal(fl["sample_leaves"], of.sample_leaves)
+
+
+def test_place_mutations_bit_exact(world):
+    gm = world.pm.download()
+    om = world.om
+    say(f"[parity] placed {len(om)} mutations")
+    assert len(om) > 50
+    assert gm.tobytes() == om.tobytes()
+
+
+@pytest.mark.parametrize("case", list(cases.SEQ_CASES))
+def test_simulate_seq_bit_exact(ctx, world, case):
+    kw = cases.SEQ_CASES[case]
+    say(f"[parity] simulate_seq case {case}")
+    a, d, p, n = core.simulate_seq_arrays(ctx, world.g, world.pf, world.pm, 99, **kw)
+    oa, od, op, on = O.simulate_seq(world.lens, world.of, world.om, 99, **kw)
+    assert n == on
+    assert np.array_equal(d, od)
+    assert np.array_equal(a, oa)
+    assert np.array_equal(p, op)
+    assert d.sum() > 0
+
+
+def test_replayed_table(ctx, world):
+    """A table exported from the oracle and uploaded (replay mode)."""
+    say("[parity] replay of an uploaded table")
+    pm = core.Mutations.upload(ctx, world.pf, world.om)
+    assert pm.download().tobytes() == world.om.tobytes()
+    kw = cases.SEQ_CASES["pe_errors"]
+    a, d, p, n = core.simulate_seq_arrays(ctx, world.g, world.pf, pm, 5, **kw)
+    oa, od, op, on = O.simulate_seq(world.lens, world.of, world.om, 5, **kw)
+    assert np.array_equal(a, oa) and np.array_equal(d, od) and np.array_equal(p, op)
+
+
+def test_dense_windows_use_global_path(ctx):
+    """Mutation windows larger than the LDS capacity (1024 records)."""
+    say("[parity] dense table: genome 20000, high rates")
+    w = build_world(ctx, [20_000], 5, dict(n_cells=400, n_samples=1, sample_size=150, seed=9),
+                    dict(snv=2e-4, indel=0.0, pre_indel=0, germline=False, drivers=False, infinite_sites=False), 77)
+    assert len(w.om) > 3000
+    assert w.pm.download().tobytes() == w.om.tobytes()
+    kw = dict(coverage=20, read_size=150, insert_size_mean=350, insert_size_stddev=10, error_rate=1e-2)
+    a, d, p, n = core.simulate_seq_arrays(ctx, w.g, w.pf, w.pm, 3, **kw)
+    oa, od, op, on = O.simulate_seq(w.lens, w.of, w.om, 3, **kw)
+    assert np.array_equal(d, od) and np.array_equal(a, oa) and np.array_equal(p, op)
+
+
+def test_finite_sites_collisions(ctx):
+    """infinite_sites_model = FALSE: shared positions and repeated SIDs."""
+    say("[parity] finite sites: genome 4000, high rates")
+    w = build_world(ctx, [4_000], 6, dict(n_cells=300, n_samples=2, sample_size=60, seed=4),
+                    dict(snv=2e-4, indel=2e-5, infinite_sites=False), 31)
+    om = w.om
+    assert w.pm.download().tobytes() == om.tobytes()
+    assert (om["sid_head"] == 0).sum() > 0 or len(np.unique(om["pos"])) < len(om)
+    for case in ("pe_errors", "se_errorless", "pe_high_error"):
+        kw = cases.SEQ_CASES[case]
+        a, d, p, n = core.simulate_seq_arrays(ctx, w.g, w.pf, w.pm, 8, **kw)
+        oa, od, op, on = O.simulate_seq(w.lens, w.of, w.om, 8, **kw)
+        assert np.array_equal(d, od) and np.array_equal(a, oa) and np.array_equal(p, op), case
+
+
+def test_batch_equals_plain(ctx, world):
+    say("[parity] batched launch of 3 jobs vs oracle")
+    spec = [(world, 11, cases.SEQ_CASES["pe_errors"]), (world, 12, cases.SEQ_CASES["purity"]),
+            (world, 13, cases.SEQ_CASES["se_errorless"])]
+    for cfg in (None, A.LaunchConfig(512, 512, 4, 2, 1), A.LaunchConfig(256, 2048, 2, 16, 0)):
+        jobs = [core.SeqJob(ctx, w.g, w.pf, w.pm, seed, **kw) for w, seed, kw in spec]
+        for j in jobs:
+            for buf in (j.alt, j.depth, j.present):
+                buf.fill(0xA5)
+        n = core.simulate_seq_batch(ctx, jobs, cfg)
+        total = 0
+        for (w, seed, kw), j in zip(spec, jobs):
+            a, d, p = j.results()
+            oa, od, op, on = O.simulate_seq(w.lens, w.of, w.om, seed, **kw)
+            total += on
+            assert np.array_equal(a, oa) and np.array_equal(d, od) and np.array_equal(p, op)
+        assert n == total
+
+
+def _reads(ctx, w, seed, kw):
+    out, n = core.simulate_seq_reads(ctx, w.g, w.pf, w.pm, seed, **kw)
+    assert n == core.num_reads(w.g, w.pf, **kw)
+    return out.numpy()[:n]
+
+
+def _recount(w, reads, kw, n_out):
+    """Count alt/depth from materialised reads (SNV and indel heads)."""
+    L = kw["read_size"]
+    pe = kw.get("insert_size_mean", 0) > 0
+    om = w.om
+    M = len(om)
+    alt = np.zeros((n_out, M), np.int64)
+    dep = np.zeros((n_out, M), np.int64)
+    chrs = kw.get("chromosomes") or list(range(len(w.lens)))
+    idx = 0
+    pos_by_chr = {c: np.where(om["chr"] == c)[0] for c in range(len(w.lens))}
+    for so in range(n_out):
+        for c in sorted(set(chrs)):
+            ln = int(w.lens[c])
+            if ln < L:
+                continue
+            T = int(np.round(kw["coverage"] * ln / (2 * L if pe else L)))
+            # llround rounds half away from zero; np.round rounds half to even
+            t_exact = kw["coverage"] * ln / (2 * L if pe else L)
+            if abs(t_exact - np.floor(t_exact) - 0.5) < 1e-12:
+                T = int(np.floor(t_exact)) + 1
+            n = T * (2 if pe else 1)
+            rows = reads[idx: idx + n]
+            idx += n
+            ids = pos_by_chr[c]
+            mp = om["pos"][ids].astype(np.int64)
+            for r in rows:
+                x = int(r[0])
+                leaf = int(r[1] & 0x3FFFFFF)
+                allele = int((r[1] >> 26) & 0xF)
+                lo = np.searchsorted(mp, x)
+                hi = np.searchsorted(mp, x + L)
+                for k in range(lo, hi):
+                    q = ids[k]
+                    off = int(om["pos"][q]) - x
+                    base = int((r[2 + off // 16] >> (2 * (off % 16))) & 3)
+                    dep[so, q] += 1
+                    if not om["sid_head"][q]:
+                        continue
+                    if om["kind"][q] == A.PSX_SNV:
+                        alt[so, q] += base == om["alt"][q]
+                    else:
+                        j, car = q, False
+                        while j < M and (j == q or not om["sid_head"][j]):
+                            rec = om[j]
+                            if rec["allele"] == allele:
+                                if leaf == w.of.n_leaves:
+                                    car |= bool(rec["nature"] == A.PSX_GERMINAL or
+                                                (rec["nature"] == A.PSX_PRENEOPLASTIC and
+                                                 kw.get("preneoplastic_in_normal", False)))
+                                else:
+                                    car |= bool(rec["leaf_lo"] <= leaf < rec["leaf_hi"])
+                            j += 1
+                        alt[so, q] += car
+    assert idx == len(reads)
+    return alt, dep
+
+
+@pytest.mark.parametrize("case", ["pe_errors", "se_errorless", "purity"])
+def test_materialised_reads_match_counts(ctx, world, case):
+    kw = cases.SEQ_CASES[case]
+    say(f"[parity] materialised reads, case {case}")
+    reads = _reads(ctx, world, 21, kw)
+    n_out = core.num_out_samples(world.pf, **kw)
+    alt, dep = _recount(world, reads, kw, n_out)
+    oa, od, op, on = O.simulate_seq(world.lens, world.of, world.om, 21, **kw)
+    assert np.array_equal(dep, od)
+    assert np.array_equal(alt, oa)
