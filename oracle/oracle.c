@@ -41,6 +41,16 @@ static int fail(const char* fmt, ...) {
 const char* orc_last_error(void) { return g_err; }
 void orc_free(void* p) { free(p); }
 
+int orc_set_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
+}
+
 /* ------------------------------------------------------------ Philox4x32-10 */
 
 void orc_philox(const uint32_t ctr[4], uint64_t key, uint32_t out[4]) {

@@ -41,6 +41,9 @@ typedef struct orc_forest {
 
 const char* orc_last_error(void);
 void orc_free(void* p);
+/* OpenMP threads used by orc_simulate_seq (0 = runtime default); returns the
+ * number that will be used. */
+int orc_set_threads(int n);
 
 uint64_t orc_genome_words(uint32_t n_chr, const uint32_t* len);
 int orc_genome_synth(uint32_t n_chr, const uint32_t* len, uint64_t seed, uint32_t* packed);

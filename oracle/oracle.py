@@ -36,6 +36,7 @@ def lib():
         sig = {
             "orc_last_error": (C.c_char_p, []),
             "orc_free": (None, [vp]),
+            "orc_set_threads": (C.c_int, [C.c_int]),
             "orc_genome_words": (C.c_uint64, [C.c_uint32, A.u32p]),
             "orc_genome_synth": (st, [C.c_uint32, A.u32p, C.c_uint64, A.u32p]),
             "orc_forest_flatten": (st, [C.POINTER(A.ForestDesc), fp]),
@@ -67,6 +68,11 @@ class OracleError(RuntimeError):
 def _check(rc):
     if rc != 0:
         raise OracleError(lib().orc_last_error().decode())
+
+
+def set_threads(n: int) -> int:
+    """OpenMP threads of simulate_seq (0 keeps the default); returns the count in use."""
+    return int(lib().orc_set_threads(int(n)))
 
 
 def philox(ctr, key: int) -> np.ndarray:

@@ -1,0 +1,126 @@
+"""Seeded synthetic workloads: the plain, large, batched and serving
+configurations of the benchmark, built from the library's synthetic inputs
+(the reference's FASTA, COSMIC signatures and spatial simulation need
+network access or stay on the CPU)."""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import core
+
+
+def log_uniform_lengths(n: int, total: int, lo: float, hi: float, rng) -> np.ndarray:
+    """n chromosome lengths, log-uniform on [lo, hi] before normalising to total."""
+    w = np.exp(rng.uniform(np.log(lo), np.log(hi), n))
+    lens = np.floor(w / w.sum() * total).astype(np.int64)
+    lens[np.argmax(lens)] += total - lens.sum()
+    return lens.astype(np.uint32)
+
+
+def dirichlet_signature(rng) -> np.ndarray:
+    return rng.dirichlet(np.ones(96))
+
+
+@dataclass
+class Workload:
+    name: str
+    lengths: np.ndarray
+    genome_seed: int
+    forest: dict
+    engine: dict
+    seq: dict
+    place_seed: int
+    seq_seed: int
+    notes: dict = field(default_factory=dict)
+
+
+def base_at(genome: core.Genome, packed_cache: dict, chr_: int, pos: int) -> int:
+    if "packed" not in packed_cache:
+        packed_cache["packed"] = genome.download()
+    lens = genome.lengths
+    off = int(sum((int(l) + 15) // 16 for l in lens[:chr_]))
+    return int((packed_cache["packed"][off + pos // 16] >> (2 * (pos % 16))) & 3)
+
+
+def plain_large(seed: int = 2026, n_cells: int = 1_000_000, sample_size: int = 10_000,
+                total_bases: int = 300_000_000) -> Workload:
+    """Two species (growth 0.2; driver 0.3 introduced at 1,000 cells) grown to
+    1e6 cells, three samples of ~1e4 cells, 300 Mb over 22 chromosomes with
+    log-uniform lengths, SNV 1e-8 and indel 1e-9 per base per division,
+    200x paired-end 150 bp reads (insert 350 +- 10), error rate 1e-3, normal
+    sample included."""
+    rng = np.random.default_rng(seed)
+    lens = log_uniform_lengths(22, total_bases, 1.0, 10.0, rng)
+    forest = core.synth_forest(growth=[0.2, 0.3], death=[0.01, 0.01], intro_at=[0, 1000], n_cells=n_cells,
+                               n_samples=3, sample_size=sample_size, seed=seed + 1)
+    sig = dirichlet_signature(rng)
+    engine = dict(snv_rate=[1e-8, 1e-8], indel_rate=[1e-9, 1e-9], sbs=sig[None, :], epoch_start=[0.0],
+                  exposure=[[1.0]], n_pre_snv=0, n_pre_indel=0, pre_sbs=0, indel_max_len=10, indel_len_p=0.5,
+                  infinite_sites=True, ploidy=2, drivers=[], germline=[])
+    seq = dict(coverage=200.0, read_size=150, insert_size_mean=350, insert_size_stddev=10, error_rate=1e-3,
+               purity=1.0, with_normal_sample=True)
+    return Workload("plain_large", lens, seed + 2, forest, engine, seq, seed + 3, seed + 4,
+                    notes=dict(n_cells=n_cells, samples=3, sample_size=sample_size, genome=int(lens.sum()),
+                               chromosomes=22, driver_species=1, driver_intro_at=1000))
+
+
+def plain_small(seed: int = 7) -> Workload:
+    """One species (growth 0.2, death 0.01) grown to ~5,000 cells, one sample
+    of 2,500 cells, a 10 Mb single-chromosome reference, SNV 1e-8, 80x
+    paired-end 150 bp reads, error rate 1e-3, normal sample included."""
+    rng = np.random.default_rng(seed)
+    lens = np.array([10_000_000], np.uint32)
+    forest = core.synth_forest(growth=[0.2], death=[0.01], intro_at=[0], n_cells=5000, n_samples=1,
+                               sample_size=2500, seed=seed + 1)
+    sig = dirichlet_signature(rng)
+    engine = dict(snv_rate=[1e-8], indel_rate=[0.0], sbs=sig[None, :], epoch_start=[0.0], exposure=[[1.0]],
+                  n_pre_snv=0, n_pre_indel=0, pre_sbs=0, indel_max_len=10, indel_len_p=0.5, infinite_sites=True,
+                  ploidy=2, drivers=[], germline=[])
+    seq = dict(coverage=80.0, read_size=150, insert_size_mean=350, insert_size_stddev=10, error_rate=1e-3,
+               purity=1.0, with_normal_sample=True)
+    return Workload("plain_small", lens, seed + 2, forest, engine, seq, seed + 3, seed + 4)
+
+
+def add_driver(w: Workload, genome: core.Genome, species: int = 1) -> None:
+    """A driver SNV of `species` at the middle of chromosome 0."""
+    cache: dict = {}
+    pos = int(w.lengths[0]) // 2
+    ref = base_at(genome, cache, 0, pos)
+    w.engine["drivers"] = [(species, 0, pos, (ref + 1) % 4, -1)]
+
+
+@dataclass
+class Built:
+    workload: Workload
+    genome: core.Genome
+    forest: core.Forest
+    muts: core.Mutations
+    place_ms: float
+
+
+def build(ctx: core.Context, w: Workload, driver_species: int | None = None) -> Built:
+    g = core.Genome(ctx, w.lengths, seed=w.genome_seed)
+    if driver_species is not None:
+        add_driver(w, g, driver_species)
+    fd = w.forest
+    f = core.Forest(ctx, fd["cell_id"], fd["parent"], fd["species"], fd["sample"], fd["birth"], fd["n_samples"])
+    m = core.Mutations.place(ctx, g, f, w.place_seed, **w.engine)
+    return Built(w, g, f, m, ctx.last_place_ms())
+
+
+def batched_jobs(seed: int = 99, n_jobs: int = 256, total_bases: int = 50_000_000, min_cells: int = 1_000,
+                 max_cells: int = 100_000, min_cov: float = 30.0, max_cov: float = 300.0):
+    """Independent jobs over one shared reference: leaves log-uniform in
+    [min_cells, max_cells], coverage log-uniform in [min_cov, max_cov]."""
+    rng = np.random.default_rng(seed)
+    lens = log_uniform_lengths(4, total_bases, 1.0, 4.0, rng)
+    jobs = []
+    for k in range(n_jobs):
+        leaves = int(np.exp(rng.uniform(np.log(min_cells), np.log(max_cells))))
+        cov = float(np.exp(rng.uniform(np.log(min_cov), np.log(max_cov))))
+        jobs.append(dict(leaves=leaves, coverage=cov, forest_seed=int(rng.integers(1 << 31)),
+                         place_seed=int(rng.integers(1 << 31)), seq_seed=int(rng.integers(1 << 31))))
+    sig = dirichlet_signature(rng)
+    return lens, sig, jobs
