@@ -256,6 +256,28 @@ psx_itable psx_insert_table(uint32_t mean, uint32_t sd) {
     return psx_binomial_table(t, p, 4096);
 }
 
+psx_ctable psx_compact_table(const psx_itable& t) {
+    const size_t n = t.T.size();
+    const uint64_t one = uint64_t(1) << 32;
+    size_t lo = 0;
+    while (lo + 1 < n && t.T[lo] == 0) ++lo;  // thresholds of 0 are never reached
+    size_t hi = lo;
+    while (hi + 1 < n && t.T[hi] < one) ++hi;  // nothing after the first 2^32 is reached
+    psx_ctable c;
+    c.k0 = t.k0 + (uint32_t)lo;
+    const size_t m = hi - lo + 1;
+    c.T.resize(m);
+    for (size_t i = 0; i < m; ++i) c.T[i] = t.T[lo + i] >= one ? 0xFFFFFFFFu : (uint32_t)t.T[lo + i];
+    c.guide.resize(PSX_GUIDE);
+    size_t k = 0;
+    for (uint32_t g = 0; g < PSX_GUIDE; ++g) {
+        const uint64_t target = (uint64_t)g << 24;
+        while (k + 1 < m && (uint64_t)c.T[k] <= target) ++k;
+        c.guide[g] = (uint16_t)k;
+    }
+    return c;
+}
+
 extern "C" psx_status psx_insert_size_check(psx_ctx* ctx, uint32_t mean, uint32_t stddev) {
     return psx_guard(ctx, [&] {
         if (mean == 0) return;

@@ -138,6 +138,14 @@ psx_itable psx_poisson_table(double lambda);
 psx_itable psx_binomial_table(uint32_t t, double p, uint32_t cap);
 psx_itable psx_error_table(uint32_t L, double e);
 psx_itable psx_insert_table(uint32_t mean, uint32_t sd);
+// Kernel form of an inversion table (psx_guided_pick in psx_rng.h): draws
+// are identical to psx_cdf_pick on the full table.
+struct psx_ctable {
+    uint32_t k0 = 0;
+    std::vector<uint32_t> T;
+    std::vector<uint16_t> guide;  // PSX_GUIDE entries
+};
+psx_ctable psx_compact_table(const psx_itable& t);
 
 uint64_t psx_sid_key(const psx_mutation& r);
 uint32_t psx_info_word(const psx_mutation& r);

@@ -61,11 +61,10 @@ PSX_HD uint64_t psx_mulhi64(uint64_t a, uint64_t b) {
 PSX_HD psx_u4 psx_philox(psx_u4 c, uint32_t k0, uint32_t k1) {
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
-        const uint32_t lo0 = 0xD2511F53u * c.x;
-        const uint32_t hi0 = psx_mulhi32(0xD2511F53u, c.x);
-        const uint32_t lo1 = 0xCD9E8D57u * c.z;
-        const uint32_t hi1 = psx_mulhi32(0xCD9E8D57u, c.z);
-        c = psx_u4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
+        // one 32x32->64 multiply each (a single v_mad_u64_u32 on gfx950)
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c.x;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c.z;
+        c = psx_u4{(uint32_t)(p1 >> 32) ^ c.y ^ k0, (uint32_t)p1, (uint32_t)(p0 >> 32) ^ c.w ^ k1, (uint32_t)p0};
         k0 += 0x9E3779B9u;
         k1 += 0xBB67AE85u;
     }
@@ -86,6 +85,18 @@ PSX_HD uint32_t psx_uniform32(uint32_t u, uint32_t n) { return psx_mulhi32(u, n)
 // floor(((hi << 32) | lo) * n / 2^64): uniform in [0, n).
 PSX_HD uint64_t psx_uniform64(uint32_t hi, uint32_t lo, uint64_t n) {
     return psx_mulhi64(((uint64_t)hi << 32) | lo, n);
+}
+
+// Compact form of an inversion table for the kernels: entries that can never
+// be drawn are dropped (leading T[k] == 0; everything after the first
+// T[k] == 2^32), so every kept threshold but the last fits in 32 bits, and a
+// 256-entry guide table g[i] = first k with T[k] > i * 2^24 starts the scan.
+// The draw equals psx_cdf_pick on the full table.
+#define PSX_GUIDE 256u
+PSX_HD uint32_t psx_guided_pick(const uint32_t* T32, const uint16_t* guide, uint32_t n, uint32_t u) {
+    uint32_t k = guide[u >> 24];
+    while (k + 1 < n && u >= T32[k]) ++k;
+    return k;
 }
 
 // First k in [0, n) with u < T[k]; T is non-decreasing with T[n-1] = 2^32.

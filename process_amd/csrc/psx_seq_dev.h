@@ -1,0 +1,553 @@
+// Device side of simulate_seq on gfx950: descriptors, the count-only kernel
+// and the read-materialising kernel. Included once, by psx_seq.hip.
+//
+// Work decomposition. A segment is one (job, output sample, chromosome); its
+// W = len - L + 1 template starts are cut into strata of PSX_STRATUM starts,
+// and stratum j owns a fixed number of templates (part of the sampling model,
+// so results do not depend on the launch). A work item is a run of strata of
+// one segment; persistent blocks pull items from a device counter.
+//
+// Per stratum a block stages in LDS the mutation records the stratum's reads
+// can reach (pos, leaf range, packed info) with a 64-base bucket index, plus
+// the job's inversion tables; reads are generated from counters and looked up
+// in LDS, counts go to LDS counters and the block flushes them with one global
+// atomic per touched record. All counting is integer.
+#pragma once
+#include "psx_internal.h"
+
+namespace psxk {
+
+constexpr uint32_t NB_SHIFT = 6;      // 64-base LDS buckets
+constexpr uint32_t MAX_READ = 4096;   // read size and insert reach limit
+constexpr uint32_t NB_MAX = (PSX_STRATUM + MAX_READ) / 64 + 2;
+constexpr uint32_t MAX_INS = 640;     // compact insert-size thresholds in LDS
+constexpr uint32_t MAX_ERRT = PSX_MAX_READ_ERRORS + 1;
+constexpr uint32_t REF_MAX = (PSX_STRATUM + MAX_READ) / 16 + 4;
+constexpr uint32_t UNRESOLVED = 0xFFFFFFFFu;
+
+// Descriptors are read-only for the whole launch: reading them through the
+// constant address space lets the compiler use scalar loads, so the fields
+// live in SGPRs instead of taking VGPRs from every lane.
+template <class T>
+__device__ __forceinline__ T uniform_at(const T* p, uint64_t i) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return ((const __attribute__((address_space(4))) T*)p)[i];
+#else
+    return p[i];
+#endif
+}
+
+struct job_dev {
+    uint64_t seed;
+    const uint32_t* pos;
+    const uint32_t* lo;
+    const uint32_t* hi;
+    const uint32_t* info;
+    const uint32_t* bucket;
+    const uint32_t* sample_leaves;
+    int32_t* alt;
+    int32_t* depth;
+    uint8_t* present;
+    const uint32_t* ins_T;       // compact insert-size thresholds
+    const uint16_t* ins_guide;
+    const uint32_t* err_T;       // compact error-count thresholds (first value 0)
+    const uint32_t* packed;      // reference (read materialisation)
+    uint64_t n_mut;
+    uint64_t thr_normal;         // floor((1 - purity) 2^32)
+    uint32_t ins_k0, ins_n, err_n;
+    uint32_t L, pe, ploidy, n_leaves, preneo, reach;
+    uint32_t n_out;
+    uint32_t purity_lt1;
+};
+
+struct seg_dev {
+    uint64_t T;             // templates of the segment
+    uint64_t out_off;       // so * n_mut
+    uint64_t read_base;     // first read index of the segment (materialisation)
+    uint64_t chr_woff;      // first packed word of the chromosome
+    uint32_t job, so, chr, len;
+    uint32_t W, J, bucket_off, n_bucket;
+    uint32_t leaf_off, n_cells;  // n_cells == 0: normal sample
+};
+
+struct item_dev {
+    uint32_t seg, j0, j1, pad;
+};
+
+struct pres_dev {  // one per (job, output sample) for the init kernel
+    uint64_t begin;        // global index of the first (so, m) entry
+    uint32_t job, so, leaf_off, n_cells;
+};
+
+struct seq_launch {
+    const job_dev* jobs;
+    const seg_dev* segs;
+    const item_dev* items;
+    uint64_t n_items;
+    unsigned long long* queue;
+    uint32_t* reads_out;     // materialisation only
+    uint32_t read_words;
+    uint32_t batch_templates;  // templates staged per LDS batch (materialisation)
+};
+
+struct lds_tables {
+    uint32_t ins[MAX_INS];
+    uint16_t guide[PSX_GUIDE];
+    uint32_t err[MAX_ERRT];
+};
+
+__device__ __forceinline__ void load_tables(const job_dev& J, lds_tables& t, uint32_t tid, uint32_t block) {
+    if (J.pe) {
+        for (uint32_t i = tid; i < J.ins_n; i += block) t.ins[i] = J.ins_T[i];
+        for (uint32_t i = tid; i < PSX_GUIDE; i += block) t.guide[i] = J.ins_guide[i];
+    }
+    for (uint32_t i = tid; i < J.err_n; i += block) t.err[i] = J.err_T[i];
+}
+
+// ------------------------------------------------------------- record views
+
+struct lds_view {
+    const uint32_t* pos;
+    const uint32_t* lo;
+    const uint32_t* hi;
+    const uint32_t* info;
+    uint32_t* alt;
+    uint32_t* dep;
+    const uint16_t* bucket;
+    uint32_t n, b;
+    __device__ __forceinline__ uint32_t first(uint32_t x) const {
+        uint32_t i = bucket[(x - b) >> NB_SHIFT];
+        while (i < n && pos[i] < x) ++i;
+        return i;
+    }
+    __device__ __forceinline__ void add_dep(uint32_t q) const { atomicAdd(&dep[q], 1u); }
+    __device__ __forceinline__ void add_alt(uint32_t q) const { atomicAdd(&alt[q], 1u); }
+};
+
+struct glb_view {  // window larger than the LDS capacity
+    const uint32_t* pos;
+    const uint32_t* lo;
+    const uint32_t* hi;
+    const uint32_t* info;
+    int32_t* alt;
+    int32_t* dep;
+    uint32_t n;
+    __device__ __forceinline__ uint32_t first(uint32_t x) const {
+        uint32_t a = 0, z = n;
+        while (a < z) {
+            const uint32_t mid = (a + z) >> 1;
+            if (pos[mid] < x) a = mid + 1; else z = mid;
+        }
+        return a;
+    }
+    __device__ __forceinline__ void add_dep(uint32_t q) const { atomicAdd(&dep[q], 1); }
+    __device__ __forceinline__ void add_alt(uint32_t q) const { atomicAdd(&alt[q], 1); }
+};
+
+// Template: start of mate 1 and mate 2, source cell (resolved to a leaf on
+// first use) and copy.
+struct tmpl {
+    uint32_t x, x2, cell, allele, leaf;
+};
+
+__device__ __forceinline__ uint32_t leaf_of(const job_dev& J, const seg_dev& S, tmpl& t) {
+    if (t.leaf == UNRESOLVED) t.leaf = J.sample_leaves[S.leaf_off + t.cell];
+    return t.leaf;
+}
+
+__device__ __forceinline__ bool carried(const job_dev& J, uint32_t info, uint32_t lo, uint32_t hi, uint32_t leaf) {
+    if (leaf == J.n_leaves) {
+        const uint32_t nat = PSX_INFO_NATURE(info);
+        return nat == PSX_GERMINAL || (nat == PSX_PRENEOPLASTIC && J.preneo);
+    }
+    return lo <= leaf && leaf < hi;
+}
+
+// Template r of stratum j (oracle: the template loop of orc_simulate_seq).
+__device__ __forceinline__ tmpl make_template(const job_dev& J, const seg_dev& S, const lds_tables& tab, uint32_t r,
+                                              uint32_t j, uint32_t b, uint32_t e) {
+    const uint32_t c2 = (S.chr << 16) | S.so;
+    const psx_u4 w = psx_draw(J.seed, r, j, c2, PSX_P_TEMPLATE << 24);
+    tmpl t;
+    t.x = b + psx_uniform32(w.x, e - b);
+    if (S.n_cells == 0 || (uint64_t)w.y < J.thr_normal) {
+        t.leaf = J.n_leaves;
+        t.cell = 0;
+        t.allele = psx_uniform32(w.z, J.ploidy);
+    } else {
+        const uint32_t X = psx_uniform32(w.z, S.n_cells * J.ploidy);
+        if (J.ploidy == 2) { t.cell = X >> 1; t.allele = X & 1u; }
+        else { t.cell = X / J.ploidy; t.allele = X - t.cell * J.ploidy; }
+        t.leaf = UNRESOLVED;
+    }
+    t.x2 = t.x;
+    if (J.pe) {
+        uint32_t I = J.ins_k0 + psx_guided_pick(tab.ins, tab.guide, J.ins_n, w.w);
+        if (I < J.L) I = J.L;
+        uint64_t x2 = (uint64_t)t.x + I - J.L;
+        if (x2 > S.len - J.L) x2 = S.len - J.L;
+        t.x2 = (uint32_t)x2;
+    }
+    return t;
+}
+
+// Error set of one read (oracle: error_set): K ~ Binomial(L, e) from word 0,
+// then K distinct offsets, one word each; a repeated offset takes the next
+// word. Slots hold offset | substitution << 12.
+__device__ uint32_t error_set(const job_dev& J, const lds_tables& tab, uint32_t r, uint32_t j, uint32_t c2,
+                              uint32_t mate, uint16_t* slots, uint32_t stride) {
+    const uint32_t base = (PSX_P_ERROR << 24) | (mate << 16);
+    psx_u4 w = psx_draw(J.seed, r, j, c2, base);
+    uint32_t K = 0;
+    while (K + 1 < J.err_n && w.x >= tab.err[K]) ++K;
+    uint32_t cnt = 0;
+    for (uint32_t t = 1; cnt < K && t < 1 + 4 * PSX_MAX_READ_ERRORS; ++t) {
+        if ((t & 3u) == 0) w = psx_draw(J.seed, r, j, c2, base | (t >> 2));
+        const uint32_t word = psx_word(w, (int)(t & 3u));
+        const uint32_t o = psx_mulhi32(word, J.L);
+        bool dup = false;
+        for (uint32_t q = 0; q < cnt; ++q) dup |= (slots[q * stride] & 0xFFFu) == o;
+        if (dup) continue;
+        slots[cnt * stride] = (uint16_t)(o | ((((word & 0xFFFFu) * 3u) >> 16) << 12));
+        ++cnt;
+    }
+    return cnt;
+}
+
+// One read [x, x + L): depth for every record at a covered position, an
+// alternative read for each SID head whose alternative is observed (SNV) or
+// carried (indel). Oracle: seq_read.
+template <class V>
+__device__ void count_read(const job_dev& J, const seg_dev& S, const V& v, const lds_tables& tab, tmpl& t,
+                           uint32_t x, uint32_t r, uint32_t j, uint32_t c2, uint32_t mate, uint16_t* slots,
+                           uint32_t stride) {
+    uint32_t i = v.first(x);
+    const uint32_t end = x + J.L;
+    if (i >= v.n || v.pos[i] >= end) return;
+    int ne = -1;
+    while (i < v.n && v.pos[i] < end) {
+        const uint32_t p = v.pos[i];
+        const uint32_t g0 = i;
+        uint32_t g1 = i + 1;
+        while (g1 < v.n && v.pos[g1] == p) ++g1;
+        uint32_t tb = PSX_INFO_REF(v.info[g0]);
+        for (uint32_t q = g0; q < g1; ++q) {
+            const uint32_t in = v.info[q];
+            if (PSX_INFO_KIND(in) == PSX_SNV && PSX_INFO_ALLELE(in) == t.allele &&
+                carried(J, in, v.lo[q], v.hi[q], leaf_of(J, S, t))) {
+                tb = PSX_INFO_ALT(in);
+                break;
+            }
+        }
+        uint32_t ob = tb;
+        if (J.err_n) {
+            if (ne < 0) ne = (int)error_set(J, tab, r, j, c2, mate, slots, stride);
+            for (int q = 0; q < ne; ++q) {
+                const uint32_t s = slots[q * stride];
+                if ((s & 0xFFFu) == p - x) {
+                    const uint32_t sub = s >> 12;
+                    ob = sub < tb ? sub : sub + 1;
+                }
+            }
+        }
+        for (uint32_t q = g0; q < g1; ++q) {
+            v.add_dep(q);
+            const uint32_t in = v.info[q];
+            if (!PSX_INFO_HEAD(in)) continue;
+            if (PSX_INFO_KIND(in) == PSX_SNV) {
+                if (ob == PSX_INFO_ALT(in)) v.add_alt(q);
+            } else {
+                bool car = false;
+                for (uint32_t q2 = q; q2 < g1 && (q2 == q || !PSX_INFO_HEAD(v.info[q2])); ++q2) {
+                    const uint32_t in2 = v.info[q2];
+                    car |= PSX_INFO_ALLELE(in2) == t.allele && carried(J, in2, v.lo[q2], v.hi[q2], leaf_of(J, S, t));
+                }
+                if (car) v.add_alt(q);
+            }
+        }
+        i = g1;
+    }
+}
+
+// Stratum geometry shared by both kernels.
+struct stratum {
+    uint32_t b, e, wend, k0, n_win;
+    uint64_t nj, before;
+};
+
+__device__ __forceinline__ stratum stratum_of(const job_dev& J, const seg_dev& S, uint32_t j) {
+    stratum s;
+    s.b = j * PSX_STRATUM;
+    s.e = min(s.b + PSX_STRATUM, S.W);
+    s.before = S.T * s.b / S.W;
+    s.nj = S.T * s.e / S.W - s.before;
+    s.wend = min(S.len, s.e - 1 + J.reach);
+    s.k0 = uniform_at(J.bucket, S.bucket_off + j);
+    const uint32_t kb = min((s.wend + PSX_STRATUM - 1) / PSX_STRATUM, S.n_bucket - 1);
+    s.n_win = uniform_at(J.bucket, S.bucket_off + kb) - s.k0;
+    return s;
+}
+
+template <int BLOCK, int CAP>
+__device__ __forceinline__ void stage_window(const job_dev& J, const stratum& s, uint32_t* s_pos, uint32_t* s_lo,
+                                             uint32_t* s_hi, uint32_t* s_info, uint32_t* s_alt, uint32_t* s_dep,
+                                             uint16_t* s_bucket, uint32_t tid) {
+    for (uint32_t i = tid; i < s.n_win; i += BLOCK) {
+        s_pos[i] = J.pos[s.k0 + i];
+        s_lo[i] = J.lo[s.k0 + i];
+        s_hi[i] = J.hi[s.k0 + i];
+        s_info[i] = J.info[s.k0 + i];
+        if (s_alt) {
+            s_alt[i] = 0;
+            s_dep[i] = 0;
+        }
+    }
+    __syncthreads();
+    const uint32_t nb = ((s.wend - s.b) >> NB_SHIFT) + 1;
+    for (uint32_t q = tid; q < nb; q += BLOCK) {
+        const uint32_t target = s.b + (q << NB_SHIFT);
+        uint32_t a = 0, z = s.n_win;
+        while (a < z) {
+            const uint32_t mid = (a + z) >> 1;
+            if (s_pos[mid] < target) a = mid + 1; else z = mid;
+        }
+        s_bucket[q] = (uint16_t)a;
+    }
+    __syncthreads();
+}
+
+// ------------------------------------------------------------ count kernel
+
+// A stratum whose window exceeds the LDS capacity: reads search the global
+// records and count with global atomics (kept out of line so the LDS path's
+// code is shared by both mates).
+template <int BLOCK>
+__device__ __forceinline__ void count_stratum_global(const job_dev& J, const seg_dev& S, const lds_tables& tab,
+                                                  const stratum& s, uint32_t j, uint32_t c2, uint16_t* slots,
+                                                  uint32_t tid) {
+    const glb_view v{J.pos + s.k0, J.lo + s.k0, J.hi + s.k0, J.info + s.k0, J.alt + S.out_off + s.k0,
+                     J.depth + S.out_off + s.k0, s.n_win};
+    for (uint32_t r = tid; r < (uint32_t)s.nj; r += BLOCK) {
+        tmpl t = make_template(J, S, tab, r, j, s.b, s.e);
+        for (uint32_t mate = 0; mate <= J.pe; ++mate)
+            count_read(J, S, v, tab, t, mate ? t.x2 : t.x, r, j, c2, mate, slots, BLOCK);
+    }
+}
+
+template <int BLOCK, int CAP>
+__global__ void __launch_bounds__(BLOCK) k_seq_count(const seq_launch L) {
+    __shared__ uint32_t s_pos[CAP], s_lo[CAP], s_hi[CAP], s_info[CAP], s_alt[CAP], s_dep[CAP];
+    __shared__ uint16_t s_bucket[NB_MAX];
+    __shared__ uint16_t s_err[PSX_MAX_READ_ERRORS * BLOCK];
+    __shared__ lds_tables s_tab;
+    __shared__ uint32_t s_item;
+    const uint32_t tid = threadIdx.x;
+    uint16_t* slots = s_err + tid;
+    for (;;) {
+        if (tid == 0) s_item = (uint32_t)atomicAdd(L.queue, 1ull);
+        __syncthreads();
+        const uint32_t item = __builtin_amdgcn_readfirstlane(s_item);
+        if (item >= L.n_items) break;
+        const item_dev it = uniform_at(L.items, item);
+        const seg_dev S = uniform_at(L.segs, it.seg);
+        const job_dev J = uniform_at(L.jobs, S.job);
+        const uint32_t c2 = (S.chr << 16) | S.so;
+        load_tables(J, s_tab, tid, BLOCK);
+        __syncthreads();
+        for (uint32_t j = it.j0; j < it.j1; ++j) {
+            const stratum s = stratum_of(J, S, j);
+            if (s.n_win <= (uint32_t)CAP) {
+                stage_window<BLOCK, CAP>(J, s, s_pos, s_lo, s_hi, s_info, s_alt, s_dep, s_bucket, tid);
+                const lds_view v{s_pos, s_lo, s_hi, s_info, s_alt, s_dep, s_bucket, s.n_win, s.b};
+                for (uint32_t r = tid; r < (uint32_t)s.nj; r += BLOCK) {
+                    tmpl t = make_template(J, S, s_tab, r, j, s.b, s.e);
+                    for (uint32_t mate = 0; mate <= J.pe; ++mate)
+                        count_read(J, S, v, s_tab, t, mate ? t.x2 : t.x, r, j, c2, mate, slots, BLOCK);
+                }
+                __syncthreads();
+                int32_t* ga = J.alt + S.out_off + s.k0;
+                int32_t* gd = J.depth + S.out_off + s.k0;
+                for (uint32_t i = tid; i < s.n_win; i += BLOCK) {
+                    const uint32_t d = s_dep[i];
+                    if (d) {
+                        atomicAdd(&gd[i], (int32_t)d);
+                        const uint32_t a = s_alt[i];
+                        if (a) atomicAdd(&ga[i], (int32_t)a);
+                    }
+                }
+                __syncthreads();
+            } else {
+                count_stratum_global<BLOCK>(J, S, s_tab, s, j, c2, slots, tid);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Presence (the sample holds the mutation) and zeroed counters; also resets
+// the work counter of the count kernel that follows on the stream.
+__global__ void __launch_bounds__(256) k_seq_init(const job_dev* __restrict__ jobs, const pres_dev* __restrict__ ps,
+                                                  uint32_t n_ps, uint64_t total,
+                                                  unsigned long long* __restrict__ queue) {
+    const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q == 0) *queue = 0ull;
+    if (q >= total) return;
+    uint32_t a = 0, z = n_ps;
+    while (a < z) {
+        const uint32_t mid = (a + z) >> 1;
+        if (ps[mid].begin <= q) a = mid + 1; else z = mid;
+    }
+    const pres_dev P = ps[a - 1];
+    const job_dev& J = jobs[P.job];
+    const uint64_t m = q - P.begin;
+    const uint64_t o = (uint64_t)P.so * J.n_mut + m;
+    J.alt[o] = 0;
+    J.depth[o] = 0;
+    const uint32_t info = J.info[m];
+    const uint32_t nat = PSX_INFO_NATURE(info);
+    const bool normal_has = nat == PSX_GERMINAL || (nat == PSX_PRENEOPLASTIC && J.preneo);
+    bool pres;
+    if (P.n_cells == 0) {
+        pres = normal_has;
+    } else {
+        pres = J.purity_lt1 && normal_has;
+        if (!pres) {
+            const uint32_t lo = J.lo[m], hi = J.hi[m];
+            const uint32_t* sl = J.sample_leaves + P.leaf_off;
+            uint32_t x = 0, y = P.n_cells;
+            while (x < y) {
+                const uint32_t mid = (x + y) >> 1;
+                if (sl[mid] < lo) x = mid + 1; else y = mid;
+            }
+            pres = x < P.n_cells && sl[x] < hi;
+        }
+    }
+    J.present[o] = pres ? 1 : 0;
+}
+
+// --------------------------------------------------- read materialisation
+
+// One read into its LDS row: word 0 = start, word 1 = leaf | allele << 26 |
+// mate << 30 | (errors > 0) << 31, then ceil(L/16) words of 2-bit bases (base
+// i of the read in bits 2 (i % 16) of word i / 16) from the staged
+// reference, with the first carried SNV at each position and the read's
+// errors applied; bases past the read end are 0.
+template <class V>
+__device__ void write_read(const job_dev& J, const seg_dev& S, const V& v, const lds_tables& tab, tmpl& t,
+                           uint32_t x, uint32_t r, uint32_t j, uint32_t c2, uint32_t mate, uint16_t* slots,
+                           uint32_t stride, const uint32_t* s_ref, uint32_t ref_w0, uint32_t n_ref, uint32_t* row) {
+    const uint32_t nw = (J.L + 15u) >> 4;
+    const uint32_t tail = J.L & 15u;
+    for (uint32_t k = 0; k < nw; ++k) {
+        const uint32_t x0 = x + (k << 4);
+        const uint32_t w = (x0 >> 4) - ref_w0, sh = (x0 & 15u) * 2u;
+        uint32_t word = s_ref[w];
+        if (sh) {
+            const uint32_t hi = w + 1 < n_ref ? s_ref[w + 1] : 0u;
+            word = (word >> sh) | (hi << (32u - sh));
+        }
+        if (k + 1 == nw && tail) word &= (1u << (2u * tail)) - 1u;
+        row[2 + k] = word;
+    }
+    uint32_t i = v.first(x);
+    const uint32_t end = x + J.L;
+    while (i < v.n && v.pos[i] < end) {
+        const uint32_t p = v.pos[i];
+        bool done = false;
+        while (i < v.n && v.pos[i] == p) {
+            const uint32_t in = v.info[i];
+            if (!done && PSX_INFO_KIND(in) == PSX_SNV && PSX_INFO_ALLELE(in) == t.allele &&
+                carried(J, in, v.lo[i], v.hi[i], leaf_of(J, S, t))) {
+                const uint32_t o = p - x, sh = (o & 15u) * 2u;
+                row[2 + (o >> 4)] = (row[2 + (o >> 4)] & ~(3u << sh)) | (PSX_INFO_ALT(in) << sh);
+                done = true;
+            }
+            ++i;
+        }
+    }
+    uint32_t ne = 0;
+    if (J.err_n) ne = error_set(J, tab, r, j, c2, mate, slots, stride);
+    for (uint32_t q = 0; q < ne; ++q) {
+        const uint32_t s = slots[q * stride];
+        const uint32_t o = s & 0xFFFu, sh = (o & 15u) * 2u;
+        uint32_t wv = row[2 + (o >> 4)];
+        const uint32_t tb = (wv >> sh) & 3u, sub = s >> 12;
+        const uint32_t nb = sub < tb ? sub : sub + 1;
+        row[2 + (o >> 4)] = (wv & ~(3u << sh)) | (nb << sh);
+    }
+    row[0] = x;
+    row[1] = leaf_of(J, S, t) | (t.allele << 26) | (mate << 30) | ((ne > 0 ? 1u : 0u) << 31);
+}
+
+template <int BLOCK, int CAP>
+__global__ void __launch_bounds__(BLOCK) k_seq_reads(const seq_launch L) {
+    __shared__ uint32_t s_pos[CAP], s_lo[CAP], s_hi[CAP], s_info[CAP];
+    __shared__ uint16_t s_bucket[NB_MAX];
+    __shared__ uint16_t s_err[PSX_MAX_READ_ERRORS * BLOCK];
+    __shared__ uint32_t s_ref[REF_MAX];
+    __shared__ lds_tables s_tab;
+    __shared__ uint32_t s_item;
+    extern __shared__ uint4 s_out4[];  // [batch reads][read words], row-major
+    uint32_t* s_out = reinterpret_cast<uint32_t*>(s_out4);
+    const uint32_t tid = threadIdx.x;
+    uint16_t* slots = s_err + tid;
+    const uint32_t RW = L.read_words;
+    const uint32_t TB = L.batch_templates;
+    for (;;) {
+        if (tid == 0) s_item = (uint32_t)atomicAdd(L.queue, 1ull);
+        __syncthreads();
+        const uint32_t item = __builtin_amdgcn_readfirstlane(s_item);
+        if (item >= L.n_items) break;
+        const item_dev it = uniform_at(L.items, item);
+        const seg_dev S = uniform_at(L.segs, it.seg);
+        const job_dev J = uniform_at(L.jobs, S.job);
+        const uint32_t c2 = (S.chr << 16) | S.so;
+        const uint32_t rpt = J.pe ? 2u : 1u;
+        const uint32_t chr_words = (S.len + 15u) >> 4;
+        load_tables(J, s_tab, tid, BLOCK);
+        __syncthreads();
+        for (uint32_t j = it.j0; j < it.j1; ++j) {
+            const stratum s = stratum_of(J, S, j);
+            const bool fits = s.n_win <= (uint32_t)CAP;
+            // reference words covering [b, wend) plus one
+            const uint32_t ref_w0 = s.b >> 4;
+            const uint32_t n_ref = min(chr_words, ((s.wend + 15u) >> 4) + 1u) - ref_w0;
+            for (uint32_t i = tid; i < n_ref; i += BLOCK) s_ref[i] = J.packed[S.chr_woff + ref_w0 + i];
+            if (fits) stage_window<BLOCK, CAP>(J, s, s_pos, s_lo, s_hi, s_info, nullptr, nullptr, s_bucket, tid);
+            else __syncthreads();
+            const lds_view lv{s_pos, s_lo, s_hi, s_info, nullptr, nullptr, s_bucket, s.n_win, s.b};
+            const glb_view gv{J.pos + s.k0, J.lo + s.k0, J.hi + s.k0, J.info + s.k0, nullptr, nullptr, s.n_win};
+            for (uint32_t r0 = 0; r0 < (uint32_t)s.nj; r0 += TB) {
+                const uint32_t n_t = min(TB, (uint32_t)s.nj - r0);
+                if (tid < n_t) {
+                    const uint32_t r = r0 + tid;
+                    tmpl t = make_template(J, S, s_tab, r, j, s.b, s.e);
+                    uint32_t* row = s_out + tid * rpt * RW;
+                    if (fits) write_read(J, S, lv, s_tab, t, t.x, r, j, c2, 0u, slots, BLOCK, s_ref, ref_w0, n_ref, row);
+                    else write_read(J, S, gv, s_tab, t, t.x, r, j, c2, 0u, slots, BLOCK, s_ref, ref_w0, n_ref, row);
+                    if (J.pe) {
+                        if (fits)
+                            write_read(J, S, lv, s_tab, t, t.x2, r, j, c2, 1u, slots, BLOCK, s_ref, ref_w0, n_ref, row + RW);
+                        else
+                            write_read(J, S, gv, s_tab, t, t.x2, r, j, c2, 1u, slots, BLOCK, s_ref, ref_w0, n_ref, row + RW);
+                    }
+                }
+                __syncthreads();
+                // reads [first, first + n_t * rpt) are contiguous in the output
+                const uint64_t first = S.read_base + (s.before + r0) * rpt;
+                const uint32_t total = n_t * rpt * RW;
+                uint32_t* dst = L.reads_out + first * RW;
+                if ((RW & 3u) == 0) {
+                    uint4* d4 = reinterpret_cast<uint4*>(dst);
+                    for (uint32_t q = tid; q < (total >> 2); q += BLOCK) d4[q] = s_out4[q];
+                } else {
+                    for (uint32_t q = tid; q < total; q += BLOCK) dst[q] = s_out[q];
+                }
+                __syncthreads();
+            }
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace psxk
