@@ -68,6 +68,8 @@ def main() -> int:
     ap.add_argument("--no-reads", action="store_true", help="skip the read-materialising mode")
     ap.add_argument("--cpu-reads", type=float, default=1.5e8, help="reads in the bounded CPU sample")
     ap.add_argument("--cells", type=int, default=1_000_000)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="only run device-to-device copies of BYTES (reference for the HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
@@ -77,6 +79,16 @@ def main() -> int:
     from process_amd import core, workloads
 
     ctx = core.Context(0)
+    if args.calibrate_copy:
+        import ctypes as C
+
+        from process_amd._lib import check, lib
+        src = core.DeviceArray(ctx, (args.calibrate_copy,), np.uint8).fill(1)
+        dst = core.DeviceArray(ctx, (args.calibrate_copy,), np.uint8)
+        for _ in range(3):
+            check(lib().psx_dev_copy(ctx.h, C.c_void_p(dst.ptr), C.c_void_p(src.ptr), args.calibrate_copy), ctx.h)
+        log(f"copied {args.calibrate_copy} bytes 3 times")
+        return 0
     t = time.perf_counter()
     w = workloads.plain_large(n_cells=args.cells)
     log(f"inputs: forest {len(w.forest['cell_id'])} nodes, genome {int(w.lengths.sum())} bases "

@@ -86,6 +86,7 @@ void psx_dev_free(psx_ctx* ctx, void* p);
 psx_status psx_dev_upload(psx_ctx* ctx, void* dst, const void* src, uint64_t bytes);
 psx_status psx_dev_download(psx_ctx* ctx, void* dst, const void* src, uint64_t bytes);
 psx_status psx_dev_fill_u8(psx_ctx* ctx, void* dst, uint8_t value, uint64_t bytes);
+psx_status psx_dev_copy(psx_ctx* ctx, void* dst, const void* src, uint64_t bytes);
 
 /* ------------------------------------------------------------------- genome */
 
@@ -245,11 +246,11 @@ psx_status psx_simulate_seq_device(psx_ctx* ctx, const psx_genome* g, const psx_
                                    uint64_t seed, int32_t* d_alt, int32_t* d_depth,
                                    uint8_t* d_present, uint64_t* n_reads);
 
-/* Read materialisation (write_SAM = TRUE analogue): every read as a 16-byte
- * header (position, source leaf, flags) plus ceil(read_size/16) words of
- * 2-bit bases with the source's mutations and sequencing errors applied.
- * d_reads holds psx_read_words(p) uint32 per read, reads ordered by
- * (sample, chromosome, stratum, template, mate). */
+/* Read materialisation (write_SAM = TRUE analogue): every read as an 8-byte
+ * header (start position; source leaf | allele << 26 | mate << 30 | has-error
+ * << 31) plus ceil(read_size/16) words of 2-bit bases with the source's SNVs
+ * and the sequencing errors applied. d_reads holds psx_read_words(p) uint32
+ * per read, reads ordered by (sample, chromosome, stratum, template, mate). */
 uint32_t psx_read_words(const psx_seq_params* p);
 psx_status psx_simulate_seq_reads(psx_ctx* ctx, const psx_genome* g, const psx_forest* f,
                                   const psx_mutations* m, const psx_seq_params* p,

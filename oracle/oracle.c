@@ -755,7 +755,9 @@ static int carried(const seq_ctx* s, const psx_mutation* r, uint32_t leaf, uint3
 
 /* Error set of one read: K ~ Binomial(L, e) from word 0, then K distinct
  * offsets, one word each (offset = floor(w L / 2^32), substitution index =
- * floor((w & 0xFFFF) 3 / 2^16)); a repeated offset takes the next word. */
+ * floor((w & 0xFFFF) 3 / 2^16)); a repeated offset takes the next word. The
+ * read's word stream is Philox(counter = (r, j, c2, ERROR << 24 | mate << 16 |
+ * t / 4)), word t % 4; at most 4 * PSX_MAX_READ_ERRORS words are spent. */
 static uint32_t error_set(const seq_ctx* s, uint32_t r, uint32_t j, uint32_t c2, uint32_t mate,
                           uint32_t* off, uint32_t* sub) {
     uint32_t w[4];

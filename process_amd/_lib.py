@@ -40,6 +40,7 @@ def _declare(lib):
         "psx_dev_upload": (st, [vp, vp, vp, C.c_uint64]),
         "psx_dev_download": (st, [vp, vp, vp, C.c_uint64]),
         "psx_dev_fill_u8": (st, [vp, vp, C.c_uint8, C.c_uint64]),
+        "psx_dev_copy": (st, [vp, vp, vp, C.c_uint64]),
         "psx_genome_synth": (st, [vp, C.c_uint32, A.u32p, C.c_uint64, pp]),
         "psx_genome_upload": (st, [vp, C.c_uint32, A.u32p, A.u32p, pp]),
         "psx_genome_words": (C.c_uint64, [C.c_uint32, A.u32p]),

@@ -140,6 +140,15 @@ extern "C" psx_status psx_dev_fill_u8(psx_ctx* ctx, void* dst, uint8_t value, ui
     });
 }
 
+extern "C" psx_status psx_dev_copy(psx_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
+    if (!ctx || (bytes && (!dst || !src))) return PSX_EINVAL;
+    return psx_guard(ctx, [&] {
+        PSX_HIP(hipSetDevice(ctx->device));
+        if (bytes) PSX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+        PSX_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
 // ---------------------------------------------------------- scratch arena
 
 size_t scratch_builder::put(const void* src, size_t bytes) {
