@@ -219,8 +219,8 @@ typedef struct psx_seq_params {
     int32_t preneoplastic_in_normal;
     uint32_t n_chromosomes;       /* 0 = every chromosome of the genome */
     const uint32_t* chromosomes;
-    uint32_t n_samples;           /* 0 = every forest sample, in index order */
-    const uint32_t* samples;
+    uint32_t n_samples;           /* forest samples to sequence, in order; with samples == NULL */
+    const uint32_t* samples;      /* every forest sample in index order (n_samples is ignored) */
 } psx_seq_params;
 
 /* Validates the paired-end insert-size model Binomial(t, p) with

@@ -865,10 +865,10 @@ int orc_simulate_seq(uint32_t n_chr, const uint32_t* len, const orc_forest* f,
         if (p->chromosomes[k] >= n_chr) rc = fail("unknown chromosome %u", p->chromosomes[k]);
         else sel[p->chromosomes[k]] = 1;
     }
-    uint32_t n_req = p->n_samples ? p->n_samples : f->n_samples;
+    uint32_t n_req = p->samples ? p->n_samples : f->n_samples;
     uint32_t* req = (uint32_t*)malloc(sizeof(uint32_t) * (n_req + 1));
     for (uint32_t k = 0; k < n_req && !rc; ++k) {
-        req[k] = p->n_samples ? p->samples[k] : k;
+        req[k] = p->samples ? p->samples[k] : k;
         if (req[k] >= f->n_samples) rc = fail("unknown sample %u", req[k]);
         else if (f->sample_begin[req[k] + 1] == f->sample_begin[req[k]]) rc = fail("sample %u has no cells", req[k]);
     }

@@ -127,9 +127,11 @@ def seq_params(*, coverage=10.0, read_size=150, insert_size_mean=0, insert_size_
                purity=1.0, with_normal_sample=True, preneoplastic_in_normal=False, chromosomes=None,
                samples=None) -> tuple[SeqParams, list]:
     ch = None if chromosomes is None else np.ascontiguousarray(chromosomes, np.uint32)
-    sm = None if samples is None else np.ascontiguousarray(samples, np.uint32)
+    # an empty list still passes a non-NULL pointer: no tumour sample
+    sm = None if samples is None else np.ascontiguousarray(samples, np.uint32) if len(samples) else \
+        np.zeros(1, np.uint32)
     p = SeqParams(float(coverage), int(read_size), int(insert_size_mean), int(insert_size_stddev),
                   float(error_rate), float(purity), int(bool(with_normal_sample)), int(bool(preneoplastic_in_normal)),
-                  0 if ch is None else len(ch), ptr(ch, C.c_uint32), 0 if sm is None else len(sm),
+                  0 if ch is None else len(ch), ptr(ch, C.c_uint32), 0 if samples is None else len(samples),
                   ptr(sm, C.c_uint32))
     return p, [ch, sm]

@@ -1,0 +1,558 @@
+"""R-level API of ProCESS's mutation engine and sequencing simulation, running
+on the gfx950 path (libpsx).
+
+Names, arguments, defaults and result layout follow the reference's Rcpp
+modules: ``MutationEngine$place_mutations`` (src/mutations.cpp:546-573,
+src/mutation_engine.hpp:160-206), ``simulate_seq`` / ``simulate_normal_seq``
+(src/sequencing.cpp:193-209, 268-282; src/seq_simulation.cpp:517-679),
+``BasicIlluminaSequencer`` / ``ErrorlessIlluminaSequencer``
+(src/sequencing.cpp:36-128, src/sequencers.cpp:80-106) and ``seq_to_long``
+(R/seq_to_long.R:31-67). Inputs that the reference downloads (reference
+FASTA, COSMIC signatures) are given explicitly or synthesised.
+"""
+from __future__ import annotations
+
+import os
+import random
+from dataclasses import dataclass
+from typing import Callable, Iterable, Mapping, Sequence
+
+import numpy as np
+import pandas as pd
+
+from . import _abi as A
+from . import core
+
+BASES = np.array(list("ACGT"))
+NATURE = {A.PSX_PASSENGER: "passenger", A.PSX_DRIVER: "driver", A.PSX_PRENEOPLASTIC: "preneoplastic",
+          A.PSX_GERMINAL: "germinal"}
+
+
+def get_random_seed(seed) -> int:
+    """src/utility.hpp:41-64: an integer seed, or one drawn when NULL."""
+    if seed is None:
+        return random.randrange(-(2**31), 2**31)
+    if isinstance(seed, (bool, np.bool_)) or not isinstance(seed, (int, np.integer)):
+        raise TypeError("The seed must be an integer or None.")
+    return int(seed)
+
+
+def _seed64(seed: int) -> int:
+    return int(seed) & 0xFFFFFFFFFFFFFFFF
+
+
+# ---------------------------------------------------------------- inputs
+
+class SamplesForest:
+    """Genealogy of the sampled cells (SamplesForest, src/samples_forest.hpp:31-82).
+
+    One row per cell: ``cell_id``, ``ancestor`` (-1 for roots), ``mutant``
+    (species name), ``sample`` (name of a sampled leaf, None otherwise),
+    ``birth_time``."""
+
+    def __init__(self, cell_id, ancestor, mutant: Sequence[str], sample: Sequence, birth_time,
+                 epistate: Sequence[str] | None = None):
+        self.cell_id = np.asarray(cell_id, np.int64)
+        self.ancestor = np.asarray(ancestor, np.int64)
+        self.mutant = np.asarray(mutant, dtype=object)
+        self.epistate = np.asarray(epistate if epistate is not None else [""] * len(self.cell_id), dtype=object)
+        self.sample = np.asarray(sample, dtype=object)
+        self.birth_time = np.asarray(birth_time, np.float64)
+        names = [s for s in dict.fromkeys(self.sample.tolist()) if s is not None and s == s]
+        self.sample_names = names
+        species = list(dict.fromkeys(zip(self.mutant.tolist(), self.epistate.tolist())))
+        self.species = species
+
+    @classmethod
+    def synthetic(cls, *, mutants: Mapping[str, tuple[float, float]], introductions: Mapping[str, int] | None = None,
+                  n_cells: int, samples: Mapping[str, int], seed: int) -> "SamplesForest":
+        """A seeded birth-death genealogy standing in for
+        ``SpatialSimulation$get_samples_forest()`` (the spatial simulation stays
+        outside the accelerated path). ``mutants`` maps a name to (growth,
+        death); the first founds the tumour; ``introductions`` gives, for the
+        others, the population size at which one founder cell converts."""
+        names = list(mutants)
+        intro = introductions or {}
+        sizes = list(samples.values())
+        if len(set(sizes)) != 1:
+            raise ValueError("synthetic samples must have equal sizes")
+        fd = core.synth_forest(growth=[mutants[n][0] for n in names], death=[mutants[n][1] for n in names],
+                               intro_at=[0] + [int(intro.get(n, 2**62)) for n in names[1:]], n_cells=n_cells,
+                               n_samples=len(samples), sample_size=sizes[0], seed=seed)
+        snames = list(samples)
+        sample = [None if s < 0 else snames[s] for s in fd["sample"].tolist()]
+        return cls(fd["cell_id"], fd["parent"], [names[s] for s in fd["species"]], sample, fd["birth"])
+
+    def get_nodes(self) -> pd.DataFrame:
+        """src/forest.hpp:29-67."""
+        return pd.DataFrame({"cell_id": self.cell_id, "ancestor": np.where(self.ancestor < 0, pd.NA, self.ancestor),
+                             "mutant": self.mutant, "epistate": self.epistate, "sample": self.sample,
+                             "birth_time": self.birth_time})
+
+    def get_samples_info(self) -> pd.DataFrame:
+        counts = pd.Series(self.sample).value_counts()
+        return pd.DataFrame({"name": self.sample_names,
+                             "tumour_cells": [int(counts.get(n, 0)) for n in self.sample_names]})
+
+    def species_index(self) -> np.ndarray:
+        idx = {s: i for i, s in enumerate(self.species)}
+        return np.array([idx[(m, e)] for m, e in zip(self.mutant.tolist(), self.epistate.tolist())], np.uint32)
+
+    def sample_index(self, names: Sequence[str]) -> np.ndarray:
+        idx = {n: i for i, n in enumerate(names)}
+        return np.array([idx.get(s, -1) if s is not None else -1 for s in self.sample.tolist()], np.int32)
+
+
+@dataclass
+class SNV:
+    """A driver SNV specification (src/sid.hpp:28-77); chr_pos is 1-based."""
+    chr: str
+    chr_pos: int
+    alt: str
+    ref: str | None = None
+    allele: int | None = None
+
+
+class ReferenceGenome:
+    """Chromosome names and lengths with a 2-bit reference: synthetic i.i.d.
+    uniform ACGT from a seed, or bases given explicitly."""
+
+    def __init__(self, chromosomes: Mapping[str, int], seed: int = 0, sequences: Mapping[str, str] | None = None):
+        self.names = list(chromosomes)
+        self.lengths = np.array([int(chromosomes[n]) for n in self.names], np.uint32)
+        self.seed = int(seed)
+        self.packed = None
+        if sequences is not None:
+            words = []
+            code = {c: i for i, c in enumerate("ACGT")}
+            for n in self.names:
+                s = sequences[n].upper()
+                b = np.array([code.get(c, 0) for c in s], np.uint32)
+                b = np.concatenate([b, np.zeros((-len(b)) % 16, np.uint32)]).reshape(-1, 16)
+                words.append((b << (2 * np.arange(16, dtype=np.uint32))).sum(axis=1, dtype=np.uint64).astype(np.uint32))
+            self.packed = np.concatenate(words)
+
+    def chr_index(self, name: str) -> int:
+        try:
+            return self.names.index(str(name))
+        except ValueError:
+            raise ValueError(f"Unknown chromosome \"{name}\"") from None
+
+
+class MutationEngine:
+    """The mutation engine (src/mutation_engine.hpp:35-52): species passenger
+    rates, drivers, timed SBS exposures, the infinite-sites flag."""
+
+    def __init__(self, reference: ReferenceGenome, SBS_signatures: Mapping[str, Sequence[float]],
+                 germline: Iterable[tuple[str, int, str, int]] = (), ctx: core.Context | None = None,
+                 indel_max_len: int = 10, indel_len_p: float = 0.5):
+        self._ctx = ctx
+        self.reference = reference
+        self.signature_names = list(SBS_signatures)
+        self.signatures = np.array([np.asarray(SBS_signatures[n], np.float64) for n in self.signature_names])
+        if self.signatures.shape[1:] != (96,):
+            raise ValueError("SBS signatures must have 96 classes")
+        self.mutants: dict[str, dict] = {}
+        self.exposures: list[tuple[float, dict]] = []
+        self.infinite_sites_model = True
+        self.germline = list(germline)
+        self.indel_max_len = indel_max_len
+        self.indel_len_p = indel_len_p
+        self._genome: core.Genome | None = None
+
+    @property
+    def ctx(self) -> core.Context:
+        if self._ctx is None:
+            self._ctx = core.default_context()
+        return self._ctx
+
+    def genome(self) -> core.Genome:
+        if self._genome is None:
+            r = self.reference
+            self._genome = core.Genome(self.ctx, r.lengths, seed=r.seed, packed=r.packed)
+            self._packed = self._genome.download()
+        return self._genome
+
+    def base(self, chr_idx: int, pos0: int) -> int:
+        self.genome()
+        off = int(sum((int(l) + 15) // 16 for l in self.reference.lengths[:chr_idx]))
+        return int((self._packed[off + pos0 // 16] >> (2 * (pos0 % 16))) & 3)
+
+    def add_mutant(self, mutant_name: str, passenger_rates: Mapping, drivers: Sequence[SNV] = ()):
+        """src/mutation_engine.cpp:1055-1083: ``passenger_rates`` is c(SNV=, indel=)
+        or a per-epistate list of them."""
+        rates = dict(passenger_rates)
+        if rates and all(isinstance(v, Mapping) for v in rates.values()):
+            per_epi = {k: dict(v) for k, v in rates.items()}
+        else:
+            per_epi = {"": rates}
+        for r in per_epi.values():
+            for k in r:
+                if k not in ("SNV", "indel", "CNA"):
+                    raise ValueError(f"Unsupported passenger rate \"{k}\"")
+        self.mutants[mutant_name] = {"rates": per_epi, "drivers": list(drivers)}
+
+    def add_exposure(self, time_or_coefficients, coefficients: Mapping[str, float] | None = None):
+        """src/mutation_engine.cpp:734-739: exposure from a simulated time on."""
+        if coefficients is None:
+            time, coefficients = 0.0, time_or_coefficients
+        else:
+            time = float(time_or_coefficients)
+        coef = {k: float(v) for k, v in coefficients.items() if k in self.signature_names}
+        if not coef:
+            raise ValueError("The exposure names no known SBS signature")
+        if abs(sum(coef.values()) - 1.0) > 1e-6:
+            raise ValueError("The SBS exposure coefficients must sum to 1")
+        self.exposures = [e for e in self.exposures if e[0] != time] + [(time, coef)]
+        self.exposures.sort(key=lambda e: e[0])
+
+    def get_species_rates(self) -> pd.DataFrame:
+        rows = []
+        for m, d in self.mutants.items():
+            for epi, r in d["rates"].items():
+                rows.append({"mutant": m, "epistate": epi, "SNV": r.get("SNV", 0.0), "indel": r.get("indel", 0.0)})
+        return pd.DataFrame(rows)
+
+    def _params(self, forest: SamplesForest, n_pre_snv: int, pre_sig: str, n_pre_indel: int) -> dict:
+        snv, ind = [], []
+        for mutant, epi in forest.species:
+            if mutant not in self.mutants:
+                raise ValueError(f"The mutant \"{mutant}\" has no passenger rates in the mutation engine")
+            rates = self.mutants[mutant]["rates"]
+            r = rates.get(epi, rates.get("", {}))
+            snv.append(float(r.get("SNV", 0.0)))
+            ind.append(float(r.get("indel", 0.0)))
+        if n_pre_snv and pre_sig not in self.signature_names:
+            raise ValueError(f"Unknown SBS signature \"{pre_sig}\"")
+        if not self.exposures and (any(snv) or n_pre_snv):
+            raise ValueError("No SBS exposure has been added to the mutation engine")
+        sp_index = {s: i for i, s in enumerate(forest.species)}
+        drivers = []
+        for mutant, d in self.mutants.items():
+            for snv_spec in d["drivers"]:
+                c = self.reference.chr_index(snv_spec.chr)
+                pos0 = int(snv_spec.chr_pos) - 1
+                if snv_spec.ref is not None and BASES[self.base(c, pos0)] != snv_spec.ref:
+                    raise ValueError(f"The reference base of {snv_spec} does not match the reference genome")
+                for (m, e), s in sp_index.items():
+                    if m == mutant:
+                        drivers.append((s, c, pos0, "ACGT".index(snv_spec.alt),
+                                        -1 if snv_spec.allele is None else int(snv_spec.allele)))
+        germ = [(self.reference.chr_index(c), int(p) - 1, "ACGT".index(a), int(al)) for c, p, a, al in self.germline]
+        epochs = [t for t, _ in self.exposures] or [0.0]
+        expo = np.array([[e.get(n, 0.0) for n in self.signature_names] for _, e in self.exposures]) \
+            if self.exposures else np.ones((1, len(self.signature_names))) / len(self.signature_names)
+        return dict(snv_rate=snv, indel_rate=ind, sbs=self.signatures, epoch_start=epochs, exposure=expo,
+                    n_pre_snv=n_pre_snv, n_pre_indel=n_pre_indel,
+                    pre_sbs=self.signature_names.index(pre_sig) if pre_sig in self.signature_names else 0,
+                    indel_max_len=self.indel_max_len, indel_len_p=self.indel_len_p,
+                    infinite_sites=bool(self.infinite_sites_model), ploidy=2, drivers=drivers, germline=germ)
+
+    def place_mutations(self, samples_forest: SamplesForest, num_of_preneoplastic_SNVs: int = 0,
+                        num_of_preneoplastic_indels: int = 0, seed=None,
+                        preneoplastic_SNV_signature_name: str = "SBS1",
+                        preneoplastic_indel_signature_name: str = "ID1") -> "PhylogeneticForest":
+        """src/mutations.cpp:546-573 and src/mutation_engine.cpp:1085-1113."""
+        c_seed = get_random_seed(seed)
+        params = self._params(samples_forest, int(num_of_preneoplastic_SNVs), preneoplastic_SNV_signature_name,
+                              int(num_of_preneoplastic_indels))
+        g = self.genome()
+        f = core.Forest(self.ctx, samples_forest.cell_id, samples_forest.ancestor, samples_forest.species_index(),
+                        samples_forest.sample_index(samples_forest.sample_names), samples_forest.birth_time,
+                        len(samples_forest.sample_names))
+        m = core.Mutations.place(self.ctx, g, f, _seed64(c_seed), **params)
+        return PhylogeneticForest(self, samples_forest, f, m, seed=c_seed)
+
+
+class PhylogeneticForest:
+    """A samples forest labelled with mutations (src/phylogenetic_forest.hpp:34-142):
+    the flattened forest and the mutation table stay device resident."""
+
+    def __init__(self, engine: MutationEngine, samples_forest: SamplesForest, forest: core.Forest,
+                 mutations: core.Mutations, seed: int):
+        self.engine = engine
+        self.samples_forest = samples_forest
+        self.forest = forest
+        self.mutations = mutations
+        self.seed = seed
+        self._recs = None
+        self._flat = None
+
+    def records(self) -> np.ndarray:
+        if self._recs is None:
+            self._recs = self.mutations.download()
+        return self._recs
+
+    def flat(self) -> dict:
+        if self._flat is None:
+            self._flat = self.forest.flat()
+        return self._flat
+
+    def mutation_strings(self, recs: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+        ref, alt = [], []
+        for r in recs:
+            b = BASES[r["ref"]]
+            seq = "".join(BASES[(int(r["seq"]) >> (2 * i)) & 3] for i in range(int(r["len"])))
+            if r["kind"] == A.PSX_SNV:
+                ref.append(b); alt.append(BASES[r["alt"]])
+            elif r["kind"] == A.PSX_DEL:
+                ref.append(b + seq); alt.append(b)
+            else:
+                ref.append(b); alt.append(b + seq)
+        return np.array(ref, dtype=object), np.array(alt, dtype=object)
+
+    def get_sampled_cell_mutations(self, cell_ids: Iterable[int] | None = None) -> pd.DataFrame:
+        """Per-cell mutation table (src/phylogenetic_forest.cpp:452-507): a leaf
+        carries record m iff leaf_lo(m) <= rank(leaf) < leaf_hi(m)."""
+        fl = self.flat()
+        recs = self.records()
+        parent = fl["parent"]
+        n = len(parent)
+        is_leaf = np.ones(n, bool)
+        is_leaf[parent[parent != A.PSX_NO_NODE]] = False
+        leaf_dfs = np.nonzero(is_leaf)[0]
+        leaf_cells = fl["cell_id"][leaf_dfs]
+        want = None if cell_ids is None else set(int(c) for c in cell_ids)
+        ref, alt = self.mutation_strings(recs)
+        chr_names = np.array(self.engine.reference.names, dtype=object)
+        rows = []
+        for rank, cell in enumerate(leaf_cells.tolist()):
+            if want is not None and cell not in want:
+                continue
+            sel = np.nonzero((recs["leaf_lo"] <= rank) & (rank < recs["leaf_hi"]))[0]
+            rows.append(pd.DataFrame({
+                "cell_id": cell, "chr": chr_names[recs["chr"][sel]], "chr_pos": recs["pos"][sel].astype(np.int64) + 1,
+                "allele": recs["allele"][sel], "ref": ref[sel], "alt": alt[sel],
+                "type": np.where(recs["kind"][sel] == A.PSX_SNV, "SNV", "indel"),
+                "cause": [self._cause(c) for c in recs["cause"][sel]],
+                "class": [NATURE[int(x)] for x in recs["nature"][sel]]}))
+        if not rows:
+            return pd.DataFrame(columns=["cell_id", "chr", "chr_pos", "allele", "ref", "alt", "type", "cause", "class"])
+        return pd.concat(rows, ignore_index=True)
+
+    def _cause(self, c) -> str | None:
+        return None if int(c) == A.PSX_NO_CAUSE else self.engine.signature_names[int(c)]
+
+    def get_first_occurrences(self, chr: str, chr_pos: int, ref: str, alt: str) -> list[int]:
+        """Cells where the mutation arises (the node of each record of the SID)."""
+        recs = self.records()
+        c = self.engine.reference.chr_index(chr)
+        r_ref, r_alt = self.mutation_strings(recs)
+        sel = np.nonzero((recs["chr"] == c) & (recs["pos"] == int(chr_pos) - 1) & (r_ref == ref) & (r_alt == alt))[0]
+        fl = self.flat()
+        return [int(fl["cell_id"][recs["node"][k]]) if recs["node"][k] != A.PSX_NO_NODE else None for k in sel]
+
+    def get_samples_info(self) -> pd.DataFrame:
+        return self.samples_forest.get_samples_info()
+
+
+# --------------------------------------------------------------- sequencers
+
+class ErrorlessIlluminaSequencer:
+    """src/sequencers.hpp:25-38."""
+    error_rate = 0.0
+
+    def show(self) -> str:
+        return "Error-less Illumina sequencer"
+
+
+class BasicIlluminaSequencer:
+    """src/sequencers.hpp:40-72; validation src/sequencers.cpp:80-106."""
+
+    def __init__(self, error_rate, random_quality_scores: bool = True):
+        if isinstance(error_rate, bool) or not isinstance(error_rate, (int, float, np.integer, np.floating)):
+            raise ValueError("The parameter \"error_rate\" must be a positive real number.")
+        if error_rate < 0:
+            raise ValueError("The parameter \"error_rate\" must be a positive real number.")
+        if not isinstance(random_quality_scores, (bool, np.bool_)):
+            raise ValueError("The parameter \"random_quality_scores\" must be a Boolean value.")
+        self.error_rate = float(error_rate)
+        self.random_quality_scores = bool(random_quality_scores)
+
+    def show(self) -> str:
+        kind = "random quality scores" if self.random_quality_scores else "constant quality scores"
+        return f"Basic Illumina sequencer (error rate: {self.error_rate} {kind})"
+
+
+def _sequencer_data(sequencer) -> dict | None:
+    if sequencer is None:
+        return None
+    if isinstance(sequencer, BasicIlluminaSequencer):
+        return {"name": "BasicIlluminaSequencer", "error_rate": sequencer.error_rate,
+                "random_quality_scores": sequencer.random_quality_scores}
+    if isinstance(sequencer, ErrorlessIlluminaSequencer):
+        return {"name": "ErrorlessIlluminaSequencer", "error_rate": 0.0}
+    raise ValueError("Unsupported sequencer type")
+
+
+# ---------------------------------------------------------------- sequencing
+
+@dataclass
+class SampledCell:
+    """The labelling function's argument (src/sampled_cell.hpp)."""
+    cell_id: int
+    mutant: str
+    epistate: str
+    species: str
+    birth_time: float
+
+
+def _relevant_chromosomes(reference: ReferenceGenome, chromosomes) -> list[int]:
+    """get_relevant_chr_set (src/seq_simulation.cpp:299-352)."""
+    if chromosomes is None:
+        return list(range(len(reference.names)))
+    if isinstance(chromosomes, str):
+        chromosomes = [chromosomes]
+    out = set()
+    for i, c in enumerate(chromosomes):
+        if not isinstance(c, str):
+            raise ValueError(f"Expected a list of string: the element {i + 1} of the list is not a string.")
+        out.add(reference.chr_index(c))
+    return sorted(out)
+
+
+def _labelled_forest(pf: PhylogeneticForest, labelling: Callable) -> tuple[core.Forest, list[str]]:
+    """apply_FACS_labels (src/seq_simulation.cpp:183-243): each sample splits
+    by the label of its cells into "<sample>_<label>" ("" keeps the name)."""
+    sf = pf.samples_forest
+    new_sample = np.full(len(sf.cell_id), -1, np.int32)
+    names: list[str] = []
+    index: dict[str, int] = {}
+    for s in sf.sample_names:
+        for i in np.nonzero(sf.sample == s)[0]:
+            cell = SampledCell(int(sf.cell_id[i]), str(sf.mutant[i]), str(sf.epistate[i]),
+                               f"{sf.mutant[i]}{sf.epistate[i]}", float(sf.birth_time[i]))
+            label = str(labelling(cell))
+            name = s if label == "" else f"{s}_{label}"
+            if name not in index:
+                index[name] = len(names)
+                names.append(name)
+            new_sample[i] = index[name]
+    f = core.Forest(pf.engine.ctx, sf.cell_id, sf.ancestor, sf.species_index(), new_sample, sf.birth_time, len(names))
+    return f, names
+
+
+def _table(pf: PhylogeneticForest, sample_names: list[str], alt, depth, present,
+           include_non_sequenced_mutations: bool) -> pd.DataFrame:
+    """get_result_dataframe (src/seq_simulation.cpp:52-181) from psx_seq_table rows."""
+    recs = pf.records()
+    rows, occ, cov, vaf = core.seq_table(recs, alt, depth, present, include_non_sequenced_mutations)
+    rows = rows.astype(np.int64)
+    sel = recs[rows]
+    ref, alt_s = pf.mutation_strings(sel)
+    causes, classes = [], []
+    heads = np.nonzero(recs["sid_head"])[0]
+    ends = np.append(heads[1:], len(recs))
+    end_of = dict(zip(heads.tolist(), ends.tolist()))
+    for h in rows.tolist():
+        grp = recs[h: end_of[h]]
+        cs = sorted({pf._cause(c) for c in grp["cause"]} - {None})
+        causes.append(";".join(cs) if cs else None)
+        classes.append(";".join(sorted({NATURE[int(x)] for x in grp["nature"]})))
+    df = pd.DataFrame({"chr": np.array(pf.engine.reference.names, dtype=object)[sel["chr"]],
+                       "chr_pos": sel["pos"].astype(np.int64) + 1, "ref": ref, "alt": alt_s,
+                       "causes": causes, "classes": classes})
+    for s, name in enumerate(sample_names):
+        df[f"{name}.occurrences"] = occ[s]
+        df[f"{name}.coverage"] = cov[s]
+        df[f"{name}.VAF"] = vaf[s]
+    return df
+
+
+def _run(pf: PhylogeneticForest, forest: core.Forest, sample_names: list[str], sequencer, chromosomes, coverage,
+         read_size, insert_size_mean, insert_size_stddev, purity, with_normal_sample, preneoplastic_in_normal,
+         include_non_sequenced_mutations, c_seed, write_SAM, output_dir, filename_prefix, template_name_prefix,
+         samples=None):
+    eng = pf.engine
+    if not (0.0 <= float(purity) <= 1.0):
+        raise ValueError("The purity must be in [0,1]")
+    if insert_size_mean:
+        from ._lib import check, lib
+        check(lib().psx_insert_size_check(eng.ctx.h, int(insert_size_mean), int(insert_size_stddev)), eng.ctx.h)
+    chrs = _relevant_chromosomes(eng.reference, chromosomes)
+    seq = dict(coverage=float(coverage), read_size=int(read_size), insert_size_mean=int(insert_size_mean),
+               insert_size_stddev=int(insert_size_stddev),
+               error_rate=0.0 if sequencer is None else float(sequencer.error_rate), purity=float(purity),
+               with_normal_sample=bool(with_normal_sample), preneoplastic_in_normal=bool(preneoplastic_in_normal),
+               chromosomes=chrs, samples=samples)
+    alt, depth, present, n_reads = core.simulate_seq_arrays(eng.ctx, eng.genome(), forest, pf.mutations,
+                                                            _seed64(c_seed), **seq)
+    names = list(sample_names) + (["normal_sample"] if with_normal_sample else [])
+    if write_SAM:
+        from .sam import write_sam
+        write_sam(pf, forest, names, seq, _seed64(c_seed), output_dir, filename_prefix, template_name_prefix)
+    return _table(pf, names, alt, depth, present, include_non_sequenced_mutations), n_reads
+
+
+def simulate_seq(phylo_forest: PhylogeneticForest, sequencer=None, reference_genome=None, chromosomes=None,
+                 coverage=10, read_size=150, insert_size_mean=0, insert_size_stddev=10, output_dir="ProCESS_SAM",
+                 write_SAM=False, update_SAM=False, cell_labelling: Callable | None = None, purity=1,
+                 with_normal_sample=True, preneoplastic_in_normal=False, filename_prefix="chr_",
+                 template_name_prefix="r", include_non_sequenced_mutations=False, seed=None) -> dict:
+    """src/sequencing.cpp:193-209 and src/seq_simulation.cpp:517-601."""
+    if reference_genome is not None:
+        raise ValueError("Only the mutation engine's reference genome is resident on the device")
+    c_seed = get_random_seed(seed)
+    pf = phylo_forest
+    if cell_labelling is not None:
+        if not callable(cell_labelling):
+            raise ValueError("The FACs_labelling_function must be a function.")
+        forest, names = _labelled_forest(pf, cell_labelling)
+    else:
+        forest, names = pf.forest, list(pf.samples_forest.sample_names)
+    if write_SAM and not update_SAM and os.path.isdir(output_dir) and os.listdir(output_dir):
+        raise ValueError(f"The output directory \"{output_dir}\" already exists")
+    df, _ = _run(pf, forest, names, sequencer, chromosomes, coverage, read_size, insert_size_mean,
+                 insert_size_stddev, purity, with_normal_sample, preneoplastic_in_normal,
+                 include_non_sequenced_mutations, c_seed, write_SAM, output_dir, filename_prefix,
+                 template_name_prefix)
+    parameters = {"sequencer": _sequencer_data(sequencer), "reference_genome": reference_genome,
+                  "chromosomes": chromosomes, "coverage": coverage, "read_size": read_size,
+                  "insert_size_mean": insert_size_mean, "insert_size_stddev": insert_size_stddev,
+                  "output_dir": output_dir, "write_SAM": write_SAM, "update_SAM": update_SAM,
+                  "cell_labelling": cell_labelling, "purity": purity, "with_normal_sample": with_normal_sample,
+                  "filename_prefix": filename_prefix, "template_name_prefix": template_name_prefix,
+                  "include_non_sequenced_mutations": include_non_sequenced_mutations, "seed": c_seed}
+    return {"mutations": df, "parameters": parameters}
+
+
+def simulate_normal_seq(phylo_forest: PhylogeneticForest, sequencer=None, reference_genome=None, chromosomes=None,
+                        coverage=10, read_size=150, insert_size_mean=0, insert_size_stddev=10,
+                        output_dir="ProCESS_normal_SAM", write_SAM=True, update_SAM=False, with_preneoplastic=False,
+                        filename_prefix="chr_", template_name_prefix="r", include_non_sequenced_mutations=False,
+                        seed=None) -> dict:
+    """src/sequencing.cpp:268-282 and src/seq_simulation.cpp:603-679: the
+    normal sample alone, purity 1."""
+    if reference_genome is not None:
+        raise ValueError("Only the mutation engine's reference genome is resident on the device")
+    c_seed = get_random_seed(seed)
+    pf = phylo_forest
+    df, _ = _run(pf, pf.forest, [], sequencer, chromosomes, coverage, read_size, insert_size_mean,
+                 insert_size_stddev, 1.0, True, with_preneoplastic, include_non_sequenced_mutations, c_seed,
+                 write_SAM, output_dir, filename_prefix, template_name_prefix, samples=[])
+    parameters = {"sequencer": _sequencer_data(sequencer), "reference_genome": reference_genome,
+                  "chromosomes": chromosomes, "coverage": coverage, "read_size": read_size,
+                  "insert_size_mean": insert_size_mean, "insert_size_stddev": insert_size_stddev,
+                  "output_dir": output_dir, "write_SAM": write_SAM, "update_SAM": update_SAM,
+                  "with_preneoplastic": with_preneoplastic, "filename_prefix": filename_prefix,
+                  "template_name_prefix": template_name_prefix,
+                  "include_non_sequenced_mutations": include_non_sequenced_mutations, "seed": c_seed}
+    return {"mutations": df, "parameters": parameters}
+
+
+def seq_to_long(seq_results) -> pd.DataFrame:
+    """R/seq_to_long.R:31-67: wide per-sample columns to rows of
+    chr, from, to, ref, alt, causes, classes, NV, DP, VAF, sample_name."""
+    df = seq_results["mutations"] if isinstance(seq_results, dict) else seq_results
+    names = [c[: -len(".VAF")] for c in df.columns if c.endswith(".VAF")]
+    parts = []
+    for sn in names:
+        part = df[["chr", "chr_pos", "ref", "alt", "causes", "classes", f"{sn}.occurrences", f"{sn}.coverage",
+                   f"{sn}.VAF"]].copy()
+        part.columns = ["chr", "chr_pos", "ref", "alt", "causes", "classes", "NV", "DP", "VAF"]
+        part["sample_name"] = sn
+        parts.append(part)
+    out = pd.concat(parts, ignore_index=True) if parts else pd.DataFrame()
+    out = out.rename(columns={"chr_pos": "from"})
+    out["to"] = out["from"]
+    return out

@@ -36,7 +36,7 @@ struct job_in {
 };
 
 uint32_t out_samples(const psx_forest* f, const psx_seq_params* p) {
-    return (p->n_samples ? p->n_samples : f->n_samples) + (p->with_normal_sample ? 1u : 0u);
+    return (p->samples ? p->n_samples : f->n_samples) + (p->with_normal_sample ? 1u : 0u);
 }
 
 // Validates one job and appends its descriptors; the ordering of output
@@ -75,10 +75,10 @@ void prepare_job(psx_ctx* ctx, prepared& P, const job_in& in, uint32_t spi) {
         if (p->chromosomes[k] >= g->n_chr) psx_throw(PSX_EINVAL, "unknown chromosome %u", p->chromosomes[k]);
         sel[p->chromosomes[k]] = 1;
     }
-    const uint32_t n_req = p->n_samples ? p->n_samples : f->n_samples;
+    const uint32_t n_req = p->samples ? p->n_samples : f->n_samples;
     std::vector<uint32_t> req(n_req);
     for (uint32_t k = 0; k < n_req; ++k) {
-        req[k] = p->n_samples ? p->samples[k] : k;
+        req[k] = p->samples ? p->samples[k] : k;
         if (req[k] >= f->n_samples) psx_throw(PSX_EINVAL, "unknown sample %u", req[k]);
         if (f->sample_begin[req[k] + 1] == f->sample_begin[req[k]])
             psx_throw(PSX_EINVAL, "sample %u has no cells", req[k]);
