@@ -1,0 +1,303 @@
+#!/usr/bin/env python3
+"""Writes the sequencing fixtures tests/golden/seq_<case>.npz.
+
+Inputs (reference, forest, mutation table) come from the library's host
+forest synthesis and the oracle's reference synthesis and placement; the
+expected alt/depth/present tables come from the brute-force model below, a
+pure-Python restatement that materialises every read as a base string and
+counts afterwards (the way the reference's ReadSimulator builds reads,
+src/seq_simulation.cpp:354-429), independently of the oracle's windowed walk
+and of the kernels. Its Philox4x32-10 is checked against the Random123
+known-answer vectors before use, and the oracle is checked against it here
+and again by tests/test_golden.py.
+
+Run from the repository root: python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import json
+import math
+import sys
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+ROOT = HERE.parent.parent
+sys.path.insert(0, str(ROOT))
+
+from oracle import oracle as O  # noqa: E402
+from process_amd import _abi as A  # noqa: E402
+from process_amd import core  # noqa: E402
+
+M32 = 0xFFFFFFFF
+STRATUM = 8192
+MAX_ERR = 16
+TEMPLATE, ERROR = 0x10, 0x11
+
+
+# ------------------------------------------------------------------ Philox
+
+def philox(key: int, c0: int, c1: int, c2: int, c3: int) -> tuple[int, int, int, int]:
+    """Philox4x32-10 (Salmon et al., SC'11)."""
+    k0, k1 = key & M32, (key >> 32) & M32
+    for _ in range(10):
+        p0 = 0xD2511F53 * c0
+        p1 = 0xCD9E8D57 * c2
+        c0, c1, c2, c3 = ((p1 >> 32) ^ c1 ^ k0) & M32, p1 & M32, ((p0 >> 32) ^ c3 ^ k1) & M32, p0 & M32
+        k0 = (k0 + 0x9E3779B9) & M32
+        k1 = (k1 + 0xBB67AE85) & M32
+    return c0, c1, c2, c3
+
+
+def check_philox() -> None:
+    kat = json.loads((HERE / "philox4x32_10_kat.json").read_text())
+    for v in kat["vectors"]:
+        key = v["key"][0] | (v["key"][1] << 32)
+        assert list(philox(key, *v["ctr"])) == v["out"], v
+
+
+def mulhi(u: int, n: int) -> int:
+    return (u * n) >> 32
+
+
+# ------------------------------------------------------- inversion tables
+
+def cdf_table(pmf: list[float]) -> list[int]:
+    """T[k] = floor(cdf_k 2^32), last entry 2^32."""
+    total = sum(pmf)
+    T, cum = [], 0.0
+    for k in range(len(pmf) - 1):
+        cum += pmf[k]
+        x = cum / total
+        T.append(1 << 32 if x >= 1.0 else int(x * 4294967296.0))
+    T.append(1 << 32)
+    return T
+
+
+def binomial_range(t: int, p: float, lo: float, hi: float) -> tuple[int, list[int]]:
+    lp, lq, lgt = math.log(p), math.log1p(-p), math.lgamma(t + 1.0)
+    pmf = []
+    for i in range(int(hi - lo + 1.0)):
+        k = lo + i
+        pmf.append(math.exp(lgt - math.lgamma(k + 1.0) - math.lgamma(t - k + 1.0) + k * lp + (t - k) * lq))
+    return int(lo), cdf_table(pmf)
+
+
+def binomial_table(t: int, p: float) -> tuple[int, list[int]]:
+    m, sd = t * p, math.sqrt(t * p * (1.0 - p))
+    lo, hi = math.floor(m - 40.0 * sd - 10.0), math.ceil(m + 40.0 * sd + 10.0)
+    return binomial_range(t, p, max(lo, 0.0), min(hi, float(t)))
+
+
+def insert_table(mean: int, sd: int) -> tuple[int, list[int]]:
+    """get_bin_dist, src/seq_simulation.cpp:431-451."""
+    p = 1.0 - (sd * sd) / mean
+    return binomial_table(int(mean / p), p)
+
+
+def error_table(L: int, e: float) -> list[int]:
+    return binomial_range(L, e, 0.0, float(min(L, MAX_ERR)))[1]
+
+
+def pick(T: list[int], u: int) -> int:
+    k = 0
+    while k + 1 < len(T) and u >= T[k]:
+        k += 1
+    return k
+
+
+# ------------------------------------------------------- brute-force model
+
+def base_at(packed, woff, c, i) -> int:
+    return int(packed[woff[c] + i // 16] >> (2 * (i % 16))) & 3
+
+
+def brute_force_seq(lens, packed, forest, recs, seed, ploidy=2, **p):
+    L = p["read_size"]
+    pe = p.get("insert_size_mean", 0) > 0
+    e = p.get("error_rate", 0.0)
+    purity = p.get("purity", 1.0)
+    preneo = p.get("preneoplastic_in_normal", False)
+    chroms = p.get("chromosomes")
+    samples = p.get("samples")
+    req = list(range(forest.n_samples)) if samples is None else list(samples)
+    n_out = len(req) + (1 if p.get("with_normal_sample", True) else 0)
+    n_leaves = forest.n_leaves
+    M = len(recs)
+    woff = np.concatenate([[0], np.cumsum((np.asarray(lens, np.int64) + 15) // 16)])
+    ins_k0, ins_T = insert_table(p["insert_size_mean"], p.get("insert_size_stddev", 10)) if pe else (0, [])
+    err_T = error_table(L, e) if e > 0 else []
+    thr_normal = int((1.0 - purity) * 4294967296.0)
+    head = recs["sid_head"].astype(bool)
+    by_pos: dict[tuple[int, int], list[int]] = {}
+    for q in range(M):
+        by_pos.setdefault((int(recs["chr"][q]), int(recs["pos"][q])), []).append(q)
+
+    def normal_has(q):
+        nat = int(recs["nature"][q])
+        return nat == A.PSX_GERMINAL or (nat == A.PSX_PRENEOPLASTIC and preneo)
+
+    def carried(q, leaf, allele):
+        if int(recs["allele"][q]) != allele:
+            return False
+        if leaf == n_leaves:
+            return normal_has(q)
+        return int(recs["leaf_lo"][q]) <= leaf < int(recs["leaf_hi"][q])
+
+    alt = np.zeros((n_out, M), np.int32)
+    dep = np.zeros((n_out, M), np.int32)
+    pres = np.zeros((n_out, M), np.uint8)
+    for so in range(n_out):
+        normal = so == len(req)
+        leaves = [] if normal else \
+            [int(v) for v in forest.sample_leaves[forest.sample_begin[req[so]]:forest.sample_begin[req[so] + 1]]]
+        for q in range(M):
+            has = normal_has(q) if (normal or purity < 1.0) else False
+            if not normal and not has:
+                has = any(int(recs["leaf_lo"][q]) <= v < int(recs["leaf_hi"][q]) for v in leaves)
+            pres[so, q] = has
+
+    def errors(r, j, c2, mate):
+        base = (ERROR << 24) | (mate << 16)
+        w = philox(seed, r, j, c2, base)
+        K = pick(err_T, w[0])
+        out: list[tuple[int, int]] = []
+        t = 1
+        while len(out) < K and t < 1 + 4 * MAX_ERR:
+            if t % 4 == 0:
+                w = philox(seed, r, j, c2, base | (t >> 2))
+            word = w[t % 4]
+            o = mulhi(word, L)
+            if all(o != oo for oo, _ in out):
+                out.append((o, ((word & 0xFFFF) * 3) >> 16))
+            t += 1
+        return out
+
+    def read(so, c, x, leaf, allele, r, j, c2, mate):
+        # the source genome's bases over [x, x + L): reference, then the
+        # first carried SNV at each position
+        true = [base_at(packed, woff, c, x + i) for i in range(L)]
+        for i in range(L):
+            for q in by_pos.get((c, x + i), []):
+                if int(recs["kind"][q]) == A.PSX_SNV and carried(q, leaf, allele):
+                    true[i] = int(recs["alt"][q])
+                    break
+        obs = list(true)
+        if e > 0:
+            for o, sub in errors(r, j, c2, mate):
+                obs[o] = sub if sub < true[o] else sub + 1
+        for i in range(L):
+            qs = by_pos.get((c, x + i), [])
+            for k, q in enumerate(qs):
+                dep[so, q] += 1
+                if not head[q]:
+                    continue
+                if int(recs["kind"][q]) == A.PSX_SNV:
+                    alt[so, q] += obs[i] == int(recs["alt"][q])
+                else:
+                    group = [q]
+                    for q2 in qs[k + 1:]:
+                        if head[q2]:
+                            break
+                        group.append(q2)
+                    alt[so, q] += any(carried(g, leaf, allele) for g in group)
+
+    n_reads = 0
+    for so in range(n_out):
+        normal = so == len(req)
+        leaves = [] if normal else \
+            [int(v) for v in forest.sample_leaves[forest.sample_begin[req[so]]:forest.sample_begin[req[so] + 1]]]
+        for c, ln in enumerate(int(v) for v in lens):
+            if (chroms is not None and c not in chroms) or ln < L:
+                continue
+            W = ln - L + 1
+            T = int(math.floor(p["coverage"] * ln / (2 * L if pe else L) + 0.5))
+            n_reads += T * (2 if pe else 1)
+            c2 = (c << 16) | so
+            for j in range((W + STRATUM - 1) // STRATUM):
+                b, e_ = j * STRATUM, min(j * STRATUM + STRATUM, W)
+                for r in range(T * e_ // W - T * b // W):
+                    w = philox(seed, r, j, c2, TEMPLATE << 24)
+                    x = b + mulhi(w[0], e_ - b)
+                    if normal or w[1] < thr_normal:
+                        leaf, allele = n_leaves, mulhi(w[2], ploidy)
+                    else:
+                        X = mulhi(w[2], len(leaves) * ploidy)
+                        leaf, allele = leaves[X // ploidy], X % ploidy
+                    read(so, c, x, leaf, allele, r, j, c2, 0)
+                    if pe:
+                        I = max(ins_k0 + pick(ins_T, w[3]), L)
+                        read(so, c, min(x + I - L, ln - L), leaf, allele, r, j, c2, 1)
+    return alt, dep, pres, n_reads
+
+
+# ------------------------------------------------------------------ cases
+
+def signatures(n, seed):
+    return np.random.default_rng(seed).dirichlet(np.ones(96), size=n)
+
+
+CASES = {
+    # finite sites (several records per position), paired ends, high error
+    # rate, purity < 1 with pre-neoplastic mutations in the normal sample,
+    # two chromosomes, the first spanning three strata
+    "pe_finite_sites": dict(
+        lens=[20_000, 9_000], genome_seed=5, forest=dict(n_cells=70, n_samples=2, sample_size=15, seed=11),
+        engine=dict(snv=6e-5, indel=8e-6, infinite_sites=False, pre_snv=5, pre_indel=2), place_seed=99,
+        seq_seed=2024, seq=dict(coverage=4, read_size=150, insert_size_mean=350, insert_size_stddev=10,
+                                error_rate=0.02, purity=0.7, preneoplastic_in_normal=True)),
+    # a short chromosome crowded with mutations under finite sites: several
+    # SIDs per position and SIDs held by several records
+    "dense_finite_sites": dict(
+        lens=[2_400], genome_seed=13, forest=dict(n_cells=80, n_samples=2, sample_size=20, seed=21),
+        engine=dict(snv=1.5e-3, indel=2e-4, infinite_sites=False, pre_snv=4, pre_indel=2), place_seed=5,
+        seq_seed=77, seq=dict(coverage=12, read_size=100, insert_size_mean=250, insert_size_stddev=9,
+                              error_rate=0.01, purity=0.8)),
+    # single-end short reads, one requested sample and one chromosome, infinite sites
+    "se_subset": dict(
+        lens=[12_000, 8_500], genome_seed=8, forest=dict(n_cells=60, n_samples=3, sample_size=10, seed=4),
+        engine=dict(snv=4e-5, indel=4e-6, infinite_sites=True, pre_snv=3, pre_indel=1), place_seed=7,
+        seq_seed=31, seq=dict(coverage=5, read_size=37, error_rate=0.01, samples=[1], chromosomes=[0])),
+}
+
+
+def engine_params(packed, lens, snv, indel, infinite_sites, pre_snv, pre_indel):
+    sig = signatures(2, 3)
+    expo = np.random.default_rng(4).dirichlet(np.ones(2), size=2)
+    woff = np.concatenate([[0], np.cumsum((np.asarray(lens, np.int64) + 15) // 16)])
+    gpos = 211
+    galt = (base_at(packed, woff, 0, gpos) + 1) % 4
+    return dict(snv_rate=[snv, 2 * snv], indel_rate=[indel, indel], sbs=sig, epoch_start=[0.0, 4.0], exposure=expo,
+                n_pre_snv=pre_snv, n_pre_indel=pre_indel, pre_sbs=1, indel_max_len=10, indel_len_p=0.5,
+                infinite_sites=infinite_sites, ploidy=2, drivers=[], germline=[(0, gpos, galt, 1)])
+
+
+def make(name, c):
+    lens = np.asarray(c["lens"], np.uint32)
+    packed = O.genome_synth(lens, c["genome_seed"])
+    fk = c["forest"]
+    fd = core.synth_forest(growth=[0.2, 0.3], death=[0.01, 0.01], intro_at=[0, fk["n_cells"] // 4],
+                           n_cells=fk["n_cells"], n_samples=fk["n_samples"], sample_size=fk["sample_size"],
+                           seed=fk["seed"])
+    of = O.Forest(fd["cell_id"], fd["parent"], fd["species"], fd["sample"], fd["birth"], fd["n_samples"])
+    eng = engine_params(packed, lens, **c["engine"])
+    recs = O.place_mutations(lens, packed, of, c["place_seed"], **eng)
+    alt, dep, pres, n = brute_force_seq(lens, packed, of, recs, c["seq_seed"], **c["seq"])
+    oa, od, op, on = O.simulate_seq(lens, of, recs, c["seq_seed"], **c["seq"])
+    agree = (np.array_equal(alt, oa) and np.array_equal(dep, od) and np.array_equal(pres, op) and n == on)
+    out = HERE / f"seq_{name}.npz"
+    np.savez_compressed(out, lens=lens, packed=packed, cell_id=fd["cell_id"], parent=fd["parent"],
+                        species=fd["species"], sample=fd["sample"], birth=fd["birth"],
+                        n_samples=np.uint32(fd["n_samples"]), recs=recs.view(np.uint8),
+                        seq=np.array(json.dumps(c["seq"])), seed=np.uint64(c["seq_seed"]),
+                        alt=alt, depth=dep, present=pres, n_reads=np.uint64(n))
+    print(f"{out.name}: {len(recs)} records, {int(np.sum(dep))} depth counts, {n} reads, "
+          f"{int((recs['sid_head'] == 0).sum())} non-head records; oracle agrees: {agree}")
+    return agree
+
+
+if __name__ == "__main__":
+    check_philox()
+    ok = all(make(k, v) for k, v in CASES.items())
+    sys.exit(0 if ok else 1)
