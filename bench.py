@@ -34,29 +34,162 @@ def log(msg: str) -> None:
 
 def cpu_baseline(w, built, recs, threads: int, target_reads: float):
     """The oracle (CPU restatement) on a bounded sample of the same workload:
-    sample 0 without the normal sample, the first chromosomes up to
-    target_reads reads."""
+    every output sample over the first chromosomes, up to target_reads reads."""
     from oracle import oracle as O
 
     fd = w.forest
     of = O.Forest(fd["cell_id"], fd["parent"], fd["species"], fd["sample"], fd["birth"], fd["n_samples"])
     seq = dict(w.seq)
     L = seq["read_size"]
+    S = fd["n_samples"] + (1 if seq.get("with_normal_sample", True) else 0)
     chrs, reads = [], 0.0
     for c, ln in enumerate(w.lengths):
         chrs.append(c)
-        reads += seq["coverage"] * int(ln) / L
+        reads += S * seq["coverage"] * int(ln) / L
         if reads >= target_reads:
             break
-    seq.update(samples=[0], with_normal_sample=False, chromosomes=chrs)
+    seq.update(chromosomes=chrs)
     used = O.set_threads(threads)
     t0 = time.perf_counter()
     _, _, _, n = O.simulate_seq(w.lengths, of, recs, w.seq_seed, **seq)
     dt = time.perf_counter() - t0
     O.set_threads(0)
     return dict(value=n / dt, unit="reads/s", cores=used, kind="port",
-                sample=f"oracle count-only simulate_seq, sample 0 only, chromosomes 0..{chrs[-1]}, "
-                       f"{n} reads in {dt:.2f} s")
+                sample=f"oracle count-only simulate_seq, all {S} output samples, chromosomes 0..{chrs[-1]} of "
+                       f"{len(w.lengths)}, {n} reads in {dt:.2f} s")
+
+
+def batched_leg(ctx, n_jobs: int, reps: int, tune: bool) -> dict:
+    """The batched workload: every job in one persistent hipGraph launch
+    against the same jobs launched one by one (each call waits for its
+    results); then the autotuner's configuration."""
+    from process_amd import _abi as A
+    from process_amd import core, workloads
+
+    t = time.perf_counter()
+    B = workloads.build_batched(ctx, n_jobs=n_jobs)
+    leaves = [s["leaves"] for s in B.specs]
+    log(f"batched: {n_jobs} jobs built in {time.perf_counter() - t:.1f} s, {B.reads:.3e} reads, "
+        f"leaves {min(leaves)}..{max(leaves)}, placement {B.place_ms:.1f} ms")
+
+    def timed_batch(cfg=None):
+        core.simulate_seq_batch(ctx, B.jobs, cfg)  # warm-up
+        walls, kms = [], []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            n = core.simulate_seq_batch(ctx, B.jobs, cfg)
+            walls.append(time.perf_counter() - t0)
+            kms.append(ctx.last_kernel_ms())
+        return n, float(np.median(walls)), float(np.median(kms))
+
+    n, wall, kms = timed_batch()
+    check = [k for k in range(0, n_jobs, max(1, n_jobs // 8))]
+    ref = {k: B.jobs[k].results() for k in check}
+    # one job at a time: each call builds descriptors, launches and synchronises
+    lat = []
+    t0 = time.perf_counter()
+    for j in B.jobs:
+        t1 = time.perf_counter()
+        core.simulate_seq_device(ctx, j.genome, j.forest, j.muts, j.seed, j.alt, j.depth, j.present, **j.seq)
+        lat.append(time.perf_counter() - t1)
+    seq_wall = time.perf_counter() - t0
+    same = all(all(np.array_equal(a, b) for a, b in zip(ref[k], B.jobs[k].results())) for k in check)
+    out = dict(workload=f"{n_jobs} jobs, leaves log-uniform 1e3-1e5, coverage log-uniform 30-300x, shared 50 Mb "
+                        "reference, PE 150 bp, error 1e-3, tumour + normal sample each",
+               reads=n, batched=dict(value=n / wall, unit="reads/s", wall_ms=wall * 1e3, graph_ms=kms,
+                                     config="default"),
+               sequential=dict(value=n / seq_wall, unit="reads/s", wall_ms=seq_wall * 1e3,
+                               job_p50_ms=float(np.percentile(lat, 50) * 1e3),
+                               job_p99_ms=float(np.percentile(lat, 99) * 1e3)),
+               batched_over_sequential=seq_wall / wall, results_identical=bool(same))
+    log(f"batched: {n / wall:.3e} reads/s ({wall * 1e3:.1f} ms), one by one {n / seq_wall:.3e} reads/s "
+        f"({seq_wall * 1e3:.1f} ms), identical {same}")
+    if tune:
+        t0 = time.perf_counter()
+        cfg, best_ms = core.autotune(ctx, B.jobs)
+        tune_s = time.perf_counter() - t0
+        cfg.use_graph = 1
+        n, wall, kms = timed_batch(cfg)
+        out["tuned"] = dict(value=n / wall, unit="reads/s", wall_ms=wall * 1e3, graph_ms=kms, tune_s=tune_s,
+                            config=dict(block_threads=cfg.block_threads, lds_records=cfg.lds_records,
+                                        strata_per_item=cfg.strata_per_item, blocks_per_cu=cfg.blocks_per_cu))
+        log(f"batched, tuned {out['tuned']['config']}: {n / wall:.3e} reads/s (tuning {tune_s:.1f} s)")
+    for j in B.jobs:
+        for a in (j.alt, j.depth, j.present):
+            a.close()
+    return out
+
+
+def serving_leg(ctx, rate: float, duration: float, saturated: int) -> dict:
+    """The serving workload: Poisson arrivals of requests over device-resident
+    forests (1e3-1e6 cells) through the continuous-batching server, against
+    the same stream served one request at a time; then the same number of
+    requests submitted at once (saturated) in both forms."""
+    import threading
+
+    from process_amd import core, workloads
+
+    f0, _ = ctx.mem_info()
+    t = time.perf_counter()
+    pool = workloads.build_serving_pool(ctx)
+    f1, _ = ctx.mem_info()
+    log(f"serving: {len(pool.forests)} forests ({pool.cells[0]}..{pool.cells[-1]} cells) resident in "
+        f"{time.perf_counter() - t:.1f} s, {(f0 - f1) / 2**20:.0f} MiB")
+    budget = 2 << 30
+
+    def run(stream, lanes, max_batch, paced, label):
+        srv = core.Server(ctx, budget, max_inflight=lanes, max_batch=max_batch)
+        tickets = []
+        t0 = time.perf_counter()
+
+        def producer():
+            for ta, k, seed, seq in stream:
+                if paced:
+                    dt = t0 + ta - time.perf_counter()
+                    if dt > 0:
+                        time.sleep(dt)
+                tickets.append(srv.submit(pool.genome, pool.forests[k], pool.muts[k], seed, **seq))
+
+        th = threading.Thread(target=producer)
+        th.start()
+        lat, queued, batch, reads = [], [], [], 0
+        last = time.perf_counter()
+        for i in range(len(stream)):
+            while len(tickets) <= i:
+                time.sleep(0.0005)
+            _, _, _, st = srv.wait(tickets[i])
+            lat.append(st["latency_ms"])
+            queued.append(st["queued_ms"])
+            batch.append(st["batch_size"])
+            reads += st["n_reads"]
+            if time.perf_counter() - last > 10:
+                last = time.perf_counter()
+                log(f"serving ({label}): {i + 1}/{len(stream)} requests done")
+        th.join()
+        wall = time.perf_counter() - t0
+        tot = srv.totals()
+        srv.close()
+        r = dict(requests=len(stream), reads=reads, value=reads / wall, unit="reads/s", wall_s=wall,
+                 p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)),
+                 queued_p99_ms=float(np.percentile(queued, 99)), mean_batch=float(np.mean(batch)),
+                 launches=tot["launches"], peak_output_bytes=tot["peak_bytes"])
+        log(f"serving ({label}): {r['value']:.3e} reads/s, p50 {r['p50_ms']:.1f} ms, p99 {r['p99_ms']:.1f} ms, "
+            f"mean batch {r['mean_batch']:.1f}")
+        return r
+
+    stream = workloads.serving_requests(pool, int(rate * duration), rate)
+    out = dict(workload=f"Poisson arrivals at {rate:g} req/s for {duration:g} s over {len(pool.forests)} resident "
+                        "forests of 1e3-1e6 cells (2-3 samples of <= 1e4 cells), shared 50 Mb reference, "
+                        "coverage log-uniform 30-500x, PE 150 bp, error 1e-3, random sample subsets + normal",
+               resident_input_bytes=int(f0 - f1), hbm_budget_bytes=budget,
+               continuous=run(stream, 2, 16, True, "continuous"),
+               one_at_a_time=run(stream, 1, 1, True, "one at a time"))
+    if saturated:
+        burst = workloads.serving_requests(pool, saturated, 0.0, seed=10)
+        out["saturated"] = dict(requests=saturated,
+                                continuous=run(burst, 2, 16, False, "saturated continuous"),
+                                one_at_a_time=run(burst, 1, 1, False, "saturated one at a time"))
+    return out
 
 
 def main() -> int:
@@ -66,8 +199,15 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-reads", action="store_true", help="skip the read-materialising mode")
-    ap.add_argument("--cpu-reads", type=float, default=1.5e8, help="reads in the bounded CPU sample")
+    ap.add_argument("--cpu-reads", type=float, default=1.2e9, help="reads in the bounded CPU sample")
     ap.add_argument("--cells", type=int, default=1_000_000)
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched workload")
+    ap.add_argument("--batched-jobs", type=int, default=256)
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner in the batched workload")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving workload")
+    ap.add_argument("--serve-rate", type=float, default=20.0, help="request arrivals per second")
+    ap.add_argument("--serve-seconds", type=float, default=20.0, help="length of the arrival stream")
+    ap.add_argument("--serve-burst", type=int, default=200, help="requests of the saturated run (0 skips it)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run device-to-device copies of BYTES (reference for the HBM byte counters)")
     args = ap.parse_args()
@@ -174,6 +314,12 @@ def main() -> int:
                                                bytes_per_read=bytes_per_read))
         log(f"reads mode: {n_all} reads, kernel {rk:.2f} ms, {ach:.0f} GB/s written ({ach / HBM_PEAK_GBS:.1%})")
         buf.close()
+
+    if not args.no_batched:
+        out["batched"] = batched_leg(ctx, args.batched_jobs, 3, not args.no_tune)
+
+    if not args.no_serving:
+        out["serving"] = serving_leg(ctx, args.serve_rate, args.serve_seconds, args.serve_burst)
 
     if not args.no_cpu:
         recs = built.muts.download()

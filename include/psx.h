@@ -301,6 +301,66 @@ psx_status psx_seq_table(const psx_mutation* recs, uint64_t n_mutations,
                          uint64_t* n_rows, uint64_t* row_rec, int32_t* occ, int32_t* cov,
                          double* vaf);
 
+/* ------------------------------------------------------------------ serving */
+
+/* A request loop over device-resident genomes, forests and mutation tables
+ * (no counterpart in the reference, whose simulate_seq runs one call at a
+ * time, src/seq_simulation.cpp:517-601). Requests are admitted in arrival
+ * order while their output arrays fit in a fixed HBM budget, allocated once
+ * at creation; up to max_inflight launches run at once on their own streams,
+ * each taking every admitted request that waits (up to max_batch) into one
+ * persistent launch, so requests that arrive while others run start as soon
+ * as a launch slot and memory are free. Results are identical to
+ * psx_simulate_seq on the same inputs, alone or in any mix. A genome, forest
+ * and mutation table stay owned by the caller and must outlive the requests
+ * that use them; a mutation table is used with one genome at a time. */
+typedef struct psx_server psx_server;
+
+typedef struct psx_server_config {
+    uint64_t hbm_budget;       /* bytes for request outputs (alt, depth, present) */
+    uint32_t max_inflight;     /* concurrent launches, 1..8 */
+    uint32_t max_batch;        /* requests per launch, >= 1 */
+    psx_launch_config launch;  /* kernel configuration of every launch (use_graph ignored) */
+} psx_server_config;
+
+typedef struct psx_request {
+    const psx_genome* genome;
+    const psx_forest* forest;
+    const psx_mutations* mutations;
+    psx_seq_params params;     /* the chromosome and sample lists are copied */
+    uint64_t seed;
+    int32_t* alt;              /* host, n_out_samples x n_mutations; NULL skips the copy */
+    int32_t* depth;
+    uint8_t* present;
+} psx_request;
+
+typedef struct psx_request_stats {
+    psx_status status;         /* of the request: PSX_OK, or why it was not run */
+    uint64_t n_reads;
+    uint32_t batch_size;       /* requests in its launch */
+    uint32_t lane;             /* launch slot that ran it */
+    double queued_ms;          /* submission -> launch */
+    double latency_ms;         /* submission -> results in host memory */
+} psx_request_stats;
+
+typedef struct psx_server_totals {
+    uint64_t submitted, completed, failed, launches, reads;
+    uint64_t budget_bytes, peak_bytes;  /* arena size and its high-water mark */
+} psx_server_totals;
+
+psx_status psx_server_create(psx_ctx* ctx, const psx_server_config* cfg, psx_server** out);
+/* Validates and queues a request (PSX_EINVAL for invalid parameters,
+ * PSX_ENOMEM_DEVICE when its outputs alone exceed the budget). */
+psx_status psx_server_submit(psx_server* s, const psx_request* r, uint64_t* ticket);
+/* Blocks until the request has finished; returns its status and forgets the
+ * ticket. timeout_ms < 0 waits without limit; PSX_ECANCELLED on time-out
+ * (the request stays queued). */
+psx_status psx_server_wait(psx_server* s, uint64_t ticket, double timeout_ms, psx_request_stats* st);
+psx_status psx_server_totals_get(psx_server* s, psx_server_totals* t);
+const char* psx_server_last_error(psx_server* s);
+/* Finishes the running launches, cancels queued requests, frees the budget. */
+void psx_server_destroy(psx_server* s);
+
 /* ------------------------------------------- synthetic forest (host input) */
 
 /* Birth-death genealogy standing in for the CPU spatial simulation, which

@@ -69,6 +69,27 @@ class LaunchConfig(C.Structure):
                 ("blocks_per_cu", C.c_uint32), ("use_graph", C.c_int32)]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget", C.c_uint64), ("max_inflight", C.c_uint32), ("max_batch", C.c_uint32),
+                ("launch", LaunchConfig)]
+
+
+class Request(C.Structure):
+    _fields_ = [("genome", C.c_void_p), ("forest", C.c_void_p), ("mutations", C.c_void_p),
+                ("params", SeqParams), ("seed", C.c_uint64), ("alt", i32p), ("depth", i32p), ("present", u8p)]
+
+
+class RequestStats(C.Structure):
+    _fields_ = [("status", C.c_int32), ("n_reads", C.c_uint64), ("batch_size", C.c_uint32), ("lane", C.c_uint32),
+                ("queued_ms", C.c_double), ("latency_ms", C.c_double)]
+
+
+class ServerTotals(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("failed", C.c_uint64),
+                ("launches", C.c_uint64), ("reads", C.c_uint64), ("budget_bytes", C.c_uint64),
+                ("peak_bytes", C.c_uint64)]
+
+
 class SynthParams(C.Structure):
     _fields_ = [("n_species", C.c_uint32), ("growth", f64p), ("death", f64p), ("intro_at", u64p),
                 ("n_cells", C.c_uint64), ("n_samples", C.c_uint32), ("sample_size", C.c_uint64)]

@@ -69,6 +69,12 @@ def _declare(lib):
                                   C.POINTER(C.c_double)]),
         "psx_seq_table": (st, [vp, C.c_uint64, C.c_uint32, A.i32p, A.i32p, A.u8p, C.c_int32, A.u64p, A.u64p,
                                A.i32p, A.i32p, A.f64p]),
+        "psx_server_create": (st, [vp, C.POINTER(A.ServerConfig), pp]),
+        "psx_server_submit": (st, [vp, C.POINTER(A.Request), A.u64p]),
+        "psx_server_wait": (st, [vp, C.c_uint64, C.c_double, C.POINTER(A.RequestStats)]),
+        "psx_server_totals_get": (st, [vp, C.POINTER(A.ServerTotals)]),
+        "psx_server_last_error": (C.c_char_p, [vp]),
+        "psx_server_destroy": (None, [vp]),
         "psx_synth_forest_run": (st, [C.POINTER(A.SynthParams), C.c_uint64, pp]),
         "psx_synth_forest_nodes": (C.c_uint32, [vp]),
         "psx_synth_forest_get": (st, [vp, A.i64p, A.i64p, A.u32p, A.i32p, A.f64p]),

@@ -9,7 +9,7 @@ import threading
 import numpy as np
 
 from . import _abi as A
-from ._lib import check, lib
+from ._lib import PsxError, check, lib
 
 _default_ctx = None
 _ctx_lock = threading.Lock()
@@ -360,3 +360,70 @@ def synth_forest(*, growth, death, intro_at, n_cells: int, n_samples: int, sampl
         lib().psx_synth_forest_free(h)
     out["n_samples"] = int(n_samples)
     return out
+
+
+class Server:
+    """Serving runtime (psx_server): continuous batching of count-only
+    sequencing requests within an HBM budget, results copied to host arrays."""
+
+    def __init__(self, ctx: Context, hbm_budget: int, max_inflight: int = 2, max_batch: int = 16,
+                 launch: A.LaunchConfig | None = None):
+        self.ctx = ctx
+        if launch is None:
+            # every lane launches a full persistent grid: a launch that starts
+            # while another runs takes its compute units as they free up
+            launch = A.LaunchConfig(256, 512, 1, 8, 0)
+        cfg = A.ServerConfig(int(hbm_budget), int(max_inflight), int(max_batch), launch)
+        h = C.c_void_p()
+        check(lib().psx_server_create(ctx.h, C.byref(cfg), C.byref(h)), ctx.h)
+        self.h = h
+        self._live: dict[int, tuple] = {}
+
+    def _err(self) -> str:
+        m = lib().psx_server_last_error(self.h)
+        return m.decode() if m else ""
+
+    def submit(self, genome: Genome, forest: Forest, muts: Mutations, seed: int, **seq) -> int:
+        p, keep = A.seq_params(**seq)
+        S = int(lib().psx_seq_num_samples(forest.h, C.byref(p)))
+        M = muts.n
+        alt = np.zeros((S, M), np.int32)
+        dep = np.zeros((S, M), np.int32)
+        pres = np.zeros((S, M), np.uint8)
+        r = A.Request(genome.h, forest.h, muts.h, p, int(seed), A.ptr(alt, C.c_int32), A.ptr(dep, C.c_int32),
+                      A.ptr(pres, C.c_uint8))
+        t = C.c_uint64()
+        st = lib().psx_server_submit(self.h, C.byref(r), C.byref(t))
+        if st != 0:
+            raise PsxError(st, self._err())
+        self._live[t.value] = (alt, dep, pres, keep)
+        return t.value
+
+    def wait(self, ticket: int, timeout_ms: float = -1.0):
+        """(alt, depth, present, stats) of a finished request."""
+        s = A.RequestStats()
+        st = lib().psx_server_wait(self.h, C.c_uint64(ticket), float(timeout_ms), C.byref(s))
+        if st == 4 and timeout_ms >= 0 and "timed out" in self._err():
+            raise TimeoutError(self._err())  # the request stays queued
+        alt, dep, pres, _ = self._live.pop(ticket)
+        if st != 0:
+            raise PsxError(st, self._err())
+        stats = dict(n_reads=s.n_reads, batch_size=s.batch_size, lane=s.lane, queued_ms=s.queued_ms,
+                     latency_ms=s.latency_ms)
+        return alt, dep, pres, stats
+
+    def totals(self) -> dict:
+        t = A.ServerTotals()
+        check(lib().psx_server_totals_get(self.h, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in A.ServerTotals._fields_}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().psx_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -6,14 +6,16 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <memory>
 
 #include "psx_seq_dev.h"
+#include "psx_seq_host.h"
 
 using namespace psxk;
 
-namespace {
+namespace psxh {
 
-struct prepared {
+struct seq_batch {
     std::vector<job_dev> jobs;
     std::vector<seg_dev> segs;
     std::vector<item_dev> items;
@@ -22,26 +24,19 @@ struct prepared {
     scratch_builder sb;
     std::vector<size_t> tab_off;  // per job: insert thresholds, insert guide, error thresholds
     size_t o_jobs = 0, o_segs = 0, o_items = 0, o_pres = 0;
-};
-
-struct job_in {
-    const psx_genome* g;
-    const psx_forest* f;
-    const psx_mutations* m;
-    const psx_seq_params* p;
-    uint64_t seed;
-    int32_t* alt;
-    int32_t* depth;
-    uint8_t* present;
+    uint8_t* dev = nullptr;       // device base of the last upload
 };
 
 uint32_t out_samples(const psx_forest* f, const psx_seq_params* p) {
     return (p->samples ? p->n_samples : f->n_samples) + (p->with_normal_sample ? 1u : 0u);
 }
 
+namespace {
+
 // Validates one job and appends its descriptors; the ordering of output
-// samples and chromosomes follows the oracle.
-void prepare_job(psx_ctx* ctx, prepared& P, const job_in& in, uint32_t spi) {
+// samples and chromosomes follows the oracle. Nothing is appended when the
+// job is invalid.
+void prepare_job(psx_ctx* ctx, seq_batch& P, const job_in& in, uint32_t spi) {
     const psx_seq_params* p = in.p;
     const psx_forest* f = in.f;
     psx_mutations* m = const_cast<psx_mutations*>(in.m);
@@ -135,39 +130,80 @@ void prepare_job(psx_ctx* ctx, prepared& P, const job_in& in, uint32_t spi) {
     }
 }
 
-// Places the descriptors in the scratch arena with device pointers patched in.
-void finalize(psx_ctx* ctx, prepared& P) {
+// Places the descriptors of a prepared batch after the per-job tables.
+void layout(seq_batch& P) {
     P.o_segs = P.sb.put_vec(P.segs);
     P.o_items = P.sb.put_vec(P.items);
     P.o_pres = P.sb.put_vec(P.pres);
     P.o_jobs = P.sb.put_vec(P.jobs);
-    ctx->scratch.reserve(P.sb.host.size() + 256);
-    uint8_t* base = (uint8_t*)ctx->scratch.p;
-    job_dev* hj = (job_dev*)(P.sb.host.data() + P.o_jobs);
-    auto at = [&](size_t off) -> const void* { return off == SIZE_MAX ? nullptr : base + off; };
-    for (size_t k = 0; k < P.jobs.size(); ++k) {
-        hj[k].ins_T = (const uint32_t*)at(P.tab_off[3 * k]);
-        hj[k].ins_guide = (const uint16_t*)at(P.tab_off[3 * k + 1]);
-        hj[k].err_T = (const uint32_t*)at(P.tab_off[3 * k + 2]);
-    }
-    psx_scratch_upload(ctx, P.sb);
 }
 
-seq_launch launch_desc(psx_ctx* ctx, const prepared& P) {
-    uint8_t* base = (uint8_t*)ctx->scratch.p;
-    ctx->queue.reserve(256);
+template <int BLOCK, int CAP>
+void launch_count_t(const seq_lane& lane, const seq_launch& L, uint32_t grid) {
+    hipLaunchKernelGGL((k_seq_count<BLOCK, CAP>), dim3(grid), dim3(BLOCK), 0, lane.stream, L);
+}
+
+void launch_count(const seq_lane& lane, const psx_launch_config& c, const seq_launch& L, uint32_t grid) {
+#define PSX_CASE(B, C) \
+    if (c.block_threads == B && c.lds_records == C) return launch_count_t<B, C>(lane, L, grid);
+    PSX_CASE(256, 512) PSX_CASE(256, 1024) PSX_CASE(256, 2048)
+    PSX_CASE(512, 512) PSX_CASE(512, 1024) PSX_CASE(512, 2048)
+#undef PSX_CASE
+    psx_throw(PSX_EINVAL, "unsupported launch configuration");
+}
+
+seq_launch launch_desc(const seq_batch& P, uint8_t* base, unsigned long long* queue) {
     seq_launch L{};
     L.jobs = (const job_dev*)(base + P.o_jobs);
     L.segs = (const seg_dev*)(base + P.o_segs);
     L.items = (const item_dev*)(base + P.o_items);
     L.n_items = P.items.size();
-    L.queue = ctx->queue.as<unsigned long long>();
+    L.queue = queue;
     return L;
 }
 
-template <int BLOCK, int CAP>
-void launch_count_t(psx_ctx* ctx, const seq_launch& L, uint32_t grid) {
-    hipLaunchKernelGGL((k_seq_count<BLOCK, CAP>), dim3(grid), dim3(BLOCK), 0, ctx->stream, L);
+}  // namespace
+
+seq_batch* batch_prepare(psx_ctx* ctx, const job_in* jobs, size_t n_jobs, uint32_t spi) {
+    auto P = std::make_unique<seq_batch>();
+    for (size_t k = 0; k < n_jobs; ++k) prepare_job(ctx, *P, jobs[k], spi);
+    layout(*P);
+    return P.release();
+}
+
+void batch_free(seq_batch* b) { delete b; }
+size_t batch_desc_bytes(const seq_batch* b) { return b->sb.host.size() + 256; }
+uint64_t batch_reads(const seq_batch* b) { return b->n_reads; }
+size_t batch_items(const seq_batch* b) { return b->items.size(); }
+
+void batch_upload(seq_batch* P, const seq_lane& lane) {
+    if (lane.desc_bytes < batch_desc_bytes(P)) psx_throw(PSX_EINVAL, "descriptor buffer too small");
+    uint8_t* base = lane.desc;
+    job_dev* hj = (job_dev*)(P->sb.host.data() + P->o_jobs);
+    auto at = [&](size_t off) -> const void* { return off == SIZE_MAX ? nullptr : base + off; };
+    for (size_t k = 0; k < P->jobs.size(); ++k) {
+        hj[k].ins_T = (const uint32_t*)at(P->tab_off[3 * k]);
+        hj[k].ins_guide = (const uint16_t*)at(P->tab_off[3 * k + 1]);
+        hj[k].err_T = (const uint32_t*)at(P->tab_off[3 * k + 2]);
+    }
+    if (!P->sb.host.empty())
+        PSX_HIP(hipMemcpyAsync(base, P->sb.host.data(), P->sb.host.size(), hipMemcpyHostToDevice, lane.stream));
+    P->dev = base;
+}
+
+void batch_launch(const seq_batch* P, const seq_lane& lane, const psx_launch_config& cfg, bool timed) {
+    if (P->dev != lane.desc) psx_throw(PSX_EINVAL, "descriptors were uploaded to another lane");
+    const seq_launch L = launch_desc(*P, lane.desc, lane.queue);
+    const pres_dev* ps = (const pres_dev*)(lane.desc + P->o_pres);
+    const uint64_t want = (uint64_t)lane.n_cu * cfg.blocks_per_cu;
+    const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, P->items.size()));
+    const uint32_t init_grid = (uint32_t)std::max<uint64_t>(1, (P->pres_total + 255) / 256);
+    hipLaunchKernelGGL(k_seq_init, dim3(init_grid), dim3(256), 0, lane.stream, L.jobs, ps, (uint32_t)P->pres.size(),
+                       P->pres_total, L.queue);
+    if (timed) PSX_HIP(hipEventRecord(lane.ev0, lane.stream));
+    launch_count(lane, cfg, L, grid);
+    PSX_HIP(hipGetLastError());
+    if (timed) PSX_HIP(hipEventRecord(lane.ev1, lane.stream));
 }
 
 psx_launch_config default_config() { return psx_launch_config{256, 512, 1, 8, 0}; }
@@ -180,13 +216,24 @@ void check_config(const psx_launch_config& c) {
                   c.block_threads, c.lds_records, c.strata_per_item, c.blocks_per_cu);
 }
 
-void launch_count(psx_ctx* ctx, const psx_launch_config& c, const seq_launch& L, uint32_t grid) {
-#define PSX_CASE(B, C) \
-    if (c.block_threads == B && c.lds_records == C) return launch_count_t<B, C>(ctx, L, grid);
-    PSX_CASE(256, 512) PSX_CASE(256, 1024) PSX_CASE(256, 2048)
-    PSX_CASE(512, 512) PSX_CASE(512, 1024) PSX_CASE(512, 2048)
-#undef PSX_CASE
-    psx_throw(PSX_EINVAL, "unsupported launch configuration");
+}  // namespace psxh
+
+using namespace psxh;
+
+namespace {
+
+seq_lane ctx_lane(psx_ctx* ctx, size_t desc_bytes) {
+    ctx->scratch.reserve(desc_bytes);
+    ctx->queue.reserve(256);
+    seq_lane lane;
+    lane.n_cu = ctx->n_cu;
+    lane.stream = ctx->stream;
+    lane.ev0 = ctx->ev0;
+    lane.ev1 = ctx->ev1;
+    lane.desc = (uint8_t*)ctx->scratch.p;
+    lane.desc_bytes = ctx->scratch.bytes;
+    lane.queue = ctx->queue.as<unsigned long long>();
+    return lane;
 }
 
 struct run_result {
@@ -195,49 +242,36 @@ struct run_result {
 };
 
 // Uploads descriptors, runs the init and count kernels (optionally as a
-// captured hipGraph) and times the count kernel with HIP events.
+// captured hipGraph) and times the count kernel with HIP events (the graph
+// launch, init included, when captured).
 run_result run_count(psx_ctx* ctx, const std::vector<job_in>& jobs, const psx_launch_config& cfg) {
     check_config(cfg);
     PSX_HIP(hipSetDevice(ctx->device));
-    prepared P;
-    for (const auto& j : jobs) prepare_job(ctx, P, j, cfg.strata_per_item);
-    finalize(ctx, P);
-    const seq_launch L = launch_desc(ctx, P);
-    const uint8_t* base = (const uint8_t*)ctx->scratch.p;
-    const pres_dev* ps = (const pres_dev*)(base + P.o_pres);
-    const uint64_t want = (uint64_t)ctx->n_cu * cfg.blocks_per_cu;
-    const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, P.items.size()));
-    const uint32_t init_grid = (uint32_t)std::max<uint64_t>(1, (P.pres_total + 255) / 256);
-    auto enqueue_init = [&] {
-        hipLaunchKernelGGL(k_seq_init, dim3(init_grid), dim3(256), 0, ctx->stream, L.jobs, ps,
-                           (uint32_t)P.pres.size(), P.pres_total, L.queue);
-    };
+    std::unique_ptr<seq_batch, void (*)(seq_batch*)> P(batch_prepare(ctx, jobs.data(), jobs.size(),
+                                                                     cfg.strata_per_item), batch_free);
+    const seq_lane lane = ctx_lane(ctx, batch_desc_bytes(P.get()));
+    batch_upload(P.get(), lane);
     if (cfg.use_graph) {
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
-        PSX_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        enqueue_init();
-        launch_count(ctx, cfg, L, grid);
-        PSX_HIP(hipStreamEndCapture(ctx->stream, &graph));
+        PSX_HIP(hipStreamBeginCapture(lane.stream, hipStreamCaptureModeThreadLocal));
+        batch_launch(P.get(), lane, cfg, false);
+        PSX_HIP(hipStreamEndCapture(lane.stream, &graph));
         PSX_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-        PSX_HIP(hipEventRecord(ctx->ev0, ctx->stream));
-        PSX_HIP(hipGraphLaunch(exec, ctx->stream));
-        PSX_HIP(hipEventRecord(ctx->ev1, ctx->stream));
-        PSX_HIP(hipStreamSynchronize(ctx->stream));
+        PSX_HIP(hipEventRecord(lane.ev0, lane.stream));
+        PSX_HIP(hipGraphLaunch(exec, lane.stream));
+        PSX_HIP(hipEventRecord(lane.ev1, lane.stream));
+        PSX_HIP(hipStreamSynchronize(lane.stream));
         (void)hipGraphExecDestroy(exec);
         (void)hipGraphDestroy(graph);
     } else {
-        enqueue_init();
-        PSX_HIP(hipEventRecord(ctx->ev0, ctx->stream));
-        launch_count(ctx, cfg, L, grid);
-        PSX_HIP(hipGetLastError());
-        PSX_HIP(hipEventRecord(ctx->ev1, ctx->stream));
-        PSX_HIP(hipStreamSynchronize(ctx->stream));
+        batch_launch(P.get(), lane, cfg, true);
+        PSX_HIP(hipStreamSynchronize(lane.stream));
     }
     float ms = 0.f;
-    PSX_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    PSX_HIP(hipEventElapsedTime(&ms, lane.ev0, lane.ev1));
     ctx->last_kernel_ms = ms;
-    return run_result{P.n_reads, (double)ms};
+    return run_result{batch_reads(P.get()), (double)ms};
 }
 
 std::vector<job_in> jobs_of(const psx_seq_job* jobs, uint32_t n_jobs) {
@@ -328,14 +362,16 @@ extern "C" psx_status psx_simulate_seq_reads(psx_ctx* ctx, const psx_genome* g, 
         PSX_HIP(hipSetDevice(ctx->device));
         constexpr int BLOCK = 256, CAP = 1024;
         constexpr uint32_t STAGE_BYTES = 32768;  // LDS read staging per block
-        prepared P;
-        prepare_job(ctx, P, job_in{g, f, m, p, seed, nullptr, nullptr, nullptr}, 1);
+        const job_in in{g, f, m, p, seed, nullptr, nullptr, nullptr};
+        std::unique_ptr<seq_batch, void (*)(seq_batch*)> B(batch_prepare(ctx, &in, 1, 1), batch_free);
+        seq_batch& P = *B;
         if (P.n_reads > capacity_reads)
             psx_throw(PSX_EINVAL, "the read buffer holds %llu reads, the run simulates %llu",
                       (unsigned long long)capacity_reads, (unsigned long long)P.n_reads);
-        finalize(ctx, P);
-        seq_launch L = launch_desc(ctx, P);
-        PSX_HIP(hipMemsetAsync(ctx->queue.p, 0, 8, ctx->stream));
+        const seq_lane lane = ctx_lane(ctx, batch_desc_bytes(&P));
+        batch_upload(&P, lane);
+        seq_launch L = launch_desc(P, lane.desc, lane.queue);
+        PSX_HIP(hipMemsetAsync(lane.queue, 0, 8, ctx->stream));
         L.reads_out = d_reads;
         L.read_words = psx_read_words(p);
         const uint32_t rpt = p->insert_size_mean > 0 ? 2u : 1u;

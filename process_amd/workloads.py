@@ -124,3 +124,103 @@ def batched_jobs(seed: int = 99, n_jobs: int = 256, total_bases: int = 50_000_00
                          place_seed=int(rng.integers(1 << 31)), seq_seed=int(rng.integers(1 << 31))))
     sig = dirichlet_signature(rng)
     return lens, sig, jobs
+
+
+@dataclass
+class BatchedSet:
+    genome: core.Genome
+    jobs: list          # core.SeqJob
+    specs: list         # dicts of batched_jobs
+    forests: list
+    muts: list
+    place_ms: float
+    reads: int
+
+
+def seq_job_params(coverage: float) -> dict:
+    return dict(coverage=coverage, read_size=150, insert_size_mean=350, insert_size_stddev=10, error_rate=1e-3,
+                purity=1.0, with_normal_sample=True)
+
+
+def build_batched(ctx: core.Context, seed: int = 99, n_jobs: int = 256, **kw) -> BatchedSet:
+    """The batched workload: n_jobs one-species forests (growth 0.2, death
+    0.01) grown to twice their sample size, one sample each, over a shared
+    50 Mb reference of 4 chromosomes; SNV 1e-8, indel 1e-9, one Dirichlet
+    signature; paired-end 150 bp reads (insert 350 +- 10), error rate 1e-3,
+    normal sample included; coverage per job from batched_jobs."""
+    lens, sig, specs = batched_jobs(seed=seed, n_jobs=n_jobs, **kw)
+    g = core.Genome(ctx, lens, seed=seed + 1)
+    engine = dict(snv_rate=[1e-8], indel_rate=[1e-9], sbs=sig[None, :], epoch_start=[0.0], exposure=[[1.0]],
+                  n_pre_snv=0, n_pre_indel=0, pre_sbs=0, indel_max_len=10, indel_len_p=0.5, infinite_sites=True,
+                  ploidy=2, drivers=[], germline=[])
+    jobs, forests, muts = [], [], []
+    place_ms, reads = 0.0, 0
+    for s in specs:
+        fd = core.synth_forest(growth=[0.2], death=[0.01], intro_at=[0], n_cells=2 * s["leaves"], n_samples=1,
+                               sample_size=s["leaves"], seed=s["forest_seed"])
+        f = core.Forest(ctx, fd["cell_id"], fd["parent"], fd["species"], fd["sample"], fd["birth"], fd["n_samples"])
+        m = core.Mutations.place(ctx, g, f, s["place_seed"], **engine)
+        place_ms += ctx.last_place_ms()
+        seq = seq_job_params(s["coverage"])
+        jobs.append(core.SeqJob(ctx, g, f, m, s["seq_seed"], **seq))
+        reads += core.num_reads(g, f, **seq)
+        forests.append(f)
+        muts.append(m)
+    return BatchedSet(g, jobs, specs, forests, muts, place_ms, reads)
+
+
+@dataclass
+class ServingPool:
+    genome: core.Genome
+    forests: list       # core.Forest
+    muts: list          # core.Mutations
+    n_samples: list
+    cells: list
+
+
+def build_serving_pool(ctx: core.Context, seed: int = 555, n_forests: int = 8, total_bases: int = 50_000_000,
+                       min_cells: int = 1_000, max_cells: int = 1_000_000) -> ServingPool:
+    """Device-resident inputs of the serving workload: n_forests two-species
+    forests with log-uniform cell counts in [min_cells, max_cells] (the
+    smallest and largest included), 2-3 samples each of up to 1e4 cells, over
+    a shared 50 Mb reference; SNV 1e-8, indel 1e-9."""
+    rng = np.random.default_rng(seed)
+    lens = log_uniform_lengths(4, total_bases, 1.0, 4.0, rng)
+    g = core.Genome(ctx, lens, seed=seed + 1)
+    sig = dirichlet_signature(rng)
+    engine = dict(snv_rate=[1e-8, 1e-8], indel_rate=[1e-9, 1e-9], sbs=sig[None, :], epoch_start=[0.0],
+                  exposure=[[1.0]], n_pre_snv=0, n_pre_indel=0, pre_sbs=0, indel_max_len=10, indel_len_p=0.5,
+                  infinite_sites=True, ploidy=2, drivers=[], germline=[])
+    sizes = np.exp(np.linspace(np.log(min_cells), np.log(max_cells), n_forests))
+    forests, muts, ns, cells = [], [], [], []
+    for k, c in enumerate(sizes.astype(int)):
+        n_s = 2 + k % 2
+        ss = int(min(10_000, c // (2 * n_s)))
+        fd = core.synth_forest(growth=[0.2, 0.3], death=[0.01, 0.01], intro_at=[0, max(2, c // 1000)], n_cells=int(c),
+                               n_samples=n_s, sample_size=ss, seed=int(rng.integers(1 << 31)))
+        f = core.Forest(ctx, fd["cell_id"], fd["parent"], fd["species"], fd["sample"], fd["birth"], fd["n_samples"])
+        m = core.Mutations.place(ctx, g, f, int(rng.integers(1 << 31)), **engine)
+        forests.append(f)
+        muts.append(m)
+        ns.append(n_s)
+        cells.append(int(c))
+    return ServingPool(g, forests, muts, ns, cells)
+
+
+def serving_requests(pool: ServingPool, n: int, rate: float, seed: int = 9, min_cov: float = 30.0,
+                     max_cov: float = 500.0):
+    """n requests with Poisson arrivals at `rate` per second: (arrival s,
+    forest index, seed, sequencing parameters); coverage log-uniform in
+    [min_cov, max_cov], a random non-empty subset of the forest's samples,
+    paired-end 150 bp reads, error rate 1e-3, normal sample included."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, n)) if rate > 0 else np.zeros(n)
+    out = []
+    for i in range(n):
+        k = int(rng.integers(len(pool.forests)))
+        ns = pool.n_samples[k]
+        pick = [s for s in range(ns) if rng.random() < 0.6] or [int(rng.integers(ns))]
+        seq = seq_job_params(float(np.exp(rng.uniform(np.log(min_cov), np.log(max_cov)))))
+        seq["samples"] = pick
+        out.append((float(t[i]), k, int(rng.integers(1 << 31)), seq))
+    return out

@@ -48,6 +48,8 @@ int main(void) {
          sizeof(psx_synth_params), sizeof(psx_driver), sizeof(psx_germline));
   printf("%zu %zu %zu %zu\\n", offsetof(psx_mutation, cause), offsetof(psx_mutation, sid_head),
          offsetof(psx_seq_params, chromosomes), offsetof(psx_engine_params, germline));
+  printf("%zu %zu %zu %zu %zu %zu\\n", sizeof(psx_server_config), sizeof(psx_request), sizeof(psx_request_stats),
+         sizeof(psx_server_totals), offsetof(psx_request, seed), offsetof(psx_request_stats, latency_ms));
   return 0;
 }""")
     exe = tmp_path / "layout"
@@ -57,6 +59,9 @@ int main(void) {
     assert sizes == [A.MUTATION_DTYPE.itemsize, C.sizeof(A.ForestDesc), C.sizeof(A.EngineParams),
                      C.sizeof(A.SeqParams), C.sizeof(A.SeqJob), C.sizeof(A.LaunchConfig), C.sizeof(A.SynthParams),
                      C.sizeof(A.Driver), C.sizeof(A.Germline)]
+    srv = list(map(int, lines[2].split()))
+    assert srv == [C.sizeof(A.ServerConfig), C.sizeof(A.Request), C.sizeof(A.RequestStats),
+                   C.sizeof(A.ServerTotals), A.Request.seed.offset, A.RequestStats.latency_ms.offset]
     offs = list(map(int, lines[1].split()))
     assert offs == [A.MUTATION_DTYPE.fields["cause"][1], A.MUTATION_DTYPE.fields["sid_head"][1],
                     A.SeqParams.chromosomes.offset, A.EngineParams.germline.offset]
