@@ -282,6 +282,17 @@ typedef struct psx_launch_config {
  * strata) items from a device queue. cfg = NULL uses the tuned default. */
 psx_status psx_simulate_seq_batch(psx_ctx* ctx, const psx_seq_job* jobs, uint32_t n_jobs,
                                   const psx_launch_config* cfg, uint64_t* n_reads);
+/* A batched launch prepared once and run many times: the descriptors are
+ * built and uploaded at creation and, with cfg->use_graph (the default when
+ * cfg is NULL), the launch is captured in a hipGraph that every run replays.
+ * Each run rewrites the jobs' output arrays; the jobs' inputs and outputs
+ * must outlive the plan. Runs on the context's stream; psx_ctx_last_kernel_ms
+ * reports the replay (init and count kernels). */
+typedef struct psx_seq_plan psx_seq_plan;
+psx_status psx_seq_plan_create(psx_ctx* ctx, const psx_seq_job* jobs, uint32_t n_jobs,
+                               const psx_launch_config* cfg, psx_seq_plan** out);
+psx_status psx_seq_plan_run(psx_seq_plan* plan, uint64_t* n_reads);
+void psx_seq_plan_free(psx_seq_plan* plan);
 /* Times candidate launch configurations on the jobs and returns the fastest
  * (results are identical for every configuration). */
 psx_status psx_autotune_seq(psx_ctx* ctx, const psx_seq_job* jobs, uint32_t n_jobs,

@@ -282,6 +282,34 @@ def simulate_seq_batch(ctx: Context, jobs: list, config: A.LaunchConfig | None =
     return nr.value
 
 
+class SeqPlan:
+    """A batched launch prepared once (psx_seq_plan): run() replays it."""
+
+    def __init__(self, ctx: Context, jobs: list, config: A.LaunchConfig | None = None):
+        self.ctx, self.jobs = ctx, list(jobs)
+        self._arr, self._keep = _job_array(self.jobs)
+        h = C.c_void_p()
+        check(lib().psx_seq_plan_create(ctx.h, self._arr, len(self.jobs), None if config is None else
+                                        C.byref(config), C.byref(h)), ctx.h)
+        self.h = h
+
+    def run(self) -> int:
+        nr = C.c_uint64()
+        check(lib().psx_seq_plan_run(self.h, C.byref(nr)), self.ctx.h)
+        return nr.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().psx_seq_plan_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def autotune(ctx: Context, jobs: list) -> tuple[A.LaunchConfig, float]:
     arr, keep = _job_array(jobs)
     best = A.LaunchConfig()

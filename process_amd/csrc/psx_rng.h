@@ -75,6 +75,22 @@ PSX_HD psx_u4 psx_draw(uint64_t key, uint32_t c0, uint32_t c1, uint32_t c2, uint
     return psx_philox(psx_u4{c0, c1, c2, c3}, (uint32_t)key, (uint32_t)(key >> 32));
 }
 
+#if defined(__HIPCC__)
+// psx_draw for a key that is uniform across the wavefront (a launch seed).
+// The empty asm keeps the key in scalar registers and stops the compiler from
+// hoisting the ten round keys out of the caller's loops: kept live, they
+// exhaust the scalar registers and are spilled to vector lanes, and every
+// round then pays vector read-lane instructions; recomputed, each round key
+// costs one scalar add.
+__device__ __forceinline__ psx_u4 psx_draw_u(uint64_t key, uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3) {
+    uint32_t k0 = (uint32_t)key, k1 = (uint32_t)(key >> 32);
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm volatile("" : "+s"(k0), "+s"(k1));
+#endif
+    return psx_philox(psx_u4{c0, c1, c2, c3}, k0, k1);
+}
+#endif
+
 PSX_HD uint32_t psx_word(const psx_u4& v, int i) {
     return i == 0 ? v.x : (i == 1 ? v.y : (i == 2 ? v.z : v.w));
 }

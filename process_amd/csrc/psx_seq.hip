@@ -360,9 +360,12 @@ extern "C" psx_status psx_simulate_seq_reads(psx_ctx* ctx, const psx_genome* g, 
     if (!ctx || !g || !f || !m || !p || !d_reads) return PSX_EINVAL;
     return psx_guard(ctx, [&] {
         PSX_HIP(hipSetDevice(ctx->device));
-        constexpr int BLOCK = 256, CAP = 1024;
-        constexpr uint32_t STAGE_BYTES = 32768;  // LDS read staging per block
+        constexpr int BLOCK = 256, CAP = 512;
+        constexpr size_t STAGE_MAX = 48u << 10;  // LDS read stage per block
         const job_in in{g, f, m, p, seed, nullptr, nullptr, nullptr};
+        if (m->ploidy > 16) psx_throw(PSX_EINVAL, "read records hold copy indices below 16 (ploidy %u)", m->ploidy);
+        if (f->n_leaves >= (1u << 26))
+            psx_throw(PSX_EINVAL, "read records hold leaf ranks below 2^26 (the forest has %u leaves)", f->n_leaves);
         std::unique_ptr<seq_batch, void (*)(seq_batch*)> B(batch_prepare(ctx, &in, 1, 1), batch_free);
         seq_batch& P = *B;
         if (P.n_reads > capacity_reads)
@@ -374,14 +377,18 @@ extern "C" psx_status psx_simulate_seq_reads(psx_ctx* ctx, const psx_genome* g, 
         PSX_HIP(hipMemsetAsync(lane.queue, 0, 8, ctx->stream));
         L.reads_out = d_reads;
         L.read_words = psx_read_words(p);
-        const uint32_t rpt = p->insert_size_mean > 0 ? 2u : 1u;
-        L.batch_templates = std::max(1u, std::min<uint32_t>(BLOCK, STAGE_BYTES / (rpt * L.read_words * 4u)));
-        const size_t dyn = (size_t)L.batch_templates * rpt * L.read_words * 4u;
+        L.out_vec4 = (L.read_words % 4 == 0 && ((uintptr_t)d_reads & 15) == 0) ? 1u : 0u;
+        const size_t stage = (size_t)(BLOCK / 64) * 64 * L.read_words * 4u;
+        const bool staged = stage <= STAGE_MAX;
         const uint64_t want = (uint64_t)ctx->n_cu * 4u;
         const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, P.items.size()));
         PSX_HIP(hipEventRecord(ctx->ev0, ctx->stream));
-        if (!P.items.empty())
-            hipLaunchKernelGGL((k_seq_reads<BLOCK, CAP>), dim3(grid), dim3(BLOCK), dyn, ctx->stream, L);
+        if (!P.items.empty()) {
+            if (staged)
+                hipLaunchKernelGGL((k_seq_reads<BLOCK, CAP, true>), dim3(grid), dim3(BLOCK), stage, ctx->stream, L);
+            else
+                hipLaunchKernelGGL((k_seq_reads<BLOCK, CAP, false>), dim3(grid), dim3(BLOCK), 0, ctx->stream, L);
+        }
         PSX_HIP(hipGetLastError());
         PSX_HIP(hipEventRecord(ctx->ev1, ctx->stream));
         PSX_HIP(hipStreamSynchronize(ctx->stream));
@@ -402,6 +409,87 @@ extern "C" psx_status psx_simulate_seq_batch(psx_ctx* ctx, const psx_seq_job* jo
         run_result r = in.empty() ? run_result{0, 0.0} : run_count(ctx, in, c);
         if (n_reads) *n_reads = r.n_reads;
     });
+}
+
+// A batched launch prepared once: descriptors in its own device buffer and,
+// with use_graph, the init and count kernels captured in a hipGraph that
+// every run replays.
+struct psx_seq_plan {
+    psx_ctx* ctx = nullptr;
+    std::unique_ptr<seq_batch, void (*)(seq_batch*)> batch{nullptr, batch_free};
+    psx_launch_config cfg{};
+    dbuf desc, queue;
+    seq_lane lane;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+extern "C" psx_status psx_seq_plan_create(psx_ctx* ctx, const psx_seq_job* jobs, uint32_t n_jobs,
+                                          const psx_launch_config* cfg, psx_seq_plan** out) {
+    if (!ctx || !jobs || !n_jobs || !out) return PSX_EINVAL;
+    *out = nullptr;
+    auto P = std::make_unique<psx_seq_plan>();
+    psx_status st = psx_guard(ctx, [&] {
+        const std::vector<job_in> in = jobs_of(jobs, n_jobs);
+        psx_launch_config c = cfg ? *cfg : (ctx->tuned ? ctx->tuned_cfg : default_config());
+        if (!cfg) c.use_graph = 1;
+        check_config(c);
+        PSX_HIP(hipSetDevice(ctx->device));
+        P->ctx = ctx;
+        P->cfg = c;
+        P->batch.reset(batch_prepare(ctx, in.data(), in.size(), c.strata_per_item));
+        P->desc.alloc(batch_desc_bytes(P->batch.get()));
+        P->queue.alloc(256);
+        P->lane.n_cu = ctx->n_cu;
+        P->lane.stream = ctx->stream;
+        P->lane.ev0 = ctx->ev0;
+        P->lane.ev1 = ctx->ev1;
+        P->lane.desc = (uint8_t*)P->desc.p;
+        P->lane.desc_bytes = P->desc.bytes;
+        P->lane.queue = P->queue.as<unsigned long long>();
+        batch_upload(P->batch.get(), P->lane);
+        PSX_HIP(hipStreamSynchronize(ctx->stream));
+        if (c.use_graph) {
+            PSX_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+            batch_launch(P->batch.get(), P->lane, c, false);
+            PSX_HIP(hipStreamEndCapture(ctx->stream, &P->graph));
+            PSX_HIP(hipGraphInstantiate(&P->exec, P->graph, nullptr, nullptr, 0));
+        }
+    });
+    if (st != PSX_OK) {
+        psx_seq_plan_free(P.release());
+        return st;
+    }
+    *out = P.release();
+    return PSX_OK;
+}
+
+extern "C" psx_status psx_seq_plan_run(psx_seq_plan* P, uint64_t* n_reads) {
+    if (!P || !P->ctx) return PSX_EINVAL;
+    psx_ctx* ctx = P->ctx;
+    return psx_guard(ctx, [&] {
+        PSX_HIP(hipSetDevice(ctx->device));
+        if (P->exec) {
+            PSX_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+            PSX_HIP(hipGraphLaunch(P->exec, ctx->stream));
+            PSX_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+        } else {
+            batch_launch(P->batch.get(), P->lane, P->cfg, true);
+        }
+        PSX_HIP(hipStreamSynchronize(ctx->stream));
+        float ms = 0.f;
+        PSX_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        ctx->last_kernel_ms = ms;
+        if (n_reads) *n_reads = batch_reads(P->batch.get());
+    });
+}
+
+extern "C" void psx_seq_plan_free(psx_seq_plan* P) {
+    if (!P) return;
+    if (P->ctx) (void)hipSetDevice(P->ctx->device);
+    if (P->exec) (void)hipGraphExecDestroy(P->exec);
+    if (P->graph) (void)hipGraphDestroy(P->graph);
+    delete P;
 }
 
 extern "C" psx_status psx_autotune_seq(psx_ctx* ctx, const psx_seq_job* jobs, uint32_t n_jobs,

@@ -87,7 +87,7 @@ struct seq_launch {
     unsigned long long* queue;
     uint32_t* reads_out;     // materialisation only
     uint32_t read_words;
-    uint32_t batch_templates;  // templates staged per LDS batch (materialisation)
+    uint32_t out_vec4;       // read_words % 4 == 0 and reads_out 16-byte aligned
 };
 
 struct lds_tables {
@@ -167,7 +167,7 @@ __device__ __forceinline__ bool carried(const job_dev& J, uint32_t info, uint32_
 __device__ __forceinline__ tmpl make_template(const job_dev& J, const seg_dev& S, const lds_tables& tab, uint32_t r,
                                               uint32_t j, uint32_t b, uint32_t e) {
     const uint32_t c2 = (S.chr << 16) | S.so;
-    const psx_u4 w = psx_draw(J.seed, r, j, c2, PSX_P_TEMPLATE << 24);
+    const psx_u4 w = psx_draw_u(J.seed, r, j, c2, PSX_P_TEMPLATE << 24);
     tmpl t;
     t.x = b + psx_uniform32(w.x, e - b);
     if (S.n_cells == 0 || (uint64_t)w.y < J.thr_normal) {
@@ -197,12 +197,12 @@ __device__ __forceinline__ tmpl make_template(const job_dev& J, const seg_dev& S
 __device__ uint32_t error_set(const job_dev& J, const lds_tables& tab, uint32_t r, uint32_t j, uint32_t c2,
                               uint32_t mate, uint16_t* slots, uint32_t stride) {
     const uint32_t base = (PSX_P_ERROR << 24) | (mate << 16);
-    psx_u4 w = psx_draw(J.seed, r, j, c2, base);
+    psx_u4 w = psx_draw_u(J.seed, r, j, c2, base);
     uint32_t K = 0;
     while (K + 1 < J.err_n && w.x >= tab.err[K]) ++K;
     uint32_t cnt = 0;
     for (uint32_t t = 1; cnt < K && t < 1 + 4 * PSX_MAX_READ_ERRORS; ++t) {
-        if ((t & 3u) == 0) w = psx_draw(J.seed, r, j, c2, base | (t >> 2));
+        if ((t & 3u) == 0) w = psx_draw_u(J.seed, r, j, c2, base | (t >> 2));
         const uint32_t word = psx_word(w, (int)(t & 3u));
         const uint32_t o = psx_mulhi32(word, J.L);
         bool dup = false;
@@ -426,28 +426,45 @@ __global__ void __launch_bounds__(256) k_seq_init(const job_dev* __restrict__ jo
 }
 
 // --------------------------------------------------- read materialisation
+//
+// Work layout. A wave takes 64 templates of the stratum at a time (one per
+// lane: draws, insert, source leaf), then writes their reads in passes of 64
+// reads, one read per lane, in output order: with paired ends lane l of pass
+// p holds mate l % 2 of template p * 32 + l / 2, whose fields it fetches from
+// the lane that drew it. A read is built in the lane's row of a wave-private
+// LDS stage and each pass is copied out by the whole wave with 16-byte
+// coalesced stores, so no block barrier separates generation from the copy.
 
-// One read into its LDS row: word 0 = start, word 1 = leaf | allele << 26 |
-// mate << 30 | (errors > 0) << 31, then ceil(L/16) words of 2-bit bases (base
-// i of the read in bits 2 (i % 16) of word i / 16) from the staged
-// reference, with the first carried SNV at each position and the read's
-// errors applied; bases past the read end are 0.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One read into row[0 .. 2 + ceil(L/16)): word 0 = start, word 1 = leaf |
+// allele << 26 | mate << 30 | (errors > 0) << 31, then the 2-bit bases (base
+// i of the read in bits 2 (i % 16) of word 2 + i / 16) taken from the staged
+// reference (s_ref[0] is reference word ref_b / 16), with the first carried
+// SNV at each position and the read's errors applied; bases past the read
+// end are 0. Oracle: the bases seq_read compares, materialised.
 template <class V>
-__device__ void write_read(const job_dev& J, const seg_dev& S, const V& v, const lds_tables& tab, tmpl& t,
-                           uint32_t x, uint32_t r, uint32_t j, uint32_t c2, uint32_t mate, uint16_t* slots,
-                           uint32_t stride, const uint32_t* s_ref, uint32_t ref_w0, uint32_t n_ref, uint32_t* row) {
+__device__ __forceinline__ void write_read(const job_dev& J, const V& v, const lds_tables& tab, uint32_t x,
+                                           uint32_t leaf, uint32_t allele, uint32_t r, uint32_t j, uint32_t c2,
+                                           uint32_t mate, uint16_t* slots, uint32_t stride, const uint32_t* s_ref,
+                                           uint32_t ref_b, uint32_t n_ref, uint32_t* row) {
     const uint32_t nw = (J.L + 15u) >> 4;
     const uint32_t tail = J.L & 15u;
+    const uint32_t xr = x - ref_b;
+    uint32_t w = xr >> 4;
+    const uint32_t sh = (xr & 15u) * 2u;
+    uint32_t lo = s_ref[w];
     for (uint32_t k = 0; k < nw; ++k) {
-        const uint32_t x0 = x + (k << 4);
-        const uint32_t w = (x0 >> 4) - ref_w0, sh = (x0 & 15u) * 2u;
-        uint32_t word = s_ref[w];
-        if (sh) {
-            const uint32_t hi = w + 1 < n_ref ? s_ref[w + 1] : 0u;
-            word = (word >> sh) | (hi << (32u - sh));
-        }
+        const uint32_t hi = w + 1 < n_ref ? s_ref[w + 1] : 0u;
+        uint32_t word = __builtin_amdgcn_alignbit(hi, lo, sh);  // bits [sh, sh + 32) of hi:lo
         if (k + 1 == nw && tail) word &= (1u << (2u * tail)) - 1u;
         row[2 + k] = word;
+        lo = hi;
+        ++w;
     }
     uint32_t i = v.first(x);
     const uint32_t end = x + J.L;
@@ -456,10 +473,10 @@ __device__ void write_read(const job_dev& J, const seg_dev& S, const V& v, const
         bool done = false;
         while (i < v.n && v.pos[i] == p) {
             const uint32_t in = v.info[i];
-            if (!done && PSX_INFO_KIND(in) == PSX_SNV && PSX_INFO_ALLELE(in) == t.allele &&
-                carried(J, in, v.lo[i], v.hi[i], leaf_of(J, S, t))) {
-                const uint32_t o = p - x, sh = (o & 15u) * 2u;
-                row[2 + (o >> 4)] = (row[2 + (o >> 4)] & ~(3u << sh)) | (PSX_INFO_ALT(in) << sh);
+            if (!done && PSX_INFO_KIND(in) == PSX_SNV && PSX_INFO_ALLELE(in) == allele &&
+                carried(J, in, v.lo[i], v.hi[i], leaf)) {
+                const uint32_t o = p - x, s2 = (o & 15u) * 2u;
+                row[2 + (o >> 4)] = (row[2 + (o >> 4)] & ~(3u << s2)) | (PSX_INFO_ALT(in) << s2);
                 done = true;
             }
             ++i;
@@ -468,31 +485,37 @@ __device__ void write_read(const job_dev& J, const seg_dev& S, const V& v, const
     uint32_t ne = 0;
     if (J.err_n) ne = error_set(J, tab, r, j, c2, mate, slots, stride);
     for (uint32_t q = 0; q < ne; ++q) {
-        const uint32_t s = slots[q * stride];
-        const uint32_t o = s & 0xFFFu, sh = (o & 15u) * 2u;
-        uint32_t wv = row[2 + (o >> 4)];
-        const uint32_t tb = (wv >> sh) & 3u, sub = s >> 12;
+        const uint32_t e = slots[q * stride];
+        const uint32_t o = e & 0xFFFu, s2 = (o & 15u) * 2u;
+        const uint32_t wv = row[2 + (o >> 4)];
+        const uint32_t tb = (wv >> s2) & 3u, sub = e >> 12;
         const uint32_t nb = sub < tb ? sub : sub + 1;
-        row[2 + (o >> 4)] = (wv & ~(3u << sh)) | (nb << sh);
+        row[2 + (o >> 4)] = (wv & ~(3u << s2)) | (nb << s2);
     }
     row[0] = x;
-    row[1] = leaf_of(J, S, t) | (t.allele << 26) | (mate << 30) | ((ne > 0 ? 1u : 0u) << 31);
+    row[1] = leaf | (allele << 26) | (mate << 30) | ((ne > 0 ? 1u : 0u) << 31);
 }
 
-template <int BLOCK, int CAP>
+// STAGE: reads are built in LDS and copied out per pass (read_words small
+// enough for 64 rows per wave); otherwise each lane builds its read in place
+// in the output.
+template <int BLOCK, int CAP, bool STAGE>
 __global__ void __launch_bounds__(BLOCK) k_seq_reads(const seq_launch L) {
+    constexpr uint32_t WAVES = BLOCK / 64;
     __shared__ uint32_t s_pos[CAP], s_lo[CAP], s_hi[CAP], s_info[CAP];
     __shared__ uint16_t s_bucket[NB_MAX];
     __shared__ uint16_t s_err[PSX_MAX_READ_ERRORS * BLOCK];
     __shared__ uint32_t s_ref[REF_MAX];
     __shared__ lds_tables s_tab;
     __shared__ uint32_t s_item;
-    extern __shared__ uint4 s_out4[];  // [batch reads][read words], row-major
-    uint32_t* s_out = reinterpret_cast<uint32_t*>(s_out4);
+    extern __shared__ uint4 s_stage4[];  // STAGE: per wave 64 rows of read_words words
     const uint32_t tid = threadIdx.x;
+    const uint32_t lane = tid & 63u, wave = tid >> 6;
     uint16_t* slots = s_err + tid;
     const uint32_t RW = L.read_words;
-    const uint32_t TB = L.batch_templates;
+    const bool vec4 = L.out_vec4 != 0;  // RW % 4 == 0 and a 16-byte aligned output
+    uint32_t* stage = reinterpret_cast<uint32_t*>(s_stage4) + wave * 64u * RW;
+    const uint4* stage4 = reinterpret_cast<const uint4*>(stage);
     for (;;) {
         if (tid == 0) s_item = (uint32_t)atomicAdd(L.queue, 1ull);
         __syncthreads();
@@ -503,6 +526,7 @@ __global__ void __launch_bounds__(BLOCK) k_seq_reads(const seq_launch L) {
         const job_dev J = uniform_at(L.jobs, S.job);
         const uint32_t c2 = (S.chr << 16) | S.so;
         const uint32_t rpt = J.pe ? 2u : 1u;
+        const uint32_t tpp = 64u / rpt;  // templates per pass
         const uint32_t chr_words = (S.len + 15u) >> 4;
         load_tables(J, s_tab, tid, BLOCK);
         __syncthreads();
@@ -517,34 +541,51 @@ __global__ void __launch_bounds__(BLOCK) k_seq_reads(const seq_launch L) {
             else __syncthreads();
             const lds_view lv{s_pos, s_lo, s_hi, s_info, nullptr, nullptr, s_bucket, s.n_win, s.b};
             const glb_view gv{J.pos + s.k0, J.lo + s.k0, J.hi + s.k0, J.info + s.k0, nullptr, nullptr, s.n_win};
-            for (uint32_t r0 = 0; r0 < (uint32_t)s.nj; r0 += TB) {
-                const uint32_t n_t = min(TB, (uint32_t)s.nj - r0);
-                if (tid < n_t) {
-                    const uint32_t r = r0 + tid;
-                    tmpl t = make_template(J, S, s_tab, r, j, s.b, s.e);
-                    uint32_t* row = s_out + tid * rpt * RW;
-                    if (fits) write_read(J, S, lv, s_tab, t, t.x, r, j, c2, 0u, slots, BLOCK, s_ref, ref_w0, n_ref, row);
-                    else write_read(J, S, gv, s_tab, t, t.x, r, j, c2, 0u, slots, BLOCK, s_ref, ref_w0, n_ref, row);
-                    if (J.pe) {
+            const uint32_t nj = (uint32_t)s.nj;
+            for (uint32_t c0 = wave * 64u; c0 < nj; c0 += WAVES * 64u) {
+                const uint32_t rt = c0 + lane;
+                uint32_t tx = 0, tx2 = 0, tla = 0;
+                if (rt < nj) {
+                    tmpl t = make_template(J, S, s_tab, rt, j, s.b, s.e);
+                    tx = t.x;
+                    tx2 = t.x2;
+                    tla = leaf_of(J, S, t) | (t.allele << 26);
+                }
+                for (uint32_t p = 0; p < rpt; ++p) {
+                    const uint32_t src = p * tpp + lane / rpt;
+                    const uint32_t mate = lane & (rpt - 1u);
+                    const uint32_t x1 = (uint32_t)__shfl((int)tx, (int)src);
+                    const uint32_t x2 = (uint32_t)__shfl((int)tx2, (int)src);
+                    const uint32_t la = (uint32_t)__shfl((int)tla, (int)src);
+                    const uint32_t r = c0 + src;
+                    const uint32_t t0 = c0 + p * tpp;  // first template of the pass
+                    const uint32_t nt = t0 < nj ? min(tpp, nj - t0) : 0u;
+                    const uint64_t first = S.read_base + (s.before + t0) * rpt;
+                    uint32_t* row = STAGE ? stage + lane * RW : L.reads_out + (first + lane) * RW;
+                    if (r < nj) {
+                        const uint32_t x = mate ? x2 : x1;
                         if (fits)
-                            write_read(J, S, lv, s_tab, t, t.x2, r, j, c2, 1u, slots, BLOCK, s_ref, ref_w0, n_ref, row + RW);
+                            write_read(J, lv, s_tab, x, la & 0x3FFFFFFu, la >> 26, r, j, c2, mate, slots, BLOCK,
+                                       s_ref, ref_w0 << 4, n_ref, row);
                         else
-                            write_read(J, S, gv, s_tab, t, t.x2, r, j, c2, 1u, slots, BLOCK, s_ref, ref_w0, n_ref, row + RW);
+                            write_read(J, gv, s_tab, x, la & 0x3FFFFFFu, la >> 26, r, j, c2, mate, slots, BLOCK,
+                                       s_ref, ref_w0 << 4, n_ref, row);
+                    }
+                    if (STAGE) {
+                        wave_sync();
+                        const uint32_t words = nt * rpt * RW;
+                        uint32_t* dst = L.reads_out + first * RW;
+                        if (vec4) {
+                            uint4* d4 = reinterpret_cast<uint4*>(dst);
+                            for (uint32_t q = lane; q < (words >> 2); q += 64u) d4[q] = stage4[q];
+                        } else {
+                            for (uint32_t q = lane; q < words; q += 64u) dst[q] = stage[q];
+                        }
+                        wave_sync();
                     }
                 }
-                __syncthreads();
-                // reads [first, first + n_t * rpt) are contiguous in the output
-                const uint64_t first = S.read_base + (s.before + r0) * rpt;
-                const uint32_t total = n_t * rpt * RW;
-                uint32_t* dst = L.reads_out + first * RW;
-                if ((RW & 3u) == 0) {
-                    uint4* d4 = reinterpret_cast<uint4*>(dst);
-                    for (uint32_t q = tid; q < (total >> 2); q += BLOCK) d4[q] = s_out4[q];
-                } else {
-                    for (uint32_t q = tid; q < total; q += BLOCK) dst[q] = s_out[q];
-                }
-                __syncthreads();
             }
+            __syncthreads();
         }
         __syncthreads();
     }

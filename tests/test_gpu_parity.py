@@ -143,6 +143,31 @@ def test_batch_equals_plain(ctx, world):
         assert n == total
 
 
+def test_plan_graph_replays(ctx, world):
+    """A prepared plan replays its captured hipGraph; every replay, with eager
+    calls on the same stream in between and the outputs poisoned before it,
+    gives the oracle's tables."""
+    say("[parity] batched plan: 3 graph replays with eager work between")
+    w = world
+    spec = [(11, cases.SEQ_CASES["pe_errors"]), (14, cases.SEQ_CASES["subset"])]
+    jobs = [core.SeqJob(ctx, w.g, w.pf, w.pm, seed, **kw) for seed, kw in spec]
+    ref = [O.simulate_seq(w.lens, w.of, w.om, seed, **kw) for seed, kw in spec]
+    for cfg in (None, A.LaunchConfig(256, 1024, 2, 4, 0)):
+        plan = core.SeqPlan(ctx, jobs, cfg)
+        for rep in range(3):
+            for j in jobs:
+                for buf in (j.alt, j.depth, j.present):
+                    buf.fill(0x5A)
+            n = plan.run()
+            assert n == sum(r[3] for r in ref)
+            for j, (oa, od, op, on) in zip(jobs, ref):
+                a, d, p = j.results()
+                assert np.array_equal(a, oa) and np.array_equal(d, od) and np.array_equal(p, op), (cfg, rep)
+            # eager work on the same context between replays
+            core.simulate_seq_arrays(ctx, w.g, w.pf, w.pm, 99, **cases.SEQ_CASES["short_reads"])
+        plan.close()
+
+
 def _reads(ctx, w, seed, kw):
     out, n = core.simulate_seq_reads(ctx, w.g, w.pf, w.pm, seed, **kw)
     assert n == core.num_reads(w.g, w.pf, **kw)
