@@ -384,10 +384,13 @@ extern "C" psx_status psx_simulate_seq_reads(psx_ctx* ctx, const psx_genome* g, 
         const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, P.items.size()));
         PSX_HIP(hipEventRecord(ctx->ev0, ctx->stream));
         if (!P.items.empty()) {
-            if (staged)
-                hipLaunchKernelGGL((k_seq_reads<BLOCK, CAP, true>), dim3(grid), dim3(BLOCK), stage, ctx->stream, L);
+            if (staged && L.out_vec4 && L.read_words == 12)  // 145-160 bp reads
+                hipLaunchKernelGGL((k_seq_reads<BLOCK, CAP, true, 10>), dim3(grid), dim3(BLOCK), stage, ctx->stream,
+                                   L);
+            else if (staged)
+                hipLaunchKernelGGL((k_seq_reads<BLOCK, CAP, true, 0>), dim3(grid), dim3(BLOCK), stage, ctx->stream, L);
             else
-                hipLaunchKernelGGL((k_seq_reads<BLOCK, CAP, false>), dim3(grid), dim3(BLOCK), 0, ctx->stream, L);
+                hipLaunchKernelGGL((k_seq_reads<BLOCK, CAP, false, 0>), dim3(grid), dim3(BLOCK), 0, ctx->stream, L);
         }
         PSX_HIP(hipGetLastError());
         PSX_HIP(hipEventRecord(ctx->ev1, ctx->stream));

@@ -447,24 +447,37 @@ __device__ __forceinline__ void wave_sync() {
 // reference (s_ref[0] is reference word ref_b / 16), with the first carried
 // SNV at each position and the read's errors applied; bases past the read
 // end are 0. Oracle: the bases seq_read compares, materialised.
-template <class V>
+template <int NW, class V>
 __device__ __forceinline__ void write_read(const job_dev& J, const V& v, const lds_tables& tab, uint32_t x,
                                            uint32_t leaf, uint32_t allele, uint32_t r, uint32_t j, uint32_t c2,
                                            uint32_t mate, uint16_t* slots, uint32_t stride, const uint32_t* s_ref,
-                                           uint32_t ref_b, uint32_t n_ref, uint32_t* row) {
-    const uint32_t nw = (J.L + 15u) >> 4;
+                                           uint32_t ref_b, uint32_t* row) {
     const uint32_t tail = J.L & 15u;
     const uint32_t xr = x - ref_b;
-    uint32_t w = xr >> 4;
+    const uint32_t w = xr >> 4;
     const uint32_t sh = (xr & 15u) * 2u;
-    uint32_t lo = s_ref[w];
-    for (uint32_t k = 0; k < nw; ++k) {
-        const uint32_t hi = w + 1 < n_ref ? s_ref[w + 1] : 0u;
-        uint32_t word = __builtin_amdgcn_alignbit(hi, lo, sh);  // bits [sh, sh + 32) of hi:lo
-        if (k + 1 == nw && tail) word &= (1u << (2u * tail)) - 1u;
-        row[2 + k] = word;
-        lo = hi;
-        ++w;
+    // s_ref holds one word past the staged range, so w + k + 1 is always readable
+    if constexpr (NW > 0) {
+        // 16-byte row: bases built in registers, stored as three 128-bit writes
+        uint32_t wd[NW];
+#pragma unroll
+        for (int k = 0; k < NW; ++k) wd[k] = __builtin_amdgcn_alignbit(s_ref[w + k + 1], s_ref[w + k], sh);
+        if (tail) wd[NW - 1] &= (1u << (2u * tail)) - 1u;
+        static_assert((NW + 2) % 4 == 0, "rows of whole 16-byte vectors");
+        uint4* r4 = reinterpret_cast<uint4*>(row);
+        r4[0] = uint4{x, 0u, wd[0], wd[1]};
+#pragma unroll
+        for (int q = 1; q < (NW + 2) / 4; ++q) r4[q] = uint4{wd[4 * q - 2], wd[4 * q - 1], wd[4 * q], wd[4 * q + 1]};
+    } else {
+        const uint32_t nw = (J.L + 15u) >> 4;
+        uint32_t lo = s_ref[w];
+        for (uint32_t k = 0; k < nw; ++k) {
+            const uint32_t hi = s_ref[w + k + 1];
+            uint32_t word = __builtin_amdgcn_alignbit(hi, lo, sh);  // bits [sh, sh + 32) of hi:lo
+            if (k + 1 == nw && tail) word &= (1u << (2u * tail)) - 1u;
+            row[2 + k] = word;
+            lo = hi;
+        }
     }
     uint32_t i = v.first(x);
     const uint32_t end = x + J.L;
@@ -492,14 +505,15 @@ __device__ __forceinline__ void write_read(const job_dev& J, const V& v, const l
         const uint32_t nb = sub < tb ? sub : sub + 1;
         row[2 + (o >> 4)] = (wv & ~(3u << s2)) | (nb << s2);
     }
-    row[0] = x;
+    if constexpr (NW == 0) row[0] = x;
     row[1] = leaf | (allele << 26) | (mate << 30) | ((ne > 0 ? 1u : 0u) << 31);
 }
 
 // STAGE: reads are built in LDS and copied out per pass (read_words small
 // enough for 64 rows per wave); otherwise each lane builds its read in place
-// in the output.
-template <int BLOCK, int CAP, bool STAGE>
+// in the output. NW > 0: reads of NW base words (rows of whole 16-byte
+// vectors, built in registers); NW = 0: any read size.
+template <int BLOCK, int CAP, bool STAGE, int NW>
 __global__ void __launch_bounds__(BLOCK) k_seq_reads(const seq_launch L) {
     constexpr uint32_t WAVES = BLOCK / 64;
     __shared__ uint32_t s_pos[CAP], s_lo[CAP], s_hi[CAP], s_info[CAP];
@@ -536,7 +550,7 @@ __global__ void __launch_bounds__(BLOCK) k_seq_reads(const seq_launch L) {
             // reference words covering [b, wend) plus one
             const uint32_t ref_w0 = s.b >> 4;
             const uint32_t n_ref = min(chr_words, ((s.wend + 15u) >> 4) + 1u) - ref_w0;
-            for (uint32_t i = tid; i < n_ref; i += BLOCK) s_ref[i] = J.packed[S.chr_woff + ref_w0 + i];
+            for (uint32_t i = tid; i <= n_ref; i += BLOCK) s_ref[i] = i < n_ref ? J.packed[S.chr_woff + ref_w0 + i] : 0u;
             if (fits) stage_window<BLOCK, CAP>(J, s, s_pos, s_lo, s_hi, s_info, nullptr, nullptr, s_bucket, tid);
             else __syncthreads();
             const lds_view lv{s_pos, s_lo, s_hi, s_info, nullptr, nullptr, s_bucket, s.n_win, s.b};
@@ -565,11 +579,11 @@ __global__ void __launch_bounds__(BLOCK) k_seq_reads(const seq_launch L) {
                     if (r < nj) {
                         const uint32_t x = mate ? x2 : x1;
                         if (fits)
-                            write_read(J, lv, s_tab, x, la & 0x3FFFFFFu, la >> 26, r, j, c2, mate, slots, BLOCK,
-                                       s_ref, ref_w0 << 4, n_ref, row);
+                            write_read<NW>(J, lv, s_tab, x, la & 0x3FFFFFFu, la >> 26, r, j, c2, mate, slots,
+                                           BLOCK, s_ref, ref_w0 << 4, row);
                         else
-                            write_read(J, gv, s_tab, x, la & 0x3FFFFFFu, la >> 26, r, j, c2, mate, slots, BLOCK,
-                                       s_ref, ref_w0 << 4, n_ref, row);
+                            write_read<NW>(J, gv, s_tab, x, la & 0x3FFFFFFu, la >> 26, r, j, c2, mate, slots,
+                                           BLOCK, s_ref, ref_w0 << 4, row);
                     }
                     if (STAGE) {
                         wave_sync();
