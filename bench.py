@@ -220,6 +220,39 @@ def serving_leg(ctx, rate: float, duration: float, saturated: int) -> dict:
     return out
 
 
+def cna_leg(ctx, steps: int) -> dict:
+    """The copy-number workload (count-only, purity 0.6)."""
+    from process_amd import core, workloads
+
+    t = time.perf_counter()
+    w = workloads.cna_purity()
+    built = workloads.build(ctx, w)
+    M = built.muts.n
+    ev = built.muts.cna()
+    seq = w.seq
+    S = core.num_out_samples(built.forest, **seq)
+    alt = core.DeviceArray(ctx, (S, M), np.int32)
+    dep = core.DeviceArray(ctx, (S, M), np.int32)
+    pres = core.DeviceArray(ctx, (S, M), np.uint8)
+    log(f"cna: {M} mutations, {len(ev)} copy-number events, built in {time.perf_counter() - t:.1f} s")
+    core.simulate_seq_device(ctx, built.genome, built.forest, built.muts, w.seq_seed, alt, dep, pres, **seq)
+    kms, n = [], 0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        n += core.simulate_seq_device(ctx, built.genome, built.forest, built.muts, w.seq_seed, alt, dep, pres, **seq)
+        kms.append(ctx.last_kernel_ms())
+    dt = (time.perf_counter() - t0) / steps
+    out = dict(workload="cna_purity: 1e5-cell two-species forest, one sample of 1e4 cells + normal, 100 Mb / 10 chr, "
+                        "12 allele-specific CNA segments (1-50 Mb, states 0/1/3/4) + WGD on the driver clone, "
+                        "purity 0.6, 500x PE 150 bp, error 1e-3, count-only",
+               value=n / steps / dt, unit="reads/s", reads_per_step=n // steps, ms_per_step=dt * 1e3,
+               kernel_ms=float(np.mean(kms)), mutations=M, cna_events=len(ev))
+    log(f"cna: {n // steps} reads/step, kernel {out['kernel_ms']:.2f} ms, {out['value']:.3e} reads/s")
+    for a in (alt, dep, pres):
+        a.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -233,6 +266,7 @@ def main() -> int:
     ap.add_argument("--batched-jobs", type=int, default=256)
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner in the batched workload")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving workload")
+    ap.add_argument("--no-cna", action="store_true", help="skip the copy-number workload")
     ap.add_argument("--serve-rate", type=float, default=20.0, help="request arrivals per second")
     ap.add_argument("--serve-seconds", type=float, default=20.0, help="length of the arrival stream")
     ap.add_argument("--serve-burst", type=int, default=200, help="requests of the saturated run (0 skips it)")
@@ -342,6 +376,9 @@ def main() -> int:
                                                bytes_per_read=bytes_per_read))
         log(f"reads mode: {n_all} reads, kernel {rk:.2f} ms, {ach:.0f} GB/s written ({ach / HBM_PEAK_GBS:.1%})")
         buf.close()
+
+    if not args.no_cna:
+        out["cna_purity"] = cna_leg(ctx, 5)
 
     if not args.no_batched:
         out["batched"] = batched_leg(ctx, args.batched_jobs, 3, not args.no_tune)

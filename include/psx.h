@@ -152,6 +152,44 @@ typedef struct psx_germline {
     uint8_t pad[2];
 } psx_germline;
 
+/* Copy-number driver of a mutant: CNA("A"/"D", chr, pos, len, allele,
+ * src_allele) and WholeGenomeDoubling of the reference (src/cna.hpp:28-94,
+ * src/mutations.cpp:355-365, applied in the mutant's driver order,
+ * src/mutation_engine.cpp:1185-1210). Copy numbers are kept per allele
+ * lineage: an amplification adds copies of an allele (src_allele in the
+ * reference), a deletion removes every copy of the allele in the region, a
+ * whole-genome doubling doubles every copy; the copies of a lineage carry
+ * its mutations. */
+enum { PSX_CNA_AMP = 0, PSX_CNA_DEL = 1, PSX_CNA_WGD = 2 };
+#define PSX_ALL_CHROMOSOMES 0xFFFFFFFFu
+#define PSX_MAX_CNA_PLOIDY 8u
+
+typedef struct psx_cna {
+    uint32_t species;  /* on the edge into every node where this species first appears on a lineage */
+    uint32_t chr;      /* ignored for PSX_CNA_WGD */
+    uint32_t start;    /* 0-based first base */
+    uint32_t len;      /* bases */
+    uint8_t allele;    /* allele lineage (AMP, DEL) */
+    uint8_t kind;      /* PSX_CNA_AMP / PSX_CNA_DEL / PSX_CNA_WGD */
+    uint8_t copies;    /* AMP: copies added */
+    uint8_t pad;
+} psx_cna;
+
+/* A placed copy-number event. Events are kept in DFS order of their node
+ * (an ancestor's events precede its descendants'), ties in driver order. */
+typedef struct psx_cna_event {
+    uint32_t node;     /* DFS index of the node whose incoming edge carries the event */
+    uint32_t leaf_lo;  /* applies to leaves [leaf_lo, leaf_hi) */
+    uint32_t leaf_hi;
+    uint32_t chr;      /* PSX_ALL_CHROMOSOMES for a whole-genome doubling */
+    uint32_t start;    /* [start, end) */
+    uint32_t end;
+    uint8_t allele;
+    uint8_t kind;
+    uint8_t copies;
+    uint8_t pad;
+} psx_cna_event;
+
 typedef struct psx_engine_params {
     uint32_t n_species;
     const double* snv_rate;     /* [n_species] SNVs per base per division */
@@ -172,6 +210,8 @@ typedef struct psx_engine_params {
     const psx_driver* drivers;
     uint32_t n_germline;
     const psx_germline* germline;
+    uint32_t n_cna;             /* copy-number drivers (ploidy <= PSX_MAX_CNA_PLOIDY) */
+    const psx_cna* cna;
 } psx_engine_params;
 
 /* One record per placed mutation; the table is sorted by SID
@@ -200,6 +240,10 @@ psx_status psx_place_mutations(psx_ctx* ctx, const psx_genome* g, const psx_fore
 psx_status psx_mutations_upload(psx_ctx* ctx, const psx_forest* f, const psx_mutation* recs,
                                 uint64_t n, uint32_t ploidy, psx_mutations** out);
 psx_status psx_mutations_count(const psx_mutations* m, uint64_t* n);
+/* Copy-number events of the table (placed, or set for a replay): *n gets
+ * their number; up to cap are copied to ev (ev may be NULL). */
+psx_status psx_mutations_cna(const psx_mutations* m, psx_cna_event* ev, uint32_t cap, uint32_t* n);
+psx_status psx_mutations_set_cna(psx_ctx* ctx, psx_mutations* m, const psx_cna_event* ev, uint32_t n);
 psx_status psx_mutations_download(psx_ctx* ctx, const psx_mutations* m, psx_mutation* recs,
                                   uint64_t n);
 void psx_mutations_free(psx_mutations* m);
@@ -229,9 +273,22 @@ psx_status psx_insert_size_check(psx_ctx* ctx, uint32_t mean, uint32_t stddev);
 
 /* Number of output samples: requested samples, plus one normal sample. */
 uint32_t psx_seq_num_samples(const psx_forest* f, const psx_seq_params* p);
-/* Reads the call simulates (closed form of the sampling model). */
+/* Reads the call simulates (closed form of the sampling model for mutation
+ * tables without copy-number events; with them the count is random and the
+ * sequencing calls report it). */
 psx_status psx_seq_num_reads(const psx_genome* g, const psx_forest* f, const psx_seq_params* p,
                              uint64_t* n_reads);
+
+/* Copy numbers. With copy-number events on a chromosome, a template draws
+ * its source cell and one of kmax copy slots (kmax = the chromosome's largest
+ * copy number over leaves and positions, at least the ploidy) and is kept
+ * when the slot is one of the source's copies at the template start: reads
+ * then come from allele lineages in proportion to their copies and depth
+ * scales with the total copy number (coverage is that of a genome with
+ * `ploidy` copies; the normal genome has `ploidy` copies everywhere, so
+ * normal admixture is weighted by DNA quantity). Both mates read the
+ * template's lineage. The number of templates drawn is scaled by
+ * kmax / ploidy; without events the model is the plain one, draw for draw. */
 
 /* Count-only sequencing (write_SAM = FALSE). Output arrays are host memory
  * of n_out_samples x n_mutations entries, sample-major: alt = reads showing

@@ -727,6 +727,52 @@ int orc_place_mutations(uint32_t n_chr, const uint32_t* len, const uint32_t* pac
     return 0;
 }
 
+/* ------------------------------------------------------ copy-number drivers */
+
+int orc_place_cna(uint32_t n_chr, const uint32_t* len, const orc_forest* f, const psx_engine_params* p,
+                  psx_cna_event** out, uint32_t* n_out) {
+    *out = NULL;
+    *n_out = 0;
+    if (p->n_cna && p->ploidy > PSX_MAX_CNA_PLOIDY) return fail("copy-number events need a small ploidy");
+    uint32_t cap = 0;
+    for (uint32_t k = 0; k < p->n_cna; ++k) {
+        const psx_cna* c = &p->cna[k];
+        int wgd = c->kind == PSX_CNA_WGD;
+        if (c->kind > PSX_CNA_WGD || c->species >= p->n_species ||
+            (!wgd && (c->chr >= n_chr || c->len == 0 || c->start >= len[c->chr] || c->len > len[c->chr] - c->start ||
+                      c->allele >= p->ploidy || (c->kind == PSX_CNA_AMP && c->copies == 0))))
+            return fail("invalid copy-number driver %u", k);
+        for (uint32_t v = 0; v < f->n_nodes; ++v) {
+            uint32_t par = f->parent[v];
+            if (f->species[v] == c->species && (par == PSX_NO_NODE || f->species[par] != c->species)) ++cap;
+        }
+    }
+    psx_cna_event* ev = (psx_cna_event*)calloc(cap ? cap : 1, sizeof(psx_cna_event));
+    uint32_t n = 0;
+    /* nodes in DFS order; at one node, drivers in their given order */
+    for (uint32_t v = 0; v < f->n_nodes; ++v) {
+        uint32_t par = f->parent[v];
+        for (uint32_t k = 0; k < p->n_cna; ++k) {
+            const psx_cna* c = &p->cna[k];
+            if (f->species[v] != c->species || (par != PSX_NO_NODE && f->species[par] == c->species)) continue;
+            int wgd = c->kind == PSX_CNA_WGD;
+            psx_cna_event* e = &ev[n++];
+            e->node = v;
+            e->leaf_lo = f->leaf_lo[v];
+            e->leaf_hi = f->leaf_hi[v];
+            e->chr = wgd ? PSX_ALL_CHROMOSOMES : c->chr;
+            e->start = wgd ? 0 : c->start;
+            e->end = wgd ? 0 : c->start + c->len;
+            e->allele = wgd ? 0 : c->allele;
+            e->kind = c->kind;
+            e->copies = c->kind == PSX_CNA_AMP ? c->copies : 0;
+        }
+    }
+    *out = ev;
+    *n_out = n;
+    return 0;
+}
+
 /* ---------------------------------------------------------------- sequencing */
 
 typedef struct seq_ctx {
@@ -741,7 +787,56 @@ typedef struct seq_ctx {
     uint64_t ins_T[4096];
     uint32_t err_n;
     uint64_t err_T[PSX_MAX_READ_ERRORS + 1];
+    const psx_cna_event* ev;  /* copy-number events, DFS order */
+    uint32_t n_ev;
 } seq_ctx;
+
+/* Copy numbers of the allele lineages of `leaf` at (c, x): one copy each,
+ * then, in order, every event of the leaf's lineage covering x; returns the
+ * total. */
+static uint32_t copies(const seq_ctx* s, uint32_t c, uint32_t x, uint32_t leaf, uint32_t* cn) {
+    for (uint32_t a = 0; a < s->ploidy; ++a) cn[a] = 1;
+    for (uint32_t i = 0; i < s->n_ev; ++i) {
+        const psx_cna_event* e = &s->ev[i];
+        if (leaf < e->leaf_lo || leaf >= e->leaf_hi) continue;
+        if (e->kind == PSX_CNA_WGD) {
+            for (uint32_t a = 0; a < s->ploidy; ++a) cn[a] *= 2;
+        } else if (e->chr == c && x >= e->start && x < e->end) {
+            if (e->kind == PSX_CNA_AMP) cn[e->allele] += e->copies; else cn[e->allele] = 0;
+        }
+    }
+    uint32_t t = 0;
+    for (uint32_t a = 0; a < s->ploidy; ++a) t += cn[a];
+    return t;
+}
+
+/* Events that act on chromosome c. */
+static int chr_has_events(const seq_ctx* s, uint32_t c) {
+    for (uint32_t i = 0; i < s->n_ev; ++i)
+        if (s->ev[i].chr == c || s->ev[i].chr == PSX_ALL_CHROMOSOMES) return 1;
+    return 0;
+}
+
+/* Largest copy number on chromosome c over every leaf and position (at
+ * least the ploidy), by brute force over the leaves at event leaf-range
+ * boundaries and the positions at event boundaries, between which the copy
+ * numbers are constant. */
+static uint32_t chr_kmax(const seq_ctx* s, uint32_t c, uint32_t len) {
+    uint32_t best = s->ploidy, cn[PSX_MAX_CNA_PLOIDY];
+    for (uint32_t i = 0; i <= s->n_ev; ++i)
+        for (uint32_t side = 0; side < 2; ++side) {
+            uint32_t leaf = i == s->n_ev ? 0 : (side ? s->ev[i].leaf_hi : s->ev[i].leaf_lo);
+            if (leaf >= s->f->n_leaves) continue;
+            for (uint32_t k = 0; k <= s->n_ev; ++k)
+                for (uint32_t side2 = 0; side2 < 2; ++side2) {
+                    uint32_t x = k == s->n_ev ? 0 : (s->ev[k].chr != c ? 0 : (side2 ? s->ev[k].end : s->ev[k].start));
+                    if (x >= len) continue;
+                    uint32_t t = copies(s, c, x, leaf, cn);
+                    if (t > best) best = t;
+                }
+        }
+    return best;
+}
 
 static int normal_carries(const seq_ctx* s, const psx_mutation* r) {
     return r->nature == PSX_GERMINAL || (r->nature == PSX_PRENEOPLASTIC && s->preneo_in_normal);
@@ -844,6 +939,14 @@ int orc_simulate_seq(uint32_t n_chr, const uint32_t* len, const orc_forest* f,
                      const psx_mutation* m, uint64_t n_mut, uint32_t ploidy,
                      const psx_seq_params* p, uint64_t seed, int32_t* alt, int32_t* depth,
                      uint8_t* present, uint64_t* n_reads) {
+    return orc_simulate_seq_cna(n_chr, len, f, m, n_mut, ploidy, NULL, 0, p, seed, alt, depth, present, n_reads);
+}
+
+int orc_simulate_seq_cna(uint32_t n_chr, const uint32_t* len, const orc_forest* f,
+                         const psx_mutation* m, uint64_t n_mut, uint32_t ploidy,
+                         const psx_cna_event* ev, uint32_t n_ev,
+                         const psx_seq_params* p, uint64_t seed, int32_t* alt, int32_t* depth,
+                         uint8_t* present, uint64_t* n_reads) {
     if (!(p->coverage >= 0.0) || !isfinite(p->coverage)) return fail("invalid coverage");
     if (p->read_size < 1) return fail("invalid read size");
     if (!(p->purity >= 0.0 && p->purity <= 1.0)) return fail("The purity must be in [0,1]");
@@ -854,7 +957,9 @@ int orc_simulate_seq(uint32_t n_chr, const uint32_t* len, const orc_forest* f,
     s->f = f; s->m = m; s->n_mut = n_mut; s->ploidy = ploidy; s->L = p->read_size;
     s->pe = p->insert_size_mean > 0; s->preneo_in_normal = p->preneoplastic_in_normal != 0;
     s->seed = seed; s->e = p->error_rate;
+    s->ev = ev; s->n_ev = n_ev;
     int rc = 0;
+    if (n_ev && ploidy > PSX_MAX_CNA_PLOIDY) rc = fail("copy-number events need a small ploidy");
     if (s->pe) rc = insert_table(p->insert_size_mean, p->insert_size_stddev, &s->ins_k0, &s->ins_n, s->ins_T);
     if (!rc && s->e > 0.0) rc = error_table(s->L, s->e, &s->err_n, s->err_T);
     /* chromosome set: sorted, unique (std::set semantics of get_relevant_chr_set) */
@@ -906,6 +1011,17 @@ int orc_simulate_seq(uint32_t n_chr, const uint32_t* len, const orc_forest* f,
             present[(uint64_t)so * n_mut + i] = (uint8_t)pres;
         }
     }
+    /* copy numbers: per chromosome, copy slots drawn per template */
+    uint32_t kmax[PSX_MAX_CHROMOSOMES];
+    uint8_t has_ev[PSX_MAX_CHROMOSOMES];
+    for (uint32_t c = 0; c < n_chr; ++c) {
+        has_ev[c] = (uint8_t)chr_has_events(s, c);
+        kmax[c] = has_ev[c] ? chr_kmax(s, c, len[c]) : ploidy;
+        if ((uint64_t)f->n_leaves * kmax[c] >= ((uint64_t)1 << 32)) {
+            free(req); free(s);
+            return fail("too many cell copies");
+        }
+    }
     /* strata */
     uint64_t n_strata = 0, reads = 0;
     for (uint32_t so = 0; so < n_out; ++so)
@@ -920,10 +1036,11 @@ int orc_simulate_seq(uint32_t n_chr, const uint32_t* len, const orc_forest* f,
             uint32_t J = (W + PSX_STRATUM - 1) / PSX_STRATUM;
             for (uint32_t j = 0; j < J; ++j) st[q++] = (stratum_ref){so, c, j};
             uint64_t T = (uint64_t)llround(p->coverage * (double)len[c] / (double)(s->pe ? 2 * s->L : s->L));
-            reads += T * (s->pe ? 2u : 1u);
+            if (!has_ev[c]) reads += T * (s->pe ? 2u : 1u);
         }
+    uint64_t accepted = 0;
 #ifdef _OPENMP
-#pragma omp parallel for schedule(dynamic, 8)
+#pragma omp parallel for schedule(dynamic, 8) reduction(+ : accepted)
 #endif
     for (int64_t qi = 0; qi < (int64_t)n_strata; ++qi) {
         uint32_t so = st[qi].so, c = st[qi].c, j = st[qi].j;
@@ -931,7 +1048,11 @@ int orc_simulate_seq(uint32_t n_chr, const uint32_t* len, const orc_forest* f,
         const uint32_t* sl = is_normal ? NULL : f->sample_leaves + f->sample_begin[req[so]];
         uint32_t ns = is_normal ? 0 : f->sample_begin[req[so] + 1] - f->sample_begin[req[so]];
         uint32_t W = len[c] - s->L + 1;
-        uint64_t T = (uint64_t)llround(p->coverage * (double)len[c] / (double)(s->pe ? 2 * s->L : s->L));
+        /* with copy-number events, one of kmax copy slots per cell is drawn,
+         * kmax / ploidy times as many templates */
+        uint32_t K = kmax[c];
+        uint64_t T = (uint64_t)llround(p->coverage * (double)len[c] / (double)(s->pe ? 2 * s->L : s->L) *
+                                       ((double)K / (double)ploidy));
         uint32_t b = j * PSX_STRATUM, e = b + PSX_STRATUM < W ? b + PSX_STRATUM : W;
         uint64_t nj = T * e / W - T * b / W;
         uint32_t c2 = (c << 16) | so;
@@ -942,13 +1063,32 @@ int orc_simulate_seq(uint32_t n_chr, const uint32_t* len, const orc_forest* f,
             philox4(seed, r, j, c2, (uint32_t)P_TEMPLATE << 24, w);
             uint32_t x = b + mulhi32(w[0], e - b);
             uint32_t leaf, allele;
-            if (is_normal || (uint64_t)w[1] < thr_normal) {
+            if (!has_ev[c]) {
+                if (is_normal || (uint64_t)w[1] < thr_normal) {
+                    leaf = f->n_leaves;
+                    allele = mulhi32(w[2], ploidy);
+                } else {
+                    uint32_t X = mulhi32(w[2], ns * ploidy);
+                    leaf = sl[X / ploidy];
+                    allele = X % ploidy;
+                }
+            } else if (is_normal || (uint64_t)w[1] < thr_normal) {
+                uint32_t k = mulhi32(w[2], K);
+                if (k >= ploidy) continue;  /* not a copy of the normal genome */
                 leaf = f->n_leaves;
-                allele = mulhi32(w[2], ploidy);
+                allele = k;
+                accepted += s->pe ? 2u : 1u;
             } else {
-                uint32_t X = mulhi32(w[2], ns * ploidy);
-                leaf = sl[X / ploidy];
-                allele = X % ploidy;
+                uint32_t X = mulhi32(w[2], ns * K);
+                uint32_t k = X % K;
+                leaf = sl[X / K];
+                uint32_t cn[PSX_MAX_CNA_PLOIDY];
+                uint32_t tot = copies(s, c, x, leaf, cn);
+                if (k >= tot) continue;  /* the cell has fewer copies at x */
+                allele = 0;
+                uint32_t cum = cn[0];
+                while (allele + 1 < ploidy && k >= cum) cum += cn[++allele];
+                accepted += s->pe ? 2u : 1u;
             }
             seq_read(s, c, x, leaf, allele, r, j, c2, 0, a_out, d_out);
             if (s->pe) {
@@ -961,7 +1101,7 @@ int orc_simulate_seq(uint32_t n_chr, const uint32_t* len, const orc_forest* f,
         }
     }
     free(st); free(req); free(s);
-    *n_reads = reads;
+    *n_reads = reads + accepted;
     return 0;
 }
 

@@ -55,6 +55,12 @@ int orc_place_mutations(uint32_t n_chr, const uint32_t* len, const uint32_t* pac
                         const orc_forest* f, const psx_engine_params* p, uint64_t seed,
                         psx_mutation** out, uint64_t* n_out);
 
+/* Copy-number drivers (psx_engine_params.cna) placed on the edge into every
+ * node where their species first appears, in DFS order of the node, ties in
+ * driver order (src/mutation_engine.cpp:1185-1210 for the driver order). */
+int orc_place_cna(uint32_t n_chr, const uint32_t* len, const orc_forest* f, const psx_engine_params* p,
+                  psx_cna_event** out, uint32_t* n_out);
+
 /* Sorts an arbitrary table into SID order and marks sid_head. */
 int orc_sort_table(psx_mutation* m, uint64_t n);
 
@@ -62,6 +68,14 @@ int orc_simulate_seq(uint32_t n_chr, const uint32_t* len, const orc_forest* f,
                      const psx_mutation* m, uint64_t n_mut, uint32_t ploidy,
                      const psx_seq_params* p, uint64_t seed, int32_t* alt, int32_t* depth,
                      uint8_t* present, uint64_t* n_reads);
+/* The same with copy-number events (DFS order): reads come from allele
+ * lineages in proportion to their copy numbers (the rejection model of
+ * include/psx.h); *n_reads counts the reads of accepted templates. */
+int orc_simulate_seq_cna(uint32_t n_chr, const uint32_t* len, const orc_forest* f,
+                         const psx_mutation* m, uint64_t n_mut, uint32_t ploidy,
+                         const psx_cna_event* ev, uint32_t n_ev,
+                         const psx_seq_params* p, uint64_t seed, int32_t* alt, int32_t* depth,
+                         uint8_t* present, uint64_t* n_reads);
 
 int orc_seq_table(const psx_mutation* recs, uint64_t n_mutations, uint32_t n_out_samples,
                   const int32_t* alt, const int32_t* depth, const uint8_t* present,

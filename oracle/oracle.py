@@ -44,6 +44,11 @@ def lib():
             "orc_place_mutations": (st, [C.c_uint32, A.u32p, A.u32p, C.POINTER(OrcForest),
                                          C.POINTER(A.EngineParams), C.c_uint64, C.POINTER(C.c_void_p), A.u64p]),
             "orc_sort_table": (st, [vp, C.c_uint64]),
+            "orc_place_cna": (st, [C.c_uint32, A.u32p, C.POINTER(OrcForest), C.POINTER(A.EngineParams),
+                                   C.POINTER(C.c_void_p), A.u32p]),
+            "orc_simulate_seq_cna": (st, [C.c_uint32, A.u32p, C.POINTER(OrcForest), vp, C.c_uint64, C.c_uint32,
+                                          vp, C.c_uint32, C.POINTER(A.SeqParams), C.c_uint64, A.i32p, A.i32p,
+                                          A.u8p, A.u64p]),
             "orc_simulate_seq": (st, [C.c_uint32, A.u32p, C.POINTER(OrcForest), vp, C.c_uint64, C.c_uint32,
                                       C.POINTER(A.SeqParams), C.c_uint64, A.i32p, A.i32p, A.u8p, A.u64p]),
             "orc_seq_table": (st, [vp, C.c_uint64, C.c_uint32, A.i32p, A.i32p, A.u8p, C.c_int32, A.u64p, A.u64p,
@@ -152,13 +157,28 @@ def place_mutations(lens, packed, forest: Forest, seed: int, **engine) -> np.nda
     return arr
 
 
+def place_cna(lens, forest: Forest, **engine) -> np.ndarray:
+    """Copy-number drivers placed on the forest (CNA_EVENT_DTYPE, DFS order)."""
+    L = np.ascontiguousarray(lens, np.uint32)
+    params, keep = A.engine_params(**engine)
+    out = C.c_void_p()
+    n = C.c_uint32()
+    _check(lib().orc_place_cna(len(L), A.ptr(L, C.c_uint32), forest.p, C.byref(params), C.byref(out), C.byref(n)))
+    nb = n.value * A.CNA_EVENT_DTYPE.itemsize
+    arr = np.frombuffer(C.string_at(out, nb), dtype=A.CNA_EVENT_DTYPE).copy() if n.value else \
+        np.zeros(0, A.CNA_EVENT_DTYPE)
+    lib().orc_free(out)
+    del keep
+    return arr
+
+
 def sort_table(recs: np.ndarray) -> np.ndarray:
     r = np.ascontiguousarray(recs, A.MUTATION_DTYPE).copy()
     _check(lib().orc_sort_table(r.ctypes.data_as(C.c_void_p), len(r)))
     return r
 
 
-def simulate_seq(lens, forest: Forest, recs: np.ndarray, seed: int, ploidy: int = 2, **seq):
+def simulate_seq(lens, forest: Forest, recs: np.ndarray, seed: int, ploidy: int = 2, cna=None, **seq):
     L = np.ascontiguousarray(lens, np.uint32)
     r = np.ascontiguousarray(recs, A.MUTATION_DTYPE)
     params, keep = A.seq_params(**seq)
@@ -169,9 +189,11 @@ def simulate_seq(lens, forest: Forest, recs: np.ndarray, seed: int, ploidy: int 
     dep = np.zeros(S * M, np.int32)
     pres = np.zeros(S * M, np.uint8)
     nr = C.c_uint64()
-    _check(lib().orc_simulate_seq(len(L), A.ptr(L, C.c_uint32), forest.p, r.ctypes.data_as(C.c_void_p), M, ploidy,
-                                  C.byref(params), C.c_uint64(seed), A.ptr(alt, C.c_int32), A.ptr(dep, C.c_int32),
-                                  A.ptr(pres, C.c_uint8), C.byref(nr)))
+    ev = np.ascontiguousarray(cna if cna is not None else np.zeros(0, A.CNA_EVENT_DTYPE), A.CNA_EVENT_DTYPE)
+    _check(lib().orc_simulate_seq_cna(len(L), A.ptr(L, C.c_uint32), forest.p, r.ctypes.data_as(C.c_void_p), M,
+                                      ploidy, ev.ctypes.data_as(C.c_void_p) if len(ev) else None, len(ev),
+                                      C.byref(params), C.c_uint64(seed), A.ptr(alt, C.c_int32),
+                                      A.ptr(dep, C.c_int32), A.ptr(pres, C.c_uint8), C.byref(nr)))
     del keep
     return alt.reshape(S, M), dep.reshape(S, M), pres.reshape(S, M), nr.value
 

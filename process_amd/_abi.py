@@ -40,6 +40,20 @@ class Germline(C.Structure):
                 ("pad", C.c_uint8 * 2)]
 
 
+PSX_CNA_AMP, PSX_CNA_DEL, PSX_CNA_WGD = 0, 1, 2
+PSX_ALL_CHROMOSOMES = 0xFFFFFFFF
+
+
+class Cna(C.Structure):
+    _fields_ = [("species", C.c_uint32), ("chr", C.c_uint32), ("start", C.c_uint32), ("len", C.c_uint32),
+                ("allele", C.c_uint8), ("kind", C.c_uint8), ("copies", C.c_uint8), ("pad", C.c_uint8)]
+
+
+CNA_EVENT_DTYPE = np.dtype([("node", "<u4"), ("leaf_lo", "<u4"), ("leaf_hi", "<u4"), ("chr", "<u4"),
+                            ("start", "<u4"), ("end", "<u4"), ("allele", "u1"), ("kind", "u1"), ("copies", "u1"),
+                            ("pad", "u1")])
+
+
 class EngineParams(C.Structure):
     _fields_ = [("n_species", C.c_uint32), ("snv_rate", f64p), ("indel_rate", f64p),
                 ("n_sbs", C.c_uint32), ("sbs", f64p), ("n_epochs", C.c_uint32), ("epoch_start", f64p),
@@ -47,7 +61,8 @@ class EngineParams(C.Structure):
                 ("n_preneoplastic_indel", C.c_uint32), ("preneoplastic_sbs", C.c_uint32),
                 ("indel_max_len", C.c_uint32), ("indel_len_p", C.c_double), ("infinite_sites", C.c_int32),
                 ("ploidy", C.c_uint32), ("n_drivers", C.c_uint32), ("drivers", C.POINTER(Driver)),
-                ("n_germline", C.c_uint32), ("germline", C.POINTER(Germline))]
+                ("n_germline", C.c_uint32), ("germline", C.POINTER(Germline)),
+                ("n_cna", C.c_uint32), ("cna", C.POINTER(Cna))]
 
 
 class SeqParams(C.Structure):
@@ -124,7 +139,8 @@ def forest_desc(cell_id, parent, species, sample, birth, n_samples) -> tuple[For
 
 def engine_params(*, snv_rate, indel_rate, sbs, epoch_start, exposure, n_pre_snv=0, n_pre_indel=0,
                   pre_sbs=0, indel_max_len=10, indel_len_p=0.5, infinite_sites=True, ploidy=2,
-                  drivers=(), germline=()) -> tuple[EngineParams, list]:
+                  drivers=(), germline=(), cna=()) -> tuple[EngineParams, list]:
+    """cna: (species, chr, start, len, allele, kind, copies) tuples (kind PSX_CNA_*)."""
     snv = np.ascontiguousarray(snv_rate, np.float64)
     ind = np.ascontiguousarray(indel_rate, np.float64)
     sbs_a = np.ascontiguousarray(sbs, np.float64).reshape(-1, 96)
@@ -136,12 +152,15 @@ def engine_params(*, snv_rate, indel_rate, sbs, epoch_start, exposure, n_pre_snv
     grm = (Germline * max(1, len(germline)))()
     for i, g in enumerate(germline):
         grm[i] = Germline(int(g[0]), int(g[1]), int(g[2]), int(g[3]))
+    cn = (Cna * max(1, len(cna)))()
+    for i, c in enumerate(cna):
+        cn[i] = Cna(*(int(v) for v in c[:7]))
     p = EngineParams(len(snv), ptr(snv, C.c_double), ptr(ind, C.c_double), sbs_a.shape[0], ptr(sbs_a, C.c_double),
                      len(es), ptr(es, C.c_double), ptr(np.ascontiguousarray(ex), C.c_double), int(n_pre_snv),
                      int(n_pre_indel), int(pre_sbs), int(indel_max_len), float(indel_len_p), int(bool(infinite_sites)),
                      int(ploidy), len(drivers), C.cast(drv, C.POINTER(Driver)), len(germline),
-                     C.cast(grm, C.POINTER(Germline)))
-    return p, [snv, ind, sbs_a, es, ex, drv, grm]
+                     C.cast(grm, C.POINTER(Germline)), len(cna), C.cast(cn, C.POINTER(Cna)))
+    return p, [snv, ind, sbs_a, es, ex, drv, grm, cn]
 
 
 def seq_params(*, coverage=10.0, read_size=150, insert_size_mean=0, insert_size_stddev=10, error_rate=0.0,

@@ -26,6 +26,7 @@ def _declare(lib):
     vp = C.c_void_p
     pp = C.POINTER(C.c_void_p)
     st = C.c_int
+    u32p_ = C.POINTER(C.c_uint32)
     sig = {
         "psx_abi_version": (C.c_int, []),
         "psx_ctx_create": (st, [C.c_int, pp]),
@@ -52,6 +53,8 @@ def _declare(lib):
         "psx_forest_free": (None, [vp]),
         "psx_place_mutations": (st, [vp, vp, vp, C.POINTER(A.EngineParams), C.c_uint64, pp]),
         "psx_mutations_upload": (st, [vp, vp, vp, C.c_uint64, C.c_uint32, pp]),
+        "psx_mutations_cna": (st, [vp, vp, C.c_uint32, u32p_]),
+        "psx_mutations_set_cna": (st, [vp, vp, vp, C.c_uint32]),
         "psx_mutations_count": (st, [vp, A.u64p]),
         "psx_mutations_download": (st, [vp, vp, vp, C.c_uint64]),
         "psx_mutations_free": (None, [vp]),

@@ -212,6 +212,20 @@ class Mutations(_Handle):
                                          C.byref(h)), ctx.h)
         return cls(ctx, h, ploidy)
 
+    def cna(self) -> np.ndarray:
+        """Copy-number events (CNA_EVENT_DTYPE), in DFS order of their node."""
+        n = C.c_uint32()
+        check(lib().psx_mutations_cna(self.h, None, 0, C.byref(n)))
+        out = np.zeros(n.value, A.CNA_EVENT_DTYPE)
+        if n.value:
+            check(lib().psx_mutations_cna(self.h, out.ctypes.data_as(C.c_void_p), n.value, C.byref(n)))
+        return out
+
+    def set_cna(self, events: np.ndarray) -> None:
+        ev = np.ascontiguousarray(events, A.CNA_EVENT_DTYPE)
+        check(lib().psx_mutations_set_cna(self.ctx.h, self.h, ev.ctypes.data_as(C.c_void_p) if len(ev) else None,
+                                          len(ev)), self.ctx.h)
+
     def download(self) -> np.ndarray:
         out = np.zeros(self.n, A.MUTATION_DTYPE)
         check(lib().psx_mutations_download(self.ctx.h, self.h, out.ctypes.data_as(C.c_void_p), self.n), self.ctx.h)

@@ -272,6 +272,38 @@ extern "C" psx_status psx_mutations_count(const psx_mutations* m, uint64_t* n) {
     return PSX_OK;
 }
 
+void psx_cna_check(const std::vector<psx_cna_event>& ev, uint32_t n_leaves, uint32_t ploidy,
+                   const std::vector<uint32_t>& chr_len) {
+    if (!ev.empty() && ploidy > PSX_MAX_CNA_PLOIDY)
+        psx_throw(PSX_EINVAL, "copy-number events need a ploidy of at most %u", PSX_MAX_CNA_PLOIDY);
+    for (size_t i = 0; i < ev.size(); ++i) {
+        const psx_cna_event& e = ev[i];
+        const bool wgd = e.kind == PSX_CNA_WGD;
+        if (e.kind > PSX_CNA_WGD || e.leaf_lo > e.leaf_hi || e.leaf_hi > n_leaves ||
+            (!wgd && (e.chr >= chr_len.size() || e.start >= e.end || e.end > chr_len[e.chr] || e.allele >= ploidy ||
+                      (e.kind == PSX_CNA_AMP && e.copies == 0))) ||
+            (wgd && e.chr != PSX_ALL_CHROMOSOMES) || (i && ev[i - 1].node > e.node))
+            psx_throw(PSX_EINVAL, "invalid copy-number event %zu", i);
+    }
+}
+
+extern "C" psx_status psx_mutations_cna(const psx_mutations* m, psx_cna_event* ev, uint32_t cap, uint32_t* n) {
+    if (!m || !n) return PSX_EINVAL;
+    *n = (uint32_t)m->cna.size();
+    if (ev)
+        for (uint32_t i = 0; i < cap && i < *n; ++i) ev[i] = m->cna[i];
+    return PSX_OK;
+}
+
+extern "C" psx_status psx_mutations_set_cna(psx_ctx* ctx, psx_mutations* m, const psx_cna_event* ev, uint32_t n) {
+    if (!ctx || !m || (!ev && n)) return PSX_EINVAL;
+    return psx_guard(ctx, [&] {
+        std::vector<psx_cna_event> v(ev, ev + n);
+        psx_cna_check(v, m->n_leaves, m->ploidy, m->chr_len);
+        m->cna = std::move(v);
+    });
+}
+
 extern "C" psx_status psx_mutations_download(psx_ctx* ctx, const psx_mutations* m, psx_mutation* recs,
                                              uint64_t n) {
     if (!ctx || !m || (!recs && m->n)) return PSX_EINVAL;

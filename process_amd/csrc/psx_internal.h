@@ -115,7 +115,12 @@ struct psx_mutations {
     dbuf d_recs;                        // psx_mutation[n]
     dbuf d_pos, d_lo, d_hi, d_info;     // uint32[n]
     dbuf d_bucket;                      // per chromosome: first record at or after k * PSX_STRATUM
+    std::vector<psx_cna_event> cna;     // copy-number events in DFS order of their node
 };
+
+// Validates copy-number events against a forest and the chromosome lengths.
+void psx_cna_check(const std::vector<psx_cna_event>& ev, uint32_t n_leaves, uint32_t ploidy,
+                   const std::vector<uint32_t>& chr_len);
 
 // info word of a device mutation record
 #define PSX_INFO_ALLELE(i) ((i) & 0xFFu)

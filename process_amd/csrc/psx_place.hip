@@ -395,6 +395,36 @@ extern "C" psx_status psx_place_mutations(psx_ctx* ctx, const psx_genome* g, con
                 gm.alt == base_at(gm.chr, gm.pos))
                 psx_throw(PSX_EINVAL, "invalid germline mutation %u", k);
         }
+        // copy-number drivers: on the edge into every node where the species first appears
+        std::vector<psx_cna_event> cna;
+        for (uint32_t k = 0; k < p->n_cna; ++k) {
+            const psx_cna& c = p->cna[k];
+            const bool wgd = c.kind == PSX_CNA_WGD;
+            if (c.kind > PSX_CNA_WGD || c.species >= p->n_species ||
+                (!wgd && (c.chr >= g->n_chr || c.len == 0 || c.start >= g->len[c.chr] ||
+                          c.len > g->len[c.chr] - c.start || c.allele >= p->ploidy ||
+                          (c.kind == PSX_CNA_AMP && c.copies == 0))))
+                psx_throw(PSX_EINVAL, "invalid copy-number driver %u", k);
+            for (uint32_t v = 0; v < f->n_nodes; ++v) {
+                const uint32_t par = f->parent[v];
+                if (f->species[v] != c.species || (par != PSX_NO_NODE && f->species[par] == c.species)) continue;
+                psx_cna_event e{};
+                e.node = v;
+                e.leaf_lo = f->leaf_lo[v];
+                e.leaf_hi = f->leaf_hi[v];
+                e.chr = wgd ? PSX_ALL_CHROMOSOMES : c.chr;
+                e.start = wgd ? 0u : c.start;
+                e.end = wgd ? 0u : c.start + c.len;
+                e.allele = wgd ? 0 : c.allele;
+                e.kind = c.kind;
+                e.copies = c.kind == PSX_CNA_AMP ? c.copies : 0;
+                cna.push_back(e);
+            }
+        }
+        std::stable_sort(cna.begin(), cna.end(),
+                         [](const psx_cna_event& a, const psx_cna_event& b) { return a.node < b.node; });
+        psx_cna_check(cna, f->n_leaves, p->ploidy, g->len);
+        mt->cna = std::move(cna);
         std::vector<uint32_t> epoch(f->n_nodes, 0);
         for (uint32_t v = 0; v < f->n_nodes; ++v)
             for (uint32_t e = 0; e < p->n_epochs; ++e)

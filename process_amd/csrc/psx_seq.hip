@@ -23,9 +23,31 @@ struct seq_batch {
     uint64_t n_reads = 0, pres_total = 0;
     scratch_builder sb;
     std::vector<size_t> tab_off;  // per job: insert thresholds, insert guide, error thresholds
-    size_t o_jobs = 0, o_segs = 0, o_items = 0, o_pres = 0;
+    std::vector<size_t> cna_off;  // per job: copy-number events (SIZE_MAX: none)
+    std::vector<uint64_t> job_reads;  // per job: reads of the sampling model without copy-number events
+    size_t o_jobs = 0, o_segs = 0, o_items = 0, o_pres = 0, o_acc = 0;
     uint8_t* dev = nullptr;       // device base of the last upload
 };
+
+// Largest copy number of any leaf on a chromosome (at least the ploidy): the
+// copy numbers change only at event boundaries, in leaf rank and position.
+static uint32_t cna_kmax(const std::vector<psx_cna_dev>& ev, uint32_t n_leaves, uint32_t len, uint32_t ploidy) {
+    std::vector<uint32_t> leaves{0}, xs{0};
+    for (const psx_cna_dev& e : ev) {
+        leaves.push_back(e.lo);
+        if (e.hi < n_leaves) leaves.push_back(e.hi);
+        xs.push_back(e.start);
+        if (e.end < len) xs.push_back(e.end);
+    }
+    uint32_t best = ploidy, cn[PSX_MAX_CNA_PLOIDY];
+    for (uint32_t l : leaves) {
+        if (l >= n_leaves) continue;
+        for (uint32_t x : xs) best = std::max(best, psx_cna_copies(ev.data(), (uint32_t)ev.size(), l, x, ploidy, cn));
+    }
+    return best;
+}
+
+bool batch_has_cna(const seq_batch* b);
 
 uint32_t out_samples(const psx_forest* f, const psx_seq_params* p) {
     return (p->samples ? p->n_samples : f->n_samples) + (p->with_normal_sample ? 1u : 0u);
@@ -50,6 +72,7 @@ void prepare_job(psx_ctx* ctx, seq_batch& P, const job_in& in, uint32_t spi) {
     if (m->n_leaves != f->n_leaves) psx_throw(PSX_EINVAL, "the mutation table belongs to another forest");
     const uint32_t ploidy = m->ploidy;
     if ((uint64_t)f->n_leaves * ploidy >= (uint64_t(1) << 32)) psx_throw(PSX_EINVAL, "too many cell copies");
+    psx_cna_check(m->cna, f->n_leaves, ploidy, g->len);
     const uint32_t L = p->read_size;
     const bool pe = p->insert_size_mean > 0;
     psx_ctable ins, err;
@@ -98,7 +121,25 @@ void prepare_job(psx_ctx* ctx, seq_batch& P, const job_in& in, uint32_t spi) {
     P.tab_off.push_back(ins.T.empty() ? SIZE_MAX : P.sb.put_vec(ins.T));
     P.tab_off.push_back(ins.guide.empty() ? SIZE_MAX : P.sb.put_vec(ins.guide));
     P.tab_off.push_back(err.T.empty() ? SIZE_MAX : P.sb.put_vec(err.T));
+    // copy-number events per chromosome (doublings on every chromosome), in DFS order
+    std::vector<psx_cna_dev> cdev;
+    std::vector<uint32_t> c_off(g->n_chr, 0), c_n(g->n_chr, 0), c_kmax(g->n_chr, ploidy);
+    for (uint32_t c = 0; c < g->n_chr && !m->cna.empty(); ++c) {
+        std::vector<psx_cna_dev> ev;
+        for (const psx_cna_event& e : m->cna)
+            if (e.chr == c || e.chr == PSX_ALL_CHROMOSOMES)
+                ev.push_back(psx_cna_dev{e.leaf_lo, e.leaf_hi, e.chr == c ? e.start : 0u,
+                                         e.chr == c ? e.end : g->len[c], psx_cna_code(e)});
+        c_off[c] = (uint32_t)cdev.size();
+        c_n[c] = (uint32_t)ev.size();
+        c_kmax[c] = cna_kmax(ev, f->n_leaves, g->len[c], ploidy);
+        if ((uint64_t)f->n_leaves * c_kmax[c] >= (uint64_t(1) << 32))
+            psx_throw(PSX_EINVAL, "too many cell copies (%u copies on chromosome %u)", c_kmax[c], c);
+        cdev.insert(cdev.end(), ev.begin(), ev.end());
+    }
+    P.cna_off.push_back(cdev.empty() ? SIZE_MAX : P.sb.put_vec(cdev));
     P.jobs.push_back(J);
+    const uint64_t reads_before = P.n_reads;
     for (uint32_t so = 0; so < n_out; ++so) {
         const bool normal = so == n_req;
         const uint32_t leaf_off = normal ? 0u : f->sample_begin[req[so]];
@@ -110,7 +151,12 @@ void prepare_job(psx_ctx* ctx, seq_batch& P, const job_in& in, uint32_t spi) {
         for (uint32_t c = 0; c < g->n_chr; ++c) {
             if (!sel[c] || g->len[c] < L) continue;
             seg_dev S{};
-            S.T = (uint64_t)std::llround(p->coverage * (double)g->len[c] / (double)(pe ? 2 * L : L));
+            S.n_cna = c_n[c];
+            S.cna_off = c_off[c];
+            S.kmax = c_kmax[c];
+            // with copy-number events a template draws one of kmax copy slots per cell
+            S.T = (uint64_t)std::llround(p->coverage * (double)g->len[c] / (double)(pe ? 2 * L : L) *
+                                         ((double)S.kmax / (double)ploidy));
             S.out_off = (uint64_t)so * m->n;
             S.read_base = P.n_reads;
             S.chr_woff = g->woff[c];
@@ -120,7 +166,7 @@ void prepare_job(psx_ctx* ctx, seq_batch& P, const job_in& in, uint32_t spi) {
             S.bucket_off = (uint32_t)m->bucket_off[c];
             S.n_bucket = (uint32_t)(m->bucket_off[c + 1] - m->bucket_off[c]);
             S.leaf_off = leaf_off; S.n_cells = n_cells;
-            P.n_reads += S.T * (pe ? 2u : 1u);
+            P.n_reads += S.T * (pe ? 2u : 1u);  // template draws: an upper bound with copy-number events
             const uint32_t seg = (uint32_t)P.segs.size();
             P.segs.push_back(S);
             if (S.T == 0) continue;
@@ -128,10 +174,13 @@ void prepare_job(psx_ctx* ctx, seq_batch& P, const job_in& in, uint32_t spi) {
                 P.items.push_back(item_dev{seg, j0, std::min(S.J, j0 + spi), 0});
         }
     }
+    P.job_reads.push_back(P.n_reads - reads_before);
 }
 
 // Places the descriptors of a prepared batch after the per-job tables.
 void layout(seq_batch& P) {
+    const std::vector<uint64_t> zeros(P.jobs.size(), 0);
+    P.o_acc = P.sb.put_vec(zeros);
     P.o_segs = P.sb.put_vec(P.segs);
     P.o_items = P.sb.put_vec(P.items);
     P.o_pres = P.sb.put_vec(P.pres);
@@ -139,13 +188,16 @@ void layout(seq_batch& P) {
 }
 
 template <int BLOCK, int CAP>
-void launch_count_t(const seq_lane& lane, const seq_launch& L, uint32_t grid) {
-    hipLaunchKernelGGL((k_seq_count<BLOCK, CAP>), dim3(grid), dim3(BLOCK), 0, lane.stream, L);
+void launch_count_t(const seq_lane& lane, const seq_launch& L, uint32_t grid, bool cna) {
+    if (cna)
+        hipLaunchKernelGGL((k_seq_count<BLOCK, CAP, true>), dim3(grid), dim3(BLOCK), 0, lane.stream, L);
+    else
+        hipLaunchKernelGGL((k_seq_count<BLOCK, CAP, false>), dim3(grid), dim3(BLOCK), 0, lane.stream, L);
 }
 
-void launch_count(const seq_lane& lane, const psx_launch_config& c, const seq_launch& L, uint32_t grid) {
+void launch_count(const seq_lane& lane, const psx_launch_config& c, const seq_launch& L, uint32_t grid, bool cna) {
 #define PSX_CASE(B, C) \
-    if (c.block_threads == B && c.lds_records == C) return launch_count_t<B, C>(lane, L, grid);
+    if (c.block_threads == B && c.lds_records == C) return launch_count_t<B, C>(lane, L, grid, cna);
     PSX_CASE(256, 512) PSX_CASE(256, 1024) PSX_CASE(256, 2048)
     PSX_CASE(512, 512) PSX_CASE(512, 1024) PSX_CASE(512, 2048)
 #undef PSX_CASE
@@ -174,6 +226,22 @@ seq_batch* batch_prepare(psx_ctx* ctx, const job_in* jobs, size_t n_jobs, uint32
 void batch_free(seq_batch* b) { delete b; }
 size_t batch_desc_bytes(const seq_batch* b) { return b->sb.host.size() + 256; }
 uint64_t batch_reads(const seq_batch* b) { return b->n_reads; }
+bool batch_has_cna(const seq_batch* b) {
+    for (size_t off : b->cna_off)
+        if (off != SIZE_MAX) return true;
+    return false;
+}
+
+std::vector<uint64_t> batch_job_reads(const seq_batch* b, const seq_lane& lane) {
+    std::vector<uint64_t> r = b->job_reads;
+    if (!batch_has_cna(b)) return r;
+    std::vector<uint64_t> acc(b->jobs.size());
+    PSX_HIP(hipMemcpyAsync(acc.data(), lane.desc + b->o_acc, 8 * acc.size(), hipMemcpyDeviceToHost, lane.stream));
+    PSX_HIP(hipStreamSynchronize(lane.stream));
+    for (size_t k = 0; k < r.size(); ++k)
+        if (b->cna_off[k] != SIZE_MAX) r[k] = acc[k];
+    return r;
+}
 size_t batch_items(const seq_batch* b) { return b->items.size(); }
 
 void batch_upload(seq_batch* P, const seq_lane& lane) {
@@ -185,6 +253,8 @@ void batch_upload(seq_batch* P, const seq_lane& lane) {
         hj[k].ins_T = (const uint32_t*)at(P->tab_off[3 * k]);
         hj[k].ins_guide = (const uint16_t*)at(P->tab_off[3 * k + 1]);
         hj[k].err_T = (const uint32_t*)at(P->tab_off[3 * k + 2]);
+        hj[k].cna = (const psx_cna_dev*)at(P->cna_off[k]);
+        hj[k].acc = P->cna_off[k] == SIZE_MAX ? nullptr : (unsigned long long*)(base + P->o_acc) + k;
     }
     if (!P->sb.host.empty())
         PSX_HIP(hipMemcpyAsync(base, P->sb.host.data(), P->sb.host.size(), hipMemcpyHostToDevice, lane.stream));
@@ -199,9 +269,9 @@ void batch_launch(const seq_batch* P, const seq_lane& lane, const psx_launch_con
     const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, P->items.size()));
     const uint32_t init_grid = (uint32_t)std::max<uint64_t>(1, (P->pres_total + 255) / 256);
     hipLaunchKernelGGL(k_seq_init, dim3(init_grid), dim3(256), 0, lane.stream, L.jobs, ps, (uint32_t)P->pres.size(),
-                       P->pres_total, L.queue);
+                       P->pres_total, L.queue, (uint32_t)P->jobs.size());
     if (timed) PSX_HIP(hipEventRecord(lane.ev0, lane.stream));
-    launch_count(lane, cfg, L, grid);
+    launch_count(lane, cfg, L, grid, batch_has_cna(P));
     PSX_HIP(hipGetLastError());
     if (timed) PSX_HIP(hipEventRecord(lane.ev1, lane.stream));
 }
@@ -271,7 +341,9 @@ run_result run_count(psx_ctx* ctx, const std::vector<job_in>& jobs, const psx_la
     float ms = 0.f;
     PSX_HIP(hipEventElapsedTime(&ms, lane.ev0, lane.ev1));
     ctx->last_kernel_ms = ms;
-    return run_result{batch_reads(P.get()), (double)ms};
+    uint64_t n = 0;
+    for (uint64_t r : batch_job_reads(P.get(), lane)) n += r;
+    return run_result{n, (double)ms};
 }
 
 std::vector<job_in> jobs_of(const psx_seq_job* jobs, uint32_t n_jobs) {
@@ -366,6 +438,8 @@ extern "C" psx_status psx_simulate_seq_reads(psx_ctx* ctx, const psx_genome* g, 
         if (m->ploidy > 16) psx_throw(PSX_EINVAL, "read records hold copy indices below 16 (ploidy %u)", m->ploidy);
         if (f->n_leaves >= (1u << 26))
             psx_throw(PSX_EINVAL, "read records hold leaf ranks below 2^26 (the forest has %u leaves)", f->n_leaves);
+        if (!m->cna.empty())
+            psx_throw(PSX_EUNSUPPORTED, "read materialisation of tables with copy-number events is not supported");
         std::unique_ptr<seq_batch, void (*)(seq_batch*)> B(batch_prepare(ctx, &in, 1, 1), batch_free);
         seq_batch& P = *B;
         if (P.n_reads > capacity_reads)
@@ -483,7 +557,11 @@ extern "C" psx_status psx_seq_plan_run(psx_seq_plan* P, uint64_t* n_reads) {
         float ms = 0.f;
         PSX_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
         ctx->last_kernel_ms = ms;
-        if (n_reads) *n_reads = batch_reads(P->batch.get());
+        if (n_reads) {
+            uint64_t n = 0;
+            for (uint64_t r : batch_job_reads(P->batch.get(), P->lane)) n += r;
+            *n_reads = n;
+        }
     });
 }
 

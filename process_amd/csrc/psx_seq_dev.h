@@ -13,6 +13,7 @@
 // in LDS, counts go to LDS counters and the block flushes them with one global
 // atomic per touched record. All counting is integer.
 #pragma once
+#include "psx_cna.h"
 #include "psx_internal.h"
 
 namespace psxk {
@@ -58,6 +59,8 @@ struct job_dev {
     uint32_t L, pe, ploidy, n_leaves, preneo, reach;
     uint32_t n_out;
     uint32_t purity_lt1;
+    const psx_cna_dev* cna;      // copy-number events of every chromosome, per segment ranges
+    unsigned long long* acc;     // reads of accepted templates (jobs with copy-number events)
 };
 
 struct seg_dev {
@@ -68,6 +71,8 @@ struct seg_dev {
     uint32_t job, so, chr, len;
     uint32_t W, J, bucket_off, n_bucket;
     uint32_t leaf_off, n_cells;  // n_cells == 0: normal sample
+    uint32_t cna_off, n_cna;     // the chromosome's copy-number events (J.cna + cna_off)
+    uint32_t kmax, pad;          // copies drawn per template: the largest copy number, >= ploidy
 };
 
 struct item_dev {
@@ -145,10 +150,34 @@ struct glb_view {  // window larger than the LDS capacity
 };
 
 // Template: start of mate 1 and mate 2, source cell (resolved to a leaf on
-// first use) and copy.
+// first use) and copy; ok == 0 when a copy-number draw rejects it.
 struct tmpl {
-    uint32_t x, x2, cell, allele, leaf;
+    uint32_t x, x2, cell, allele, leaf, ok;
 };
+
+// Copy numbers of a leaf at base x of the segment's chromosome, from its
+// events read with scalar loads (the list is the same for every lane);
+// returns the total. Oracle: copies.
+__device__ __forceinline__ uint32_t cna_copies(const psx_cna_dev* ev, uint32_t n, uint32_t leaf, uint32_t x,
+                                               uint32_t ploidy, uint32_t* cn) {
+#pragma unroll
+    for (uint32_t a = 0; a < PSX_MAX_CNA_PLOIDY; ++a) cn[a] = a < ploidy ? 1u : 0u;
+    for (uint32_t i = 0; i < n; ++i) {
+        const psx_cna_dev e = uniform_at(ev, i);
+        const bool hit = leaf >= e.lo && leaf < e.hi && x >= e.start && x < e.end;
+        const uint32_t a = e.code & 0xFFu, kind = (e.code >> 8) & 0xFFu, c = e.code >> 16;
+#pragma unroll
+        for (uint32_t q = 0; q < PSX_MAX_CNA_PLOIDY; ++q) {
+            const uint32_t v = cn[q];
+            const uint32_t nv = kind == PSX_CNA_WGD ? 2u * v : (q != a ? v : (kind == PSX_CNA_AMP ? v + c : 0u));
+            cn[q] = hit ? nv : v;
+        }
+    }
+    uint32_t total = 0;
+#pragma unroll
+    for (uint32_t a = 0; a < PSX_MAX_CNA_PLOIDY; ++a) total += cn[a];
+    return total;
+}
 
 __device__ __forceinline__ uint32_t leaf_of(const job_dev& J, const seg_dev& S, tmpl& t) {
     if (t.leaf == UNRESOLVED) t.leaf = J.sample_leaves[S.leaf_off + t.cell];
@@ -164,21 +193,51 @@ __device__ __forceinline__ bool carried(const job_dev& J, uint32_t info, uint32_
 }
 
 // Template r of stratum j (oracle: the template loop of orc_simulate_seq).
+template <bool CNA = false>
 __device__ __forceinline__ tmpl make_template(const job_dev& J, const seg_dev& S, const lds_tables& tab, uint32_t r,
                                               uint32_t j, uint32_t b, uint32_t e) {
     const uint32_t c2 = (S.chr << 16) | S.so;
     const psx_u4 w = psx_draw_u(J.seed, r, j, c2, PSX_P_TEMPLATE << 24);
     tmpl t;
     t.x = b + psx_uniform32(w.x, e - b);
-    if (S.n_cells == 0 || (uint64_t)w.y < J.thr_normal) {
-        t.leaf = J.n_leaves;
-        t.cell = 0;
-        t.allele = psx_uniform32(w.z, J.ploidy);
+    t.ok = 1u;
+    if (!CNA || S.n_cna == 0) {
+        if (S.n_cells == 0 || (uint64_t)w.y < J.thr_normal) {
+            t.leaf = J.n_leaves;
+            t.cell = 0;
+            t.allele = psx_uniform32(w.z, J.ploidy);
+        } else {
+            const uint32_t X = psx_uniform32(w.z, S.n_cells * J.ploidy);
+            if (J.ploidy == 2) { t.cell = X >> 1; t.allele = X & 1u; }
+            else { t.cell = X / J.ploidy; t.allele = X - t.cell * J.ploidy; }
+            t.leaf = UNRESOLVED;
+        }
     } else {
-        const uint32_t X = psx_uniform32(w.z, S.n_cells * J.ploidy);
-        if (J.ploidy == 2) { t.cell = X >> 1; t.allele = X & 1u; }
-        else { t.cell = X / J.ploidy; t.allele = X - t.cell * J.ploidy; }
-        t.leaf = UNRESOLVED;
+        // one of kmax copy slots per cell; the template is kept when the slot
+        // is one of the source's copies at x, whose lineage it then reads
+        const uint32_t K = S.kmax;
+        if (S.n_cells == 0 || (uint64_t)w.y < J.thr_normal) {
+            const uint32_t k = psx_uniform32(w.z, K);
+            t.leaf = J.n_leaves;
+            t.cell = 0;
+            t.allele = k;
+            t.ok = k < J.ploidy;
+        } else {
+            const uint32_t X = psx_uniform32(w.z, S.n_cells * K);
+            t.cell = X / K;
+            const uint32_t k = X - t.cell * K;
+            t.leaf = J.sample_leaves[S.leaf_off + t.cell];
+            uint32_t cn[PSX_MAX_CNA_PLOIDY];
+            const uint32_t C = cna_copies(J.cna + S.cna_off, S.n_cna, t.leaf, t.x, J.ploidy, cn);
+            uint32_t cum = 0, al = 0;
+#pragma unroll
+            for (uint32_t a = 0; a < PSX_MAX_CNA_PLOIDY; ++a) {
+                cum += cn[a];
+                al += (a + 1 < J.ploidy && k >= cum) ? 1u : 0u;
+            }
+            t.allele = al;
+            t.ok = k < C;
+        }
     }
     t.x2 = t.x;
     if (J.pe) {
@@ -321,30 +380,39 @@ __device__ __forceinline__ void stage_window(const job_dev& J, const stratum& s,
 // A stratum whose window exceeds the LDS capacity: reads search the global
 // records and count with global atomics (kept out of line so the LDS path's
 // code is shared by both mates).
-template <int BLOCK>
+template <int BLOCK, bool CNA>
 __device__ __forceinline__ void count_stratum_global(const job_dev& J, const seg_dev& S, const lds_tables& tab,
                                                   const stratum& s, uint32_t j, uint32_t c2, uint16_t* slots,
-                                                  uint32_t tid) {
+                                                  uint32_t tid, uint32_t* acc) {
     const glb_view v{J.pos + s.k0, J.lo + s.k0, J.hi + s.k0, J.info + s.k0, J.alt + S.out_off + s.k0,
                      J.depth + S.out_off + s.k0, s.n_win};
     for (uint32_t r = tid; r < (uint32_t)s.nj; r += BLOCK) {
-        tmpl t = make_template(J, S, tab, r, j, s.b, s.e);
+        tmpl t = make_template<CNA>(J, S, tab, r, j, s.b, s.e);
+        if (CNA) {
+            if (!t.ok) continue;
+            *acc += 1u;
+        }
         for (uint32_t mate = 0; mate <= J.pe; ++mate)
             count_read(J, S, v, tab, t, mate ? t.x2 : t.x, r, j, c2, mate, slots, BLOCK);
     }
 }
 
-template <int BLOCK, int CAP>
+// CNA: some job has copy-number events (templates may be rejected and are
+// counted); the plain instantiation carries none of that code.
+template <int BLOCK, int CAP, bool CNA>
 __global__ void __launch_bounds__(BLOCK) k_seq_count(const seq_launch L) {
     __shared__ uint32_t s_pos[CAP], s_lo[CAP], s_hi[CAP], s_info[CAP], s_alt[CAP], s_dep[CAP];
     __shared__ uint16_t s_bucket[NB_MAX];
     __shared__ uint16_t s_err[PSX_MAX_READ_ERRORS * BLOCK];
     __shared__ lds_tables s_tab;
-    __shared__ uint32_t s_item;
+    __shared__ uint32_t s_item, s_acc;
     const uint32_t tid = threadIdx.x;
     uint16_t* slots = s_err + tid;
     for (;;) {
-        if (tid == 0) s_item = (uint32_t)atomicAdd(L.queue, 1ull);
+        if (tid == 0) {
+            s_item = (uint32_t)atomicAdd(L.queue, 1ull);
+            s_acc = 0;
+        }
         __syncthreads();
         const uint32_t item = __builtin_amdgcn_readfirstlane(s_item);
         if (item >= L.n_items) break;
@@ -352,6 +420,7 @@ __global__ void __launch_bounds__(BLOCK) k_seq_count(const seq_launch L) {
         const seg_dev S = uniform_at(L.segs, it.seg);
         const job_dev J = uniform_at(L.jobs, S.job);
         const uint32_t c2 = (S.chr << 16) | S.so;
+        uint32_t acc = 0;  // accepted templates (copy-number events)
         load_tables(J, s_tab, tid, BLOCK);
         __syncthreads();
         for (uint32_t j = it.j0; j < it.j1; ++j) {
@@ -360,7 +429,11 @@ __global__ void __launch_bounds__(BLOCK) k_seq_count(const seq_launch L) {
                 stage_window<BLOCK, CAP>(J, s, s_pos, s_lo, s_hi, s_info, s_alt, s_dep, s_bucket, tid);
                 const lds_view v{s_pos, s_lo, s_hi, s_info, s_alt, s_dep, s_bucket, s.n_win, s.b};
                 for (uint32_t r = tid; r < (uint32_t)s.nj; r += BLOCK) {
-                    tmpl t = make_template(J, S, s_tab, r, j, s.b, s.e);
+                    tmpl t = make_template<CNA>(J, S, s_tab, r, j, s.b, s.e);
+                    if (CNA) {
+                        if (!t.ok) continue;
+                        ++acc;
+                    }
                     for (uint32_t mate = 0; mate <= J.pe; ++mate)
                         count_read(J, S, v, s_tab, t, mate ? t.x2 : t.x, r, j, c2, mate, slots, BLOCK);
                 }
@@ -377,8 +450,13 @@ __global__ void __launch_bounds__(BLOCK) k_seq_count(const seq_launch L) {
                 }
                 __syncthreads();
             } else {
-                count_stratum_global<BLOCK>(J, S, s_tab, s, j, c2, slots, tid);
+                count_stratum_global<BLOCK, CNA>(J, S, s_tab, s, j, c2, slots, tid, &acc);
             }
+        }
+        if (CNA && S.n_cna) {
+            if (acc) atomicAdd(&s_acc, acc);
+            __syncthreads();
+            if (tid == 0 && s_acc) atomicAdd(J.acc, (unsigned long long)s_acc * (J.pe + 1u));
         }
         __syncthreads();
     }
@@ -388,9 +466,13 @@ __global__ void __launch_bounds__(BLOCK) k_seq_count(const seq_launch L) {
 // the work counter of the count kernel that follows on the stream.
 __global__ void __launch_bounds__(256) k_seq_init(const job_dev* __restrict__ jobs, const pres_dev* __restrict__ ps,
                                                   uint32_t n_ps, uint64_t total,
-                                                  unsigned long long* __restrict__ queue) {
+                                                  unsigned long long* __restrict__ queue, uint32_t n_jobs) {
     const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q == 0) *queue = 0ull;
+    if (q == 0) {
+        *queue = 0ull;
+        for (uint32_t k = 0; k < n_jobs; ++k)
+            if (jobs[k].acc) *jobs[k].acc = 0ull;
+    }
     if (q >= total) return;
     uint32_t a = 0, z = n_ps;
     while (a < z) {

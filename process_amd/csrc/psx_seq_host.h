@@ -41,6 +41,9 @@ void batch_free(seq_batch* b);
 size_t batch_desc_bytes(const seq_batch* b);
 uint64_t batch_reads(const seq_batch* b);
 size_t batch_items(const seq_batch* b);
+// Reads simulated per job once the launch has finished (jobs with
+// copy-number events: the device count of accepted templates).
+std::vector<uint64_t> batch_job_reads(const seq_batch* b, const seq_lane& lane);
 // Copies the descriptors to lane.desc (which must hold batch_desc_bytes).
 void batch_upload(seq_batch* b, const seq_lane& lane);
 // Enqueues the init kernel and the count kernel on lane.stream; with

@@ -123,6 +123,8 @@ void run_batch(psx_server* s, lane_state& L, std::vector<srv_req*>& batch) {
             PSX_HIP(hipMemcpyAsync(q->r.present, o + op, q->n_elems, hipMemcpyDeviceToHost, L.lane.stream));
     }
     PSX_HIP(hipStreamSynchronize(L.lane.stream));
+    const std::vector<uint64_t> reads = batch_job_reads(B.get(), L.lane);
+    for (size_t k = 0; k < batch.size(); ++k) batch[k]->st.n_reads = reads[k];
 }
 
 void worker(psx_server* s, uint32_t k) {

@@ -224,3 +224,51 @@ def serving_requests(pool: ServingPool, n: int, rate: float, seed: int = 9, min_
         seq["samples"] = pick
         out.append((float(t[i]), k, int(rng.integers(1 << 31)), seq))
     return out
+
+
+def cna_segments(lens, rng, n: int, species: int) -> list:
+    """n allele-specific copy-number segments of one species: uniform start,
+    log-uniform length in [1, 50] Mb (clipped to the chromosome), total copy
+    state drawn from {0, 1, 3, 4} of a diploid segment: 0 deletes both
+    lineages, 1 one lineage, 3 adds one copy to a lineage, 4 two copies."""
+    from . import _abi as A
+    out = []
+    for _ in range(n):
+        c = int(rng.integers(len(lens)))
+        ln = int(lens[c])
+        start = int(rng.integers(ln))
+        length = int(min(np.exp(rng.uniform(np.log(1e6), np.log(50e6))), ln - start))
+        state = int(rng.choice([0, 1, 3, 4]))
+        a = int(rng.integers(2))
+        if state == 0:
+            out += [(species, c, start, length, 0, A.PSX_CNA_DEL, 0), (species, c, start, length, 1, A.PSX_CNA_DEL, 0)]
+        elif state == 1:
+            out.append((species, c, start, length, a, A.PSX_CNA_DEL, 0))
+        else:
+            out.append((species, c, start, length, a, A.PSX_CNA_AMP, state - 2))
+    return out
+
+
+def cna_purity(seed: int = 404, n_cells: int = 100_000, sample_size: int = 10_000,
+               total_bases: int = 100_000_000) -> Workload:
+    """Two species (growth 0.2; driver 0.3 introduced at 1,000 cells) grown to
+    1e5 cells, one sample of 1e4 cells, a 100 Mb reference over 10
+    chromosomes; eight copy-number segments in the founder and four in the
+    driver clone (cna_segments), one whole-genome doubling on the driver
+    clone; SNV 1e-8, indel 1e-9; 500x paired-end 150 bp reads (insert 350 +-
+    10), error rate 1e-3, purity 0.6, normal sample included."""
+    from . import _abi as A
+    rng = np.random.default_rng(seed)
+    lens = log_uniform_lengths(10, total_bases, 1.0, 5.0, rng)
+    forest = core.synth_forest(growth=[0.2, 0.3], death=[0.01, 0.01], intro_at=[0, 1000], n_cells=n_cells,
+                               n_samples=1, sample_size=sample_size, seed=seed + 1)
+    sig = dirichlet_signature(rng)
+    cna = cna_segments(lens, rng, 8, 0) + [(1, 0, 0, 0, 0, A.PSX_CNA_WGD, 0)] + cna_segments(lens, rng, 4, 1)
+    engine = dict(snv_rate=[1e-8, 1e-8], indel_rate=[1e-9, 1e-9], sbs=sig[None, :], epoch_start=[0.0],
+                  exposure=[[1.0]], n_pre_snv=0, n_pre_indel=0, pre_sbs=0, indel_max_len=10, indel_len_p=0.5,
+                  infinite_sites=True, ploidy=2, drivers=[], germline=[], cna=cna)
+    seq = dict(coverage=500.0, read_size=150, insert_size_mean=350, insert_size_stddev=10, error_rate=1e-3,
+               purity=0.6, with_normal_sample=True)
+    return Workload("cna_purity", lens, seed + 2, forest, engine, seq, seed + 3, seed + 4,
+                    notes=dict(n_cells=n_cells, sample_size=sample_size, genome=int(lens.sum()), chromosomes=10,
+                               cna_drivers=len(cna)))

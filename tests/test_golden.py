@@ -21,12 +21,13 @@ def load(case):
     z = np.load(GOLDEN / f"seq_{case}.npz")
     d = {k: z[k] for k in z.files}
     d["recs"] = d["recs"].view(A.MUTATION_DTYPE)
+    d["cna"] = d["cna"].view(A.CNA_EVENT_DTYPE) if "cna" in d else np.zeros(0, A.CNA_EVENT_DTYPE)
     d["seq"] = json.loads(str(d["seq"]))
     return d
 
 
 def test_fixtures_present():
-    assert set(CASES) >= {"pe_finite_sites", "dense_finite_sites", "se_subset"}
+    assert set(CASES) >= {"pe_finite_sites", "dense_finite_sites", "se_subset", "cna_purity"}
 
 
 @pytest.mark.parametrize("case", CASES)
@@ -43,7 +44,7 @@ def test_fixture_shapes(case):
 def test_oracle_matches_fixture(case):
     d = load(case)
     of = O.Forest(d["cell_id"], d["parent"], d["species"], d["sample"], d["birth"], int(d["n_samples"]))
-    a, dp, pr, n = O.simulate_seq(d["lens"], of, d["recs"], int(d["seed"]), **d["seq"])
+    a, dp, pr, n = O.simulate_seq(d["lens"], of, d["recs"], int(d["seed"]), cna=d["cna"], **d["seq"])
     assert n == int(d["n_reads"])
     assert np.array_equal(a, d["alt"]) and np.array_equal(dp, d["depth"]) and np.array_equal(pr, d["present"])
 
@@ -58,6 +59,7 @@ def test_gpu_matches_fixture(ctx, case):
     g = core.Genome(ctx, d["lens"], packed=d["packed"])
     f = core.Forest(ctx, d["cell_id"], d["parent"], d["species"], d["sample"], d["birth"], int(d["n_samples"]))
     m = core.Mutations.upload(ctx, f, d["recs"])
+    m.set_cna(d["cna"])
     assert np.array_equal(m.download(), d["recs"])
     a, dp, pr, n = core.simulate_seq_arrays(ctx, g, f, m, int(d["seed"]), **d["seq"])
     assert n == int(d["n_reads"])

@@ -13,6 +13,7 @@ import cases
 from oracle import oracle as O
 from process_amd import _abi as A
 from process_amd import core
+from process_amd._lib import PsxError
 
 pytestmark = pytest.mark.gpu
 
@@ -242,3 +243,80 @@ def test_materialised_reads_match_counts(ctx, world, case):
     oa, od, op, on = O.simulate_seq(world.lens, world.of, world.om, 21, **kw)
     assert np.array_equal(dep, od)
     assert np.array_equal(alt, oa)
+
+
+# ------------------------------------------------------------ copy numbers
+
+CNA_LENS = np.array([70_000, 33_000], np.uint32)
+
+
+@pytest.fixture(scope="module")
+def cna_world(ctx):
+    say("[parity] building copy-number world: genome 70000+33000, 500-cell forest, 5 CNA drivers")
+    w = World()
+    w.lens = CNA_LENS
+    w.packed = O.genome_synth(w.lens, 31)
+    w.g = core.Genome(ctx, w.lens, seed=31)
+    fd = core.synth_forest(growth=[0.2, 0.45], death=[0.01, 0.01], intro_at=[0, 20], n_cells=500, n_samples=2,
+                           sample_size=80, seed=3)
+    w.fd = fd
+    w.of = O.Forest(fd["cell_id"], fd["parent"], fd["species"], fd["sample"], fd["birth"], fd["n_samples"])
+    w.pf = core.Forest(ctx, fd["cell_id"], fd["parent"], fd["species"], fd["sample"], fd["birth"], fd["n_samples"])
+    w.eng = cases.engine(w.packed, w.lens, snv=6e-6, indel=6e-7, pre_snv=20, pre_indel=3)
+    w.eng["cna"] = [(0, 0, 5_000, 20_000, 0, A.PSX_CNA_AMP, 2), (0, 1, 1_000, 9_000, 1, A.PSX_CNA_DEL, 0),
+                    (1, 0, 0, 0, 0, A.PSX_CNA_WGD, 0), (1, 0, 40_000, 15_000, 1, A.PSX_CNA_AMP, 1),
+                    (1, 0, 12_000, 4_000, 0, A.PSX_CNA_DEL, 0)]
+    w.om = O.place_mutations(w.lens, w.packed, w.of, 4242, **w.eng)
+    w.oev = O.place_cna(w.lens, w.of, **w.eng)
+    w.pm = core.Mutations.place(ctx, w.g, w.pf, 4242, **w.eng)
+    return w
+
+
+def test_cna_placement_matches(cna_world):
+    w = cna_world
+    assert np.array_equal(w.pm.download(), w.om)
+    assert w.pm.cna().tobytes() == w.oev.tobytes()
+    assert len(w.oev) >= 5
+
+
+CNA_SEQ = {
+    "pe_purity": dict(coverage=30, read_size=150, insert_size_mean=350, insert_size_stddev=10, error_rate=1e-3,
+                      purity=0.6),
+    "se_pure": dict(coverage=25, read_size=100, error_rate=5e-3, with_normal_sample=False),
+    "subset_normal": dict(coverage=20, read_size=120, samples=[1], chromosomes=[0], purity=0.8,
+                          preneoplastic_in_normal=True),
+}
+
+
+@pytest.mark.parametrize("case", sorted(CNA_SEQ))
+def test_cna_seq_bit_exact(ctx, cna_world, case):
+    say(f"[parity] copy-number sequencing {case}")
+    w = cna_world
+    kw = CNA_SEQ[case]
+    a, d, p, n = core.simulate_seq_arrays(ctx, w.g, w.pf, w.pm, 17, **kw)
+    oa, od, op, on = O.simulate_seq(w.lens, w.of, w.om, 17, cna=w.oev, **kw)
+    assert n == on
+    assert np.array_equal(d, od) and np.array_equal(a, oa) and np.array_equal(p, op)
+
+
+def test_cna_replay_and_batch(ctx, cna_world):
+    """A replayed table with its events, batched with a plain job, matches the
+    oracle job by job; read materialisation refuses copy-number tables."""
+    say("[parity] copy-number replay + batch")
+    w = cna_world
+    rm = core.Mutations.upload(ctx, w.pf, w.om)
+    rm.set_cna(w.oev)
+    assert np.array_equal(rm.cna(), w.oev)
+    kw = CNA_SEQ["pe_purity"]
+    jobs = [core.SeqJob(ctx, w.g, w.pf, rm, 5, **kw),
+            core.SeqJob(ctx, w.g, w.pf, core.Mutations.upload(ctx, w.pf, w.om), 6, **kw)]
+    n = core.simulate_seq_batch(ctx, jobs)
+    oa, od, op, on = O.simulate_seq(w.lens, w.of, w.om, 5, cna=w.oev, **kw)
+    pa, pd, pp, pn = O.simulate_seq(w.lens, w.of, w.om, 6, **kw)
+    a, d, p = jobs[0].results()
+    assert np.array_equal(a, oa) and np.array_equal(d, od) and np.array_equal(p, op)
+    a, d, p = jobs[1].results()
+    assert np.array_equal(a, pa) and np.array_equal(d, pd)
+    assert n == on + pn
+    with pytest.raises(PsxError, match="copy-number"):
+        core.simulate_seq_reads(ctx, w.g, w.pf, rm, 1, **kw)
