@@ -125,6 +125,18 @@ struct lds_view {
         while (i < n && pos[i] < x) ++i;
         return i;
     }
+    // Same result; the wave steps together (uniform trip count, no exec
+    // masking), every load in bounds.
+    __device__ __forceinline__ uint32_t first_u(uint32_t x) const {
+        uint32_t i = bucket[(x - b) >> NB_SHIFT];
+        bool go = i < n && pos[min(i, n - 1u)] < x;
+        while (__ballot(go)) {
+            i += go ? 1u : 0u;
+            go = go && i < n && pos[min(i, n - 1u)] < x;
+        }
+        return i;
+    }
+    __device__ __forceinline__ uint32_t at(uint32_t i) const { return min(i, n - 1u); }  // n > 0
     __device__ __forceinline__ void add_dep(uint32_t q) const { atomicAdd(&dep[q], 1u); }
     __device__ __forceinline__ void add_alt(uint32_t q) const { atomicAdd(&alt[q], 1u); }
 };
@@ -145,6 +157,8 @@ struct glb_view {  // window larger than the LDS capacity
         }
         return a;
     }
+    __device__ __forceinline__ uint32_t first_u(uint32_t x) const { return first(x); }
+    __device__ __forceinline__ uint32_t at(uint32_t i) const { return min(i, n - 1u); }  // n > 0
     __device__ __forceinline__ void add_dep(uint32_t q) const { atomicAdd(&dep[q], 1); }
     __device__ __forceinline__ void add_alt(uint32_t q) const { atomicAdd(&alt[q], 1); }
 };
@@ -269,6 +283,36 @@ __device__ uint32_t error_set(const job_dev& J, const lds_tables& tab, uint32_t 
         if (dup) continue;
         slots[cnt * stride] = (uint16_t)(o | ((((word & 0xFFFFu) * 3u) >> 16) << 12));
         ++cnt;
+    }
+    return cnt;
+}
+
+// error_set with wave-uniform loops: every lane steps through the same
+// draws (the word counter t is uniform), effects are predicated, so the
+// draw sequence and the result of each lane are those of error_set.
+__device__ __forceinline__ uint32_t error_set_u(const job_dev& J, const lds_tables& tab, uint32_t r, uint32_t j,
+                                                uint32_t c2, uint32_t mate, uint16_t* slots, uint32_t stride) {
+    const uint32_t base = (PSX_P_ERROR << 24) | (mate << 16);
+    psx_u4 w = psx_draw_u(J.seed, r, j, c2, base);
+    uint32_t K = 0;
+    bool more = 1u < J.err_n && w.x >= tab.err[0];
+    while (__ballot(more)) {
+        K += more ? 1u : 0u;
+        more = more && K + 1u < J.err_n && w.x >= tab.err[min(K, J.err_n - 1u)];
+    }
+    uint32_t cnt = 0;
+    bool need = K > 0;
+    for (uint32_t t = 1; t < 1u + 4u * PSX_MAX_READ_ERRORS && __ballot(need); ++t) {
+        if ((t & 3u) == 0) w = psx_draw_u(J.seed, r, j, c2, base | (t >> 2));
+        const uint32_t word = psx_word(w, (int)(t & 3u));
+        const uint32_t o = psx_mulhi32(word, J.L);
+        bool dup = false;
+        const uint32_t qn = min(t - 1u, PSX_MAX_READ_ERRORS);
+        for (uint32_t q = 0; q < qn; ++q) dup |= q < cnt && (slots[q * stride] & 0xFFFu) == o;
+        const bool take = need && !dup;
+        if (take) slots[cnt * stride] = (uint16_t)(o | ((((word & 0xFFFFu) * 3u) >> 16) << 12));
+        cnt += take ? 1u : 0u;
+        need = cnt < K;
     }
     return cnt;
 }
@@ -561,31 +605,35 @@ __device__ __forceinline__ void write_read(const job_dev& J, const V& v, const l
             lo = hi;
         }
     }
-    uint32_t i = v.first(x);
+    // SNVs at [x, x + L) in table order; the first carried SNV at a position
+    // sets its base. Uniform trip count, branch-free read-modify-writes of the
+    // lane's own row (an unchanged word when nothing applies).
     const uint32_t end = x + J.L;
-    while (i < v.n && v.pos[i] < end) {
-        const uint32_t p = v.pos[i];
-        bool done = false;
-        while (i < v.n && v.pos[i] == p) {
-            const uint32_t in = v.info[i];
-            if (!done && PSX_INFO_KIND(in) == PSX_SNV && PSX_INFO_ALLELE(in) == allele &&
-                carried(J, in, v.lo[i], v.hi[i], leaf)) {
-                const uint32_t o = p - x, s2 = (o & 15u) * 2u;
-                row[2 + (o >> 4)] = (row[2 + (o >> 4)] & ~(3u << s2)) | (PSX_INFO_ALT(in) << s2);
-                done = true;
-            }
-            ++i;
-        }
+    uint32_t i = v.first_u(x);
+    bool go = i < v.n && v.pos[v.at(i)] < end;
+    uint32_t last = 0xFFFFFFFFu;
+    while (__ballot(go)) {
+        const uint32_t ic = go ? i : 0u;
+        const uint32_t p = v.pos[ic], in = v.info[ic];
+        const bool apply = go && p != last && PSX_INFO_KIND(in) == PSX_SNV && PSX_INFO_ALLELE(in) == allele &&
+                           carried(J, in, v.lo[ic], v.hi[ic], leaf);
+        const uint32_t o = apply ? p - x : 0u, s2 = (o & 15u) * 2u;
+        const uint32_t wv = row[2 + (o >> 4)];
+        row[2 + (o >> 4)] = apply ? (wv & ~(3u << s2)) | (PSX_INFO_ALT(in) << s2) : wv;
+        last = apply ? p : last;
+        i += go ? 1u : 0u;
+        go = go && i < v.n && v.pos[v.at(i)] < end;
     }
     uint32_t ne = 0;
-    if (J.err_n) ne = error_set(J, tab, r, j, c2, mate, slots, stride);
-    for (uint32_t q = 0; q < ne; ++q) {
-        const uint32_t e = slots[q * stride];
+    if (J.err_n) ne = error_set_u(J, tab, r, j, c2, mate, slots, stride);
+    for (uint32_t q = 0; __ballot(q < ne); ++q) {
+        const bool on = q < ne;
+        const uint32_t e = on ? slots[q * stride] : 0u;
         const uint32_t o = e & 0xFFFu, s2 = (o & 15u) * 2u;
         const uint32_t wv = row[2 + (o >> 4)];
         const uint32_t tb = (wv >> s2) & 3u, sub = e >> 12;
         const uint32_t nb = sub < tb ? sub : sub + 1;
-        row[2 + (o >> 4)] = (wv & ~(3u << s2)) | (nb << s2);
+        row[2 + (o >> 4)] = on ? (wv & ~(3u << s2)) | (nb << s2) : wv;
     }
     if constexpr (NW == 0) row[0] = x;
     row[1] = leaf | (allele << 26) | (mate << 30) | ((ne > 0 ? 1u : 0u) << 31);
