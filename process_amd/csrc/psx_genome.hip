@@ -45,9 +45,12 @@ __global__ void __launch_bounds__(256) k_mut_soa(const psx_mutation* __restrict_
     pos[i] = r.pos;
     lo[i] = r.leaf_lo;
     hi[i] = r.leaf_hi;
+    // alone at its position (records are in SID order: by chromosome, then position)
+    const bool solo = (i == 0 || recs[i - 1].chr != r.chr || recs[i - 1].pos != r.pos) &&
+                      (i + 1 == n || recs[i + 1].chr != r.chr || recs[i + 1].pos != r.pos);
     info[i] = uint32_t(r.allele) | (uint32_t(r.ref & 3u) << 8) | (uint32_t(r.alt & 3u) << 10) |
               (uint32_t(r.kind & 3u) << 12) | (uint32_t(r.nature & 3u) << 14) |
-              (uint32_t(r.sid_head & 1u) << 16);
+              (uint32_t(r.sid_head & 1u) << 16) | (uint32_t(solo) << 17);
 }
 
 struct bucket_args {

@@ -302,12 +302,6 @@ uint64_t psx_sid_key(const psx_mutation& r) {
            (uint64_t(r.len & 15u) << 20) | (payload & 0xFFFFFu);
 }
 
-uint32_t psx_info_word(const psx_mutation& r) {
-    return uint32_t(r.allele) | (uint32_t(r.ref & 3u) << 8) | (uint32_t(r.alt & 3u) << 10) |
-           (uint32_t(r.kind & 3u) << 12) | (uint32_t(r.nature & 3u) << 14) |
-           (uint32_t(r.sid_head & 1u) << 16);
-}
-
 // ------------------------------------------------------------------ forest
 
 // SamplesForest node table (src/forest.hpp:29-67) flattened in DFS pre-order:

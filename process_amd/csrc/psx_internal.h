@@ -122,13 +122,14 @@ struct psx_mutations {
 void psx_cna_check(const std::vector<psx_cna_event>& ev, uint32_t n_leaves, uint32_t ploidy,
                    const std::vector<uint32_t>& chr_len);
 
-// info word of a device mutation record
+// info word of a device mutation record (built by k_mut_soa)
 #define PSX_INFO_ALLELE(i) ((i) & 0xFFu)
 #define PSX_INFO_REF(i) (((i) >> 8) & 3u)
 #define PSX_INFO_ALT(i) (((i) >> 10) & 3u)
 #define PSX_INFO_KIND(i) (((i) >> 12) & 3u)
 #define PSX_INFO_NATURE(i) (((i) >> 14) & 3u)
 #define PSX_INFO_HEAD(i) (((i) >> 16) & 1u)
+#define PSX_INFO_SOLO(i) (((i) >> 17) & 1u)  // the only record at its position
 
 // ------------------------------------------------------------ host helpers
 
@@ -153,7 +154,6 @@ struct psx_ctable {
 psx_ctable psx_compact_table(const psx_itable& t);
 
 uint64_t psx_sid_key(const psx_mutation& r);
-uint32_t psx_info_word(const psx_mutation& r);
 
 // Sorts recs into SID order, marks sid_head and builds the device table.
 void psx_mutations_build(psx_ctx* ctx, psx_mutations* m, std::vector<psx_mutation>& recs,

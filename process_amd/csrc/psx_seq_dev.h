@@ -290,12 +290,14 @@ __device__ uint32_t error_set(const job_dev& J, const lds_tables& tab, uint32_t 
 // error_set with wave-uniform loops: every lane steps through the same
 // draws (the word counter t is uniform), effects are predicated, so the
 // draw sequence and the result of each lane are those of error_set.
+// Lanes with want == false return 0 without stepping the loops.
 __device__ __forceinline__ uint32_t error_set_u(const job_dev& J, const lds_tables& tab, uint32_t r, uint32_t j,
-                                                uint32_t c2, uint32_t mate, uint16_t* slots, uint32_t stride) {
+                                                uint32_t c2, uint32_t mate, uint16_t* slots, uint32_t stride,
+                                                bool want = true) {
     const uint32_t base = (PSX_P_ERROR << 24) | (mate << 16);
     psx_u4 w = psx_draw_u(J.seed, r, j, c2, base);
     uint32_t K = 0;
-    bool more = 1u < J.err_n && w.x >= tab.err[0];
+    bool more = want && 1u < J.err_n && w.x >= tab.err[0];
     while (__ballot(more)) {
         K += more ? 1u : 0u;
         more = more && K + 1u < J.err_n && w.x >= tab.err[min(K, J.err_n - 1u)];
@@ -317,58 +319,91 @@ __device__ __forceinline__ uint32_t error_set_u(const job_dev& J, const lds_tabl
     return cnt;
 }
 
+// Records of one position shared by several records, from index i (the
+// oracle's seq_read step): the first carried SNV sets the true base, the
+// read's error at the position substitutes it, every record gains depth and
+// each SID head an alternative read when its alternative is observed (SNV)
+// or carried (indel). Returns the index past the position.
+template <class V>
+__device__ __forceinline__ uint32_t count_group(const job_dev& J, const V& v, uint32_t i, uint32_t x, uint32_t allele,
+                                             uint32_t leaf, uint32_t ne, const uint16_t* slots, uint32_t stride) {
+    const uint32_t p = v.pos[i];
+    const uint32_t g0 = i;
+    uint32_t g1 = i + 1;
+    while (g1 < v.n && v.pos[g1] == p) ++g1;
+    uint32_t tb = PSX_INFO_REF(v.info[g0]);
+    for (uint32_t q = g0; q < g1; ++q) {
+        const uint32_t in = v.info[q];
+        if (PSX_INFO_KIND(in) == PSX_SNV && PSX_INFO_ALLELE(in) == allele && carried(J, in, v.lo[q], v.hi[q], leaf)) {
+            tb = PSX_INFO_ALT(in);
+            break;
+        }
+    }
+    uint32_t ob = tb;
+    for (uint32_t q = 0; q < ne; ++q) {
+        const uint32_t e = slots[q * stride];
+        if ((e & 0xFFFu) == p - x) {
+            const uint32_t sub = e >> 12;
+            ob = sub < tb ? sub : sub + 1;
+        }
+    }
+    for (uint32_t q = g0; q < g1; ++q) {
+        v.add_dep(q);
+        const uint32_t in = v.info[q];
+        if (!PSX_INFO_HEAD(in)) continue;
+        if (PSX_INFO_KIND(in) == PSX_SNV) {
+            if (ob == PSX_INFO_ALT(in)) v.add_alt(q);
+        } else {
+            bool car = false;
+            for (uint32_t q2 = q; q2 < g1 && (q2 == q || !PSX_INFO_HEAD(v.info[q2])); ++q2) {
+                const uint32_t in2 = v.info[q2];
+                car |= PSX_INFO_ALLELE(in2) == allele && carried(J, in2, v.lo[q2], v.hi[q2], leaf);
+            }
+            if (car) v.add_alt(q);
+        }
+    }
+    return g1;
+}
+
 // One read [x, x + L): depth for every record at a covered position, an
 // alternative read for each SID head whose alternative is observed (SNV) or
-// carried (indel). Oracle: seq_read.
+// carried (indel). Oracle: seq_read. A record alone at its position (every
+// record under the infinite-sites model) takes one branch-free step of a
+// wave-uniform loop; a position shared by several records goes through
+// count_group.
 template <class V>
-__device__ void count_read(const job_dev& J, const seg_dev& S, const V& v, const lds_tables& tab, tmpl& t,
-                           uint32_t x, uint32_t r, uint32_t j, uint32_t c2, uint32_t mate, uint16_t* slots,
-                           uint32_t stride) {
-    uint32_t i = v.first(x);
+__device__ __forceinline__ void count_read(const job_dev& J, const seg_dev& S, const V& v, const lds_tables& tab,
+                                           tmpl& t, uint32_t x, uint32_t r, uint32_t j, uint32_t c2, uint32_t mate,
+                                           uint16_t* slots, uint32_t stride) {
     const uint32_t end = x + J.L;
-    if (i >= v.n || v.pos[i] >= end) return;
-    int ne = -1;
-    while (i < v.n && v.pos[i] < end) {
-        const uint32_t p = v.pos[i];
-        const uint32_t g0 = i;
-        uint32_t g1 = i + 1;
-        while (g1 < v.n && v.pos[g1] == p) ++g1;
-        uint32_t tb = PSX_INFO_REF(v.info[g0]);
-        for (uint32_t q = g0; q < g1; ++q) {
-            const uint32_t in = v.info[q];
-            if (PSX_INFO_KIND(in) == PSX_SNV && PSX_INFO_ALLELE(in) == t.allele &&
-                carried(J, in, v.lo[q], v.hi[q], leaf_of(J, S, t))) {
-                tb = PSX_INFO_ALT(in);
-                break;
-            }
+    uint32_t i = v.first_u(x);
+    bool go = i < v.n && v.pos[v.at(i)] < end;
+    if (!__ballot(go)) return;
+    const uint32_t leaf = leaf_of(J, S, t);
+    const uint32_t ne = J.err_n ? error_set_u(J, tab, r, j, c2, mate, slots, stride, go) : 0u;
+    while (__ballot(go)) {
+        const uint32_t ic = go ? i : 0u;
+        const uint32_t p = v.pos[ic], in = v.info[ic];
+        if (__ballot(go && !PSX_INFO_SOLO(in))) {
+            if (go && !PSX_INFO_SOLO(in)) i = count_group(J, v, ic, x, t.allele, leaf, ne, slots, stride);
         }
+        const bool one = go && PSX_INFO_SOLO(in);
+        const bool car = PSX_INFO_ALLELE(in) == t.allele && carried(J, in, v.lo[ic], v.hi[ic], leaf);
+        const bool snv = PSX_INFO_KIND(in) == PSX_SNV;
+        const uint32_t tb = snv && car ? PSX_INFO_ALT(in) : PSX_INFO_REF(in);
         uint32_t ob = tb;
-        if (J.err_n) {
-            if (ne < 0) ne = (int)error_set(J, tab, r, j, c2, mate, slots, stride);
-            for (int q = 0; q < ne; ++q) {
-                const uint32_t s = slots[q * stride];
-                if ((s & 0xFFFu) == p - x) {
-                    const uint32_t sub = s >> 12;
-                    ob = sub < tb ? sub : sub + 1;
-                }
-            }
+        for (uint32_t q = 0; __ballot(one && q < ne); ++q) {
+            const bool on = one && q < ne;
+            const uint32_t e = on ? slots[q * stride] : 0u;
+            const uint32_t sub = e >> 12;
+            ob = on && (e & 0xFFFu) == p - x ? (sub < tb ? sub : sub + 1) : ob;
         }
-        for (uint32_t q = g0; q < g1; ++q) {
-            v.add_dep(q);
-            const uint32_t in = v.info[q];
-            if (!PSX_INFO_HEAD(in)) continue;
-            if (PSX_INFO_KIND(in) == PSX_SNV) {
-                if (ob == PSX_INFO_ALT(in)) v.add_alt(q);
-            } else {
-                bool car = false;
-                for (uint32_t q2 = q; q2 < g1 && (q2 == q || !PSX_INFO_HEAD(v.info[q2])); ++q2) {
-                    const uint32_t in2 = v.info[q2];
-                    car |= PSX_INFO_ALLELE(in2) == t.allele && carried(J, in2, v.lo[q2], v.hi[q2], leaf_of(J, S, t));
-                }
-                if (car) v.add_alt(q);
-            }
+        if (one) {
+            v.add_dep(ic);
+            if (PSX_INFO_HEAD(in) && (snv ? ob == PSX_INFO_ALT(in) : car)) v.add_alt(ic);
         }
-        i = g1;
+        i += one ? 1u : 0u;
+        go = go && i < v.n && v.pos[v.at(i)] < end;
     }
 }
 
