@@ -652,23 +652,23 @@ __device__ __forceinline__ void write_read(const job_dev& J, const V& v, const l
         const uint32_t p = v.pos[ic], in = v.info[ic];
         const bool apply = go && p != last && PSX_INFO_KIND(in) == PSX_SNV && PSX_INFO_ALLELE(in) == allele &&
                            carried(J, in, v.lo[ic], v.hi[ic], leaf);
-        const uint32_t o = apply ? p - x : 0u, s2 = (o & 15u) * 2u;
-        const uint32_t wv = row[2 + (o >> 4)];
-        row[2 + (o >> 4)] = apply ? (wv & ~(3u << s2)) | (PSX_INFO_ALT(in) << s2) : wv;
+        if (apply) {  // rare: only applying lanes touch their row (no bank conflicts from the others)
+            const uint32_t o = p - x, s2 = (o & 15u) * 2u;
+            row[2 + (o >> 4)] = (row[2 + (o >> 4)] & ~(3u << s2)) | (PSX_INFO_ALT(in) << s2);
+        }
         last = apply ? p : last;
         i += go ? 1u : 0u;
         go = go && i < v.n && v.pos[v.at(i)] < end;
     }
     uint32_t ne = 0;
     if (J.err_n) ne = error_set_u(J, tab, r, j, c2, mate, slots, stride);
-    for (uint32_t q = 0; __ballot(q < ne); ++q) {
-        const bool on = q < ne;
-        const uint32_t e = on ? slots[q * stride] : 0u;
+    for (uint32_t q = 0; q < ne; ++q) {  // rare: about one read in seven at e = 1e-3
+        const uint32_t e = slots[q * stride];
         const uint32_t o = e & 0xFFFu, s2 = (o & 15u) * 2u;
         const uint32_t wv = row[2 + (o >> 4)];
         const uint32_t tb = (wv >> s2) & 3u, sub = e >> 12;
         const uint32_t nb = sub < tb ? sub : sub + 1;
-        row[2 + (o >> 4)] = on ? (wv & ~(3u << s2)) | (nb << s2) : wv;
+        row[2 + (o >> 4)] = (wv & ~(3u << s2)) | (nb << s2);
     }
     if constexpr (NW == 0) row[0] = x;
     row[1] = leaf | (allele << 26) | (mate << 30) | ((ne > 0 ? 1u : 0u) << 31);
