@@ -109,6 +109,13 @@ __device__ __forceinline__ void load_tables(const job_dev& J, lds_tables& t, uin
     for (uint32_t i = tid; i < J.err_n; i += block) t.err[i] = J.err_T[i];
 }
 
+// LDS writes of a wave visible to its other lanes.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // ------------------------------------------------------------- record views
 
 struct lds_view {
@@ -407,6 +414,82 @@ __device__ __forceinline__ void count_read(const job_dev& J, const seg_dev& S, c
     }
 }
 
+// One mate of 64 templates of a wave (one per lane): every lane finds the
+// positions its read covers and queues (lane, first record of the position)
+// in the wave's LDS queue; the queue is drained 64 entries at a time, one
+// per lane, so the per-position work runs on full waves instead of waiting
+// for the lane whose read covers the most positions. Same counts as
+// count_read. Lanes with valid == false take no part (x must still lie in
+// the stratum).
+template <int BLOCK>
+__device__ __forceinline__ void count_mate_wave(const job_dev& J, const seg_dev& S, const lds_view& v,
+                                                const lds_tables& tab, tmpl& t, bool valid, uint32_t x, uint32_t r,
+                                                uint32_t j, uint32_t c2, uint32_t mate, uint16_t* s_err,
+                                                uint32_t* q, uint32_t lane, uint32_t wave) {
+    const uint32_t end = x + J.L;
+    uint32_t i = v.first_u(x);
+    bool go = valid && i < v.n && v.pos[v.at(i)] < end;
+    if (!__ballot(go)) return;
+    uint32_t leaf = 0;
+    if (go) leaf = leaf_of(J, S, t);
+    uint16_t* wslots = s_err + wave * 64u;  // slot q of lane l: wslots[l + q * BLOCK]
+    const uint32_t ne = J.err_n ? error_set_u(J, tab, r, j, c2, mate, wslots + lane, BLOCK, go) : 0u;
+    const uint32_t an = t.allele | (ne << 8);
+    // one drain: entries qb[0 .. count), one per lane
+    auto drain = [&](const uint32_t* qb, uint32_t count) {
+        const bool on = lane < count;
+        const uint32_t e = on ? qb[lane] : 0u;
+        const uint32_t ol = e >> 26, ri = e & 0x3FFFFFFu;
+        const uint32_t xo = (uint32_t)__shfl((int)x, (int)ol);
+        const uint32_t lo = (uint32_t)__shfl((int)leaf, (int)ol);
+        const uint32_t ao = (uint32_t)__shfl((int)an, (int)ol);
+        const uint32_t al = ao & 0xFFu, neo = ao >> 8;
+        const uint16_t* sl = wslots + ol;
+        const uint32_t in = v.info[ri], p = v.pos[ri];
+        if (__ballot(on && !PSX_INFO_SOLO(in))) {
+            if (on && !PSX_INFO_SOLO(in)) count_group(J, v, ri, xo, al, lo, neo, sl, BLOCK);
+        }
+        const bool one = on && PSX_INFO_SOLO(in);
+        const bool car = PSX_INFO_ALLELE(in) == al && carried(J, in, v.lo[ri], v.hi[ri], lo);
+        const bool snv = PSX_INFO_KIND(in) == PSX_SNV;
+        const uint32_t tb = snv && car ? PSX_INFO_ALT(in) : PSX_INFO_REF(in);
+        uint32_t ob = tb;
+        for (uint32_t k = 0; __ballot(one && k < neo); ++k) {
+            const bool hit = one && k < neo;
+            const uint32_t ew = hit ? sl[k * BLOCK] : 0u;
+            const uint32_t sub = ew >> 12;
+            ob = hit && (ew & 0xFFFu) == p - xo ? (sub < tb ? sub : sub + 1) : ob;
+        }
+        if (one) {
+            v.add_dep(ri);
+            if (PSX_INFO_HEAD(in) && (snv ? ob == PSX_INFO_ALT(in) : car)) v.add_alt(ri);
+        }
+    };
+    const uint32_t i0 = i;
+    uint32_t qn = 0;  // wave-uniform
+    while (__ballot(go)) {
+        const uint32_t ic = go ? i : 0u;
+        const bool starts = go && (i == i0 || v.pos[ic] != v.pos[ic ? ic - 1u : 0u]);
+        const unsigned long long m = __ballot(starts);
+        const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+        if (starts) q[qn + below] = (lane << 26) | ic;
+        qn += (uint32_t)__popcll(m);
+        i += go ? 1u : 0u;
+        go = go && i < v.n && v.pos[v.at(i)] < end;
+        if (qn >= 64u) {
+            wave_sync();
+            drain(q + qn - 64u, 64u);
+            qn -= 64u;
+            wave_sync();
+        }
+    }
+    if (qn) {
+        wave_sync();
+        drain(q, qn);
+        wave_sync();
+    }
+}
+
 // Stratum geometry shared by both kernels.
 struct stratum {
     uint32_t b, e, wend, k0, n_win;
@@ -485,7 +568,9 @@ __global__ void __launch_bounds__(BLOCK) k_seq_count(const seq_launch L) {
     __shared__ uint16_t s_err[PSX_MAX_READ_ERRORS * BLOCK];
     __shared__ lds_tables s_tab;
     __shared__ uint32_t s_item, s_acc;
+    __shared__ uint32_t s_queue[BLOCK / 64][128];  // per wave: (lane, record) entries
     const uint32_t tid = threadIdx.x;
+    const uint32_t lane = tid & 63u, wave = tid >> 6;
     uint16_t* slots = s_err + tid;
     for (;;) {
         if (tid == 0) {
@@ -507,14 +592,18 @@ __global__ void __launch_bounds__(BLOCK) k_seq_count(const seq_launch L) {
             if (s.n_win <= (uint32_t)CAP) {
                 stage_window<BLOCK, CAP>(J, s, s_pos, s_lo, s_hi, s_info, s_alt, s_dep, s_bucket, tid);
                 const lds_view v{s_pos, s_lo, s_hi, s_info, s_alt, s_dep, s_bucket, s.n_win, s.b};
-                for (uint32_t r = tid; r < (uint32_t)s.nj; r += BLOCK) {
-                    tmpl t = make_template<CNA>(J, S, s_tab, r, j, s.b, s.e);
+                const uint32_t nj = (uint32_t)s.nj;
+                for (uint32_t base = wave * 64u; base < nj; base += BLOCK) {  // wave-uniform
+                    const uint32_t r = base + lane;
+                    bool valid = r < nj;
+                    tmpl t = make_template<CNA>(J, S, s_tab, valid ? r : base, j, s.b, s.e);
                     if (CNA) {
-                        if (!t.ok) continue;
-                        ++acc;
+                        valid = valid && t.ok;
+                        acc += valid ? 1u : 0u;
                     }
                     for (uint32_t mate = 0; mate <= J.pe; ++mate)
-                        count_read(J, S, v, s_tab, t, mate ? t.x2 : t.x, r, j, c2, mate, slots, BLOCK);
+                        count_mate_wave<BLOCK>(J, S, v, s_tab, t, valid, mate ? t.x2 : t.x, r, j, c2, mate, s_err,
+                                               s_queue[wave], lane, wave);
                 }
                 __syncthreads();
                 int32_t* ga = J.alt + S.out_off + s.k0;
@@ -595,12 +684,6 @@ __global__ void __launch_bounds__(256) k_seq_init(const job_dev* __restrict__ jo
 // the lane that drew it. A read is built in the lane's row of a wave-private
 // LDS stage and each pass is copied out by the whole wave with 16-byte
 // coalesced stores, so no block barrier separates generation from the copy.
-
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // One read into row[0 .. 2 + ceil(L/16)): word 0 = start, word 1 = leaf |
 // allele << 26 | mate << 30 | (errors > 0) << 31, then the 2-bit bases (base
