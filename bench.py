@@ -27,6 +27,13 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per launch of the dominant kernels on the plain_large workload, from rocprofv3 FETCH_SIZE and
+# WRITE_SIZE runs calibrated against a device copy, and their issue-slot use from SQ counters (the rocprofv3
+# summaries under profiles/, named for the kernel and workload). Count-only sequencing is bound by instruction
+# issue, not HBM.
+PROFILED = {"k_seq_count": dict(traffic=302.3e6, valu_busy=0.76, salu_busy=0.80),
+            "k_seq_reads": dict(traffic=77.63e9, valu_busy=0.57, salu_busy=0.57)}
+
 
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
@@ -264,7 +271,7 @@ def main() -> int:
     ap.add_argument("--cells", type=int, default=1_000_000)
     ap.add_argument("--no-batched", action="store_true", help="skip the batched workload")
     ap.add_argument("--batched-jobs", type=int, default=256)
-    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner in the batched workload")
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (plain and batched workloads)")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving workload")
     ap.add_argument("--no-cna", action="store_true", help="skip the copy-number workload")
     ap.add_argument("--serve-rate", type=float, default=20.0, help="request arrivals per second")
@@ -307,6 +314,18 @@ def main() -> int:
     def step():
         return core.simulate_seq_device(ctx, built.genome, built.forest, built.muts, w.seq_seed, alt, dep, pres, **seq)
 
+    tuned = None
+    if not args.no_tune:
+        # launch shape (block, LDS records, strata per item, blocks per CU) chosen by the autotuner on this workload
+        t0 = time.perf_counter()
+        job = core.SeqJob(ctx, built.genome, built.forest, built.muts, w.seq_seed, **seq)
+        cfg, best_ms = core.autotune(ctx, [job])
+        for a in (job.alt, job.depth, job.present):
+            a.close()
+        tuned = dict(block_threads=cfg.block_threads, lds_records=cfg.lds_records,
+                     strata_per_item=cfg.strata_per_item, blocks_per_cu=cfg.blocks_per_cu)
+        log(f"autotuned {tuned} in {time.perf_counter() - t0:.1f} s ({best_ms:.2f} ms)")
+
     for _ in range(args.warmup):
         step()
     ctx.synchronize()
@@ -330,8 +349,11 @@ def main() -> int:
     leaves = int(sum(int((w.forest["sample"] == s).sum()) for s in range(w.forest["n_samples"])))
     algo_bytes = 16 * M + 8 * M * S + 4 * leaves
     achieved = algo_bytes / (kernel_ms / 1e3) / 1e9
+    prof = PROFILED["k_seq_count"] if args.cells == 1_000_000 else {}
     roofline = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
-                    traffic=None, kernel="k_seq_count", kernel_ms=kernel_ms, algo_bytes=algo_bytes)
+                    traffic=prof.get("traffic"), kernel="k_seq_count", kernel_ms=kernel_ms, algo_bytes=algo_bytes,
+                    issue_bound=dict(valu_busy=prof.get("valu_busy"), salu_busy=prof.get("salu_busy"),
+                                     note="count-only moves ~0.04 B per read; VALU/SALU issue slots bound it"))
 
     out = {
         "metric": "simulated reads/s on 1 MI355X (plain vs batched); p50/p99 job latency",
@@ -349,7 +371,8 @@ def main() -> int:
         "config": {"workload": "plain_large: 1e6-cell two-species forest, 3 samples x 1e4 cells + normal, "
                                "300 Mb / 22 chr, SNV 1e-8, indel 1e-9, 200x PE 150 bp, error 1e-3, count-only",
                    "reads_per_step": reads_per_step, "mutations": M, "samples": S,
-                   "forest_nodes": len(w.forest["cell_id"]), "place_mutations_ms": built.place_ms},
+                   "forest_nodes": len(w.forest["cell_id"]), "place_mutations_ms": built.place_ms,
+                   "launch": tuned or "default"},
         "roofline": roofline,
     }
 
@@ -372,8 +395,10 @@ def main() -> int:
         ach = n_all * bytes_per_read / (rk / 1e3) / 1e9
         out["reads_mode"] = dict(value=n_all / rdt, unit="reads/s", ms_per_step=rdt * 1e3, kernel="k_seq_reads",
                                  roofline=dict(bound="hbm", achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s",
-                                               frac=ach / HBM_PEAK_GBS, traffic=None, kernel_ms=rk,
-                                               bytes_per_read=bytes_per_read))
+                                               frac=ach / HBM_PEAK_GBS,
+                                               traffic=PROFILED["k_seq_reads"]["traffic"]
+                                               if args.cells == 1_000_000 else None,
+                                               kernel_ms=rk, bytes_per_read=bytes_per_read))
         log(f"reads mode: {n_all} reads, kernel {rk:.2f} ms, {ach:.0f} GB/s written ({ach / HBM_PEAK_GBS:.1%})")
         buf.close()
 
