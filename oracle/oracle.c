@@ -165,12 +165,29 @@ int orc_binomial_table(uint32_t t, double p, uint32_t* k0, uint32_t* n, uint64_t
     return binomial_range(t, p, lo, hi, cap, k0, n, T);
 }
 
-/* Number of sequencing errors in one read: Binomial(L, e) on
- * [0, min(L, PSX_MAX_READ_ERRORS)]. */
-static int error_table(uint32_t L, double e, uint32_t* n, uint64_t* T) {
-    uint32_t k0;
-    double hi = L < PSX_MAX_READ_ERRORS ? (double)L : (double)PSX_MAX_READ_ERRORS;
-    return binomial_range(L, e, 0.0, hi, PSX_MAX_READ_ERRORS + 1, &k0, n, T);
+/* Templates per error chunk: the largest E of 64, 32, ..., 1 for which
+ * Binomial(E rpt L, e) exceeds PSX_ERR_KCAP with probability below 2^-40
+ * (the tail summed term by term from PSX_ERR_KCAP + 1); 0 if none. */
+static uint32_t error_chunk(uint32_t L, uint32_t rpt, double e) {
+    for (uint32_t E = PSX_ERR_CHUNK; E >= 1; E >>= 1) {
+        uint32_t N = E * rpt * L;
+        if (N <= PSX_ERR_KCAP) return E;
+        double lp = log(e), lq = log1p(-e), lgn = lgamma((double)N + 1.0), tail = 0.0;
+        for (uint32_t k = PSX_ERR_KCAP + 1; k <= N; ++k) {
+            double term = exp(lgn - lgamma((double)k + 1.0) - lgamma((double)(N - k) + 1.0) + k * lp +
+                              (double)(N - k) * lq);
+            tail += term;
+            if (tail >= 0x1p-40 || (term < 1e-300 && (double)k > (double)N * e)) break;
+        }
+        if (tail < 0x1p-40) return E;
+    }
+    return 0;
+}
+
+/* Errors in an error chunk of N bases: Binomial(N, e) on [0, min(N, PSX_ERR_KCAP)]. */
+static int error_table(uint32_t N, double e, uint32_t* k0, uint32_t* n, uint64_t* T) {
+    double hi = N < PSX_ERR_KCAP ? (double)N : (double)PSX_ERR_KCAP;
+    return binomial_range(N, e, 0.0, hi, PSX_ERR_KCAP + 1, k0, n, T);
 }
 
 /* get_bin_dist, src/seq_simulation.cpp:431-451: q = sd^2/mean, p = 1 - q,
@@ -785,8 +802,8 @@ typedef struct seq_ctx {
     double e;
     uint32_t ins_k0, ins_n;
     uint64_t ins_T[4096];
-    uint32_t err_n;
-    uint64_t err_T[PSX_MAX_READ_ERRORS + 1];
+    uint32_t err_E, err_N, err_k0, err_n;  /* templates and bases per error chunk, K table */
+    uint64_t err_T[PSX_ERR_KCAP + 1];
     const psx_cna_event* ev;  /* copy-number events, DFS order */
     uint32_t n_ev;
 } seq_ctx;
@@ -848,27 +865,29 @@ static int carried(const seq_ctx* s, const psx_mutation* r, uint32_t leaf, uint3
     return r->leaf_lo <= leaf && leaf < r->leaf_hi;
 }
 
-/* Error set of one read: K ~ Binomial(L, e) from word 0, then K distinct
- * offsets, one word each (offset = floor(w L / 2^32), substitution index =
- * floor((w & 0xFFFF) 3 / 2^16)); a repeated offset takes the next word. The
- * read's word stream is Philox(counter = (r, j, c2, ERROR << 24 | mate << 16 |
- * t / 4)), word t % 4; at most 4 * PSX_MAX_READ_ERRORS words are spent. */
-static uint32_t error_set(const seq_ctx* s, uint32_t r, uint32_t j, uint32_t c2, uint32_t mate,
-                          uint32_t* off, uint32_t* sub) {
+/* Sequencing errors (independent per base: the base is replaced by one of
+ * the three others, uniformly, with probability e), drawn per error chunk:
+ * templates [c E, c E + E) of stratum j, N = E rpt L bases, base index
+ * b = (t rpt + mate) L + offset for template c E + t. The chunk draws
+ * K ~ Binomial(N, e) by inversion from word y of Philox(counter = (c, j, c2,
+ * ERROR << 24)); stream word t = 0, 1, ... is word x of Philox(c, j, c2,
+ * ERROR << 24 | t) and gives a base index floor(w N / 2^32) and a
+ * substitution index floor((w & 0xFFFF) 3 / 2^16); an index already taken is
+ * skipped; at most 4 PSX_ERR_KCAP stream words are read. Returns the number
+ * of errors; b and sub receive them in draw order. */
+static uint32_t chunk_errors(const seq_ctx* s, uint32_t c, uint32_t j, uint32_t c2, uint32_t* b, uint32_t* sub) {
     uint32_t w[4];
-    uint32_t base = ((uint32_t)P_ERROR << 24) | (mate << 16);
-    philox4(s->seed, r, j, c2, base, w);
-    uint32_t K = pick(s->err_T, s->err_n, w[0]);
+    philox4(s->seed, c, j, c2, (uint32_t)P_ERROR << 24, w);
+    uint32_t K = s->err_k0 + pick(s->err_T, s->err_n, w[1]);
     uint32_t cnt = 0;
-    for (uint32_t t = 1; cnt < K && t < 1 + 4 * PSX_MAX_READ_ERRORS; ++t) {
-        if ((t & 3u) == 0) philox4(s->seed, r, j, c2, base | (t >> 2), w);
-        uint32_t word = w[t & 3u];
-        uint32_t o = mulhi32(word, s->L);
+    for (uint32_t t = 0; cnt < K && t < 4 * PSX_ERR_KCAP; ++t) {
+        philox4(s->seed, c, j, c2, ((uint32_t)P_ERROR << 24) | t, w);
+        uint32_t x = mulhi32(w[0], s->err_N);
         int dup = 0;
-        for (uint32_t q = 0; q < cnt; ++q) dup |= off[q] == o;
+        for (uint32_t q = 0; q < cnt; ++q) dup |= b[q] == x;
         if (dup) continue;
-        off[cnt] = o;
-        sub[cnt] = ((word & 0xFFFFu) * 3u) >> 16;
+        b[cnt] = x;
+        sub[cnt] = ((w[0] & 0xFFFFu) * 3u) >> 16;
         ++cnt;
     }
     return cnt;
@@ -887,16 +906,13 @@ static void add32(int32_t* p, int32_t v) {
  * carried SNV at the position, else the reference, then the read's error, if
  * any, substitutes it) or when the source carries the SID (indel). */
 static void seq_read(const seq_ctx* s, uint32_t c, uint32_t x, uint32_t leaf, uint32_t allele,
-                     uint32_t r, uint32_t j, uint32_t c2, uint32_t mate, int32_t* alt,
-                     int32_t* depth) {
+                     const uint32_t* off, const uint32_t* sub, uint32_t ne, int32_t* alt, int32_t* depth) {
     uint64_t lo = s->chr_begin[c], hi = s->chr_begin[c + 1];
     while (lo < hi) {
         uint64_t mid = lo + (hi - lo) / 2;
         if (s->m[mid].pos < x) lo = mid + 1; else hi = mid;
     }
     uint64_t i = lo, end = s->chr_begin[c + 1];
-    int have_err = 0;
-    uint32_t ne = 0, off[PSX_MAX_READ_ERRORS], sub[PSX_MAX_READ_ERRORS];
     while (i < end && s->m[i].pos < x + s->L) {
         uint32_t pos = s->m[i].pos;
         uint64_t g0 = i;
@@ -905,11 +921,8 @@ static void seq_read(const seq_ctx* s, uint32_t c, uint32_t x, uint32_t leaf, ui
         for (uint64_t q = g0; q < i; ++q)
             if (s->m[q].kind == PSX_SNV && carried(s, &s->m[q], leaf, allele)) { tb = s->m[q].alt; break; }
         uint32_t ob = tb;
-        if (s->e > 0.0) {
-            if (!have_err) { ne = error_set(s, r, j, c2, mate, off, sub); have_err = 1; }
-            for (uint32_t q = 0; q < ne; ++q)
-                if (off[q] == pos - x) ob = sub[q] < tb ? sub[q] : sub[q] + 1;
-        }
+        for (uint32_t q = 0; q < ne; ++q)
+            if (off[q] == pos - x) ob = sub[q] < tb ? sub[q] : sub[q] + 1;
         for (uint64_t q = g0; q < i; ++q) {
             add32(&depth[q], 1);
             if (!s->m[q].sid_head) continue;
@@ -961,7 +974,12 @@ int orc_simulate_seq_cna(uint32_t n_chr, const uint32_t* len, const orc_forest* 
     int rc = 0;
     if (n_ev && ploidy > PSX_MAX_CNA_PLOIDY) rc = fail("copy-number events need a small ploidy");
     if (s->pe) rc = insert_table(p->insert_size_mean, p->insert_size_stddev, &s->ins_k0, &s->ins_n, s->ins_T);
-    if (!rc && s->e > 0.0) rc = error_table(s->L, s->e, &s->err_n, s->err_T);
+    if (!rc && s->e > 0.0) {
+        s->err_E = error_chunk(s->L, s->pe ? 2u : 1u, s->e);
+        if (s->err_E == 0) rc = fail("error rate too high for %u-base reads", s->L);
+        s->err_N = s->err_E * (s->pe ? 2u : 1u) * s->L;
+        if (!rc) rc = error_table(s->err_N, s->e, &s->err_k0, &s->err_n, s->err_T);
+    }
     /* chromosome set: sorted, unique (std::set semantics of get_relevant_chr_set) */
     uint8_t sel[PSX_MAX_CHROMOSOMES];
     memset(sel, 0, sizeof sel);
@@ -1058,7 +1076,27 @@ int orc_simulate_seq_cna(uint32_t n_chr, const uint32_t* len, const orc_forest* 
         uint32_t c2 = (c << 16) | so;
         int32_t* a_out = alt + (uint64_t)so * n_mut;
         int32_t* d_out = depth + (uint64_t)so * n_mut;
+        /* errors of the current chunk, grouped by read slot t rpt + mate */
+        uint32_t rpt = s->pe ? 2u : 1u;
+        uint32_t eb[PSX_ERR_KCAP], es[PSX_ERR_KCAP], eoff[PSX_ERR_KCAP], esub[PSX_ERR_KCAP];
+        uint32_t first[2 * PSX_ERR_CHUNK + 1];
+        memset(first, 0, sizeof first);
         for (uint32_t r = 0; r < (uint32_t)nj; ++r) {
+            if (s->e > 0.0 && r % s->err_E == 0) {
+                uint32_t ne = chunk_errors(s, r / s->err_E, j, c2, eb, es);
+                uint32_t slots = s->err_E * rpt;
+                memset(first, 0, sizeof(uint32_t) * (slots + 1));
+                for (uint32_t q = 0; q < ne; ++q) ++first[eb[q] / s->L + 1];
+                for (uint32_t k = 0; k < slots; ++k) first[k + 1] += first[k];
+                uint32_t fill[2 * PSX_ERR_CHUNK];
+                memcpy(fill, first, sizeof(uint32_t) * slots);
+                for (uint32_t q = 0; q < ne; ++q) {  /* draw order within a read */
+                    uint32_t slot = eb[q] / s->L;
+                    eoff[fill[slot]] = eb[q] % s->L;
+                    esub[fill[slot]] = es[q];
+                    ++fill[slot];
+                }
+            }
             uint32_t w[4];
             philox4(seed, r, j, c2, (uint32_t)P_TEMPLATE << 24, w);
             uint32_t x = b + mulhi32(w[0], e - b);
@@ -1090,13 +1128,17 @@ int orc_simulate_seq_cna(uint32_t n_chr, const uint32_t* len, const orc_forest* 
                 while (allele + 1 < ploidy && k >= cum) cum += cn[++allele];
                 accepted += s->pe ? 2u : 1u;
             }
-            seq_read(s, c, x, leaf, allele, r, j, c2, 0, a_out, d_out);
+            uint32_t slot = s->e > 0.0 ? (r % s->err_E) * rpt : 0;
+            uint32_t n0 = s->e > 0.0 ? first[slot + 1] - first[slot] : 0;
+            seq_read(s, c, x, leaf, allele, eoff + first[slot], esub + first[slot], n0, a_out, d_out);
             if (s->pe) {
                 uint32_t I = s->ins_k0 + pick(s->ins_T, s->ins_n, w[3]);
                 if (I < s->L) I = s->L;
                 uint64_t x2 = (uint64_t)x + I - s->L;
                 if (x2 > len[c] - s->L) x2 = len[c] - s->L;
-                seq_read(s, c, (uint32_t)x2, leaf, allele, r, j, c2, 1, a_out, d_out);
+                uint32_t n1 = s->e > 0.0 ? first[slot + 2] - first[slot + 1] : 0;
+                seq_read(s, c, (uint32_t)x2, leaf, allele, eoff + first[slot + 1], esub + first[slot + 1], n1, a_out,
+                         d_out);
             }
         }
     }

@@ -246,9 +246,26 @@ psx_itable psx_binomial_table(uint32_t n, double p, uint32_t cap) {
     return binomial_range(n, p, lo, hi, cap);
 }
 
-psx_itable psx_error_table(uint32_t L, double e) {
-    const double hi = L < PSX_MAX_READ_ERRORS ? (double)L : (double)PSX_MAX_READ_ERRORS;
-    return binomial_range(L, e, 0.0, hi, PSX_MAX_READ_ERRORS + 1);
+uint32_t psx_error_chunk(uint32_t L, uint32_t rpt, double e) {
+    for (uint32_t E = PSX_ERR_CHUNK; E >= 1; E >>= 1) {
+        const uint32_t N = E * rpt * L;
+        if (N <= PSX_ERR_KCAP) return E;
+        const double lp = std::log(e), lq = std::log1p(-e), lgn = std::lgamma((double)N + 1.0);
+        double tail = 0.0;
+        for (uint32_t k = PSX_ERR_KCAP + 1; k <= N; ++k) {
+            const double term = std::exp(lgn - std::lgamma((double)k + 1.0) - std::lgamma((double)(N - k) + 1.0) +
+                                         k * lp + (double)(N - k) * lq);
+            tail += term;
+            if (tail >= 0x1p-40 || (term < 1e-300 && (double)k > (double)N * e)) break;
+        }
+        if (tail < 0x1p-40) return E;
+    }
+    return 0;
+}
+
+psx_itable psx_error_table(uint32_t N, double e) {
+    const double hi = N < PSX_ERR_KCAP ? (double)N : (double)PSX_ERR_KCAP;
+    return binomial_range(N, e, 0.0, hi, PSX_ERR_KCAP + 1);
 }
 
 // get_bin_dist (src/seq_simulation.cpp:431-451).

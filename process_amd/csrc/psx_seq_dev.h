@@ -22,7 +22,7 @@ constexpr uint32_t NB_SHIFT = 6;      // 64-base LDS buckets
 constexpr uint32_t MAX_READ = 4096;   // read size and insert reach limit
 constexpr uint32_t NB_MAX = (PSX_STRATUM + MAX_READ) / 64 + 2;
 constexpr uint32_t MAX_INS = 640;     // compact insert-size thresholds in LDS
-constexpr uint32_t MAX_ERRT = PSX_MAX_READ_ERRORS + 1;
+constexpr uint32_t MAX_ERRT = PSX_ERR_KCAP + 1;
 constexpr uint32_t REF_MAX = (PSX_STRATUM + MAX_READ) / 16 + 4;
 constexpr uint32_t UNRESOLVED = 0xFFFFFFFFu;
 
@@ -51,11 +51,13 @@ struct job_dev {
     uint8_t* present;
     const uint32_t* ins_T;       // compact insert-size thresholds
     const uint16_t* ins_guide;
-    const uint32_t* err_T;       // compact error-count thresholds (first value 0)
+    const uint32_t* err_T;       // compact thresholds of the errors per error chunk
     const uint32_t* packed;      // reference (read materialisation)
     uint64_t n_mut;
     uint64_t thr_normal;         // floor((1 - purity) 2^32)
-    uint32_t ins_k0, ins_n, err_n;
+    uint32_t ins_k0, ins_n;
+    uint32_t err_n, err_k0;      // error table (err_n == 0: no sequencing errors)
+    uint32_t err_E, err_N;       // templates and bases per error chunk
     uint32_t L, pe, ploidy, n_leaves, preneo, reach;
     uint32_t n_out;
     uint32_t purity_lt1;
@@ -146,6 +148,7 @@ struct lds_view {
     __device__ __forceinline__ uint32_t at(uint32_t i) const { return min(i, n - 1u); }  // n > 0
     __device__ __forceinline__ void add_dep(uint32_t q) const { atomicAdd(&dep[q], 1u); }
     __device__ __forceinline__ void add_alt(uint32_t q) const { atomicAdd(&alt[q], 1u); }
+    __device__ __forceinline__ void add_alt_n(uint32_t q, int d) const { atomicAdd(&alt[q], (uint32_t)d); }
 };
 
 struct glb_view {  // window larger than the LDS capacity
@@ -168,6 +171,7 @@ struct glb_view {  // window larger than the LDS capacity
     __device__ __forceinline__ uint32_t at(uint32_t i) const { return min(i, n - 1u); }  // n > 0
     __device__ __forceinline__ void add_dep(uint32_t q) const { atomicAdd(&dep[q], 1); }
     __device__ __forceinline__ void add_alt(uint32_t q) const { atomicAdd(&alt[q], 1); }
+    __device__ __forceinline__ void add_alt_n(uint32_t q, int d) const { atomicAdd(&alt[q], d); }
 };
 
 // Template: start of mate 1 and mate 2, source cell (resolved to a leaf on
@@ -271,69 +275,77 @@ __device__ __forceinline__ tmpl make_template(const job_dev& J, const seg_dev& S
     return t;
 }
 
-// Error set of one read (oracle: error_set): K ~ Binomial(L, e) from word 0,
-// then K distinct offsets, one word each; a repeated offset takes the next
-// word. Slots hold offset | substitution << 12.
-__device__ uint32_t error_set(const job_dev& J, const lds_tables& tab, uint32_t r, uint32_t j, uint32_t c2,
-                              uint32_t mate, uint16_t* slots, uint32_t stride) {
-    const uint32_t base = (PSX_P_ERROR << 24) | (mate << 16);
-    psx_u4 w = psx_draw_u(J.seed, r, j, c2, base);
-    uint32_t K = 0;
-    while (K + 1 < J.err_n && w.x >= tab.err[K]) ++K;
-    uint32_t cnt = 0;
-    for (uint32_t t = 1; cnt < K && t < 1 + 4 * PSX_MAX_READ_ERRORS; ++t) {
-        if ((t & 3u) == 0) w = psx_draw_u(J.seed, r, j, c2, base | (t >> 2));
-        const uint32_t word = psx_word(w, (int)(t & 3u));
-        const uint32_t o = psx_mulhi32(word, J.L);
-        bool dup = false;
-        for (uint32_t q = 0; q < cnt; ++q) dup |= (slots[q * stride] & 0xFFFu) == o;
-        if (dup) continue;
-        slots[cnt * stride] = (uint16_t)(o | ((((word & 0xFFFFu) * 3u) >> 16) << 12));
-        ++cnt;
-    }
-    return cnt;
-}
+// Per-wave LDS of the error lists: the errors of one chunk, then a 2048-bit
+// hashed bitmap of the base indices drawn so far.
+constexpr uint32_t ELIST = PSX_ERR_KCAP + 64u;
 
-// error_set with wave-uniform loops: every lane steps through the same
-// draws (the word counter t is uniform), effects are predicated, so the
-// draw sequence and the result of each lane are those of error_set.
-// Lanes with want == false return 0 without stepping the loops.
-__device__ __forceinline__ uint32_t error_set_u(const job_dev& J, const lds_tables& tab, uint32_t r, uint32_t j,
-                                                uint32_t c2, uint32_t mate, uint16_t* slots, uint32_t stride,
-                                                bool want = true) {
-    const uint32_t base = (PSX_P_ERROR << 24) | (mate << 16);
-    psx_u4 w = psx_draw_u(J.seed, r, j, c2, base);
-    uint32_t K = 0;
-    bool more = want && 1u < J.err_n && w.x >= tab.err[0];
-    while (__ballot(more)) {
-        K += more ? 1u : 0u;
-        more = more && K + 1u < J.err_n && w.x >= tab.err[min(K, J.err_n - 1u)];
-    }
-    uint32_t cnt = 0;
-    bool need = K > 0;
-    for (uint32_t t = 1; t < 1u + 4u * PSX_MAX_READ_ERRORS && __ballot(need); ++t) {
-        if ((t & 3u) == 0) w = psx_draw_u(J.seed, r, j, c2, base | (t >> 2));
-        const uint32_t word = psx_word(w, (int)(t & 3u));
-        const uint32_t o = psx_mulhi32(word, J.L);
-        bool dup = false;
-        const uint32_t qn = min(t - 1u, PSX_MAX_READ_ERRORS);
-        for (uint32_t q = 0; q < qn; ++q) dup |= q < cnt && (slots[q * stride] & 0xFFFu) == o;
-        const bool take = need && !dup;
-        if (take) slots[cnt * stride] = (uint16_t)(o | ((((word & 0xFFFFu) * 3u) >> 16) << 12));
-        cnt += take ? 1u : 0u;
-        need = cnt < K;
+// Sequencing errors of error chunk c of stratum j, generated by one wave
+// (oracle: chunk_errors): the count K by inversion from word y of the
+// chunk's first draw (a parallel compare against the thresholds), then the
+// chunk's stream words in batches, lane l drawing stream word t0 + l. A word
+// is new when no earlier stream word gives the same base index (an earlier
+// equal index was taken or was itself a repeat of a taken one), and the first
+// K new words are the errors. Repeats are rare: the bitmap flags a batch that
+// may hold one, and only such a batch runs the exact check. Writes the errors
+// to list[0 .. K) as b | sub << 20 and returns K (wave-uniform); every lane
+// steps through the same loops.
+__device__ __forceinline__ uint32_t chunk_errors(const job_dev& J, const lds_tables& tab, uint32_t c, uint32_t j,
+                                                 uint32_t c2, uint32_t* list, uint32_t lane) {
+    constexpr uint32_t TMAX = 4u * PSX_ERR_KCAP;  // stream words available
+    uint32_t* bits = list + PSX_ERR_KCAP;
+    uint32_t K = 0, cnt = 0;
+    for (uint32_t t0 = 0; t0 < TMAX;) {
+        const uint32_t t = t0 + lane;
+        const psx_u4 w = psx_draw_u(J.seed, c, j, c2, (PSX_P_ERROR << 24) | t);
+        if (t0 == 0) {
+            // K - err_k0 = the number of thresholds k < err_n - 1 with u >= T[k]
+            const uint32_t u = (uint32_t)__builtin_amdgcn_readfirstlane((int)w.y);
+            uint32_t k = 0;
+            for (uint32_t i0 = 0; i0 + 1u < J.err_n; i0 += 64u) {
+                const uint32_t i = i0 + lane;
+                k += (uint32_t)__popcll(__ballot(i + 1u < J.err_n && u >= tab.err[min(i, J.err_n - 1u)]));
+            }
+            K = J.err_k0 + k;
+            if (K == 0) return 0;
+            bits[lane] = 0u;
+            wave_sync();
+        }
+        const uint32_t B = min(64u, K - cnt);  // lanes [0, B) may be needed
+        const uint32_t x = psx_mulhi32(w.x, J.err_N);
+        const bool live = lane < B && t < TMAX;
+        const uint32_t h = (x * 0x9E3779B1u) >> 21;
+        const uint32_t seen = live ? atomicOr(&bits[h >> 5], 1u << (h & 31u)) & (1u << (h & 31u)) : 0u;
+        unsigned long long dupm = 0;
+        if (__ballot(seen != 0u)) {  // exact check against the batch and the earlier batches
+            for (uint32_t q = 0; q + 1u < B; ++q) {
+                const uint32_t xq = (uint32_t)__builtin_amdgcn_readlane((int)x, (int)q);
+                dupm |= __ballot(live && x == xq) & (~0ull << (q + 1u));
+            }
+            bool d = false;
+            for (uint32_t q = 0; q < cnt; ++q) d |= (list[q] & 0xFFFFFu) == x;
+            dupm |= __ballot(live && d);
+        }
+        const bool fresh = live && !((dupm >> lane) & 1ull);
+        const unsigned long long m = __ballot(fresh);
+        const uint32_t rank =
+            cnt + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+        if (fresh && rank < K) list[rank] = x | ((((w.x & 0xFFFFu) * 3u) >> 16) << 20);
+        cnt = min(K, cnt + (uint32_t)__popcll(m));
+        if (cnt == K) break;
+        t0 += B;
+        wave_sync();  // list entries of this batch seen by every lane
     }
     return cnt;
 }
 
 // Records of one position shared by several records, from index i (the
-// oracle's seq_read step): the first carried SNV sets the true base, the
-// read's error at the position substitutes it, every record gains depth and
-// each SID head an alternative read when its alternative is observed (SNV)
-// or carried (indel). Returns the index past the position.
+// oracle's seq_read step, errors aside): the first carried SNV sets the true
+// base, every record gains depth and each SID head an alternative read when
+// its alternative is the true base (SNV) or carried (indel). Returns the index
+// past the position.
 template <class V>
-__device__ __forceinline__ uint32_t count_group(const job_dev& J, const V& v, uint32_t i, uint32_t x, uint32_t allele,
-                                             uint32_t leaf, uint32_t ne, const uint16_t* slots, uint32_t stride) {
+__device__ __forceinline__ uint32_t count_group(const job_dev& J, const V& v, uint32_t i, uint32_t allele,
+                                                uint32_t leaf) {
     const uint32_t p = v.pos[i];
     const uint32_t g0 = i;
     uint32_t g1 = i + 1;
@@ -346,20 +358,12 @@ __device__ __forceinline__ uint32_t count_group(const job_dev& J, const V& v, ui
             break;
         }
     }
-    uint32_t ob = tb;
-    for (uint32_t q = 0; q < ne; ++q) {
-        const uint32_t e = slots[q * stride];
-        if ((e & 0xFFFu) == p - x) {
-            const uint32_t sub = e >> 12;
-            ob = sub < tb ? sub : sub + 1;
-        }
-    }
     for (uint32_t q = g0; q < g1; ++q) {
         v.add_dep(q);
         const uint32_t in = v.info[q];
         if (!PSX_INFO_HEAD(in)) continue;
         if (PSX_INFO_KIND(in) == PSX_SNV) {
-            if (ob == PSX_INFO_ALT(in)) v.add_alt(q);
+            if (tb == PSX_INFO_ALT(in)) v.add_alt(q);
         } else {
             bool car = false;
             for (uint32_t q2 = q; q2 < g1 && (q2 == q || !PSX_INFO_HEAD(v.info[q2])); ++q2) {
@@ -372,97 +376,141 @@ __device__ __forceinline__ uint32_t count_group(const job_dev& J, const V& v, ui
     return g1;
 }
 
-// One read [x, x + L): depth for every record at a covered position, an
-// alternative read for each SID head whose alternative is observed (SNV) or
-// carried (indel). Oracle: seq_read. A record alone at its position (every
-// record under the infinite-sites model) takes one branch-free step of a
-// wave-uniform loop; a position shared by several records goes through
-// count_group.
+// A sequencing error replacing the true base at the position of record i
+// (the first of its position) by substitution index sub: the counts above
+// took the true base, so each SNV head there changes its alternative count
+// by [observed == alt] - [true == alt]. Depth and indel counts do not depend
+// on the observed base. Oracle: seq_read with the read's errors.
 template <class V>
-__device__ __forceinline__ void count_read(const job_dev& J, const seg_dev& S, const V& v, const lds_tables& tab,
-                                           tmpl& t, uint32_t x, uint32_t r, uint32_t j, uint32_t c2, uint32_t mate,
-                                           uint16_t* slots, uint32_t stride) {
+__device__ __forceinline__ void error_fix(const job_dev& J, const V& v, uint32_t i, uint32_t allele, uint32_t leaf,
+                                          uint32_t sub) {
+    const uint32_t p = v.pos[i];
+    uint32_t g1 = i + 1;
+    while (g1 < v.n && v.pos[g1] == p) ++g1;
+    uint32_t tb = PSX_INFO_REF(v.info[i]);
+    for (uint32_t q = i; q < g1; ++q) {
+        const uint32_t in = v.info[q];
+        if (PSX_INFO_KIND(in) == PSX_SNV && PSX_INFO_ALLELE(in) == allele && carried(J, in, v.lo[q], v.hi[q], leaf)) {
+            tb = PSX_INFO_ALT(in);
+            break;
+        }
+    }
+    const uint32_t ob = sub < tb ? sub : sub + 1u;
+    for (uint32_t q = i; q < g1; ++q) {
+        const uint32_t in = v.info[q];
+        if (!PSX_INFO_HEAD(in) || PSX_INFO_KIND(in) != PSX_SNV) continue;
+        const int d = (ob == PSX_INFO_ALT(in) ? 1 : 0) - (tb == PSX_INFO_ALT(in) ? 1 : 0);
+        if (d) v.add_alt_n(q, d);
+    }
+}
+
+// The sequencing errors of a wave's templates [base, base + 64) (one per
+// lane: t, counted when valid), chunk by chunk, one error per lane: the
+// error's read is found from its base index, and an error at a position that
+// holds records corrects the counts there. A read covering that position
+// resolved its template's leaf while it was counted.
+template <class V>
+__device__ __forceinline__ void count_errors(const job_dev& J, const V& v, const lds_tables& tab, const tmpl& t,
+                                             bool valid, uint32_t base, uint32_t nj, uint32_t j, uint32_t c2,
+                                             uint32_t* list, uint32_t lane) {
+    const uint32_t E = J.err_E;
+    const uint32_t top = min(nj, base + 64u);
+    const uint32_t al = t.allele | (valid ? 0x80000000u : 0u);
+    for (uint32_t c = base / E; c * E < top; ++c) {  // E divides 64: the chunks of the wave's templates
+        wave_sync();
+        const uint32_t n = chunk_errors(J, tab, c, j, c2, list, lane);
+        wave_sync();
+        for (uint32_t k0 = 0; k0 < n; k0 += 64u) {
+            const uint32_t k = k0 + lane;
+            const uint32_t e = k < n ? list[k] : 0u;
+            const uint32_t b = e & 0xFFFFFu, sub = e >> 20;
+            const uint32_t slot = b / J.L;  // read of the chunk: template * (pe + 1) + mate
+            const uint32_t o = b - slot * J.L;
+            const uint32_t r = c * E + (slot >> J.pe);
+            const uint32_t src = (r - base) & 63u;
+            const uint32_t x1 = (uint32_t)__shfl((int)t.x, (int)src);
+            const uint32_t x2 = (uint32_t)__shfl((int)t.x2, (int)src);
+            const uint32_t ls = (uint32_t)__shfl((int)t.leaf, (int)src);
+            const uint32_t as = (uint32_t)__shfl((int)al, (int)src);
+            if (k < n && r < top && (as >> 31)) {  // rare: an error of a counted read
+                const uint32_t P = ((slot & J.pe) ? x2 : x1) + o;
+                const uint32_t i = v.first(P);
+                if (i < v.n && v.pos[i] == P) error_fix(J, v, i, as & 0x7FFFFFFFu, ls, sub);
+            }
+        }
+    }
+}
+
+// One read [x, x + L) counted with its true bases: depth for every record at
+// a covered position, an alternative read for each SID head whose
+// alternative is the true base (SNV) or carried (indel). Oracle: seq_read
+// without errors. A record alone at its position (every record under the
+// infinite-sites model) takes one branch-free step of a wave-uniform loop; a
+// position shared by several records goes through count_group. Lanes with
+// valid == false take no part. Returns whether any lane's read covers a
+// record (wave-uniform).
+template <class V>
+__device__ __forceinline__ bool count_read(const job_dev& J, const seg_dev& S, const V& v, tmpl& t, bool valid,
+                                           uint32_t x) {
     const uint32_t end = x + J.L;
     uint32_t i = v.first_u(x);
-    bool go = i < v.n && v.pos[v.at(i)] < end;
-    if (!__ballot(go)) return;
-    const uint32_t leaf = leaf_of(J, S, t);
-    const uint32_t ne = J.err_n ? error_set_u(J, tab, r, j, c2, mate, slots, stride, go) : 0u;
+    bool go = valid && i < v.n && v.pos[v.at(i)] < end;
+    if (!__ballot(go)) return false;
+    uint32_t leaf = 0;
+    if (go) leaf = leaf_of(J, S, t);
     while (__ballot(go)) {
         const uint32_t ic = go ? i : 0u;
-        const uint32_t p = v.pos[ic], in = v.info[ic];
+        const uint32_t in = v.info[ic];
         if (__ballot(go && !PSX_INFO_SOLO(in))) {
-            if (go && !PSX_INFO_SOLO(in)) i = count_group(J, v, ic, x, t.allele, leaf, ne, slots, stride);
+            if (go && !PSX_INFO_SOLO(in)) i = count_group(J, v, ic, t.allele, leaf);
         }
         const bool one = go && PSX_INFO_SOLO(in);
         const bool car = PSX_INFO_ALLELE(in) == t.allele && carried(J, in, v.lo[ic], v.hi[ic], leaf);
         const bool snv = PSX_INFO_KIND(in) == PSX_SNV;
         const uint32_t tb = snv && car ? PSX_INFO_ALT(in) : PSX_INFO_REF(in);
-        uint32_t ob = tb;
-        for (uint32_t q = 0; __ballot(one && q < ne); ++q) {
-            const bool on = one && q < ne;
-            const uint32_t e = on ? slots[q * stride] : 0u;
-            const uint32_t sub = e >> 12;
-            ob = on && (e & 0xFFFu) == p - x ? (sub < tb ? sub : sub + 1) : ob;
-        }
         if (one) {
             v.add_dep(ic);
-            if (PSX_INFO_HEAD(in) && (snv ? ob == PSX_INFO_ALT(in) : car)) v.add_alt(ic);
+            if (PSX_INFO_HEAD(in) && (snv ? tb == PSX_INFO_ALT(in) : car)) v.add_alt(ic);
         }
         i += one ? 1u : 0u;
         go = go && i < v.n && v.pos[v.at(i)] < end;
     }
+    return true;
 }
 
-// One mate of 64 templates of a wave (one per lane): every lane finds the
-// positions its read covers and queues (lane, first record of the position)
-// in the wave's LDS queue; the queue is drained 64 entries at a time, one
-// per lane, so the per-position work runs on full waves instead of waiting
-// for the lane whose read covers the most positions. Same counts as
-// count_read. Lanes with valid == false take no part (x must still lie in
-// the stratum).
-template <int BLOCK>
-__device__ __forceinline__ void count_mate_wave(const job_dev& J, const seg_dev& S, const lds_view& v,
-                                                const lds_tables& tab, tmpl& t, bool valid, uint32_t x, uint32_t r,
-                                                uint32_t j, uint32_t c2, uint32_t mate, uint16_t* s_err,
-                                                uint32_t* q, uint32_t lane, uint32_t wave) {
+// One mate of 64 templates of a wave (one per lane), counted with true bases:
+// every lane finds the positions its read covers and queues (lane, first
+// record of the position) in the wave's LDS queue; the queue is drained 64
+// entries at a time, one per lane, so the per-position work runs on full
+// waves instead of waiting for the lane whose read covers the most positions.
+// Same counts and result as count_read. Lanes with valid == false take no
+// part (x must still lie in the stratum).
+__device__ __forceinline__ bool count_mate_wave(const job_dev& J, const seg_dev& S, const lds_view& v, tmpl& t,
+                                                bool valid, uint32_t x, uint32_t* q, uint32_t lane) {
     const uint32_t end = x + J.L;
     uint32_t i = v.first_u(x);
     bool go = valid && i < v.n && v.pos[v.at(i)] < end;
-    if (!__ballot(go)) return;
+    if (!__ballot(go)) return false;
     uint32_t leaf = 0;
     if (go) leaf = leaf_of(J, S, t);
-    uint16_t* wslots = s_err + wave * 64u;  // slot q of lane l: wslots[l + q * BLOCK]
-    const uint32_t ne = J.err_n ? error_set_u(J, tab, r, j, c2, mate, wslots + lane, BLOCK, go) : 0u;
-    const uint32_t an = t.allele | (ne << 8);
     // one drain: entries qb[0 .. count), one per lane
     auto drain = [&](const uint32_t* qb, uint32_t count) {
         const bool on = lane < count;
         const uint32_t e = on ? qb[lane] : 0u;
         const uint32_t ol = e >> 26, ri = e & 0x3FFFFFFu;
-        const uint32_t xo = (uint32_t)__shfl((int)x, (int)ol);
         const uint32_t lo = (uint32_t)__shfl((int)leaf, (int)ol);
-        const uint32_t ao = (uint32_t)__shfl((int)an, (int)ol);
-        const uint32_t al = ao & 0xFFu, neo = ao >> 8;
-        const uint16_t* sl = wslots + ol;
-        const uint32_t in = v.info[ri], p = v.pos[ri];
+        const uint32_t al = (uint32_t)__shfl((int)t.allele, (int)ol);
+        const uint32_t in = v.info[ri];
         if (__ballot(on && !PSX_INFO_SOLO(in))) {
-            if (on && !PSX_INFO_SOLO(in)) count_group(J, v, ri, xo, al, lo, neo, sl, BLOCK);
+            if (on && !PSX_INFO_SOLO(in)) count_group(J, v, ri, al, lo);
         }
         const bool one = on && PSX_INFO_SOLO(in);
         const bool car = PSX_INFO_ALLELE(in) == al && carried(J, in, v.lo[ri], v.hi[ri], lo);
         const bool snv = PSX_INFO_KIND(in) == PSX_SNV;
         const uint32_t tb = snv && car ? PSX_INFO_ALT(in) : PSX_INFO_REF(in);
-        uint32_t ob = tb;
-        for (uint32_t k = 0; __ballot(one && k < neo); ++k) {
-            const bool hit = one && k < neo;
-            const uint32_t ew = hit ? sl[k * BLOCK] : 0u;
-            const uint32_t sub = ew >> 12;
-            ob = hit && (ew & 0xFFFu) == p - xo ? (sub < tb ? sub : sub + 1) : ob;
-        }
         if (one) {
             v.add_dep(ri);
-            if (PSX_INFO_HEAD(in) && (snv ? ob == PSX_INFO_ALT(in) : car)) v.add_alt(ri);
+            if (PSX_INFO_HEAD(in) && (snv ? tb == PSX_INFO_ALT(in) : car)) v.add_alt(ri);
         }
     };
     const uint32_t i0 = i;
@@ -488,6 +536,7 @@ __device__ __forceinline__ void count_mate_wave(const job_dev& J, const seg_dev&
         drain(q, qn);
         wave_sync();
     }
+    return true;
 }
 
 // Stratum geometry shared by both kernels.
@@ -544,18 +593,22 @@ __device__ __forceinline__ void stage_window(const job_dev& J, const stratum& s,
 // code is shared by both mates).
 template <int BLOCK, bool CNA>
 __device__ __forceinline__ void count_stratum_global(const job_dev& J, const seg_dev& S, const lds_tables& tab,
-                                                  const stratum& s, uint32_t j, uint32_t c2, uint16_t* slots,
-                                                  uint32_t tid, uint32_t* acc) {
+                                                  const stratum& s, uint32_t j, uint32_t c2, uint32_t* list,
+                                                  uint32_t lane, uint32_t wave, uint32_t* acc) {
     const glb_view v{J.pos + s.k0, J.lo + s.k0, J.hi + s.k0, J.info + s.k0, J.alt + S.out_off + s.k0,
                      J.depth + S.out_off + s.k0, s.n_win};
-    for (uint32_t r = tid; r < (uint32_t)s.nj; r += BLOCK) {
-        tmpl t = make_template<CNA>(J, S, tab, r, j, s.b, s.e);
+    const uint32_t nj = (uint32_t)s.nj;
+    for (uint32_t base = wave * 64u; base < nj; base += BLOCK) {  // wave-uniform
+        const uint32_t r = base + lane;
+        bool valid = r < nj;
+        tmpl t = make_template<CNA>(J, S, tab, valid ? r : base, j, s.b, s.e);
         if (CNA) {
-            if (!t.ok) continue;
-            *acc += 1u;
+            valid = valid && t.ok;
+            *acc += valid ? 1u : 0u;
         }
-        for (uint32_t mate = 0; mate <= J.pe; ++mate)
-            count_read(J, S, v, tab, t, mate ? t.x2 : t.x, r, j, c2, mate, slots, BLOCK);
+        bool hit = false;  // errors only matter where a read covers a record
+        for (uint32_t mate = 0; mate <= J.pe; ++mate) hit |= count_read(J, S, v, t, valid, mate ? t.x2 : t.x);
+        if (J.err_n && hit) count_errors(J, v, tab, t, valid, base, nj, j, c2, list, lane);
     }
 }
 
@@ -565,13 +618,12 @@ template <int BLOCK, int CAP, bool CNA>
 __global__ void __launch_bounds__(BLOCK) k_seq_count(const seq_launch L) {
     __shared__ uint32_t s_pos[CAP], s_lo[CAP], s_hi[CAP], s_info[CAP], s_alt[CAP], s_dep[CAP];
     __shared__ uint16_t s_bucket[NB_MAX];
-    __shared__ uint16_t s_err[PSX_MAX_READ_ERRORS * BLOCK];
     __shared__ lds_tables s_tab;
     __shared__ uint32_t s_item, s_acc;
-    __shared__ uint32_t s_queue[BLOCK / 64][128];  // per wave: (lane, record) entries
+    // per wave: the (lane, record) queue of a mate, then the errors of a chunk
+    __shared__ uint32_t s_wave[BLOCK / 64][ELIST];
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63u, wave = tid >> 6;
-    uint16_t* slots = s_err + tid;
     for (;;) {
         if (tid == 0) {
             s_item = (uint32_t)atomicAdd(L.queue, 1ull);
@@ -601,9 +653,10 @@ __global__ void __launch_bounds__(BLOCK) k_seq_count(const seq_launch L) {
                         valid = valid && t.ok;
                         acc += valid ? 1u : 0u;
                     }
+                    bool hit = false;  // errors only matter where a read covers a record
                     for (uint32_t mate = 0; mate <= J.pe; ++mate)
-                        count_mate_wave<BLOCK>(J, S, v, s_tab, t, valid, mate ? t.x2 : t.x, r, j, c2, mate, s_err,
-                                               s_queue[wave], lane, wave);
+                        hit |= count_mate_wave(J, S, v, t, valid, mate ? t.x2 : t.x, s_wave[wave], lane);
+                    if (J.err_n && hit) count_errors(J, v, s_tab, t, valid, base, nj, j, c2, s_wave[wave], lane);
                 }
                 __syncthreads();
                 int32_t* ga = J.alt + S.out_off + s.k0;
@@ -618,7 +671,7 @@ __global__ void __launch_bounds__(BLOCK) k_seq_count(const seq_launch L) {
                 }
                 __syncthreads();
             } else {
-                count_stratum_global<BLOCK, CNA>(J, S, s_tab, s, j, c2, slots, tid, &acc);
+                count_stratum_global<BLOCK, CNA>(J, S, s_tab, s, j, c2, s_wave[wave], lane, wave, &acc);
             }
         }
         if (CNA && S.n_cna) {
@@ -686,16 +739,15 @@ __global__ void __launch_bounds__(256) k_seq_init(const job_dev* __restrict__ jo
 // coalesced stores, so no block barrier separates generation from the copy.
 
 // One read into row[0 .. 2 + ceil(L/16)): word 0 = start, word 1 = leaf |
-// allele << 26 | mate << 30 | (errors > 0) << 31, then the 2-bit bases (base
-// i of the read in bits 2 (i % 16) of word 2 + i / 16) taken from the staged
-// reference (s_ref[0] is reference word ref_b / 16), with the first carried
-// SNV at each position and the read's errors applied; bases past the read
-// end are 0. Oracle: the bases seq_read compares, materialised.
+// allele << 26 | mate << 30 (bit 31, set later, flags a read with errors),
+// then the 2-bit true bases (base i of the read in bits 2 (i % 16) of word
+// 2 + i / 16) taken from the staged reference (s_ref[0] is reference word
+// ref_b / 16) with the first carried SNV at each position applied; bases
+// past the read end are 0. The read's sequencing errors are applied to the
+// row afterwards (apply_error). Oracle: the bases seq_read compares.
 template <int NW, class V>
-__device__ __forceinline__ void write_read(const job_dev& J, const V& v, const lds_tables& tab, uint32_t x,
-                                           uint32_t leaf, uint32_t allele, uint32_t r, uint32_t j, uint32_t c2,
-                                           uint32_t mate, uint16_t* slots, uint32_t stride, const uint32_t* s_ref,
-                                           uint32_t ref_b, uint32_t* row) {
+__device__ __forceinline__ void write_read(const job_dev& J, const V& v, uint32_t x, uint32_t leaf, uint32_t allele,
+                                           uint32_t mate, const uint32_t* s_ref, uint32_t ref_b, uint32_t* row) {
     const uint32_t tail = J.L & 15u;
     const uint32_t xr = x - ref_b;
     const uint32_t w = xr >> 4;
@@ -743,18 +795,28 @@ __device__ __forceinline__ void write_read(const job_dev& J, const V& v, const l
         i += go ? 1u : 0u;
         go = go && i < v.n && v.pos[v.at(i)] < end;
     }
-    uint32_t ne = 0;
-    if (J.err_n) ne = error_set_u(J, tab, r, j, c2, mate, slots, stride);
-    for (uint32_t q = 0; q < ne; ++q) {  // rare: about one read in seven at e = 1e-3
-        const uint32_t e = slots[q * stride];
-        const uint32_t o = e & 0xFFFu, s2 = (o & 15u) * 2u;
-        const uint32_t wv = row[2 + (o >> 4)];
-        const uint32_t tb = (wv >> s2) & 3u, sub = e >> 12;
-        const uint32_t nb = sub < tb ? sub : sub + 1;
-        row[2 + (o >> 4)] = (wv & ~(3u << s2)) | (nb << s2);
-    }
     if constexpr (NW == 0) row[0] = x;
-    row[1] = leaf | (allele << 26) | (mate << 30) | ((ne > 0 ? 1u : 0u) << 31);
+    row[1] = leaf | (allele << 26) | (mate << 30);
+}
+
+// A sequencing error at offset o of a written read: the base there is
+// replaced by substitution index sub among the three other bases, and the
+// read is flagged. ATOMIC: the row is shared with other lanes that may patch
+// other bases of the same word at the same time (no two errors share a base,
+// so the base read here is stable).
+template <bool ATOMIC>
+__device__ __forceinline__ void apply_error(uint32_t* row, uint32_t o, uint32_t sub) {
+    uint32_t* wp = row + 2 + (o >> 4);
+    const uint32_t s2 = (o & 15u) * 2u;
+    const uint32_t tb = (*wp >> s2) & 3u;
+    const uint32_t nb = sub < tb ? sub : sub + 1u;
+    if (ATOMIC) {
+        atomicXor(wp, (tb ^ nb) << s2);
+        atomicOr(row + 1, 0x80000000u);
+    } else {
+        *wp ^= (tb ^ nb) << s2;
+        row[1] |= 0x80000000u;
+    }
 }
 
 // STAGE: reads are built in LDS and copied out per pass (read_words small
@@ -766,14 +828,14 @@ __global__ void __launch_bounds__(BLOCK) k_seq_reads(const seq_launch L) {
     constexpr uint32_t WAVES = BLOCK / 64;
     __shared__ uint32_t s_pos[CAP], s_lo[CAP], s_hi[CAP], s_info[CAP];
     __shared__ uint16_t s_bucket[NB_MAX];
-    __shared__ uint16_t s_err[PSX_MAX_READ_ERRORS * BLOCK];
     __shared__ uint32_t s_ref[REF_MAX];
     __shared__ lds_tables s_tab;
     __shared__ uint32_t s_item;
+    __shared__ uint32_t s_elist[WAVES][ELIST];  // per wave: errors of one chunk
     extern __shared__ uint4 s_stage4[];  // STAGE: per wave 64 rows of read_words words
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63u, wave = tid >> 6;
-    uint16_t* slots = s_err + tid;
+    uint32_t* list = s_elist[wave];
     const uint32_t RW = L.read_words;
     const bool vec4 = L.out_vec4 != 0;  // RW % 4 == 0 and a 16-byte aligned output
     uint32_t* stage = reinterpret_cast<uint32_t*>(s_stage4) + wave * 64u * RW;
@@ -805,6 +867,7 @@ __global__ void __launch_bounds__(BLOCK) k_seq_reads(const seq_launch L) {
             const glb_view gv{J.pos + s.k0, J.lo + s.k0, J.hi + s.k0, J.info + s.k0, nullptr, nullptr, s.n_win};
             const uint32_t nj = (uint32_t)s.nj;
             for (uint32_t c0 = wave * 64u; c0 < nj; c0 += WAVES * 64u) {
+                uint32_t chunk = UNRESOLVED, n_err = 0;  // the chunk in the wave's list
                 const uint32_t rt = c0 + lane;
                 uint32_t tx = 0, tx2 = 0, tla = 0;
                 if (rt < nj) {
@@ -827,11 +890,44 @@ __global__ void __launch_bounds__(BLOCK) k_seq_reads(const seq_launch L) {
                     if (r < nj) {
                         const uint32_t x = mate ? x2 : x1;
                         if (fits)
-                            write_read<NW>(J, lv, s_tab, x, la & 0x3FFFFFFu, la >> 26, r, j, c2, mate, slots,
-                                           BLOCK, s_ref, ref_w0 << 4, row);
+                            write_read<NW>(J, lv, x, la & 0x3FFFFFFu, la >> 26, mate, s_ref, ref_w0 << 4, row);
                         else
-                            write_read<NW>(J, gv, s_tab, x, la & 0x3FFFFFFu, la >> 26, r, j, c2, mate, slots,
-                                           BLOCK, s_ref, ref_w0 << 4, row);
+                            write_read<NW>(J, gv, x, la & 0x3FFFFFFu, la >> 26, mate, s_ref, ref_w0 << 4, row);
+                    }
+                    if (J.err_n && nt) {
+                        // errors of the pass's templates [t0, t0 + nt), chunk by chunk (E divides 64, so
+                        // a chunk lies in one pass or spans both passes of a wave step)
+                        const uint32_t E = J.err_E;
+                        for (uint32_t c = t0 / E; c * E < t0 + nt; ++c) {
+                            if (c != chunk) {
+                                wave_sync();
+                                n_err = chunk_errors(J, s_tab, c, j, c2, list, lane);
+                                chunk = c;
+                            }
+                            wave_sync();  // list entries and (STAGE) every row of the pass written
+                            for (uint32_t k0 = 0; k0 < n_err; k0 += 64u) {
+                                const uint32_t k = k0 + lane;
+                                const uint32_t e = k < n_err ? list[k] : 0u;
+                                const uint32_t b = e & 0xFFFFFu, sub = e >> 20;
+                                const uint32_t slot = b / J.L;  // read of the chunk
+                                const uint32_t o = b - slot * J.L;
+                                const uint32_t tr = c * E + (slot >> J.pe);
+                                const uint32_t rl = (tr - t0) * rpt + (slot & J.pe);  // its lane in the pass
+                                const bool on = k < n_err && tr >= t0 && tr < t0 + nt;
+                                if (STAGE) {
+                                    if (on) apply_error<true>(stage + rl * RW, o, sub);
+                                } else {
+                                    // rows live in global memory: each lane patches its own read
+                                    for (uint32_t q = 0; q < 64u && k0 + q < n_err; ++q) {
+                                        const uint32_t rq = (uint32_t)__shfl((int)rl, (int)q);
+                                        const uint32_t oq = (uint32_t)__shfl((int)o, (int)q);
+                                        const uint32_t sq = (uint32_t)__shfl((int)sub, (int)q);
+                                        const uint32_t onq = (uint32_t)__shfl(on ? 1 : 0, (int)q);
+                                        if (onq && rq == lane) apply_error<false>(row, oq, sq);
+                                    }
+                                }
+                            }
+                        }
                     }
                     if (STAGE) {
                         wave_sync();

@@ -32,7 +32,7 @@ from process_amd import core  # noqa: E402
 
 M32 = 0xFFFFFFFF
 STRATUM = 8192
-MAX_ERR = 16
+ERR_CHUNK, ERR_KCAP = 64, 128
 TEMPLATE, ERROR = 0x10, 0x11
 
 
@@ -96,8 +96,28 @@ def insert_table(mean: int, sd: int) -> tuple[int, list[int]]:
     return binomial_table(int(mean / p), p)
 
 
-def error_table(L: int, e: float) -> list[int]:
-    return binomial_range(L, e, 0.0, float(min(L, MAX_ERR)))[1]
+def error_chunk(L: int, rpt: int, e: float) -> int:
+    """Templates per error chunk: the largest power of two <= 64 whose chunk of
+    N = E rpt L bases holds more than 256 errors with probability below 2^-40."""
+    E = ERR_CHUNK
+    while E >= 1:
+        N = E * rpt * L
+        if N <= ERR_KCAP:
+            return E
+        lp, lq, lgn, tail = math.log(e), math.log1p(-e), math.lgamma(N + 1.0), 0.0
+        for k in range(ERR_KCAP + 1, N + 1):
+            term = math.exp(lgn - math.lgamma(k + 1.0) - math.lgamma(N - k + 1.0) + k * lp + (N - k) * lq)
+            tail += term
+            if tail >= 2.0 ** -40 or (term < 1e-300 and k > N * e):
+                break
+        if tail < 2.0 ** -40:
+            return E
+        E //= 2
+    return 0
+
+
+def error_table(N: int, e: float) -> list[int]:
+    return binomial_range(N, e, 0.0, float(min(N, ERR_KCAP)))[1]
 
 
 def pick(T: list[int], u: int) -> int:
@@ -141,7 +161,10 @@ def brute_force_seq(lens, packed, forest, recs, seed, ploidy=2, cna=None, **p):
     M = len(recs)
     woff = np.concatenate([[0], np.cumsum((np.asarray(lens, np.int64) + 15) // 16)])
     ins_k0, ins_T = insert_table(p["insert_size_mean"], p.get("insert_size_stddev", 10)) if pe else (0, [])
-    err_T = error_table(L, e) if e > 0 else []
+    rpt = 2 if pe else 1
+    err_E = error_chunk(L, rpt, e) if e > 0 else 0
+    err_N = err_E * rpt * L
+    err_T = error_table(err_N, e) if e > 0 else []
     thr_normal = int((1.0 - purity) * 4294967296.0)
     head = recs["sid_head"].astype(bool)
     by_pos: dict[tuple[int, int], list[int]] = {}
@@ -172,23 +195,22 @@ def brute_force_seq(lens, packed, forest, recs, seed, ploidy=2, cna=None, **p):
                 has = any(int(recs["leaf_lo"][q]) <= v < int(recs["leaf_hi"][q]) for v in leaves)
             pres[so, q] = has
 
-    def errors(r, j, c2, mate):
-        base = (ERROR << 24) | (mate << 16)
-        w = philox(seed, r, j, c2, base)
-        K = pick(err_T, w[0])
-        out: list[tuple[int, int]] = []
-        t = 1
-        while len(out) < K and t < 1 + 4 * MAX_ERR:
-            if t % 4 == 0:
-                w = philox(seed, r, j, c2, base | (t >> 2))
-            word = w[t % 4]
-            o = mulhi(word, L)
-            if all(o != oo for oo, _ in out):
-                out.append((o, ((word & 0xFFFF) * 3) >> 16))
+    def chunk_errors(c, j, c2):
+        """{(template in chunk, mate): [(offset, substitution), ...]} of error chunk c."""
+        K = pick(err_T, philox(seed, c, j, c2, ERROR << 24)[1])
+        taken: list[int] = []
+        out: dict[tuple[int, int], list[tuple[int, int]]] = {}
+        t = 0
+        while len(taken) < K and t < 4 * ERR_KCAP:
+            word = philox(seed, c, j, c2, (ERROR << 24) | t)[0]
+            b = mulhi(word, err_N)
+            if b not in taken:
+                taken.append(b)
+                out.setdefault((b // (rpt * L), (b // L) % rpt), []).append((b % L, ((word & 0xFFFF) * 3) >> 16))
             t += 1
         return out
 
-    def read(so, c, x, leaf, allele, r, j, c2, mate):
+    def read(so, c, x, leaf, allele, errs):
         # the source genome's bases over [x, x + L): reference, then the
         # first carried SNV at each position
         true = [base_at(packed, woff, c, x + i) for i in range(L)]
@@ -198,9 +220,8 @@ def brute_force_seq(lens, packed, forest, recs, seed, ploidy=2, cna=None, **p):
                     true[i] = int(recs["alt"][q])
                     break
         obs = list(true)
-        if e > 0:
-            for o, sub in errors(r, j, c2, mate):
-                obs[o] = sub if sub < true[o] else sub + 1
+        for o, sub in errs:
+            obs[o] = sub if sub < true[o] else sub + 1
         for i in range(L):
             qs = by_pos.get((c, x + i), [])
             for k, q in enumerate(qs):
@@ -241,7 +262,10 @@ def brute_force_seq(lens, packed, forest, recs, seed, ploidy=2, cna=None, **p):
             c2 = (c << 16) | so
             for j in range((W + STRATUM - 1) // STRATUM):
                 b, e_ = j * STRATUM, min(j * STRATUM + STRATUM, W)
+                chunk: dict = {}
                 for r in range(T * e_ // W - T * b // W):
+                    if e > 0 and r % err_E == 0:
+                        chunk = chunk_errors(r // err_E, j, c2)
                     w = philox(seed, r, j, c2, TEMPLATE << 24)
                     x = b + mulhi(w[0], e_ - b)
                     if not has_ev:
@@ -263,10 +287,11 @@ def brute_force_seq(lens, packed, forest, recs, seed, ploidy=2, cna=None, **p):
                             continue
                         allele = int(np.searchsorted(cum, k, side="right"))
                         n_reads += 2 if pe else 1
-                    read(so, c, x, leaf, allele, r, j, c2, 0)
+                    tl = r % err_E if e > 0 else 0
+                    read(so, c, x, leaf, allele, chunk.get((tl, 0), []))
                     if pe:
                         I = max(ins_k0 + pick(ins_T, w[3]), L)
-                        read(so, c, min(x + I - L, ln - L), leaf, allele, r, j, c2, 1)
+                        read(so, c, min(x + I - L, ln - L), leaf, allele, chunk.get((tl, 1), []))
     return alt, dep, pres, n_reads
 
 
