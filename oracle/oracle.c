@@ -806,6 +806,14 @@ typedef struct seq_ctx {
     uint64_t err_T[PSX_ERR_KCAP + 1];
     const psx_cna_event* ev;  /* copy-number events, DFS order */
     uint32_t n_ev;
+    /* read materialisation (rows != NULL): the packed reference, the first
+     * word of each chromosome, rw words per read, the first read of each
+     * (output sample, chromosome) segment */
+    const uint32_t* packed;
+    uint64_t chr_woff[PSX_MAX_CHROMOSOMES];
+    uint32_t* rows;
+    uint32_t rw;
+    uint64_t* seg_base;
 } seq_ctx;
 
 /* Copy numbers of the allele lineages of `leaf` at (c, x): one copy each,
@@ -893,6 +901,42 @@ static uint32_t chunk_errors(const seq_ctx* s, uint32_t c, uint32_t j, uint32_t 
     return cnt;
 }
 
+/* One materialised read (the layout of psx_simulate_seq_reads): word 0 = start,
+ * word 1 = leaf | allele << 26 | mate << 30 | (errors > 0) << 31, then base i
+ * of the read in bits 2 (i % 16) of word 2 + i / 16: the reference base, the
+ * first carried SNV at the position (table order), then the read's errors;
+ * bits past the read end are 0. */
+static void read_row(const seq_ctx* s, uint32_t c, uint32_t x, uint32_t leaf, uint32_t allele, uint32_t mate,
+                     const uint32_t* off, const uint32_t* sub, uint32_t ne, uint32_t* row) {
+    memset(row, 0, sizeof(uint32_t) * s->rw);
+    row[0] = x;
+    row[1] = leaf | (allele << 26) | (mate << 30) | ((ne > 0 ? 1u : 0u) << 31);
+    uint32_t* bases = row + 2;
+    for (uint32_t i = 0; i < s->L; ++i) {
+        uint32_t p = x + i;
+        uint32_t b = (s->packed[s->chr_woff[c] + p / 16] >> (2 * (p % 16))) & 3u;
+        bases[i / 16] |= b << (2 * (i % 16));
+    }
+    uint64_t lo = s->chr_begin[c], hi = s->chr_begin[c + 1];
+    while (lo < hi) {
+        uint64_t mid = lo + (hi - lo) / 2;
+        if (s->m[mid].pos < x) lo = mid + 1; else hi = mid;
+    }
+    uint32_t last = 0xFFFFFFFFu;
+    for (uint64_t i = lo; i < s->chr_begin[c + 1] && s->m[i].pos < x + s->L; ++i) {
+        const psx_mutation* r = &s->m[i];
+        if (r->pos == last || r->kind != PSX_SNV || !carried(s, r, leaf, allele)) continue;
+        uint32_t o = r->pos - x;
+        bases[o / 16] = (bases[o / 16] & ~(3u << (2 * (o % 16)))) | ((uint32_t)r->alt << (2 * (o % 16)));
+        last = r->pos;
+    }
+    for (uint32_t q = 0; q < ne; ++q) {
+        uint32_t o = off[q], tb = (bases[o / 16] >> (2 * (o % 16))) & 3u;
+        uint32_t nb = sub[q] < tb ? sub[q] : sub[q] + 1;
+        bases[o / 16] = (bases[o / 16] & ~(3u << (2 * (o % 16)))) | (nb << (2 * (o % 16)));
+    }
+}
+
 static void add32(int32_t* p, int32_t v) {
 #ifdef _OPENMP
 #pragma omp atomic
@@ -955,11 +999,12 @@ int orc_simulate_seq(uint32_t n_chr, const uint32_t* len, const orc_forest* f,
     return orc_simulate_seq_cna(n_chr, len, f, m, n_mut, ploidy, NULL, 0, p, seed, alt, depth, present, n_reads);
 }
 
-int orc_simulate_seq_cna(uint32_t n_chr, const uint32_t* len, const orc_forest* f,
-                         const psx_mutation* m, uint64_t n_mut, uint32_t ploidy,
-                         const psx_cna_event* ev, uint32_t n_ev,
-                         const psx_seq_params* p, uint64_t seed, int32_t* alt, int32_t* depth,
-                         uint8_t* present, uint64_t* n_reads) {
+/* simulate_seq; with rows != NULL also every read, materialised into rows
+ * (capacity reads of psx_read_words words; no copy-number events). */
+static int seq_run(uint32_t n_chr, const uint32_t* len, const uint32_t* packed, const orc_forest* f,
+                   const psx_mutation* m, uint64_t n_mut, uint32_t ploidy, const psx_cna_event* ev, uint32_t n_ev,
+                   const psx_seq_params* p, uint64_t seed, int32_t* alt, int32_t* depth, uint8_t* present,
+                   uint32_t* rows, uint64_t capacity, uint64_t* n_reads) {
     if (!(p->coverage >= 0.0) || !isfinite(p->coverage)) return fail("invalid coverage");
     if (p->read_size < 1) return fail("invalid read size");
     if (!(p->purity >= 0.0 && p->purity <= 1.0)) return fail("The purity must be in [0,1]");
@@ -971,7 +1016,13 @@ int orc_simulate_seq_cna(uint32_t n_chr, const uint32_t* len, const orc_forest* 
     s->pe = p->insert_size_mean > 0; s->preneo_in_normal = p->preneoplastic_in_normal != 0;
     s->seed = seed; s->e = p->error_rate;
     s->ev = ev; s->n_ev = n_ev;
+    s->packed = packed; s->rows = rows; s->rw = 2u + (s->L + 15u) / 16u;
+    for (uint32_t c = 0, w = 0; c < n_chr && c < PSX_MAX_CHROMOSOMES; ++c) {
+        s->chr_woff[c] = w;
+        w += (len[c] + 15u) / 16u;
+    }
     int rc = 0;
+    if (rows && n_ev) rc = fail("read materialisation does not support copy-number events");
     if (n_ev && ploidy > PSX_MAX_CNA_PLOIDY) rc = fail("copy-number events need a small ploidy");
     if (s->pe) rc = insert_table(p->insert_size_mean, p->insert_size_stddev, &s->ins_k0, &s->ins_n, s->ins_T);
     if (!rc && s->e > 0.0) {
@@ -1040,8 +1091,10 @@ int orc_simulate_seq_cna(uint32_t n_chr, const uint32_t* len, const orc_forest* 
             return fail("too many cell copies");
         }
     }
-    /* strata */
+    /* strata; the first read of each segment, in (output sample, chromosome) order */
     uint64_t n_strata = 0, reads = 0;
+    uint64_t* seg_base = (uint64_t*)calloc((size_t)n_out * n_chr + 1, sizeof(uint64_t));
+    s->seg_base = seg_base;
     for (uint32_t so = 0; so < n_out; ++so)
         for (uint32_t c = 0; c < n_chr; ++c)
             if (sel[c] && len[c] >= s->L) n_strata += (len[c] - s->L + 1 + PSX_STRATUM - 1) / PSX_STRATUM;
@@ -1054,8 +1107,14 @@ int orc_simulate_seq_cna(uint32_t n_chr, const uint32_t* len, const orc_forest* 
             uint32_t J = (W + PSX_STRATUM - 1) / PSX_STRATUM;
             for (uint32_t j = 0; j < J; ++j) st[q++] = (stratum_ref){so, c, j};
             uint64_t T = (uint64_t)llround(p->coverage * (double)len[c] / (double)(s->pe ? 2 * s->L : s->L));
+            seg_base[(uint64_t)so * n_chr + c] = reads;
             if (!has_ev[c]) reads += T * (s->pe ? 2u : 1u);
         }
+    if (rows && reads > capacity) {
+        free(st); free(seg_base); free(req); free(s);
+        return fail("the read buffer holds %llu reads, %llu needed", (unsigned long long)capacity,
+                    (unsigned long long)reads);
+    }
     uint64_t accepted = 0;
 #ifdef _OPENMP
 #pragma omp parallel for schedule(dynamic, 8) reduction(+ : accepted)
@@ -1130,7 +1189,9 @@ int orc_simulate_seq_cna(uint32_t n_chr, const uint32_t* len, const orc_forest* 
             }
             uint32_t slot = s->e > 0.0 ? (r % s->err_E) * rpt : 0;
             uint32_t n0 = s->e > 0.0 ? first[slot + 1] - first[slot] : 0;
+            uint64_t ri = seg_base[(uint64_t)so * n_chr + c] + (T * b / W + r) * rpt;  /* read index (rows) */
             seq_read(s, c, x, leaf, allele, eoff + first[slot], esub + first[slot], n0, a_out, d_out);
+            if (rows) read_row(s, c, x, leaf, allele, 0, eoff + first[slot], esub + first[slot], n0, rows + ri * s->rw);
             if (s->pe) {
                 uint32_t I = s->ins_k0 + pick(s->ins_T, s->ins_n, w[3]);
                 if (I < s->L) I = s->L;
@@ -1139,12 +1200,37 @@ int orc_simulate_seq_cna(uint32_t n_chr, const uint32_t* len, const orc_forest* 
                 uint32_t n1 = s->e > 0.0 ? first[slot + 2] - first[slot + 1] : 0;
                 seq_read(s, c, (uint32_t)x2, leaf, allele, eoff + first[slot + 1], esub + first[slot + 1], n1, a_out,
                          d_out);
+                if (rows)
+                    read_row(s, c, (uint32_t)x2, leaf, allele, 1, eoff + first[slot + 1], esub + first[slot + 1], n1,
+                             rows + (ri + 1) * s->rw);
             }
         }
     }
-    free(st); free(req); free(s);
+    free(st); free(seg_base); free(req); free(s);
     *n_reads = reads + accepted;
     return 0;
+}
+
+int orc_simulate_seq_cna(uint32_t n_chr, const uint32_t* len, const orc_forest* f,
+                         const psx_mutation* m, uint64_t n_mut, uint32_t ploidy,
+                         const psx_cna_event* ev, uint32_t n_ev,
+                         const psx_seq_params* p, uint64_t seed, int32_t* alt, int32_t* depth,
+                         uint8_t* present, uint64_t* n_reads) {
+    return seq_run(n_chr, len, NULL, f, m, n_mut, ploidy, ev, n_ev, p, seed, alt, depth, present, NULL, 0, n_reads);
+}
+
+int orc_simulate_reads(uint32_t n_chr, const uint32_t* len, const uint32_t* packed, const orc_forest* f,
+                       const psx_mutation* m, uint64_t n_mut, uint32_t ploidy, const psx_seq_params* p,
+                       uint64_t seed, uint32_t* rows, uint64_t capacity, uint64_t* n_reads) {
+    uint32_t n_out = (p->samples ? p->n_samples : f->n_samples) + (p->with_normal_sample ? 1u : 0u);
+    size_t n = (size_t)n_out * (n_mut ? n_mut : 1);
+    int32_t* alt = (int32_t*)malloc(n * sizeof(int32_t));
+    int32_t* depth = (int32_t*)malloc(n * sizeof(int32_t));
+    uint8_t* present = (uint8_t*)malloc(n);
+    int rc = seq_run(n_chr, len, packed, f, m, n_mut, ploidy, NULL, 0, p, seed, alt, depth, present, rows, capacity,
+                     n_reads);
+    free(alt); free(depth); free(present);
+    return rc;
 }
 
 /* ------------------------------------------------------------- result table */

@@ -77,6 +77,14 @@ int orc_simulate_seq_cna(uint32_t n_chr, const uint32_t* len, const orc_forest* 
                          const psx_seq_params* p, uint64_t seed, int32_t* alt, int32_t* depth,
                          uint8_t* present, uint64_t* n_reads);
 
+/* simulate_seq in read-materialising form (psx_simulate_seq_reads): every
+ * read into rows (capacity reads of 2 + ceil(L / 16) words each, in output
+ * order: output sample, chromosome, stratum, template, mate); no copy-number
+ * events. */
+int orc_simulate_reads(uint32_t n_chr, const uint32_t* len, const uint32_t* packed, const orc_forest* f,
+                       const psx_mutation* m, uint64_t n_mut, uint32_t ploidy, const psx_seq_params* p,
+                       uint64_t seed, uint32_t* rows, uint64_t capacity, uint64_t* n_reads);
+
 int orc_seq_table(const psx_mutation* recs, uint64_t n_mutations, uint32_t n_out_samples,
                   const int32_t* alt, const int32_t* depth, const uint8_t* present,
                   int32_t include_non_sequenced, uint64_t* n_rows, uint64_t* row_rec,

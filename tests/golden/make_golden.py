@@ -147,7 +147,10 @@ def profile(events, c, leaf, x, ploidy):
     return cn
 
 
-def brute_force_seq(lens, packed, forest, recs, seed, ploidy=2, cna=None, **p):
+def brute_force_seq(lens, packed, forest, recs, seed, ploidy=2, cna=None, rows=None, **p):
+    """alt, depth, present and the read count; with a list `rows`, also every
+    read materialised (start, leaf | allele << 26 | mate << 30 | (errors > 0)
+    << 31, then 2-bit observed bases, 16 per word) in output order."""
     L = p["read_size"]
     pe = p.get("insert_size_mean", 0) > 0
     e = p.get("error_rate", 0.0)
@@ -210,7 +213,7 @@ def brute_force_seq(lens, packed, forest, recs, seed, ploidy=2, cna=None, **p):
             t += 1
         return out
 
-    def read(so, c, x, leaf, allele, errs):
+    def read(so, c, x, leaf, allele, errs, mate):
         # the source genome's bases over [x, x + L): reference, then the
         # first carried SNV at each position
         true = [base_at(packed, woff, c, x + i) for i in range(L)]
@@ -222,6 +225,11 @@ def brute_force_seq(lens, packed, forest, recs, seed, ploidy=2, cna=None, **p):
         obs = list(true)
         for o, sub in errs:
             obs[o] = sub if sub < true[o] else sub + 1
+        if rows is not None:
+            words = [0] * ((L + 15) // 16)
+            for i, v in enumerate(obs):
+                words[i // 16] |= v << (2 * (i % 16))
+            rows.append([x, leaf | (allele << 26) | (mate << 30) | ((1 if errs else 0) << 31)] + words)
         for i in range(L):
             qs = by_pos.get((c, x + i), [])
             for k, q in enumerate(qs):
@@ -288,10 +296,10 @@ def brute_force_seq(lens, packed, forest, recs, seed, ploidy=2, cna=None, **p):
                         allele = int(np.searchsorted(cum, k, side="right"))
                         n_reads += 2 if pe else 1
                     tl = r % err_E if e > 0 else 0
-                    read(so, c, x, leaf, allele, chunk.get((tl, 0), []))
+                    read(so, c, x, leaf, allele, chunk.get((tl, 0), []), 0)
                     if pe:
                         I = max(ins_k0 + pick(ins_T, w[3]), L)
-                        read(so, c, min(x + I - L, ln - L), leaf, allele, chunk.get((tl, 1), []))
+                        read(so, c, min(x + I - L, ln - L), leaf, allele, chunk.get((tl, 1), []), 1)
     return alt, dep, pres, n_reads
 
 
@@ -359,16 +367,21 @@ def make(name, c):
     eng = engine_params(packed, lens, **c["engine"])
     recs = O.place_mutations(lens, packed, of, c["place_seed"], **eng)
     ev = O.place_cna(lens, of, **eng)
-    alt, dep, pres, n = brute_force_seq(lens, packed, of, recs, c["seq_seed"], cna=ev, **c["seq"])
+    rows = [] if len(ev) == 0 else None  # reads are materialised without copy-number events
+    alt, dep, pres, n = brute_force_seq(lens, packed, of, recs, c["seq_seed"], cna=ev, rows=rows, **c["seq"])
     oa, od, op, on = O.simulate_seq(lens, of, recs, c["seq_seed"], cna=ev, **c["seq"])
     agree = (np.array_equal(alt, oa) and np.array_equal(dep, od) and np.array_equal(pres, op) and n == on)
+    reads = np.zeros((0, 2 + (c["seq"]["read_size"] + 15) // 16), np.uint32)
+    if rows is not None:
+        reads = np.asarray(rows, np.uint64).astype(np.uint32)
+        agree = agree and np.array_equal(O.simulate_reads(lens, packed, of, recs, c["seq_seed"], n, **c["seq"]), reads)
     out = HERE / f"seq_{name}.npz"
     np.savez_compressed(out, lens=lens, packed=packed, cell_id=fd["cell_id"], parent=fd["parent"],
                         species=fd["species"], sample=fd["sample"], birth=fd["birth"],
                         n_samples=np.uint32(fd["n_samples"]), recs=recs.view(np.uint8), cna=ev.view(np.uint8),
                         seq=np.array(json.dumps(c["seq"])), seed=np.uint64(c["seq_seed"]),
-                        alt=alt, depth=dep, present=pres, n_reads=np.uint64(n))
-    print(f"{out.name}: {len(recs)} records, {int(np.sum(dep))} depth counts, {n} reads, "
+                        alt=alt, depth=dep, present=pres, n_reads=np.uint64(n), reads=reads)
+    print(f"{out.name}: {len(recs)} records, {int(np.sum(dep))} depth counts, {n} reads ({len(reads)} materialised), "
           f"{int((recs['sid_head'] == 0).sum())} non-head records; oracle agrees: {agree}")
     return agree
 

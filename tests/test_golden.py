@@ -66,6 +66,40 @@ def test_gpu_matches_fixture(ctx, case):
     assert np.array_equal(a, d["alt"]) and np.array_equal(dp, d["depth"]) and np.array_equal(pr, d["present"])
 
 
+READ_CASES = [c for c in CASES if c != "cna_purity"]  # reads are materialised without copy-number events
+
+
+@pytest.mark.parametrize("case", READ_CASES)
+def test_oracle_reads_match_fixture(case):
+    """Every read, bit for bit: start, source leaf, allele, mate, error flag
+    and 2-bit bases (reference, carried SNVs, sequencing errors)."""
+    d = load(case)
+    assert len(d["reads"]) == int(d["n_reads"]) > 0
+    of = O.Forest(d["cell_id"], d["parent"], d["species"], d["sample"], d["birth"], int(d["n_samples"]))
+    rows = O.simulate_reads(d["lens"], d["packed"], of, d["recs"], int(d["seed"]), len(d["reads"]), **d["seq"])
+    assert np.array_equal(rows, d["reads"])
+    # errors flagged on some reads only, and reads of both mates when paired
+    flag = d["reads"][:, 1] >> 31
+    assert 0 < flag.sum() < len(flag)
+    if d["seq"].get("insert_size_mean", 0) > 0:
+        assert ((d["reads"][:, 1] >> 30) & 1).tolist() == [0, 1] * (len(d["reads"]) // 2)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", READ_CASES)
+def test_gpu_reads_match_fixture(ctx, case):
+    from process_amd import core
+
+    print(f"[golden] reads {case}", flush=True)
+    d = load(case)
+    g = core.Genome(ctx, d["lens"], packed=d["packed"])
+    f = core.Forest(ctx, d["cell_id"], d["parent"], d["species"], d["sample"], d["birth"], int(d["n_samples"]))
+    m = core.Mutations.upload(ctx, f, d["recs"])
+    out, n = core.simulate_seq_reads(ctx, g, f, m, int(d["seed"]), **d["seq"])
+    assert n == len(d["reads"])
+    assert np.array_equal(out.numpy()[:n], d["reads"])
+
+
 def hand_table():
     """Four records: SID A held by two records (head + member), SID B, an
     indel SID C; two samples. Expected rows follow get_active_mutations and

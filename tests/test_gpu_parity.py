@@ -108,6 +108,9 @@ def test_dense_windows_use_global_path(ctx):
     a, d, p, n = core.simulate_seq_arrays(ctx, w.g, w.pf, w.pm, 3, **kw)
     oa, od, op, on = O.simulate_seq(w.lens, w.of, w.om, 3, **kw)
     assert np.array_equal(d, od) and np.array_equal(a, oa) and np.array_equal(p, op)
+    say("[parity] dense table: materialised reads")
+    reads = _reads(ctx, w, 3, kw)
+    assert np.array_equal(reads, _oracle_reads(w, 3, kw, len(reads)))
 
 
 def test_finite_sites_collisions(ctx):
@@ -123,6 +126,8 @@ def test_finite_sites_collisions(ctx):
         a, d, p, n = core.simulate_seq_arrays(ctx, w.g, w.pf, w.pm, 8, **kw)
         oa, od, op, on = O.simulate_seq(w.lens, w.of, w.om, 8, **kw)
         assert np.array_equal(d, od) and np.array_equal(a, oa) and np.array_equal(p, op), case
+        reads = _reads(ctx, w, 8, kw)
+        assert np.array_equal(reads, _oracle_reads(w, 8, kw, len(reads))), case
 
 
 def test_batch_equals_plain(ctx, world):
@@ -231,6 +236,30 @@ def _recount(w, reads, kw, n_out):
                         alt[so, q] += car
     assert idx == len(reads)
     return alt, dep
+
+
+def _oracle_reads(w, seed, kw, n):
+    return O.simulate_reads(w.lens, w.packed, w.of, w.om, seed, n, **kw)
+
+
+READ_CASES = dict(cases.SEQ_CASES,
+                  # rows too wide for the LDS stage: each lane writes its read in place
+                  long_reads=dict(coverage=3, read_size=1000, insert_size_mean=1500, insert_size_stddev=20,
+                                  error_rate=4e-3))
+
+
+@pytest.mark.parametrize("case", list(READ_CASES))
+def test_materialised_reads_bit_exact(ctx, world, case):
+    """Every read, every bit: start, leaf, allele, mate, error flag, bases."""
+    kw = READ_CASES[case]
+    say(f"[parity] materialised reads bit-exact, case {case}")
+    reads = _reads(ctx, world, 23, kw)
+    ref = _oracle_reads(world, 23, kw, len(reads))
+    assert reads.shape == ref.shape
+    bad = np.nonzero((reads != ref).any(axis=1))[0]
+    assert len(bad) == 0, (len(bad), bad[:5], reads[bad[:2]], ref[bad[:2]])
+    if kw.get("error_rate", 0) > 0:
+        assert (reads[:, 1] >> 31).sum() > 0
 
 
 @pytest.mark.parametrize("case", ["pe_errors", "se_errorless", "purity"])
