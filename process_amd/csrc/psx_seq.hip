@@ -427,6 +427,18 @@ extern "C" psx_status psx_simulate_seq(psx_ctx* ctx, const psx_genome* g, const 
     });
 }
 
+// One persistent read-materialising launch: as many blocks as fit on every CU
+// at once (capped by the work items).
+template <int BLOCK, int CAP, bool STAGE, int NW>
+void launch_reads(psx_ctx* ctx, const seq_launch& L, size_t items, size_t stage) {
+    int per_cu = 0;
+    PSX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_seq_reads<BLOCK, CAP, STAGE, NW>, BLOCK, stage));
+    const uint64_t want = (uint64_t)ctx->n_cu * (uint64_t)std::max(per_cu, 1);
+    const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, items));
+    PSX_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    hipLaunchKernelGGL((k_seq_reads<BLOCK, CAP, STAGE, NW>), dim3(grid), dim3(BLOCK), stage, ctx->stream, L);
+}
+
 extern "C" uint32_t psx_read_words(const psx_seq_params* p) {
     return p ? 2u + (p->read_size + 15u) / 16u : 0u;
 }
@@ -437,7 +449,7 @@ extern "C" psx_status psx_simulate_seq_reads(psx_ctx* ctx, const psx_genome* g, 
     if (!ctx || !g || !f || !m || !p || !d_reads) return PSX_EINVAL;
     return psx_guard(ctx, [&] {
         PSX_HIP(hipSetDevice(ctx->device));
-        constexpr int BLOCK = 256, CAP = 512;
+        constexpr int CAP = 512;
         constexpr size_t STAGE_MAX = 48u << 10;  // LDS read stage per block
         const job_in in{g, f, m, p, seed, nullptr, nullptr, nullptr};
         if (m->ploidy > 16) psx_throw(PSX_EINVAL, "read records hold copy indices below 16 (ploidy %u)", m->ploidy);
@@ -457,19 +469,22 @@ extern "C" psx_status psx_simulate_seq_reads(psx_ctx* ctx, const psx_genome* g, 
         L.reads_out = d_reads;
         L.read_words = psx_read_words(p);
         L.out_vec4 = (L.read_words % 4 == 0 && ((uintptr_t)d_reads & 15) == 0) ? 1u : 0u;
-        const size_t stage = (size_t)(BLOCK / 64) * 64 * L.read_words * 4u;
-        const bool staged = stage <= STAGE_MAX;
-        const uint64_t want = (uint64_t)ctx->n_cu * 4u;
-        const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, P.items.size()));
-        PSX_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+        // 64 rows of read_words words per wave staged in LDS; the block is the
+        // largest of 512 and 256 threads whose stage fits STAGE_MAX (more waves
+        // per CU share one copy of the block's window, reference and tables:
+        // 512 threads run 6 waves per SIMD at 150 bp, 256 threads 4 or 5)
+        const size_t wave_stage = (size_t)64 * L.read_words * 4u;
+        if (P.items.empty()) PSX_HIP(hipEventRecord(ctx->ev0, ctx->stream));
         if (!P.items.empty()) {
-            if (staged && L.out_vec4 && L.read_words == 12)  // 145-160 bp reads
-                hipLaunchKernelGGL((k_seq_reads<BLOCK, CAP, true, 10>), dim3(grid), dim3(BLOCK), stage, ctx->stream,
-                                   L);
-            else if (staged)
-                hipLaunchKernelGGL((k_seq_reads<BLOCK, CAP, true, 0>), dim3(grid), dim3(BLOCK), stage, ctx->stream, L);
+            const size_t items = P.items.size();
+            if (L.out_vec4 && L.read_words == 12)  // 145-160 bp reads
+                launch_reads<512, CAP, true, 10>(ctx, L, items, 8 * wave_stage);
+            else if (8 * wave_stage <= STAGE_MAX)
+                launch_reads<512, CAP, true, 0>(ctx, L, items, 8 * wave_stage);
+            else if (4 * wave_stage <= STAGE_MAX)
+                launch_reads<256, CAP, true, 0>(ctx, L, items, 4 * wave_stage);
             else
-                hipLaunchKernelGGL((k_seq_reads<BLOCK, CAP, false, 0>), dim3(grid), dim3(BLOCK), 0, ctx->stream, L);
+                launch_reads<256, CAP, false, 0>(ctx, L, items, 0);
         }
         PSX_HIP(hipGetLastError());
         PSX_HIP(hipEventRecord(ctx->ev1, ctx->stream));
