@@ -120,6 +120,7 @@ void prepare_job(psx_ctx* ctx, seq_batch& P, const job_in& in, uint32_t spi) {
     J.err_k0 = err.k0;
     J.err_E = err_E;
     J.err_N = err_E * (pe ? 2u : 1u) * L;
+    J.inv_L = (uint32_t)(((uint64_t(1) << 32) + L - 1) / L);
     J.L = L; J.pe = pe ? 1u : 0u; J.ploidy = ploidy; J.n_leaves = f->n_leaves;
     J.preneo = p->preneoplastic_in_normal ? 1u : 0u; J.reach = reach; J.n_out = n_out;
     J.purity_lt1 = p->purity < 1.0 ? 1u : 0u;

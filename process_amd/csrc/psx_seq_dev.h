@@ -58,6 +58,7 @@ struct job_dev {
     uint32_t ins_k0, ins_n;
     uint32_t err_n, err_k0;      // error table (err_n == 0: no sequencing errors)
     uint32_t err_E, err_N;       // templates and bases per error chunk
+    uint32_t inv_L;              // ceil(2^32 / L): b / L = mulhi(b, inv_L) for b < 2^20
     uint32_t L, pe, ploidy, n_leaves, preneo, reach;
     uint32_t n_out;
     uint32_t purity_lt1;
@@ -415,7 +416,8 @@ __device__ __forceinline__ void count_errors(const job_dev& J, const V& v, const
                                              uint32_t* list, uint32_t lane) {
     const uint32_t E = J.err_E;
     const uint32_t top = min(nj, base + 64u);
-    const uint32_t al = t.allele | (valid ? 0x80000000u : 0u);
+    // allele (8 bits), mate 2 start - mate 1 start (< 2^13), counted
+    const uint32_t al = t.allele | ((t.x2 - t.x) << 8) | (valid ? 0x80000000u : 0u);
     for (uint32_t c = base / E; c * E < top; ++c) {  // E divides 64: the chunks of the wave's templates
         wave_sync();
         const uint32_t n = chunk_errors(J, tab, c, j, c2, list, lane);
@@ -424,18 +426,17 @@ __device__ __forceinline__ void count_errors(const job_dev& J, const V& v, const
             const uint32_t k = k0 + lane;
             const uint32_t e = k < n ? list[k] : 0u;
             const uint32_t b = e & 0xFFFFFu, sub = e >> 20;
-            const uint32_t slot = b / J.L;  // read of the chunk: template * (pe + 1) + mate
+            const uint32_t slot = __umulhi(b, J.inv_L);  // read of the chunk: template * (pe + 1) + mate
             const uint32_t o = b - slot * J.L;
             const uint32_t r = c * E + (slot >> J.pe);
             const uint32_t src = (r - base) & 63u;
-            const uint32_t x1 = (uint32_t)__shfl((int)t.x, (int)src);
-            const uint32_t x2 = (uint32_t)__shfl((int)t.x2, (int)src);
+            const uint32_t xs = (uint32_t)__shfl((int)t.x, (int)src);
             const uint32_t ls = (uint32_t)__shfl((int)t.leaf, (int)src);
             const uint32_t as = (uint32_t)__shfl((int)al, (int)src);
             if (k < n && r < top && (as >> 31)) {  // rare: an error of a counted read
-                const uint32_t P = ((slot & J.pe) ? x2 : x1) + o;
+                const uint32_t P = xs + ((slot & J.pe) ? (as >> 8) & 0x1FFFu : 0u) + o;
                 const uint32_t i = v.first(P);
-                if (i < v.n && v.pos[i] == P) error_fix(J, v, i, as & 0x7FFFFFFFu, ls, sub);
+                if (i < v.n && v.pos[i] == P) error_fix(J, v, i, as & 0xFFu, ls, sub);
             }
         }
     }
@@ -909,7 +910,7 @@ __global__ void __launch_bounds__(BLOCK) k_seq_reads(const seq_launch L) {
                                 const uint32_t k = k0 + lane;
                                 const uint32_t e = k < n_err ? list[k] : 0u;
                                 const uint32_t b = e & 0xFFFFFu, sub = e >> 20;
-                                const uint32_t slot = b / J.L;  // read of the chunk
+                                const uint32_t slot = __umulhi(b, J.inv_L);  // read of the chunk
                                 const uint32_t o = b - slot * J.L;
                                 const uint32_t tr = c * E + (slot >> J.pe);
                                 const uint32_t rl = (tr - t0) * rpt + (slot & J.pe);  // its lane in the pass
