@@ -39,11 +39,10 @@ extern "C" {
 /* Reads whose template starts fall in one stratum of PSX_STRATUM reference
  * positions are drawn from one counter range; part of the sampling model. */
 #define PSX_STRATUM 8192u
-/* Sequencing errors are independent per base and drawn per error chunk of
- * a stratum's templates: PSX_ERR_CHUNK templates, or a power-of-two fraction
- * of that at high error rates, so that a chunk holds more than PSX_ERR_KCAP
- * errors with probability below 2^-40. Part of the sampling model. */
-#define PSX_ERR_CHUNK 64u
+/* Sequencing errors are independent per base and drawn per template (both
+ * mates); an error rate is accepted when a template holds more than
+ * PSX_ERR_KCAP errors with probability below 2^-40. Part of the sampling
+ * model. */
 #define PSX_ERR_KCAP 128u
 #define PSX_MAX_CHROMOSOMES 64u
 #define PSX_NO_NODE 0xFFFFFFFFu

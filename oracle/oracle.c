@@ -165,26 +165,22 @@ int orc_binomial_table(uint32_t t, double p, uint32_t* k0, uint32_t* n, uint64_t
     return binomial_range(t, p, lo, hi, cap, k0, n, T);
 }
 
-/* Templates per error chunk: the largest E of 64, 32, ..., 1 for which
- * Binomial(E rpt L, e) exceeds PSX_ERR_KCAP with probability below 2^-40
- * (the tail summed term by term from PSX_ERR_KCAP + 1); 0 if none. */
-static uint32_t error_chunk(uint32_t L, uint32_t rpt, double e) {
-    for (uint32_t E = PSX_ERR_CHUNK; E >= 1; E >>= 1) {
-        uint32_t N = E * rpt * L;
-        if (N <= PSX_ERR_KCAP) return E;
-        double lp = log(e), lq = log1p(-e), lgn = lgamma((double)N + 1.0), tail = 0.0;
-        for (uint32_t k = PSX_ERR_KCAP + 1; k <= N; ++k) {
-            double term = exp(lgn - lgamma((double)k + 1.0) - lgamma((double)(N - k) + 1.0) + k * lp +
-                              (double)(N - k) * lq);
-            tail += term;
-            if (tail >= 0x1p-40 || (term < 1e-300 && (double)k > (double)N * e)) break;
-        }
-        if (tail < 0x1p-40) return E;
+/* Whether a template's N = rpt L bases hold more than PSX_ERR_KCAP errors
+ * with probability below 2^-40 (Binomial(N, e) tail summed term by term from
+ * PSX_ERR_KCAP + 1). */
+static int error_rate_ok(uint32_t N, double e) {
+    if (N <= PSX_ERR_KCAP) return 1;
+    double lp = log(e), lq = log1p(-e), lgn = lgamma((double)N + 1.0), tail = 0.0;
+    for (uint32_t k = PSX_ERR_KCAP + 1; k <= N; ++k) {
+        double term = exp(lgn - lgamma((double)k + 1.0) - lgamma((double)(N - k) + 1.0) + k * lp +
+                          (double)(N - k) * lq);
+        tail += term;
+        if (tail >= 0x1p-40 || (term < 1e-300 && (double)k > (double)N * e)) break;
     }
-    return 0;
+    return tail < 0x1p-40;
 }
 
-/* Errors in an error chunk of N bases: Binomial(N, e) on [0, min(N, PSX_ERR_KCAP)]. */
+/* Errors of a template's N bases: Binomial(N, e) on [0, min(N, PSX_ERR_KCAP)]. */
 static int error_table(uint32_t N, double e, uint32_t* k0, uint32_t* n, uint64_t* T) {
     double hi = N < PSX_ERR_KCAP ? (double)N : (double)PSX_ERR_KCAP;
     return binomial_range(N, e, 0.0, hi, PSX_ERR_KCAP + 1, k0, n, T);
@@ -802,7 +798,7 @@ typedef struct seq_ctx {
     double e;
     uint32_t ins_k0, ins_n;
     uint64_t ins_T[4096];
-    uint32_t err_E, err_N, err_k0, err_n;  /* templates and bases per error chunk, K table */
+    uint32_t err_N, err_k0, err_n;  /* bases per template, error-count table */
     uint64_t err_T[PSX_ERR_KCAP + 1];
     const psx_cna_event* ev;  /* copy-number events, DFS order */
     uint32_t n_ev;
@@ -874,28 +870,33 @@ static int carried(const seq_ctx* s, const psx_mutation* r, uint32_t leaf, uint3
 }
 
 /* Sequencing errors (independent per base: the base is replaced by one of
- * the three others, uniformly, with probability e), drawn per error chunk:
- * templates [c E, c E + E) of stratum j, N = E rpt L bases, base index
- * b = (t rpt + mate) L + offset for template c E + t. The chunk draws
- * K ~ Binomial(N, e) by inversion from word y of Philox(counter = (c, j, c2,
- * ERROR << 24)); stream word t = 0, 1, ... is word x of Philox(c, j, c2,
- * ERROR << 24 | t) and gives a base index floor(w N / 2^32) and a
- * substitution index floor((w & 0xFFFF) 3 / 2^16); an index already taken is
- * skipped; at most 4 PSX_ERR_KCAP stream words are read. Returns the number
- * of errors; b and sub receive them in draw order. */
-static uint32_t chunk_errors(const seq_ctx* s, uint32_t c, uint32_t j, uint32_t c2, uint32_t* b, uint32_t* sub) {
-    uint32_t w[4];
-    philox4(s->seed, c, j, c2, (uint32_t)P_ERROR << 24, w);
-    uint32_t K = s->err_k0 + pick(s->err_T, s->err_n, w[1]);
+ * the three others, uniformly, with probability e), drawn per template r of
+ * stratum j over its N = rpt L bases, base index b = mate L + offset. With
+ * w_k = Philox(counter = (r, j, c2, ERROR << 24 | k)): the count K ~
+ * Binomial(N, e) by inversion from word y of w_0; stream words are x, z, w of
+ * w_0, then the four words of w_1, w_2, ...; each gives a base index
+ * floor(w N / 2^32) and a substitution index floor((w & 0xFFFF) 3 / 2^16);
+ * an index already taken is skipped; at most 4 PSX_ERR_KCAP stream words are
+ * read. Returns the number of errors; b and sub receive them in draw order. */
+static uint32_t template_errors(const seq_ctx* s, uint32_t r, uint32_t j, uint32_t c2, uint32_t* b, uint32_t* sub) {
+    uint32_t w0[4], w[4];
+    philox4(s->seed, r, j, c2, (uint32_t)P_ERROR << 24, w0);
+    uint32_t K = s->err_k0 + pick(s->err_T, s->err_n, w0[1]);
     uint32_t cnt = 0;
     for (uint32_t t = 0; cnt < K && t < 4 * PSX_ERR_KCAP; ++t) {
-        philox4(s->seed, c, j, c2, ((uint32_t)P_ERROR << 24) | t, w);
-        uint32_t x = mulhi32(w[0], s->err_N);
+        uint32_t word;
+        if (t < 3) {
+            word = w0[t == 0 ? 0 : t + 1];
+        } else {
+            if ((t - 3) % 4 == 0) philox4(s->seed, r, j, c2, ((uint32_t)P_ERROR << 24) | (1 + (t - 3) / 4), w);
+            word = w[(t - 3) % 4];
+        }
+        uint32_t x = mulhi32(word, s->err_N);
         int dup = 0;
         for (uint32_t q = 0; q < cnt; ++q) dup |= b[q] == x;
         if (dup) continue;
         b[cnt] = x;
-        sub[cnt] = ((w[0] & 0xFFFFu) * 3u) >> 16;
+        sub[cnt] = ((word & 0xFFFFu) * 3u) >> 16;
         ++cnt;
     }
     return cnt;
@@ -1026,9 +1027,8 @@ static int seq_run(uint32_t n_chr, const uint32_t* len, const uint32_t* packed, 
     if (n_ev && ploidy > PSX_MAX_CNA_PLOIDY) rc = fail("copy-number events need a small ploidy");
     if (s->pe) rc = insert_table(p->insert_size_mean, p->insert_size_stddev, &s->ins_k0, &s->ins_n, s->ins_T);
     if (!rc && s->e > 0.0) {
-        s->err_E = error_chunk(s->L, s->pe ? 2u : 1u, s->e);
-        if (s->err_E == 0) rc = fail("error rate too high for %u-base reads", s->L);
-        s->err_N = s->err_E * (s->pe ? 2u : 1u) * s->L;
+        s->err_N = (s->pe ? 2u : 1u) * s->L;
+        if (!error_rate_ok(s->err_N, s->e)) rc = fail("error rate too high for %u-base reads", s->L);
         if (!rc) rc = error_table(s->err_N, s->e, &s->err_k0, &s->err_n, s->err_T);
     }
     /* chromosome set: sorted, unique (std::set semantics of get_relevant_chr_set) */
@@ -1135,27 +1135,9 @@ static int seq_run(uint32_t n_chr, const uint32_t* len, const uint32_t* packed, 
         uint32_t c2 = (c << 16) | so;
         int32_t* a_out = alt + (uint64_t)so * n_mut;
         int32_t* d_out = depth + (uint64_t)so * n_mut;
-        /* errors of the current chunk, grouped by read slot t rpt + mate */
         uint32_t rpt = s->pe ? 2u : 1u;
-        uint32_t eb[PSX_ERR_KCAP], es[PSX_ERR_KCAP], eoff[PSX_ERR_KCAP], esub[PSX_ERR_KCAP];
-        uint32_t first[2 * PSX_ERR_CHUNK + 1];
-        memset(first, 0, sizeof first);
+        uint32_t eb[PSX_ERR_KCAP], es[PSX_ERR_KCAP], eoff[2][PSX_ERR_KCAP], esub[2][PSX_ERR_KCAP];
         for (uint32_t r = 0; r < (uint32_t)nj; ++r) {
-            if (s->e > 0.0 && r % s->err_E == 0) {
-                uint32_t ne = chunk_errors(s, r / s->err_E, j, c2, eb, es);
-                uint32_t slots = s->err_E * rpt;
-                memset(first, 0, sizeof(uint32_t) * (slots + 1));
-                for (uint32_t q = 0; q < ne; ++q) ++first[eb[q] / s->L + 1];
-                for (uint32_t k = 0; k < slots; ++k) first[k + 1] += first[k];
-                uint32_t fill[2 * PSX_ERR_CHUNK];
-                memcpy(fill, first, sizeof(uint32_t) * slots);
-                for (uint32_t q = 0; q < ne; ++q) {  /* draw order within a read */
-                    uint32_t slot = eb[q] / s->L;
-                    eoff[fill[slot]] = eb[q] % s->L;
-                    esub[fill[slot]] = es[q];
-                    ++fill[slot];
-                }
-            }
             uint32_t w[4];
             philox4(seed, r, j, c2, (uint32_t)P_TEMPLATE << 24, w);
             uint32_t x = b + mulhi32(w[0], e - b);
@@ -1187,22 +1169,27 @@ static int seq_run(uint32_t n_chr, const uint32_t* len, const uint32_t* packed, 
                 while (allele + 1 < ploidy && k >= cum) cum += cn[++allele];
                 accepted += s->pe ? 2u : 1u;
             }
-            uint32_t slot = s->e > 0.0 ? (r % s->err_E) * rpt : 0;
-            uint32_t n0 = s->e > 0.0 ? first[slot + 1] - first[slot] : 0;
+            /* the template's errors per mate, in draw order */
+            uint32_t ne[2] = {0, 0};
+            if (s->e > 0.0) {
+                uint32_t n = template_errors(s, r, j, c2, eb, es);
+                for (uint32_t q = 0; q < n; ++q) {
+                    uint32_t m = eb[q] / s->L;
+                    eoff[m][ne[m]] = eb[q] % s->L;
+                    esub[m][ne[m]] = es[q];
+                    ++ne[m];
+                }
+            }
             uint64_t ri = seg_base[(uint64_t)so * n_chr + c] + (T * b / W + r) * rpt;  /* read index (rows) */
-            seq_read(s, c, x, leaf, allele, eoff + first[slot], esub + first[slot], n0, a_out, d_out);
-            if (rows) read_row(s, c, x, leaf, allele, 0, eoff + first[slot], esub + first[slot], n0, rows + ri * s->rw);
+            seq_read(s, c, x, leaf, allele, eoff[0], esub[0], ne[0], a_out, d_out);
+            if (rows) read_row(s, c, x, leaf, allele, 0, eoff[0], esub[0], ne[0], rows + ri * s->rw);
             if (s->pe) {
                 uint32_t I = s->ins_k0 + pick(s->ins_T, s->ins_n, w[3]);
                 if (I < s->L) I = s->L;
                 uint64_t x2 = (uint64_t)x + I - s->L;
                 if (x2 > len[c] - s->L) x2 = len[c] - s->L;
-                uint32_t n1 = s->e > 0.0 ? first[slot + 2] - first[slot + 1] : 0;
-                seq_read(s, c, (uint32_t)x2, leaf, allele, eoff + first[slot + 1], esub + first[slot + 1], n1, a_out,
-                         d_out);
-                if (rows)
-                    read_row(s, c, (uint32_t)x2, leaf, allele, 1, eoff + first[slot + 1], esub + first[slot + 1], n1,
-                             rows + (ri + 1) * s->rw);
+                seq_read(s, c, (uint32_t)x2, leaf, allele, eoff[1], esub[1], ne[1], a_out, d_out);
+                if (rows) read_row(s, c, (uint32_t)x2, leaf, allele, 1, eoff[1], esub[1], ne[1], rows + (ri + 1) * s->rw);
             }
         }
     }

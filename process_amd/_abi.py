@@ -6,7 +6,6 @@ import ctypes as C
 import numpy as np
 
 PSX_STRATUM = 8192
-PSX_ERR_CHUNK = 64
 PSX_ERR_KCAP = 128
 PSX_MAX_CHROMOSOMES = 64
 PSX_NO_NODE = 0xFFFFFFFF

@@ -246,21 +246,17 @@ psx_itable psx_binomial_table(uint32_t n, double p, uint32_t cap) {
     return binomial_range(n, p, lo, hi, cap);
 }
 
-uint32_t psx_error_chunk(uint32_t L, uint32_t rpt, double e) {
-    for (uint32_t E = PSX_ERR_CHUNK; E >= 1; E >>= 1) {
-        const uint32_t N = E * rpt * L;
-        if (N <= PSX_ERR_KCAP) return E;
-        const double lp = std::log(e), lq = std::log1p(-e), lgn = std::lgamma((double)N + 1.0);
-        double tail = 0.0;
-        for (uint32_t k = PSX_ERR_KCAP + 1; k <= N; ++k) {
-            const double term = std::exp(lgn - std::lgamma((double)k + 1.0) - std::lgamma((double)(N - k) + 1.0) +
-                                         k * lp + (double)(N - k) * lq);
-            tail += term;
-            if (tail >= 0x1p-40 || (term < 1e-300 && (double)k > (double)N * e)) break;
-        }
-        if (tail < 0x1p-40) return E;
+bool psx_error_rate_ok(uint32_t N, double e) {
+    if (N <= PSX_ERR_KCAP) return true;
+    const double lp = std::log(e), lq = std::log1p(-e), lgn = std::lgamma((double)N + 1.0);
+    double tail = 0.0;
+    for (uint32_t k = PSX_ERR_KCAP + 1; k <= N; ++k) {
+        const double term = std::exp(lgn - std::lgamma((double)k + 1.0) - std::lgamma((double)(N - k) + 1.0) +
+                                     k * lp + (double)(N - k) * lq);
+        tail += term;
+        if (tail >= 0x1p-40 || (term < 1e-300 && (double)k > (double)N * e)) break;
     }
-    return 0;
+    return tail < 0x1p-40;
 }
 
 psx_itable psx_error_table(uint32_t N, double e) {

@@ -142,11 +142,10 @@ struct psx_itable {
 };
 psx_itable psx_poisson_table(double lambda);
 psx_itable psx_binomial_table(uint32_t t, double p, uint32_t cap);
-// Sequencing errors are drawn per error chunk of E templates (N = E rpt L
-// bases): E = the largest power of two <= PSX_ERR_CHUNK for which a chunk
-// holds more than PSX_ERR_KCAP errors with probability below 2^-40 (0 if
-// none); the chunk's count is Binomial(N, e) on [0, min(N, PSX_ERR_KCAP)].
-uint32_t psx_error_chunk(uint32_t L, uint32_t rpt, double e);
+// Sequencing errors are drawn per template over its N = rpt L bases: the
+// rate is accepted when P(Binomial(N, e) > PSX_ERR_KCAP) < 2^-40; the count
+// is Binomial(N, e) on [0, min(N, PSX_ERR_KCAP)].
+bool psx_error_rate_ok(uint32_t N, double e);
 psx_itable psx_error_table(uint32_t N, double e);
 psx_itable psx_insert_table(uint32_t mean, uint32_t sd);
 // Kernel form of an inversion table (psx_guided_pick in psx_rng.h): draws

@@ -76,7 +76,6 @@ void prepare_job(psx_ctx* ctx, seq_batch& P, const job_in& in, uint32_t spi) {
     const uint32_t L = p->read_size;
     const bool pe = p->insert_size_mean > 0;
     psx_ctable ins, err;
-    uint32_t err_E = 0;
     uint32_t reach = L;
     if (pe) {
         ins = psx_compact_table(psx_insert_table(p->insert_size_mean, p->insert_size_stddev));
@@ -86,9 +85,9 @@ void prepare_job(psx_ctx* ctx, seq_batch& P, const job_in& in, uint32_t spi) {
         if (reach > MAX_READ) psx_throw(PSX_EINVAL, "insert sizes above %u are not supported", MAX_READ);
     }
     if (p->error_rate > 0.0) {
-        err_E = psx_error_chunk(L, pe ? 2u : 1u, p->error_rate);
-        if (err_E == 0) psx_throw(PSX_EINVAL, "error rate too high for %u-base reads", L);
-        err = psx_compact_table(psx_error_table(err_E * (pe ? 2u : 1u) * L, p->error_rate));
+        if (!psx_error_rate_ok((pe ? 2u : 1u) * L, p->error_rate))
+            psx_throw(PSX_EINVAL, "error rate too high for %u-base reads", L);
+        err = psx_compact_table(psx_error_table((pe ? 2u : 1u) * L, p->error_rate));
     }
     std::vector<uint8_t> sel(g->n_chr, p->n_chromosomes == 0 ? 1 : 0);
     for (uint32_t k = 0; k < p->n_chromosomes; ++k) {
@@ -118,8 +117,7 @@ void prepare_job(psx_ctx* ctx, seq_batch& P, const job_in& in, uint32_t spi) {
     J.ins_k0 = ins.k0; J.ins_n = (uint32_t)ins.T.size();
     J.err_n = (uint32_t)err.T.size();
     J.err_k0 = err.k0;
-    J.err_E = err_E;
-    J.err_N = err_E * (pe ? 2u : 1u) * L;
+    J.err_N = (pe ? 2u : 1u) * L;
     J.inv_L = (uint32_t)(((uint64_t(1) << 32) + L - 1) / L);
     J.L = L; J.pe = pe ? 1u : 0u; J.ploidy = ploidy; J.n_leaves = f->n_leaves;
     J.preneo = p->preneoplastic_in_normal ? 1u : 0u; J.reach = reach; J.n_out = n_out;

@@ -57,7 +57,7 @@ struct job_dev {
     uint64_t thr_normal;         // floor((1 - purity) 2^32)
     uint32_t ins_k0, ins_n;
     uint32_t err_n, err_k0;      // error table (err_n == 0: no sequencing errors)
-    uint32_t err_E, err_N;       // templates and bases per error chunk
+    uint32_t err_N;              // bases per template: (pe + 1) L
     uint32_t inv_L;              // ceil(2^32 / L): b / L = mulhi(b, inv_L) for b < 2^20
     uint32_t L, pe, ploidy, n_leaves, preneo, reach;
     uint32_t n_out;
@@ -276,43 +276,100 @@ __device__ __forceinline__ tmpl make_template(const job_dev& J, const seg_dev& S
     return t;
 }
 
-// Per-wave LDS of the error lists: the errors of one chunk, then a 2048-bit
-// hashed bitmap of the base indices drawn so far.
+// Sequencing errors of a template (oracle: template_errors). Most templates
+// are settled by their first draw w_0: K <= 3 and the stream words x, z, w of
+// w_0 give distinct base indices; their errors travel in a two-word summary,
+// each error as e = b | sub << 13 (b < 2 L <= 8192 the template's base
+// index, sub the substitution index). The others (slow) are generated by the
+// whole wave (wave_template_errors).
+struct terr {
+    uint32_t w01;  // e_0 | e_1 << 16
+    uint32_t w2n;  // e_2 | n << 16 | slow << 18
+    __device__ __forceinline__ uint32_t n() const { return (w2n >> 16) & 3u; }
+    __device__ __forceinline__ bool slow() const { return (w2n >> 18) & 1u; }
+};
+
+// The summary of template r of stratum j; lanes with want == false get none.
+// Every lane steps through the same count loop.
+__device__ __forceinline__ terr template_errors(const job_dev& J, const lds_tables& tab, uint32_t r, uint32_t j,
+                                                uint32_t c2, bool want) {
+    const psx_u4 w = psx_draw_u(J.seed, r, j, c2, PSX_P_ERROR << 24);
+    // K - err_k0 = the first k with u < T[k] (capped at err_n - 1); K is small
+    uint32_t k = 0;
+    bool more = want && 1u < J.err_n && w.y >= tab.err[0];
+    while (__ballot(more)) {
+        k += more ? 1u : 0u;
+        more = more && k + 1u < J.err_n && w.y >= tab.err[min(k, J.err_n - 1u)];
+    }
+    const uint32_t K = want ? J.err_k0 + k : 0u;
+    const uint32_t b0 = psx_mulhi32(w.x, J.err_N), b1 = psx_mulhi32(w.z, J.err_N), b2 = psx_mulhi32(w.w, J.err_N);
+    const bool slow = K > 3u || (K >= 2u && b1 == b0) || (K >= 3u && (b2 == b0 || b2 == b1));
+    const uint32_t n = slow ? 0u : K;
+    const uint32_t e0 = b0 | ((((w.x & 0xFFFFu) * 3u) >> 16) << 13);
+    const uint32_t e1 = b1 | ((((w.z & 0xFFFFu) * 3u) >> 16) << 13);
+    const uint32_t e2 = b2 | ((((w.w & 0xFFFFu) * 3u) >> 16) << 13);
+    terr e;
+    e.w01 = (n > 0u ? e0 : 0u) | ((n > 1u ? e1 : 0u) << 16);
+    e.w2n = (n > 2u ? e2 : 0u) | (n << 16) | ((slow ? 1u : 0u) << 18);
+    return e;
+}
+
+__device__ __forceinline__ terr shfl_terr(const terr& e, uint32_t src) {
+    return terr{(uint32_t)__shfl((int)e.w01, (int)src), (uint32_t)__shfl((int)e.w2n, (int)src)};
+}
+
+// The observed base at template base index b whose true base is tb, with the
+// summary's errors (none for a slow template).
+__device__ __forceinline__ uint32_t observe(const terr& e, uint32_t b, uint32_t tb) {
+    const uint32_t n = e.n();
+    const uint32_t e0 = e.w01 & 0xFFFFu, e1 = e.w01 >> 16, e2 = e.w2n & 0xFFFFu;
+    uint32_t sub = 4u;
+    sub = n > 0u && (e0 & 0x1FFFu) == b ? e0 >> 13 : sub;
+    sub = n > 1u && (e1 & 0x1FFFu) == b ? e1 >> 13 : sub;
+    sub = n > 2u && (e2 & 0x1FFFu) == b ? e2 >> 13 : sub;
+    return sub == 4u ? tb : (sub < tb ? sub : sub + 1u);
+}
+
+// Per-wave LDS of a slow template's errors: the list, then a 2048-bit hashed
+// bitmap of the base indices drawn so far.
 constexpr uint32_t ELIST = PSX_ERR_KCAP + 64u;
 
-// Sequencing errors of error chunk c of stratum j, generated by one wave
-// (oracle: chunk_errors): the count K by inversion from word y of the
-// chunk's first draw (a parallel compare against the thresholds), then the
-// chunk's stream words in batches, lane l drawing stream word t0 + l. A word
-// is new when no earlier stream word gives the same base index (an earlier
-// equal index was taken or was itself a repeat of a taken one), and the first
-// K new words are the errors. Repeats are rare: the bitmap flags a batch that
-// may hold one, and only such a batch runs the exact check. Writes the errors
-// to list[0 .. K) as b | sub << 20 and returns K (wave-uniform); every lane
+// All errors of template r (oracle: template_errors), generated by one wave:
+// the count K from word y of w_0 (a parallel compare against the
+// thresholds), then the stream words in batches, lane l drawing stream word
+// t0 + l (x, z, w of w_0, then the four words of w_1, w_2, ...). A word is new
+// when no earlier stream word gives the same base index (an earlier equal
+// index was taken or was itself a repeat of a taken one), and the first K new
+// words are the errors. Repeats are rare: the bitmap flags a batch that may
+// hold one, and only such a batch runs the exact check. Writes the errors to
+// list[0 .. K) as b | sub << 20 and returns K (wave-uniform); every lane
 // steps through the same loops.
-__device__ __forceinline__ uint32_t chunk_errors(const job_dev& J, const lds_tables& tab, uint32_t c, uint32_t j,
-                                                 uint32_t c2, uint32_t* list, uint32_t lane) {
+__device__ __forceinline__ uint32_t wave_template_errors(uint64_t seed, uint32_t err_n, uint32_t err_k0, uint32_t err_N,
+                                                      const uint32_t* err_T, uint32_t r, uint32_t j, uint32_t c2,
+                                                      uint32_t* list, uint32_t lane) {
     constexpr uint32_t TMAX = 4u * PSX_ERR_KCAP;  // stream words available
     uint32_t* bits = list + PSX_ERR_KCAP;
     uint32_t K = 0, cnt = 0;
     for (uint32_t t0 = 0; t0 < TMAX;) {
         const uint32_t t = t0 + lane;
-        const psx_u4 w = psx_draw_u(J.seed, c, j, c2, (PSX_P_ERROR << 24) | t);
+        const uint32_t call = t < 3u ? 0u : 1u + ((t - 3u) >> 2);
+        const psx_u4 w = psx_draw_u(seed, r, j, c2, (PSX_P_ERROR << 24) | call);
+        const uint32_t word = psx_word(w, t < 3u ? (t == 0u ? 0 : (int)t + 1) : (int)((t - 3u) & 3u));
         if (t0 == 0) {
             // K - err_k0 = the number of thresholds k < err_n - 1 with u >= T[k]
-            const uint32_t u = (uint32_t)__builtin_amdgcn_readfirstlane((int)w.y);
+            const uint32_t u = (uint32_t)__builtin_amdgcn_readfirstlane((int)w.y);  // lane 0 drew w_0
             uint32_t k = 0;
-            for (uint32_t i0 = 0; i0 + 1u < J.err_n; i0 += 64u) {
+            for (uint32_t i0 = 0; i0 + 1u < err_n; i0 += 64u) {
                 const uint32_t i = i0 + lane;
-                k += (uint32_t)__popcll(__ballot(i + 1u < J.err_n && u >= tab.err[min(i, J.err_n - 1u)]));
+                k += (uint32_t)__popcll(__ballot(i + 1u < err_n && u >= err_T[min(i, err_n - 1u)]));
             }
-            K = J.err_k0 + k;
+            K = err_k0 + k;
             if (K == 0) return 0;
             bits[lane] = 0u;
             wave_sync();
         }
         const uint32_t B = min(64u, K - cnt);  // lanes [0, B) may be needed
-        const uint32_t x = psx_mulhi32(w.x, J.err_N);
+        const uint32_t x = psx_mulhi32(word, err_N);
         const bool live = lane < B && t < TMAX;
         const uint32_t h = (x * 0x9E3779B1u) >> 21;
         const uint32_t seen = live ? atomicOr(&bits[h >> 5], 1u << (h & 31u)) & (1u << (h & 31u)) : 0u;
@@ -330,7 +387,7 @@ __device__ __forceinline__ uint32_t chunk_errors(const job_dev& J, const lds_tab
         const unsigned long long m = __ballot(fresh);
         const uint32_t rank =
             cnt + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-        if (fresh && rank < K) list[rank] = x | ((((w.x & 0xFFFFu) * 3u) >> 16) << 20);
+        if (fresh && rank < K) list[rank] = x | ((((word & 0xFFFFu) * 3u) >> 16) << 20);
         cnt = min(K, cnt + (uint32_t)__popcll(m));
         if (cnt == K) break;
         t0 += B;
@@ -340,13 +397,14 @@ __device__ __forceinline__ uint32_t chunk_errors(const job_dev& J, const lds_tab
 }
 
 // Records of one position shared by several records, from index i (the
-// oracle's seq_read step, errors aside): the first carried SNV sets the true
-// base, every record gains depth and each SID head an alternative read when
-// its alternative is the true base (SNV) or carried (indel). Returns the index
-// past the position.
+// oracle's seq_read step): the first carried SNV sets the true base, the
+// read's error there (summary e; base index p + boff) substitutes it, every
+// record gains depth and each SID head an alternative read when its
+// alternative is observed (SNV) or carried (indel). Returns the index past
+// the position.
 template <class V>
 __device__ __forceinline__ uint32_t count_group(const job_dev& J, const V& v, uint32_t i, uint32_t allele,
-                                                uint32_t leaf) {
+                                                uint32_t leaf, const terr& e, uint32_t boff) {
     const uint32_t p = v.pos[i];
     const uint32_t g0 = i;
     uint32_t g1 = i + 1;
@@ -359,12 +417,13 @@ __device__ __forceinline__ uint32_t count_group(const job_dev& J, const V& v, ui
             break;
         }
     }
+    const uint32_t ob = observe(e, p + boff, tb);
     for (uint32_t q = g0; q < g1; ++q) {
         v.add_dep(q);
         const uint32_t in = v.info[q];
         if (!PSX_INFO_HEAD(in)) continue;
         if (PSX_INFO_KIND(in) == PSX_SNV) {
-            if (tb == PSX_INFO_ALT(in)) v.add_alt(q);
+            if (ob == PSX_INFO_ALT(in)) v.add_alt(q);
         } else {
             bool car = false;
             for (uint32_t q2 = q; q2 < g1 && (q2 == q || !PSX_INFO_HEAD(v.info[q2])); ++q2) {
@@ -377,11 +436,11 @@ __device__ __forceinline__ uint32_t count_group(const job_dev& J, const V& v, ui
     return g1;
 }
 
-// A sequencing error replacing the true base at the position of record i
-// (the first of its position) by substitution index sub: the counts above
-// took the true base, so each SNV head there changes its alternative count
-// by [observed == alt] - [true == alt]. Depth and indel counts do not depend
-// on the observed base. Oracle: seq_read with the read's errors.
+// A sequencing error of a slow template replacing the true base at the
+// position of record i (the first of its position) by substitution index
+// sub: the counts took the true base there, so each SNV head changes its
+// alternative count by [observed == alt] - [true == alt]. Depth and indel
+// counts do not depend on the observed base.
 template <class V>
 __device__ __forceinline__ void error_fix(const job_dev& J, const V& v, uint32_t i, uint32_t allele, uint32_t leaf,
                                           uint32_t sub) {
@@ -405,73 +464,73 @@ __device__ __forceinline__ void error_fix(const job_dev& J, const V& v, uint32_t
     }
 }
 
-// The sequencing errors of a wave's templates [base, base + 64) (one per
-// lane: t, counted when valid), chunk by chunk, one error per lane: the
-// error's read is found from its base index, and an error at a position that
-// holds records corrects the counts there. A read covering that position
-// resolved its template's leaf while it was counted.
+// The slow templates among a wave's 64 (one per lane: t, e, counted when
+// valid) were counted without errors: for each, the wave generates its
+// errors and each lane takes one, correcting the counts where the error's
+// base holds records. A read covering that position resolved its
+// template's leaf while it was counted.
 template <class V>
-__device__ __forceinline__ void count_errors(const job_dev& J, const V& v, const lds_tables& tab, const tmpl& t,
-                                             bool valid, uint32_t base, uint32_t nj, uint32_t j, uint32_t c2,
-                                             uint32_t* list, uint32_t lane) {
-    const uint32_t E = J.err_E;
-    const uint32_t top = min(nj, base + 64u);
-    // allele (8 bits), mate 2 start - mate 1 start (< 2^13), counted
-    const uint32_t al = t.allele | ((t.x2 - t.x) << 8) | (valid ? 0x80000000u : 0u);
-    for (uint32_t c = base / E; c * E < top; ++c) {  // E divides 64: the chunks of the wave's templates
+__device__ __forceinline__ void count_slow_errors(const job_dev& J, const V& v, const lds_tables& tab, const tmpl& t,
+                                                  const terr& e, bool valid, uint32_t base, uint32_t j, uint32_t c2,
+                                                  uint32_t* list, uint32_t lane) {
+    unsigned long long sm = __ballot(valid && e.slow());
+    while (sm) {  // wave-uniform
+        const uint32_t sl = (uint32_t)__ffsll((long long)sm) - 1u;
+        sm &= sm - 1ull;
+        const uint32_t xs = (uint32_t)__builtin_amdgcn_readlane((int)t.x, (int)sl);
+        const uint32_t x2s = (uint32_t)__builtin_amdgcn_readlane((int)t.x2, (int)sl);
+        const uint32_t ls = (uint32_t)__builtin_amdgcn_readlane((int)t.leaf, (int)sl);
+        const uint32_t as = (uint32_t)__builtin_amdgcn_readlane((int)t.allele, (int)sl);
         wave_sync();
-        const uint32_t n = chunk_errors(J, tab, c, j, c2, list, lane);
+        const uint32_t n = wave_template_errors(J.seed, J.err_n, J.err_k0, J.err_N, tab.err, base + sl, j, c2, list,
+                                                lane);
         wave_sync();
         for (uint32_t k0 = 0; k0 < n; k0 += 64u) {
             const uint32_t k = k0 + lane;
-            const uint32_t e = k < n ? list[k] : 0u;
-            const uint32_t b = e & 0xFFFFFu, sub = e >> 20;
-            const uint32_t slot = __umulhi(b, J.inv_L);  // read of the chunk: template * (pe + 1) + mate
-            const uint32_t o = b - slot * J.L;
-            const uint32_t r = c * E + (slot >> J.pe);
-            const uint32_t src = (r - base) & 63u;
-            const uint32_t xs = (uint32_t)__shfl((int)t.x, (int)src);
-            const uint32_t ls = (uint32_t)__shfl((int)t.leaf, (int)src);
-            const uint32_t as = (uint32_t)__shfl((int)al, (int)src);
-            if (k < n && r < top && (as >> 31)) {  // rare: an error of a counted read
-                const uint32_t P = xs + ((slot & J.pe) ? (as >> 8) & 0x1FFFu : 0u) + o;
+            if (k < n) {
+                const uint32_t w = list[k];
+                const uint32_t b = w & 0xFFFFFu, sub = w >> 20;
+                const uint32_t m = b >= J.L ? 1u : 0u;
+                const uint32_t P = (m ? x2s : xs) + b - m * J.L;
                 const uint32_t i = v.first(P);
-                if (i < v.n && v.pos[i] == P) error_fix(J, v, i, as & 0xFFu, ls, sub);
+                if (i < v.n && v.pos[i] == P) error_fix(J, v, i, as, ls, sub);
             }
         }
     }
 }
 
-// One read [x, x + L) counted with its true bases: depth for every record at
-// a covered position, an alternative read for each SID head whose
-// alternative is the true base (SNV) or carried (indel). Oracle: seq_read
-// without errors. A record alone at its position (every record under the
+// One read [x, x + L) (mate `mate` of the lane's template; summary e): depth
+// for every record at a covered position, an alternative read for each SID
+// head whose alternative is observed (SNV) or carried (indel). Oracle:
+// seq_read. A record alone at its position (every record under the
 // infinite-sites model) takes one branch-free step of a wave-uniform loop; a
 // position shared by several records goes through count_group. Lanes with
 // valid == false take no part. Returns whether any lane's read covers a
 // record (wave-uniform).
 template <class V>
-__device__ __forceinline__ bool count_read(const job_dev& J, const seg_dev& S, const V& v, tmpl& t, bool valid,
-                                           uint32_t x) {
+__device__ __forceinline__ bool count_read(const job_dev& J, const seg_dev& S, const V& v, tmpl& t, const terr& e,
+                                           bool valid, uint32_t x, uint32_t mate) {
     const uint32_t end = x + J.L;
     uint32_t i = v.first_u(x);
     bool go = valid && i < v.n && v.pos[v.at(i)] < end;
     if (!__ballot(go)) return false;
     uint32_t leaf = 0;
     if (go) leaf = leaf_of(J, S, t);
+    const uint32_t boff = mate * J.L - x;  // base index of position p: p + boff
     while (__ballot(go)) {
         const uint32_t ic = go ? i : 0u;
-        const uint32_t in = v.info[ic];
+        const uint32_t p = v.pos[ic], in = v.info[ic];
         if (__ballot(go && !PSX_INFO_SOLO(in))) {
-            if (go && !PSX_INFO_SOLO(in)) i = count_group(J, v, ic, t.allele, leaf);
+            if (go && !PSX_INFO_SOLO(in)) i = count_group(J, v, ic, t.allele, leaf, e, boff);
         }
         const bool one = go && PSX_INFO_SOLO(in);
         const bool car = PSX_INFO_ALLELE(in) == t.allele && carried(J, in, v.lo[ic], v.hi[ic], leaf);
         const bool snv = PSX_INFO_KIND(in) == PSX_SNV;
         const uint32_t tb = snv && car ? PSX_INFO_ALT(in) : PSX_INFO_REF(in);
+        const uint32_t ob = observe(e, p + boff, tb);
         if (one) {
             v.add_dep(ic);
-            if (PSX_INFO_HEAD(in) && (snv ? tb == PSX_INFO_ALT(in) : car)) v.add_alt(ic);
+            if (PSX_INFO_HEAD(in) && (snv ? ob == PSX_INFO_ALT(in) : car)) v.add_alt(ic);
         }
         i += one ? 1u : 0u;
         go = go && i < v.n && v.pos[v.at(i)] < end;
@@ -479,15 +538,16 @@ __device__ __forceinline__ bool count_read(const job_dev& J, const seg_dev& S, c
     return true;
 }
 
-// One mate of 64 templates of a wave (one per lane), counted with true bases:
-// every lane finds the positions its read covers and queues (lane, first
-// record of the position) in the wave's LDS queue; the queue is drained 64
-// entries at a time, one per lane, so the per-position work runs on full
-// waves instead of waiting for the lane whose read covers the most positions.
-// Same counts and result as count_read. Lanes with valid == false take no
-// part (x must still lie in the stratum).
+// One mate of 64 templates of a wave (one per lane): every lane finds the
+// positions its read covers and queues (lane, first record of the position)
+// in the wave's LDS queue; the queue is drained 64 entries at a time, one per
+// lane, so the per-position work runs on full waves instead of waiting for
+// the lane whose read covers the most positions. Same counts and result as
+// count_read. Lanes with valid == false take no part (x must still lie in the
+// stratum).
 __device__ __forceinline__ bool count_mate_wave(const job_dev& J, const seg_dev& S, const lds_view& v, tmpl& t,
-                                                bool valid, uint32_t x, uint32_t* q, uint32_t lane) {
+                                                const terr& e, bool valid, uint32_t x, uint32_t mate, uint32_t* q,
+                                                uint32_t lane) {
     const uint32_t end = x + J.L;
     uint32_t i = v.first_u(x);
     bool go = valid && i < v.n && v.pos[v.at(i)] < end;
@@ -497,21 +557,25 @@ __device__ __forceinline__ bool count_mate_wave(const job_dev& J, const seg_dev&
     // one drain: entries qb[0 .. count), one per lane
     auto drain = [&](const uint32_t* qb, uint32_t count) {
         const bool on = lane < count;
-        const uint32_t e = on ? qb[lane] : 0u;
-        const uint32_t ol = e >> 26, ri = e & 0x3FFFFFFu;
+        const uint32_t qe = on ? qb[lane] : 0u;
+        const uint32_t ol = qe >> 26, ri = qe & 0x3FFFFFFu;
+        const uint32_t xo = (uint32_t)__shfl((int)x, (int)ol);
         const uint32_t lo = (uint32_t)__shfl((int)leaf, (int)ol);
         const uint32_t al = (uint32_t)__shfl((int)t.allele, (int)ol);
-        const uint32_t in = v.info[ri];
+        const terr eo = shfl_terr(e, ol);
+        const uint32_t boff = mate * J.L - xo;
+        const uint32_t in = v.info[ri], p = v.pos[ri];
         if (__ballot(on && !PSX_INFO_SOLO(in))) {
-            if (on && !PSX_INFO_SOLO(in)) count_group(J, v, ri, al, lo);
+            if (on && !PSX_INFO_SOLO(in)) count_group(J, v, ri, al, lo, eo, boff);
         }
         const bool one = on && PSX_INFO_SOLO(in);
         const bool car = PSX_INFO_ALLELE(in) == al && carried(J, in, v.lo[ri], v.hi[ri], lo);
         const bool snv = PSX_INFO_KIND(in) == PSX_SNV;
         const uint32_t tb = snv && car ? PSX_INFO_ALT(in) : PSX_INFO_REF(in);
+        const uint32_t ob = observe(eo, p + boff, tb);
         if (one) {
             v.add_dep(ri);
-            if (PSX_INFO_HEAD(in) && (snv ? tb == PSX_INFO_ALT(in) : car)) v.add_alt(ri);
+            if (PSX_INFO_HEAD(in) && (snv ? ob == PSX_INFO_ALT(in) : car)) v.add_alt(ri);
         }
     };
     const uint32_t i0 = i;
@@ -607,9 +671,11 @@ __device__ __forceinline__ void count_stratum_global(const job_dev& J, const seg
             valid = valid && t.ok;
             *acc += valid ? 1u : 0u;
         }
+        const terr e = J.err_n ? template_errors(J, tab, valid ? r : base, j, c2, valid) : terr{0u, 0u};
         bool hit = false;  // errors only matter where a read covers a record
-        for (uint32_t mate = 0; mate <= J.pe; ++mate) hit |= count_read(J, S, v, t, valid, mate ? t.x2 : t.x);
-        if (J.err_n && hit) count_errors(J, v, tab, t, valid, base, nj, j, c2, list, lane);
+        for (uint32_t mate = 0; mate <= J.pe; ++mate)
+            hit |= count_read(J, S, v, t, e, valid, mate ? t.x2 : t.x, mate);
+        if (J.err_n && hit) count_slow_errors(J, v, tab, t, e, valid, base, j, c2, list, lane);
     }
 }
 
@@ -621,7 +687,7 @@ __global__ void __launch_bounds__(BLOCK) k_seq_count(const seq_launch L) {
     __shared__ uint16_t s_bucket[NB_MAX];
     __shared__ lds_tables s_tab;
     __shared__ uint32_t s_item, s_acc;
-    // per wave: the (lane, record) queue of a mate, then the errors of a chunk
+    // per wave: the (lane, record) queue of a mate, then the errors of a slow template
     __shared__ uint32_t s_wave[BLOCK / 64][ELIST];
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63u, wave = tid >> 6;
@@ -654,10 +720,12 @@ __global__ void __launch_bounds__(BLOCK) k_seq_count(const seq_launch L) {
                         valid = valid && t.ok;
                         acc += valid ? 1u : 0u;
                     }
+                    const terr e = J.err_n ? template_errors(J, s_tab, valid ? r : base, j, c2, valid) : terr{0u, 0u};
                     bool hit = false;  // errors only matter where a read covers a record
                     for (uint32_t mate = 0; mate <= J.pe; ++mate)
-                        hit |= count_mate_wave(J, S, v, t, valid, mate ? t.x2 : t.x, s_wave[wave], lane);
-                    if (J.err_n && hit) count_errors(J, v, s_tab, t, valid, base, nj, j, c2, s_wave[wave], lane);
+                        hit |= count_mate_wave(J, S, v, t, e, valid, mate ? t.x2 : t.x, mate, s_wave[wave], lane);
+                    if (J.err_n && hit)
+                        count_slow_errors(J, v, s_tab, t, e, valid, base, j, c2, s_wave[wave], lane);
                 }
                 __syncthreads();
                 int32_t* ga = J.alt + S.out_off + s.k0;
@@ -740,15 +808,16 @@ __global__ void __launch_bounds__(256) k_seq_init(const job_dev* __restrict__ jo
 // coalesced stores, so no block barrier separates generation from the copy.
 
 // One read into row[0 .. 2 + ceil(L/16)): word 0 = start, word 1 = leaf |
-// allele << 26 | mate << 30 (bit 31, set later, flags a read with errors),
-// then the 2-bit true bases (base i of the read in bits 2 (i % 16) of word
-// 2 + i / 16) taken from the staged reference (s_ref[0] is reference word
-// ref_b / 16) with the first carried SNV at each position applied; bases
-// past the read end are 0. The read's sequencing errors are applied to the
-// row afterwards (apply_error). Oracle: the bases seq_read compares.
+// allele << 26 | mate << 30 | (errors > 0) << 31, then the 2-bit bases (base
+// i of the read in bits 2 (i % 16) of word 2 + i / 16) taken from the staged
+// reference (s_ref[0] is reference word ref_b / 16), with the first carried
+// SNV at each position and the template summary's errors of this mate
+// applied; bases past the read end are 0. A slow template's errors are
+// applied to the row afterwards (apply_error). Oracle: read_row.
 template <int NW, class V>
 __device__ __forceinline__ void write_read(const job_dev& J, const V& v, uint32_t x, uint32_t leaf, uint32_t allele,
-                                           uint32_t mate, const uint32_t* s_ref, uint32_t ref_b, uint32_t* row) {
+                                           uint32_t mate, const terr& e, const uint32_t* s_ref, uint32_t ref_b,
+                                           uint32_t* row) {
     const uint32_t tail = J.L & 15u;
     const uint32_t xr = x - ref_b;
     const uint32_t w = xr >> 4;
@@ -796,8 +865,24 @@ __device__ __forceinline__ void write_read(const job_dev& J, const V& v, uint32_
         i += go ? 1u : 0u;
         go = go && i < v.n && v.pos[v.at(i)] < end;
     }
+    // the summary's errors on this mate (wave-uniform trip count, predicated)
+    const uint32_t ne = e.n();
+    uint32_t flag = 0;
+    for (uint32_t q = 0; __ballot(q < ne); ++q) {
+        const uint32_t eq = q == 0u ? (e.w01 & 0xFFFFu) : (q == 1u ? e.w01 >> 16 : e.w2n & 0xFFFFu);
+        const uint32_t b = eq & 0x1FFFu;
+        const uint32_t m = b >= J.L ? 1u : 0u;
+        if (q < ne && m == mate) {
+            const uint32_t o = b - m * J.L, s2 = (o & 15u) * 2u;
+            const uint32_t wv = row[2 + (o >> 4)];
+            const uint32_t tb = (wv >> s2) & 3u, sub = eq >> 13;
+            const uint32_t nb = sub < tb ? sub : sub + 1u;
+            row[2 + (o >> 4)] = (wv & ~(3u << s2)) | (nb << s2);
+            flag = 1u;
+        }
+    }
     if constexpr (NW == 0) row[0] = x;
-    row[1] = leaf | (allele << 26) | (mate << 30);
+    row[1] = leaf | (allele << 26) | (mate << 30) | (flag << 31);
 }
 
 // A sequencing error at offset o of a written read: the base there is
@@ -832,7 +917,7 @@ __global__ void __launch_bounds__(BLOCK) k_seq_reads(const seq_launch L) {
     __shared__ uint32_t s_ref[REF_MAX];
     __shared__ lds_tables s_tab;
     __shared__ uint32_t s_item;
-    __shared__ uint32_t s_elist[WAVES][ELIST];  // per wave: errors of one chunk
+    __shared__ uint32_t s_elist[WAVES][ELIST];  // per wave: errors of a slow template
     extern __shared__ uint4 s_stage4[];  // STAGE: per wave 64 rows of read_words words
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63u, wave = tid >> 6;
@@ -868,7 +953,6 @@ __global__ void __launch_bounds__(BLOCK) k_seq_reads(const seq_launch L) {
             const glb_view gv{J.pos + s.k0, J.lo + s.k0, J.hi + s.k0, J.info + s.k0, nullptr, nullptr, s.n_win};
             const uint32_t nj = (uint32_t)s.nj;
             for (uint32_t c0 = wave * 64u; c0 < nj; c0 += WAVES * 64u) {
-                uint32_t chunk = UNRESOLVED, n_err = 0;  // the chunk in the wave's list
                 const uint32_t rt = c0 + lane;
                 uint32_t tx = 0, tx2 = 0, tla = 0;
                 if (rt < nj) {
@@ -877,12 +961,14 @@ __global__ void __launch_bounds__(BLOCK) k_seq_reads(const seq_launch L) {
                     tx2 = t.x2;
                     tla = leaf_of(J, S, t) | (t.allele << 26);
                 }
+                const terr te = J.err_n ? template_errors(J, s_tab, rt < nj ? rt : c0, j, c2, rt < nj) : terr{0u, 0u};
                 for (uint32_t p = 0; p < rpt; ++p) {
                     const uint32_t src = p * tpp + lane / rpt;
                     const uint32_t mate = lane & (rpt - 1u);
                     const uint32_t x1 = (uint32_t)__shfl((int)tx, (int)src);
                     const uint32_t x2 = (uint32_t)__shfl((int)tx2, (int)src);
                     const uint32_t la = (uint32_t)__shfl((int)tla, (int)src);
+                    const terr e = shfl_terr(te, src);
                     const uint32_t r = c0 + src;
                     const uint32_t t0 = c0 + p * tpp;  // first template of the pass
                     const uint32_t nt = t0 < nj ? min(tpp, nj - t0) : 0u;
@@ -891,41 +977,37 @@ __global__ void __launch_bounds__(BLOCK) k_seq_reads(const seq_launch L) {
                     if (r < nj) {
                         const uint32_t x = mate ? x2 : x1;
                         if (fits)
-                            write_read<NW>(J, lv, x, la & 0x3FFFFFFu, la >> 26, mate, s_ref, ref_w0 << 4, row);
+                            write_read<NW>(J, lv, x, la & 0x3FFFFFFu, la >> 26, mate, e, s_ref, ref_w0 << 4, row);
                         else
-                            write_read<NW>(J, gv, x, la & 0x3FFFFFFu, la >> 26, mate, s_ref, ref_w0 << 4, row);
+                            write_read<NW>(J, gv, x, la & 0x3FFFFFFu, la >> 26, mate, e, s_ref, ref_w0 << 4, row);
                     }
-                    if (J.err_n && nt) {
-                        // errors of the pass's templates [t0, t0 + nt), chunk by chunk (E divides 64, so
-                        // a chunk lies in one pass or spans both passes of a wave step)
-                        const uint32_t E = J.err_E;
-                        for (uint32_t c = t0 / E; c * E < t0 + nt; ++c) {
-                            if (c != chunk) {
-                                wave_sync();
-                                n_err = chunk_errors(J, s_tab, c, j, c2, list, lane);
-                                chunk = c;
-                            }
-                            wave_sync();  // list entries and (STAGE) every row of the pass written
-                            for (uint32_t k0 = 0; k0 < n_err; k0 += 64u) {
-                                const uint32_t k = k0 + lane;
-                                const uint32_t e = k < n_err ? list[k] : 0u;
-                                const uint32_t b = e & 0xFFFFFu, sub = e >> 20;
-                                const uint32_t slot = __umulhi(b, J.inv_L);  // read of the chunk
-                                const uint32_t o = b - slot * J.L;
-                                const uint32_t tr = c * E + (slot >> J.pe);
-                                const uint32_t rl = (tr - t0) * rpt + (slot & J.pe);  // its lane in the pass
-                                const bool on = k < n_err && tr >= t0 && tr < t0 + nt;
-                                if (STAGE) {
-                                    if (on) apply_error<true>(stage + rl * RW, o, sub);
-                                } else {
-                                    // rows live in global memory: each lane patches its own read
-                                    for (uint32_t q = 0; q < 64u && k0 + q < n_err; ++q) {
-                                        const uint32_t rq = (uint32_t)__shfl((int)rl, (int)q);
-                                        const uint32_t oq = (uint32_t)__shfl((int)o, (int)q);
-                                        const uint32_t sq = (uint32_t)__shfl((int)sub, (int)q);
-                                        const uint32_t onq = (uint32_t)__shfl(on ? 1 : 0, (int)q);
-                                        if (onq && rq == lane) apply_error<false>(row, oq, sq);
-                                    }
+                    // slow templates of the pass (template lanes [p tpp, p tpp + nt)): the wave
+                    // generates each one's errors and patches its reads
+                    unsigned long long sm = __ballot(te.slow() && lane >= p * tpp && lane < p * tpp + nt);
+                    while (sm) {  // wave-uniform, rare
+                        const uint32_t sl = (uint32_t)__ffsll((long long)sm) - 1u;
+                        sm &= sm - 1ull;
+                        wave_sync();
+                        const uint32_t n_err = wave_template_errors(J.seed, J.err_n, J.err_k0, J.err_N, s_tab.err,
+                                                                    c0 + sl, j, c2, list, lane);
+                        wave_sync();  // list entries and (STAGE) every row of the pass written
+                        for (uint32_t k0 = 0; k0 < n_err; k0 += 64u) {
+                            const uint32_t k = k0 + lane;
+                            const uint32_t w = k < n_err ? list[k] : 0u;
+                            const uint32_t b = w & 0xFFFFFu, sub = w >> 20;
+                            const uint32_t m = b >= J.L ? 1u : 0u;
+                            const uint32_t o = b - m * J.L;
+                            const uint32_t rl = (sl - p * tpp) * rpt + m;  // the read's lane in the pass
+                            const bool on = k < n_err;
+                            if (STAGE) {
+                                if (on) apply_error<true>(stage + rl * RW, o, sub);
+                            } else {
+                                // rows live in global memory: each lane patches its own read
+                                for (uint32_t q = 0; q < 64u && k0 + q < n_err; ++q) {
+                                    const uint32_t rq = (uint32_t)__shfl((int)rl, (int)q);
+                                    const uint32_t oq = (uint32_t)__shfl((int)o, (int)q);
+                                    const uint32_t sq = (uint32_t)__shfl((int)sub, (int)q);
+                                    if (rq == lane) apply_error<false>(row, oq, sq);
                                 }
                             }
                         }

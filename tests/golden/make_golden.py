@@ -32,7 +32,7 @@ from process_amd import core  # noqa: E402
 
 M32 = 0xFFFFFFFF
 STRATUM = 8192
-ERR_CHUNK, ERR_KCAP = 64, 128
+ERR_KCAP = 128
 TEMPLATE, ERROR = 0x10, 0x11
 
 
@@ -96,24 +96,18 @@ def insert_table(mean: int, sd: int) -> tuple[int, list[int]]:
     return binomial_table(int(mean / p), p)
 
 
-def error_chunk(L: int, rpt: int, e: float) -> int:
-    """Templates per error chunk: the largest power of two <= 64 whose chunk of
-    N = E rpt L bases holds more than 256 errors with probability below 2^-40."""
-    E = ERR_CHUNK
-    while E >= 1:
-        N = E * rpt * L
-        if N <= ERR_KCAP:
-            return E
-        lp, lq, lgn, tail = math.log(e), math.log1p(-e), math.lgamma(N + 1.0), 0.0
-        for k in range(ERR_KCAP + 1, N + 1):
-            term = math.exp(lgn - math.lgamma(k + 1.0) - math.lgamma(N - k + 1.0) + k * lp + (N - k) * lq)
-            tail += term
-            if tail >= 2.0 ** -40 or (term < 1e-300 and k > N * e):
-                break
-        if tail < 2.0 ** -40:
-            return E
-        E //= 2
-    return 0
+def error_rate_ok(N: int, e: float) -> bool:
+    """A template's N bases hold more than ERR_KCAP errors with probability
+    below 2^-40."""
+    if N <= ERR_KCAP:
+        return True
+    lp, lq, lgn, tail = math.log(e), math.log1p(-e), math.lgamma(N + 1.0), 0.0
+    for k in range(ERR_KCAP + 1, N + 1):
+        term = math.exp(lgn - math.lgamma(k + 1.0) - math.lgamma(N - k + 1.0) + k * lp + (N - k) * lq)
+        tail += term
+        if tail >= 2.0 ** -40 or (term < 1e-300 and k > N * e):
+            break
+    return tail < 2.0 ** -40
 
 
 def error_table(N: int, e: float) -> list[int]:
@@ -165,8 +159,8 @@ def brute_force_seq(lens, packed, forest, recs, seed, ploidy=2, cna=None, rows=N
     woff = np.concatenate([[0], np.cumsum((np.asarray(lens, np.int64) + 15) // 16)])
     ins_k0, ins_T = insert_table(p["insert_size_mean"], p.get("insert_size_stddev", 10)) if pe else (0, [])
     rpt = 2 if pe else 1
-    err_E = error_chunk(L, rpt, e) if e > 0 else 0
-    err_N = err_E * rpt * L
+    err_N = rpt * L
+    assert e == 0 or error_rate_ok(err_N, e)
     err_T = error_table(err_N, e) if e > 0 else []
     thr_normal = int((1.0 - purity) * 4294967296.0)
     head = recs["sid_head"].astype(bool)
@@ -198,18 +192,24 @@ def brute_force_seq(lens, packed, forest, recs, seed, ploidy=2, cna=None, rows=N
                 has = any(int(recs["leaf_lo"][q]) <= v < int(recs["leaf_hi"][q]) for v in leaves)
             pres[so, q] = has
 
-    def chunk_errors(c, j, c2):
-        """{(template in chunk, mate): [(offset, substitution), ...]} of error chunk c."""
-        K = pick(err_T, philox(seed, c, j, c2, ERROR << 24)[1])
+    def template_errors(r, j, c2):
+        """{mate: [(offset, substitution), ...]} of template r: K from word y of
+        the first draw, then distinct base indices from x, z, w and the words
+        of the following draws."""
+        w0 = philox(seed, r, j, c2, ERROR << 24)
+        K = pick(err_T, w0[1])
         taken: list[int] = []
-        out: dict[tuple[int, int], list[tuple[int, int]]] = {}
+        out: dict[int, list[tuple[int, int]]] = {}
         t = 0
         while len(taken) < K and t < 4 * ERR_KCAP:
-            word = philox(seed, c, j, c2, (ERROR << 24) | t)[0]
+            if t < 3:
+                word = w0[(0, 2, 3)[t]]
+            else:
+                word = philox(seed, r, j, c2, (ERROR << 24) | (1 + (t - 3) // 4))[(t - 3) % 4]
             b = mulhi(word, err_N)
             if b not in taken:
                 taken.append(b)
-                out.setdefault((b // (rpt * L), (b // L) % rpt), []).append((b % L, ((word & 0xFFFF) * 3) >> 16))
+                out.setdefault(b // L, []).append((b % L, ((word & 0xFFFF) * 3) >> 16))
             t += 1
         return out
 
@@ -270,10 +270,7 @@ def brute_force_seq(lens, packed, forest, recs, seed, ploidy=2, cna=None, rows=N
             c2 = (c << 16) | so
             for j in range((W + STRATUM - 1) // STRATUM):
                 b, e_ = j * STRATUM, min(j * STRATUM + STRATUM, W)
-                chunk: dict = {}
                 for r in range(T * e_ // W - T * b // W):
-                    if e > 0 and r % err_E == 0:
-                        chunk = chunk_errors(r // err_E, j, c2)
                     w = philox(seed, r, j, c2, TEMPLATE << 24)
                     x = b + mulhi(w[0], e_ - b)
                     if not has_ev:
@@ -295,11 +292,11 @@ def brute_force_seq(lens, packed, forest, recs, seed, ploidy=2, cna=None, rows=N
                             continue
                         allele = int(np.searchsorted(cum, k, side="right"))
                         n_reads += 2 if pe else 1
-                    tl = r % err_E if e > 0 else 0
-                    read(so, c, x, leaf, allele, chunk.get((tl, 0), []), 0)
+                    errs = template_errors(r, j, c2) if e > 0 else {}
+                    read(so, c, x, leaf, allele, errs.get(0, []), 0)
                     if pe:
                         I = max(ins_k0 + pick(ins_T, w[3]), L)
-                        read(so, c, min(x + I - L, ln - L), leaf, allele, chunk.get((tl, 1), []), 1)
+                        read(so, c, min(x + I - L, ln - L), leaf, allele, errs.get(1, []), 1)
     return alt, dep, pres, n_reads
 
 
