@@ -25,6 +25,7 @@ struct seq_batch {
     std::vector<size_t> tab_off;  // per job: insert thresholds, insert guide, error thresholds
     std::vector<size_t> cna_off;  // per job: copy-number events (SIZE_MAX: none)
     std::vector<uint64_t> job_reads;  // per job: reads of the sampling model without copy-number events
+    double cover = 0.0;  // largest expected number of covered positions per 64 templates over the jobs
     size_t o_jobs = 0, o_segs = 0, o_items = 0, o_pres = 0, o_acc = 0;
     uint8_t* dev = nullptr;       // device base of the last upload
 };
@@ -143,6 +144,18 @@ void prepare_job(psx_ctx* ctx, seq_batch& P, const job_in& in, uint32_t spi) {
     }
     P.cna_off.push_back(cdev.empty() ? SIZE_MAX : P.sb.put_vec(cdev));
     P.jobs.push_back(J);
+    if (p->error_rate > 0.0) {
+        // expected positions covered by 64 templates: records per base x bases read, times the share of
+        // templates kept under copy numbers (ploidy / kmax, length-weighted)
+        double recs = 0.0, bases = 0.0, kept = 0.0;
+        for (uint32_t c = 0; c < g->n_chr; ++c) {
+            if (!sel[c] || g->len[c] < L) continue;
+            recs += (double)(m->chr_begin[c + 1] - m->chr_begin[c]);
+            bases += (double)g->len[c];
+            kept += (double)g->len[c] * (double)ploidy / (double)c_kmax[c];
+        }
+        if (bases > 0.0) P.cover = std::max(P.cover, 64.0 * (pe ? 2.0 : 1.0) * L * recs / bases * kept / bases);
+    }
     const uint64_t reads_before = P.n_reads;
     for (uint32_t so = 0; so < n_out; ++so) {
         const bool normal = so == n_req;
@@ -192,16 +205,21 @@ void layout(seq_batch& P) {
 }
 
 template <int BLOCK, int CAP>
-void launch_count_t(const seq_lane& lane, const seq_launch& L, uint32_t grid, bool cna) {
-    if (cna)
-        hipLaunchKernelGGL((k_seq_count<BLOCK, CAP, true>), dim3(grid), dim3(BLOCK), 0, lane.stream, L);
+void launch_count_t(const seq_lane& lane, const seq_launch& L, uint32_t grid, bool cna, bool defer) {
+    if (cna && defer)
+        hipLaunchKernelGGL((k_seq_count<BLOCK, CAP, true, true>), dim3(grid), dim3(BLOCK), 0, lane.stream, L);
+    else if (cna)
+        hipLaunchKernelGGL((k_seq_count<BLOCK, CAP, true, false>), dim3(grid), dim3(BLOCK), 0, lane.stream, L);
+    else if (defer)
+        hipLaunchKernelGGL((k_seq_count<BLOCK, CAP, false, true>), dim3(grid), dim3(BLOCK), 0, lane.stream, L);
     else
-        hipLaunchKernelGGL((k_seq_count<BLOCK, CAP, false>), dim3(grid), dim3(BLOCK), 0, lane.stream, L);
+        hipLaunchKernelGGL((k_seq_count<BLOCK, CAP, false, false>), dim3(grid), dim3(BLOCK), 0, lane.stream, L);
 }
 
-void launch_count(const seq_lane& lane, const psx_launch_config& c, const seq_launch& L, uint32_t grid, bool cna) {
+void launch_count(const seq_lane& lane, const psx_launch_config& c, const seq_launch& L, uint32_t grid, bool cna,
+                  bool defer) {
 #define PSX_CASE(B, C) \
-    if (c.block_threads == B && c.lds_records == C) return launch_count_t<B, C>(lane, L, grid, cna);
+    if (c.block_threads == B && c.lds_records == C) return launch_count_t<B, C>(lane, L, grid, cna, defer);
     PSX_CASE(256, 512) PSX_CASE(256, 1024) PSX_CASE(256, 2048)
     PSX_CASE(512, 512) PSX_CASE(512, 1024) PSX_CASE(512, 2048)
 #undef PSX_CASE
@@ -275,7 +293,8 @@ void batch_launch(const seq_batch* P, const seq_lane& lane, const psx_launch_con
     hipLaunchKernelGGL(k_seq_init, dim3(init_grid), dim3(256), 0, lane.stream, L.jobs, ps, (uint32_t)P->pres.size(),
                        P->pres_total, L.queue, (uint32_t)P->jobs.size());
     if (timed) PSX_HIP(hipEventRecord(lane.ev0, lane.stream));
-    launch_count(lane, cfg, L, grid, batch_has_cna(P));
+    // errors per covered position when every job's reads cover fewer than 48 positions per 64 templates
+    launch_count(lane, cfg, L, grid, batch_has_cna(P), P->cover < 48.0);
     PSX_HIP(hipGetLastError());
     if (timed) PSX_HIP(hipEventRecord(lane.ev1, lane.stream));
 }

@@ -330,6 +330,18 @@ __device__ __forceinline__ uint32_t observe(const terr& e, uint32_t b, uint32_t 
     return sub == 4u ? tb : (sub < tb ? sub : sub + 1u);
 }
 
+// The substitution index of the summary's error at template base index b, or
+// 4 when b has none.
+__device__ __forceinline__ uint32_t error_sub(const terr& e, uint32_t b) {
+    const uint32_t n = e.n();
+    const uint32_t e0 = e.w01 & 0xFFFFu, e1 = e.w01 >> 16, e2 = e.w2n & 0xFFFFu;
+    uint32_t sub = 4u;
+    sub = n > 0u && (e0 & 0x1FFFu) == b ? e0 >> 13 : sub;
+    sub = n > 1u && (e1 & 0x1FFFu) == b ? e1 >> 13 : sub;
+    sub = n > 2u && (e2 & 0x1FFFu) == b ? e2 >> 13 : sub;
+    return sub;
+}
+
 // Per-wave LDS of a slow template's errors: the list, then a 2048-bit hashed
 // bitmap of the base indices drawn so far.
 constexpr uint32_t ELIST = PSX_ERR_KCAP + 64u;
@@ -538,20 +550,78 @@ __device__ __forceinline__ bool count_read(const job_dev& J, const seg_dev& S, c
     return true;
 }
 
+// Deferred error checks of a wave (count kernel, LDS windows): a ring of
+// 128 entries of three words (template r; read base index b | allele << 13 |
+// record << 21; leaf), filled by the queue drains with every covered position
+// whose observed base matters (an SNV head, or several records) and checked
+// 64 at a time, so one draw per lane serves 64 covered positions instead of
+// one per template. head is 0 or 64 and count < 64 between drains.
+struct chk_ring {
+    uint32_t* e;  // 3 * 128 words, entry-major
+    uint32_t head, count;  // wave-uniform
+};
+
+// Checks the n <= 64 entries at the ring's head, one per lane: the entry's
+// template draws its error summary, and an error at the entry's base index
+// corrects the counts there (error_fix). The entries are read first, so a
+// slow template's error list can use their half of the ring. Oracle:
+// seq_read with the read's errors.
+__device__ __forceinline__ void check_errors(const job_dev& J, const lds_view& v, const lds_tables& tab, uint32_t j,
+                                             uint32_t c2, chk_ring& q, uint32_t n, uint32_t lane) {
+    const bool on = lane < n;
+    uint32_t* half = q.e + q.head * 3u;
+    const uint32_t r = on ? half[lane * 3u] : 0u;
+    const uint32_t w = on ? half[lane * 3u + 1u] : 0u;
+    const uint32_t leaf = on ? half[lane * 3u + 2u] : 0u;
+    q.head = (q.head + 64u) & 127u;
+    q.count -= n;
+    const uint32_t b = w & 0x1FFFu, al = (w >> 13) & 0xFFu, ri = w >> 21;
+    const terr e = template_errors(J, tab, r, j, c2, on);
+    const uint32_t sub = error_sub(e, b);
+    if (on && sub < 4u) error_fix(J, v, ri, al, leaf, sub);
+    unsigned long long sm = __ballot(on && e.slow());
+    while (sm) {  // wave-uniform, rare
+        const uint32_t sl = (uint32_t)__ffsll((long long)sm) - 1u;
+        sm &= sm - 1ull;
+        const uint32_t rs = (uint32_t)__builtin_amdgcn_readlane((int)r, (int)sl);
+        const uint32_t bs = (uint32_t)__builtin_amdgcn_readlane((int)b, (int)sl);
+        wave_sync();
+        const uint32_t ne = wave_template_errors(J.seed, J.err_n, J.err_k0, J.err_N, tab.err, rs, j, c2, half, lane);
+        wave_sync();
+        uint32_t found = 4u;  // at most one of the template's errors is at bs
+        for (uint32_t k0 = 0; k0 < ne; k0 += 64u) {
+            const uint32_t k = k0 + lane;
+            const uint32_t le = k < ne ? half[k] : 0u;
+            found = k < ne && (le & 0xFFFFFu) == bs ? le >> 20 : found;
+        }
+        const unsigned long long fm = __ballot(found < 4u);
+        if (fm) {
+            const uint32_t fs = (uint32_t)__builtin_amdgcn_readlane((int)found, (int)((uint32_t)__ffsll((long long)fm) - 1u));
+            if (lane == sl) error_fix(J, v, ri, al, leaf, fs);
+        }
+    }
+    wave_sync();  // the half is free again
+}
+
 // One mate of 64 templates of a wave (one per lane): every lane finds the
 // positions its read covers and queues (lane, first record of the position)
 // in the wave's LDS queue; the queue is drained 64 entries at a time, one per
 // lane, so the per-position work runs on full waves instead of waiting for
-// the lane whose read covers the most positions. Same counts and result as
-// count_read. Lanes with valid == false take no part (x must still lie in the
-// stratum).
-__device__ __forceinline__ bool count_mate_wave(const job_dev& J, const seg_dev& S, const lds_view& v, tmpl& t,
-                                                const terr& e, bool valid, uint32_t x, uint32_t mate, uint32_t* q,
-                                                uint32_t lane) {
+// the lane whose read covers the most positions. Sequencing errors: with
+// DEFER the reads are counted with their true bases and each drained
+// position whose observed base matters goes to the deferred checks (template
+// base + lane); otherwise the template summaries e give the observed bases
+// (slow templates are corrected afterwards). Lanes with valid == false take
+// no part (x must still lie in the stratum).
+template <bool DEFER>
+__device__ __forceinline__ void count_mate_wave(const job_dev& J, const seg_dev& S, const lds_view& v,
+                                                const lds_tables& tab, tmpl& t, const terr& e, bool valid,
+                                                uint32_t x, uint32_t mate, uint32_t base, uint32_t j, uint32_t c2,
+                                                uint32_t* q, chk_ring& ck, uint32_t lane) {
     const uint32_t end = x + J.L;
     uint32_t i = v.first_u(x);
     bool go = valid && i < v.n && v.pos[v.at(i)] < end;
-    if (!__ballot(go)) return false;
+    if (!__ballot(go)) return;
     uint32_t leaf = 0;
     if (go) leaf = leaf_of(J, S, t);
     // one drain: entries qb[0 .. count), one per lane
@@ -559,23 +629,43 @@ __device__ __forceinline__ bool count_mate_wave(const job_dev& J, const seg_dev&
         const bool on = lane < count;
         const uint32_t qe = on ? qb[lane] : 0u;
         const uint32_t ol = qe >> 26, ri = qe & 0x3FFFFFFu;
-        const uint32_t xo = (uint32_t)__shfl((int)x, (int)ol);
         const uint32_t lo = (uint32_t)__shfl((int)leaf, (int)ol);
         const uint32_t al = (uint32_t)__shfl((int)t.allele, (int)ol);
-        const terr eo = shfl_terr(e, ol);
-        const uint32_t boff = mate * J.L - xo;
         const uint32_t in = v.info[ri], p = v.pos[ri];
-        if (__ballot(on && !PSX_INFO_SOLO(in))) {
-            if (on && !PSX_INFO_SOLO(in)) count_group(J, v, ri, al, lo, eo, boff);
+        const bool grp = !PSX_INFO_SOLO(in);
+        terr eo{0u, 0u};
+        uint32_t boff = 0u;
+        if (!DEFER) {
+            eo = shfl_terr(e, ol);
+            boff = mate * J.L - (uint32_t)__shfl((int)x, (int)ol);
         }
-        const bool one = on && PSX_INFO_SOLO(in);
+        if (__ballot(on && grp)) {
+            if (on && grp) count_group(J, v, ri, al, lo, eo, boff);
+        }
+        const bool one = on && !grp;
         const bool car = PSX_INFO_ALLELE(in) == al && carried(J, in, v.lo[ri], v.hi[ri], lo);
         const bool snv = PSX_INFO_KIND(in) == PSX_SNV;
         const uint32_t tb = snv && car ? PSX_INFO_ALT(in) : PSX_INFO_REF(in);
-        const uint32_t ob = observe(eo, p + boff, tb);
+        const uint32_t ob = DEFER ? tb : observe(eo, p + boff, tb);
         if (one) {
             v.add_dep(ri);
             if (PSX_INFO_HEAD(in) && (snv ? ob == PSX_INFO_ALT(in) : car)) v.add_alt(ri);
+        }
+        if (DEFER) {
+            const bool chk = on && (grp || (snv && PSX_INFO_HEAD(in)));
+            const unsigned long long m = __ballot(chk);
+            if (m) {
+                const uint32_t xo = (uint32_t)__shfl((int)x, (int)ol);
+                const uint32_t rank =
+                    __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                if (chk) {
+                    uint32_t* slot = ck.e + ((ck.head + ck.count + rank) & 127u) * 3u;
+                    slot[0] = base + ol;
+                    slot[1] = (mate * J.L + p - xo) | (al << 13) | (ri << 21);
+                    slot[2] = lo;
+                }
+                ck.count += (uint32_t)__popcll(m);
+            }
         }
     };
     const uint32_t i0 = i;
@@ -594,14 +684,15 @@ __device__ __forceinline__ bool count_mate_wave(const job_dev& J, const seg_dev&
             drain(q + qn - 64u, 64u);
             qn -= 64u;
             wave_sync();
+            if (DEFER && ck.count >= 64u) check_errors(J, v, tab, j, c2, ck, 64u, lane);
         }
     }
     if (qn) {
         wave_sync();
         drain(q, qn);
         wave_sync();
+        if (DEFER && ck.count >= 64u) check_errors(J, v, tab, j, c2, ck, 64u, lane);
     }
-    return true;
 }
 
 // Stratum geometry shared by both kernels.
@@ -680,15 +771,20 @@ __device__ __forceinline__ void count_stratum_global(const job_dev& J, const seg
 }
 
 // CNA: some job has copy-number events (templates may be rejected and are
-// counted); the plain instantiation carries none of that code.
-template <int BLOCK, int CAP, bool CNA>
+// counted); the plain instantiation carries none of that code. DEFER: the
+// jobs' reads cover few positions per wave (sparse tables, rejected
+// templates), so errors are drawn per covered position (deferred checks, one
+// draw per 64 positions) instead of per template; both give the same counts.
+template <int BLOCK, int CAP, bool CNA, bool DEFER>
 __global__ void __launch_bounds__(BLOCK) k_seq_count(const seq_launch L) {
     __shared__ uint32_t s_pos[CAP], s_lo[CAP], s_hi[CAP], s_info[CAP], s_alt[CAP], s_dep[CAP];
     __shared__ uint16_t s_bucket[NB_MAX];
     __shared__ lds_tables s_tab;
     __shared__ uint32_t s_item, s_acc;
-    // per wave: the (lane, record) queue of a mate, then the errors of a slow template
+    // per wave: the (lane, record) queue of a mate (also a slow template's errors on
+    // the global path), and the deferred error checks
     __shared__ uint32_t s_wave[BLOCK / 64][ELIST];
+    __shared__ uint32_t s_chk[BLOCK / 64][DEFER ? 3 * 128 : 1];
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63u, wave = tid >> 6;
     for (;;) {
@@ -712,20 +808,37 @@ __global__ void __launch_bounds__(BLOCK) k_seq_count(const seq_launch L) {
                 stage_window<BLOCK, CAP>(J, s, s_pos, s_lo, s_hi, s_info, s_alt, s_dep, s_bucket, tid);
                 const lds_view v{s_pos, s_lo, s_hi, s_info, s_alt, s_dep, s_bucket, s.n_win, s.b};
                 const uint32_t nj = (uint32_t)s.nj;
-                for (uint32_t base = wave * 64u; base < nj; base += BLOCK) {  // wave-uniform
-                    const uint32_t r = base + lane;
-                    bool valid = r < nj;
-                    tmpl t = make_template<CNA>(J, S, s_tab, valid ? r : base, j, s.b, s.e);
-                    if (CNA) {
-                        valid = valid && t.ok;
-                        acc += valid ? 1u : 0u;
+                chk_ring ck{s_chk[wave], 0u, 0u};
+                if (DEFER) {
+                    for (uint32_t base = wave * 64u; base < nj; base += BLOCK) {  // wave-uniform
+                        const uint32_t r = base + lane;
+                        bool valid = r < nj;
+                        tmpl t = make_template<CNA>(J, S, s_tab, valid ? r : base, j, s.b, s.e);
+                        if (CNA) {
+                            valid = valid && t.ok;
+                            acc += valid ? 1u : 0u;
+                        }
+                        for (uint32_t mate = 0; mate <= J.pe; ++mate)
+                            count_mate_wave<true>(J, S, v, s_tab, t, terr{0u, 0u}, valid, mate ? t.x2 : t.x, mate,
+                                                  base, j, c2, s_wave[wave], ck, lane);
                     }
-                    const terr e = J.err_n ? template_errors(J, s_tab, valid ? r : base, j, c2, valid) : terr{0u, 0u};
-                    bool hit = false;  // errors only matter where a read covers a record
-                    for (uint32_t mate = 0; mate <= J.pe; ++mate)
-                        hit |= count_mate_wave(J, S, v, t, e, valid, mate ? t.x2 : t.x, mate, s_wave[wave], lane);
-                    if (J.err_n && hit)
-                        count_slow_errors(J, v, s_tab, t, e, valid, base, j, c2, s_wave[wave], lane);
+                    if (J.err_n && ck.count) check_errors(J, v, s_tab, j, c2, ck, ck.count, lane);
+                } else {
+                    for (uint32_t base = wave * 64u; base < nj; base += BLOCK) {  // wave-uniform
+                        const uint32_t r = base + lane;
+                        bool valid = r < nj;
+                        tmpl t = make_template<CNA>(J, S, s_tab, valid ? r : base, j, s.b, s.e);
+                        if (CNA) {
+                            valid = valid && t.ok;
+                            acc += valid ? 1u : 0u;
+                        }
+                        const terr e =
+                            J.err_n ? template_errors(J, s_tab, valid ? r : base, j, c2, valid) : terr{0u, 0u};
+                        for (uint32_t mate = 0; mate <= J.pe; ++mate)
+                            count_mate_wave<false>(J, S, v, s_tab, t, e, valid, mate ? t.x2 : t.x, mate, base, j, c2,
+                                                   s_wave[wave], ck, lane);
+                        if (J.err_n) count_slow_errors(J, v, s_tab, t, e, valid, base, j, c2, s_wave[wave], lane);
+                    }
                 }
                 __syncthreads();
                 int32_t* ga = J.alt + S.out_off + s.k0;
