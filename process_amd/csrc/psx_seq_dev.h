@@ -775,8 +775,9 @@ __device__ __forceinline__ void count_stratum_global(const job_dev& J, const seg
 // jobs' reads cover few positions per wave (sparse tables, rejected
 // templates), so errors are drawn per covered position (deferred checks, one
 // draw per 64 positions) instead of per template; both give the same counts.
+// The deferred form is held to 6 waves per SIMD (80 VGPRs; a few spill).
 template <int BLOCK, int CAP, bool CNA, bool DEFER>
-__global__ void __launch_bounds__(BLOCK) k_seq_count(const seq_launch L) {
+__global__ void __launch_bounds__(BLOCK, DEFER ? 6 : 1) k_seq_count(const seq_launch L) {
     __shared__ uint32_t s_pos[CAP], s_lo[CAP], s_hi[CAP], s_info[CAP], s_alt[CAP], s_dep[CAP];
     __shared__ uint16_t s_bucket[NB_MAX];
     __shared__ lds_tables s_tab;
