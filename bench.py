@@ -66,6 +66,34 @@ def cpu_baseline(w, built, recs, threads: int, target_reads: float):
                        f"{len(w.lengths)}, {n} reads in {dt:.2f} s")
 
 
+def cpu_reference_shaped(w, packed, recs, threads: int, target_reads: float):
+    """The reference-shaped CPU restatement (each read built as base and
+    quality strings, mutations and statistics in ordered maps, as RACES'
+    ReadSimulator works; same reads and counts as the oracle) on a bounded
+    sample of the same workload: every output sample over the first
+    chromosomes, up to target_reads reads."""
+    from oracle import oracle as O
+
+    fd = w.forest
+    of = O.Forest(fd["cell_id"], fd["parent"], fd["species"], fd["sample"], fd["birth"], fd["n_samples"])
+    seq = dict(w.seq)
+    L = seq["read_size"]
+    S = fd["n_samples"] + (1 if seq.get("with_normal_sample", True) else 0)
+    chrs, reads = [], 0.0
+    for c in np.argsort(w.lengths, kind="stable"):  # shortest chromosomes first: a short bounded sample
+        chrs.append(int(c))
+        reads += S * seq["coverage"] * int(w.lengths[c]) / L
+        if reads >= target_reads:
+            break
+    seq.update(chromosomes=sorted(chrs))
+    t0 = time.perf_counter()
+    _, _, n = O.refshape_seq(w.lengths, packed, of, recs, w.seq_seed, threads, **seq)
+    dt = time.perf_counter() - t0
+    return dict(value=n / dt, unit="reads/s", cores=threads, kind="port",
+                sample=f"reference-shaped restatement (read strings, ordered maps), all {S} output samples, "
+                       f"chromosomes {sorted(chrs)} of {len(w.lengths)}, {n} reads in {dt:.2f} s")
+
+
 def batched_leg(ctx, n_jobs: int, reps: int, tune: bool) -> dict:
     """The batched workload: every job in one persistent hipGraph launch
     against the same jobs launched one by one (each call waits for its
@@ -418,6 +446,14 @@ def main() -> int:
         out["cpu_baseline"] = cb
         log(f"cpu baseline: {cb['value']:.3e} reads/s on {cb['cores']} threads")
         out["gpu_vs_cpu"] = value / cb["value"]
+        # the reference's own shape of the work (RACES is single-threaded), then on every core
+        packed = built.genome.download()
+        rs1 = cpu_reference_shaped(w, packed, recs, 1, args.cpu_reads / 60)
+        rsn = cpu_reference_shaped(w, packed, recs, threads, args.cpu_reads / 3)
+        out["cpu_baseline_reference_shaped"] = dict(single_thread=rs1, all_cores=rsn,
+                                                    gpu_vs_single_thread=value / rs1["value"],
+                                                    gpu_vs_all_cores=value / rsn["value"])
+        log(f"cpu reference-shaped: {rs1['value']:.3e} reads/s on 1 thread, {rsn['value']:.3e} on {threads}")
     print(json.dumps(out), flush=True)
     return 0
 

@@ -810,6 +810,9 @@ typedef struct seq_ctx {
     uint32_t* rows;
     uint32_t rw;
     uint64_t* seg_base;
+    /* a consumer of every read in place of the counting (orc_seq_walk) */
+    orc_read_fn hook;
+    void* hook_user;
 } seq_ctx;
 
 /* Copy numbers of the allele lineages of `leaf` at (c, x): one copy each,
@@ -1005,7 +1008,7 @@ int orc_simulate_seq(uint32_t n_chr, const uint32_t* len, const orc_forest* f,
 static int seq_run(uint32_t n_chr, const uint32_t* len, const uint32_t* packed, const orc_forest* f,
                    const psx_mutation* m, uint64_t n_mut, uint32_t ploidy, const psx_cna_event* ev, uint32_t n_ev,
                    const psx_seq_params* p, uint64_t seed, int32_t* alt, int32_t* depth, uint8_t* present,
-                   uint32_t* rows, uint64_t capacity, uint64_t* n_reads) {
+                   uint32_t* rows, uint64_t capacity, orc_read_fn hook, void* hook_user, uint64_t* n_reads) {
     if (!(p->coverage >= 0.0) || !isfinite(p->coverage)) return fail("invalid coverage");
     if (p->read_size < 1) return fail("invalid read size");
     if (!(p->purity >= 0.0 && p->purity <= 1.0)) return fail("The purity must be in [0,1]");
@@ -1018,6 +1021,7 @@ static int seq_run(uint32_t n_chr, const uint32_t* len, const uint32_t* packed, 
     s->seed = seed; s->e = p->error_rate;
     s->ev = ev; s->n_ev = n_ev;
     s->packed = packed; s->rows = rows; s->rw = 2u + (s->L + 15u) / 16u;
+    s->hook = hook; s->hook_user = hook_user;
     for (uint32_t c = 0, w = 0; c < n_chr && c < PSX_MAX_CHROMOSOMES; ++c) {
         s->chr_woff[c] = w;
         w += (len[c] + 15u) / 16u;
@@ -1181,14 +1185,16 @@ static int seq_run(uint32_t n_chr, const uint32_t* len, const uint32_t* packed, 
                 }
             }
             uint64_t ri = seg_base[(uint64_t)so * n_chr + c] + (T * b / W + r) * rpt;  /* read index (rows) */
-            seq_read(s, c, x, leaf, allele, eoff[0], esub[0], ne[0], a_out, d_out);
+            if (s->hook) s->hook(s->hook_user, so, c, x, leaf, allele, 0, eoff[0], esub[0], ne[0]);
+            else seq_read(s, c, x, leaf, allele, eoff[0], esub[0], ne[0], a_out, d_out);
             if (rows) read_row(s, c, x, leaf, allele, 0, eoff[0], esub[0], ne[0], rows + ri * s->rw);
             if (s->pe) {
                 uint32_t I = s->ins_k0 + pick(s->ins_T, s->ins_n, w[3]);
                 if (I < s->L) I = s->L;
                 uint64_t x2 = (uint64_t)x + I - s->L;
                 if (x2 > len[c] - s->L) x2 = len[c] - s->L;
-                seq_read(s, c, (uint32_t)x2, leaf, allele, eoff[1], esub[1], ne[1], a_out, d_out);
+                if (s->hook) s->hook(s->hook_user, so, c, (uint32_t)x2, leaf, allele, 1, eoff[1], esub[1], ne[1]);
+                else seq_read(s, c, (uint32_t)x2, leaf, allele, eoff[1], esub[1], ne[1], a_out, d_out);
                 if (rows) read_row(s, c, (uint32_t)x2, leaf, allele, 1, eoff[1], esub[1], ne[1], rows + (ri + 1) * s->rw);
             }
         }
@@ -1203,7 +1209,8 @@ int orc_simulate_seq_cna(uint32_t n_chr, const uint32_t* len, const orc_forest* 
                          const psx_cna_event* ev, uint32_t n_ev,
                          const psx_seq_params* p, uint64_t seed, int32_t* alt, int32_t* depth,
                          uint8_t* present, uint64_t* n_reads) {
-    return seq_run(n_chr, len, NULL, f, m, n_mut, ploidy, ev, n_ev, p, seed, alt, depth, present, NULL, 0, n_reads);
+    return seq_run(n_chr, len, NULL, f, m, n_mut, ploidy, ev, n_ev, p, seed, alt, depth, present, NULL, 0, NULL, NULL,
+                   n_reads);
 }
 
 int orc_simulate_reads(uint32_t n_chr, const uint32_t* len, const uint32_t* packed, const orc_forest* f,
@@ -1215,6 +1222,20 @@ int orc_simulate_reads(uint32_t n_chr, const uint32_t* len, const uint32_t* pack
     int32_t* depth = (int32_t*)malloc(n * sizeof(int32_t));
     uint8_t* present = (uint8_t*)malloc(n);
     int rc = seq_run(n_chr, len, packed, f, m, n_mut, ploidy, NULL, 0, p, seed, alt, depth, present, rows, capacity,
+                     NULL, NULL, n_reads);
+    free(alt); free(depth); free(present);
+    return rc;
+}
+
+int orc_seq_walk(uint32_t n_chr, const uint32_t* len, const orc_forest* f, const psx_mutation* m, uint64_t n_mut,
+                 uint32_t ploidy, const psx_seq_params* p, uint64_t seed, orc_read_fn fn, void* user,
+                 uint64_t* n_reads) {
+    uint32_t n_out = (p->samples ? p->n_samples : f->n_samples) + (p->with_normal_sample ? 1u : 0u);
+    size_t n = (size_t)n_out * (n_mut ? n_mut : 1);
+    int32_t* alt = (int32_t*)malloc(n * sizeof(int32_t));
+    int32_t* depth = (int32_t*)malloc(n * sizeof(int32_t));
+    uint8_t* present = (uint8_t*)malloc(n);
+    int rc = seq_run(n_chr, len, NULL, f, m, n_mut, ploidy, NULL, 0, p, seed, alt, depth, present, NULL, 0, fn, user,
                      n_reads);
     free(alt); free(depth); free(present);
     return rc;

@@ -85,6 +85,25 @@ int orc_simulate_reads(uint32_t n_chr, const uint32_t* len, const uint32_t* pack
                        const psx_mutation* m, uint64_t n_mut, uint32_t ploidy, const psx_seq_params* p,
                        uint64_t seed, uint32_t* rows, uint64_t capacity, uint64_t* n_reads);
 
+/* Every read of simulate_seq (no copy-number events) handed to fn instead of
+ * being counted: output sample, chromosome, start, source leaf (n_leaves for
+ * the normal genome), allele, mate and the read's errors (offsets and
+ * substitution indices). fn runs on the OpenMP threads of the walk. */
+typedef void (*orc_read_fn)(void* user, uint32_t so, uint32_t chr, uint32_t x, uint32_t leaf, uint32_t allele,
+                            uint32_t mate, const uint32_t* off, const uint32_t* sub, uint32_t n_err);
+int orc_seq_walk(uint32_t n_chr, const uint32_t* len, const orc_forest* f, const psx_mutation* m, uint64_t n_mut,
+                 uint32_t ploidy, const psx_seq_params* p, uint64_t seed, orc_read_fn fn, void* user,
+                 uint64_t* n_reads);
+
+/* The reference-shaped CPU restatement (refshape.cpp): the reads of
+ * orc_seq_walk each built as base and quality strings from the reference,
+ * their sources' mutations found in ordered maps and the statistics kept in
+ * ordered maps, as RACES' ReadSimulator does; alt and depth [S][M] as
+ * orc_simulate_seq. threads: OpenMP threads (1: the single-threaded shape). */
+int orc_refshape_seq(uint32_t n_chr, const uint32_t* len, const uint32_t* packed, const orc_forest* f,
+                     const psx_mutation* m, uint64_t n_mut, uint32_t ploidy, const psx_seq_params* p, uint64_t seed,
+                     int threads, int32_t* alt, int32_t* depth, uint64_t* n_reads);
+
 int orc_seq_table(const psx_mutation* recs, uint64_t n_mutations, uint32_t n_out_samples,
                   const int32_t* alt, const int32_t* depth, const uint8_t* present,
                   int32_t include_non_sequenced, uint64_t* n_rows, uint64_t* row_rec,

@@ -54,6 +54,8 @@ def lib():
             "orc_simulate_reads": (st, [C.c_uint32, A.u32p, A.u32p, C.POINTER(OrcForest), vp, C.c_uint64,
                                         C.c_uint32, C.POINTER(A.SeqParams), C.c_uint64, A.u32p, C.c_uint64,
                                         A.u64p]),
+            "orc_refshape_seq": (st, [C.c_uint32, A.u32p, A.u32p, C.POINTER(OrcForest), vp, C.c_uint64, C.c_uint32,
+                                      C.POINTER(A.SeqParams), C.c_uint64, C.c_int, A.i32p, A.i32p, A.u64p]),
             "orc_seq_table": (st, [vp, C.c_uint64, C.c_uint32, A.i32p, A.i32p, A.u8p, C.c_int32, A.u64p, A.u64p,
                                    A.i32p, A.i32p, A.f64p]),
             "orc_philox": (None, [A.u32p, C.c_uint64, A.u32p]),
@@ -217,6 +219,28 @@ def simulate_reads(lens, packed, forest: Forest, recs: np.ndarray, seed: int, ca
                                     A.ptr(rows, C.c_uint32), capacity, C.byref(nr)))
     del keep
     return rows[: nr.value * rw].reshape(nr.value, rw)
+
+
+def refshape_seq(lens, packed, forest: Forest, recs: np.ndarray, seed: int, threads: int = 1, ploidy: int = 2,
+                 **seq):
+    """The reference-shaped restatement (per-read base and quality strings,
+    ordered-map lookups and statistics): (alt, depth, n_reads) equal to
+    simulate_seq's; threads = OpenMP threads."""
+    L = np.ascontiguousarray(lens, np.uint32)
+    P = np.ascontiguousarray(packed, np.uint32)
+    r = np.ascontiguousarray(recs, A.MUTATION_DTYPE)
+    params, keep = A.seq_params(**seq)
+    n_req = len(seq["samples"]) if seq.get("samples") is not None else forest.n_samples
+    S = n_req + (1 if seq.get("with_normal_sample", True) else 0)
+    M = len(r)
+    alt = np.zeros(S * max(M, 1), np.int32)
+    dep = np.zeros(S * max(M, 1), np.int32)
+    nr = C.c_uint64()
+    _check(lib().orc_refshape_seq(len(L), A.ptr(L, C.c_uint32), A.ptr(P, C.c_uint32), forest.p,
+                                  r.ctypes.data_as(C.c_void_p), M, ploidy, C.byref(params), C.c_uint64(seed),
+                                  int(threads), A.ptr(alt, C.c_int32), A.ptr(dep, C.c_int32), C.byref(nr)))
+    del keep
+    return alt[: S * M].reshape(S, M), dep[: S * M].reshape(S, M), nr.value
 
 
 def seq_table(recs, alt, depth, present, include_non_sequenced=False):

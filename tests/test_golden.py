@@ -132,3 +132,15 @@ def test_result_table_hand_fixture(impl):
     assert cov.tolist() == [[10, 7, 5], [8, 0, 4]]
     assert vaf[0].tolist() == [0.3, 0.0, 0.0]
     assert vaf[1, 0] == 0.0 and np.isnan(vaf[1, 1]) and vaf[1, 2] == 0.5
+
+
+@pytest.mark.parametrize("case", READ_CASES)
+@pytest.mark.parametrize("threads", [1, 3])
+def test_reference_shaped_restatement_matches(case, threads):
+    """The reference-shaped CPU restatement (read strings, ordered maps) gives
+    the fixture's counts: the CPU baseline it times does the same work."""
+    d = load(case)
+    of = O.Forest(d["cell_id"], d["parent"], d["species"], d["sample"], d["birth"], int(d["n_samples"]))
+    a, dp, n = O.refshape_seq(d["lens"], d["packed"], of, d["recs"], int(d["seed"]), threads, **d["seq"])
+    assert n == int(d["n_reads"])
+    assert np.array_equal(a, d["alt"]) and np.array_equal(dp, d["depth"])
