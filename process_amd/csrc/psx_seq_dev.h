@@ -218,6 +218,24 @@ __device__ __forceinline__ bool carried(const job_dev& J, uint32_t info, uint32_
     return lo <= leaf && leaf < hi;
 }
 
+// psx_guided_pick with a wave-uniform binary search inside the guide bucket:
+// the answer lies in [guide[g], guide[g + 1]] (the last bucket ends at n - 1),
+// and the tail buckets of a peaked distribution hold tens of entries, which a
+// linear scan would walk for the whole wave. Same result as psx_guided_pick.
+__device__ __forceinline__ uint32_t guided_pick_u(const uint32_t* T32, const uint16_t* guide, uint32_t n, uint32_t u) {
+    const uint32_t g = u >> 24;
+    uint32_t lo = guide[g];
+    uint32_t hi = g < PSX_GUIDE - 1u ? guide[g + 1u] : n - 1u;
+    while (__ballot(lo < hi)) {
+        const uint32_t mid = (lo + hi) >> 1;
+        const bool below = u < T32[mid];
+        const bool on = lo < hi;
+        hi = on && below ? mid : hi;
+        lo = on && !below ? mid + 1u : lo;
+    }
+    return lo;
+}
+
 // Template r of stratum j (oracle: the template loop of orc_simulate_seq).
 template <bool CNA = false>
 __device__ __forceinline__ tmpl make_template(const job_dev& J, const seg_dev& S, const lds_tables& tab, uint32_t r,
@@ -267,7 +285,7 @@ __device__ __forceinline__ tmpl make_template(const job_dev& J, const seg_dev& S
     }
     t.x2 = t.x;
     if (J.pe) {
-        uint32_t I = J.ins_k0 + psx_guided_pick(tab.ins, tab.guide, J.ins_n, w.w);
+        uint32_t I = J.ins_k0 + guided_pick_u(tab.ins, tab.guide, J.ins_n, w.w);
         if (I < J.L) I = J.L;
         uint64_t x2 = (uint64_t)t.x + I - J.L;
         if (x2 > S.len - J.L) x2 = S.len - J.L;
