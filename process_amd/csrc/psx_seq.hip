@@ -140,6 +140,8 @@ void prepare_job(psx_ctx* ctx, seq_batch& P, const job_in& in, uint32_t spi) {
         c_kmax[c] = cna_kmax(ev, f->n_leaves, g->len[c], ploidy);
         if ((uint64_t)f->n_leaves * c_kmax[c] >= (uint64_t(1) << 32))
             psx_throw(PSX_EINVAL, "too many cell copies (%u copies on chromosome %u)", c_kmax[c], c);
+        if (c_kmax[c] > 0xFFFFu)
+            psx_throw(PSX_EINVAL, "copy numbers above 65535 are not supported (%u on chromosome %u)", c_kmax[c], c);
         cdev.insert(cdev.end(), ev.begin(), ev.end());
     }
     P.cna_off.push_back(cdev.empty() ? SIZE_MAX : P.sb.put_vec(cdev));

@@ -205,6 +205,31 @@ __device__ __forceinline__ uint32_t cna_copies(const psx_cna_dev* ev, uint32_t n
     return total;
 }
 
+// cna_copies for ploidy 2: copies of lineage 0 | lineage 1 << 16 (copy
+// numbers stay below 2^16: the host bounds them by the largest copy number).
+__device__ __forceinline__ uint32_t cna_copies2(const psx_cna_dev* ev, uint32_t n, uint32_t leaf, uint32_t x) {
+    uint32_t c0 = 1u, c1 = 1u;
+    for (uint32_t i = 0; i < n; ++i) {
+        const psx_cna_dev e = uniform_at(ev, i);
+        const bool hit = leaf >= e.lo && leaf < e.hi && x >= e.start && x < e.end;
+        const uint32_t a = e.code & 0xFFu, kind = (e.code >> 8) & 0xFFu, c = e.code >> 16;
+        uint32_t n0, n1;
+        if (kind == PSX_CNA_WGD) {  // uniform branches: the event is the same for every lane
+            n0 = 2u * c0;
+            n1 = 2u * c1;
+        } else if (kind == PSX_CNA_AMP) {
+            n0 = a == 0u ? c0 + c : c0;
+            n1 = a == 1u ? c1 + c : c1;
+        } else {
+            n0 = a == 0u ? 0u : c0;
+            n1 = a == 1u ? 0u : c1;
+        }
+        c0 = hit ? n0 : c0;
+        c1 = hit ? n1 : c1;
+    }
+    return c0 | (c1 << 16);
+}
+
 __device__ __forceinline__ uint32_t leaf_of(const job_dev& J, const seg_dev& S, tmpl& t) {
     if (t.leaf == UNRESOLVED) t.leaf = J.sample_leaves[S.leaf_off + t.cell];
     return t.leaf;
@@ -271,16 +296,24 @@ __device__ __forceinline__ tmpl make_template(const job_dev& J, const seg_dev& S
             t.cell = X / K;
             const uint32_t k = X - t.cell * K;
             t.leaf = J.sample_leaves[S.leaf_off + t.cell];
-            uint32_t cn[PSX_MAX_CNA_PLOIDY];
-            const uint32_t C = cna_copies(J.cna + S.cna_off, S.n_cna, t.leaf, t.x, J.ploidy, cn);
-            uint32_t cum = 0, al = 0;
+            if (J.ploidy == 2u) {
+                // diploid: the two lineages' copy numbers in two registers
+                const uint32_t C0 = cna_copies2(J.cna + S.cna_off, S.n_cna, t.leaf, t.x);
+                const uint32_t c0 = C0 & 0xFFFFu, c1 = C0 >> 16;
+                t.allele = k >= c0 ? 1u : 0u;
+                t.ok = k < c0 + c1;
+            } else {
+                uint32_t cn[PSX_MAX_CNA_PLOIDY];
+                const uint32_t C = cna_copies(J.cna + S.cna_off, S.n_cna, t.leaf, t.x, J.ploidy, cn);
+                uint32_t cum = 0, al = 0;
 #pragma unroll
-            for (uint32_t a = 0; a < PSX_MAX_CNA_PLOIDY; ++a) {
-                cum += cn[a];
-                al += (a + 1 < J.ploidy && k >= cum) ? 1u : 0u;
+                for (uint32_t a = 0; a < PSX_MAX_CNA_PLOIDY; ++a) {
+                    cum += cn[a];
+                    al += (a + 1 < J.ploidy && k >= cum) ? 1u : 0u;
+                }
+                t.allele = al;
+                t.ok = k < C;
             }
-            t.allele = al;
-            t.ok = k < C;
         }
     }
     t.x2 = t.x;
@@ -312,13 +345,12 @@ struct terr {
 __device__ __forceinline__ terr template_errors(const job_dev& J, const lds_tables& tab, uint32_t r, uint32_t j,
                                                 uint32_t c2, bool want) {
     const psx_u4 w = psx_draw_u(J.seed, r, j, c2, PSX_P_ERROR << 24);
-    // K - err_k0 = the first k with u < T[k] (capped at err_n - 1); K is small
+    // K = err_k0 + the number of thresholds k < err_n - 1 with u >= T[k] (T is
+    // non-decreasing); only K <= 3 matters here, so four compares against the
+    // first thresholds (read by every lane alike) give min(K, 4) without a loop
     uint32_t k = 0;
-    bool more = want && 1u < J.err_n && w.y >= tab.err[0];
-    while (__ballot(more)) {
-        k += more ? 1u : 0u;
-        more = more && k + 1u < J.err_n && w.y >= tab.err[min(k, J.err_n - 1u)];
-    }
+#pragma unroll
+    for (uint32_t q = 0; q < 4u; ++q) k += (q + 1u < J.err_n && w.y >= tab.err[min(q, J.err_n - 1u)]) ? 1u : 0u;
     const uint32_t K = want ? J.err_k0 + k : 0u;
     const uint32_t b0 = psx_mulhi32(w.x, J.err_N), b1 = psx_mulhi32(w.z, J.err_N), b2 = psx_mulhi32(w.w, J.err_N);
     const bool slow = K > 3u || (K >= 2u && b1 == b0) || (K >= 3u && (b2 == b0 || b2 == b1));
