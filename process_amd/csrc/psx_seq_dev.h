@@ -653,42 +653,39 @@ __device__ __forceinline__ void check_errors(const job_dev& J, const lds_view& v
     wave_sync();  // the half is free again
 }
 
-// One mate of 64 templates of a wave (one per lane): every lane finds the
-// positions its read covers and queues (lane, first record of the position)
-// in the wave's LDS queue; the queue is drained 64 entries at a time, one per
-// lane, so the per-position work runs on full waves instead of waiting for
-// the lane whose read covers the most positions. Sequencing errors: with
-// DEFER the reads are counted with their true bases and each drained
-// position whose observed base matters goes to the deferred checks (template
-// base + lane); otherwise the template summaries e give the observed bases
-// (slow templates are corrected afterwards). Lanes with valid == false take
-// no part (x must still lie in the stratum).
+// Both mates of 64 templates of a wave (one per lane): every lane finds the
+// positions its reads cover and queues (lane, mate, first record of the
+// position) in the wave's LDS queue, mate 1 after mate 0; the queue is
+// drained 64 entries at a time, one per lane, so the per-position work runs on
+// full waves instead of waiting for the lane whose reads cover the most
+// positions, and the two mates share the last, partial drain. Sequencing
+// errors: with DEFER the reads are counted with their true bases and each
+// drained position whose observed base matters goes to the deferred checks
+// (template base + lane); otherwise the template summaries e give the
+// observed bases (slow templates are corrected afterwards). Lanes with valid
+// == false take no part (their starts must still lie in the stratum).
 template <bool DEFER>
-__device__ __forceinline__ void count_mate_wave(const job_dev& J, const seg_dev& S, const lds_view& v,
-                                                const lds_tables& tab, tmpl& t, const terr& e, bool valid,
-                                                uint32_t x, uint32_t mate, uint32_t base, uint32_t j, uint32_t c2,
-                                                uint32_t* q, chk_ring& ck, uint32_t lane) {
-    const uint32_t end = x + J.L;
-    uint32_t i = v.first_u(x);
-    bool go = valid && i < v.n && v.pos[v.at(i)] < end;
-    if (!__ballot(go)) return;
-    uint32_t leaf = 0;
-    if (go) leaf = leaf_of(J, S, t);
+__device__ __forceinline__ void count_mates_wave(const job_dev& J, const seg_dev& S, const lds_view& v,
+                                                 const lds_tables& tab, tmpl& t, const terr& e, bool valid,
+                                                 uint32_t base, uint32_t j, uint32_t c2, uint32_t* q, chk_ring& ck,
+                                                 uint32_t lane) {
+    uint32_t leaf = 0;  // the lane's source leaf, resolved once a read of it covers a record
+    // allele | (x2 - x) << 8, for the drains (x2 - x < 2^13)
+    const uint32_t ad = t.allele | ((t.x2 - t.x) << 8);
     // one drain: entries qb[0 .. count), one per lane
     auto drain = [&](const uint32_t* qb, uint32_t count) {
         const bool on = lane < count;
         const uint32_t qe = on ? qb[lane] : 0u;
-        const uint32_t ol = qe >> 26, ri = qe & 0x3FFFFFFu;
+        const uint32_t ol = qe >> 26, mate = (qe >> 25) & 1u, ri = qe & 0x1FFFFFFu;
         const uint32_t lo = (uint32_t)__shfl((int)leaf, (int)ol);
-        const uint32_t al = (uint32_t)__shfl((int)t.allele, (int)ol);
+        const uint32_t ado = (uint32_t)__shfl((int)ad, (int)ol);
+        const uint32_t al = ado & 0xFFu;
+        const uint32_t xo = (uint32_t)__shfl((int)t.x, (int)ol) + (mate ? ado >> 8 : 0u);  // the read's start
         const uint32_t in = v.info[ri], p = v.pos[ri];
         const bool grp = !PSX_INFO_SOLO(in);
         terr eo{0u, 0u};
-        uint32_t boff = 0u;
-        if (!DEFER) {
-            eo = shfl_terr(e, ol);
-            boff = mate * J.L - (uint32_t)__shfl((int)x, (int)ol);
-        }
+        const uint32_t boff = mate * J.L - xo;  // base index in the template of position p: p + boff
+        if (!DEFER) eo = shfl_terr(e, ol);
         if (__ballot(on && grp)) {
             if (on && grp) count_group(J, v, ri, al, lo, eo, boff);
         }
@@ -705,36 +702,44 @@ __device__ __forceinline__ void count_mate_wave(const job_dev& J, const seg_dev&
             const bool chk = on && (grp || (snv && PSX_INFO_HEAD(in)));
             const unsigned long long m = __ballot(chk);
             if (m) {
-                const uint32_t xo = (uint32_t)__shfl((int)x, (int)ol);
                 const uint32_t rank =
                     __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
                 if (chk) {
                     uint32_t* slot = ck.e + ((ck.head + ck.count + rank) & 127u) * 3u;
                     slot[0] = base + ol;
-                    slot[1] = (mate * J.L + p - xo) | (al << 13) | (ri << 21);
+                    slot[1] = (p + boff) | (al << 13) | (ri << 21);
                     slot[2] = lo;
                 }
                 ck.count += (uint32_t)__popcll(m);
             }
         }
     };
-    const uint32_t i0 = i;
     uint32_t qn = 0;  // wave-uniform
-    while (__ballot(go)) {
-        const uint32_t ic = go ? i : 0u;
-        const bool starts = go && (i == i0 || v.pos[ic] != v.pos[ic ? ic - 1u : 0u]);
-        const unsigned long long m = __ballot(starts);
-        const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-        if (starts) q[qn + below] = (lane << 26) | ic;
-        qn += (uint32_t)__popcll(m);
-        i += go ? 1u : 0u;
-        go = go && i < v.n && v.pos[v.at(i)] < end;
-        if (qn >= 64u) {
-            wave_sync();
-            drain(q + qn - 64u, 64u);
-            qn -= 64u;
-            wave_sync();
-            if (DEFER && ck.count >= 64u) check_errors(J, v, tab, j, c2, ck, 64u, lane);
+    for (uint32_t mate = 0; mate <= J.pe; ++mate) {
+        const uint32_t x = mate ? t.x2 : t.x;
+        const uint32_t end = x + J.L;
+        uint32_t i = v.first_u(x);
+        bool go = valid && i < v.n && v.pos[v.at(i)] < end;
+        if (!__ballot(go)) continue;
+        if (go) leaf = leaf_of(J, S, t);
+        const uint32_t i0 = i;
+        while (__ballot(go)) {
+            const uint32_t ic = go ? i : 0u;
+            const bool starts = go && (i == i0 || v.pos[ic] != v.pos[ic ? ic - 1u : 0u]);
+            const unsigned long long m = __ballot(starts);
+            const uint32_t below =
+                __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+            if (starts) q[qn + below] = (lane << 26) | (mate << 25) | ic;
+            qn += (uint32_t)__popcll(m);
+            i += go ? 1u : 0u;
+            go = go && i < v.n && v.pos[v.at(i)] < end;
+            if (qn >= 64u) {
+                wave_sync();
+                drain(q + qn - 64u, 64u);
+                qn -= 64u;
+                wave_sync();
+                if (DEFER && ck.count >= 64u) check_errors(J, v, tab, j, c2, ck, 64u, lane);
+            }
         }
     }
     if (qn) {
@@ -869,9 +874,8 @@ __global__ void __launch_bounds__(BLOCK, DEFER ? 6 : 1) k_seq_count(const seq_la
                             valid = valid && t.ok;
                             acc += valid ? 1u : 0u;
                         }
-                        for (uint32_t mate = 0; mate <= J.pe; ++mate)
-                            count_mate_wave<true>(J, S, v, s_tab, t, terr{0u, 0u}, valid, mate ? t.x2 : t.x, mate,
-                                                  base, j, c2, s_wave[wave], ck, lane);
+                        count_mates_wave<true>(J, S, v, s_tab, t, terr{0u, 0u}, valid, base, j, c2, s_wave[wave], ck,
+                                               lane);
                     }
                     if (J.err_n && ck.count) check_errors(J, v, s_tab, j, c2, ck, ck.count, lane);
                 } else {
@@ -885,9 +889,7 @@ __global__ void __launch_bounds__(BLOCK, DEFER ? 6 : 1) k_seq_count(const seq_la
                         }
                         const terr e =
                             J.err_n ? template_errors(J, s_tab, valid ? r : base, j, c2, valid) : terr{0u, 0u};
-                        for (uint32_t mate = 0; mate <= J.pe; ++mate)
-                            count_mate_wave<false>(J, S, v, s_tab, t, e, valid, mate ? t.x2 : t.x, mate, base, j, c2,
-                                                   s_wave[wave], ck, lane);
+                        count_mates_wave<false>(J, S, v, s_tab, t, e, valid, base, j, c2, s_wave[wave], ck, lane);
                         if (J.err_n) count_slow_errors(J, v, s_tab, t, e, valid, base, j, c2, s_wave[wave], lane);
                     }
                 }
