@@ -175,8 +175,8 @@ struct glb_view {  // window larger than the LDS capacity
     __device__ __forceinline__ void add_alt_n(uint32_t q, int d) const { atomicAdd(&alt[q], d); }
 };
 
-// Template: start of mate 1 and mate 2, source cell (resolved to a leaf on
-// first use) and copy; ok == 0 when a copy-number draw rejects it.
+// Template: start of mate 1 and mate 2, source cell and its leaf, and copy;
+// ok == 0 when a copy-number draw rejects it.
 struct tmpl {
     uint32_t x, x2, cell, allele, leaf, ok;
 };
@@ -279,7 +279,7 @@ __device__ __forceinline__ tmpl make_template(const job_dev& J, const seg_dev& S
             const uint32_t X = psx_uniform32(w.z, S.n_cells * J.ploidy);
             if (J.ploidy == 2) { t.cell = X >> 1; t.allele = X & 1u; }
             else { t.cell = X / J.ploidy; t.allele = X - t.cell * J.ploidy; }
-            t.leaf = UNRESOLVED;
+            t.leaf = J.sample_leaves[S.leaf_off + t.cell];  // issued early: its latency overlaps the draws
         }
     } else {
         // one of kmax copy slots per cell; the template is kept when the slot
