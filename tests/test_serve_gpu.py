@@ -140,3 +140,44 @@ def test_arrivals_while_running(ctx, pool):
         assert srv.totals()["completed"] == 10
     finally:
         srv.close()
+
+
+def test_copy_number_requests_in_the_mix(ctx, pool):
+    """Requests on a table with copy-number events (amplification, deletion,
+    whole-genome doubling, purity < 1) served together with plain ones: each
+    result, and its count of accepted reads, equals the request run alone,
+    and the oracle reproduces one of them."""
+    from process_amd import _abi as A
+
+    g, items = pool
+    packed = O.genome_synth(LENS, 41)
+    fd = core.synth_forest(growth=[0.2, 0.45], death=[0.01, 0.01], intro_at=[0, 20], n_cells=600, n_samples=2,
+                           sample_size=90, seed=17)
+    f = core.Forest(ctx, fd["cell_id"], fd["parent"], fd["species"], fd["sample"], fd["birth"], fd["n_samples"])
+    eng = cases.engine(packed, LENS, snv=5e-6, indel=5e-7, pre_snv=10, pre_indel=2)
+    eng["cna"] = [(0, 0, 4_000, 20_000, 0, A.PSX_CNA_AMP, 2), (0, 1, 2_000, 9_000, 1, A.PSX_CNA_DEL, 0),
+                  (1, 0, 0, 0, 0, A.PSX_CNA_WGD, 0), (1, 0, 30_000, 12_000, 1, A.PSX_CNA_AMP, 1)]
+    m = core.Mutations.place(ctx, g, f, 77, **eng)
+    assert len(m.cna()) > 0
+    cna_seq = [dict(coverage=30, read_size=150, insert_size_mean=350, insert_size_stddev=10, error_rate=1e-3,
+                    purity=0.6),
+               dict(coverage=20, read_size=100, error_rate=2e-3)]
+    reqs = [(f, m, 300 + k, cna_seq[k % 2]) for k in range(4)]
+    reqs += [(items[k % 3][0], items[k % 3][1], 400 + k, dict(coverage=15, read_size=100, error_rate=1e-3))
+             for k in range(4)]
+    say(f"[serve] {len(reqs)} requests, 4 with copy-number events, alone")
+    alone = [core.simulate_seq_arrays(ctx, g, rf, rm, seed, **seq) for rf, rm, seed, seq in reqs]
+    of = O.Forest(fd["cell_id"], fd["parent"], fd["species"], fd["sample"], fd["birth"], fd["n_samples"])
+    oa, od, op, on = O.simulate_seq(LENS, of, m.download(), 300, cna=m.cna(), **cna_seq[0])
+    assert np.array_equal(alone[0][0], oa) and np.array_equal(alone[0][1], od) and alone[0][3] == on
+    srv = core.Server(ctx, 64 << 20, max_inflight=2, max_batch=8)
+    try:
+        tickets = [srv.submit(g, rf, rm, seed, **seq) for rf, rm, seed, seq in reqs]
+        for k, t in enumerate(tickets):
+            a, d, p, st = srv.wait(t, timeout_ms=60_000)
+            assert np.array_equal(a, alone[k][0]) and np.array_equal(d, alone[k][1]), k
+            assert np.array_equal(p, alone[k][2]), k
+            assert st["n_reads"] == alone[k][3], k
+        assert srv.totals()["completed"] == len(reqs)
+    finally:
+        srv.close()
