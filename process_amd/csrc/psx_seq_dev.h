@@ -653,12 +653,12 @@ __device__ __forceinline__ void check_errors(const job_dev& J, const lds_view& v
     wave_sync();  // the half is free again
 }
 
-// Both mates of 64 templates of a wave (one per lane): every lane finds the
-// positions its reads cover and queues (lane, mate, first record of the
-// position) in the wave's LDS queue, mate 1 after mate 0; the queue is
-// drained 64 entries at a time, one per lane, so the per-position work runs on
-// full waves instead of waiting for the lane whose reads cover the most
-// positions, and the two mates share the last, partial drain. Sequencing
+// Both mates of 64 templates of a wave (one per lane): every lane walks the
+// positions its reads cover (mate 0's, then mate 1's, in one loop) and queues
+// (lane, mate, first record of the position) in the wave's LDS queue; the
+// queue is drained 64 entries at a time, one per lane, so the per-position
+// work runs on full waves instead of waiting for the lane whose reads cover
+// the most positions. Sequencing
 // errors: with DEFER the reads are counted with their true bases and each
 // drained position whose observed base matters goes to the deferred checks
 // (template base + lane); otherwise the template summaries e give the
@@ -715,31 +715,39 @@ __device__ __forceinline__ void count_mates_wave(const job_dev& J, const seg_dev
         }
     };
     uint32_t qn = 0;  // wave-uniform
-    for (uint32_t mate = 0; mate <= J.pe; ++mate) {
-        const uint32_t x = mate ? t.x2 : t.x;
-        const uint32_t end = x + J.L;
-        uint32_t i = v.first_u(x);
-        bool go = valid && i < v.n && v.pos[v.at(i)] < end;
-        if (!__ballot(go)) continue;
-        if (go) leaf = leaf_of(J, S, t);
-        const uint32_t i0 = i;
-        while (__ballot(go)) {
-            const uint32_t ic = go ? i : 0u;
-            const bool starts = go && (i == i0 || v.pos[ic] != v.pos[ic ? ic - 1u : 0u]);
-            const unsigned long long m = __ballot(starts);
-            const uint32_t below =
-                __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-            if (starts) q[qn + below] = (lane << 26) | (mate << 25) | ic;
-            qn += (uint32_t)__popcll(m);
-            i += go ? 1u : 0u;
-            go = go && i < v.n && v.pos[v.at(i)] < end;
-            if (qn >= 64u) {
-                wave_sync();
-                drain(q + qn - 64u, 64u);
-                qn -= 64u;
-                wave_sync();
-                if (DEFER && ck.count >= 64u) check_errors(J, v, tab, j, c2, ck, 64u, lane);
-            }
+    // one walk over both reads of the lane's template: mate 0's positions, then mate 1's
+    const uint32_t e0 = t.x + J.L, e1 = t.x2 + J.L;
+    uint32_t i = v.first_u(t.x);
+    const uint32_t j1 = J.pe ? v.first_u(t.x2) : 0u;
+    bool go0 = valid && i < v.n && v.pos[v.at(i)] < e0;
+    const bool go1 = valid && J.pe && j1 < v.n && v.pos[v.at(j1)] < e1;
+    if (!__ballot(go0 || go1)) return;
+    if (go0 || go1) leaf = leaf_of(J, S, t);
+    uint32_t mate = go0 ? 0u : 1u, end = go0 ? e0 : e1, i0 = go0 ? i : j1;
+    i = i0;
+    bool go = go0 || go1;
+    while (__ballot(go)) {
+        const uint32_t ic = go ? i : 0u;
+        const bool starts = go && (i == i0 || v.pos[ic] != v.pos[ic ? ic - 1u : 0u]);
+        const unsigned long long m = __ballot(starts);
+        const uint32_t below =
+            __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+        if (starts) q[qn + below] = (lane << 26) | (mate << 25) | ic;
+        qn += (uint32_t)__popcll(m);
+        i += go ? 1u : 0u;
+        go = go && i < v.n && v.pos[v.at(i)] < end;
+        if (!go && mate == 0u && go1) {  // mate 0 done: continue with mate 1
+            mate = 1u;
+            end = e1;
+            i = i0 = j1;
+            go = true;
+        }
+        if (qn >= 64u) {
+            wave_sync();
+            drain(q + qn - 64u, 64u);
+            qn -= 64u;
+            wave_sync();
+            if (DEFER && ck.count >= 64u) check_errors(J, v, tab, j, c2, ck, 64u, lane);
         }
     }
     if (qn) {
