@@ -31,8 +31,8 @@ HBM_PEAK_GBS = 8000.0
 # WRITE_SIZE runs calibrated against a device copy, and their issue-slot use from SQ counters (the rocprofv3
 # summaries under profiles/, named for the kernel and workload). Count-only sequencing is bound by instruction
 # issue, not HBM.
-PROFILED = {"k_seq_count": dict(traffic=302.1e6, valu_busy=0.78, salu_busy=0.79),
-            "k_seq_reads": dict(traffic=77.66e9, valu_busy=0.68, salu_busy=0.65)}
+PROFILED = {"k_seq_count": dict(traffic=302.2e6, valu_busy=0.87, salu_busy=0.76),
+            "k_seq_reads": dict(traffic=77.71e9, valu_busy=0.69, salu_busy=0.60)}
 
 
 def log(msg: str) -> None:

@@ -51,7 +51,7 @@ struct job_dev {
     uint8_t* present;
     const uint32_t* ins_T;       // compact insert-size thresholds
     const uint16_t* ins_guide;
-    const uint32_t* err_T;       // compact thresholds of the errors per error chunk
+    const uint32_t* err_T;       // compact thresholds of the errors per template
     const uint32_t* packed;      // reference (read materialisation)
     uint64_t n_mut;
     uint64_t thr_normal;         // floor((1 - purity) 2^32)
@@ -658,12 +658,12 @@ __device__ __forceinline__ void check_errors(const job_dev& J, const lds_view& v
 // (lane, mate, first record of the position) in the wave's LDS queue; the
 // queue is drained 64 entries at a time, one per lane, so the per-position
 // work runs on full waves instead of waiting for the lane whose reads cover
-// the most positions. Sequencing
-// errors: with DEFER the reads are counted with their true bases and each
-// drained position whose observed base matters goes to the deferred checks
-// (template base + lane); otherwise the template summaries e give the
-// observed bases (slow templates are corrected afterwards). Lanes with valid
-// == false take no part (their starts must still lie in the stratum).
+// the most positions. Sequencing errors: with DEFER the reads are counted
+// with their true bases and each drained position whose observed base
+// matters goes to the deferred checks (template base + lane); otherwise the
+// template summaries e give the observed bases (slow templates are corrected
+// afterwards). Lanes with valid == false take no part (their starts must
+// still lie in the stratum).
 template <bool DEFER>
 __device__ __forceinline__ void count_mates_wave(const job_dev& J, const seg_dev& S, const lds_view& v,
                                                  const lds_tables& tab, tmpl& t, const terr& e, bool valid,

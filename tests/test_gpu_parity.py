@@ -314,6 +314,10 @@ CNA_SEQ = {
     "se_pure": dict(coverage=25, read_size=100, error_rate=5e-3, with_normal_sample=False),
     "subset_normal": dict(coverage=20, read_size=120, samples=[1], chromosomes=[0], purity=0.8,
                           preneoplastic_in_normal=True),
+    # short reads cover fewer than 48 positions per 64 templates here: errors checked per covered position
+    "se_sparse": dict(coverage=40, read_size=30, error_rate=2e-2),
+    "pe_sparse": dict(coverage=30, read_size=36, insert_size_mean=100, insert_size_stddev=8, error_rate=1e-2,
+                      purity=0.7),
 }
 
 
