@@ -146,6 +146,8 @@ struct lds_view {
         }
         return i;
     }
+    // a record index at or before the first record at or after x
+    __device__ __forceinline__ uint32_t start(uint32_t x) const { return bucket[(x - b) >> NB_SHIFT]; }
     __device__ __forceinline__ uint32_t at(uint32_t i) const { return min(i, n - 1u); }  // n > 0
     __device__ __forceinline__ void add_dep(uint32_t q) const { atomicAdd(&dep[q], 1u); }
     __device__ __forceinline__ void add_alt(uint32_t q) const { atomicAdd(&alt[q], 1u); }
@@ -169,6 +171,7 @@ struct glb_view {  // window larger than the LDS capacity
         return a;
     }
     __device__ __forceinline__ uint32_t first_u(uint32_t x) const { return first(x); }
+    __device__ __forceinline__ uint32_t start(uint32_t x) const { return first(x); }
     __device__ __forceinline__ uint32_t at(uint32_t i) const { return min(i, n - 1u); }  // n > 0
     __device__ __forceinline__ void add_dep(uint32_t q) const { atomicAdd(&dep[q], 1); }
     __device__ __forceinline__ void add_alt(uint32_t q) const { atomicAdd(&alt[q], 1); }
@@ -1020,24 +1023,28 @@ __device__ __forceinline__ void write_read(const job_dev& J, const V& v, uint32_
         }
     }
     // SNVs at [x, x + L) in table order; the first carried SNV at a position
-    // sets its base. Uniform trip count, branch-free read-modify-writes of the
-    // lane's own row (an unchanged word when nothing applies).
+    // sets its base. One loop from the lane's bucket start (records before x
+    // are passed over), uniform trip count, branch-free read-modify-writes of
+    // the lane's own row (an unchanged word when nothing applies).
     const uint32_t end = x + J.L;
-    uint32_t i = v.first_u(x);
-    bool go = i < v.n && v.pos[v.at(i)] < end;
+    uint32_t i = v.start(x);
+    uint32_t p = v.n ? v.pos[v.at(i)] : end;  // at() needs n > 0
+    bool go = i < v.n && p < end;
     uint32_t last = 0xFFFFFFFFu;
     while (__ballot(go)) {
         const uint32_t ic = go ? i : 0u;
-        const uint32_t p = v.pos[ic], in = v.info[ic];
-        const bool apply = go && p != last && PSX_INFO_KIND(in) == PSX_SNV && PSX_INFO_ALLELE(in) == allele &&
-                           carried(J, in, v.lo[ic], v.hi[ic], leaf);
+        const uint32_t in = v.info[ic], lo = v.lo[ic], hi = v.hi[ic];
+        i += go ? 1u : 0u;
+        const uint32_t pn = v.pos[v.at(i)];
+        const bool apply = go && p >= x && p != last && PSX_INFO_KIND(in) == PSX_SNV &&
+                           PSX_INFO_ALLELE(in) == allele && carried(J, in, lo, hi, leaf);
         if (apply) {  // rare: only applying lanes touch their row (no bank conflicts from the others)
             const uint32_t o = p - x, s2 = (o & 15u) * 2u;
             row[2 + (o >> 4)] = (row[2 + (o >> 4)] & ~(3u << s2)) | (PSX_INFO_ALT(in) << s2);
         }
         last = apply ? p : last;
-        i += go ? 1u : 0u;
-        go = go && i < v.n && v.pos[v.at(i)] < end;
+        go = go && i < v.n && pn < end;
+        p = pn;
     }
     // the summary's errors on this mate (wave-uniform trip count, predicated)
     const uint32_t ne = e.n();
@@ -1078,6 +1085,8 @@ __device__ __forceinline__ void apply_error(uint32_t* row, uint32_t o, uint32_t 
         row[1] |= 0x80000000u;
     }
 }
+
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 // STAGE: reads are built in LDS and copied out per pass (read_words small
 // enough for 64 rows per wave); otherwise each lane builds its read in place
@@ -1191,8 +1200,11 @@ __global__ void __launch_bounds__(BLOCK) k_seq_reads(const seq_launch L) {
                         const uint32_t words = nt * rpt * RW;
                         uint32_t* dst = L.reads_out + first * RW;
                         if (vec4) {
-                            uint4* d4 = reinterpret_cast<uint4*>(dst);
-                            for (uint32_t q = lane; q < (words >> 2); q += 64u) d4[q] = stage4[q];
+                            // streaming stores: the rows are not read again by this kernel
+                            u32x4* d4 = reinterpret_cast<u32x4*>(dst);
+                            const u32x4* s4 = reinterpret_cast<const u32x4*>(stage4);
+                            for (uint32_t q = lane; q < (words >> 2); q += 64u)
+                                __builtin_nontemporal_store(s4[q], d4 + q);
                         } else {
                             for (uint32_t q = lane; q < words; q += 64u) dst[q] = stage[q];
                         }
