@@ -8,7 +8,7 @@
 // one segment; persistent blocks pull items from a device counter.
 //
 // Per stratum a block stages in LDS the mutation records the stratum's reads
-// can reach (pos, leaf range, packed info) with a 64-base bucket index, plus
+// can reach (pos, leaf range, packed info) with a 32-base bucket index, plus
 // the job's inversion tables; reads are generated from counters and looked up
 // in LDS, counts go to LDS counters and the block flushes them with one global
 // atomic per touched record. All counting is integer.
@@ -18,9 +18,9 @@
 
 namespace psxk {
 
-constexpr uint32_t NB_SHIFT = 6;      // 64-base LDS buckets
+constexpr uint32_t NB_SHIFT = 5;      // 32-base LDS buckets
 constexpr uint32_t MAX_READ = 4096;   // read size and insert reach limit
-constexpr uint32_t NB_MAX = (PSX_STRATUM + MAX_READ) / 64 + 2;
+constexpr uint32_t NB_MAX = ((PSX_STRATUM + MAX_READ) >> NB_SHIFT) + 2;
 constexpr uint32_t MAX_INS = 640;     // compact insert-size thresholds in LDS
 constexpr uint32_t MAX_ERRT = PSX_ERR_KCAP + 1;
 constexpr uint32_t REF_MAX = (PSX_STRATUM + MAX_READ) / 16 + 4;
@@ -133,17 +133,6 @@ struct lds_view {
     __device__ __forceinline__ uint32_t first(uint32_t x) const {
         uint32_t i = bucket[(x - b) >> NB_SHIFT];
         while (i < n && pos[i] < x) ++i;
-        return i;
-    }
-    // Same result; the wave steps together (uniform trip count, no exec
-    // masking), every load in bounds.
-    __device__ __forceinline__ uint32_t first_u(uint32_t x) const {
-        uint32_t i = bucket[(x - b) >> NB_SHIFT];
-        bool go = i < n && pos[min(i, n - 1u)] < x;
-        while (__ballot(go)) {
-            i += go ? 1u : 0u;
-            go = go && i < n && pos[min(i, n - 1u)] < x;
-        }
         return i;
     }
     // a record index at or before the first record at or after x
@@ -718,31 +707,43 @@ __device__ __forceinline__ void count_mates_wave(const job_dev& J, const seg_dev
         }
     };
     uint32_t qn = 0;  // wave-uniform
-    // one walk over both reads of the lane's template: mate 0's positions, then mate 1's
+    // one walk over both reads of the lane's template: mate 0's positions, then
+    // mate 1's, each from its bucket start (records before the read are passed
+    // over); p is the position of record i, prev that of the record before it
     const uint32_t e0 = t.x + J.L, e1 = t.x2 + J.L;
-    uint32_t i = v.first_u(t.x);
-    const uint32_t j1 = J.pe ? v.first_u(t.x2) : 0u;
-    bool go0 = valid && i < v.n && v.pos[v.at(i)] < e0;
-    const bool go1 = valid && J.pe && j1 < v.n && v.pos[v.at(j1)] < e1;
+    if (v.n == 0u) return;  // at() needs n > 0
+    uint32_t i = v.start(t.x);
+    const uint32_t j1 = J.pe ? v.start(t.x2) : 0u;
+    uint32_t p = v.pos[v.at(i)];
+    const uint32_t p1 = v.pos[v.at(j1)];
+    const bool go0 = valid && i < v.n && p < e0;
+    const bool go1 = valid && J.pe && j1 < v.n && p1 < e1;
     if (!__ballot(go0 || go1)) return;
     if (go0 || go1) leaf = leaf_of(J, S, t);
-    uint32_t mate = go0 ? 0u : 1u, end = go0 ? e0 : e1, i0 = go0 ? i : j1;
-    i = i0;
+    uint32_t mate = go0 ? 0u : 1u, end = go0 ? e0 : e1, x0 = go0 ? t.x : t.x2, prev = 0xFFFFFFFFu;
+    i = go0 ? i : j1;
+    p = go0 ? p : p1;
     bool go = go0 || go1;
     while (__ballot(go)) {
         const uint32_t ic = go ? i : 0u;
-        const bool starts = go && (i == i0 || v.pos[ic] != v.pos[ic ? ic - 1u : 0u]);
+        const bool starts = go && p >= x0 && p != prev;
         const unsigned long long m = __ballot(starts);
         const uint32_t below =
             __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
         if (starts) q[qn + below] = (lane << 26) | (mate << 25) | ic;
         qn += (uint32_t)__popcll(m);
         i += go ? 1u : 0u;
-        go = go && i < v.n && v.pos[v.at(i)] < end;
+        const uint32_t pn = v.pos[v.at(i)];
+        prev = p;
+        go = go && i < v.n && pn < end;
+        p = pn;
         if (!go && mate == 0u && go1) {  // mate 0 done: continue with mate 1
             mate = 1u;
             end = e1;
-            i = i0 = j1;
+            x0 = t.x2;
+            i = j1;
+            p = p1;
+            prev = 0xFFFFFFFFu;
             go = true;
         }
         if (qn >= 64u) {
