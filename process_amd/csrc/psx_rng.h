@@ -58,13 +58,23 @@ PSX_HD uint64_t psx_mulhi64(uint64_t a, uint64_t b) {
 #endif
 }
 
+// a ^ b ^ c (one v_bitop3_b32 on gfx950; truth table 0x96 is the three-input xor)
+PSX_HD uint32_t psx_xor3(uint32_t a, uint32_t b, uint32_t c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+#else
+    return a ^ b ^ c;
+#endif
+}
+
 PSX_HD psx_u4 psx_philox(psx_u4 c, uint32_t k0, uint32_t k1) {
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
         // one 32x32->64 multiply each (a single v_mad_u64_u32 on gfx950)
         const uint64_t p0 = (uint64_t)0xD2511F53u * c.x;
         const uint64_t p1 = (uint64_t)0xCD9E8D57u * c.z;
-        c = psx_u4{(uint32_t)(p1 >> 32) ^ c.y ^ k0, (uint32_t)p1, (uint32_t)(p0 >> 32) ^ c.w ^ k1, (uint32_t)p0};
+        c = psx_u4{psx_xor3((uint32_t)(p1 >> 32), c.y, k0), (uint32_t)p1, psx_xor3((uint32_t)(p0 >> 32), c.w, k1),
+                   (uint32_t)p0};
         k0 += 0x9E3779B9u;
         k1 += 0xBB67AE85u;
     }
