@@ -482,10 +482,22 @@ extern "C" psx_status psx_simulate_seq_reads(psx_ctx* ctx, const psx_genome* g, 
         if (P.n_reads > capacity_reads)
             psx_throw(PSX_EINVAL, "the read buffer holds %llu reads, the run simulates %llu",
                       (unsigned long long)capacity_reads, (unsigned long long)P.n_reads);
+        // presence flags per (output sample, mutation): the read kernel stages
+        // only the SNVs a read of the sample can carry
+        dbuf d_present;
+        d_present.alloc(P.pres_total);
+        job_dev* hj = (job_dev*)(P.sb.host.data() + P.o_jobs);  // the uploaded descriptors
+        for (size_t k = 0; k < P.jobs.size(); ++k) hj[k].present = d_present.as<uint8_t>();
         const seq_lane lane = ctx_lane(ctx, batch_desc_bytes(&P));
         batch_upload(&P, lane);
         seq_launch L = launch_desc(P, lane.desc, lane.queue);
         PSX_HIP(hipMemsetAsync(lane.queue, 0, 8, ctx->stream));
+        if (P.pres_total) {
+            const pres_dev* ps = (const pres_dev*)(lane.desc + P.o_pres);
+            const uint32_t pgrid = (uint32_t)((P.pres_total + 255) / 256);
+            hipLaunchKernelGGL(k_reads_present, dim3(pgrid), dim3(256), 0, ctx->stream, L.jobs, ps,
+                               (uint32_t)P.pres.size(), P.pres_total);
+        }
         L.reads_out = d_reads;
         L.read_words = psx_read_words(p);
         L.out_vec4 = (L.read_words % 4 == 0 && ((uintptr_t)d_reads & 15) == 0) ? 1u : 0u;

@@ -809,6 +809,61 @@ __device__ __forceinline__ void stage_window(const job_dev& J, const stratum& s,
     __syncthreads();
 }
 
+// The read kernel's window: only the SNVs of [k0, k0 + n_win) that a read of
+// the segment's output sample can carry (present flags), compacted in table
+// order, with the bucket index over them; returns their number. Reads take
+// their bases from the first carried SNV at each position, so the records
+// left out (indels, records no source of the sample carries) never change a
+// read. s_cnt: one word per wave.
+template <int BLOCK, int CAP>
+__device__ __forceinline__ uint32_t stage_snvs(const job_dev& J, const seg_dev& S, const stratum& s, uint32_t* s_pos,
+                                               uint32_t* s_lo, uint32_t* s_hi, uint32_t* s_info, uint16_t* s_bucket,
+                                               uint32_t* s_cnt, uint32_t tid) {
+    constexpr uint32_t WAVES = BLOCK / 64;
+    const uint8_t* pres = J.present + (uint64_t)S.so * J.n_mut + s.k0;
+    const uint32_t wave = tid >> 6;
+    uint32_t n = 0;  // block-uniform
+    for (uint32_t c = 0; c < s.n_win; c += BLOCK) {
+        const uint32_t i = c + tid;
+        const bool in = i < s.n_win;
+        const uint32_t info = in ? J.info[s.k0 + i] : 0u;
+        const uint32_t pos = in ? J.pos[s.k0 + i] : 0u, lo = in ? J.lo[s.k0 + i] : 0u, hi = in ? J.hi[s.k0 + i] : 0u;
+        const bool keep = in && PSX_INFO_KIND(info) == PSX_SNV && pres[i] != 0;
+        const unsigned long long m = __ballot(keep);
+        if ((tid & 63u) == 0) s_cnt[wave] = (uint32_t)__popcll(m);
+        __syncthreads();
+        uint32_t off = n, tot = 0;
+#pragma unroll
+        for (uint32_t w = 0; w < WAVES; ++w) {
+            const uint32_t cw = s_cnt[w];
+            off += w < wave ? cw : 0u;
+            tot += cw;
+        }
+        if (keep) {
+            const uint32_t r =
+                off + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+            s_pos[r] = pos;
+            s_lo[r] = lo;
+            s_hi[r] = hi;
+            s_info[r] = info;
+        }
+        n += tot;
+        __syncthreads();
+    }
+    const uint32_t nb = ((s.wend - s.b) >> NB_SHIFT) + 1;
+    for (uint32_t q = tid; q < nb; q += BLOCK) {
+        const uint32_t target = s.b + (q << NB_SHIFT);
+        uint32_t a = 0, z = n;
+        while (a < z) {
+            const uint32_t mid = (a + z) >> 1;
+            if (s_pos[mid] < target) a = mid + 1; else z = mid;
+        }
+        s_bucket[q] = (uint16_t)a;
+    }
+    __syncthreads();
+    return n;
+}
+
 // ------------------------------------------------------------ count kernel
 
 // A stratum whose window exceeds the LDS capacity: reads search the global
@@ -932,27 +987,11 @@ __global__ void __launch_bounds__(BLOCK, DEFER ? 6 : 1) k_seq_count(const seq_la
 
 // Presence (the sample holds the mutation) and zeroed counters; also resets
 // the work counter of the count kernel that follows on the stream.
-__global__ void __launch_bounds__(256) k_seq_init(const job_dev* __restrict__ jobs, const pres_dev* __restrict__ ps,
-                                                  uint32_t n_ps, uint64_t total,
-                                                  unsigned long long* __restrict__ queue, uint32_t n_jobs) {
-    const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q == 0) {
-        *queue = 0ull;
-        for (uint32_t k = 0; k < n_jobs; ++k)
-            if (jobs[k].acc) *jobs[k].acc = 0ull;
-    }
-    if (q >= total) return;
-    uint32_t a = 0, z = n_ps;
-    while (a < z) {
-        const uint32_t mid = (a + z) >> 1;
-        if (ps[mid].begin <= q) a = mid + 1; else z = mid;
-    }
-    const pres_dev P = ps[a - 1];
-    const job_dev& J = jobs[P.job];
-    const uint64_t m = q - P.begin;
-    const uint64_t o = (uint64_t)P.so * J.n_mut + m;
-    J.alt[o] = 0;
-    J.depth[o] = 0;
+// Whether record m can be carried by a read of output sample P: a leaf of the
+// sample lies in its leaf range, or the normal genome carries it and the
+// sample's reads can come from the normal genome (the normal sample, or purity
+// below 1). Oracle: the presence flags of orc_simulate_seq.
+__device__ __forceinline__ bool record_present(const job_dev& J, const pres_dev& P, uint64_t m) {
     const uint32_t info = J.info[m];
     const uint32_t nat = PSX_INFO_NATURE(info);
     const bool normal_has = nat == PSX_GERMINAL || (nat == PSX_PRENEOPLASTIC && J.preneo);
@@ -972,7 +1011,49 @@ __global__ void __launch_bounds__(256) k_seq_init(const job_dev* __restrict__ jo
             pres = x < P.n_cells && sl[x] < hi;
         }
     }
-    J.present[o] = pres ? 1 : 0;
+    return pres;
+}
+
+// The (output sample, mutation) pair of thread q: ps lists each pair range's
+// first index.
+__device__ __forceinline__ const pres_dev& pres_of(const pres_dev* __restrict__ ps, uint32_t n_ps, uint64_t q) {
+    uint32_t a = 0, z = n_ps;
+    while (a < z) {
+        const uint32_t mid = (a + z) >> 1;
+        if (ps[mid].begin <= q) a = mid + 1; else z = mid;
+    }
+    return ps[a - 1];
+}
+
+__global__ void __launch_bounds__(256) k_seq_init(const job_dev* __restrict__ jobs, const pres_dev* __restrict__ ps,
+                                                  uint32_t n_ps, uint64_t total,
+                                                  unsigned long long* __restrict__ queue, uint32_t n_jobs) {
+    const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q == 0) {
+        *queue = 0ull;
+        for (uint32_t k = 0; k < n_jobs; ++k)
+            if (jobs[k].acc) *jobs[k].acc = 0ull;
+    }
+    if (q >= total) return;
+    const pres_dev P = pres_of(ps, n_ps, q);
+    const job_dev& J = jobs[P.job];
+    const uint64_t m = q - P.begin;
+    const uint64_t o = (uint64_t)P.so * J.n_mut + m;
+    J.alt[o] = 0;
+    J.depth[o] = 0;
+    J.present[o] = record_present(J, P, m) ? 1 : 0;
+}
+
+// Presence flags only (read materialisation: the read kernel stages the SNVs
+// a read of the sample can carry).
+__global__ void __launch_bounds__(256) k_reads_present(const job_dev* __restrict__ jobs,
+                                                       const pres_dev* __restrict__ ps, uint32_t n_ps, uint64_t total) {
+    const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= total) return;
+    const pres_dev P = pres_of(ps, n_ps, q);
+    const job_dev& J = jobs[P.job];
+    const uint64_t m = q - P.begin;
+    J.present[(uint64_t)P.so * J.n_mut + m] = record_present(J, P, m) ? 1 : 0;
 }
 
 // --------------------------------------------------- read materialisation
@@ -1087,8 +1168,6 @@ __device__ __forceinline__ void apply_error(uint32_t* row, uint32_t o, uint32_t 
     }
 }
 
-typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-
 // STAGE: reads are built in LDS and copied out per pass (read_words small
 // enough for 64 rows per wave); otherwise each lane builds its read in place
 // in the output. NW > 0: reads of NW base words (rows of whole 16-byte
@@ -1101,6 +1180,7 @@ __global__ void __launch_bounds__(BLOCK) k_seq_reads(const seq_launch L) {
     __shared__ uint32_t s_ref[REF_MAX];
     __shared__ lds_tables s_tab;
     __shared__ uint32_t s_item;
+    __shared__ uint32_t s_cnt[WAVES];
     __shared__ uint32_t s_elist[WAVES][ELIST];  // per wave: errors of a slow template
     extern __shared__ uint4 s_stage4[];  // STAGE: per wave 64 rows of read_words words
     const uint32_t tid = threadIdx.x;
@@ -1131,9 +1211,10 @@ __global__ void __launch_bounds__(BLOCK) k_seq_reads(const seq_launch L) {
             const uint32_t ref_w0 = s.b >> 4;
             const uint32_t n_ref = min(chr_words, ((s.wend + 15u) >> 4) + 1u) - ref_w0;
             for (uint32_t i = tid; i <= n_ref; i += BLOCK) s_ref[i] = i < n_ref ? J.packed[S.chr_woff + ref_w0 + i] : 0u;
-            if (fits) stage_window<BLOCK, CAP>(J, s, s_pos, s_lo, s_hi, s_info, nullptr, nullptr, s_bucket, tid);
+            uint32_t n_snv = 0;
+            if (fits) n_snv = stage_snvs<BLOCK, CAP>(J, S, s, s_pos, s_lo, s_hi, s_info, s_bucket, s_cnt, tid);
             else __syncthreads();
-            const lds_view lv{s_pos, s_lo, s_hi, s_info, nullptr, nullptr, s_bucket, s.n_win, s.b};
+            const lds_view lv{s_pos, s_lo, s_hi, s_info, nullptr, nullptr, s_bucket, n_snv, s.b};
             const glb_view gv{J.pos + s.k0, J.lo + s.k0, J.hi + s.k0, J.info + s.k0, nullptr, nullptr, s.n_win};
             const uint32_t nj = (uint32_t)s.nj;
             for (uint32_t c0 = wave * 64u; c0 < nj; c0 += WAVES * 64u) {
@@ -1201,11 +1282,9 @@ __global__ void __launch_bounds__(BLOCK) k_seq_reads(const seq_launch L) {
                         const uint32_t words = nt * rpt * RW;
                         uint32_t* dst = L.reads_out + first * RW;
                         if (vec4) {
-                            // streaming stores: the rows are not read again by this kernel
-                            u32x4* d4 = reinterpret_cast<u32x4*>(dst);
-                            const u32x4* s4 = reinterpret_cast<const u32x4*>(stage4);
-                            for (uint32_t q = lane; q < (words >> 2); q += 64u)
-                                __builtin_nontemporal_store(s4[q], d4 + q);
+                            // plain stores: non-temporal ones cap this write pattern near 4.5 TB/s
+                            uint4* d4 = reinterpret_cast<uint4*>(dst);
+                            for (uint32_t q = lane; q < (words >> 2); q += 64u) d4[q] = stage4[q];
                         } else {
                             for (uint32_t q = lane; q < words; q += 64u) dst[q] = stage[q];
                         }
