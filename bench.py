@@ -31,8 +31,8 @@ HBM_PEAK_GBS = 8000.0
 # WRITE_SIZE runs calibrated against a device copy, and their issue-slot use from SQ counters (the rocprofv3
 # summaries under profiles/, named for the kernel and workload). Count-only sequencing is bound by instruction
 # issue, not HBM.
-PROFILED = {"k_seq_count": dict(traffic=302.2e6, valu_busy=0.87, salu_busy=0.76),
-            "k_seq_reads": dict(traffic=77.71e9, valu_busy=0.69, salu_busy=0.60)}
+PROFILED = {"k_seq_count": dict(traffic=302.0e6, valu_per_cu_cycle=1.05, salu_per_cu_cycle=0.62),
+            "k_seq_reads": dict(traffic=77.66e9, valu_per_cu_cycle=0.70, salu_per_cu_cycle=0.39)}
 
 
 def log(msg: str) -> None:
@@ -380,7 +380,8 @@ def main() -> int:
     prof = PROFILED["k_seq_count"] if args.cells == 1_000_000 else {}
     roofline = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
                     traffic=prof.get("traffic"), kernel="k_seq_count", kernel_ms=kernel_ms, algo_bytes=algo_bytes,
-                    issue_bound=dict(valu_busy=prof.get("valu_busy"), salu_busy=prof.get("salu_busy"),
+                    issue_bound=dict(valu_per_cu_cycle=prof.get("valu_per_cu_cycle"),
+                                     salu_per_cu_cycle=prof.get("salu_per_cu_cycle"),
                                      note="count-only moves ~0.04 B per read; VALU/SALU issue slots bound it"))
 
     out = {
