@@ -18,9 +18,10 @@ namespace psxh {
 struct seq_batch {
     std::vector<job_dev> jobs;
     std::vector<seg_dev> segs;
-    std::vector<item_dev> items;
     std::vector<pres_dev> pres;
     uint64_t n_reads = 0, pres_total = 0;
+    uint64_t n_items = 0;  // written on the device (k_seq_items) from the segments
+    uint32_t spi = 1;      // strata per item
     scratch_builder sb;
     std::vector<size_t> tab_off;  // per job: insert thresholds, insert guide, error thresholds
     std::vector<size_t> cna_off;  // per job: copy-number events (SIZE_MAX: none)
@@ -186,11 +187,10 @@ void prepare_job(psx_ctx* ctx, seq_batch& P, const job_in& in, uint32_t spi) {
             S.n_bucket = (uint32_t)(m->bucket_off[c + 1] - m->bucket_off[c]);
             S.leaf_off = leaf_off; S.n_cells = n_cells;
             P.n_reads += S.T * (pe ? 2u : 1u);  // template draws: an upper bound with copy-number events
-            const uint32_t seg = (uint32_t)P.segs.size();
+            S.first_item = (uint32_t)P.n_items;
+            if (S.T) P.n_items += (S.J + spi - 1) / spi;
+            if (P.n_items >= (uint64_t(1) << 32)) psx_throw(PSX_EINVAL, "too many work items in one batch");
             P.segs.push_back(S);
-            if (S.T == 0) continue;
-            for (uint32_t j0 = 0; j0 < S.J; j0 += spi)
-                P.items.push_back(item_dev{seg, j0, std::min(S.J, j0 + spi), 0});
         }
     }
     P.job_reads.push_back(P.n_reads - reads_before);
@@ -201,9 +201,9 @@ void layout(seq_batch& P) {
     const std::vector<uint64_t> zeros(P.jobs.size(), 0);
     P.o_acc = P.sb.put_vec(zeros);
     P.o_segs = P.sb.put_vec(P.segs);
-    P.o_items = P.sb.put_vec(P.items);
     P.o_pres = P.sb.put_vec(P.pres);
     P.o_jobs = P.sb.put_vec(P.jobs);
+    P.o_items = (P.sb.host.size() + 255) & ~size_t(255);  // device-only, after the uploaded descriptors
 }
 
 template <int BLOCK, int CAP>
@@ -233,7 +233,7 @@ seq_launch launch_desc(const seq_batch& P, uint8_t* base, unsigned long long* qu
     L.jobs = (const job_dev*)(base + P.o_jobs);
     L.segs = (const seg_dev*)(base + P.o_segs);
     L.items = (const item_dev*)(base + P.o_items);
-    L.n_items = P.items.size();
+    L.n_items = P.n_items;
     L.queue = queue;
     return L;
 }
@@ -242,13 +242,14 @@ seq_launch launch_desc(const seq_batch& P, uint8_t* base, unsigned long long* qu
 
 seq_batch* batch_prepare(psx_ctx* ctx, const job_in* jobs, size_t n_jobs, uint32_t spi) {
     auto P = std::make_unique<seq_batch>();
+    P->spi = spi;
     for (size_t k = 0; k < n_jobs; ++k) prepare_job(ctx, *P, jobs[k], spi);
     layout(*P);
     return P.release();
 }
 
 void batch_free(seq_batch* b) { delete b; }
-size_t batch_desc_bytes(const seq_batch* b) { return b->sb.host.size() + 256; }
+size_t batch_desc_bytes(const seq_batch* b) { return b->o_items + b->n_items * sizeof(item_dev) + 256; }
 uint64_t batch_reads(const seq_batch* b) { return b->n_reads; }
 bool batch_has_cna(const seq_batch* b) {
     for (size_t off : b->cna_off)
@@ -266,7 +267,7 @@ std::vector<uint64_t> batch_job_reads(const seq_batch* b, const seq_lane& lane) 
         if (b->cna_off[k] != SIZE_MAX) r[k] = acc[k];
     return r;
 }
-size_t batch_items(const seq_batch* b) { return b->items.size(); }
+size_t batch_items(const seq_batch* b) { return b->n_items; }
 
 void batch_upload(seq_batch* P, const seq_lane& lane) {
     if (lane.desc_bytes < batch_desc_bytes(P)) psx_throw(PSX_EINVAL, "descriptor buffer too small");
@@ -282,6 +283,12 @@ void batch_upload(seq_batch* P, const seq_lane& lane) {
     }
     if (!P->sb.host.empty())
         PSX_HIP(hipMemcpyAsync(base, P->sb.host.data(), P->sb.host.size(), hipMemcpyHostToDevice, lane.stream));
+    if (P->n_items) {
+        const uint32_t grid = (uint32_t)((P->n_items + 255) / 256);
+        hipLaunchKernelGGL(k_seq_items, dim3(grid), dim3(256), 0, lane.stream, (const seg_dev*)(base + P->o_segs),
+                           (uint32_t)P->segs.size(), (item_dev*)(base + P->o_items), (uint32_t)P->n_items, P->spi);
+        PSX_HIP(hipGetLastError());
+    }
     P->dev = base;
 }
 
@@ -290,7 +297,7 @@ void batch_launch(const seq_batch* P, const seq_lane& lane, const psx_launch_con
     const seq_launch L = launch_desc(*P, lane.desc, lane.queue);
     const pres_dev* ps = (const pres_dev*)(lane.desc + P->o_pres);
     const uint64_t want = (uint64_t)lane.n_cu * cfg.blocks_per_cu;
-    const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, P->items.size()));
+    const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, P->n_items));
     const uint32_t init_grid = (uint32_t)std::max<uint64_t>(1, (P->pres_total + 255) / 256);
     hipLaunchKernelGGL(k_seq_init, dim3(init_grid), dim3(256), 0, lane.stream, L.jobs, ps, (uint32_t)P->pres.size(),
                        P->pres_total, L.queue, (uint32_t)P->jobs.size());
@@ -506,9 +513,9 @@ extern "C" psx_status psx_simulate_seq_reads(psx_ctx* ctx, const psx_genome* g, 
         // per CU share one copy of the block's window, reference and tables:
         // 512 threads run 6 waves per SIMD at 150 bp, 256 threads 4 or 5)
         const size_t wave_stage = (size_t)64 * L.read_words * 4u;
-        if (P.items.empty()) PSX_HIP(hipEventRecord(ctx->ev0, ctx->stream));
-        if (!P.items.empty()) {
-            const size_t items = P.items.size();
+        if (P.n_items == 0) PSX_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+        if (P.n_items) {
+            const size_t items = P.n_items;
             if (L.out_vec4 && L.read_words == 12)  // 145-160 bp reads
                 launch_reads<512, CAP, true, 10>(ctx, L, items, 8 * wave_stage);
             else if (8 * wave_stage <= STAGE_MAX)

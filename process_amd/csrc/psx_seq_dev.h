@@ -75,12 +75,31 @@ struct seg_dev {
     uint32_t W, J, bucket_off, n_bucket;
     uint32_t leaf_off, n_cells;  // n_cells == 0: normal sample
     uint32_t cna_off, n_cna;     // the chromosome's copy-number events (J.cna + cna_off)
-    uint32_t kmax, pad;          // copies drawn per template: the largest copy number, >= ploidy
+    uint32_t kmax;               // copies drawn per template: the largest copy number, >= ploidy
+    uint32_t first_item;         // its work items: first_item .. first_item + ceil(J / strata per item) - 1
 };
 
 struct item_dev {
     uint32_t seg, j0, j1, pad;
 };
+
+// The work items of a batch, written on the device from the segments (the
+// host lists segments only): item q belongs to the last segment whose first
+// item is at or below q (a segment without templates has no items and shares
+// its successor's first item) and covers spi strata from its first.
+__global__ void __launch_bounds__(256) k_seq_items(const seg_dev* __restrict__ segs, uint32_t n_segs,
+                                                   item_dev* __restrict__ items, uint32_t n_items, uint32_t spi) {
+    const uint32_t q = blockIdx.x * 256u + threadIdx.x;
+    if (q >= n_items) return;
+    uint32_t a = 0, z = n_segs;
+    while (a < z) {
+        const uint32_t mid = (a + z) >> 1;
+        if (segs[mid].first_item <= q) a = mid + 1; else z = mid;
+    }
+    const seg_dev& S = segs[a - 1];
+    const uint32_t j0 = (q - S.first_item) * spi;
+    items[q] = item_dev{a - 1, j0, min(S.J, j0 + spi), 0u};
+}
 
 struct pres_dev {  // one per (job, output sample) for the init kernel
     uint64_t begin;        // global index of the first (so, m) entry
