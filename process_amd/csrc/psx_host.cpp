@@ -293,7 +293,7 @@ psx_ctable psx_compact_table(const psx_itable& t) {
     c.guide.resize(PSX_GUIDE);
     size_t k = 0;
     for (uint32_t g = 0; g < PSX_GUIDE; ++g) {
-        const uint64_t target = (uint64_t)g << 24;
+        const uint64_t target = (uint64_t)g << (32u - PSX_GUIDE_BITS);
         while (k + 1 < m && (uint64_t)c.T[k] <= target) ++k;
         c.guide[g] = (uint16_t)k;
     }

@@ -116,11 +116,13 @@ PSX_HD uint64_t psx_uniform64(uint32_t hi, uint32_t lo, uint64_t n) {
 // Compact form of an inversion table for the kernels: entries that can never
 // be drawn are dropped (leading T[k] == 0; everything after the first
 // T[k] == 2^32), so every kept threshold but the last fits in 32 bits, and a
-// 256-entry guide table g[i] = first k with T[k] > i * 2^24 starts the scan.
-// The draw equals psx_cdf_pick on the full table.
-#define PSX_GUIDE 256u
+// guide table of PSX_GUIDE entries, g[i] = first k with T[k] > i * 2^(32 -
+// PSX_GUIDE_BITS), starts the scan. The draw equals psx_cdf_pick on the full
+// table whatever the guide's size.
+#define PSX_GUIDE_BITS 11u
+#define PSX_GUIDE (1u << PSX_GUIDE_BITS)
 PSX_HD uint32_t psx_guided_pick(const uint32_t* T32, const uint16_t* guide, uint32_t n, uint32_t u) {
-    uint32_t k = guide[u >> 24];
+    uint32_t k = guide[u >> (32u - PSX_GUIDE_BITS)];
     while (k + 1 < n && u >= T32[k]) ++k;
     return k;
 }

@@ -117,16 +117,25 @@ struct seq_launch {
     uint32_t out_vec4;       // read_words % 4 == 0 and reads_out 16-byte aligned
 };
 
-struct lds_tables {
+// The job's inversion tables in LDS; the insert-size guide keeps 2^GB of the
+// host table's PSX_GUIDE entries (every 2^(PSX_GUIDE_BITS - GB)-th, the same
+// guide at a coarser step), so a kernel short of LDS can hold a smaller one.
+template <uint32_t GB>
+struct lds_tables_t {
+    static constexpr uint32_t GUIDE_BITS = GB;
     uint32_t ins[MAX_INS];
-    uint16_t guide[PSX_GUIDE];
+    uint16_t guide[1u << GB];
     uint32_t err[MAX_ERRT];
 };
+using lds_tables = lds_tables_t<PSX_GUIDE_BITS>;
 
-__device__ __forceinline__ void load_tables(const job_dev& J, lds_tables& t, uint32_t tid, uint32_t block) {
+template <class TAB>
+__device__ __forceinline__ void load_tables(const job_dev& J, TAB& t, uint32_t tid, uint32_t block) {
+    constexpr uint32_t GB = TAB::GUIDE_BITS;
+    static_assert(GB <= PSX_GUIDE_BITS, "the host guide is the finest");
     if (J.pe) {
         for (uint32_t i = tid; i < J.ins_n; i += block) t.ins[i] = J.ins_T[i];
-        for (uint32_t i = tid; i < PSX_GUIDE; i += block) t.guide[i] = J.ins_guide[i];
+        for (uint32_t i = tid; i < (1u << GB); i += block) t.guide[i] = J.ins_guide[i << (PSX_GUIDE_BITS - GB)];
     }
     for (uint32_t i = tid; i < J.err_n; i += block) t.err[i] = J.err_T[i];
 }
@@ -258,10 +267,11 @@ __device__ __forceinline__ bool carried(const job_dev& J, uint32_t info, uint32_
 // the answer lies in [guide[g], guide[g + 1]] (the last bucket ends at n - 1),
 // and the tail buckets of a peaked distribution hold tens of entries, which a
 // linear scan would walk for the whole wave. Same result as psx_guided_pick.
+template <uint32_t GB>
 __device__ __forceinline__ uint32_t guided_pick_u(const uint32_t* T32, const uint16_t* guide, uint32_t n, uint32_t u) {
-    const uint32_t g = u >> 24;
+    const uint32_t g = u >> (32u - GB);
     uint32_t lo = guide[g];
-    uint32_t hi = g < PSX_GUIDE - 1u ? guide[g + 1u] : n - 1u;
+    uint32_t hi = g < (1u << GB) - 1u ? guide[g + 1u] : n - 1u;
     while (__ballot(lo < hi)) {
         const uint32_t mid = (lo + hi) >> 1;
         const bool below = u < T32[mid];
@@ -273,8 +283,8 @@ __device__ __forceinline__ uint32_t guided_pick_u(const uint32_t* T32, const uin
 }
 
 // Template r of stratum j (oracle: the template loop of orc_simulate_seq).
-template <bool CNA = false>
-__device__ __forceinline__ tmpl make_template(const job_dev& J, const seg_dev& S, const lds_tables& tab, uint32_t r,
+template <bool CNA = false, class TAB>
+__device__ __forceinline__ tmpl make_template(const job_dev& J, const seg_dev& S, const TAB& tab, uint32_t r,
                                               uint32_t j, uint32_t b, uint32_t e) {
     const uint32_t c2 = (S.chr << 16) | S.so;
     const psx_u4 w = psx_draw_u(J.seed, r, j, c2, PSX_P_TEMPLATE << 24);
@@ -329,7 +339,7 @@ __device__ __forceinline__ tmpl make_template(const job_dev& J, const seg_dev& S
     }
     t.x2 = t.x;
     if (J.pe) {
-        uint32_t I = J.ins_k0 + guided_pick_u(tab.ins, tab.guide, J.ins_n, w.w);
+        uint32_t I = J.ins_k0 + guided_pick_u<TAB::GUIDE_BITS>(tab.ins, tab.guide, J.ins_n, w.w);
         if (I < J.L) I = J.L;
         uint64_t x2 = (uint64_t)t.x + I - J.L;
         if (x2 > S.len - J.L) x2 = S.len - J.L;
@@ -353,7 +363,8 @@ struct terr {
 
 // The summary of template r of stratum j; lanes with want == false get none.
 // Every lane steps through the same count loop.
-__device__ __forceinline__ terr template_errors(const job_dev& J, const lds_tables& tab, uint32_t r, uint32_t j,
+template <class TAB>
+__device__ __forceinline__ terr template_errors(const job_dev& J, const TAB& tab, uint32_t r, uint32_t j,
                                                 uint32_t c2, bool want) {
     const psx_u4 w = psx_draw_u(J.seed, r, j, c2, PSX_P_ERROR << 24);
     // K = err_k0 + the number of thresholds k < err_n - 1 with u >= T[k] (T is
@@ -542,8 +553,8 @@ __device__ __forceinline__ void error_fix(const job_dev& J, const V& v, uint32_t
 // errors and each lane takes one, correcting the counts where the error's
 // base holds records. A read covering that position resolved its
 // template's leaf while it was counted.
-template <class V>
-__device__ __forceinline__ void count_slow_errors(const job_dev& J, const V& v, const lds_tables& tab, const tmpl& t,
+template <class V, class TAB>
+__device__ __forceinline__ void count_slow_errors(const job_dev& J, const V& v, const TAB& tab, const tmpl& t,
                                                   const terr& e, bool valid, uint32_t base, uint32_t j, uint32_t c2,
                                                   uint32_t* list, uint32_t lane) {
     unsigned long long sm = __ballot(valid && e.slow());
@@ -627,7 +638,8 @@ struct chk_ring {
 // corrects the counts there (error_fix). The entries are read first, so a
 // slow template's error list can use their half of the ring. Oracle:
 // seq_read with the read's errors.
-__device__ __forceinline__ void check_errors(const job_dev& J, const lds_view& v, const lds_tables& tab, uint32_t j,
+template <class TAB>
+__device__ __forceinline__ void check_errors(const job_dev& J, const lds_view& v, const TAB& tab, uint32_t j,
                                              uint32_t c2, chk_ring& q, uint32_t n, uint32_t lane) {
     const bool on = lane < n;
     uint32_t* half = q.e + q.head * 3u;
@@ -675,9 +687,9 @@ __device__ __forceinline__ void check_errors(const job_dev& J, const lds_view& v
 // template summaries e give the observed bases (slow templates are corrected
 // afterwards). Lanes with valid == false take no part (their starts must
 // still lie in the stratum).
-template <bool DEFER>
+template <bool DEFER, class TAB>
 __device__ __forceinline__ void count_mates_wave(const job_dev& J, const seg_dev& S, const lds_view& v,
-                                                 const lds_tables& tab, tmpl& t, const terr& e, bool valid,
+                                                 const TAB& tab, tmpl& t, const terr& e, bool valid,
                                                  uint32_t base, uint32_t j, uint32_t c2, uint32_t* q, chk_ring& ck,
                                                  uint32_t lane) {
     uint32_t leaf = 0;  // the lane's source leaf, resolved once a read of it covers a record
@@ -888,8 +900,8 @@ __device__ __forceinline__ uint32_t stage_snvs(const job_dev& J, const seg_dev& 
 // A stratum whose window exceeds the LDS capacity: reads search the global
 // records and count with global atomics (kept out of line so the LDS path's
 // code is shared by both mates).
-template <int BLOCK, bool CNA>
-__device__ __forceinline__ void count_stratum_global(const job_dev& J, const seg_dev& S, const lds_tables& tab,
+template <int BLOCK, bool CNA, class TAB>
+__device__ __forceinline__ void count_stratum_global(const job_dev& J, const seg_dev& S, const TAB& tab,
                                                   const stratum& s, uint32_t j, uint32_t c2, uint32_t* list,
                                                   uint32_t lane, uint32_t wave, uint32_t* acc) {
     const glb_view v{J.pos + s.k0, J.lo + s.k0, J.hi + s.k0, J.info + s.k0, J.alt + S.out_off + s.k0,
@@ -921,7 +933,8 @@ template <int BLOCK, int CAP, bool CNA, bool DEFER>
 __global__ void __launch_bounds__(BLOCK, DEFER ? 6 : 1) k_seq_count(const seq_launch L) {
     __shared__ uint32_t s_pos[CAP], s_lo[CAP], s_hi[CAP], s_info[CAP], s_alt[CAP], s_dep[CAP];
     __shared__ uint16_t s_bucket[NB_MAX];
-    __shared__ lds_tables s_tab;
+    // the deferred form keeps a 256-entry guide: its LDS also holds the error-check rings
+    __shared__ lds_tables_t<DEFER ? 8u : PSX_GUIDE_BITS> s_tab;
     __shared__ uint32_t s_item, s_acc;
     // per wave: the (lane, record) queue of a mate (also a slow template's errors on
     // the global path), and the deferred error checks
