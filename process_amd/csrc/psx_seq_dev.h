@@ -548,14 +548,14 @@ __device__ __forceinline__ void error_fix(const job_dev& J, const V& v, uint32_t
     }
 }
 
-// The slow templates among a wave's 64 (one per lane: t, e, counted when
-// valid) were counted without errors: for each, the wave generates its
-// errors and each lane takes one, correcting the counts where the error's
-// base holds records. A read covering that position resolved its
+// The slow templates among a wave's 64 (one per lane: t, e, template index
+// rr, counted when valid) were counted without errors: for each, the wave
+// generates its errors and each lane takes one, correcting the counts where
+// the error's base holds records. A read covering that position resolved its
 // template's leaf while it was counted.
 template <class V, class TAB>
 __device__ __forceinline__ void count_slow_errors(const job_dev& J, const V& v, const TAB& tab, const tmpl& t,
-                                                  const terr& e, bool valid, uint32_t base, uint32_t j, uint32_t c2,
+                                                  const terr& e, bool valid, uint32_t rr, uint32_t j, uint32_t c2,
                                                   uint32_t* list, uint32_t lane) {
     unsigned long long sm = __ballot(valid && e.slow());
     while (sm) {  // wave-uniform
@@ -565,9 +565,9 @@ __device__ __forceinline__ void count_slow_errors(const job_dev& J, const V& v, 
         const uint32_t x2s = (uint32_t)__builtin_amdgcn_readlane((int)t.x2, (int)sl);
         const uint32_t ls = (uint32_t)__builtin_amdgcn_readlane((int)t.leaf, (int)sl);
         const uint32_t as = (uint32_t)__builtin_amdgcn_readlane((int)t.allele, (int)sl);
+        const uint32_t rs = (uint32_t)__builtin_amdgcn_readlane((int)rr, (int)sl);
         wave_sync();
-        const uint32_t n = wave_template_errors(J.seed, J.err_n, J.err_k0, J.err_N, tab.err, base + sl, j, c2, list,
-                                                lane);
+        const uint32_t n = wave_template_errors(J.seed, J.err_n, J.err_k0, J.err_N, tab.err, rs, j, c2, list, lane);
         wave_sync();
         for (uint32_t k0 = 0; k0 < n; k0 += 64u) {
             const uint32_t k = k0 + lane;
@@ -683,14 +683,14 @@ __device__ __forceinline__ void check_errors(const job_dev& J, const lds_view& v
 // work runs on full waves instead of waiting for the lane whose reads cover
 // the most positions. Sequencing errors: with DEFER the reads are counted
 // with their true bases and each drained position whose observed base
-// matters goes to the deferred checks (template base + lane); otherwise the
+// matters goes to the deferred checks (with the lane's template index rr); otherwise the
 // template summaries e give the observed bases (slow templates are corrected
 // afterwards). Lanes with valid == false take no part (their starts must
 // still lie in the stratum).
 template <bool DEFER, class TAB>
 __device__ __forceinline__ void count_mates_wave(const job_dev& J, const seg_dev& S, const lds_view& v,
                                                  const TAB& tab, tmpl& t, const terr& e, bool valid,
-                                                 uint32_t base, uint32_t j, uint32_t c2, uint32_t* q, chk_ring& ck,
+                                                 uint32_t rr, uint32_t j, uint32_t c2, uint32_t* q, chk_ring& ck,
                                                  uint32_t lane) {
     uint32_t leaf = 0;  // the lane's source leaf, resolved once a read of it covers a record
     // allele | (x2 - x) << 8, for the drains (x2 - x < 2^13)
@@ -727,9 +727,10 @@ __device__ __forceinline__ void count_mates_wave(const job_dev& J, const seg_dev
             if (m) {
                 const uint32_t rank =
                     __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                const uint32_t ro = (uint32_t)__shfl((int)rr, (int)ol);  // the entry's template
                 if (chk) {
                     uint32_t* slot = ck.e + ((ck.head + ck.count + rank) & 127u) * 3u;
-                    slot[0] = base + ol;
+                    slot[0] = ro;
                     slot[1] = (p + boff) | (al << 13) | (ri << 21);
                     slot[2] = lo;
                 }
@@ -919,7 +920,7 @@ __device__ __forceinline__ void count_stratum_global(const job_dev& J, const seg
         bool hit = false;  // errors only matter where a read covers a record
         for (uint32_t mate = 0; mate <= J.pe; ++mate)
             hit |= count_read(J, S, v, t, e, valid, mate ? t.x2 : t.x, mate);
-        if (J.err_n && hit) count_slow_errors(J, v, tab, t, e, valid, base, j, c2, list, lane);
+        if (J.err_n && hit) count_slow_errors(J, v, tab, t, e, valid, r, j, c2, list, lane);
     }
 }
 
@@ -964,34 +965,84 @@ __global__ void __launch_bounds__(BLOCK, DEFER ? 6 : 1) k_seq_count(const seq_la
                 const lds_view v{s_pos, s_lo, s_hi, s_info, s_alt, s_dep, s_bucket, s.n_win, s.b};
                 const uint32_t nj = (uint32_t)s.nj;
                 chk_ring ck{s_chk[wave], 0u, 0u};
-                if (DEFER) {
-                    for (uint32_t base = wave * 64u; base < nj; base += BLOCK) {  // wave-uniform
-                        const uint32_t r = base + lane;
-                        bool valid = r < nj;
-                        tmpl t = make_template<CNA>(J, S, s_tab, valid ? r : base, j, s.b, s.e);
-                        if (CNA) {
-                            valid = valid && t.ok;
-                            acc += valid ? 1u : 0u;
-                        }
-                        count_mates_wave<true>(J, S, v, s_tab, t, terr{0u, 0u}, valid, base, j, c2, s_wave[wave], ck,
+                // the counting of a wave's templates t (lanes with valid; template indices rr)
+                auto count_wave = [&](tmpl& t, uint32_t rr, bool valid) {
+                    if (DEFER) {
+                        count_mates_wave<true>(J, S, v, s_tab, t, terr{0u, 0u}, valid, rr, j, c2, s_wave[wave], ck,
                                                lane);
+                    } else {
+                        const terr e = J.err_n ? template_errors(J, s_tab, rr, j, c2, valid) : terr{0u, 0u};
+                        count_mates_wave<false>(J, S, v, s_tab, t, e, valid, rr, j, c2, s_wave[wave], ck, lane);
+                        if (J.err_n) count_slow_errors(J, v, s_tab, t, e, valid, rr, j, c2, s_wave[wave], lane);
                     }
-                    if (J.err_n && ck.count) check_errors(J, v, s_tab, j, c2, ck, ck.count, lane);
+                };
+                if (!CNA && DEFER) {
+                    for (uint32_t base = wave * 64u; base < nj; base += BLOCK) {  // wave-uniform
+                        const uint32_t r = base + lane;
+                        const bool valid = r < nj;
+                        tmpl t = make_template<CNA>(J, S, s_tab, valid ? r : base, j, s.b, s.e);
+                        count_mates_wave<true>(J, S, v, s_tab, t, terr{0u, 0u}, valid, valid ? r : base, j, c2,
+                                               s_wave[wave], ck, lane);
+                    }
+                } else if (!CNA) {
+                    for (uint32_t base = wave * 64u; base < nj; base += BLOCK) {  // wave-uniform
+                        const uint32_t r = base + lane;
+                        const bool valid = r < nj;
+                        const uint32_t rv = valid ? r : base;
+                        tmpl t = make_template<CNA>(J, S, s_tab, rv, j, s.b, s.e);
+                        const terr e = J.err_n ? template_errors(J, s_tab, rv, j, c2, valid) : terr{0u, 0u};
+                        count_mates_wave<false>(J, S, v, s_tab, t, e, valid, rv, j, c2, s_wave[wave], ck, lane);
+                        if (J.err_n) count_slow_errors(J, v, s_tab, t, e, valid, rv, j, c2, s_wave[wave], lane);
+                    }
                 } else {
+                    // copy-number draws reject templates: the accepted ones are packed
+                    // into the lanes [0, pc) of a pending set and counted once the next
+                    // group would overflow it, so the walks run on nearly full waves
+                    uint32_t pc = 0;  // wave-uniform
+                    uint32_t px = 0, pad = 0, pleaf = 0, pr = 0;  // x; x2 - x | allele << 13; leaf; index
+                    auto flush = [&]() {
+                        const bool on = lane < pc;
+                        tmpl t;
+                        t.x = on ? px : s.b;  // idle lanes keep a start inside the stratum
+                        t.x2 = on ? px + (pad & 0x1FFFu) : s.b;
+                        t.allele = pad >> 13;
+                        t.leaf = pleaf;
+                        t.cell = 0;
+                        t.ok = 1;
+                        count_wave(t, pr, on);
+                        pc = 0;
+                    };
                     for (uint32_t base = wave * 64u; base < nj; base += BLOCK) {  // wave-uniform
                         const uint32_t r = base + lane;
                         bool valid = r < nj;
                         tmpl t = make_template<CNA>(J, S, s_tab, valid ? r : base, j, s.b, s.e);
-                        if (CNA) {
-                            valid = valid && t.ok;
-                            acc += valid ? 1u : 0u;
-                        }
-                        const terr e =
-                            J.err_n ? template_errors(J, s_tab, valid ? r : base, j, c2, valid) : terr{0u, 0u};
-                        count_mates_wave<false>(J, S, v, s_tab, t, e, valid, base, j, c2, s_wave[wave], ck, lane);
-                        if (J.err_n) count_slow_errors(J, v, s_tab, t, e, valid, base, j, c2, s_wave[wave], lane);
+                        valid = valid && t.ok;
+                        acc += valid ? 1u : 0u;
+                        const unsigned long long m = __ballot(valid);
+                        const uint32_t n = (uint32_t)__popcll(m);
+                        if (pc + n > 64u) flush();
+                        // a permutation of the lanes: accepted ones to [pc, pc + n) in lane
+                        // order, the others to the remaining lanes
+                        const uint32_t below =
+                            __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                        const uint32_t dst = valid ? pc + below : (pc + n + (lane - below)) & 63u;
+                        const int a4 = (int)(dst * 4u);
+                        const uint32_t nx = (uint32_t)__builtin_amdgcn_ds_permute(a4, (int)t.x);
+                        const uint32_t nad = (uint32_t)__builtin_amdgcn_ds_permute(
+                            a4, (int)((t.x2 - t.x) | (t.allele << 13)));
+                        const uint32_t nleaf = (uint32_t)__builtin_amdgcn_ds_permute(a4, (int)t.leaf);
+                        const uint32_t nr = (uint32_t)__builtin_amdgcn_ds_permute(a4, (int)r);
+                        const bool recv = lane >= pc && lane < pc + n;
+                        px = recv ? nx : px;
+                        pad = recv ? nad : pad;
+                        pleaf = recv ? nleaf : pleaf;
+                        pr = recv ? nr : pr;
+                        pc += n;
+                        if (pc == 64u) flush();
                     }
+                    if (pc) flush();
                 }
+                if (DEFER && J.err_n && ck.count) check_errors(J, v, s_tab, j, c2, ck, ck.count, lane);
                 __syncthreads();
                 int32_t* ga = J.alt + S.out_off + s.k0;
                 int32_t* gd = J.depth + S.out_off + s.k0;
