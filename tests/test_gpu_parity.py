@@ -149,6 +149,29 @@ def test_batch_equals_plain(ctx, world):
         assert n == total
 
 
+def test_batch_with_empty_segments(ctx, world):
+    """Work items are written on the device from the segment table; segments
+    without templates (zero coverage) own no items and share their successor's
+    first item, which the item lookup must pass over."""
+    say("[parity] batched launch with zero-coverage jobs between others")
+    zero = dict(cases.SEQ_CASES["pe_errors"], coverage=0)
+    spec = [(world, 21, zero), (world, 22, cases.SEQ_CASES["subset"]), (world, 23, zero),
+            (world, 24, cases.SEQ_CASES["pe_high_error"]), (world, 25, zero)]
+    for cfg in (None, A.LaunchConfig(256, 512, 3, 8, 0)):
+        jobs = [core.SeqJob(ctx, w.g, w.pf, w.pm, seed, **kw) for w, seed, kw in spec]
+        for j in jobs:
+            for buf in (j.alt, j.depth, j.present):
+                buf.fill(0xA5)
+        n = core.simulate_seq_batch(ctx, jobs, cfg)
+        total = 0
+        for (w, seed, kw), j in zip(spec, jobs):
+            a, d, p = j.results()
+            oa, od, op, on = O.simulate_seq(w.lens, w.of, w.om, seed, **kw)
+            total += on
+            assert np.array_equal(a, oa) and np.array_equal(d, od) and np.array_equal(p, op), (seed, cfg)
+        assert n == total
+
+
 def test_plan_graph_replays(ctx, world):
     """A prepared plan replays its captured hipGraph; every replay, with eager
     calls on the same stream in between and the outputs poisoned before it,
