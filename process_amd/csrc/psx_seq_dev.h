@@ -929,13 +929,16 @@ __device__ __forceinline__ void count_stratum_global(const job_dev& J, const seg
 // jobs' reads cover few positions per wave (sparse tables, rejected
 // templates), so errors are drawn per covered position (deferred checks, one
 // draw per 64 positions) instead of per template; both give the same counts.
-// The deferred form is held to 6 waves per SIMD (80 VGPRs; a few spill).
+// The deferred form is held to 6 waves per SIMD (80 VGPRs; a few spill), the
+// plain form to 7 (72 VGPRs, with a 1024-entry guide so that 7 blocks of 256
+// threads fit in LDS).
 template <int BLOCK, int CAP, bool CNA, bool DEFER>
-__global__ void __launch_bounds__(BLOCK, DEFER ? 6 : 1) k_seq_count(const seq_launch L) {
+__global__ void __launch_bounds__(BLOCK, DEFER ? 6 : (CNA ? 1 : 7)) k_seq_count(const seq_launch L) {
     __shared__ uint32_t s_pos[CAP], s_lo[CAP], s_hi[CAP], s_info[CAP], s_alt[CAP], s_dep[CAP];
     __shared__ uint16_t s_bucket[NB_MAX];
-    // the deferred form keeps a 256-entry guide: its LDS also holds the error-check rings
-    __shared__ lds_tables_t<DEFER ? 8u : PSX_GUIDE_BITS> s_tab;
+    // the deferred form keeps a 256-entry guide (its LDS also holds the error-check rings),
+    // the plain form a 1024-entry one
+    __shared__ lds_tables_t<DEFER ? 8u : (CNA ? PSX_GUIDE_BITS : 10u)> s_tab;
     __shared__ uint32_t s_item, s_acc;
     // per wave: the (lane, record) queue of a mate (also a slow template's errors on
     // the global path), and the deferred error checks
