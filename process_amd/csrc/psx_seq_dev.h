@@ -955,8 +955,11 @@ __global__ void __launch_bounds__(BLOCK, DEFER ? 6 : (CNA ? 1 : 7)) k_seq_count(
         const uint32_t item = __builtin_amdgcn_readfirstlane(s_item);
         if (item >= L.n_items) break;
         const item_dev it = uniform_at(L.items, item);
-        const seg_dev S = uniform_at(L.segs, it.seg);
-        const job_dev J = uniform_at(L.jobs, S.job);
+        // the segment's and the job's fields are read where they are used (scalar
+        // loads from the constant address space) instead of being held in SGPRs
+        // all along, which spilled them to VGPR lanes
+        const seg_dev& S = *(const seg_dev*)((const __attribute__((address_space(4))) seg_dev*)L.segs + it.seg);
+        const job_dev& J = *(const job_dev*)((const __attribute__((address_space(4))) job_dev*)L.jobs + S.job);
         const uint32_t c2 = (S.chr << 16) | S.so;
         uint32_t acc = 0;  // accepted templates (copy-number events)
         load_tables(J, s_tab, tid, BLOCK);
