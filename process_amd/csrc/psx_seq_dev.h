@@ -138,6 +138,8 @@ __device__ __forceinline__ void load_tables(const job_dev& J, TAB& t, uint32_t t
         for (uint32_t i = tid; i < (1u << GB); i += block) t.guide[i] = J.ins_guide[i << (PSX_GUIDE_BITS - GB)];
     }
     for (uint32_t i = tid; i < J.err_n; i += block) t.err[i] = J.err_T[i];
+    // the error-count pick reads thresholds 0..3 whatever the table's length
+    for (uint32_t i = J.err_n + tid; i < 4u; i += block) t.err[i] = 0xFFFFFFFFu;
 }
 
 // LDS writes of a wave visible to its other lanes.
@@ -368,11 +370,15 @@ __device__ __forceinline__ terr template_errors(const job_dev& J, const TAB& tab
                                                 uint32_t c2, bool want) {
     const psx_u4 w = psx_draw_u(J.seed, r, j, c2, PSX_P_ERROR << 24);
     // K = err_k0 + the number of thresholds k < err_n - 1 with u >= T[k] (T is
-    // non-decreasing); only K <= 3 matters here, so four compares against the
-    // first thresholds (read by every lane alike) give min(K, 4) without a loop
+    // non-decreasing, every kept threshold but the last below 2^32, the last and
+    // the padding 2^32 - 1); only K <= 3 matters here, so four compares against
+    // the first thresholds (read by every lane alike) give min(K, 4) without a
+    // loop, and capping at err_n - 1 drops the last threshold and the padding,
+    // which only u = 2^32 - 1 reaches
     uint32_t k = 0;
 #pragma unroll
-    for (uint32_t q = 0; q < 4u; ++q) k += (q + 1u < J.err_n && w.y >= tab.err[min(q, J.err_n - 1u)]) ? 1u : 0u;
+    for (uint32_t q = 0; q < 4u; ++q) k += w.y >= tab.err[q] ? 1u : 0u;
+    k = min(k, J.err_n - 1u);
     const uint32_t K = want ? J.err_k0 + k : 0u;
     const uint32_t b0 = psx_mulhi32(w.x, J.err_N), b1 = psx_mulhi32(w.z, J.err_N), b2 = psx_mulhi32(w.w, J.err_N);
     const bool slow = K > 3u || (K >= 2u && b1 == b0) || (K >= 3u && (b2 == b0 || b2 == b1));
