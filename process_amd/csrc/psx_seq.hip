@@ -21,12 +21,14 @@ struct seq_batch {
     std::vector<pres_dev> pres;
     uint64_t n_reads = 0, pres_total = 0;
     uint64_t n_items = 0;  // written on the device (k_seq_items) from the segments
+    uint64_t n_items_sparse = 0;  // the first items: those of sparse jobs (layout)
     uint32_t spi = 1;      // strata per item
     scratch_builder sb;
     std::vector<size_t> tab_off;  // per job: insert thresholds, insert guide, error thresholds
     std::vector<size_t> cna_off;  // per job: copy-number events (SIZE_MAX: none)
     std::vector<uint64_t> job_reads;  // per job: reads of the sampling model without copy-number events
-    double cover = 0.0;  // largest expected number of covered positions per 64 templates over the jobs
+    std::vector<double> job_cover;  // per job: expected covered positions per 64 templates (0: no errors)
+    bool cna_sparse = false, cna_dense = false;  // copy-number events among the sparse / the other jobs
     size_t o_jobs = 0, o_segs = 0, o_items = 0, o_pres = 0, o_acc = 0;
     uint8_t* dev = nullptr;       // device base of the last upload
 };
@@ -147,6 +149,7 @@ void prepare_job(psx_ctx* ctx, seq_batch& P, const job_in& in, uint32_t spi) {
     }
     P.cna_off.push_back(cdev.empty() ? SIZE_MAX : P.sb.put_vec(cdev));
     P.jobs.push_back(J);
+    double cover = 0.0;
     if (p->error_rate > 0.0) {
         // expected positions covered by 64 templates: records per base x bases read, times the share of
         // templates kept under copy numbers (ploidy / kmax, length-weighted)
@@ -157,8 +160,9 @@ void prepare_job(psx_ctx* ctx, seq_batch& P, const job_in& in, uint32_t spi) {
             bases += (double)g->len[c];
             kept += (double)g->len[c] * (double)ploidy / (double)c_kmax[c];
         }
-        if (bases > 0.0) P.cover = std::max(P.cover, 64.0 * (pe ? 2.0 : 1.0) * L * recs / bases * kept / bases);
+        if (bases > 0.0) cover = 64.0 * (pe ? 2.0 : 1.0) * L * recs / bases * kept / bases;
     }
+    P.job_cover.push_back(cover);
     const uint64_t reads_before = P.n_reads;
     for (uint32_t so = 0; so < n_out; ++so) {
         const bool normal = so == n_req;
@@ -187,17 +191,30 @@ void prepare_job(psx_ctx* ctx, seq_batch& P, const job_in& in, uint32_t spi) {
             S.n_bucket = (uint32_t)(m->bucket_off[c + 1] - m->bucket_off[c]);
             S.leaf_off = leaf_off; S.n_cells = n_cells;
             P.n_reads += S.T * (pe ? 2u : 1u);  // template draws: an upper bound with copy-number events
-            S.first_item = (uint32_t)P.n_items;
-            if (S.T) P.n_items += (S.J + spi - 1) / spi;
-            if (P.n_items >= (uint64_t(1) << 32)) psx_throw(PSX_EINVAL, "too many work items in one batch");
-            P.segs.push_back(S);
+            P.segs.push_back(S);  // its work items are numbered in layout()
         }
     }
     P.job_reads.push_back(P.n_reads - reads_before);
 }
 
-// Places the descriptors of a prepared batch after the per-job tables.
+// A job whose reads cover fewer than 48 positions per 64 templates (or that
+// draws no errors) counts with errors checked per covered position.
+bool job_sparse(const seq_batch& P, uint32_t job) { return P.job_cover[job] < 48.0; }
+
+// Numbers the work items (the sparse jobs' segments first, so that each kind
+// of job is one range of items for its own kernel) and places the descriptors
+// of a prepared batch after the per-job tables.
 void layout(seq_batch& P) {
+    std::stable_partition(P.segs.begin(), P.segs.end(), [&](const seg_dev& S) { return job_sparse(P, S.job); });
+    P.n_items = 0;
+    for (seg_dev& S : P.segs) {
+        S.first_item = (uint32_t)P.n_items;
+        if (S.T) P.n_items += (S.J + P.spi - 1) / P.spi;
+        if (P.n_items >= (uint64_t(1) << 32)) psx_throw(PSX_EINVAL, "too many work items in one batch");
+        if (job_sparse(P, S.job)) P.n_items_sparse = P.n_items;
+    }
+    for (uint32_t k = 0; k < (uint32_t)P.jobs.size(); ++k)
+        if (P.cna_off[k] != SIZE_MAX) (job_sparse(P, k) ? P.cna_sparse : P.cna_dense) = true;
     const std::vector<uint64_t> zeros(P.jobs.size(), 0);
     P.o_acc = P.sb.put_vec(zeros);
     P.o_segs = P.sb.put_vec(P.segs);
@@ -297,13 +314,25 @@ void batch_launch(const seq_batch* P, const seq_lane& lane, const psx_launch_con
     const seq_launch L = launch_desc(*P, lane.desc, lane.queue);
     const pres_dev* ps = (const pres_dev*)(lane.desc + P->o_pres);
     const uint64_t want = (uint64_t)lane.n_cu * cfg.blocks_per_cu;
-    const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, P->n_items));
     const uint32_t init_grid = (uint32_t)std::max<uint64_t>(1, (P->pres_total + 255) / 256);
     hipLaunchKernelGGL(k_seq_init, dim3(init_grid), dim3(256), 0, lane.stream, L.jobs, ps, (uint32_t)P->pres.size(),
                        P->pres_total, L.queue, (uint32_t)P->jobs.size());
     if (timed) PSX_HIP(hipEventRecord(lane.ev0, lane.stream));
-    // errors per covered position when every job's reads cover fewer than 48 positions per 64 templates
-    launch_count(lane, cfg, L, grid, batch_has_cna(P), P->cover < 48.0);
+    // the sparse jobs' items (errors checked per covered position), then the others' (errors per template), each
+    // range from its own counter
+    const uint64_t ns = P->n_items_sparse, nd = P->n_items - ns;
+    if (ns || !nd) {
+        seq_launch Ls = L;
+        Ls.n_items = ns;
+        launch_count(lane, cfg, Ls, (uint32_t)std::max<uint64_t>(1, std::min(want, ns)), P->cna_sparse, true);
+    }
+    if (nd) {
+        seq_launch Ld = L;
+        Ld.items = L.items + ns;
+        Ld.n_items = nd;
+        Ld.queue = L.queue + 1;
+        launch_count(lane, cfg, Ld, (uint32_t)std::max<uint64_t>(1, std::min(want, nd)), P->cna_dense, false);
+    }
     PSX_HIP(hipGetLastError());
     if (timed) PSX_HIP(hipEventRecord(lane.ev1, lane.stream));
 }

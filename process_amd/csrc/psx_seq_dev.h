@@ -1125,7 +1125,8 @@ __global__ void __launch_bounds__(256) k_seq_init(const job_dev* __restrict__ jo
                                                   unsigned long long* __restrict__ queue, uint32_t n_jobs) {
     const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q == 0) {
-        *queue = 0ull;
+        queue[0] = 0ull;  // the sparse jobs' item counter
+        queue[1] = 0ull;  // the others'
         for (uint32_t k = 0; k < n_jobs; ++k)
             if (jobs[k].acc) *jobs[k].acc = 0ull;
     }
