@@ -376,3 +376,33 @@ def test_cna_replay_and_batch(ctx, cna_world):
     assert n == on + pn
     with pytest.raises(PsxError, match="copy-number"):
         core.simulate_seq_reads(ctx, w.g, w.pf, rm, 1, **kw)
+
+
+def test_batch_mixes_sparse_and_dense_jobs(ctx, world, cna_world):
+    """One batched call with jobs whose reads cover few positions (errors
+    checked per covered position) and jobs that cover many (errors per
+    template), with and without copy-number events: the call runs one launch
+    per kind, and every job matches the oracle."""
+    say("[parity] batched launch mixing sparse and dense jobs, plain and copy-number")
+    dense = build_world(ctx, [6_000], 8, dict(n_cells=200, n_samples=2, sample_size=50, seed=5),
+                        dict(snv=1e-4, indel=1e-5), 41)
+    cw = cna_world
+    spec = [(world, None, 31, cases.SEQ_CASES["pe_errors"]),               # sparse, plain
+            (dense, None, 32, cases.SEQ_CASES["pe_errors"]),               # dense, plain
+            (cw, cw.oev, 33, CNA_SEQ["pe_purity"]),                        # dense, copy numbers
+            (cw, cw.oev, 34, CNA_SEQ["se_sparse"]),                        # sparse, copy numbers
+            (dense, None, 35, cases.SEQ_CASES["pe_high_error"]),           # dense, plain
+            (world, None, 36, cases.SEQ_CASES["se_errorless"])]            # no errors
+    for cfg in (None, A.LaunchConfig(256, 1024, 2, 8, 0)):
+        jobs = [core.SeqJob(ctx, w.g, w.pf, w.pm, seed, **kw) for w, ev, seed, kw in spec]
+        for j in jobs:
+            for buf in (j.alt, j.depth, j.present):
+                buf.fill(0xA5)
+        n = core.simulate_seq_batch(ctx, jobs, cfg)
+        total = 0
+        for (w, ev, seed, kw), j in zip(spec, jobs):
+            a, d, p = j.results()
+            oa, od, op, on = O.simulate_seq(w.lens, w.of, w.om, seed, cna=ev, **kw)
+            total += on
+            assert np.array_equal(d, od) and np.array_equal(a, oa) and np.array_equal(p, op), (seed, cfg)
+        assert n == total
