@@ -309,8 +309,9 @@ def main() -> int:
                     help="only run device-to-device copies of BYTES (reference for the HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
-              "see DESIGN.md for how the path would shard.", file=sys.stderr)
+        print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}). "
+              "Sequencing would shard by whole (output sample, chromosome) segments, one share per GPU, each "
+              "writing its own rows of the alt/depth tables.", file=sys.stderr)
         return 2
 
     from process_amd import core, workloads
