@@ -3,7 +3,7 @@
  *
  * A plain-C restatement of the sampling model of mutation placement and
  * count-only sequencing. Only tests/, the smoke check and the CPU-baseline
- * leg of bench.py may load it, and only as the checker: the product library
+ * leg of bench.py may load it, and only to check results against: the product library
  * never calls it. Structurally it is the naive form (one read at a time,
  * binary search over the whole mutation table, sequential placement), written
  * independently of the HIP kernels; it shares with them only the plain-data

@@ -1,7 +1,7 @@
 """ctypes loader of the CPU oracle (oracle/liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, the smoke check and the CPU
-baseline leg of bench.py, always as the checker or the CPU reference timing,
+baseline leg of bench.py, always to check results against or as the CPU timing,
 never by the product package.
 """
 from __future__ import annotations

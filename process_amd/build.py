@@ -1,6 +1,6 @@
 """Builds libpsx.so (gfx950 HIP kernels + host runtime) in-tree with hipcc.
 
-Run ``python -m process_amd.build``; ``__graft_entry__.build()`` calls it too.
+Run ``python -m process_amd.build``; the repository-level build() entry point calls it too.
 """
 from __future__ import annotations
 
