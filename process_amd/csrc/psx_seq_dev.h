@@ -378,7 +378,7 @@ __device__ __forceinline__ terr template_errors(const job_dev& J, const TAB& tab
     uint32_t k = 0;
 #pragma unroll
     for (uint32_t q = 0; q < 4u; ++q) k += w.y >= tab.err[q] ? 1u : 0u;
-    k = min(k, J.err_n - 1u);
+    k = min(k, J.err_n ? J.err_n - 1u : 0u);
     const uint32_t K = want ? J.err_k0 + k : 0u;
     const uint32_t b0 = psx_mulhi32(w.x, J.err_N), b1 = psx_mulhi32(w.z, J.err_N), b2 = psx_mulhi32(w.w, J.err_N);
     const bool slow = K > 3u || (K >= 2u && b1 == b0) || (K >= 3u && (b2 == b0 || b2 == b1));
@@ -728,7 +728,7 @@ __device__ __forceinline__ void count_mates_wave(const job_dev& J, const seg_dev
             if (PSX_INFO_HEAD(in) && (snv ? ob == PSX_INFO_ALT(in) : car)) v.add_alt(ri);
         }
         if (DEFER) {
-            const bool chk = on && (grp || (snv && PSX_INFO_HEAD(in)));
+            const bool chk = J.err_n && on && (grp || (snv && PSX_INFO_HEAD(in)));  // no errors: nothing to check
             const unsigned long long m = __ballot(chk);
             if (m) {
                 const uint32_t rank =
