@@ -338,8 +338,11 @@ typedef struct psx_launch_config {
     int32_t use_graph;         /* capture the launch sequence in a hipGraph */
 } psx_launch_config;
 
-/* All jobs in one persistent launch pulling (job, sample, chromosome,
- * strata) items from a device queue. cfg = NULL uses the tuned default. */
+/* All jobs in persistent launches pulling (job, sample, chromosome, strata)
+ * items from device queues: one launch for the jobs whose reads cover few
+ * mutation positions (errors checked per covered position) and one for the
+ * others (errors drawn per template); the same counts either way. cfg = NULL
+ * uses the tuned default. */
 psx_status psx_simulate_seq_batch(psx_ctx* ctx, const psx_seq_job* jobs, uint32_t n_jobs,
                                   const psx_launch_config* cfg, uint64_t* n_reads);
 /* A batched launch prepared once and run many times: the descriptors are
