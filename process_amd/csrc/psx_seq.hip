@@ -6,7 +6,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
 
 #include "psx_seq_dev.h"
 #include "psx_seq_host.h"
@@ -32,6 +34,55 @@ struct seq_batch {
     size_t o_jobs = 0, o_segs = 0, o_items = 0, o_pres = 0, o_acc = 0;
     uint8_t* dev = nullptr;       // device base of the last upload
 };
+
+// The compact insert-size and error-count tables depend only on their
+// parameters, jobs of a batch and requests of a server mostly share them, and
+// building one takes tens of microseconds, so the process keeps up to 256 of
+// each (copies are handed out; failures are not kept).
+struct table_cache {
+    std::mutex mu;
+    std::map<std::pair<uint32_t, uint32_t>, psx_ctable> ins;  // (mean, sd)
+    std::map<std::pair<uint32_t, uint64_t>, psx_ctable> err;  // (bases per template, bits of the rate)
+};
+static table_cache& tables() {
+    static table_cache c;
+    return c;
+}
+
+static psx_ctable insert_ctable(uint32_t mean, uint32_t sd) {
+    table_cache& c = tables();
+    const auto key = std::make_pair(mean, sd);
+    {
+        std::lock_guard<std::mutex> g(c.mu);
+        auto it = c.ins.find(key);
+        if (it != c.ins.end()) return it->second;
+    }
+    psx_ctable t = psx_compact_table(psx_insert_table(mean, sd));  // throws on invalid parameters
+    std::lock_guard<std::mutex> g(c.mu);
+    if (c.ins.size() >= 256) c.ins.clear();
+    c.ins.emplace(key, t);
+    return t;
+}
+
+// The error-count table of N bases per template at rate e; throws when the
+// rate is too high for N.
+static psx_ctable error_ctable(uint32_t N, double e, uint32_t L) {
+    table_cache& c = tables();
+    uint64_t bits;
+    std::memcpy(&bits, &e, sizeof bits);
+    const auto key = std::make_pair(N, bits);
+    {
+        std::lock_guard<std::mutex> g(c.mu);
+        auto it = c.err.find(key);
+        if (it != c.err.end()) return it->second;
+    }
+    if (!psx_error_rate_ok(N, e)) psx_throw(PSX_EINVAL, "error rate too high for %u-base reads", L);
+    psx_ctable t = psx_compact_table(psx_error_table(N, e));
+    std::lock_guard<std::mutex> g(c.mu);
+    if (c.err.size() >= 256) c.err.clear();
+    c.err.emplace(key, t);
+    return t;
+}
 
 // Largest copy number of any leaf on a chromosome (at least the ploidy): the
 // copy numbers change only at event boundaries, in leaf rank and position.
@@ -82,17 +133,13 @@ void prepare_job(psx_ctx* ctx, seq_batch& P, const job_in& in, uint32_t spi) {
     psx_ctable ins, err;
     uint32_t reach = L;
     if (pe) {
-        ins = psx_compact_table(psx_insert_table(p->insert_size_mean, p->insert_size_stddev));
+        ins = insert_ctable(p->insert_size_mean, p->insert_size_stddev);
         if (ins.T.size() > MAX_INS)
             psx_throw(PSX_EINVAL, "the insert-size distribution is too wide (sd %u)", p->insert_size_stddev);
         reach = std::max(reach, ins.k0 + (uint32_t)ins.T.size() - 1);
         if (reach > MAX_READ) psx_throw(PSX_EINVAL, "insert sizes above %u are not supported", MAX_READ);
     }
-    if (p->error_rate > 0.0) {
-        if (!psx_error_rate_ok((pe ? 2u : 1u) * L, p->error_rate))
-            psx_throw(PSX_EINVAL, "error rate too high for %u-base reads", L);
-        err = psx_compact_table(psx_error_table((pe ? 2u : 1u) * L, p->error_rate));
-    }
+    if (p->error_rate > 0.0) err = error_ctable((pe ? 2u : 1u) * L, p->error_rate, L);
     std::vector<uint8_t> sel(g->n_chr, p->n_chromosomes == 0 ? 1 : 0);
     for (uint32_t k = 0; k < p->n_chromosomes; ++k) {
         if (p->chromosomes[k] >= g->n_chr) psx_throw(PSX_EINVAL, "unknown chromosome %u", p->chromosomes[k]);
