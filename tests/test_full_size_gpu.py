@@ -115,3 +115,29 @@ def test_full_size_reads_match_oracle(ctx, large):
         bad = np.nonzero((rows != exp).any(axis=1))[0]
         assert len(bad) == 0, (so, len(bad), bad[:3])
     out.close()
+
+
+def test_plain_small_whole_table_matches_oracle(ctx):
+    """The small plain configuration (5,000-cell forest, one 2,500-cell sample
+    and the normal sample, 10 Mb, 80x paired-end, 1.07e7 reads): placement and
+    the whole count table against the oracle, and the materialised reads."""
+    say("[full] plain-small: placement and whole table against the oracle")
+    w = workloads.plain_small()
+    b = workloads.build(ctx, w)
+    lens = np.asarray(w.lengths, np.uint32)
+    packed = O.genome_synth(lens, w.genome_seed)
+    assert np.array_equal(b.genome.download(), packed)
+    fd = w.forest
+    of = O.Forest(fd["cell_id"], fd["parent"], fd["species"], fd["sample"], fd["birth"], fd["n_samples"])
+    om = O.place_mutations(lens, packed, of, w.place_seed, **w.engine)
+    assert b.muts.download().tobytes() == om.tobytes()
+    a, d, p, n = core.simulate_seq_arrays(ctx, b.genome, b.forest, b.muts, w.seq_seed, **w.seq)
+    oa, od, op, on = O.simulate_seq(lens, of, om, w.seq_seed, **w.seq)
+    assert n == on and n > 10_000_000
+    assert np.array_equal(d, od.reshape(d.shape)) and np.array_equal(a, oa.reshape(a.shape))
+    assert np.array_equal(p, op.reshape(p.shape))
+    out, got = core.simulate_seq_reads(ctx, b.genome, b.forest, b.muts, w.seq_seed, **w.seq)
+    rows = out.numpy()[:got]
+    out.close()
+    ref = O.simulate_reads(lens, packed, of, om, w.seq_seed, got, **w.seq)
+    assert np.array_equal(rows, ref)
