@@ -422,6 +422,23 @@ extern "C" psx_status psx_forest_upload(psx_ctx* ctx, const psx_forest_desc* des
         upload_vec(ctx, f->d_species, f->species);
         upload_vec(ctx, f->d_parent, f->parent);
         upload_vec(ctx, f->d_sample_leaves, f->sample_leaves);
+        // membership of every leaf rank in every sample, with running counts: whether a sample holds a leaf of
+        // a rank range [lo, hi) is count(hi) > count(lo), two lookups
+        f->leaf_blocks = f->n_leaves / 64u + 1u;
+        std::vector<uint32_t> lc((size_t)f->n_samples * f->leaf_blocks * 4u, 0u);
+        for (uint32_t smp = 0; smp < f->n_samples; ++smp) {
+            uint32_t* e = lc.data() + (size_t)smp * f->leaf_blocks * 4u;
+            for (uint32_t k = f->sample_begin[smp]; k < f->sample_begin[smp + 1]; ++k) {
+                const uint32_t r = f->sample_leaves[k];
+                e[(r >> 6) * 4u + 1u + ((r >> 5) & 1u)] |= 1u << (r & 31u);
+            }
+            uint32_t run = 0;
+            for (uint32_t b = 0; b < f->leaf_blocks; ++b) {
+                e[b * 4u] = run;
+                run += (uint32_t)__builtin_popcount(e[b * 4u + 1u]) + (uint32_t)__builtin_popcount(e[b * 4u + 2u]);
+            }
+        }
+        upload_vec(ctx, f->d_leaf_count, lc);
         PSX_HIP(hipStreamSynchronize(ctx->stream));
     });
     if (st != PSX_OK) { delete f; return st; }

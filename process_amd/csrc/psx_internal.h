@@ -101,6 +101,10 @@ struct psx_forest {
     std::vector<uint32_t> sample_begin, sample_leaves;
     // device copies
     dbuf d_leaf_lo, d_leaf_hi, d_species, d_parent, d_sample_leaves;
+    // per sample, per block of 64 leaf ranks: {sample leaves below the block, the block's membership bits (low,
+    // high word), 0}; leaf_blocks = n_leaves / 64 + 1 entries per sample, so rank n_leaves has an entry too
+    uint32_t leaf_blocks = 0;
+    dbuf d_leaf_count;
 };
 
 // Mutation table in SID order: device SoA for the kernels, AoS for download.

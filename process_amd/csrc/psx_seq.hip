@@ -161,6 +161,8 @@ void prepare_job(psx_ctx* ctx, seq_batch& P, const job_in& in, uint32_t spi) {
     J.pos = m->d_pos.as<uint32_t>(); J.lo = m->d_lo.as<uint32_t>(); J.hi = m->d_hi.as<uint32_t>();
     J.info = m->d_info.as<uint32_t>(); J.bucket = m->d_bucket.as<uint32_t>();
     J.sample_leaves = f->d_sample_leaves.as<uint32_t>();
+    J.leaf_count = f->d_leaf_count.as<uint4>();
+    J.leaf_blocks = f->leaf_blocks;
     J.alt = in.alt; J.depth = in.depth; J.present = in.present;
     J.packed = g->packed.as<uint32_t>();
     J.n_mut = m->n;
@@ -216,7 +218,7 @@ void prepare_job(psx_ctx* ctx, seq_batch& P, const job_in& in, uint32_t spi) {
         const uint32_t leaf_off = normal ? 0u : f->sample_begin[req[so]];
         const uint32_t n_cells = normal ? 0u : f->sample_begin[req[so] + 1] - f->sample_begin[req[so]];
         if (m->n) {
-            P.pres.push_back(pres_dev{P.pres_total, job, so, leaf_off, n_cells});
+            P.pres.push_back(pres_dev{P.pres_total, job, so, leaf_off, n_cells, normal ? 0u : req[so], 0u});
             P.pres_total += m->n;
         }
         for (uint32_t c = 0; c < g->n_chr; ++c) {

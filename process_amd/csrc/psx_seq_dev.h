@@ -46,6 +46,7 @@ struct job_dev {
     const uint32_t* info;
     const uint32_t* bucket;
     const uint32_t* sample_leaves;
+    const uint4* leaf_count;     // per sample and 64 leaf ranks: {leaves below, membership bits, 0}
     int32_t* alt;
     int32_t* depth;
     uint8_t* present;
@@ -62,6 +63,7 @@ struct job_dev {
     uint32_t L, pe, ploidy, n_leaves, preneo, reach;
     uint32_t n_out;
     uint32_t purity_lt1;
+    uint32_t leaf_blocks;        // leaf_count entries per sample
     const psx_cna_dev* cna;      // copy-number events of every chromosome, per segment ranges
     unsigned long long* acc;     // reads of accepted templates (jobs with copy-number events)
 };
@@ -104,6 +106,7 @@ __global__ void __launch_bounds__(256) k_seq_items(const seg_dev* __restrict__ s
 struct pres_dev {  // one per (job, output sample) for the init kernel
     uint64_t begin;        // global index of the first (so, m) entry
     uint32_t job, so, leaf_off, n_cells;
+    uint32_t sample, pad;  // the forest sample of the output sample (tumour samples)
 };
 
 struct seq_launch {
@@ -1096,14 +1099,14 @@ __device__ __forceinline__ bool record_present(const job_dev& J, const pres_dev&
     } else {
         pres = J.purity_lt1 && normal_has;
         if (!pres) {
-            const uint32_t lo = J.lo[m], hi = J.hi[m];
-            const uint32_t* sl = J.sample_leaves + P.leaf_off;
-            uint32_t x = 0, y = P.n_cells;
-            while (x < y) {
-                const uint32_t mid = (x + y) >> 1;
-                if (sl[mid] < lo) x = mid + 1; else y = mid;
-            }
-            pres = x < P.n_cells && sl[x] < hi;
+            // sample leaves of rank below x, from the sample's running counts
+            const uint4* lc = J.leaf_count + (uint64_t)P.sample * J.leaf_blocks;
+            auto below = [&](uint32_t x) {
+                const uint4 e = lc[x >> 6];
+                const uint64_t bits = (uint64_t)e.y | ((uint64_t)e.z << 32);
+                return e.x + (uint32_t)__popcll(bits & ((1ull << (x & 63u)) - 1ull));
+            };
+            pres = below(J.hi[m]) > below(J.lo[m]);
         }
     }
     return pres;
