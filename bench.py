@@ -31,7 +31,7 @@ HBM_PEAK_GBS = 8000.0
 # WRITE_SIZE runs calibrated against a device copy, and their issue-slot use from SQ counters (the rocprofv3
 # summaries under profiles/, named for the kernel and workload). Count-only sequencing is bound by instruction
 # issue, not HBM.
-PROFILED = {"k_seq_count": dict(traffic=302.0e6, valu_per_cu_cycle=1.10, salu_per_cu_cycle=0.68),
+PROFILED = {"k_seq_count": dict(traffic=302.1e6, valu_per_cu_cycle=1.09, salu_per_cu_cycle=0.68),
             "k_seq_reads": dict(traffic=77.67e9, valu_per_cu_cycle=0.69, salu_per_cu_cycle=0.38)}
 
 
