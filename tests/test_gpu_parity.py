@@ -406,3 +406,23 @@ def test_batch_mixes_sparse_and_dense_jobs(ctx, world, cna_world):
             total += on
             assert np.array_equal(d, od) and np.array_equal(a, oa) and np.array_equal(p, op), (seed, cfg)
         assert n == total
+
+
+def test_empty_table_and_zero_coverage(ctx):
+    """A mutation table with no records (every read is the reference) and a
+    run with zero coverage (no reads), in both modes."""
+    say("[parity] empty mutation table; zero coverage")
+    w = build_world(ctx, [9_000, 3_000], 12, dict(n_cells=60, n_samples=1, sample_size=20, seed=2),
+                    dict(snv=0.0, indel=0.0, pre_snv=0, pre_indel=0, drivers=False, germline=False), 5)
+    assert len(w.om) == 0 and w.pm.n == 0
+    kw = cases.SEQ_CASES["pe_errors"]
+    a, d, p, n = core.simulate_seq_arrays(ctx, w.g, w.pf, w.pm, 7, **kw)
+    oa, od, op, on = O.simulate_seq(w.lens, w.of, w.om, 7, **kw)
+    assert n == on and n > 0 and a.size == 0 and d.size == 0
+    reads = _reads(ctx, w, 7, kw)
+    assert np.array_equal(reads, _oracle_reads(w, 7, kw, len(reads)))
+    zero = dict(kw, coverage=0)
+    a, d, p, n = core.simulate_seq_arrays(ctx, w.g, w.pf, w.pm, 7, **zero)
+    assert n == 0
+    out, got = core.simulate_seq_reads(ctx, w.g, w.pf, w.pm, 7, **zero)
+    assert got == 0
