@@ -426,3 +426,21 @@ def test_empty_table_and_zero_coverage(ctx):
     assert n == 0
     out, got = core.simulate_seq_reads(ctx, w.g, w.pf, w.pm, 7, **zero)
     assert got == 0
+
+
+def test_longest_reads_and_ragged_chromosomes(ctx):
+    """Reads of the largest supported size (4096 bases): a chromosome exactly
+    one read long (one start), one shorter than a read (no reads), one of a
+    length that is no multiple of 16; single and paired ends at the largest
+    reach; both modes."""
+    say("[parity] 4096-base reads on chromosomes of 4096, 4095 and 5003 bases")
+    w = build_world(ctx, [4_096, 4_095, 5_003], 14, dict(n_cells=80, n_samples=1, sample_size=30, seed=6),
+                    dict(snv=3e-4, indel=3e-5), 9)
+    for kw in (dict(coverage=30, read_size=4096, error_rate=1e-3),
+               dict(coverage=12, read_size=2000, insert_size_mean=3000, insert_size_stddev=30, error_rate=2e-3)):
+        a, d, p, n = core.simulate_seq_arrays(ctx, w.g, w.pf, w.pm, 10, **kw)
+        oa, od, op, on = O.simulate_seq(w.lens, w.of, w.om, 10, **kw)
+        assert n == on and n > 0
+        assert np.array_equal(d, od) and np.array_equal(a, oa) and np.array_equal(p, op), kw
+        reads = _reads(ctx, w, 10, kw)
+        assert np.array_equal(reads, _oracle_reads(w, 10, kw, len(reads))), kw
