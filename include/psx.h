@@ -375,6 +375,16 @@ psx_status psx_seq_table(const psx_mutation* recs, uint64_t n_mutations,
                          uint64_t* n_rows, uint64_t* row_rec, int32_t* occ, int32_t* cov,
                          double* vaf);
 
+/* The VAF column of get_result_dataframe (src/seq_simulation.cpp:123-130) in
+ * fp32 on the device, for every record of the device arrays of
+ * psx_simulate_seq_device ([n_out_samples][n_mutations]): at a SID head,
+ * alt / depth (IEEE fp32 division) when the sample holds the SID (an
+ * alternative read, or a record of the SID present in the sample; NaN when
+ * depth is 0), else 0; at the other records of a SID, 0. d_vaf is device
+ * memory of n_out_samples x n_mutations floats. */
+psx_status psx_seq_vaf_f32(psx_ctx* ctx, const psx_mutations* m, uint32_t n_out_samples, const int32_t* d_alt,
+                           const int32_t* d_depth, const uint8_t* d_present, float* d_vaf);
+
 /* ------------------------------------------------------------------ serving */
 
 /* A request loop over device-resident genomes, forests and mutation tables

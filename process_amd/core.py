@@ -341,6 +341,17 @@ def simulate_seq_device(ctx: Context, genome: Genome, forest: Forest, muts: Muta
     return nr.value
 
 
+def seq_vaf_f32(ctx: Context, muts: Mutations, alt: DeviceArray, depth: DeviceArray,
+                present: DeviceArray) -> DeviceArray:
+    """The result table's VAF in fp32 on the device for every record of the
+    [n_out_samples, n_mutations] device arrays of simulate_seq_device (SID
+    heads; 0 elsewhere)."""
+    S = alt.shape[0]
+    out = DeviceArray(ctx, (S, max(muts.n, 1)), np.float32)
+    check(lib().psx_seq_vaf_f32(ctx.h, muts.h, S, alt.ptr, depth.ptr, present.ptr, out.ptr), ctx.h)
+    return out
+
+
 def read_words(**seq) -> int:
     p, keep = A.seq_params(**seq)
     return int(lib().psx_read_words(C.byref(p)))

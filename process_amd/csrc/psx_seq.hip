@@ -507,6 +507,39 @@ extern "C" psx_status psx_simulate_seq_device(psx_ctx* ctx, const psx_genome* g,
     });
 }
 
+// fp32 VAF per (output sample, record): the SID's alternative reads over its
+// depth at a SID head when the sample holds the SID, else 0 (psx_seq_table's
+// rule, without the row filter).
+__global__ void __launch_bounds__(256) k_seq_vaf(const uint32_t* __restrict__ info, uint64_t n, uint64_t total,
+                                                 const int32_t* __restrict__ alt, const int32_t* __restrict__ depth,
+                                                 const uint8_t* __restrict__ present, float* __restrict__ vaf) {
+    const uint64_t q = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    if (q >= total) return;
+    const uint64_t m = q % n, base = q - m;
+    float v = 0.0f;
+    if (PSX_INFO_HEAD(info[m])) {
+        bool held = alt[q] > 0;
+        for (uint64_t k = m; k < n && !held && (k == m || !PSX_INFO_HEAD(info[k])); ++k) held = present[base + k] != 0;
+        if (held) v = (float)alt[q] / (float)depth[q];
+    }
+    vaf[q] = v;
+}
+
+extern "C" psx_status psx_seq_vaf_f32(psx_ctx* ctx, const psx_mutations* m, uint32_t n_out_samples,
+                                      const int32_t* d_alt, const int32_t* d_depth, const uint8_t* d_present,
+                                      float* d_vaf) {
+    if (!ctx || !m || (m->n && (!d_alt || !d_depth || !d_present || !d_vaf))) return PSX_EINVAL;
+    return psx_guard(ctx, [&] {
+        const uint64_t total = (uint64_t)n_out_samples * m->n;
+        if (!total) return;
+        PSX_HIP(hipSetDevice(ctx->device));
+        hipLaunchKernelGGL(k_seq_vaf, dim3((uint32_t)((total + 255) / 256)), dim3(256), 0, ctx->stream,
+                           m->d_info.as<uint32_t>(), m->n, total, d_alt, d_depth, d_present, d_vaf);
+        PSX_HIP(hipGetLastError());
+        PSX_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
 extern "C" psx_status psx_simulate_seq(psx_ctx* ctx, const psx_genome* g, const psx_forest* f,
                                        const psx_mutations* m, const psx_seq_params* p, uint64_t seed, int32_t* alt,
                                        int32_t* depth, uint8_t* present, uint64_t* n_reads) {

@@ -444,3 +444,50 @@ def test_longest_reads_and_ragged_chromosomes(ctx):
         assert np.array_equal(d, od) and np.array_equal(a, oa) and np.array_equal(p, op), kw
         reads = _reads(ctx, w, 10, kw)
         assert np.array_equal(reads, _oracle_reads(w, 10, kw, len(reads))), kw
+
+
+def _expected_vaf32(om, a, d, p):
+    """The table's VAF rule per record in fp32 (numpy's IEEE division)."""
+    S, M = a.shape
+    head = om["sid_head"] != 0
+    exp = np.zeros((S, M), np.float32)
+    heads = np.nonzero(head)[0]
+    ends = np.append(heads[1:], M)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        for i, e in zip(heads, ends):
+            held = (a[:, i] > 0) | (p[:, i:e] != 0).any(axis=1)
+            exp[held, i] = np.float32(a[held, i]) / np.float32(d[held, i])
+    return exp
+
+
+def test_vaf_f32_on_device(ctx, world):
+    """The result table's VAF in fp32 on the device: bit for bit the fp32
+    quotient (NaN where a held SID has no depth), 0 off SID heads and where
+    the sample does not hold the SID; within one fp32 rounding of the
+    double-precision table of psx_seq_table. Infinite and finite sites (SIDs
+    of several records)."""
+    say("[parity] fp32 VAF on the device")
+    fin = build_world(ctx, [4_000], 6, dict(n_cells=300, n_samples=2, sample_size=60, seed=4),
+                      dict(snv=2e-4, indel=2e-5, infinite_sites=False), 31)
+    for w, kw in ((world, cases.SEQ_CASES["purity"]), (fin, cases.SEQ_CASES["pe_high_error"])):
+        S = core.num_out_samples(w.pf, **kw)
+        M = w.pm.n
+        alt = core.DeviceArray(ctx, (S, M), np.int32)
+        dep = core.DeviceArray(ctx, (S, M), np.int32)
+        pres = core.DeviceArray(ctx, (S, M), np.uint8)
+        core.simulate_seq_device(ctx, w.g, w.pf, w.pm, 17, alt, dep, pres, **kw)
+        vd = core.seq_vaf_f32(ctx, w.pm, alt, dep, pres)
+        vaf = vd.numpy()[:, :M]
+        a, d, p = alt.numpy(), dep.numpy(), pres.numpy()
+        for x in (alt, dep, pres, vd):
+            x.close()
+        exp = _expected_vaf32(w.om, a, d, p)
+        assert np.array_equal(np.isnan(vaf), np.isnan(exp))
+        ok = ~np.isnan(exp)
+        assert np.array_equal(vaf[ok].view(np.uint32), exp[ok].view(np.uint32))
+        assert (vaf[ok] > 0).sum() > 0
+        rows, occ, cov, v64 = core.seq_table(w.om, a, d, p, include_non_sequenced=True)
+        v32 = vaf[:, rows.astype(np.int64)]
+        fin64 = ~np.isnan(v64)
+        assert np.array_equal(np.isnan(v32), ~fin64)
+        assert np.all(np.abs(v32[fin64].astype(np.float64) - v64[fin64]) <= 2.0 ** -24 * np.abs(v64[fin64]))
