@@ -556,3 +556,68 @@ def seq_to_long(seq_results) -> pd.DataFrame:
     out = out.rename(columns={"chr_pos": "from"})
     out["to"] = out["from"]
     return out
+
+
+def _wide(seq_results) -> pd.DataFrame:
+    return seq_results["mutations"] if isinstance(seq_results, dict) else seq_results
+
+
+def validate_chromosomes(seq_results, chromosomes=None) -> list:
+    """R/ggplot_config.R:75-94: the table's chromosomes in order of first
+    appearance when none are given, else the given ones, every one of which
+    must occur in the table."""
+    present = list(pd.unique(_wide(seq_results)["chr"]))
+    if chromosomes is None:
+        return present
+    chromosomes = [chromosomes] if isinstance(chromosomes, str) else list(chromosomes)
+    unknown = list(dict.fromkeys(c for c in chromosomes if c not in present))
+    if unknown:
+        names = ", ".join(str(c) for c in unknown)
+        msg = f"The chromosomes {names} are" if len(unknown) > 1 else f"The chromosome {names} is"
+        raise ValueError(msg + " not present in the sequence reference data.")
+    return chromosomes
+
+
+def get_seq_data(seq_results, sample, chromosomes=None) -> dict:
+    """R/plot_genome_wide_mutations.R:3-19: the long table restricted to the
+    chosen chromosomes (chr becomes a categorical in that order), split into
+    the sample's rows ("tumour") and the normal sample's rows ("normal")."""
+    chromosomes = validate_chromosomes(seq_results, chromosomes)
+    data = seq_to_long(_wide(seq_results))
+    data = data[data["chr"].isin(chromosomes)].copy()
+    data["chr"] = pd.Categorical(data["chr"], categories=chromosomes)
+    names = list(pd.unique(data["sample_name"]))
+    if sample not in names:
+        raise ValueError("Inserted 'sample' is not available, available samples are: " + ", ".join(names))
+    return {"tumour": data[data["sample_name"] == sample].reset_index(drop=True),
+            "normal": data[data["sample_name"] == "normal_sample"].reset_index(drop=True)}
+
+
+def seq_dr(seq_results, sample, chromosomes=None) -> pd.DataFrame:
+    """The data of R/plot_genome_wide_mutations.R:75-111 (plot_DR): the
+    sample's rows left-joined with the normal sample's on (chr, from, ref,
+    alt), DR = DP.tumour / DP.normal (inf or NaN where the normal depth is 0),
+    ordered by (chromosome, position) with abs_pos = 1..n. plot_DR draws a
+    random subset of N rows for display; every row is returned here."""
+    df = _wide(seq_results)
+    if not all(f"normal_sample.{k}" in df.columns for k in ("occurrences", "coverage", "VAF")):
+        raise ValueError('The parameter "seq_result" does not contain the mandatory normal sample "normal_sample".')
+    data = get_seq_data(df, sample, chromosomes)
+    d = data["tumour"].merge(data["normal"], how="left", on=["chr", "from", "ref", "alt"],
+                             suffixes=(".tumour", ".normal"))
+    d["DR"] = d["DP.tumour"].astype(np.float64) / d["DP.normal"].astype(np.float64)
+    d = d.sort_values(["chr", "from"], kind="mergesort").reset_index(drop=True)
+    d["abs_pos"] = np.arange(1, len(d) + 1)
+    return d
+
+
+def seq_baf(seq_results, sample, chromosomes=None, cuts=(0.0, 1.0)) -> pd.DataFrame:
+    """The data of R/plot_genome_wide_mutations.R:192-240 (plot_BAF): the
+    sample's rows ordered by (chromosome, position), abs_pos = 1..n numbered
+    before the VAF filter min(cuts) <= VAF <= max(cuts). plot_BAF draws a
+    random subset of N rows for display; every row is returned here."""
+    d = get_seq_data(seq_results, sample, chromosomes)["tumour"]
+    d = d.sort_values(["chr", "from"], kind="mergesort").reset_index(drop=True)
+    d["abs_pos"] = np.arange(1, len(d) + 1)
+    lo, hi = min(cuts), max(cuts)
+    return d[(d["VAF"] <= hi) & (d["VAF"] >= lo)].reset_index(drop=True)

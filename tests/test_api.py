@@ -40,6 +40,50 @@ def test_seq_to_long_reshapes_like_the_reference():
     assert list(long["from"]) == list(long["to"]) == [100, 200, 100, 200]
 
 
+def _dr_baf_table():
+    return pd.DataFrame({"chr": ["1", "2", "1", "2"], "chr_pos": [100, 50, 30, 10], "ref": ["A", "C", "G", "T"],
+                         "alt": ["T", "G", "A", "C"], "causes": ["SBS1"] * 4, "classes": ["germinal"] * 4,
+                         "S.occurrences": [30, 0, 5, 1], "S.coverage": [60, 0, 20, 4],
+                         "S.VAF": [0.5, 0.0, 0.25, 0.25], "normal_sample.occurrences": [15, 0, 4, 2],
+                         "normal_sample.coverage": [30, 0, 0, 8], "normal_sample.VAF": [0.5, 0.0, 0.0, 0.25]})
+
+
+def test_depth_ratio_data_like_plot_dr():
+    """R/plot_genome_wide_mutations.R:75-111 without the display subsample:
+    (chr, from) order by chromosome level, DR = DP.tumour / DP.normal."""
+    d = pa.seq_dr({"mutations": _dr_baf_table()}, "S")
+    assert list(zip(d["chr"], d["from"])) == [("1", 30), ("1", 100), ("2", 10), ("2", 50)]
+    assert list(d["abs_pos"]) == [1, 2, 3, 4]
+    assert d["DR"][0] == np.inf and d["DR"][1] == 2.0 and d["DR"][2] == 0.5 and np.isnan(d["DR"][3])
+    assert list(d["NV.tumour"]) == [5, 30, 1, 0] and list(d["NV.normal"]) == [4, 15, 2, 0]
+    # chromosome order given by the caller, one chromosome left out
+    d2 = pa.seq_dr(_dr_baf_table(), "S", chromosomes=["2"])
+    assert list(d2["from"]) == [10, 50] and list(d2["DR"][:1]) == [0.5]
+    d3 = pa.seq_dr(_dr_baf_table(), "S", chromosomes=["2", "1"])
+    assert list(zip(d3["chr"], d3["from"])) == [("2", 10), ("2", 50), ("1", 30), ("1", 100)]
+
+
+def test_baf_data_like_plot_baf():
+    """R/plot_genome_wide_mutations.R:192-240 without the display subsample:
+    abs_pos is numbered before the VAF cut."""
+    b = pa.seq_baf(_dr_baf_table(), "S", cuts=(0.4, 0.1))
+    assert list(b["from"]) == [30, 10] and list(b["abs_pos"]) == [1, 3]
+    assert len(pa.seq_baf(_dr_baf_table(), "S")) == 4
+
+
+def test_dr_baf_errors_like_the_reference():
+    """R/ggplot_config.R:82-90, R/plot_genome_wide_mutations.R:12-15,87-94."""
+    t = _dr_baf_table()
+    with pytest.raises(ValueError, match="^The chromosome X is not present in the sequence reference data.$"):
+        pa.seq_baf(t, "S", chromosomes=["1", "X"])
+    with pytest.raises(ValueError, match="^The chromosomes X, Y are not present"):
+        pa.seq_dr(t, "S", chromosomes=["X", "Y", "X"])
+    with pytest.raises(ValueError, match="available samples are: S, normal_sample$"):
+        pa.seq_baf(t, "T")
+    with pytest.raises(ValueError, match="mandatory normal sample"):
+        pa.seq_dr(t.drop(columns=["normal_sample.VAF"]), "S")
+
+
 def test_random_seed():
     assert pa.get_random_seed(5) == 5
     s = pa.get_random_seed(None)
@@ -107,6 +151,12 @@ def test_end_to_end_against_oracle(ctx):
     assert len(g) == 1 and g["classes"].iloc[0] == "germinal"
     assert 0.2 < g["normal_sample.VAF"].iloc[0] < 0.8
     assert set(df["classes"]) <= {"passenger", "driver", "germinal", "preneoplastic"}
+    # the DR / BAF consumers on the simulated table: one row per table row, depths taken from it
+    dr = pa.seq_dr(res, "S1")
+    assert len(dr) == len(df) and list(dr["abs_pos"]) == list(range(1, len(df) + 1))
+    key = dict(zip(zip(df["chr"], df["chr_pos"]), zip(df["S1.coverage"], df["normal_sample.coverage"])))
+    assert all(key[(c, f)] == (t, n) for c, f, t, n in zip(dr["chr"], dr["from"], dr["DP.tumour"], dr["DP.normal"]))
+    assert len(pa.seq_baf(res, "S2", chromosomes=["22"])) == int((df["chr"] == "22").sum())
 
 
 @pytest.mark.gpu
