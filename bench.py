@@ -309,7 +309,7 @@ def main() -> int:
                     help="only run device-to-device copies of BYTES (reference for the HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}). "
+        print(f"bench.py runs on one GPU: the library has no multi-GPU path (asked for --gpus {args.gpus}). "
               "Sequencing would shard by whole (output sample, chromosome) segments, one share per GPU, each "
               "writing its own rows of the alt/depth tables.", file=sys.stderr)
         return 2
