@@ -345,6 +345,16 @@ typedef struct psx_launch_config {
  * uses the tuned default. */
 psx_status psx_simulate_seq_batch(psx_ctx* ctx, const psx_seq_job* jobs, uint32_t n_jobs,
                                   const psx_launch_config* cfg, uint64_t* n_reads);
+/* Read materialisation of many jobs in one persistent launch (the batched
+ * form of psx_simulate_seq_reads; src/seq_simulation.cpp:536-549,564 writes
+ * one SAM set per simulate_seq call): job k's reads are rows
+ * [job_first[k], job_first[k + 1]) of d_reads, each job's rows exactly those
+ * psx_simulate_seq_reads writes for it alone. Every job needs the same
+ * psx_read_words; the jobs' d_alt, d_depth and d_present are not used.
+ * job_first (host, n_jobs + 1 entries) may be NULL. */
+psx_status psx_simulate_seq_reads_batch(psx_ctx* ctx, const psx_seq_job* jobs, uint32_t n_jobs,
+                                        uint32_t* d_reads, uint64_t capacity_reads, uint64_t* job_first,
+                                        uint64_t* n_reads);
 /* A batched launch prepared once and run many times: the descriptors are
  * built and uploaded at creation and, with cfg->use_graph (the default when
  * cfg is NULL), the launch is captured in a hipGraph that every run replays.

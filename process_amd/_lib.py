@@ -69,6 +69,8 @@ def _declare(lib):
         "psx_simulate_seq_reads": (st, [vp, vp, vp, vp, C.POINTER(A.SeqParams), C.c_uint64, vp, C.c_uint64,
                                         A.u64p]),
         "psx_simulate_seq_batch": (st, [vp, C.POINTER(A.SeqJob), C.c_uint32, C.POINTER(A.LaunchConfig), A.u64p]),
+        "psx_simulate_seq_reads_batch": (st, [vp, C.POINTER(A.SeqJob), C.c_uint32, vp, C.c_uint64, A.u64p,
+                                              A.u64p]),
         "psx_seq_plan_create": (st, [vp, C.POINTER(A.SeqJob), C.c_uint32, C.POINTER(A.LaunchConfig), pp]),
         "psx_seq_plan_run": (st, [vp, A.u64p]),
         "psx_seq_plan_free": (None, [vp]),

@@ -371,6 +371,22 @@ def simulate_seq_reads(ctx: Context, genome: Genome, forest: Forest, muts: Mutat
     return out, got.value
 
 
+def simulate_seq_reads_batch(ctx: Context, jobs: list, out: DeviceArray | None = None):
+    """Materialised reads of every job in one launch: (buffer [n_reads,
+    read_words], job_first with len(jobs) + 1 entries, n_reads); job k's reads
+    are rows job_first[k]:job_first[k + 1]."""
+    n = sum(num_reads(j.genome, j.forest, **j.seq) for j in jobs)
+    rw = read_words(**jobs[0].seq) if jobs else 1
+    if out is None:
+        out = DeviceArray(ctx, (max(n, 1), rw), np.uint32)
+    arr, keep = _job_array(jobs)
+    first = (C.c_uint64 * (len(jobs) + 1))()
+    got = C.c_uint64()
+    check(lib().psx_simulate_seq_reads_batch(ctx.h, arr, len(jobs), out.ptr, out.shape[0], first, C.byref(got)),
+          ctx.h)
+    return out, [int(x) for x in first], got.value
+
+
 def seq_table(recs: np.ndarray, alt, depth, present, include_non_sequenced: bool = False):
     """psx_seq_table: (row record indices, occ, cov, vaf) with [sample, row] arrays."""
     r = np.ascontiguousarray(recs, A.MUTATION_DTYPE)

@@ -581,31 +581,52 @@ extern "C" uint32_t psx_read_words(const psx_seq_params* p) {
     return p ? 2u + (p->read_size + 15u) / 16u : 0u;
 }
 
-extern "C" psx_status psx_simulate_seq_reads(psx_ctx* ctx, const psx_genome* g, const psx_forest* f,
-                                             const psx_mutations* m, const psx_seq_params* p, uint64_t seed,
-                                             uint32_t* d_reads, uint64_t capacity_reads, uint64_t* n_reads) {
-    if (!ctx || !g || !f || !m || !p || !d_reads) return PSX_EINVAL;
-    return psx_guard(ctx, [&] {
-        PSX_HIP(hipSetDevice(ctx->device));
-        constexpr int CAP = 512;
-        constexpr size_t STAGE_MAX = 48u << 10;  // LDS read stage per block
-        const job_in in{g, f, m, p, seed, nullptr, nullptr, nullptr};
-        if (m->ploidy > 16) psx_throw(PSX_EINVAL, "read records hold copy indices below 16 (ploidy %u)", m->ploidy);
+namespace {
+
+// Read materialisation of jobs in one persistent launch: job k's reads start
+// at read index sum of the earlier jobs' reads (segments keep their job-order
+// read_base whatever order their work items take).
+void run_reads(psx_ctx* ctx, const std::vector<job_in>& in, uint32_t* d_reads, uint64_t capacity_reads,
+               uint64_t* job_first, uint64_t* n_reads) {
+    PSX_HIP(hipSetDevice(ctx->device));
+    constexpr int CAP = 512;
+    constexpr size_t STAGE_MAX = 48u << 10;  // LDS read stage per block
+    const uint32_t rw = in.empty() ? 0u : psx_read_words(in[0].p);
+    for (size_t k = 0; k < in.size(); ++k) {
+        const psx_mutations* m = in[k].m;
+        const psx_forest* f = in[k].f;
+        if (m->ploidy > 16)
+            psx_throw(PSX_EINVAL, "read records hold copy indices below 16 (job %zu: ploidy %u)", k, m->ploidy);
         if (f->n_leaves >= (1u << 26))
-            psx_throw(PSX_EINVAL, "read records hold leaf ranks below 2^26 (the forest has %u leaves)", f->n_leaves);
+            psx_throw(PSX_EINVAL, "read records hold leaf ranks below 2^26 (job %zu: the forest has %u leaves)", k,
+                      f->n_leaves);
         if (!m->cna.empty())
             psx_throw(PSX_EUNSUPPORTED, "read materialisation of tables with copy-number events is not supported");
-        std::unique_ptr<seq_batch, void (*)(seq_batch*)> B(batch_prepare(ctx, &in, 1, 1), batch_free);
+        if (psx_read_words(in[k].p) != rw)
+            psx_throw(PSX_EINVAL, "jobs of one read buffer need one read size (job %zu: %u words per read, job 0: %u)",
+                      k, psx_read_words(in[k].p), rw);
+    }
+    {
+        std::unique_ptr<seq_batch, void (*)(seq_batch*)> B(batch_prepare(ctx, in.data(), in.size(), 1), batch_free);
         seq_batch& P = *B;
         if (P.n_reads > capacity_reads)
             psx_throw(PSX_EINVAL, "the read buffer holds %llu reads, the run simulates %llu",
                       (unsigned long long)capacity_reads, (unsigned long long)P.n_reads);
+        if (job_first) {
+            job_first[0] = 0;
+            for (size_t k = 0; k < P.job_reads.size(); ++k) job_first[k + 1] = job_first[k] + P.job_reads[k];
+        }
         // presence flags per (output sample, mutation): the read kernel stages
-        // only the SNVs a read of the sample can carry
+        // only the SNVs a read of the sample can carry. Kernels index a job's
+        // flags from its own base (so * n_mut + m), so each job gets the slice
+        // that starts at its first (job, output sample) entry.
         dbuf d_present;
         d_present.alloc(P.pres_total);
+        std::vector<uint64_t> pres_base(P.jobs.size(), 0);
+        for (const pres_dev& q : P.pres)
+            if (q.so == 0) pres_base[q.job] = q.begin;
         job_dev* hj = (job_dev*)(P.sb.host.data() + P.o_jobs);  // the uploaded descriptors
-        for (size_t k = 0; k < P.jobs.size(); ++k) hj[k].present = d_present.as<uint8_t>();
+        for (size_t k = 0; k < P.jobs.size(); ++k) hj[k].present = d_present.as<uint8_t>() + pres_base[k];
         const seq_lane lane = ctx_lane(ctx, batch_desc_bytes(&P));
         batch_upload(&P, lane);
         seq_launch L = launch_desc(P, lane.desc, lane.queue);
@@ -617,7 +638,7 @@ extern "C" psx_status psx_simulate_seq_reads(psx_ctx* ctx, const psx_genome* g, 
                                (uint32_t)P.pres.size(), P.pres_total);
         }
         L.reads_out = d_reads;
-        L.read_words = psx_read_words(p);
+        L.read_words = rw;
         L.out_vec4 = (L.read_words % 4 == 0 && ((uintptr_t)d_reads & 15) == 0) ? 1u : 0u;
         // 64 rows of read_words words per wave staged in LDS; the block is the
         // largest of 512 and 256 threads whose stage fits STAGE_MAX (more waves
@@ -643,6 +664,38 @@ extern "C" psx_status psx_simulate_seq_reads(psx_ctx* ctx, const psx_genome* g, 
         PSX_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
         ctx->last_kernel_ms = ms;
         if (n_reads) *n_reads = P.n_reads;
+    }
+}
+
+}  // namespace
+
+extern "C" psx_status psx_simulate_seq_reads(psx_ctx* ctx, const psx_genome* g, const psx_forest* f,
+                                             const psx_mutations* m, const psx_seq_params* p, uint64_t seed,
+                                             uint32_t* d_reads, uint64_t capacity_reads, uint64_t* n_reads) {
+    if (!ctx || !g || !f || !m || !p || !d_reads) return PSX_EINVAL;
+    return psx_guard(ctx, [&] {
+        run_reads(ctx, {job_in{g, f, m, p, seed, nullptr, nullptr, nullptr}}, d_reads, capacity_reads, nullptr,
+                  n_reads);
+    });
+}
+
+extern "C" psx_status psx_simulate_seq_reads_batch(psx_ctx* ctx, const psx_seq_job* jobs, uint32_t n_jobs,
+                                                   uint32_t* d_reads, uint64_t capacity_reads, uint64_t* job_first,
+                                                   uint64_t* n_reads) {
+    if (!ctx || (!jobs && n_jobs) || (!d_reads && n_jobs)) return PSX_EINVAL;
+    return psx_guard(ctx, [&] {
+        std::vector<job_in> in;
+        for (uint32_t k = 0; k < n_jobs; ++k) {
+            const psx_seq_job& J = jobs[k];
+            if (!J.genome || !J.forest || !J.mutations) psx_throw(PSX_EINVAL, "job %u is incomplete", k);
+            in.push_back(job_in{J.genome, J.forest, J.mutations, &J.params, J.seed, nullptr, nullptr, nullptr});
+        }
+        if (in.empty()) {
+            if (job_first) job_first[0] = 0;
+            if (n_reads) *n_reads = 0;
+            return;
+        }
+        run_reads(ctx, in, d_reads, capacity_reads, job_first, n_reads);
     });
 }
 

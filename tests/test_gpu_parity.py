@@ -297,6 +297,41 @@ def test_materialised_reads_match_counts(ctx, world, case):
     assert np.array_equal(alt, oa)
 
 
+def test_batched_reads_equal_each_job_alone(ctx, world):
+    """psx_simulate_seq_reads_batch: job k's rows are the rows the single-job
+    call writes for it, for jobs of two forests, SE and PE, purity and
+    subsets, at 150 bp (16-byte rows) and 100 bp (any read size)."""
+    say("[parity] batched read materialisation: building a second world (30000 bases, 300 cells)")
+    w2 = build_world(ctx, [30_000], 5, dict(n_cells=300, n_samples=1, sample_size=50, seed=8), {}, 999)
+    subset = dict(coverage=40, read_size=150, insert_size_mean=350, insert_size_stddev=10, error_rate=1e-3,
+                  chromosomes=[1], samples=[0], with_normal_sample=False)
+    groups = {150: [(world, 23, cases.SEQ_CASES["pe_errors"]), (w2, 9, dict(coverage=12, read_size=150,
+                                                                           error_rate=2e-3)),
+                    (world, 5, cases.SEQ_CASES["purity"]), (world, 3, subset)],
+              100: [(world, 4, cases.SEQ_CASES["se_errorless"]), (w2, 6, dict(coverage=20, read_size=100,
+                                                                              insert_size_mean=250,
+                                                                              insert_size_stddev=15,
+                                                                              error_rate=5e-3))]}
+    for L, spec in groups.items():
+        say(f"[parity] batched reads, {len(spec)} jobs of {L} bp")
+        jobs = [core.SeqJob(ctx, w.g, w.pf, w.pm, seed, **kw) for w, seed, kw in spec]
+        out, first, n = core.simulate_seq_reads_batch(ctx, jobs)
+        assert n == first[-1] == sum(core.num_reads(w.g, w.pf, **kw) for w, _, kw in spec)
+        got = out.numpy()[:n]
+        for k, (w, seed, kw) in enumerate(spec):
+            alone = _reads(ctx, w, seed, kw)
+            assert np.array_equal(got[first[k]:first[k + 1]], alone), (L, k)
+        w0, s0, kw0 = spec[0]
+        assert np.array_equal(got[first[0]:first[1]], _oracle_reads(w0, s0, kw0, first[1]))
+    mixed = [core.SeqJob(ctx, world.g, world.pf, world.pm, 1, **cases.SEQ_CASES["pe_errors"]),
+             core.SeqJob(ctx, world.g, world.pf, world.pm, 1, **cases.SEQ_CASES["se_errorless"])]
+    with pytest.raises(PsxError, match="one read size"):
+        core.simulate_seq_reads_batch(ctx, mixed)
+    small = core.DeviceArray(ctx, (10, core.read_words(**cases.SEQ_CASES["pe_errors"])), np.uint32)
+    with pytest.raises(PsxError, match="the read buffer holds 10 reads"):
+        core.simulate_seq_reads_batch(ctx, mixed[:1], out=small)
+
+
 # ------------------------------------------------------------ copy numbers
 
 CNA_LENS = np.array([70_000, 33_000], np.uint32)
