@@ -32,7 +32,9 @@ HBM_PEAK_GBS = 8000.0
 # summaries under profiles/, named for the kernel and workload). Count-only sequencing is bound by instruction
 # issue, not HBM.
 PROFILED = {"k_seq_count": dict(traffic=302.1e6, valu_per_cu_cycle=1.09, salu_per_cu_cycle=0.68),
-            "k_seq_reads": dict(traffic=77.67e9, valu_per_cu_cycle=0.69, salu_per_cu_cycle=0.38)}
+            "k_seq_reads": dict(traffic=77.67e9, valu_per_cu_cycle=0.69, salu_per_cu_cycle=0.38),
+            # batched read materialisation of the 256-job batched workload's leading 19 jobs
+            "k_seq_reads_batched": dict(traffic=72.74e9)}
 
 
 def log(msg: str) -> None:
@@ -157,34 +159,9 @@ def batched_leg(ctx, n_jobs: int, reps: int, tune: bool) -> dict:
         f"({plan_wall * 1e3:.1f} ms), one by one {n / seq_wall:.3e} reads/s ({seq_wall * 1e3:.1f} ms), "
         f"identical {same}")
     out["plan_over_sequential"] = seq_wall / plan_wall
-    # read materialisation of the leading jobs whose reads fit 1.6e9 rows (76.8 GB), all in one launch
-    sel, tot = [], 0
-    for j in B.jobs:
-        r = core.num_reads(j.genome, j.forest, **j.seq)
-        if tot + r > 1_600_000_000:
-            break
-        sel.append(j)
-        tot += r
-    if sel:
-        rw = core.read_words(**sel[0].seq)
-        buf = core.DeviceArray(ctx, (tot, rw), np.uint32)
-        core.simulate_seq_reads_batch(ctx, sel, buf)
-        rk, rwl = [], []
-        for _ in range(reps):
-            t0 = time.perf_counter()
-            _, first, nr = core.simulate_seq_reads_batch(ctx, sel, buf)
-            rwl.append(time.perf_counter() - t0)
-            rk.append(ctx.last_kernel_ms())
-        buf.close()
-        rkm, rwall = float(np.median(rk)), float(np.median(rwl))
-        ach = nr * 4 * rw / (rkm / 1e3) / 1e9
-        out["reads"] = dict(workload=f"read materialisation of the first {len(sel)} jobs in one launch",
-                            jobs=len(sel), reads=nr, value=nr / rwall, unit="reads/s", wall_ms=rwall * 1e3,
-                            kernel="k_seq_reads", kernel_ms=rkm,
-                            roofline=dict(bound="hbm", achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s",
-                                          frac=ach / HBM_PEAK_GBS, traffic=None, bytes_per_read=4 * rw))
-        log(f"batched reads: {len(sel)} jobs, {nr} reads, kernel {rkm:.2f} ms, {ach:.0f} GB/s written "
-            f"({ach / HBM_PEAK_GBS:.1%})")
+    reads = batched_reads_leg(ctx, B, reps)
+    if reads:
+        out["reads"] = reads
     if tune:
         t0 = time.perf_counter()
         cfg, best_ms = core.autotune(ctx, B.jobs)
@@ -209,6 +186,45 @@ def batched_leg(ctx, n_jobs: int, reps: int, tune: bool) -> dict:
         for a in (j.alt, j.depth, j.present):
             a.close()
     return out
+
+
+def batched_reads_leg(ctx, B, reps: int) -> dict:
+    """Read materialisation of the batched workload's leading jobs whose
+    reads fit 1.6e9 rows (76.8 GB), all in one launch."""
+    from process_amd import core
+
+    sel, tot = [], 0
+    for j in B.jobs:
+        r = core.num_reads(j.genome, j.forest, **j.seq)
+        if tot + r > 1_600_000_000:
+            break
+        sel.append(j)
+        tot += r
+    if sel:
+        rw = core.read_words(**sel[0].seq)
+        buf = core.DeviceArray(ctx, (tot, rw), np.uint32)
+        core.simulate_seq_reads_batch(ctx, sel, buf)
+        rk, rwl = [], []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            _, first, nr = core.simulate_seq_reads_batch(ctx, sel, buf)
+            rwl.append(time.perf_counter() - t0)
+            rk.append(ctx.last_kernel_ms())
+        buf.close()
+        rkm, rwall = float(np.median(rk)), float(np.median(rwl))
+        ach = nr * 4 * rw / (rkm / 1e3) / 1e9
+        res = dict(workload=f"read materialisation of the first {len(sel)} jobs in one launch",
+                   jobs=len(sel), reads=nr, value=nr / rwall, unit="reads/s", wall_ms=rwall * 1e3,
+                   kernel="k_seq_reads", kernel_ms=rkm,
+                   roofline=dict(bound="hbm", achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s",
+                                 frac=ach / HBM_PEAK_GBS,
+                                 traffic=PROFILED["k_seq_reads_batched"]["traffic"]
+                                 if len(B.jobs) == 256 and len(sel) == 19 else None,
+                                 bytes_per_read=4 * rw))
+        log(f"batched reads: {len(sel)} jobs, {nr} reads, kernel {rkm:.2f} ms, {ach:.0f} GB/s written "
+            f"({ach / HBM_PEAK_GBS:.1%})")
+        return res
+    return {}
 
 
 def serving_leg(ctx, rate: float, duration: float, saturated: int) -> dict:
@@ -335,6 +351,8 @@ def main() -> int:
     ap.add_argument("--serve-burst", type=int, default=200, help="requests of the saturated run (0 skips it)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run device-to-device copies of BYTES (reference for the HBM byte counters)")
+    ap.add_argument("--batched-reads-only", action="store_true",
+                    help="only run the batched read materialisation (one launch per repetition; for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: the library has no multi-GPU path (asked for --gpus {args.gpus}). "
@@ -354,6 +372,10 @@ def main() -> int:
         for _ in range(3):
             check(lib().psx_dev_copy(ctx.h, C.c_void_p(dst.ptr), C.c_void_p(src.ptr), args.calibrate_copy), ctx.h)
         log(f"copied {args.calibrate_copy} bytes 3 times")
+        return 0
+    if args.batched_reads_only:
+        B = workloads.build_batched(ctx, n_jobs=args.batched_jobs)
+        print(json.dumps(batched_reads_leg(ctx, B, max(1, args.steps))), flush=True)
         return 0
     t = time.perf_counter()
     w = workloads.plain_large(n_cells=args.cells)
