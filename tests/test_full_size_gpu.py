@@ -13,7 +13,10 @@ through properties that do not depend on the size:
 * the materialised reads of a chromosome (all 1.6e9 reads written, 76.8 GB)
   equal, row for row, the oracle's reads of a run on that chromosome alone:
   each (sample, chromosome) segment's rows start where the segments before it
-  end.
+  end;
+* batched read materialisation of the batched workload's leading jobs up to
+  1.6e9 reads in one launch: a job's rows equal its reads alone (and the
+  oracle's reads of that job).
 """
 from __future__ import annotations
 
@@ -141,3 +144,46 @@ def test_plain_small_whole_table_matches_oracle(ctx):
     out.close()
     ref = O.simulate_reads(lens, packed, of, om, w.seq_seed, got, **w.seq)
     assert np.array_equal(rows, ref)
+
+
+def test_batched_reads_full_size(ctx):
+    """Batched read materialisation at the benchmark's size: the batched
+    workload's leading jobs up to 1.6e9 reads (19 jobs, 72 GB) in one launch;
+    the smallest jobs' rows equal their single-job reads, and the smallest
+    job's rows equal the oracle's reads."""
+    say("[full] batched reads: building the 256-job batched workload")
+    B = workloads.build_batched(ctx, n_jobs=256)
+    lens, _, specs = workloads.batched_jobs(seed=99, n_jobs=256)
+    sel, tot = [], 0
+    for j in B.jobs:
+        r = core.num_reads(j.genome, j.forest, **j.seq)
+        if tot + r > 1_600_000_000:
+            break
+        sel.append(j)
+        tot += r
+    say(f"[full] batched reads: {len(sel)} jobs, {tot} reads in one launch")
+    out, first, n = core.simulate_seq_reads_batch(ctx, sel)
+    assert n == tot == first[-1] and len(sel) > 8
+    rw = core.read_words(**sel[0].seq)
+    small = sorted(range(len(sel)), key=lambda k: first[k + 1] - first[k])[:3]
+    for k in small:
+        j = sel[k]
+        cnt = first[k + 1] - first[k]
+        rows = np.empty((cnt, rw), np.uint32)
+        st = lib().psx_dev_download(ctx.h, rows.ctypes.data_as(C.c_void_p), C.c_void_p(out.ptr + first[k] * rw * 4),
+                                    C.c_uint64(rows.nbytes))
+        assert st == 0
+        alone, got = core.simulate_seq_reads(ctx, j.genome, j.forest, j.muts, j.seed, **j.seq)
+        assert got == cnt
+        assert np.array_equal(rows, alone.numpy()[:cnt]), k
+        alone.close()
+        if k == small[0]:
+            s = specs[k]
+            say(f"[full] batched reads: oracle reads of job {k} ({s['leaves']} leaves, {cnt} reads)")
+            fd = core.synth_forest(growth=[0.2], death=[0.01], intro_at=[0], n_cells=2 * s["leaves"], n_samples=1,
+                                   sample_size=s["leaves"], seed=s["forest_seed"])
+            of = O.Forest(fd["cell_id"], fd["parent"], fd["species"], fd["sample"], fd["birth"], fd["n_samples"])
+            ref = O.simulate_reads(np.asarray(lens, np.uint32), B.genome.download(), of, j.muts.download(), j.seed,
+                                   cnt, **j.seq)
+            assert np.array_equal(rows, ref)
+    out.close()
