@@ -606,65 +606,63 @@ void run_reads(psx_ctx* ctx, const std::vector<job_in>& in, uint32_t* d_reads, u
             psx_throw(PSX_EINVAL, "jobs of one read buffer need one read size (job %zu: %u words per read, job 0: %u)",
                       k, psx_read_words(in[k].p), rw);
     }
-    {
-        std::unique_ptr<seq_batch, void (*)(seq_batch*)> B(batch_prepare(ctx, in.data(), in.size(), 1), batch_free);
-        seq_batch& P = *B;
-        if (P.n_reads > capacity_reads)
-            psx_throw(PSX_EINVAL, "the read buffer holds %llu reads, the run simulates %llu",
-                      (unsigned long long)capacity_reads, (unsigned long long)P.n_reads);
-        if (job_first) {
-            job_first[0] = 0;
-            for (size_t k = 0; k < P.job_reads.size(); ++k) job_first[k + 1] = job_first[k] + P.job_reads[k];
-        }
-        // presence flags per (output sample, mutation): the read kernel stages
-        // only the SNVs a read of the sample can carry. Kernels index a job's
-        // flags from its own base (so * n_mut + m), so each job gets the slice
-        // that starts at its first (job, output sample) entry.
-        dbuf d_present;
-        d_present.alloc(P.pres_total);
-        std::vector<uint64_t> pres_base(P.jobs.size(), 0);
-        for (const pres_dev& q : P.pres)
-            if (q.so == 0) pres_base[q.job] = q.begin;
-        job_dev* hj = (job_dev*)(P.sb.host.data() + P.o_jobs);  // the uploaded descriptors
-        for (size_t k = 0; k < P.jobs.size(); ++k) hj[k].present = d_present.as<uint8_t>() + pres_base[k];
-        const seq_lane lane = ctx_lane(ctx, batch_desc_bytes(&P));
-        batch_upload(&P, lane);
-        seq_launch L = launch_desc(P, lane.desc, lane.queue);
-        PSX_HIP(hipMemsetAsync(lane.queue, 0, 8, ctx->stream));
-        if (P.pres_total) {
-            const pres_dev* ps = (const pres_dev*)(lane.desc + P.o_pres);
-            const uint32_t pgrid = (uint32_t)((P.pres_total + 255) / 256);
-            hipLaunchKernelGGL(k_reads_present, dim3(pgrid), dim3(256), 0, ctx->stream, L.jobs, ps,
-                               (uint32_t)P.pres.size(), P.pres_total);
-        }
-        L.reads_out = d_reads;
-        L.read_words = rw;
-        L.out_vec4 = (L.read_words % 4 == 0 && ((uintptr_t)d_reads & 15) == 0) ? 1u : 0u;
-        // 64 rows of read_words words per wave staged in LDS; the block is the
-        // largest of 512 and 256 threads whose stage fits STAGE_MAX (more waves
-        // per CU share one copy of the block's window, reference and tables:
-        // 512 threads run 6 waves per SIMD at 150 bp, 256 threads 4 or 5)
-        const size_t wave_stage = (size_t)64 * L.read_words * 4u;
-        if (P.n_items == 0) PSX_HIP(hipEventRecord(ctx->ev0, ctx->stream));
-        if (P.n_items) {
-            const size_t items = P.n_items;
-            if (L.out_vec4 && L.read_words == 12)  // 145-160 bp reads
-                launch_reads<512, CAP, true, 10>(ctx, L, items, 8 * wave_stage);
-            else if (8 * wave_stage <= STAGE_MAX)
-                launch_reads<512, CAP, true, 0>(ctx, L, items, 8 * wave_stage);
-            else if (4 * wave_stage <= STAGE_MAX)
-                launch_reads<256, CAP, true, 0>(ctx, L, items, 4 * wave_stage);
-            else
-                launch_reads<256, CAP, false, 0>(ctx, L, items, 0);
-        }
-        PSX_HIP(hipGetLastError());
-        PSX_HIP(hipEventRecord(ctx->ev1, ctx->stream));
-        PSX_HIP(hipStreamSynchronize(ctx->stream));
-        float ms = 0.f;
-        PSX_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-        ctx->last_kernel_ms = ms;
-        if (n_reads) *n_reads = P.n_reads;
+    std::unique_ptr<seq_batch, void (*)(seq_batch*)> B(batch_prepare(ctx, in.data(), in.size(), 1), batch_free);
+    seq_batch& P = *B;
+    if (P.n_reads > capacity_reads)
+        psx_throw(PSX_EINVAL, "the read buffer holds %llu reads, the run simulates %llu",
+                  (unsigned long long)capacity_reads, (unsigned long long)P.n_reads);
+    if (job_first) {
+        job_first[0] = 0;
+        for (size_t k = 0; k < P.job_reads.size(); ++k) job_first[k + 1] = job_first[k] + P.job_reads[k];
     }
+    // presence flags per (output sample, mutation): the read kernel stages
+    // only the SNVs a read of the sample can carry. Kernels index a job's
+    // flags from its own base (so * n_mut + m), so each job gets the slice
+    // that starts at its first (job, output sample) entry.
+    dbuf d_present;
+    d_present.alloc(P.pres_total);
+    std::vector<uint64_t> pres_base(P.jobs.size(), 0);
+    for (const pres_dev& q : P.pres)
+        if (q.so == 0) pres_base[q.job] = q.begin;
+    job_dev* hj = (job_dev*)(P.sb.host.data() + P.o_jobs);  // the uploaded descriptors
+    for (size_t k = 0; k < P.jobs.size(); ++k) hj[k].present = d_present.as<uint8_t>() + pres_base[k];
+    const seq_lane lane = ctx_lane(ctx, batch_desc_bytes(&P));
+    batch_upload(&P, lane);
+    seq_launch L = launch_desc(P, lane.desc, lane.queue);
+    PSX_HIP(hipMemsetAsync(lane.queue, 0, 8, ctx->stream));
+    if (P.pres_total) {
+        const pres_dev* ps = (const pres_dev*)(lane.desc + P.o_pres);
+        const uint32_t pgrid = (uint32_t)((P.pres_total + 255) / 256);
+        hipLaunchKernelGGL(k_reads_present, dim3(pgrid), dim3(256), 0, ctx->stream, L.jobs, ps,
+                           (uint32_t)P.pres.size(), P.pres_total);
+    }
+    L.reads_out = d_reads;
+    L.read_words = rw;
+    L.out_vec4 = (L.read_words % 4 == 0 && ((uintptr_t)d_reads & 15) == 0) ? 1u : 0u;
+    // 64 rows of read_words words per wave staged in LDS; the block is the
+    // largest of 512 and 256 threads whose stage fits STAGE_MAX (more waves
+    // per CU share one copy of the block's window, reference and tables:
+    // 512 threads run 6 waves per SIMD at 150 bp, 256 threads 4 or 5)
+    const size_t wave_stage = (size_t)64 * L.read_words * 4u;
+    if (P.n_items == 0) PSX_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    if (P.n_items) {
+        const size_t items = P.n_items;
+        if (L.out_vec4 && L.read_words == 12)  // 145-160 bp reads
+            launch_reads<512, CAP, true, 10>(ctx, L, items, 8 * wave_stage);
+        else if (8 * wave_stage <= STAGE_MAX)
+            launch_reads<512, CAP, true, 0>(ctx, L, items, 8 * wave_stage);
+        else if (4 * wave_stage <= STAGE_MAX)
+            launch_reads<256, CAP, true, 0>(ctx, L, items, 4 * wave_stage);
+        else
+            launch_reads<256, CAP, false, 0>(ctx, L, items, 0);
+    }
+    PSX_HIP(hipGetLastError());
+    PSX_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    PSX_HIP(hipStreamSynchronize(ctx->stream));
+    float ms = 0.f;
+    PSX_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    ctx->last_kernel_ms = ms;
+    if (n_reads) *n_reads = P.n_reads;
 }
 
 }  // namespace
