@@ -222,6 +222,8 @@ class MutationEngine:
             r = rates.get(epi, rates.get("", {}))
             snv.append(float(r.get("SNV", 0.0)))
             ind.append(float(r.get("indel", 0.0)))
+            if float(r.get("CNA", 0.0)) != 0.0:
+                raise NotImplementedError(f"passenger CNA rates are not supported (mutant \"{mutant}\")")
         if n_pre_snv and pre_sig not in self.signature_names:
             raise ValueError(f"Unknown SBS signature \"{pre_sig}\"")
         if not self.exposures and (any(snv) or n_pre_snv):

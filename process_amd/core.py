@@ -357,6 +357,13 @@ def read_words(**seq) -> int:
     return int(lib().psx_read_words(C.byref(p)))
 
 
+def _check_read_buffer(out: DeviceArray, rw: int) -> None:
+    """The library writes rows of rw uint32 words: a caller's buffer must be
+    a [rows, rw] uint32 array, or rows would be written past its end."""
+    if out.dtype != np.uint32 or len(out.shape) != 2 or out.shape[1] != rw:
+        raise ValueError(f"the read buffer must be a [reads, {rw}] uint32 array (got {out.shape} {out.dtype})")
+
+
 def simulate_seq_reads(ctx: Context, genome: Genome, forest: Forest, muts: Mutations, seed: int,
                        out: DeviceArray | None = None, **seq):
     """Materialised reads into a device buffer of shape [n_reads, read_words]."""
@@ -364,6 +371,7 @@ def simulate_seq_reads(ctx: Context, genome: Genome, forest: Forest, muts: Mutat
     rw = read_words(**seq)
     if out is None:
         out = DeviceArray(ctx, (max(n, 1), rw), np.uint32)
+    _check_read_buffer(out, rw)
     p, keep = A.seq_params(**seq)
     got = C.c_uint64()
     check(lib().psx_simulate_seq_reads(ctx.h, genome.h, forest.h, muts.h, C.byref(p), C.c_uint64(seed), out.ptr,
@@ -379,6 +387,7 @@ def simulate_seq_reads_batch(ctx: Context, jobs: list, out: DeviceArray | None =
     rw = read_words(**jobs[0].seq) if jobs else 1
     if out is None:
         out = DeviceArray(ctx, (max(n, 1), rw), np.uint32)
+    _check_read_buffer(out, rw)
     arr, keep = _job_array(jobs)
     first = (C.c_uint64 * (len(jobs) + 1))()
     got = C.c_uint64()

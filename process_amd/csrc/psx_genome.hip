@@ -186,23 +186,38 @@ void psx_mutations_finish(psx_ctx* ctx, psx_mutations* m, const std::vector<uint
     psx_mutations_index(ctx, m, m->chr_len);
 }
 
-void psx_mutations_index(psx_ctx* ctx, psx_mutations* m, const std::vector<uint32_t>& chr_len) {
+const psx_mut_index* psx_mutations_find_index(psx_mutations* m, const std::vector<uint32_t>& chr_len) {
+    std::lock_guard<std::mutex> g(m->idx_mu);
+    for (const auto& ix : m->index)
+        if (ix->chr_len == chr_len) return ix.get();
+    return nullptr;
+}
+
+const psx_mut_index& psx_mutations_index(psx_ctx* ctx, psx_mutations* m, const std::vector<uint32_t>& chr_len) {
     const uint32_t n_chr = (uint32_t)chr_len.size();
     if (n_chr == 0 || n_chr > PSX_MAX_CHROMOSOMES) psx_throw(PSX_EINVAL, "invalid number of chromosomes");
-    m->chr_len = chr_len;
+    std::lock_guard<std::mutex> g(m->idx_mu);
+    for (const auto& ix : m->index)
+        if (ix->chr_len == chr_len) return *ix;
+    auto ix = std::make_unique<psx_mut_index>();
+    ix->chr_len = chr_len;
     bucket_args a{};
     a.pos = m->d_pos.as<uint32_t>();
     a.n_chr = n_chr;
-    m->bucket_off.assign(n_chr + 1, 0);
+    ix->bucket_off.assign(n_chr + 1, 0);
     for (uint32_t c = 0; c <= PSX_MAX_CHROMOSOMES; ++c) a.chr_begin[c] = m->chr_begin[c];
     for (uint32_t c = 0; c < n_chr; ++c)
-        m->bucket_off[c + 1] = m->bucket_off[c] + (chr_len[c] + PSX_STRATUM - 1) / PSX_STRATUM + 1;
-    for (uint32_t c = 0; c <= n_chr; ++c) a.boff[c] = m->bucket_off[c];
-    m->d_bucket.alloc(m->bucket_off[n_chr] * 4u);
-    a.bucket = m->d_bucket.as<uint32_t>();
-    hipLaunchKernelGGL(k_mut_bucket, dim3((uint32_t)((m->bucket_off[n_chr] + 255) / 256)), dim3(256), 0,
+        ix->bucket_off[c + 1] = ix->bucket_off[c] + (chr_len[c] + PSX_STRATUM - 1) / PSX_STRATUM + 1;
+    for (uint32_t c = 0; c <= n_chr; ++c) a.boff[c] = ix->bucket_off[c];
+    ix->d_bucket.alloc(ix->bucket_off[n_chr] * 4u);
+    a.bucket = ix->d_bucket.as<uint32_t>();
+    hipLaunchKernelGGL(k_mut_bucket, dim3((uint32_t)((ix->bucket_off[n_chr] + 255) / 256)), dim3(256), 0,
                        ctx->stream, a);
     PSX_HIP(hipGetLastError());
+    // callers on other streams (serving lanes) read the index without waiting on ctx's stream
+    PSX_HIP(hipStreamSynchronize(ctx->stream));
+    m->index.push_back(std::move(ix));
+    return *m->index.back();
 }
 
 // Host path (replay uploads): SID sort with ties in input order.

@@ -5,6 +5,8 @@
 
 #include <cstdarg>
 #include <cstdint>
+#include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -107,6 +109,13 @@ struct psx_forest {
     dbuf d_leaf_count;
 };
 
+// Per-stratum index of a mutation table for one set of chromosome lengths.
+struct psx_mut_index {
+    std::vector<uint32_t> chr_len;
+    std::vector<uint64_t> bucket_off;   // per chromosome offset into d_bucket
+    dbuf d_bucket;                      // per chromosome: first record at or after k * PSX_STRATUM
+};
+
 // Mutation table in SID order: device SoA for the kernels, AoS for download.
 struct psx_mutations {
     psx_ctx* ctx = nullptr;
@@ -114,11 +123,13 @@ struct psx_mutations {
     uint32_t ploidy = 2;
     uint32_t n_leaves = 0;
     std::vector<uint64_t> chr_begin;    // PSX_MAX_CHROMOSOMES + 1
-    std::vector<uint64_t> bucket_off;   // per chromosome offset into d_bucket
-    std::vector<uint32_t> chr_len;      // chromosome lengths of the genome used
+    std::vector<uint32_t> chr_len;      // chromosome lengths at placement (uploads: largest position + 1)
     dbuf d_recs;                        // psx_mutation[n]
     dbuf d_pos, d_lo, d_hi, d_info;     // uint32[n]
-    dbuf d_bucket;                      // per chromosome: first record at or after k * PSX_STRATUM
+    // every stratum index built so far, one per set of chromosome lengths;
+    // kept until the table is freed, since launches in flight may read any
+    std::mutex idx_mu;
+    std::vector<std::unique_ptr<psx_mut_index>> index;
     std::vector<psx_cna_event> cna;     // copy-number events in DFS order of their node
 };
 
@@ -169,8 +180,12 @@ void psx_mutations_build(psx_ctx* ctx, psx_mutations* m, std::vector<psx_mutatio
 // d_recs holds m->n records in SID order: builds the SoA arrays and the
 // per-stratum index for the chromosome lengths in m->chr_len.
 void psx_mutations_finish(psx_ctx* ctx, psx_mutations* m, const std::vector<uint64_t>& chr_begin);
-// Rebuilds the per-stratum index for other chromosome lengths.
-void psx_mutations_index(psx_ctx* ctx, psx_mutations* m, const std::vector<uint32_t>& chr_len);
+// The per-stratum index of m for chr_len. The first request for a set of
+// lengths builds it on ctx's stream and synchronises that stream, so launches
+// on any stream may read it at once; it stays valid until m is freed.
+const psx_mut_index& psx_mutations_index(psx_ctx* ctx, psx_mutations* m, const std::vector<uint32_t>& chr_len);
+// The index of m for chr_len if one was built, else nullptr (allocates nothing).
+const psx_mut_index* psx_mutations_find_index(psx_mutations* m, const std::vector<uint32_t>& chr_len);
 
 // Copies host bytes into the context scratch arena at a 256-byte aligned
 // offset and returns the device address.

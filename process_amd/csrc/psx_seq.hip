@@ -25,6 +25,7 @@ struct seq_batch {
     uint64_t n_items = 0;  // written on the device (k_seq_items) from the segments
     uint64_t n_items_sparse = 0;  // the first items: those of sparse jobs (layout)
     uint32_t spi = 1;      // strata per item
+    bool no_alloc = false;  // refuse jobs whose mutation table lacks an index for their genome
     scratch_builder sb;
     std::vector<size_t> tab_off;  // per job: insert thresholds, insert guide, error thresholds
     std::vector<size_t> cna_off;  // per job: copy-number events (SIZE_MAX: none)
@@ -153,13 +154,16 @@ void prepare_job(psx_ctx* ctx, seq_batch& P, const job_in& in, uint32_t spi) {
         if (f->sample_begin[req[k] + 1] == f->sample_begin[req[k]])
             psx_throw(PSX_EINVAL, "sample %u has no cells", req[k]);
     }
-    if (m->chr_len != g->len) psx_mutations_index(ctx, m, g->len);
+    const psx_mut_index* ix = P.no_alloc ? psx_mutations_find_index(m, g->len) : &psx_mutations_index(ctx, m, g->len);
+    if (!ix)
+        psx_throw(PSX_EINVAL, "the mutation table has no index for this genome's chromosome lengths yet (an "
+                              "allocating sequencing call builds it)");
     const uint32_t n_out = n_req + (p->with_normal_sample ? 1u : 0u);
     const uint32_t job = (uint32_t)P.jobs.size();
     job_dev J{};
     J.seed = in.seed;
     J.pos = m->d_pos.as<uint32_t>(); J.lo = m->d_lo.as<uint32_t>(); J.hi = m->d_hi.as<uint32_t>();
-    J.info = m->d_info.as<uint32_t>(); J.bucket = m->d_bucket.as<uint32_t>();
+    J.info = m->d_info.as<uint32_t>(); J.bucket = ix->d_bucket.as<uint32_t>();
     J.sample_leaves = f->d_sample_leaves.as<uint32_t>();
     J.leaf_count = f->d_leaf_count.as<uint4>();
     J.leaf_blocks = f->leaf_blocks;
@@ -228,16 +232,21 @@ void prepare_job(psx_ctx* ctx, seq_batch& P, const job_in& in, uint32_t spi) {
             S.cna_off = c_off[c];
             S.kmax = c_kmax[c];
             // with copy-number events a template draws one of kmax copy slots per cell
-            S.T = (uint64_t)std::llround(p->coverage * (double)g->len[c] / (double)(pe ? 2 * L : L) *
-                                         ((double)S.kmax / (double)ploidy));
+            const double t_real = p->coverage * (double)g->len[c] / (double)(pe ? 2 * L : L) *
+                                  ((double)S.kmax / (double)ploidy);
+            if (!(t_real < 9.0e18)) psx_throw(PSX_EINVAL, "too many templates on chromosome %u", c);
+            S.T = (uint64_t)std::llround(t_real);
             S.out_off = (uint64_t)so * m->n;
             S.read_base = P.n_reads;
             S.chr_woff = g->woff[c];
             S.job = job; S.so = so; S.chr = c; S.len = g->len[c];
             S.W = g->len[c] - L + 1;
             S.J = (S.W + PSX_STRATUM - 1) / PSX_STRATUM;
-            S.bucket_off = (uint32_t)m->bucket_off[c];
-            S.n_bucket = (uint32_t)(m->bucket_off[c + 1] - m->bucket_off[c]);
+            if (S.T && S.W > UINT64_MAX / S.T)  // stratum bounds are T * start / W in 64 bits
+                psx_throw(PSX_EINVAL, "too many templates on chromosome %u (%llu over %u starts)", c,
+                          (unsigned long long)S.T, S.W);
+            S.bucket_off = (uint32_t)ix->bucket_off[c];
+            S.n_bucket = (uint32_t)(ix->bucket_off[c + 1] - ix->bucket_off[c]);
             S.leaf_off = leaf_off; S.n_cells = n_cells;
             P.n_reads += S.T * (pe ? 2u : 1u);  // template draws: an upper bound with copy-number events
             P.segs.push_back(S);  // its work items are numbered in layout()
@@ -306,9 +315,10 @@ seq_launch launch_desc(const seq_batch& P, uint8_t* base, unsigned long long* qu
 
 }  // namespace
 
-seq_batch* batch_prepare(psx_ctx* ctx, const job_in* jobs, size_t n_jobs, uint32_t spi) {
+seq_batch* batch_prepare(psx_ctx* ctx, const job_in* jobs, size_t n_jobs, uint32_t spi, bool no_alloc) {
     auto P = std::make_unique<seq_batch>();
     P->spi = spi;
+    P->no_alloc = no_alloc;
     for (size_t k = 0; k < n_jobs; ++k) prepare_job(ctx, *P, jobs[k], spi);
     layout(*P);
     return P.release();
@@ -421,15 +431,11 @@ struct run_result {
     double kernel_ms;
 };
 
-// Uploads descriptors, runs the init and count kernels (optionally as a
-// captured hipGraph) and times the count kernel with HIP events (the graph
-// launch, init included, when captured).
-run_result run_count(psx_ctx* ctx, const std::vector<job_in>& jobs, const psx_launch_config& cfg) {
-    check_config(cfg);
-    PSX_HIP(hipSetDevice(ctx->device));
-    std::unique_ptr<seq_batch, void (*)(seq_batch*)> P(batch_prepare(ctx, jobs.data(), jobs.size(),
-                                                                     cfg.strata_per_item), batch_free);
-    const seq_lane lane = ctx_lane(ctx, batch_desc_bytes(P.get()));
+// Uploads a prepared batch's descriptors to the lane, runs the init and count
+// kernels (optionally as a captured hipGraph) and times the count kernel with
+// HIP events (the graph launch, init included, when captured).
+run_result run_prepared(psx_ctx* ctx, seq_batch* batch, const seq_lane& lane, const psx_launch_config& cfg) {
+    std::unique_ptr<seq_batch, void (*)(seq_batch*)> P(batch, batch_free);
     batch_upload(P.get(), lane);
     if (cfg.use_graph) {
         hipGraph_t graph = nullptr;
@@ -454,6 +460,21 @@ run_result run_count(psx_ctx* ctx, const std::vector<job_in>& jobs, const psx_la
     uint64_t n = 0;
     for (uint64_t r : batch_job_reads(P.get(), lane)) n += r;
     return run_result{n, (double)ms};
+}
+
+// The jobs in one launch with descriptors in the context's scratch buffer.
+run_result run_count(psx_ctx* ctx, const std::vector<job_in>& jobs, const psx_launch_config& cfg) {
+    check_config(cfg);
+    PSX_HIP(hipSetDevice(ctx->device));
+    seq_batch* P = batch_prepare(ctx, jobs.data(), jobs.size(), cfg.strata_per_item);
+    seq_lane lane;
+    try {
+        lane = ctx_lane(ctx, batch_desc_bytes(P));
+    } catch (...) {
+        batch_free(P);
+        throw;
+    }
+    return run_prepared(ctx, P, lane, cfg);
 }
 
 std::vector<job_in> jobs_of(const psx_seq_job* jobs, uint32_t n_jobs) {

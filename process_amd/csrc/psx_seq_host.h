@@ -33,10 +33,11 @@ struct seq_lane {
 struct seq_batch;  // descriptors of one launch over one or more jobs
 
 // Validates the jobs (psx_error on the first invalid one) and builds their
-// descriptors on the host. Rebuilds a mutation table's stratum index when it
-// was indexed for other chromosome lengths (a stream-ordered device call on
-// ctx's stream).
-seq_batch* batch_prepare(psx_ctx* ctx, const job_in* jobs, size_t n_jobs, uint32_t strata_per_item);
+// descriptors on the host. Builds a mutation table's stratum index for a
+// genome it was not indexed for yet (allocates and synchronises ctx's stream)
+// unless no_alloc, which refuses such a job instead.
+seq_batch* batch_prepare(psx_ctx* ctx, const job_in* jobs, size_t n_jobs, uint32_t strata_per_item,
+                         bool no_alloc = false);
 void batch_free(seq_batch* b);
 size_t batch_desc_bytes(const seq_batch* b);
 uint64_t batch_reads(const seq_batch* b);

@@ -105,6 +105,18 @@ def test_engine_rejects_unknown_signature_and_rates():
         eng.add_exposure({"SBS1": 0.5})
 
 
+def test_read_buffer_shape_is_checked():
+    """A caller's read buffer narrower than a read, or of another dtype, is
+    refused before the library writes rows into it."""
+    from types import SimpleNamespace
+
+    from process_amd import core
+    core._check_read_buffer(SimpleNamespace(dtype=np.dtype(np.uint32), shape=(10, 12)), 12)
+    for shape, dt in [((10, 11), np.uint32), ((10, 12), np.int64), ((120,), np.uint32)]:
+        with pytest.raises(ValueError, match="read buffer"):
+            core._check_read_buffer(SimpleNamespace(dtype=np.dtype(dt), shape=shape), 12)
+
+
 # ------------------------------------------------------------------- GPU
 
 def build(seed=3):

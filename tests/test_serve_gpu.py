@@ -181,3 +181,38 @@ def test_copy_number_requests_in_the_mix(ctx, pool):
         assert srv.totals()["completed"] == len(reqs)
     finally:
         srv.close()
+
+
+def test_freshly_uploaded_tables(ctx, pool):
+    """Replayed (uploaded) tables are indexed for their largest positions, so
+    the first request on a real genome builds the genome's stratum index while
+    the serving lanes run on their own streams: every such request, submitted
+    before any other call touched its table, equals the oracle, and the same
+    table indexed for two genomes of different lengths in one batch keeps
+    both indexes valid."""
+    g, items = pool
+    seq = dict(coverage=20, read_size=150, insert_size_mean=350, insert_size_stddev=10, error_rate=1e-3)
+    recs = [items[k][1].download() for k in range(3)]
+    ups = [core.Mutations.upload(ctx, items[k][0], recs[k]) for k in range(3)]
+    lens2 = np.array([LENS[0], LENS[1] + 2_000], np.uint32)  # the same bases up to LENS, a longer chromosome 1
+    g2 = core.Genome(ctx, lens2, seed=41)
+    srv = core.Server(ctx, 64 << 20, max_inflight=2, max_batch=8)
+    try:
+        tickets = [srv.submit(g, items[k][0], ups[k], 60 + k, **seq) for k in range(3)]
+        tickets.append(srv.submit(g2, items[0][0], ups[0], 70, **seq))
+        got = [srv.wait(t, timeout_ms=60_000) for t in tickets]
+    finally:
+        srv.close()
+    for k in range(3):
+        oa, od, op, on = O.simulate_seq(LENS, items[k][2], recs[k], 60 + k, **seq)
+        assert np.array_equal(got[k][0], oa) and np.array_equal(got[k][1], od), k
+        assert np.array_equal(got[k][2], op) and got[k][3]["n_reads"] == on, k
+    oa, od, op, on = O.simulate_seq(lens2, items[0][2], recs[0], 70, **seq)
+    assert np.array_equal(got[3][0], oa) and np.array_equal(got[3][1], od) and got[3][3]["n_reads"] == on
+    # one batch, one table, two genomes
+    jobs = [core.SeqJob(ctx, g, items[0][0], ups[0], 80, **seq), core.SeqJob(ctx, g2, items[0][0], ups[0], 81, **seq)]
+    core.simulate_seq_batch(ctx, jobs)
+    for j, lens in zip(jobs, (LENS, lens2)):
+        oa, od, op, on = O.simulate_seq(lens, items[0][2], recs[0], j.seed, **seq)
+        a, d, p = j.results()
+        assert np.array_equal(a, oa) and np.array_equal(d, od) and np.array_equal(p, op)
