@@ -406,34 +406,6 @@ void check_config(const psx_launch_config& c) {
                   c.block_threads, c.lds_records, c.strata_per_item, c.blocks_per_cu);
 }
 
-}  // namespace psxh
-
-using namespace psxh;
-
-namespace {
-
-seq_lane ctx_lane(psx_ctx* ctx, size_t desc_bytes) {
-    ctx->scratch.reserve(desc_bytes);
-    ctx->queue.reserve(256);
-    seq_lane lane;
-    lane.n_cu = ctx->n_cu;
-    lane.stream = ctx->stream;
-    lane.ev0 = ctx->ev0;
-    lane.ev1 = ctx->ev1;
-    lane.desc = (uint8_t*)ctx->scratch.p;
-    lane.desc_bytes = ctx->scratch.bytes;
-    lane.queue = ctx->queue.as<unsigned long long>();
-    return lane;
-}
-
-struct run_result {
-    uint64_t n_reads;
-    double kernel_ms;
-};
-
-// Uploads a prepared batch's descriptors to the lane, runs the init and count
-// kernels (optionally as a captured hipGraph) and times the count kernel with
-// HIP events (the graph launch, init included, when captured).
 run_result run_prepared(psx_ctx* ctx, seq_batch* batch, const seq_lane& lane, const psx_launch_config& cfg) {
     std::unique_ptr<seq_batch, void (*)(seq_batch*)> P(batch, batch_free);
     batch_upload(P.get(), lane);
@@ -460,6 +432,26 @@ run_result run_prepared(psx_ctx* ctx, seq_batch* batch, const seq_lane& lane, co
     uint64_t n = 0;
     for (uint64_t r : batch_job_reads(P.get(), lane)) n += r;
     return run_result{n, (double)ms};
+}
+
+}  // namespace psxh
+
+using namespace psxh;
+
+namespace {
+
+seq_lane ctx_lane(psx_ctx* ctx, size_t desc_bytes) {
+    ctx->scratch.reserve(desc_bytes);
+    ctx->queue.reserve(256);
+    seq_lane lane;
+    lane.n_cu = ctx->n_cu;
+    lane.stream = ctx->stream;
+    lane.ev0 = ctx->ev0;
+    lane.ev1 = ctx->ev1;
+    lane.desc = (uint8_t*)ctx->scratch.p;
+    lane.desc_bytes = ctx->scratch.bytes;
+    lane.queue = ctx->queue.as<unsigned long long>();
+    return lane;
 }
 
 // The jobs in one launch with descriptors in the context's scratch buffer.

@@ -51,6 +51,17 @@ void batch_upload(seq_batch* b, const seq_lane& lane);
 // timed != 0 the count kernel is bracketed by lane.ev0 / lane.ev1.
 void batch_launch(const seq_batch* b, const seq_lane& lane, const psx_launch_config& cfg, bool timed);
 
+struct run_result {
+    uint64_t n_reads;
+    double kernel_ms;
+};
+// Takes ownership of a prepared batch: uploads its descriptors to the lane,
+// runs the init and count kernels (captured in a hipGraph and replayed once
+// with cfg.use_graph), waits, and returns the reads simulated and the
+// HIP-event time of the count kernel (of the graph launch when captured),
+// also kept in ctx->last_kernel_ms.
+run_result run_prepared(psx_ctx* ctx, seq_batch* batch, const seq_lane& lane, const psx_launch_config& cfg);
+
 uint32_t out_samples(const psx_forest* f, const psx_seq_params* p);
 psx_launch_config default_config();
 void check_config(const psx_launch_config& c);
