@@ -28,13 +28,71 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 # HBM bytes per launch of the dominant kernels on the plain_large workload, from rocprofv3 FETCH_SIZE and
-# WRITE_SIZE runs calibrated against a device copy, and their issue-slot use from SQ counters (the rocprofv3
-# summaries under profiles/, named for the kernel and workload). Count-only sequencing is bound by instruction
-# issue, not HBM.
-PROFILED = {"k_seq_count": dict(traffic=302.1e6, valu_per_cu_cycle=1.09, salu_per_cu_cycle=0.68),
-            "k_seq_reads": dict(traffic=77.67e9, valu_per_cu_cycle=0.69, salu_per_cu_cycle=0.38),
+# WRITE_SIZE runs calibrated against a device copy, and vector instructions per launch from SQ_INSTS_VALU (the
+# rocprofv3 summaries under profiles/, named for the kernel and workload). Count-only sequencing is bound by
+# vector instruction issue, not HBM; the probe kernels' instruction counts per lane-iteration come from the
+# same counter (profiles/k_probe_valu.md).
+PROFILED = {"k_seq_count": dict(traffic=302.1e6, valu_insts=6.206e9),
+            "k_seq_reads": dict(traffic=77.67e9),
             # batched read materialisation of the 256-job batched workload's leading 19 jobs
-            "k_seq_reads_batched": dict(traffic=72.74e9)}
+            "k_seq_reads_batched": dict(traffic=72.74e9),
+            # wave64 VALU instructions per wave per loop iteration of the probe kernels
+            "k_probe_valu": dict(philox_per_iter=None, int_per_iter=None)}
+
+# Algorithmic bytes of one materialised read: an 8-byte header (start, source) and the 2-bit bases,
+# 8 + ceil(L / 4) = 46 B at L = 150. The stored row is 8 + 4 * ceil(L / 16) = 48 B (word-aligned bases);
+# the difference is counted in `traffic`, not in `achieved`.
+
+
+def read_algo_bytes(L: int) -> int:
+    return 8 + (L + 3) // 4
+
+
+def cpu_info() -> dict:
+    """The host the CPU baseline ran on: logical CPUs, those this process may
+    use, the OpenMP thread setting and the CPU model."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return dict(cpu_count=os.cpu_count(), usable=usable, omp_num_threads=os.environ.get("OMP_NUM_THREADS"),
+                model=model)
+
+
+def cpu_threads(info: dict) -> int:
+    """Every CPU this process may use, capped by OMP_NUM_THREADS when set
+    (the host's CPU share for one GPU)."""
+    n = info["usable"]
+    omp = info.get("omp_num_threads")
+    if omp and omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))
+    return max(1, n)
+
+
+def valu_ceiling(ctx, reps: int = 3) -> dict:
+    """The vector-ALU issue ceiling measured on this GPU by the probe kernels
+    (psx_probe_valu): HIP-event time of a launch of every lane running a
+    Philox loop, and of eight independent integer chains per lane."""
+    from process_amd import core
+
+    out = {}
+    for mode, iters in (("philox", 4096), ("int", 16384)):
+        core.probe_valu(ctx, mode, iters)
+        ms = min(core.probe_valu(ctx, mode, iters)[0] for _ in range(reps))
+        lane_iters = core.probe_valu(ctx, mode, iters)[1]
+        per_iter = PROFILED["k_probe_valu"][f"{mode}_per_iter"]
+        r = dict(ms=ms, lane_iters=lane_iters, lane_iters_per_s=lane_iters / (ms / 1e3))
+        if per_iter:
+            r["valu_insts"] = lane_iters / 64 * per_iter
+            r["valu_insts_per_s"] = r["valu_insts"] / (ms / 1e3)
+        out[mode] = r
+    return out
 
 
 def log(msg: str) -> None:
@@ -212,7 +270,8 @@ def batched_reads_leg(ctx, B, reps: int) -> dict:
             rk.append(ctx.last_kernel_ms())
         buf.close()
         rkm, rwall = float(np.median(rk)), float(np.median(rwl))
-        ach = nr * 4 * rw / (rkm / 1e3) / 1e9
+        algo = read_algo_bytes(sel[0].seq["read_size"])
+        ach = nr * algo / (rkm / 1e3) / 1e9
         res = dict(workload=f"read materialisation of the first {len(sel)} jobs in one launch",
                    jobs=len(sel), reads=nr, value=nr / rwall, unit="reads/s", wall_ms=rwall * 1e3,
                    kernel="k_seq_reads", kernel_ms=rkm,
@@ -220,8 +279,9 @@ def batched_reads_leg(ctx, B, reps: int) -> dict:
                                  frac=ach / HBM_PEAK_GBS,
                                  traffic=PROFILED["k_seq_reads_batched"]["traffic"]
                                  if len(B.jobs) == 256 and len(sel) == 19 else None,
-                                 bytes_per_read=4 * rw))
-        log(f"batched reads: {len(sel)} jobs, {nr} reads, kernel {rkm:.2f} ms, {ach:.0f} GB/s written "
+                                 algo_bytes_per_read=algo, row_bytes=4 * rw,
+                                 row_write_rate_gbs=nr * 4 * rw / (rkm / 1e3) / 1e9))
+        log(f"batched reads: {len(sel)} jobs, {nr} reads, kernel {rkm:.2f} ms, {ach:.0f} GB/s of read bytes "
             f"({ach / HBM_PEAK_GBS:.1%})")
         return res
     return {}
@@ -351,6 +411,8 @@ def main() -> int:
     ap.add_argument("--serve-burst", type=int, default=200, help="requests of the saturated run (0 skips it)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run device-to-device copies of BYTES (reference for the HBM byte counters)")
+    ap.add_argument("--probe-only", action="store_true",
+                    help="only run the vector-ALU probe kernels (for profiling their instruction counts)")
     ap.add_argument("--batched-reads-only", action="store_true",
                     help="only run the batched read materialisation (one launch per repetition; for profiling)")
     args = ap.parse_args()
@@ -373,6 +435,9 @@ def main() -> int:
             check(lib().psx_dev_copy(ctx.h, C.c_void_p(dst.ptr), C.c_void_p(src.ptr), args.calibrate_copy), ctx.h)
         log(f"copied {args.calibrate_copy} bytes 3 times")
         return 0
+    if args.probe_only:
+        print(json.dumps(valu_ceiling(ctx)), flush=True)
+        return 0
     if args.batched_reads_only:
         B = workloads.build_batched(ctx, n_jobs=args.batched_jobs)
         print(json.dumps(batched_reads_leg(ctx, B, max(1, args.steps))), flush=True)
@@ -389,9 +454,14 @@ def main() -> int:
     alt = core.DeviceArray(ctx, (S, M), np.int32)
     dep = core.DeviceArray(ctx, (S, M), np.int32)
     pres = core.DeviceArray(ctx, (S, M), np.uint8)
+    vaf = core.DeviceArray(ctx, (S, M), np.float32)
 
     def step():
-        return core.simulate_seq_device(ctx, built.genome, built.forest, built.muts, w.seq_seed, alt, dep, pres, **seq)
+        # one simulate_seq over every output sample: alt, depth, presence and the fp32 VAF, all left in HBM
+        n = core.simulate_seq_device(ctx, built.genome, built.forest, built.muts, w.seq_seed, alt, dep, pres, **seq)
+        k_ms = ctx.last_kernel_ms()
+        core.seq_vaf_f32(ctx, built.muts, alt, dep, pres, vaf)
+        return n, k_ms
 
     tuned = None
     if not args.no_tune:
@@ -412,8 +482,9 @@ def main() -> int:
     t0 = time.perf_counter()
     n_reads = 0
     for _ in range(args.steps):
-        n_reads += step()
-        kms.append(ctx.last_kernel_ms())
+        n, k_ms = step()
+        n_reads += n
+        kms.append(k_ms)
     ctx.synchronize()
     t1 = time.perf_counter()
     elapsed = t1 - t0
@@ -423,6 +494,17 @@ def main() -> int:
     log(f"count-only: {reads_per_step} reads/step, {elapsed / args.steps * 1e3:.2f} ms/step, "
         f"kernel {kernel_ms:.2f} ms, {value:.3e} reads/s")
 
+    # the drop-in psx_simulate_seq: the same step plus the copy of alt, depth and presence to host memory
+    host_ms = []
+    for _ in range(3):
+        t2 = time.perf_counter()
+        core.simulate_seq_arrays(ctx, built.genome, built.forest, built.muts, w.seq_seed, **seq)
+        host_ms.append((time.perf_counter() - t2) * 1e3)
+    host_out = dict(value=reads_per_step / (np.median(host_ms) / 1e3), unit="reads/s",
+                    ms_per_step=float(np.median(host_ms)), table_bytes=9 * M * S,
+                    note="psx_simulate_seq: device outputs allocated per call and copied to host arrays")
+    log(f"host outputs: {host_out['ms_per_step']:.2f} ms/step, {host_out['value']:.3e} reads/s")
+
     # algorithmic bytes of one count-only launch: mutation records read once
     # (16 B), alt + depth written per output sample (8 B), sample leaf lists (4 B/cell)
     leaves = int(sum(int((w.forest["sample"] == s).sum()) for s in range(w.forest["n_samples"])))
@@ -430,10 +512,21 @@ def main() -> int:
     achieved = algo_bytes / (kernel_ms / 1e3) / 1e9
     prof = PROFILED["k_seq_count"] if args.cells == 1_000_000 else {}
     roofline = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
-                    traffic=prof.get("traffic"), kernel="k_seq_count", kernel_ms=kernel_ms, algo_bytes=algo_bytes,
-                    issue_bound=dict(valu_per_cu_cycle=prof.get("valu_per_cu_cycle"),
-                                     salu_per_cu_cycle=prof.get("salu_per_cu_cycle"),
-                                     note="count-only moves ~0.04 B per read; VALU/SALU issue slots bound it"))
+                    traffic=prof.get("traffic"), kernel="k_seq_count", kernel_ms=kernel_ms, algo_bytes=algo_bytes)
+    # the bound that applies: vector instruction issue, against the probe kernels' rate on this GPU
+    ceil = valu_ceiling(ctx)
+    valu = dict(probe=ceil, note="count-only moves ~0.04 B per read; vector instruction issue bounds it")
+    if prof.get("valu_insts") and ceil["int"].get("valu_insts_per_s"):
+        rate = prof["valu_insts"] / (kernel_ms / 1e3)
+        valu.update(kernel_valu_insts=prof["valu_insts"], achieved_insts_per_s=rate,
+                    peak_insts_per_s=ceil["int"]["valu_insts_per_s"],
+                    frac=rate / ceil["int"]["valu_insts_per_s"],
+                    philox_peak_insts_per_s=ceil["philox"].get("valu_insts_per_s"),
+                    frac_of_philox_rate=rate / ceil["philox"]["valu_insts_per_s"]
+                    if ceil["philox"].get("valu_insts_per_s") else None)
+        log(f"VALU issue: {rate:.3e} insts/s, {valu['frac']:.1%} of the integer probe's "
+            f"{valu['peak_insts_per_s']:.3e}")
+    roofline["valu"] = valu
 
     out = {
         "metric": "simulated reads/s on 1 MI355X (plain vs batched); p50/p99 job latency",
@@ -454,6 +547,7 @@ def main() -> int:
                    "forest_nodes": len(w.forest["cell_id"]), "place_mutations_ms": built.place_ms,
                    "launch": tuned or "default"},
         "roofline": roofline,
+        "host_outputs": host_out,
     }
 
     if not args.no_reads:
@@ -471,15 +565,16 @@ def main() -> int:
         ctx.synchronize()
         rdt = (time.perf_counter() - t0) / reps
         rk = float(np.mean(rms))
-        bytes_per_read = 4 * rw
-        ach = n_all * bytes_per_read / (rk / 1e3) / 1e9
+        algo = read_algo_bytes(seq["read_size"])
+        ach = n_all * algo / (rk / 1e3) / 1e9
         out["reads_mode"] = dict(value=n_all / rdt, unit="reads/s", ms_per_step=rdt * 1e3, kernel="k_seq_reads",
                                  roofline=dict(bound="hbm", achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s",
                                                frac=ach / HBM_PEAK_GBS,
                                                traffic=PROFILED["k_seq_reads"]["traffic"]
                                                if args.cells == 1_000_000 else None,
-                                               kernel_ms=rk, bytes_per_read=bytes_per_read))
-        log(f"reads mode: {n_all} reads, kernel {rk:.2f} ms, {ach:.0f} GB/s written ({ach / HBM_PEAK_GBS:.1%})")
+                                               kernel_ms=rk, algo_bytes_per_read=algo, row_bytes=4 * rw,
+                                               row_write_rate_gbs=n_all * 4 * rw / (rk / 1e3) / 1e9))
+        log(f"reads mode: {n_all} reads, kernel {rk:.2f} ms, {ach:.0f} GB/s of read bytes ({ach / HBM_PEAK_GBS:.1%})")
         buf.close()
 
     if not args.no_cna:
@@ -493,8 +588,10 @@ def main() -> int:
 
     if not args.no_cpu:
         recs = built.muts.download()
-        threads = min(os.cpu_count() or 1, 16)
+        info = cpu_info()
+        threads = cpu_threads(info)
         cb = cpu_baseline(w, built, recs, threads, args.cpu_reads)
+        cb["host"] = info
         out["cpu_baseline"] = cb
         log(f"cpu baseline: {cb['value']:.3e} reads/s on {cb['cores']} threads")
         out["gpu_vs_cpu"] = value / cb["value"]

@@ -167,12 +167,14 @@ enum { PSX_CNA_AMP = 0, PSX_CNA_DEL = 1, PSX_CNA_WGD = 2 };
 #define PSX_ALL_CHROMOSOMES 0xFFFFFFFFu
 #define PSX_MAX_CNA_PLOIDY 8u
 
+#define PSX_CNA_RANDOM_ALLELE 0xFFu
+
 typedef struct psx_cna {
     uint32_t species;  /* on the edge into every node where this species first appears on a lineage */
     uint32_t chr;      /* ignored for PSX_CNA_WGD */
     uint32_t start;    /* 0-based first base */
     uint32_t len;      /* bases */
-    uint8_t allele;    /* allele lineage (AMP, DEL) */
+    uint8_t allele;    /* allele lineage (AMP, DEL); PSX_CNA_RANDOM_ALLELE: uniform per placement node */
     uint8_t kind;      /* PSX_CNA_AMP / PSX_CNA_DEL / PSX_CNA_WGD */
     uint8_t copies;    /* AMP: copies added */
     uint8_t pad;
@@ -193,6 +195,25 @@ typedef struct psx_cna_event {
     uint8_t pad;
 } psx_cna_event;
 
+/* A region of the passenger CNA catalogue (load_passenger_CNAs,
+ * src/mutation_engine.cpp:441-487: a row with major copy number > 1 or NA
+ * gives an amplification, one with minor copy number < 1 or NA a deletion). */
+typedef struct psx_cna_region {
+    uint32_t chr;
+    uint32_t start;    /* 0-based first base */
+    uint32_t len;
+    uint8_t kind;      /* PSX_CNA_AMP or PSX_CNA_DEL */
+    uint8_t pad[3];
+} psx_cna_region;
+
+/* COSMIC indel (ID83) classes, in COSMIC order: 1 bp deletions of C, then of
+ * T, at homopolymers of 1..6+ bases (12); 1 bp insertions of C, then of T,
+ * next to 0..5+ copies (12); deletions of 2, 3, 4, 5+ bp units at 1..6+
+ * copies (24); insertions of 2, 3, 4, 5+ bp units next to 0..5+ copies
+ * (24); deletions of 2, 3, 4, 5+ bp with microhomology of 1, 1-2, 1-3, 1-5
+ * bases (11). */
+#define PSX_ID_CLASSES 83u
+
 typedef struct psx_engine_params {
     uint32_t n_species;
     const double* snv_rate;     /* [n_species] SNVs per base per division */
@@ -205,7 +226,7 @@ typedef struct psx_engine_params {
     uint32_t n_preneoplastic_snv;
     uint32_t n_preneoplastic_indel;
     uint32_t preneoplastic_sbs; /* signature index of the preneoplastic SNVs */
-    uint32_t indel_max_len;     /* indel lengths are geometric on [1, indel_max_len] */
+    uint32_t indel_max_len;     /* without ID signatures: indel lengths are geometric on [1, indel_max_len] */
     double indel_len_p;
     int32_t infinite_sites;     /* MutationEngine$infinite_sites_model */
     uint32_t ploidy;            /* copies of every chromosome (2) */
@@ -215,6 +236,21 @@ typedef struct psx_engine_params {
     const psx_germline* germline;
     uint32_t n_cna;             /* copy-number drivers (ploidy <= PSX_MAX_CNA_PLOIDY) */
     const psx_cna* cna;
+    /* passenger CNAs: per edge Poisson(cna_rate(species) x ploidy x genome
+     * length) events, each a uniform catalogue region on a uniform allele
+     * (an amplification adds one copy of it); cna_rate NULL = none */
+    const double* cna_rate;     /* [n_species] passenger CNAs per base per division */
+    uint32_t n_cna_catalogue;
+    const psx_cna_region* cna_catalogue;
+    /* indel signatures (n_id > 0): an indel's ID class is drawn from the ID
+     * exposure of its node's epoch, its site uniformly among the reference
+     * sites of the class (classes without a site in the reference are left
+     * out and the rest renormalised); n_id == 0 keeps geometric-length indels
+     * at uniform positions with no cause */
+    uint32_t n_id;              /* ID signatures, PSX_ID_CLASSES classes each */
+    const double* id;           /* [n_id][PSX_ID_CLASSES] */
+    const double* id_exposure;  /* [n_epochs][n_id] */
+    uint32_t preneoplastic_id;  /* signature index of the preneoplastic indels */
 } psx_engine_params;
 
 /* One record per placed mutation; the table is sorted by SID
@@ -454,6 +490,19 @@ psx_status psx_server_totals_get(psx_server* s, psx_server_totals* t);
 const char* psx_server_last_error(psx_server* s);
 /* Finishes the running launches, cancels queued requests, frees the budget. */
 void psx_server_destroy(psx_server* s);
+
+/* -------------------------------------------------------------- measurement */
+
+/* Vector-ALU issue ceiling of the context's GPU (no counterpart in the
+ * reference; the count-only sequencing kernel is bound by vector instruction
+ * issue, not by HBM, and is reported against this). Runs one launch of
+ * `iters` loop iterations per lane over every compute unit (256-thread
+ * blocks, 8 per CU) and returns its HIP-event time; lane_iters (nullable)
+ * gets lanes x iters. PSX_PROBE_PHILOX iterations draw one Philox4x32-10
+ * block each, PSX_PROBE_INT iterations run eight independent integer
+ * add/shift/xor chains. */
+enum { PSX_PROBE_PHILOX = 0, PSX_PROBE_INT = 1 };
+psx_status psx_probe_valu(psx_ctx* ctx, int mode, uint32_t iters, double* ms, uint64_t* lane_iters);
 
 /* ------------------------------------------- synthetic forest (host input) */
 

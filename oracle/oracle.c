@@ -19,7 +19,7 @@
 
 enum {
     P_REF = 0x01, P_TEMPLATE = 0x10, P_ERROR = 0x11, P_EDGE = 0x20, P_SNV = 0x21,
-    P_SITE = 0x22, P_INDEL = 0x23, P_DRIVER = 0x25
+    P_SITE = 0x22, P_INDEL = 0x23, P_IDCLS = 0x24, P_DRIVER = 0x25, P_IDSITE = 0x26, P_PCNA = 0x27
 };
 
 #define SITE_TRIES (1u << 20)
@@ -414,6 +414,268 @@ int orc_sort_table(psx_mutation* m, uint64_t n) {
     return 0;
 }
 
+/* ------------------------------------------------------ indel (ID83) classes */
+
+/* COSMIC ID83 classes over the reference (the repeated-sequence index of the
+ * reference engine, src/mutation_engine.cpp:289-359, restated for this
+ * model). For a position q of a chromosome b[0..len) (the anchor base of the
+ * indel is b[q - 1]) and a unit length u in 1..5, U = b[q..q+u) and q is a
+ * run start when b[q-u..q) != U (q >= u, q + u <= len); rep(q, u) counts the
+ * copies of U from q on. Sites of a class (a payload selects the inserted
+ * unit of the insertions next to 0 copies):
+ *   deletion, u = 1, base class B, n copies (1..6+): run start, B(b[q]) = B,
+ *       rep = n;
+ *   deletion, u >= 2, repeats r (0..5+): run start, rep = r + 1, and for
+ *       r = 0 no microhomology;
+ *   deletion with microhomology h, u >= 2: run start, rep = 1, and the first
+ *       h bases of U equal the h bases after it (h < u);
+ *   insertion, u = 1, base class B, n copies (1..5+): run start, B(b[q]) = B,
+ *       rep = n; the inserted base is b[q];
+ *   insertion, u >= 2, n copies (1..5+): run start, rep = n; inserted U;
+ *   insertion next to 0 copies: the inserted unit s (u = 1: one of the two
+ *       bases of class B, payload bit; u >= 2: any u-mer, payload = its 2u
+ *       bits) differs from b[q..q+u) and from b[q-u..q).
+ * The deleted or inserted unit follows the anchor: record pos = q - 1. B is
+ * the pyrimidine of the base's pair (C for C/G, T for T/A). */
+
+enum { ID_DEL1 = 0, ID_INS1 = 12, ID_DELR = 24, ID_INSR = 48, ID_MH = 72 };
+#define ID_DENSE_TRIES 1024u  /* a class whose rejection needs more tries on average is listed */
+
+static const uint32_t ID_MH_BASE[6] = {0, 0, 72, 73, 75, 78}; /* first MH class of u = 2..5 */
+static const uint32_t ID_MH_MAX[6] = {0, 0, 1, 2, 3, 5};
+
+typedef struct id_class {
+    uint32_t del, u, b, n, h, zero; /* zero: insertion next to 0 copies */
+} id_class;
+
+static id_class id_decode(uint32_t c) {
+    id_class k;
+    memset(&k, 0, sizeof k);
+    if (c < ID_INS1) { k.del = 1; k.u = 1; k.b = c / 6; k.n = c % 6 + 1; }
+    else if (c < ID_DELR) { k.u = 1; k.b = (c - ID_INS1) / 6; k.n = (c - ID_INS1) % 6; k.zero = k.n == 0; }
+    else if (c < ID_INSR) { k.del = 1; k.u = 2 + (c - ID_DELR) / 6; k.n = (c - ID_DELR) % 6 + 1; }
+    else if (c < ID_MH) { k.u = 2 + (c - ID_INSR) / 6; k.n = (c - ID_INSR) % 6; k.zero = k.n == 0; }
+    else {
+        k.del = 1;
+        for (uint32_t u = 5; u >= 2; --u)
+            if (c >= ID_MH_BASE[u]) { k.u = u; k.h = c - ID_MH_BASE[u] + 1; k.n = 1; break; }
+    }
+    return k;
+}
+
+/* payloads of a class: 2 (1 bp insertion next to 0 copies), 4^u (longer
+ * insertions next to 0 copies), 1 otherwise */
+static uint32_t id_payloads(uint32_t c) {
+    id_class k = id_decode(c);
+    return k.zero ? (k.u == 1 ? 2u : 1u << (2 * k.u)) : 1u;
+}
+
+static uint32_t pyr_class(uint32_t base) { return (base == 1u || base == 2u) ? 0u : 1u; } /* C/G -> 0, T/A -> 1 */
+
+static int units_equal(const genome_view* g, uint32_t c, uint32_t a, uint32_t b, uint32_t u) {
+    for (uint32_t i = 0; i < u; ++i)
+        if (base_at(g, c, a + i) != base_at(g, c, b + i)) return 0;
+    return 1;
+}
+
+static uint32_t id_rep(const genome_view* g, uint32_t c, uint32_t q, uint32_t u, uint32_t cap) {
+    uint32_t n = 1;
+    while (n < cap && (uint64_t)q + (uint64_t)(n + 1) * u <= g->len[c] && units_equal(g, c, q, q + n * u, u)) ++n;
+    return n;
+}
+
+static uint32_t id_mh(const genome_view* g, uint32_t c, uint32_t q, uint32_t u) {
+    uint32_t h = 0;
+    while (h + 1 < u && (uint64_t)q + u + h < g->len[c] && base_at(g, c, q + h) == base_at(g, c, q + u + h)) ++h;
+    return h;
+}
+
+static uint32_t unit_at(const genome_view* g, uint32_t c, uint32_t q, uint32_t u) {
+    uint32_t s = 0;
+    for (uint32_t i = 0; i < u; ++i) s |= base_at(g, c, q + i) << (2 * i);
+    return s;
+}
+
+/* the inserted unit of a class-zero payload */
+static uint32_t id_zero_unit(uint32_t u, uint32_t b, uint32_t payload) {
+    if (u == 1) return b == 0 ? (payload & 1u ? 2u : 1u) : (payload & 1u ? 0u : 3u);
+    return payload;
+}
+
+int orc_id_is_site(const genome_view* g, uint32_t c, uint32_t q, uint32_t payload, uint32_t cls) {
+    id_class k = id_decode(cls);
+    const uint32_t u = k.u;
+    if (q < u || (uint64_t)q + u > g->len[c]) return 0;
+    if (k.zero) {
+        if (payload >= id_payloads(cls)) return 0;
+        uint32_t s = id_zero_unit(u, k.b, payload);
+        return s != unit_at(g, c, q, u) && s != unit_at(g, c, q - u, u);
+    }
+    if (units_equal(g, c, q - u, q, u)) return 0; /* not a run start */
+    if (u == 1 && pyr_class(base_at(g, c, q)) != k.b) return 0;
+    if (!k.del) return id_rep(g, c, q, u, 5) == k.n;
+    uint32_t n = id_rep(g, c, q, u, 6);
+    if (u == 1) return n == k.n;
+    if (cls >= ID_MH) {
+        uint32_t h = n == 1 ? id_mh(g, c, q, u) : 0;
+        return h >= 1 && (h < ID_MH_MAX[u] ? h : ID_MH_MAX[u]) == k.h;
+    }
+    if (n != k.n) return 0;
+    return n >= 2 || id_mh(g, c, q, u) == 0;
+}
+
+/* The classes with copies that position q is a site of (a deletion and an
+ * insertion class per unit length at a run start): written to cls, their
+ * number returned. */
+static uint32_t id_sites_at(const genome_view* g, uint32_t c, uint32_t q, uint32_t cls[10]) {
+    uint32_t n = 0;
+    for (uint32_t u = 1; u <= 5; ++u) {
+        if (q < u || (uint64_t)q + u > g->len[c]) continue;
+        uint32_t cur = unit_at(g, c, q, u);
+        if (cur == unit_at(g, c, q - u, u)) continue; /* not a run start */
+        uint32_t n6 = id_rep(g, c, q, u, 6), n5 = n6 < 5 ? n6 : 5;
+        if (u == 1) {
+            uint32_t b = pyr_class(cur);
+            cls[n++] = ID_DEL1 + b * 6 + n6 - 1;
+            cls[n++] = ID_INS1 + b * 6 + n5;
+        } else {
+            if (n6 >= 2) cls[n++] = ID_DELR + (u - 2) * 6 + n6 - 1;
+            else {
+                uint32_t h = id_mh(g, c, q, u);
+                cls[n++] = h ? ID_MH_BASE[u] + (h < ID_MH_MAX[u] ? h : ID_MH_MAX[u]) - 1 : ID_DELR + (u - 2) * 6;
+            }
+            cls[n++] = ID_INSR + (u - 2) * 6 + n5;
+        }
+    }
+    return n;
+}
+
+/* Sites of the insertion classes next to 0 copies at q: payloads counted per
+ * class (index: u = 1 of class C, u = 1 of class T, then u = 2..5). */
+static void id_zero_counts(const genome_view* g, uint32_t c, uint32_t q, uint64_t z[6]) {
+    for (uint32_t u = 1; u <= 5; ++u) {
+        if (q < u || (uint64_t)q + u > g->len[c]) continue;
+        uint32_t cur = unit_at(g, c, q, u), prev = unit_at(g, c, q - u, u);
+        if (u == 1) {
+            for (uint32_t b = 0; b < 2; ++b)
+                for (uint32_t bit = 0; bit < 2; ++bit) {
+                    uint32_t s = id_zero_unit(1, b, bit);
+                    if (s != cur && s != prev) z[b]++;
+                }
+        } else {
+            z[u] += (1ull << (2 * u)) - (cur == prev ? 1u : 2u);
+        }
+    }
+}
+
+static const uint32_t ID_ZERO_CLASS[6] = {ID_INS1, ID_INS1 + 6, ID_INSR, ID_INSR + 6, ID_INSR + 12, ID_INSR + 18};
+
+typedef struct id_index {
+    uint64_t count[PSX_ID_CLASSES];
+    uint8_t listed[PSX_ID_CLASSES];       /* sparse classes: sites listed */
+    uint64_t off[PSX_ID_CLASSES + 1];     /* list ranges per class */
+    uint64_t* list;                       /* global q << 10 | payload, ascending per class */
+} id_index;
+
+static int cmp_u64(const void* a, const void* b) {
+    uint64_t x = *(const uint64_t*)a, y = *(const uint64_t*)b;
+    return x < y ? -1 : (x > y ? 1 : 0);
+}
+
+static int id_index_build(const genome_view* g, id_index* ix) {
+    memset(ix, 0, sizeof *ix);
+    const uint64_t G = g->boff[g->n_chr];
+    for (uint32_t c = 0; c < g->n_chr; ++c) {
+        const int64_t len = g->len[c];
+#pragma omp parallel
+        {
+            uint64_t cnt[PSX_ID_CLASSES] = {0}, z[6] = {0};
+            uint32_t cls[10];
+#pragma omp for schedule(static)
+            for (int64_t q = 0; q < len; ++q) {
+                uint32_t n = id_sites_at(g, c, (uint32_t)q, cls);
+                for (uint32_t i = 0; i < n; ++i) cnt[cls[i]]++;
+                id_zero_counts(g, c, (uint32_t)q, z);
+            }
+            for (uint32_t i = 0; i < 6; ++i) cnt[ID_ZERO_CLASS[i]] += z[i];
+#pragma omp critical
+            for (uint32_t k = 0; k < PSX_ID_CLASSES; ++k) ix->count[k] += cnt[k];
+        }
+    }
+    uint64_t total = 0;
+    for (uint32_t k = 0; k < PSX_ID_CLASSES; ++k) {
+        ix->off[k] = total;
+        unsigned __int128 need = (unsigned __int128)G * id_payloads(k);
+        ix->listed[k] = ix->count[k] > 0 && (unsigned __int128)ix->count[k] * ID_DENSE_TRIES < need;
+        if (ix->listed[k]) total += ix->count[k];
+    }
+    ix->off[PSX_ID_CLASSES] = total;
+    ix->list = (uint64_t*)malloc(sizeof(uint64_t) * (total ? total : 1));
+    uint64_t fill[PSX_ID_CLASSES];
+    for (uint32_t k = 0; k < PSX_ID_CLASSES; ++k) fill[k] = ix->off[k];
+    int zero_listed = 0;
+    for (uint32_t i = 0; i < 6; ++i) zero_listed |= ix->listed[ID_ZERO_CLASS[i]];
+    for (uint32_t c = 0; c < g->n_chr && total; ++c) {
+        const int64_t len = g->len[c];
+#pragma omp parallel for schedule(static)
+        for (int64_t q = 0; q < len; ++q) {
+            uint32_t cls[10];
+            uint32_t n = id_sites_at(g, c, (uint32_t)q, cls);
+            const uint64_t gq = g->boff[c] + (uint64_t)q;
+            for (uint32_t i = 0; i < n; ++i) {
+                if (!ix->listed[cls[i]]) continue;
+                uint64_t slot;
+#pragma omp atomic capture
+                slot = fill[cls[i]]++;
+                ix->list[slot] = gq << 10;
+            }
+            if (!zero_listed) continue;
+            for (uint32_t i = 0; i < 6; ++i) {
+                const uint32_t k = ID_ZERO_CLASS[i];
+                if (!ix->listed[k]) continue;
+                const uint32_t np = id_payloads(k);
+                for (uint32_t pl = 0; pl < np; ++pl) {
+                    if (!orc_id_is_site(g, c, (uint32_t)q, pl, k)) continue;
+                    uint64_t slot;
+#pragma omp atomic capture
+                    slot = fill[k]++;
+                    ix->list[slot] = (gq << 10) | pl;
+                }
+            }
+        }
+    }
+    for (uint32_t k = 0; k < PSX_ID_CLASSES; ++k) {
+        if (fill[k] != ix->off[k + 1])
+            return fail("indel index: class %u lists %llu sites, counted %llu", k,
+                        (unsigned long long)(fill[k] - ix->off[k]), (unsigned long long)ix->count[k]);
+        qsort(ix->list + ix->off[k], ix->off[k + 1] - ix->off[k], sizeof(uint64_t), cmp_u64);
+    }
+    return 0;
+}
+
+int orc_id_index(uint32_t n_chr, const uint32_t* len, const uint32_t* packed, uint64_t* counts, uint8_t* listed,
+                 uint64_t* off, uint64_t** list) {
+    if (n_chr == 0 || n_chr > PSX_MAX_CHROMOSOMES) return fail("invalid number of chromosomes");
+    genome_view g;
+    gv_init(&g, n_chr, len, packed);
+    id_index ix;
+    int rc = id_index_build(&g, &ix);
+    if (rc) { free(ix.list); return rc; }
+    memcpy(counts, ix.count, sizeof ix.count);
+    memcpy(listed, ix.listed, sizeof ix.listed);
+    memcpy(off, ix.off, sizeof ix.off);
+    *list = ix.list;
+    return 0;
+}
+
+int orc_id_site(uint32_t n_chr, const uint32_t* len, const uint32_t* packed, uint32_t chr, uint32_t q,
+                uint32_t payload, uint32_t cls) {
+    genome_view g;
+    gv_init(&g, n_chr, len, packed);
+    if (chr >= n_chr || cls >= PSX_ID_CLASSES) return 0;
+    return orc_id_is_site(&g, chr, q, payload, cls);
+}
+
 /* ----------------------------------------------------------------- placement */
 
 typedef struct { uint32_t k0, n; uint64_t* T; } table;
@@ -435,6 +697,10 @@ typedef struct place_ctx {
     table* expo;     /* per epoch, over signatures */
     table* cls;      /* per signature, over 96 classes */
     table len_tab;   /* indel lengths, value = 1 + index */
+    int use_id;      /* indels from ID signatures */
+    id_index idx;    /* ID classes of the reference */
+    table* id_expo;  /* per epoch, over ID signatures */
+    table* id_cls;   /* per ID signature, over the classes with a site */
 } place_ctx;
 
 static const uint32_t SUB_TARGET[6] = {0, 2, 3, 0, 1, 2}; /* C>A C>G C>T T>A T>C T>G */
@@ -516,6 +782,53 @@ static int draw_indel(const place_ctx* x, uint32_t v, uint32_t k, uint32_t a, ps
     return fail("no room for an indel");
 }
 
+/* Indel from ID signatures: signature from the ID exposure at the cell's
+ * birth, ID class from the signature (over the classes with a site in the
+ * reference), allele uniform, site uniform among the class's sites: by
+ * rejection over (position, payload) for classes dense enough, by a uniform
+ * index into the class's sorted site list otherwise. */
+static int draw_indel_id(const place_ctx* x, uint32_t v, uint32_t k, uint32_t a, int fixed_sig, psx_mutation* r) {
+    uint32_t w[4];
+    philox4(x->seed, k, v, a, (uint32_t)P_IDCLS << 24, w);
+    uint32_t sig;
+    if (fixed_sig >= 0) sig = (uint32_t)fixed_sig;
+    else {
+        const table* e = &x->id_expo[epoch_of(x->p, x->f->birth[v])];
+        sig = pick(e->T, e->n, w[0]);
+    }
+    const table* ct = &x->id_cls[sig];
+    const uint32_t cls = pick(ct->T, ct->n, w[1]);
+    const uint32_t allele = mulhi32(w[2], x->p->ploidy);
+    uint64_t gq = 0;
+    uint32_t payload = 0, c = 0, q = 0;
+    if (x->idx.listed[cls]) {
+        uint32_t ws[4];
+        philox4(x->seed, k, v, a, (uint32_t)P_IDSITE << 24, ws);
+        const uint64_t e = x->idx.list[x->idx.off[cls] + mulhi64(ws[0], ws[1], x->idx.count[cls])];
+        gq = e >> 10;
+        payload = (uint32_t)(e & 1023u);
+        locate(&x->g, gq, &c, &q);
+    } else {
+        const uint32_t np = id_payloads(cls);
+        uint32_t t;
+        for (t = 0; t < SITE_TRIES; ++t) {
+            uint32_t ws[4];
+            philox4(x->seed, k, v, a, ((uint32_t)P_IDSITE << 24) | t, ws);
+            gq = mulhi64(ws[0], ws[1], x->G);
+            payload = mulhi32(ws[2], np);
+            locate(&x->g, gq, &c, &q);
+            if (orc_id_is_site(&x->g, c, q, payload, cls)) break;
+        }
+        if (t == SITE_TRIES) return fail("no reference site matches ID class %u", cls);
+    }
+    const id_class kk = id_decode(cls);
+    r->chr = c; r->pos = q - 1; r->kind = kk.del ? PSX_DEL : PSX_INS;
+    r->ref = (uint8_t)base_at(&x->g, c, q - 1); r->alt = 0; r->len = (uint8_t)kk.u;
+    r->seq = kk.zero ? id_zero_unit(kk.u, kk.b, payload) : unit_at(&x->g, c, q, kk.u);
+    r->allele = (uint8_t)allele; r->cause = (uint16_t)(x->p->n_sbs + sig);
+    return 0;
+}
+
 static int draw_slot(const place_ctx* x, const slot* s, psx_mutation* r) {
     const orc_forest* f = x->f;
     const psx_engine_params* p = x->p;
@@ -550,14 +863,16 @@ static int draw_slot(const place_ctx* x, const slot* s, psx_mutation* r) {
         r->nature = PSX_PRENEOPLASTIC; r->node = PSX_NO_NODE; r->leaf_lo = 0; r->leaf_hi = f->n_leaves;
         return rc;
     case SLOT_PRE_INDEL:
-        rc = draw_indel(x, PSX_NO_NODE, s->index, s->attempt, r);
+        rc = x->use_id ? draw_indel_id(x, PSX_NO_NODE, s->index, s->attempt, (int)p->preneoplastic_id, r)
+                       : draw_indel(x, PSX_NO_NODE, s->index, s->attempt, r);
         r->nature = PSX_PRENEOPLASTIC; r->node = PSX_NO_NODE; r->leaf_lo = 0; r->leaf_hi = f->n_leaves;
         return rc;
     case SLOT_SNV:
         rc = draw_snv(x, s->node, s->index, s->attempt, -1, r);
         break;
     default:
-        rc = draw_indel(x, s->node, s->index, s->attempt, r);
+        rc = x->use_id ? draw_indel_id(x, s->node, s->index, s->attempt, -1, r)
+                       : draw_indel(x, s->node, s->index, s->attempt, r);
         break;
     }
     r->nature = PSX_PASSENGER; r->node = s->node;
@@ -582,7 +897,37 @@ static int touches(const footprint* a, const footprint* b) {
 static void free_tables(place_ctx* x) {
     if (x->expo) for (uint32_t e = 0; e < x->p->n_epochs; ++e) free(x->expo[e].T);
     if (x->cls) for (uint32_t s = 0; s < x->p->n_sbs; ++s) free(x->cls[s].T);
-    free(x->expo); free(x->cls); free(x->len_tab.T);
+    if (x->id_expo) for (uint32_t e = 0; e < x->p->n_epochs; ++e) free(x->id_expo[e].T);
+    if (x->id_cls) for (uint32_t s = 0; s < x->p->n_id; ++s) free(x->id_cls[s].T);
+    free(x->expo); free(x->cls); free(x->len_tab.T); free(x->id_expo); free(x->id_cls); free(x->idx.list);
+}
+
+/* ID signature tables: per epoch over the signatures, and per signature over
+ * the 83 classes with every class that has no site in the reference given
+ * probability 0 (a signature left without mass is an error). */
+static int id_tables(place_ctx* x) {
+    const psx_engine_params* p = x->p;
+    int rc = id_index_build(&x->g, &x->idx);
+    x->id_expo = (table*)calloc(p->n_epochs ? p->n_epochs : 1, sizeof(table));
+    x->id_cls = (table*)calloc(p->n_id, sizeof(table));
+    for (uint32_t e = 0; e < p->n_epochs && !rc; ++e) {
+        x->id_expo[e].T = (uint64_t*)malloc(sizeof(uint64_t) * p->n_id);
+        x->id_expo[e].n = p->n_id;
+        rc = orc_cdf_table(p->id_exposure + (size_t)e * p->n_id, p->n_id, x->id_expo[e].T);
+    }
+    for (uint32_t s = 0; s < p->n_id && !rc; ++s) {
+        double pmf[PSX_ID_CLASSES];
+        double mass = 0.0;
+        for (uint32_t k = 0; k < PSX_ID_CLASSES; ++k) {
+            pmf[k] = x->idx.count[k] ? p->id[(size_t)s * PSX_ID_CLASSES + k] : 0.0;
+            mass += pmf[k];
+        }
+        if (!(mass > 0.0)) return fail("ID signature %u has no class with a site in the reference", s);
+        x->id_cls[s].T = (uint64_t*)malloc(sizeof(uint64_t) * PSX_ID_CLASSES);
+        x->id_cls[s].n = PSX_ID_CLASSES;
+        rc = orc_cdf_table(pmf, PSX_ID_CLASSES, x->id_cls[s].T);
+    }
+    return rc;
 }
 
 /* MutationEngine::place_mutations (src/mutation_engine.cpp:1085-1113): per
@@ -620,6 +965,18 @@ int orc_place_mutations(uint32_t n_chr, const uint32_t* len, const uint32_t* pac
         x.cls[s].T = (uint64_t*)malloc(sizeof(uint64_t) * 96);
         x.cls[s].n = 96;
         rc = orc_cdf_table(p->sbs + (size_t)s * 96, 96, x.cls[s].T);
+    }
+    if (p->n_id) {
+        int need_id = p->n_preneoplastic_indel > 0;
+        for (uint32_t s = 0; s < p->n_species; ++s) need_id |= p->indel_rate[s] > 0.0;
+        if (!p->id || (need_id && (!p->id_exposure || p->n_epochs == 0)))
+            rc = fail("indels need an ID signature and an exposure");
+        else if (p->n_preneoplastic_indel > 0 && p->preneoplastic_id >= p->n_id)
+            rc = fail("unknown preneoplastic ID signature");
+        else if (need_id) {
+            x.use_id = 1;
+            rc = id_tables(&x);
+        }
     }
     double lp[10];
     for (uint32_t l = 1; l <= p->indel_max_len; ++l)
@@ -743,26 +1100,63 @@ int orc_place_mutations(uint32_t n_chr, const uint32_t* len, const uint32_t* pac
 /* ------------------------------------------------------ copy-number drivers */
 
 int orc_place_cna(uint32_t n_chr, const uint32_t* len, const orc_forest* f, const psx_engine_params* p,
-                  psx_cna_event** out, uint32_t* n_out) {
+                  uint64_t seed, psx_cna_event** out, uint32_t* n_out) {
     *out = NULL;
     *n_out = 0;
     if (p->n_cna && p->ploidy > PSX_MAX_CNA_PLOIDY) return fail("copy-number events need a small ploidy");
-    uint32_t cap = 0;
+    uint64_t G = 0;
+    for (uint32_t c = 0; c < n_chr; ++c) G += len[c];
+    uint64_t cap = 0;
     for (uint32_t k = 0; k < p->n_cna; ++k) {
         const psx_cna* c = &p->cna[k];
         int wgd = c->kind == PSX_CNA_WGD;
         if (c->kind > PSX_CNA_WGD || c->species >= p->n_species ||
             (!wgd && (c->chr >= n_chr || c->len == 0 || c->start >= len[c->chr] || c->len > len[c->chr] - c->start ||
-                      c->allele >= p->ploidy || (c->kind == PSX_CNA_AMP && c->copies == 0))))
+                      (c->allele >= p->ploidy && c->allele != PSX_CNA_RANDOM_ALLELE) ||
+                      (c->kind == PSX_CNA_AMP && c->copies == 0))))
             return fail("invalid copy-number driver %u", k);
         for (uint32_t v = 0; v < f->n_nodes; ++v) {
             uint32_t par = f->parent[v];
             if (f->species[v] == c->species && (par == PSX_NO_NODE || f->species[par] != c->species)) ++cap;
         }
     }
+    /* passenger CNAs (src/mutation_engine.cpp:441-487, 901-927): per non-root node Poisson(cna_rate(species) x
+     * ploidy x G) events, the count from word z of the node's edge draw, each event a uniform region of the
+     * catalogue on a uniform allele; an amplification adds one copy */
+    int any = 0;
+    for (uint32_t s = 0; s < p->n_species && p->cna_rate; ++s) {
+        if (!(p->cna_rate[s] >= 0.0) || !isfinite(p->cna_rate[s])) return fail("CNA rates must be non-negative");
+        any |= p->cna_rate[s] > 0.0;
+    }
+    uint32_t* npass = (uint32_t*)calloc(f->n_nodes ? f->n_nodes : 1, sizeof(uint32_t));
+    if (any) {
+        if (p->n_cna_catalogue == 0 || !p->cna_catalogue) { free(npass); return fail("passenger CNAs need a CNA catalogue"); }
+        for (uint32_t k = 0; k < p->n_cna_catalogue; ++k) {
+            const psx_cna_region* r = &p->cna_catalogue[k];
+            if (r->chr >= n_chr || r->len == 0 || r->start >= len[r->chr] || r->len > len[r->chr] - r->start ||
+                (r->kind != PSX_CNA_AMP && r->kind != PSX_CNA_DEL)) {
+                free(npass);
+                return fail("invalid passenger CNA catalogue region %u", k);
+            }
+        }
+        double allelic = (double)((uint64_t)p->ploidy * G);
+        uint64_t* T = (uint64_t*)malloc(sizeof(uint64_t) * TABLE_CAP);
+        for (uint32_t v = 0; v < f->n_nodes; ++v) {
+            if (f->parent[v] == PSX_NO_NODE) continue;
+            double lam = p->cna_rate[f->species[v]] * allelic;
+            if (lam == 0.0) continue;
+            uint32_t k0, n;
+            if (orc_poisson_table(lam, &k0, &n, T, TABLE_CAP)) { free(T); free(npass); return -1; }
+            uint32_t w[4];
+            philox4(seed, v, 0, 0, (uint32_t)P_EDGE << 24, w);
+            npass[v] = k0 + pick(T, n, w[2]);
+            cap += npass[v];
+        }
+        free(T);
+    }
     psx_cna_event* ev = (psx_cna_event*)calloc(cap ? cap : 1, sizeof(psx_cna_event));
     uint32_t n = 0;
-    /* nodes in DFS order; at one node, drivers in their given order */
+    /* nodes in DFS order; at one node, drivers in their given order, then the passengers */
     for (uint32_t v = 0; v < f->n_nodes; ++v) {
         uint32_t par = f->parent[v];
         for (uint32_t k = 0; k < p->n_cna; ++k) {
@@ -777,10 +1171,31 @@ int orc_place_cna(uint32_t n_chr, const uint32_t* len, const orc_forest* f, cons
             e->start = wgd ? 0 : c->start;
             e->end = wgd ? 0 : c->start + c->len;
             e->allele = wgd ? 0 : c->allele;
+            if (!wgd && c->allele == PSX_CNA_RANDOM_ALLELE) {
+                uint32_t w[4];
+                philox4(seed, k, v, 1, (uint32_t)P_DRIVER << 24, w);
+                e->allele = (uint8_t)mulhi32(w[0], p->ploidy);
+            }
             e->kind = c->kind;
             e->copies = c->kind == PSX_CNA_AMP ? c->copies : 0;
         }
+        for (uint32_t k = 0; k < npass[v]; ++k) {
+            uint32_t w[4];
+            philox4(seed, k, v, 0, (uint32_t)P_PCNA << 24, w);
+            const psx_cna_region* r = &p->cna_catalogue[mulhi32(w[0], p->n_cna_catalogue)];
+            psx_cna_event* e = &ev[n++];
+            e->node = v;
+            e->leaf_lo = f->leaf_lo[v];
+            e->leaf_hi = f->leaf_hi[v];
+            e->chr = r->chr;
+            e->start = r->start;
+            e->end = r->start + r->len;
+            e->allele = (uint8_t)mulhi32(w[1], p->ploidy);
+            e->kind = r->kind;
+            e->copies = r->kind == PSX_CNA_AMP ? 1 : 0;
+        }
     }
+    free(npass);
     *out = ev;
     *n_out = n;
     return 0;

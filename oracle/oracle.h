@@ -57,9 +57,20 @@ int orc_place_mutations(uint32_t n_chr, const uint32_t* len, const uint32_t* pac
 
 /* Copy-number drivers (psx_engine_params.cna) placed on the edge into every
  * node where their species first appears, in DFS order of the node, ties in
- * driver order (src/mutation_engine.cpp:1185-1210 for the driver order). */
+ * driver order (src/mutation_engine.cpp:1185-1210 for the driver order), and
+ * the passenger CNAs of every non-root edge after that node's drivers. */
 int orc_place_cna(uint32_t n_chr, const uint32_t* len, const orc_forest* f, const psx_engine_params* p,
-                  psx_cna_event** out, uint32_t* n_out);
+                  uint64_t seed, psx_cna_event** out, uint32_t* n_out);
+
+/* Indel (ID83) classes of a reference: sites per class, the classes whose
+ * sites are listed (sparse: rejection would need more than 1024 tries on
+ * average), list ranges off[84] and the lists (global position << 10 |
+ * payload, ascending per class; free with orc_free). */
+int orc_id_index(uint32_t n_chr, const uint32_t* len, const uint32_t* packed, uint64_t* counts,
+                 uint8_t* listed, uint64_t* off, uint64_t** list);
+/* 1 when (chromosome position q, payload) is a site of ID class cls. */
+int orc_id_site(uint32_t n_chr, const uint32_t* len, const uint32_t* packed, uint32_t chr, uint32_t q,
+                uint32_t payload, uint32_t cls);
 
 /* Sorts an arbitrary table into SID order and marks sid_head. */
 int orc_sort_table(psx_mutation* m, uint64_t n);

@@ -44,7 +44,9 @@ def lib():
             "orc_place_mutations": (st, [C.c_uint32, A.u32p, A.u32p, C.POINTER(OrcForest),
                                          C.POINTER(A.EngineParams), C.c_uint64, C.POINTER(C.c_void_p), A.u64p]),
             "orc_sort_table": (st, [vp, C.c_uint64]),
-            "orc_place_cna": (st, [C.c_uint32, A.u32p, C.POINTER(OrcForest), C.POINTER(A.EngineParams),
+            "orc_id_index": (st, [C.c_uint32, A.u32p, A.u32p, A.u64p, A.u8p, A.u64p, C.POINTER(C.c_void_p)]),
+            "orc_id_site": (C.c_int, [C.c_uint32, A.u32p, A.u32p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]),
+            "orc_place_cna": (st, [C.c_uint32, A.u32p, C.POINTER(OrcForest), C.POINTER(A.EngineParams), C.c_uint64,
                                    C.POINTER(C.c_void_p), A.u32p]),
             "orc_simulate_seq_cna": (st, [C.c_uint32, A.u32p, C.POINTER(OrcForest), vp, C.c_uint64, C.c_uint32,
                                           vp, C.c_uint32, C.POINTER(A.SeqParams), C.c_uint64, A.i32p, A.i32p,
@@ -133,6 +135,7 @@ class Forest:
         n = f.n_nodes
         self.cell_id = np.ctypeslib.as_array(f.cell_id, (n,)).copy()
         self.parent = np.ctypeslib.as_array(f.parent, (n,)).copy()
+        self.species = np.ctypeslib.as_array(f.species, (n,)).copy()
         self.leaf_lo = np.ctypeslib.as_array(f.leaf_lo, (n,)).copy()
         self.leaf_hi = np.ctypeslib.as_array(f.leaf_hi, (n,)).copy()
         self.sample_begin = np.ctypeslib.as_array(f.sample_begin, (f.n_samples + 1,)).copy()
@@ -162,19 +165,45 @@ def place_mutations(lens, packed, forest: Forest, seed: int, **engine) -> np.nda
     return arr
 
 
-def place_cna(lens, forest: Forest, **engine) -> np.ndarray:
-    """Copy-number drivers placed on the forest (CNA_EVENT_DTYPE, DFS order)."""
+def place_cna(lens, forest: Forest, seed: int = 0, **engine) -> np.ndarray:
+    """Copy-number drivers and passenger CNAs placed on the forest
+    (CNA_EVENT_DTYPE, DFS order); seed: the placement seed (passengers)."""
     L = np.ascontiguousarray(lens, np.uint32)
     params, keep = A.engine_params(**engine)
     out = C.c_void_p()
     n = C.c_uint32()
-    _check(lib().orc_place_cna(len(L), A.ptr(L, C.c_uint32), forest.p, C.byref(params), C.byref(out), C.byref(n)))
+    _check(lib().orc_place_cna(len(L), A.ptr(L, C.c_uint32), forest.p, C.byref(params), C.c_uint64(seed),
+                               C.byref(out), C.byref(n)))
     nb = n.value * A.CNA_EVENT_DTYPE.itemsize
     arr = np.frombuffer(C.string_at(out, nb), dtype=A.CNA_EVENT_DTYPE).copy() if n.value else \
         np.zeros(0, A.CNA_EVENT_DTYPE)
     lib().orc_free(out)
     del keep
     return arr
+
+
+def id_index(lens, packed) -> dict:
+    """Indel (ID83) classes of a reference: counts[83] sites per class,
+    listed[83] (sites listed), off[84] list ranges and the lists (global
+    position << 10 | payload, ascending per class)."""
+    L = np.ascontiguousarray(lens, np.uint32)
+    pk = np.ascontiguousarray(packed, np.uint32)
+    counts = np.zeros(A.PSX_ID_CLASSES, np.uint64)
+    listed = np.zeros(A.PSX_ID_CLASSES, np.uint8)
+    off = np.zeros(A.PSX_ID_CLASSES + 1, np.uint64)
+    out = C.c_void_p()
+    _check(lib().orc_id_index(len(L), A.ptr(L, C.c_uint32), A.ptr(pk, C.c_uint32), A.ptr(counts, C.c_uint64),
+                              A.ptr(listed, C.c_uint8), A.ptr(off, C.c_uint64), C.byref(out)))
+    n = int(off[-1])
+    lst = np.frombuffer(C.string_at(out, 8 * n), np.uint64).copy() if n else np.zeros(0, np.uint64)
+    lib().orc_free(out)
+    return dict(counts=counts, listed=listed.astype(bool), off=off, list=lst)
+
+
+def id_site(lens, packed, chr_: int, q: int, payload: int, cls: int) -> bool:
+    L = np.ascontiguousarray(lens, np.uint32)
+    pk = np.ascontiguousarray(packed, np.uint32)
+    return bool(lib().orc_id_site(len(L), A.ptr(L, C.c_uint32), A.ptr(pk, C.c_uint32), chr_, q, payload, cls))
 
 
 def sort_table(recs: np.ndarray) -> np.ndarray:

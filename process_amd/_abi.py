@@ -42,12 +42,21 @@ class Germline(C.Structure):
 
 PSX_CNA_AMP, PSX_CNA_DEL, PSX_CNA_WGD = 0, 1, 2
 PSX_ALL_CHROMOSOMES = 0xFFFFFFFF
+PSX_CNA_RANDOM_ALLELE = 0xFF
+PSX_PROBE_PHILOX, PSX_PROBE_INT = 0, 1
 
 
 class Cna(C.Structure):
     _fields_ = [("species", C.c_uint32), ("chr", C.c_uint32), ("start", C.c_uint32), ("len", C.c_uint32),
                 ("allele", C.c_uint8), ("kind", C.c_uint8), ("copies", C.c_uint8), ("pad", C.c_uint8)]
 
+
+class CnaRegion(C.Structure):
+    _fields_ = [("chr", C.c_uint32), ("start", C.c_uint32), ("len", C.c_uint32), ("kind", C.c_uint8),
+                ("pad", C.c_uint8 * 3)]
+
+
+PSX_ID_CLASSES = 83
 
 CNA_EVENT_DTYPE = np.dtype([("node", "<u4"), ("leaf_lo", "<u4"), ("leaf_hi", "<u4"), ("chr", "<u4"),
                             ("start", "<u4"), ("end", "<u4"), ("allele", "u1"), ("kind", "u1"), ("copies", "u1"),
@@ -62,7 +71,9 @@ class EngineParams(C.Structure):
                 ("indel_max_len", C.c_uint32), ("indel_len_p", C.c_double), ("infinite_sites", C.c_int32),
                 ("ploidy", C.c_uint32), ("n_drivers", C.c_uint32), ("drivers", C.POINTER(Driver)),
                 ("n_germline", C.c_uint32), ("germline", C.POINTER(Germline)),
-                ("n_cna", C.c_uint32), ("cna", C.POINTER(Cna))]
+                ("n_cna", C.c_uint32), ("cna", C.POINTER(Cna)),
+                ("cna_rate", f64p), ("n_cna_catalogue", C.c_uint32), ("cna_catalogue", C.POINTER(CnaRegion)),
+                ("n_id", C.c_uint32), ("id", f64p), ("id_exposure", f64p), ("preneoplastic_id", C.c_uint32)]
 
 
 class SeqParams(C.Structure):
@@ -139,8 +150,11 @@ def forest_desc(cell_id, parent, species, sample, birth, n_samples) -> tuple[For
 
 def engine_params(*, snv_rate, indel_rate, sbs, epoch_start, exposure, n_pre_snv=0, n_pre_indel=0,
                   pre_sbs=0, indel_max_len=10, indel_len_p=0.5, infinite_sites=True, ploidy=2,
-                  drivers=(), germline=(), cna=()) -> tuple[EngineParams, list]:
-    """cna: (species, chr, start, len, allele, kind, copies) tuples (kind PSX_CNA_*)."""
+                  drivers=(), germline=(), cna=(), cna_rate=None, cna_catalogue=(), id_sig=None,
+                  id_exposure=None, pre_id=0) -> tuple[EngineParams, list]:
+    """cna: (species, chr, start, len, allele, kind, copies) tuples (kind PSX_CNA_*); cna_rate: passenger
+    CNAs per base per division per species (None: none); cna_catalogue: (chr, start, len, kind) regions;
+    id_sig: [n_id][83] indel signatures; id_exposure: [n_epochs][n_id]."""
     snv = np.ascontiguousarray(snv_rate, np.float64)
     ind = np.ascontiguousarray(indel_rate, np.float64)
     sbs_a = np.ascontiguousarray(sbs, np.float64).reshape(-1, 96)
@@ -155,12 +169,21 @@ def engine_params(*, snv_rate, indel_rate, sbs, epoch_start, exposure, n_pre_snv
     cn = (Cna * max(1, len(cna)))()
     for i, c in enumerate(cna):
         cn[i] = Cna(*(int(v) for v in c[:7]))
+    crate = None if cna_rate is None else np.ascontiguousarray(cna_rate, np.float64)
+    cat = (CnaRegion * max(1, len(cna_catalogue)))()
+    for i, r in enumerate(cna_catalogue):
+        cat[i] = CnaRegion(int(r[0]), int(r[1]), int(r[2]), int(r[3]))
+    ids = None if id_sig is None else np.ascontiguousarray(id_sig, np.float64).reshape(-1, PSX_ID_CLASSES)
+    n_id = 0 if ids is None else ids.shape[0]
+    idx = None if ids is None else np.ascontiguousarray(id_exposure, np.float64).reshape(len(es), n_id)
     p = EngineParams(len(snv), ptr(snv, C.c_double), ptr(ind, C.c_double), sbs_a.shape[0], ptr(sbs_a, C.c_double),
                      len(es), ptr(es, C.c_double), ptr(np.ascontiguousarray(ex), C.c_double), int(n_pre_snv),
                      int(n_pre_indel), int(pre_sbs), int(indel_max_len), float(indel_len_p), int(bool(infinite_sites)),
                      int(ploidy), len(drivers), C.cast(drv, C.POINTER(Driver)), len(germline),
-                     C.cast(grm, C.POINTER(Germline)), len(cna), C.cast(cn, C.POINTER(Cna)))
-    return p, [snv, ind, sbs_a, es, ex, drv, grm, cn]
+                     C.cast(grm, C.POINTER(Germline)), len(cna), C.cast(cn, C.POINTER(Cna)),
+                     ptr(crate, C.c_double), len(cna_catalogue), C.cast(cat, C.POINTER(CnaRegion)),
+                     n_id, ptr(ids, C.c_double), ptr(idx, C.c_double), int(pre_id))
+    return p, [snv, ind, sbs_a, es, ex, drv, grm, cn, crate, cat, ids, idx]
 
 
 def seq_params(*, coverage=10.0, read_size=150, insert_size_mean=0, insert_size_stddev=10, error_rate=0.0,

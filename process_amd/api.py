@@ -113,6 +113,92 @@ class SNV:
     allele: int | None = None
 
 
+class CNA:
+    """A copy-number alteration driver, CNA(type, chr, chr_pos, len, allele,
+    src_allele) (src/mutations.cpp:198-262, src/cna.cpp:100-138): type "A"
+    (amplification of the src_allele lineage) or "D" (deletion of the allele
+    lineage); chr_pos is 1-based; an allele left None is drawn uniformly at
+    placement."""
+
+    def __init__(self, type: str, chr, chr_pos, len, allele=None, src_allele=None):
+        if type not in ("A", "D"):
+            raise ValueError(f"Unknown CNA type \"{type}\". Supported types are \"A\" and \"D\" for amplification "
+                             "and deletion, respectively")
+        if int(chr_pos) < 0:
+            raise ValueError("Position in chromosome must be a non-negative number")
+        if int(len) < 0:
+            raise ValueError("Region lengths must be non-negative numbers")
+        self.type, self.chr, self.chr_pos, self.len = type, str(chr), int(chr_pos), int(len)
+        self.allele = None if allele is None else int(allele)
+        self.src_allele = None if src_allele is None else int(src_allele)
+
+    def lineage(self):
+        """The allele lineage the event acts on: the source of an amplification, the allele of a deletion."""
+        return self.src_allele if self.type == "A" else self.allele
+
+    def __repr__(self):
+        return f"CNA(\"{self.type}\", \"{self.chr}\", {self.chr_pos}, {self.len})"
+
+
+def Amplification(chr, chr_pos, len, allele=None, src_allele=None) -> CNA:
+    """src/mutations.cpp:224-240."""
+    return CNA("A", chr, chr_pos, len, allele, src_allele)
+
+
+def Deletion(chr, chr_pos, len, allele=None) -> CNA:
+    """src/mutations.cpp:244-258."""
+    return CNA("D", chr, chr_pos, len, allele)
+
+
+class _WGD:
+    """Whole-genome doubling driver (src/wg_doubling.hpp, src/mutations.cpp:355-365)."""
+
+    def __repr__(self):
+        return "WGD"
+
+
+WGD = _WGD()
+
+
+def load_passenger_CNAs(path: str, tumour_type: str = "", tumour_study: str = "") -> list[tuple[str, int, int, str]]:
+    """The passenger CNA catalogue of a tab-separated file with a header
+    (load_passenger_CNAs, src/mutation_engine.cpp:397-487): columns chr,
+    begin, end, major, minor, tumour type, study; rows of other tumour types
+    or studies are skipped when those are given; a row gives an amplification
+    when major is NA or > 1 and a deletion when minor is NA or < 1, over the
+    1-based region [begin, end]. Returns (chr, begin, end, "A"/"D") tuples,
+    without duplicates, in (chr, begin, length, type) order."""
+    out = set()
+    with open(path) as f:
+        lines = f.read().splitlines()
+    for row_num, line in enumerate(lines[1:], start=2):
+        if not line.strip():
+            continue
+        row = line.split("\t")
+        if len(row) < 7:
+            raise ValueError("The CNA CSV must contains at least 7 columns")
+        if (tumour_type and row[5] != tumour_type) or (tumour_study and row[6] != tumour_study):
+            continue
+        for k, what in ((1, "begin"), (2, "end")):
+            if not row[k].strip().isdigit():
+                raise ValueError(f"Unknown {what} specification {row[k]} in row number {row_num}.")
+        begin, end = int(row[1]), int(row[2])
+        if begin > end:
+            raise ValueError(f"The CNA begin lays after the end in row number {row_num}")
+        major, minor = row[3], row[4]
+        try:
+            if major == "NA" or int(major) > 1:
+                out.add((row[0], begin, end, "A"))
+        except ValueError:
+            raise ValueError(f"Unknown major specification {major} in row number {row_num}.") from None
+        try:
+            if minor == "NA" or int(minor) < 1:
+                out.add((row[0], begin, end, "D"))
+        except ValueError:
+            raise ValueError(f"Unknown minor specification {major} in row number {row_num}.") from None
+    return sorted(out, key=lambda r: (r[0], r[1], r[2] - r[1], r[3]))
+
+
 class ReferenceGenome:
     """Chromosome names and lengths with a 2-bit reference: synthetic i.i.d.
     uniform ACGT from a seed, or bases given explicitly."""
@@ -141,17 +227,37 @@ class ReferenceGenome:
 
 class MutationEngine:
     """The mutation engine (src/mutation_engine.hpp:35-52): species passenger
-    rates, drivers, timed SBS exposures, the infinite-sites flag."""
+    rates (SNV, indel, CNA), drivers (SNVs, CNAs, whole-genome doublings),
+    timed SBS and ID exposures, the passenger CNA catalogue and the
+    infinite-sites flag."""
 
     def __init__(self, reference: ReferenceGenome, SBS_signatures: Mapping[str, Sequence[float]],
                  germline: Iterable[tuple[str, int, str, int]] = (), ctx: core.Context | None = None,
-                 indel_max_len: int = 10, indel_len_p: float = 0.5):
+                 indel_max_len: int = 10, indel_len_p: float = 0.5,
+                 ID_signatures: Mapping[str, Sequence[float]] | None = None, passenger_CNAs=None,
+                 tumour_type: str = "", tumour_study: str = ""):
+        """ID_signatures: COSMIC ID83 signatures (83 classes each); without
+        them indels have geometric lengths and uniform positions.
+        passenger_CNAs: the passenger CNA catalogue, a tab-separated file
+        (load_passenger_CNAs, filtered by tumour_type and tumour_study) or
+        (chr, begin, end, "A"/"D") tuples with 1-based inclusive regions."""
         self._ctx = ctx
         self.reference = reference
         self.signature_names = list(SBS_signatures)
         self.signatures = np.array([np.asarray(SBS_signatures[n], np.float64) for n in self.signature_names])
         if self.signatures.shape[1:] != (96,):
             raise ValueError("SBS signatures must have 96 classes")
+        self.id_signature_names = list(ID_signatures or {})
+        self.id_signatures = np.array([np.asarray(ID_signatures[n], np.float64) for n in self.id_signature_names]) \
+            if ID_signatures else np.zeros((0, A.PSX_ID_CLASSES))
+        if self.id_signatures.shape[1:] != (A.PSX_ID_CLASSES,):
+            raise ValueError(f"ID signatures must have {A.PSX_ID_CLASSES} classes")
+        overlap = set(self.signature_names) & set(self.id_signature_names)
+        if overlap:
+            raise ValueError(f"signature names used twice: {sorted(overlap)}")
+        if isinstance(passenger_CNAs, (str, os.PathLike)):
+            passenger_CNAs = load_passenger_CNAs(str(passenger_CNAs), tumour_type, tumour_study)
+        self.passenger_CNAs = [(str(c), int(b), int(e), str(t)) for c, b, e, t in (passenger_CNAs or [])]
         self.mutants: dict[str, dict] = {}
         self.exposures: list[tuple[float, dict]] = []
         self.infinite_sites_model = True
@@ -178,31 +284,46 @@ class MutationEngine:
         off = int(sum((int(l) + 15) // 16 for l in self.reference.lengths[:chr_idx]))
         return int((self._packed[off + pos0 // 16] >> (2 * (pos0 % 16))) & 3)
 
-    def add_mutant(self, mutant_name: str, passenger_rates: Mapping, drivers: Sequence[SNV] = ()):
-        """src/mutation_engine.cpp:1055-1083: ``passenger_rates`` is c(SNV=, indel=)
-        or a per-epistate list of them."""
+    def add_mutant(self, mutant_name: str, passenger_rates: Mapping, drivers: Sequence = ()):
+        """src/mutation_engine.cpp:1055-1083 and :901-944: ``passenger_rates``
+        is c(SNV=, indel=, CNA=) or a per-epistate list of them; ``drivers``
+        holds SNVs, CNAs and WGD, applied in the given order."""
         rates = dict(passenger_rates)
         if rates and all(isinstance(v, Mapping) for v in rates.values()):
             per_epi = {k: dict(v) for k, v in rates.items()}
         else:
             per_epi = {"": rates}
         for r in per_epi.values():
-            for k in r:
+            for k, v in r.items():
                 if k not in ("SNV", "indel", "CNA"):
-                    raise ValueError(f"Unsupported passenger rate \"{k}\"")
+                    raise ValueError(f"\"{k}\" is an unsupported name in passenger rates list.")
+                if not (float(v) >= 0.0):
+                    raise ValueError(f": {k} rates must be non-negative")
+        for d in drivers:
+            if not isinstance(d, (SNV, CNA, _WGD)):
+                raise ValueError(f"unsupported driver {d!r}: SNV, CNA or WGD")
         self.mutants[mutant_name] = {"rates": per_epi, "drivers": list(drivers)}
 
     def add_exposure(self, time_or_coefficients, coefficients: Mapping[str, float] | None = None):
-        """src/mutation_engine.cpp:734-739: exposure from a simulated time on."""
+        """src/mutation_engine.cpp:734-739: exposure from a simulated time on;
+        SBS and ID coefficients may share one list, each part summing to 1. A
+        part left out keeps the exposure of the earlier time."""
         if coefficients is None:
             time, coefficients = 0.0, time_or_coefficients
         else:
             time = float(time_or_coefficients)
-        coef = {k: float(v) for k, v in coefficients.items() if k in self.signature_names}
-        if not coef:
-            raise ValueError("The exposure names no known SBS signature")
-        if abs(sum(coef.values()) - 1.0) > 1e-6:
-            raise ValueError("The SBS exposure coefficients must sum to 1")
+        sbs = {k: float(v) for k, v in coefficients.items() if k in self.signature_names}
+        ids = {k: float(v) for k, v in coefficients.items() if k in self.id_signature_names}
+        unknown = [k for k in coefficients if k not in sbs and k not in ids]
+        if unknown or not (sbs or ids):
+            raise ValueError("The exposure names no known SBS signature" if not (sbs or ids) else
+                             f"Unknown signature {unknown[0]}")
+        for part in (sbs, ids):
+            if part and abs(sum(part.values()) - 1.0) > 1e-6:
+                raise ValueError("The SBS exposure coefficients must sum to 1" if part is sbs else
+                                 "The ID exposure coefficients must sum to 1")
+        coef = dict(sbs)
+        coef.update(ids)
         self.exposures = [e for e in self.exposures if e[0] != time] + [(time, coef)]
         self.exposures.sort(key=lambda e: e[0])
 
@@ -210,11 +331,28 @@ class MutationEngine:
         rows = []
         for m, d in self.mutants.items():
             for epi, r in d["rates"].items():
-                rows.append({"mutant": m, "epistate": epi, "SNV": r.get("SNV", 0.0), "indel": r.get("indel", 0.0)})
+                rows.append({"mutant": m, "epistate": epi, "SNV": r.get("SNV", 0.0), "indel": r.get("indel", 0.0),
+                             "CNA": r.get("CNA", 0.0)})
         return pd.DataFrame(rows)
 
-    def _params(self, forest: SamplesForest, n_pre_snv: int, pre_sig: str, n_pre_indel: int) -> dict:
-        snv, ind = [], []
+    def _epoch_exposures(self, names: list[str]) -> tuple[list[float], np.ndarray] | None:
+        """Per exposure time, the coefficients over `names`: a time whose list
+        names none of them keeps the previous time's (None if no time does)."""
+        times, rows, cur = [], [], None
+        for t, coef in self.exposures:
+            part = [coef.get(n, 0.0) for n in names] if any(n in coef for n in names) else None
+            if part is not None:
+                cur = part
+            times.append(t)
+            rows.append(cur)
+        if cur is None:
+            return None
+        first = next(r for r in rows if r is not None)
+        return times, np.array([r if r is not None else first for r in rows], np.float64)
+
+    def _params(self, forest: SamplesForest, n_pre_snv: int, pre_sig: str, n_pre_indel: int,
+                pre_id_sig: str = "ID1") -> dict:
+        snv, ind, cna = [], [], []
         for mutant, epi in forest.species:
             if mutant not in self.mutants:
                 raise ValueError(f"The mutant \"{mutant}\" has no passenger rates in the mutation engine")
@@ -222,33 +360,68 @@ class MutationEngine:
             r = rates.get(epi, rates.get("", {}))
             snv.append(float(r.get("SNV", 0.0)))
             ind.append(float(r.get("indel", 0.0)))
-            if float(r.get("CNA", 0.0)) != 0.0:
-                raise NotImplementedError(f"passenger CNA rates are not supported (mutant \"{mutant}\")")
+            cna.append(float(r.get("CNA", 0.0)))
         if n_pre_snv and pre_sig not in self.signature_names:
             raise ValueError(f"Unknown SBS signature \"{pre_sig}\"")
-        if not self.exposures and (any(snv) or n_pre_snv):
+        sbs_expo = self._epoch_exposures(self.signature_names)
+        if sbs_expo is None and (any(snv) or n_pre_snv):
             raise ValueError("No SBS exposure has been added to the mutation engine")
+        use_id = len(self.id_signature_names) > 0
+        id_expo = self._epoch_exposures(self.id_signature_names) if use_id else None
+        if use_id:
+            if n_pre_indel and pre_id_sig not in self.id_signature_names:
+                raise ValueError(f"Unknown ID signature \"{pre_id_sig}\"")
+            if id_expo is None and any(ind):
+                raise ValueError("No ID exposure has been added to the mutation engine")
+        if any(cna) and not self.passenger_CNAs:
+            raise ValueError("Passenger CNA rates need a passenger CNA catalogue")
         sp_index = {s: i for i, s in enumerate(forest.species)}
-        drivers = []
+        drivers, cna_drivers = [], []
         for mutant, d in self.mutants.items():
-            for snv_spec in d["drivers"]:
-                c = self.reference.chr_index(snv_spec.chr)
-                pos0 = int(snv_spec.chr_pos) - 1
-                if snv_spec.ref is not None and BASES[self.base(c, pos0)] != snv_spec.ref:
-                    raise ValueError(f"The reference base of {snv_spec} does not match the reference genome")
-                for (m, e), s in sp_index.items():
-                    if m == mutant:
-                        drivers.append((s, c, pos0, "ACGT".index(snv_spec.alt),
-                                        -1 if snv_spec.allele is None else int(snv_spec.allele)))
+            species = [s for (m, e), s in sp_index.items() if m == mutant]
+            for spec in d["drivers"]:
+                if isinstance(spec, SNV):
+                    c = self.reference.chr_index(spec.chr)
+                    pos0 = int(spec.chr_pos) - 1
+                    if spec.ref is not None and BASES[self.base(c, pos0)] != spec.ref:
+                        raise ValueError(f"The reference base of {spec} does not match the reference genome")
+                    for s in species:
+                        drivers.append((s, c, pos0, "ACGT".index(spec.alt),
+                                        -1 if spec.allele is None else int(spec.allele)))
+                elif isinstance(spec, CNA):
+                    c = self.reference.chr_index(spec.chr)
+                    lin = spec.lineage()
+                    for s in species:
+                        cna_drivers.append((s, c, max(spec.chr_pos, 1) - 1, spec.len,
+                                            A.PSX_CNA_RANDOM_ALLELE if lin is None else lin,
+                                            A.PSX_CNA_AMP if spec.type == "A" else A.PSX_CNA_DEL,
+                                            1 if spec.type == "A" else 0))
+                else:
+                    for s in species:
+                        cna_drivers.append((s, 0, 0, 0, 0, A.PSX_CNA_WGD, 0))
         germ = [(self.reference.chr_index(c), int(p) - 1, "ACGT".index(a), int(al)) for c, p, a, al in self.germline]
+        # one epoch per exposure time: SBS and ID parts each from the latest time that set them
         epochs = [t for t, _ in self.exposures] or [0.0]
-        expo = np.array([[e.get(n, 0.0) for n in self.signature_names] for _, e in self.exposures]) \
-            if self.exposures else np.ones((1, len(self.signature_names))) / len(self.signature_names)
-        return dict(snv_rate=snv, indel_rate=ind, sbs=self.signatures, epoch_start=epochs, exposure=expo,
-                    n_pre_snv=n_pre_snv, n_pre_indel=n_pre_indel,
-                    pre_sbs=self.signature_names.index(pre_sig) if pre_sig in self.signature_names else 0,
-                    indel_max_len=self.indel_max_len, indel_len_p=self.indel_len_p,
-                    infinite_sites=bool(self.infinite_sites_model), ploidy=2, drivers=drivers, germline=germ)
+        n_ep = len(epochs)
+        expo = sbs_expo[1] if sbs_expo is not None else np.ones((n_ep, len(self.signature_names))) / \
+            max(1, len(self.signature_names))
+        out = dict(snv_rate=snv, indel_rate=ind, sbs=self.signatures, epoch_start=epochs, exposure=expo,
+                   n_pre_snv=n_pre_snv, n_pre_indel=n_pre_indel,
+                   pre_sbs=self.signature_names.index(pre_sig) if pre_sig in self.signature_names else 0,
+                   indel_max_len=self.indel_max_len, indel_len_p=self.indel_len_p,
+                   infinite_sites=bool(self.infinite_sites_model), ploidy=2, drivers=drivers, germline=germ,
+                   cna=cna_drivers)
+        if any(cna):
+            cat = [(self.reference.chr_index(c), b - 1, e + 1 - b, A.PSX_CNA_AMP if t == "A" else A.PSX_CNA_DEL)
+                   for c, b, e, t in self.passenger_CNAs]
+            out.update(cna_rate=cna, cna_catalogue=cat)
+        if use_id:
+            idx = id_expo[1] if id_expo is not None else np.ones((n_ep, len(self.id_signature_names))) / \
+                len(self.id_signature_names)
+            out.update(id_sig=self.id_signatures, id_exposure=idx,
+                       pre_id=self.id_signature_names.index(pre_id_sig) if pre_id_sig in self.id_signature_names
+                       else 0)
+        return out
 
     def place_mutations(self, samples_forest: SamplesForest, num_of_preneoplastic_SNVs: int = 0,
                         num_of_preneoplastic_indels: int = 0, seed=None,
@@ -257,7 +430,7 @@ class MutationEngine:
         """src/mutations.cpp:546-573 and src/mutation_engine.cpp:1085-1113."""
         c_seed = get_random_seed(seed)
         params = self._params(samples_forest, int(num_of_preneoplastic_SNVs), preneoplastic_SNV_signature_name,
-                              int(num_of_preneoplastic_indels))
+                              int(num_of_preneoplastic_indels), preneoplastic_indel_signature_name)
         g = self.genome()
         f = core.Forest(self.ctx, samples_forest.cell_id, samples_forest.ancestor, samples_forest.species_index(),
                         samples_forest.sample_index(samples_forest.sample_names), samples_forest.birth_time,
@@ -303,18 +476,26 @@ class PhylogeneticForest:
                 ref.append(b); alt.append(b + seq)
         return np.array(ref, dtype=object), np.array(alt, dtype=object)
 
-    def get_sampled_cell_mutations(self, cell_ids: Iterable[int] | None = None) -> pd.DataFrame:
-        """Per-cell mutation table (src/phylogenetic_forest.cpp:452-507): a leaf
-        carries record m iff leaf_lo(m) <= rank(leaf) < leaf_hi(m)."""
+    def leaf_cells(self) -> np.ndarray:
+        """Cell ids of the forest's leaves in leaf-rank (DFS) order."""
         fl = self.flat()
-        recs = self.records()
         parent = fl["parent"]
-        n = len(parent)
-        is_leaf = np.ones(n, bool)
+        is_leaf = np.ones(len(parent), bool)
         is_leaf[parent[parent != A.PSX_NO_NODE]] = False
-        leaf_dfs = np.nonzero(is_leaf)[0]
-        leaf_cells = fl["cell_id"][leaf_dfs]
-        want = None if cell_ids is None else set(int(c) for c in cell_ids)
+        return fl["cell_id"][np.nonzero(is_leaf)[0]]
+
+    def get_sampled_cell_mutations(self, cell_ids: Iterable[int] | int | None = None) -> pd.DataFrame:
+        """Per-cell mutation table (src/phylogenetic_forest.cpp:452-507): a leaf
+        carries record m iff leaf_lo(m) <= rank(leaf) < leaf_hi(m). Asking for
+        a cell that is not a leaf raises the reference's error (:476-484)."""
+        recs = self.records()
+        leaf_cells = self.leaf_cells()
+        want = None
+        if cell_ids is not None:
+            want = {int(cell_ids)} if isinstance(cell_ids, (int, np.integer)) else set(int(c) for c in cell_ids)
+            known = set(leaf_cells.tolist())
+            for c in sorted(want - known):
+                raise ValueError(f"Cell \"{c}\" is not a leaf")
         ref, alt = self.mutation_strings(recs)
         chr_names = np.array(self.engine.reference.names, dtype=object)
         rows = []
@@ -333,16 +514,40 @@ class PhylogeneticForest:
         return pd.concat(rows, ignore_index=True)
 
     def _cause(self, c) -> str | None:
-        return None if int(c) == A.PSX_NO_CAUSE else self.engine.signature_names[int(c)]
+        """SBS signature of an SNV; ID signature (index past the SBS ones) of an indel."""
+        c = int(c)
+        if c == A.PSX_NO_CAUSE:
+            return None
+        n_sbs = len(self.engine.signature_names)
+        return self.engine.signature_names[c] if c < n_sbs else self.engine.id_signature_names[c - n_sbs]
 
-    def get_first_occurrences(self, chr: str, chr_pos: int, ref: str, alt: str) -> list[int]:
-        """Cells where the mutation arises (the node of each record of the SID)."""
+    def get_first_occurrences(self, chr: str, chr_pos: int, ref: str, alt: str) -> list:
+        """Cells where the mutation first arises, ascending
+        (src/phylogenetic_forest.cpp:564-600): the cell of each record's edge
+        node; pre-neoplastic mutations arise in the roots. A mutation held only
+        by the germline gives [None] (R's NA); one absent from the sampled
+        cells raises the reference's error."""
         recs = self.records()
         c = self.engine.reference.chr_index(chr)
         r_ref, r_alt = self.mutation_strings(recs)
         sel = np.nonzero((recs["chr"] == c) & (recs["pos"] == int(chr_pos) - 1) & (r_ref == ref) & (r_alt == alt))[0]
         fl = self.flat()
-        return [int(fl["cell_id"][recs["node"][k]]) if recs["node"][k] != A.PSX_NO_NODE else None for k in sel]
+        roots = fl["cell_id"][fl["parent"] == A.PSX_NO_NODE]
+        cells: set[int] = set()
+        germline = False
+        for k in sel.tolist():
+            nat = int(recs["nature"][k])
+            if nat == A.PSX_GERMINAL:
+                germline = True
+            elif recs["node"][k] != A.PSX_NO_NODE:
+                cells.add(int(fl["cell_id"][recs["node"][k]]))
+            else:
+                cells.update(int(r) for r in roots)
+        if cells:
+            return sorted(cells)
+        if germline:
+            return [None]
+        raise ValueError(f"The mutation {chr}:{int(chr_pos)} {ref}>{alt} does not occurs in the sampled cells.")
 
     def get_samples_info(self) -> pd.DataFrame:
         return self.samples_forest.get_samples_info()

@@ -342,12 +342,15 @@ def simulate_seq_device(ctx: Context, genome: Genome, forest: Forest, muts: Muta
 
 
 def seq_vaf_f32(ctx: Context, muts: Mutations, alt: DeviceArray, depth: DeviceArray,
-                present: DeviceArray) -> DeviceArray:
+                present: DeviceArray, out: DeviceArray | None = None) -> DeviceArray:
     """The result table's VAF in fp32 on the device for every record of the
     [n_out_samples, n_mutations] device arrays of simulate_seq_device (SID
-    heads; 0 elsewhere)."""
+    heads; 0 elsewhere), into `out` when given."""
     S = alt.shape[0]
-    out = DeviceArray(ctx, (S, max(muts.n, 1)), np.float32)
+    if out is None:
+        out = DeviceArray(ctx, (S, max(muts.n, 1)), np.float32)
+    if out.dtype != np.float32 or out.nbytes < 4 * S * muts.n:
+        raise ValueError(f"the VAF buffer must hold {S} x {muts.n} float32 values")
     check(lib().psx_seq_vaf_f32(ctx.h, muts.h, S, alt.ptr, depth.ptr, present.ptr, out.ptr), ctx.h)
     return out
 
@@ -416,6 +419,15 @@ def seq_table(recs: np.ndarray, alt, depth, present, include_non_sequenced: bool
     n = nr.value
     return (rows[:n].copy(), occ.reshape(S, Mx)[:, :n].copy(), cov.reshape(S, Mx)[:, :n].copy(),
             vaf.reshape(S, Mx)[:, :n].copy())
+
+
+def probe_valu(ctx: Context, mode: str, iters: int) -> tuple[float, int]:
+    """The vector-ALU ceiling kernels (psx_probe_valu): mode "philox" or
+    "int"; returns (HIP-event ms of one launch, lanes x iterations)."""
+    ms, n = C.c_double(), C.c_uint64()
+    m = {"philox": A.PSX_PROBE_PHILOX, "int": A.PSX_PROBE_INT}[mode]
+    check(lib().psx_probe_valu(ctx.h, m, int(iters), C.byref(ms), C.byref(n)), ctx.h)
+    return ms.value, n.value
 
 
 def synth_forest(*, growth, death, intro_at, n_cells: int, n_samples: int, sample_size: int, seed: int) -> dict:

@@ -337,6 +337,7 @@ extern "C" psx_status psx_place_mutations(psx_ctx* ctx, const psx_genome* g, con
             psx_throw(PSX_EINVAL, "SNVs need a signature and an exposure");
         if (p->n_preneoplastic_snv > 0 && p->preneoplastic_sbs >= p->n_sbs)
             psx_throw(PSX_EINVAL, "unknown preneoplastic signature");
+        if (p->n_id) psx_throw(PSX_EUNSUPPORTED, "indel placement from ID signatures is not built yet");
         const uint64_t G = g->boff[g->n_chr];
         // ---- integer tables (host, from the double pmfs) -> one u64 array
         std::vector<uint64_t> T;
@@ -402,7 +403,8 @@ extern "C" psx_status psx_place_mutations(psx_ctx* ctx, const psx_genome* g, con
             const bool wgd = c.kind == PSX_CNA_WGD;
             if (c.kind > PSX_CNA_WGD || c.species >= p->n_species ||
                 (!wgd && (c.chr >= g->n_chr || c.len == 0 || c.start >= g->len[c.chr] ||
-                          c.len > g->len[c.chr] - c.start || c.allele >= p->ploidy ||
+                          c.len > g->len[c.chr] - c.start ||
+                          (c.allele >= p->ploidy && c.allele != PSX_CNA_RANDOM_ALLELE) ||
                           (c.kind == PSX_CNA_AMP && c.copies == 0))))
                 psx_throw(PSX_EINVAL, "invalid copy-number driver %u", k);
             for (uint32_t v = 0; v < f->n_nodes; ++v) {
@@ -416,11 +418,57 @@ extern "C" psx_status psx_place_mutations(psx_ctx* ctx, const psx_genome* g, con
                 e.start = wgd ? 0u : c.start;
                 e.end = wgd ? 0u : c.start + c.len;
                 e.allele = wgd ? 0 : c.allele;
+                if (!wgd && c.allele == PSX_CNA_RANDOM_ALLELE)  // drawn per node: (driver, node, 1, DRIVER)
+                    e.allele = (uint8_t)psx_uniform32(psx_draw(seed, k, v, 1u, PSX_P_DRIVER << 24).x, p->ploidy);
                 e.kind = c.kind;
                 e.copies = c.kind == PSX_CNA_AMP ? c.copies : 0;
                 cna.push_back(e);
             }
         }
+        // passenger CNAs (src/mutation_engine.cpp:441-487, 901-927): on the edge into every non-root node,
+        // Poisson(cna_rate(species) x ploidy x G) events counted from word z of the node's edge draw (words x
+        // and y count its SNVs and indels), each a uniform catalogue region on a uniform allele
+        bool any_pcna = false;
+        for (uint32_t sp = 0; sp < p->n_species && p->cna_rate; ++sp) {
+            if (!(p->cna_rate[sp] >= 0.0) || !std::isfinite(p->cna_rate[sp]))
+                psx_throw(PSX_EINVAL, "CNA rates must be non-negative");
+            any_pcna |= p->cna_rate[sp] > 0.0;
+        }
+        if (any_pcna) {
+            if (p->n_cna_catalogue == 0 || !p->cna_catalogue)
+                psx_throw(PSX_EINVAL, "passenger CNAs need a CNA catalogue");
+            for (uint32_t k = 0; k < p->n_cna_catalogue; ++k) {
+                const psx_cna_region& r = p->cna_catalogue[k];
+                if (r.chr >= g->n_chr || r.len == 0 || r.start >= g->len[r.chr] || r.len > g->len[r.chr] - r.start ||
+                    (r.kind != PSX_CNA_AMP && r.kind != PSX_CNA_DEL))
+                    psx_throw(PSX_EINVAL, "invalid passenger CNA catalogue region %u", k);
+            }
+            std::vector<psx_itable> ct(p->n_species);
+            for (uint32_t sp = 0; sp < p->n_species; ++sp) ct[sp] = psx_poisson_table(p->cna_rate[sp] * allelic);
+            for (uint32_t v = 0; v < f->n_nodes; ++v) {
+                if (f->parent[v] == PSX_NO_NODE) continue;
+                const psx_itable& t = ct[f->species[v]];
+                if (t.T.size() == 1 && t.k0 == 0) continue;  // rate 0
+                const psx_u4 w = psx_draw(seed, v, 0u, 0u, PSX_P_EDGE << 24);
+                const uint32_t n_ev = t.k0 + psx_cdf_pick(t.T.data(), (uint32_t)t.T.size(), w.z);
+                for (uint32_t k = 0; k < n_ev; ++k) {
+                    const psx_u4 e = psx_draw(seed, k, v, 0u, PSX_P_PCNA << 24);
+                    const psx_cna_region& r = p->cna_catalogue[psx_uniform32(e.x, p->n_cna_catalogue)];
+                    psx_cna_event ev{};
+                    ev.node = v;
+                    ev.leaf_lo = f->leaf_lo[v];
+                    ev.leaf_hi = f->leaf_hi[v];
+                    ev.chr = r.chr;
+                    ev.start = r.start;
+                    ev.end = r.start + r.len;
+                    ev.allele = (uint8_t)psx_uniform32(e.y, p->ploidy);
+                    ev.kind = r.kind;
+                    ev.copies = r.kind == PSX_CNA_AMP ? 1 : 0;
+                    cna.push_back(ev);
+                }
+            }
+        }
+        // DFS order of the nodes; at one node the drivers in their order, then the passengers
         std::stable_sort(cna.begin(), cna.end(),
                          [](const psx_cna_event& a, const psx_cna_event& b) { return a.node < b.node; });
         psx_cna_check(cna, f->n_leaves, p->ploidy, g->len);

@@ -33,7 +33,10 @@ enum psx_purpose : uint32_t {
     PSX_P_SNV = 0x21u,       // SNV class and allele
     PSX_P_SITE = 0x22u,      // SNV site search (rejection on context)
     PSX_P_INDEL = 0x23u,     // indel type, length, allele, position
+    PSX_P_IDCLS = 0x24u,     // indel signature, ID class and allele (ID signatures)
     PSX_P_DRIVER = 0x25u,    // driver allele
+    PSX_P_IDSITE = 0x26u,    // indel site search (ID signatures)
+    PSX_P_PCNA = 0x27u,      // passenger CNA: catalogue region and allele
     PSX_P_FOREST = 0x30u,    // synthetic birth-death forest
     PSX_P_PICK = 0x31u,      // synthetic sampling of cells
 };

@@ -176,6 +176,7 @@ class ServingPool:
     muts: list          # core.Mutations
     n_samples: list
     cells: list
+    descs: list = field(default_factory=list)  # the forests' node tables (synth_forest dicts)
 
 
 def build_serving_pool(ctx: core.Context, seed: int = 555, n_forests: int = 8, total_bases: int = 50_000_000,
@@ -192,7 +193,7 @@ def build_serving_pool(ctx: core.Context, seed: int = 555, n_forests: int = 8, t
                   exposure=[[1.0]], n_pre_snv=0, n_pre_indel=0, pre_sbs=0, indel_max_len=10, indel_len_p=0.5,
                   infinite_sites=True, ploidy=2, drivers=[], germline=[])
     sizes = np.exp(np.linspace(np.log(min_cells), np.log(max_cells), n_forests))
-    forests, muts, ns, cells = [], [], [], []
+    forests, muts, ns, cells, descs = [], [], [], [], []
     for k, c in enumerate(sizes.astype(int)):
         n_s = 2 + k % 2
         ss = int(min(10_000, c // (2 * n_s)))
@@ -204,7 +205,8 @@ def build_serving_pool(ctx: core.Context, seed: int = 555, n_forests: int = 8, t
         muts.append(m)
         ns.append(n_s)
         cells.append(int(c))
-    return ServingPool(g, forests, muts, ns, cells)
+        descs.append(fd)
+    return ServingPool(g, forests, muts, ns, cells, descs)
 
 
 def serving_requests(pool: ServingPool, n: int, rate: float, seed: int = 9, min_cov: float = 30.0,

@@ -97,8 +97,21 @@ def test_engine_rejects_unknown_signature_and_rates():
     with pytest.raises(ValueError, match="96 classes"):
         pa.MutationEngine(ref, {"SBS1": [1.0] * 10})
     eng = pa.MutationEngine(ref, {"SBS1": cases.signatures(1, 0)[0]})
-    with pytest.raises(ValueError, match="Unsupported passenger rate"):
+    # src/mutation_engine.cpp:917-928
+    with pytest.raises(ValueError, match='^"SNP" is an unsupported name in passenger rates list.$'):
         eng.add_mutant("A", {"SNP": 1e-8})
+    with pytest.raises(ValueError, match="CNA rates must be non-negative"):
+        eng.add_mutant("A", {"SNV": 1e-8, "CNA": -1.0})
+    # src/cna.cpp:100-138
+    with pytest.raises(ValueError, match='Unknown CNA type "X"'):
+        pa.CNA("X", "1", 10, 100)
+    with pytest.raises(ValueError, match="Region lengths must be non-negative numbers"):
+        pa.Deletion("1", 10, -5)
+    eng.add_mutant("B", {"SNV": 1e-8, "CNA": 1e-10}, drivers=[pa.WGD, pa.Amplification("1", 11, 40, src_allele=0)])
+    sf = pa.SamplesForest([1, 2], [-1, 1], ["B", "B"], [None, "S"], [0.0, 1.0])
+    eng.add_exposure({"SBS1": 1.0})
+    with pytest.raises(ValueError, match="need a passenger CNA catalogue"):
+        eng._params(sf, 0, "SBS1", 0)
     with pytest.raises(ValueError, match="no known SBS signature"):
         eng.add_exposure({"SBS5": 1.0})
     with pytest.raises(ValueError, match="sum to 1"):

@@ -1,0 +1,128 @@
+"""Per-cell mutation tables of a placed forest (PhylogeneticForest,
+src/phylogenetic_forest.cpp:452-507 and :564-600) on GPU-placed tables,
+against brute force on the original node table:
+
+* get_sampled_cell_mutations: a sampled leaf holds exactly the mutations on
+  the edges into its ancestors (itself included, walked through the parent
+  links of SamplesForest$get_nodes()), plus the pre-neoplastic and germline
+  ones; asking for a cell that is not a leaf raises the reference's error;
+* get_first_occurrences: the cells whose incoming edge carries the SID;
+* the infinite-sites check of vignettes/infinite_site_model.Rmd:154-189: with
+  the model on, no non-germinal SNV held by several cells first arises in two
+  unrelated cells; with it off, on a genome small enough for collisions, the
+  same check finds one.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import cases
+import process_amd as pa
+
+pytestmark = pytest.mark.gpu
+
+
+def engine(length=60_000, snv=5e-6, infinite_sites=True):
+    sig = cases.signatures(2, 5)
+    ref = pa.ReferenceGenome({"1": length, "2": length // 2}, seed=9)
+    eng = pa.MutationEngine(ref, {"SBS1": sig[0], "SBS5": sig[1]}, germline=[("2", 101, "C", 0)])
+    eng.add_mutant("A", {"SNV": snv, "indel": snv / 10})
+    eng.add_mutant("B", {"SNV": 2 * snv, "indel": snv / 10})
+    eng.add_exposure({"SBS1": 0.5, "SBS5": 0.5})
+    eng.infinite_sites_model = infinite_sites
+    return eng
+
+
+def forest(seed=4, n_cells=400):
+    return pa.SamplesForest.synthetic(mutants={"A": (0.2, 0.01), "B": (0.3, 0.01)}, introductions={"B": 60},
+                                      n_cells=n_cells, samples={"S1": 60, "S2": 60}, seed=seed)
+
+
+def ancestors(sf):
+    parent = dict(zip(sf.cell_id.tolist(), sf.ancestor.tolist()))
+
+    def walk(c):
+        out = []
+        while c >= 0:
+            out.append(c)
+            c = parent[c]
+        return out
+    return walk
+
+
+def test_sampled_cell_mutations_by_ancestor_walk(ctx):
+    eng, sf = engine(), forest()
+    pf = eng.place_mutations(sf, 6, 2, seed=13)
+    recs = pf.records()
+    fl = pf.flat()
+    node_cell = [None if n == pa.api.A.PSX_NO_NODE else int(fl["cell_id"][n]) for n in recs["node"].tolist()]
+    ref, alt = pf.mutation_strings(recs)
+    walk = ancestors(sf)
+    leaves = [int(c) for c, s in zip(sf.cell_id.tolist(), sf.sample.tolist()) if s is not None]
+    assert len(leaves) == 120 and sorted(pf.leaf_cells().tolist()) == sorted(leaves)
+    table = pf.get_sampled_cell_mutations()
+    assert len(table) > 0
+    got = {}
+    for row in table.itertuples(index=False):
+        got.setdefault(int(row.cell_id), []).append((row.chr, int(row.chr_pos), int(row.allele), row.ref, row.alt))
+    names = eng.reference.names
+    for cell in leaves:
+        lineage = set(walk(cell))
+        exp = [(names[int(recs["chr"][k])], int(recs["pos"][k]) + 1, int(recs["allele"][k]), ref[k], alt[k])
+               for k in range(len(recs)) if node_cell[k] is None or node_cell[k] in lineage]
+        assert sorted(got.get(cell, [])) == sorted(exp), cell
+    one = pf.get_sampled_cell_mutations(leaves[3])
+    assert set(one["cell_id"]) == {leaves[3]} and len(one) == len(got[leaves[3]])
+    inner = int(sf.ancestor[sf.ancestor >= 0][0])
+    with pytest.raises(ValueError, match=f'^Cell "{inner}" is not a leaf$'):
+        pf.get_sampled_cell_mutations(inner)
+
+
+def test_first_occurrences_by_brute_force(ctx):
+    eng, sf = engine(), forest(seed=6)
+    pf = eng.place_mutations(sf, 4, 1, seed=3)
+    recs = pf.records()
+    fl = pf.flat()
+    ref, alt = pf.mutation_strings(recs)
+    names = eng.reference.names
+    roots = sorted(int(c) for c, a in zip(sf.cell_id.tolist(), sf.ancestor.tolist()) if a < 0)
+    heads = np.nonzero(recs["sid_head"])[0]
+    for h in heads[:: max(1, len(heads) // 60)].tolist():
+        same = [k for k in range(len(recs)) if recs["chr"][k] == recs["chr"][h] and recs["pos"][k] == recs["pos"][h]
+                and ref[k] == ref[h] and alt[k] == alt[h]]
+        exp = set()
+        for k in same:
+            if recs["nature"][k] == pa.api.A.PSX_GERMINAL:
+                continue
+            n = int(recs["node"][k])
+            exp |= set(roots) if n == pa.api.A.PSX_NO_NODE else {int(fl["cell_id"][n])}
+        got = pf.get_first_occurrences(names[int(recs["chr"][h])], int(recs["pos"][h]) + 1, ref[h], alt[h])
+        assert got == (sorted(exp) if exp else [None])
+    germ = recs[recs["nature"] == pa.api.A.PSX_GERMINAL][0]
+    assert pf.get_first_occurrences("2", int(germ["pos"]) + 1, "ACGT"[germ["ref"]], "ACGT"[germ["alt"]]) == [None]
+    with pytest.raises(ValueError, match="does not occurs in the sampled cells"):
+        pf.get_first_occurrences("1", 5, "A", "ACGT")
+
+
+def multi_origin_snvs(pf):
+    """The vignette's test: non-germinal SNVs held more than once in the
+    per-cell table whose first occurrences are two or more cells."""
+    t = pf.get_sampled_cell_mutations()
+    t = t[(t["class"] != "germinal") & (t["type"] == "SNV")]
+    counts = t.groupby(["chr", "chr_pos", "ref", "alt"]).size()
+    bad = []
+    for (c, p, r, a), n in counts.items():
+        if n > 1 and len(pf.get_first_occurrences(c, int(p), r, a)) >= 2:
+            bad.append((c, int(p), r, a))
+    return bad
+
+
+def test_infinite_sites_vignette_check(ctx):
+    sf = forest(seed=8, n_cells=600)
+    # a 3 kb genome and high rates: with the model off, the same SNV arises on two unrelated edges
+    pf_on = engine(length=2_000, snv=1.5e-4, infinite_sites=True).place_mutations(sf, 0, 0, seed=1)
+    assert len(pf_on.records()) > 300
+    assert multi_origin_snvs(pf_on) == []
+    pf_off = engine(length=2_000, snv=1.5e-4, infinite_sites=False).place_mutations(sf, 0, 0, seed=1)
+    assert len(multi_origin_snvs(pf_off)) > 0

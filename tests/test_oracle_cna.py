@@ -157,3 +157,96 @@ def test_invalid_copy_number_driver_rejected():
     eng["cna"] = [(0, 0, 10, 100, 3, A.PSX_CNA_AMP, 1)]  # allele 3 of a diploid genome
     with pytest.raises(O.OracleError):
         O.place_cna(LENS, of, **eng)
+
+
+# ---------------------------------------------------------- passenger CNAs
+
+CATALOGUE = [(0, 1_000, 5_000, A.PSX_CNA_AMP), (0, 20_000, 3_000, A.PSX_CNA_DEL),
+             (0, 60_000, 10_000, A.PSX_CNA_AMP), (0, 80_000, 2_500, A.PSX_CNA_DEL)]
+
+
+def passenger_engine(world, rates=(2e-6, 6e-6), catalogue=CATALOGUE):
+    eng = dict(world["eng"])
+    eng["cna"] = cna_drivers(wgd=False)
+    eng["cna_rate"] = list(rates)
+    eng["cna_catalogue"] = catalogue
+    return eng
+
+
+def test_passenger_cna_counts_are_poisson(world):
+    """Per non-root edge Poisson(cna_rate(species) x ploidy x G) passenger
+    events: over 300 placements, the total's mean and variance equal the
+    closed-form sum of the edges' means (rates chosen so it is ~ 60)."""
+    of, fd = world["of"], world["fd"]
+    eng = passenger_engine(world, rates=(1e-7, 3e-7))
+    flat_parent = of.parent
+    G = int(LENS.sum())
+    lam = sum((1e-7 if of.species[v] == 0 else 3e-7) * 2 * G for v in range(of.n_nodes)
+              if flat_parent[v] != A.PSX_NO_NODE)
+    n_drv = len(O.place_cna(LENS, of, seed=0, **dict(eng, cna_rate=None)))
+    totals = np.array([len(O.place_cna(LENS, of, seed=s, **eng)) - n_drv for s in range(300)], np.float64)
+    se = np.sqrt(lam / 300)
+    assert abs(totals.mean() - lam) < 4 * se, (totals.mean(), lam)
+    # variance of a Poisson total equals its mean (chi-square on 299 degrees of freedom, p in (0.001, 0.999))
+    from scipy import stats
+    chi = totals.var(ddof=1) * 299 / lam
+    assert stats.chi2.ppf(0.001, 299) < chi < stats.chi2.ppf(0.999, 299)
+
+
+def test_passenger_cna_events(world):
+    """Passenger events sit on non-root edges after that node's drivers, in
+    DFS order; regions are uniform over the catalogue and alleles uniform
+    (chi-square over 200 placements); amplifications add one copy."""
+    from scipy import stats
+    of = world["of"]
+    eng = passenger_engine(world, rates=(5e-7, 5e-7))
+    drv = O.place_cna(LENS, of, seed=1, **dict(eng, cna_rate=None))
+    region, allele = np.zeros(4), np.zeros(2)
+    for s in range(200):
+        ev = O.place_cna(LENS, of, seed=s, **eng)
+        assert (np.diff(ev["node"].astype(np.int64)) >= 0).all()
+        is_drv = np.zeros(len(ev), bool)
+        k = 0
+        for i, e in enumerate(ev):  # the drivers keep their order and precede the node's passengers
+            if k < len(drv) and e.tobytes() == drv[k].tobytes():
+                is_drv[i] = True
+                k += 1
+        assert k == len(drv)
+        pas = ev[~is_drv]
+        assert (of.parent[pas["node"]] != A.PSX_NO_NODE).all()
+        for e in pas:
+            key = (int(e["chr"]), int(e["start"]), int(e["end"] - e["start"]), int(e["kind"]))
+            j = [(c, st, ln, kd) for c, st, ln, kd in CATALOGUE].index(key)
+            region[j] += 1
+            allele[e["allele"]] += 1
+            assert e["copies"] == (1 if e["kind"] == A.PSX_CNA_AMP else 0)
+            assert e["leaf_lo"] == of.leaf_lo[e["node"]] and e["leaf_hi"] == of.leaf_hi[e["node"]]
+        # a node's passengers come after its drivers
+        for v in set(drv["node"].tolist()):
+            idx = np.nonzero(ev["node"] == v)[0]
+            assert is_drv[idx[: (drv["node"] == v).sum()]].all()
+    assert region.sum() > 500
+    assert stats.chisquare(region).pvalue > 0.001 and stats.chisquare(allele).pvalue > 0.001
+
+
+def test_passenger_cna_errors(world):
+    of = world["of"]
+    with pytest.raises(RuntimeError, match="need a CNA catalogue"):
+        O.place_cna(LENS, of, seed=1, **passenger_engine(world, catalogue=[]))
+    with pytest.raises(RuntimeError, match="invalid passenger CNA catalogue region 0"):
+        O.place_cna(LENS, of, seed=1, **passenger_engine(world, catalogue=[(0, 89_000, 5_000, A.PSX_CNA_AMP)]))
+    with pytest.raises(RuntimeError, match="CNA rates must be non-negative"):
+        O.place_cna(LENS, of, seed=1, **passenger_engine(world, rates=(-1.0, 0.0)))
+
+
+def test_random_allele_cna_drivers(world):
+    """A CNA driver with no allele draws one per placement node, uniformly."""
+    of = world["of"]
+    eng = dict(world["eng"])
+    eng["cna"] = [(0, 0, 1_000, 2_000, A.PSX_CNA_RANDOM_ALLELE, A.PSX_CNA_DEL, 0)]
+    seen = set()
+    for s in range(40):
+        ev = O.place_cna(LENS, of, seed=s, **eng)
+        assert len(ev) >= 1
+        seen |= set(ev["allele"].tolist())
+    assert seen == {0, 1}
