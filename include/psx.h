@@ -342,6 +342,25 @@ psx_status psx_simulate_seq_device(psx_ctx* ctx, const psx_genome* g, const psx_
                                    uint64_t seed, int32_t* d_alt, int32_t* d_depth,
                                    uint8_t* d_present, uint64_t* n_reads);
 
+/* psx_simulate_seq_device with no device allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). The launch
+ * descriptors and work counters live in a fixed device array of
+ * PSX_NOALLOC_SCRATCH_BYTES per GPU (one copy per device, held until the
+ * process ends), so a call whose descriptors exceed it fails with
+ * PSX_EINVAL (descriptors take about 6 MB per output sample of a 3 Gb
+ * genome, 2 bytes per 1,000 bases per output sample), as does a
+ * call whose mutation table has not been indexed for the genome's chromosome
+ * lengths yet (placement indexes a table for its own genome; an allocating
+ * sequencing call indexes it for another). Every call on a GPU shares that
+ * GPU's array: calls that overlap on one GPU, from any context, race. Results
+ * equal psx_simulate_seq_device's. */
+#define PSX_NOALLOC_SCRATCH_BYTES (32u << 20)
+psx_status psx_simulate_seq_device_noalloc(psx_ctx* ctx, const psx_genome* g, const psx_forest* f,
+                                           const psx_mutations* m, const psx_seq_params* p,
+                                           uint64_t seed, int32_t* d_alt, int32_t* d_depth,
+                                           uint8_t* d_present, uint64_t* n_reads);
+
 /* Read materialisation (write_SAM = TRUE analogue): every read as an 8-byte
  * header (start position; source leaf | allele << 26 | mate << 30 | has-error
  * << 31) plus ceil(read_size/16) words of 2-bit bases with the source's SNVs

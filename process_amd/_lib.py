@@ -64,6 +64,8 @@ def _declare(lib):
         "psx_simulate_seq": (st, [vp, vp, vp, vp, C.POINTER(A.SeqParams), C.c_uint64, A.i32p, A.i32p, A.u8p,
                                   A.u64p]),
         "psx_simulate_seq_device": (st, [vp, vp, vp, vp, C.POINTER(A.SeqParams), C.c_uint64, vp, vp, vp, A.u64p]),
+        "psx_simulate_seq_device_noalloc": (st, [vp, vp, vp, vp, C.POINTER(A.SeqParams), C.c_uint64, vp, vp, vp,
+                                                 A.u64p]),
         "psx_seq_vaf_f32": (st, [vp, vp, C.c_uint32, vp, vp, vp, vp]),
         "psx_read_words": (C.c_uint32, [C.POINTER(A.SeqParams)]),
         "psx_simulate_seq_reads": (st, [vp, vp, vp, vp, C.POINTER(A.SeqParams), C.c_uint64, vp, C.c_uint64,

@@ -341,6 +341,17 @@ def simulate_seq_device(ctx: Context, genome: Genome, forest: Forest, muts: Muta
     return nr.value
 
 
+def simulate_seq_device_noalloc(ctx: Context, genome: Genome, forest: Forest, muts: Mutations, seed: int,
+                                alt: DeviceArray, depth: DeviceArray, present: DeviceArray, **seq) -> int:
+    """simulate_seq_device with no device allocation (descriptors in the
+    library's fixed per-GPU scratch array; calls on one GPU must not overlap)."""
+    p, keep = A.seq_params(**seq)
+    nr = C.c_uint64()
+    check(lib().psx_simulate_seq_device_noalloc(ctx.h, genome.h, forest.h, muts.h, C.byref(p), C.c_uint64(seed),
+                                                alt.ptr, depth.ptr, present.ptr, C.byref(nr)), ctx.h)
+    return nr.value
+
+
 def seq_vaf_f32(ctx: Context, muts: Mutations, alt: DeviceArray, depth: DeviceArray,
                 present: DeviceArray, out: DeviceArray | None = None) -> DeviceArray:
     """The result table's VAF in fp32 on the device for every record of the

@@ -38,6 +38,7 @@ extern "C" psx_status psx_ctx_create(int device, psx_ctx** out) {
         PSX_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
         PSX_HIP(hipEventCreate(&ctx->ev0));
         PSX_HIP(hipEventCreate(&ctx->ev1));
+        ctx->noalloc_scratch = psx_noalloc_scratch_init(device);
     });
     if (st != PSX_OK) {
         // keep the context so the caller can read the message

@@ -79,9 +79,14 @@ struct psx_ctx {
     double last_place_ms = 0.0;
     dbuf scratch;       // per-call tables and descriptors
     dbuf queue;         // persistent-kernel work counters
+    uint8_t* noalloc_scratch = nullptr;  // device address of the device's no-allocation scratch
     bool tuned = false;
     psx_launch_config tuned_cfg{};
 };
+
+// Device address of the no-allocation entry point's scratch array on
+// `device`, which must be the current device; looked up once per device.
+uint8_t* psx_noalloc_scratch_init(int device);
 
 struct psx_genome {
     psx_ctx* ctx = nullptr;
