@@ -37,7 +37,7 @@ PROFILED = {"k_seq_count": dict(traffic=302.1e6, valu_insts=6.206e9),
             # batched read materialisation of the 256-job batched workload's leading 19 jobs
             "k_seq_reads_batched": dict(traffic=72.74e9),
             # wave64 VALU instructions per wave per loop iteration of the probe kernels
-            "k_probe_valu": dict(philox_per_iter=None, int_per_iter=None)}
+            "k_probe_valu": dict(philox_per_iter=43.0, int_per_iter=24.0)}
 
 # Algorithmic bytes of one materialised read: an 8-byte header (start, source) and the 2-bit bases,
 # 8 + ceil(L / 4) = 46 B at L = 150. The stored row is 8 + 4 * ceil(L / 16) = 48 B (word-aligned bases);
@@ -93,6 +93,24 @@ def valu_ceiling(ctx, reps: int = 3) -> dict:
             r["valu_insts_per_s"] = r["valu_insts"] / (ms / 1e3)
         out[mode] = r
     return out
+
+
+def place_leg(ctx, w, built, reps: int = 5) -> dict:
+    """place_mutations on the plain-large workload, repeated: wall clock per
+    call (host set-up, kernels, sorts, the table's device index) and the
+    HIP-event time of the placement kernels."""
+    from process_amd import core
+
+    walls, kms = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        m = core.Mutations.place(ctx, built.genome, built.forest, w.place_seed, **w.engine)
+        walls.append((time.perf_counter() - t0) * 1e3)
+        kms.append(ctx.last_place_ms())
+        m.close()
+    return dict(mutations=built.muts.n, nodes=len(w.forest["cell_id"]), wall_ms=float(np.median(walls)),
+                kernel_ms=float(np.median(kms)),
+                value=built.muts.n / (np.median(walls) / 1e3), unit="mutations/s")
 
 
 def log(msg: str) -> None:
@@ -411,6 +429,8 @@ def main() -> int:
     ap.add_argument("--serve-burst", type=int, default=200, help="requests of the saturated run (0 skips it)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run device-to-device copies of BYTES (reference for the HBM byte counters)")
+    ap.add_argument("--place-only", action="store_true",
+                    help="only build the plain-large workload and repeat its placement (for profiling)")
     ap.add_argument("--probe-only", action="store_true",
                     help="only run the vector-ALU probe kernels (for profiling their instruction counts)")
     ap.add_argument("--batched-reads-only", action="store_true",
@@ -449,6 +469,11 @@ def main() -> int:
     built = workloads.build(ctx, w, driver_species=1)
     M = built.muts.n
     log(f"placed {M} mutations in {built.place_ms:.2f} ms (GPU)")
+    placement = place_leg(ctx, w, built)
+    log(f"placement: {placement['wall_ms']:.2f} ms per call, kernels {placement['kernel_ms']:.2f} ms")
+    if args.place_only:
+        print(json.dumps(placement), flush=True)
+        return 0
     seq = w.seq
     S = core.num_out_samples(built.forest, **seq)
     alt = core.DeviceArray(ctx, (S, M), np.int32)
@@ -544,10 +569,11 @@ def main() -> int:
         "config": {"workload": "plain_large: 1e6-cell two-species forest, 3 samples x 1e4 cells + normal, "
                                "300 Mb / 22 chr, SNV 1e-8, indel 1e-9, 200x PE 150 bp, error 1e-3, count-only",
                    "reads_per_step": reads_per_step, "mutations": M, "samples": S,
-                   "forest_nodes": len(w.forest["cell_id"]), "place_mutations_ms": built.place_ms,
+                   "forest_nodes": len(w.forest["cell_id"]), "place_mutations_ms": placement["wall_ms"],
                    "launch": tuned or "default"},
         "roofline": roofline,
         "host_outputs": host_out,
+        "placement": placement,
     }
 
     if not args.no_reads:

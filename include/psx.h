@@ -372,6 +372,33 @@ psx_status psx_simulate_seq_reads(psx_ctx* ctx, const psx_genome* g, const psx_f
                                   uint64_t seed, uint32_t* d_reads, uint64_t capacity_reads,
                                   uint64_t* n_reads);
 
+/* SAM output of simulate_seq(write_SAM = TRUE) (src/seq_simulation.cpp:536-564;
+ * vignettes/sequencing.Rmd:137-180, 272-309): one file per selected
+ * chromosome, "<filename_prefix><chromosome name>.sam", holding the reads of
+ * every output sample, one read group per sample. In update mode a
+ * chromosome whose file exists is written to the first free
+ * "<filename_prefix><chromosome name>_<n>.sam" (n = 1, 2, ...); otherwise an
+ * existing file is an error. Reads are those of psx_simulate_seq_reads, made
+ * one chromosome at a time on the device and written from pinned chunks
+ * (no full copy of the reads in host memory). Template names are
+ * "<template_name_prefix><k>", k numbering templates in (output sample,
+ * chromosome) order. Qualities (Phred+33): constant at
+ * min(40, round(-10 log10 e)) (40 when e = 0); with random_quality, each
+ * base's score is drawn from Philox(seed, read, base) uniformly within 5 of
+ * that value, clipped to [2, 41]. */
+typedef struct psx_sam_params {
+    const char* output_dir;            /* must exist */
+    const char* filename_prefix;       /* "chr_" */
+    const char* template_name_prefix;  /* "r" */
+    int32_t update;                    /* UPDATE mode (update_SAM) */
+    int32_t random_quality;            /* BasicIlluminaSequencer$random_quality_scores */
+    const char* const* sample_names;   /* one per output sample (psx_seq_num_samples) */
+    const char* const* chr_names;      /* one per genome chromosome */
+} psx_sam_params;
+psx_status psx_write_sam(psx_ctx* ctx, const psx_genome* g, const psx_forest* f, const psx_mutations* m,
+                         const psx_seq_params* p, uint64_t seed, const psx_sam_params* sam,
+                         uint64_t* n_reads);
+
 /* ------------------------------------------------------------- batched mode */
 
 typedef struct psx_seq_job {

@@ -90,6 +90,12 @@ class SeqJob(C.Structure):
                 ("d_present", C.c_void_p)]
 
 
+class SamParams(C.Structure):
+    _fields_ = [("output_dir", C.c_char_p), ("filename_prefix", C.c_char_p), ("template_name_prefix", C.c_char_p),
+                ("update", C.c_int32), ("random_quality", C.c_int32), ("sample_names", C.POINTER(C.c_char_p)),
+                ("chr_names", C.POINTER(C.c_char_p))]
+
+
 class LaunchConfig(C.Structure):
     _fields_ = [("block_threads", C.c_uint32), ("lds_records", C.c_uint32), ("strata_per_item", C.c_uint32),
                 ("blocks_per_cu", C.c_uint32), ("use_graph", C.c_int32)]

@@ -70,6 +70,7 @@ def _declare(lib):
         "psx_read_words": (C.c_uint32, [C.POINTER(A.SeqParams)]),
         "psx_simulate_seq_reads": (st, [vp, vp, vp, vp, C.POINTER(A.SeqParams), C.c_uint64, vp, C.c_uint64,
                                         A.u64p]),
+        "psx_write_sam": (st, [vp, vp, vp, vp, C.POINTER(A.SeqParams), C.c_uint64, C.POINTER(A.SamParams), A.u64p]),
         "psx_simulate_seq_batch": (st, [vp, C.POINTER(A.SeqJob), C.c_uint32, C.POINTER(A.LaunchConfig), A.u64p]),
         "psx_simulate_seq_reads_batch": (st, [vp, C.POINTER(A.SeqJob), C.c_uint32, vp, C.c_uint64, A.u64p,
                                               A.u64p]),

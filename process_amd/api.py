@@ -669,7 +669,7 @@ def _table(pf: PhylogeneticForest, sample_names: list[str], alt, depth, present,
 def _run(pf: PhylogeneticForest, forest: core.Forest, sample_names: list[str], sequencer, chromosomes, coverage,
          read_size, insert_size_mean, insert_size_stddev, purity, with_normal_sample, preneoplastic_in_normal,
          include_non_sequenced_mutations, c_seed, write_SAM, output_dir, filename_prefix, template_name_prefix,
-         samples=None):
+         samples=None, update_SAM=False, reference_genome=None):
     eng = pf.engine
     if not (0.0 <= float(purity) <= 1.0):
         raise ValueError("The purity must be in [0,1]")
@@ -687,8 +687,30 @@ def _run(pf: PhylogeneticForest, forest: core.Forest, sample_names: list[str], s
     names = list(sample_names) + (["normal_sample"] if with_normal_sample else [])
     if write_SAM:
         from .sam import write_sam
-        write_sam(pf, forest, names, seq, _seed64(c_seed), output_dir, filename_prefix, template_name_prefix)
+        write_sam(pf, forest, names, seq, _seed64(c_seed), output_dir, filename_prefix, template_name_prefix,
+                  update=bool(update_SAM), genome=_read_genome(eng, reference_genome),
+                  random_quality=bool(getattr(sequencer, "random_quality_scores", False)))
     return _table(pf, names, alt, depth, present, include_non_sequenced_mutations), n_reads
+
+
+def _read_genome(eng: MutationEngine, reference_genome) -> core.Genome:
+    """The genome reads are taken from: the engine's, or a reference_genome
+    given to simulate_seq (src/seq_simulation.cpp:245-280), which must have
+    the engine's chromosome names and lengths (the mutations' positions refer
+    to them)."""
+    if reference_genome is None:
+        return eng.genome()
+    if not isinstance(reference_genome, ReferenceGenome):
+        raise ValueError("The reference genome must be a ReferenceGenome")
+    if reference_genome.names != eng.reference.names or \
+            not np.array_equal(reference_genome.lengths, eng.reference.lengths):
+        raise ValueError("The reference genome must have the mutation engine's chromosome names and lengths")
+    return core.Genome(eng.ctx, reference_genome.lengths, seed=reference_genome.seed, packed=reference_genome.packed)
+
+
+def _check_sam_dir(write_SAM, update_SAM, output_dir):
+    if write_SAM and not update_SAM and os.path.isdir(output_dir) and os.listdir(output_dir):
+        raise ValueError(f"The output directory \"{output_dir}\" already exists")
 
 
 def simulate_seq(phylo_forest: PhylogeneticForest, sequencer=None, reference_genome=None, chromosomes=None,
@@ -696,9 +718,10 @@ def simulate_seq(phylo_forest: PhylogeneticForest, sequencer=None, reference_gen
                  write_SAM=False, update_SAM=False, cell_labelling: Callable | None = None, purity=1,
                  with_normal_sample=True, preneoplastic_in_normal=False, filename_prefix="chr_",
                  template_name_prefix="r", include_non_sequenced_mutations=False, seed=None) -> dict:
-    """src/sequencing.cpp:193-209 and src/seq_simulation.cpp:517-601."""
-    if reference_genome is not None:
-        raise ValueError("Only the mutation engine's reference genome is resident on the device")
+    """src/sequencing.cpp:193-209 and src/seq_simulation.cpp:517-601.
+    reference_genome (a ReferenceGenome with the engine's chromosomes) only
+    changes the bases of written reads; the counts do not read the
+    reference."""
     c_seed = get_random_seed(seed)
     pf = phylo_forest
     if cell_labelling is not None:
@@ -707,12 +730,11 @@ def simulate_seq(phylo_forest: PhylogeneticForest, sequencer=None, reference_gen
         forest, names = _labelled_forest(pf, cell_labelling)
     else:
         forest, names = pf.forest, list(pf.samples_forest.sample_names)
-    if write_SAM and not update_SAM and os.path.isdir(output_dir) and os.listdir(output_dir):
-        raise ValueError(f"The output directory \"{output_dir}\" already exists")
+    _check_sam_dir(write_SAM, update_SAM, output_dir)
     df, _ = _run(pf, forest, names, sequencer, chromosomes, coverage, read_size, insert_size_mean,
                  insert_size_stddev, purity, with_normal_sample, preneoplastic_in_normal,
                  include_non_sequenced_mutations, c_seed, write_SAM, output_dir, filename_prefix,
-                 template_name_prefix)
+                 template_name_prefix, update_SAM=update_SAM, reference_genome=reference_genome)
     parameters = {"sequencer": _sequencer_data(sequencer), "reference_genome": reference_genome,
                   "chromosomes": chromosomes, "coverage": coverage, "read_size": read_size,
                   "insert_size_mean": insert_size_mean, "insert_size_stddev": insert_size_stddev,
@@ -730,13 +752,13 @@ def simulate_normal_seq(phylo_forest: PhylogeneticForest, sequencer=None, refere
                         seed=None) -> dict:
     """src/sequencing.cpp:268-282 and src/seq_simulation.cpp:603-679: the
     normal sample alone, purity 1."""
-    if reference_genome is not None:
-        raise ValueError("Only the mutation engine's reference genome is resident on the device")
     c_seed = get_random_seed(seed)
     pf = phylo_forest
+    _check_sam_dir(write_SAM, update_SAM, output_dir)
     df, _ = _run(pf, pf.forest, [], sequencer, chromosomes, coverage, read_size, insert_size_mean,
                  insert_size_stddev, 1.0, True, with_preneoplastic, include_non_sequenced_mutations, c_seed,
-                 write_SAM, output_dir, filename_prefix, template_name_prefix, samples=[])
+                 write_SAM, output_dir, filename_prefix, template_name_prefix, samples=[], update_SAM=update_SAM,
+                 reference_genome=reference_genome)
     parameters = {"sequencer": _sequencer_data(sequencer), "reference_genome": reference_genome,
                   "chromosomes": chromosomes, "coverage": coverage, "read_size": read_size,
                   "insert_size_mean": insert_size_mean, "insert_size_stddev": insert_size_stddev,

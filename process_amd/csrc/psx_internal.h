@@ -88,6 +88,17 @@ struct psx_ctx {
 // `device`, which must be the current device; looked up once per device.
 uint8_t* psx_noalloc_scratch_init(int device);
 
+// Indel (ID83) classes of a genome's reference: sites per class and, for the
+// classes too sparse for a rejection search, their sites listed (global
+// position << 10 | payload, ascending per class). Built on first use.
+struct psx_id_index {
+    uint64_t count[PSX_ID_CLASSES] = {};
+    uint32_t listed[PSX_ID_CLASSES] = {};
+    uint64_t off[PSX_ID_CLASSES + 1] = {};
+    dbuf d_list;
+    double build_ms = 0.0;
+};
+
 struct psx_genome {
     psx_ctx* ctx = nullptr;
     uint32_t n_chr = 0;
@@ -96,7 +107,12 @@ struct psx_genome {
     std::vector<uint64_t> boff;   // absolute base offset, n_chr + 1 entries
     uint64_t words = 0;
     dbuf packed;
+    std::mutex id_mu;
+    std::unique_ptr<psx_id_index> id_index;
 };
+
+// The genome's indel index, built on ctx's stream at the first call.
+const psx_id_index& psx_genome_id_index(psx_ctx* ctx, psx_genome* g);
 
 struct psx_forest {
     psx_ctx* ctx = nullptr;

@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cmath>
 
+#include "psx_id.h"
 #include "psx_internal.h"
 
 namespace {
@@ -39,11 +40,111 @@ __device__ __forceinline__ uint32_t gbase(const gdev& g, uint32_t c, uint32_t i)
     return (g.packed[g.woff[c] + (i >> 4)] >> (2u * (i & 15u))) & 3u;
 }
 
+// chromosome of absolute base x: the last c with boff[c] <= x (binary search over n_chr + 1 offsets)
 __device__ __forceinline__ void glocate(const gdev& g, uint64_t x, uint32_t& c, uint32_t& pos) {
-    uint32_t k = 0;
-    while (x >= g.boff[k + 1]) ++k;
-    c = k;
-    pos = (uint32_t)(x - g.boff[k]);
+    uint32_t lo = 0, hi = g.n_chr;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (g.boff[mid] <= x) lo = mid; else hi = mid;
+    }
+    c = lo;
+    pos = (uint32_t)(x - g.boff[lo]);
+}
+
+// bases of one chromosome from HBM, for psxid::view
+struct gbases {
+    const uint32_t* words;  // the chromosome's first packed word
+    __device__ __forceinline__ uint32_t operator()(uint32_t i) const { return (words[i >> 4] >> (2u * (i & 15u))) & 3u; }
+};
+
+// ------------------------------------------------------------ indel index
+//
+// One block per tile of ID_TILE positions of a chromosome: the tile's bases
+// (with the halo the classes read: 5 before, 35 after) are staged in LDS, each
+// thread classifies positions with psxid::view::sites_at, and the counts go
+// through LDS counters to one global atomic per class and block. The list
+// pass writes the sites of the listed classes, tagged with their class in
+// bits 44.., for one radix sort.
+
+constexpr uint32_t ID_TILE = 4096;
+constexpr uint32_t ID_HALO = 64;  // bases staged before and after the tile (the classes reach 5 before, 35 after)
+constexpr uint32_t ID_WORDS = (ID_TILE + 2 * ID_HALO) / 16 + 2;
+
+struct id_args {
+    const uint32_t* packed;
+    const uint64_t* woff;
+    const uint32_t* len;
+    const uint64_t* boff;
+    const uint32_t* tile_chr;
+    const uint32_t* tile_first;
+    unsigned long long* counts;   // [PSX_ID_CLASSES] (count pass)
+    const uint32_t* listed;       // [PSX_ID_CLASSES] (list pass)
+    unsigned long long* cursor;   // [PSX_ID_CLASSES] next slot of each listed class (list pass)
+    uint64_t* list;
+    uint32_t zero_listed;         // some class next to 0 copies is listed
+};
+
+struct lds_bases {
+    const uint32_t* w;
+    uint32_t w0;  // chromosome word of w[0]
+    __device__ __forceinline__ uint32_t operator()(uint32_t i) const {
+        return (w[(i >> 4) - w0] >> (2u * (i & 15u))) & 3u;
+    }
+};
+
+template <bool LIST>
+__global__ void __launch_bounds__(256) k_id_scan(const id_args a) {
+    __shared__ uint32_t s_w[ID_WORDS];
+    __shared__ uint32_t s_cnt[PSX_ID_CLASSES];
+    const uint32_t c = a.tile_chr[blockIdx.x], first = a.tile_first[blockIdx.x];
+    const uint32_t len = a.len[c];
+    const uint32_t end = min(first + ID_TILE, len);
+    const uint32_t lo = first > ID_HALO ? first - ID_HALO : 0u;
+    const uint32_t hi = min(len, end + ID_HALO);
+    const uint32_t w0 = lo >> 4, w1 = (hi + 15u) >> 4;
+    const uint32_t* words = a.packed + a.woff[c];
+    for (uint32_t i = threadIdx.x; i < w1 - w0; i += 256) s_w[i] = words[w0 + i];
+    if (!LIST)
+        for (uint32_t i = threadIdx.x; i < PSX_ID_CLASSES; i += 256) s_cnt[i] = 0;
+    __syncthreads();
+    const psxid::view<lds_bases> v{lds_bases{s_w, w0}, len};
+    uint32_t z[6] = {0, 0, 0, 0, 0, 0};
+    uint32_t cls[10];
+    for (uint32_t q = first + threadIdx.x; q < end; q += 256) {
+        const uint32_t n = v.sites_at(q, cls, z);
+        if (!LIST) {
+            for (uint32_t i = 0; i < n; ++i) atomicAdd(&s_cnt[cls[i]], 1u);
+            continue;
+        }
+        const uint64_t gq = a.boff[c] + q;
+        for (uint32_t i = 0; i < n; ++i) {
+            if (!a.listed[cls[i]]) continue;
+            const unsigned long long slot = atomicAdd(&a.cursor[cls[i]], 1ull);
+            a.list[slot] = ((uint64_t)cls[i] << 44) | (gq << 10);
+        }
+        if (!a.zero_listed) continue;
+        for (uint32_t zi = 0; zi < 6; ++zi) {
+            const uint32_t k = psxid::zero_class(zi);
+            if (!a.listed[k]) continue;
+            const uint32_t np = psxid::payloads(k);
+            for (uint32_t pl = 0; pl < np; ++pl) {
+                if (!v.is_site(q, pl, k)) continue;
+                const unsigned long long slot = atomicAdd(&a.cursor[k], 1ull);
+                a.list[slot] = ((uint64_t)k << 44) | (gq << 10) | pl;
+            }
+        }
+    }
+    if (LIST) return;
+    for (uint32_t zi = 0; zi < 6; ++zi)
+        if (z[zi]) atomicAdd(&s_cnt[psxid::zero_class(zi)], z[zi]);
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < PSX_ID_CLASSES; i += 256)
+        if (s_cnt[i]) atomicAdd(&a.counts[i], (unsigned long long)s_cnt[i]);
+}
+
+__global__ void __launch_bounds__(256) k_id_untag(uint64_t* __restrict__ list, uint64_t n) {
+    const uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    if (i < n) list[i] &= (1ull << 44) - 1ull;
 }
 
 struct place_args {
@@ -74,6 +175,17 @@ struct place_args {
     uint32_t* attempt;
     const uint8_t* redraw;  // nullptr = every id
     uint32_t* status;
+    // indels from ID signatures (use_id): per-epoch exposure and per-signature class tables (classes without a
+    // site in the reference have probability 0), the reference's sites per class and the listed classes' sites
+    uint32_t use_id;
+    uint32_t n_id;
+    uint32_t pre_id;
+    const uint64_t* id_expo_T;  // [n_epochs][n_id]
+    const uint64_t* id_cls_T;   // [n_id][PSX_ID_CLASSES]
+    const uint64_t* id_count;   // [PSX_ID_CLASSES]
+    const uint32_t* id_listed;  // [PSX_ID_CLASSES]
+    const uint64_t* id_off;     // [PSX_ID_CLASSES + 1]
+    const uint64_t* id_list;
 };
 
 __device__ const uint32_t SUB_TARGET[6] = {0, 2, 3, 0, 1, 2};  // C>A C>G C>T T>A T>C T>G
@@ -134,6 +246,44 @@ __device__ uint32_t draw_indel(const place_args& a, uint32_t v, uint32_t k, uint
     return ST_NO_INDEL_ROOM;
 }
 
+// Indel from ID signatures: signature from the ID exposure of the node's
+// epoch (or the preneoplastic one), class from the signature, allele uniform,
+// site uniform among the class's sites: rejection over (position, payload)
+// for dense classes, a uniform index into the class's sorted sites otherwise.
+__device__ uint32_t draw_indel_id(const place_args& a, uint32_t v, uint32_t k, uint32_t att, int fixed_sig,
+                                  psx_mutation& r) {
+    const psx_u4 w = psx_draw(a.seed, k, v, att, PSX_P_IDCLS << 24);
+    const uint32_t sig = fixed_sig >= 0 ? (uint32_t)fixed_sig
+                                        : psx_cdf_pick(a.id_expo_T + (size_t)a.node_epoch[v] * a.n_id, a.n_id, w.x);
+    const uint32_t cls = psx_cdf_pick(a.id_cls_T + (size_t)sig * PSX_ID_CLASSES, PSX_ID_CLASSES, w.y);
+    const uint32_t allele = psx_uniform32(w.z, a.ploidy);
+    uint32_t c = 0, q = 0, payload = 0;
+    if (a.id_listed[cls]) {
+        const psx_u4 ws = psx_draw(a.seed, k, v, att, PSX_P_IDSITE << 24);
+        const uint64_t e = a.id_list[a.id_off[cls] + psx_uniform64(ws.x, ws.y, a.id_count[cls])];
+        payload = (uint32_t)(e & 1023u);
+        glocate(a.g, e >> 10, c, q);
+    } else {
+        const uint32_t np = psxid::payloads(cls);
+        uint32_t t = 0;
+        for (; t < SITE_TRIES; ++t) {
+            const psx_u4 ws = psx_draw(a.seed, k, v, att, (PSX_P_IDSITE << 24) | t);
+            glocate(a.g, psx_uniform64(ws.x, ws.y, a.g.G), c, q);
+            payload = psx_uniform32(ws.z, np);
+            const psxid::view<gbases> vw{gbases{a.g.packed + a.g.woff[c]}, a.g.len[c]};
+            if (vw.is_site(q, payload, cls)) break;
+        }
+        if (t == SITE_TRIES) return ST_NO_SITE;
+    }
+    const psxid::cls_t kk = psxid::decode(cls);
+    const psxid::view<gbases> vw{gbases{a.g.packed + a.g.woff[c]}, a.g.len[c]};
+    r.chr = c; r.pos = q - 1; r.kind = kk.del ? PSX_DEL : PSX_INS; r.ref = (uint8_t)gbase(a.g, c, q - 1);
+    r.alt = 0; r.len = (uint8_t)kk.u;
+    r.seq = kk.zero ? psxid::zero_unit(kk.u, kk.b, payload) : vw.unit(q, kk.u);
+    r.allele = (uint8_t)allele; r.cause = (uint16_t)(a.n_sbs + sig);
+    return ST_OK;
+}
+
 __device__ uint32_t place_one(const place_args& a, uint64_t id, uint32_t att, psx_mutation& r) {
     r = psx_mutation{};
     uint64_t i = id;
@@ -165,7 +315,8 @@ __device__ uint32_t place_one(const place_args& a, uint64_t id, uint32_t att, ps
     }
     i -= a.n_pre_snv;
     if (i < a.n_pre_indel) {
-        st = draw_indel(a, PSX_NO_NODE, (uint32_t)i, att, r);
+        st = a.use_id ? draw_indel_id(a, PSX_NO_NODE, (uint32_t)i, att, (int)a.pre_id, r)
+                      : draw_indel(a, PSX_NO_NODE, (uint32_t)i, att, r);
         r.nature = PSX_PRENEOPLASTIC; r.node = PSX_NO_NODE; r.leaf_lo = 0; r.leaf_hi = a.n_leaves;
         return st;
     }
@@ -179,6 +330,7 @@ __device__ uint32_t place_one(const place_args& a, uint64_t id, uint32_t att, ps
     const uint32_t v = lo - 1;
     const uint32_t q = (uint32_t)(i - a.node_off[v]);
     if (q < a.cnt_snv[v]) st = draw_snv(a, v, q, att, -1, r);
+    else if (a.use_id) st = draw_indel_id(a, v, q - a.cnt_snv[v], att, -1, r);
     else st = draw_indel(a, v, q - a.cnt_snv[v], att, r);
     r.nature = PSX_PASSENGER; r.node = v; r.leaf_lo = a.leaf_lo[v]; r.leaf_hi = a.leaf_hi[v];
     return st;
@@ -318,6 +470,94 @@ inline uint32_t grid_of(uint64_t n) { return (uint32_t)((n + 255) / 256); }
 
 }  // namespace
 
+const psx_id_index& psx_genome_id_index(psx_ctx* ctx, psx_genome* g) {
+    std::lock_guard<std::mutex> lk(g->id_mu);
+    if (g->id_index) return *g->id_index;
+    auto ix = std::make_unique<psx_id_index>();
+    hipStream_t s = ctx->stream;
+    std::vector<uint32_t> tile_chr, tile_first;
+    for (uint32_t c = 0; c < g->n_chr; ++c)
+        for (uint64_t f = 0; f < g->len[c]; f += ID_TILE) {
+            tile_chr.push_back(c);
+            tile_first.push_back((uint32_t)f);
+        }
+    scratch_builder sb;
+    const size_t o_len = sb.put_vec(g->len), o_woff = sb.put_vec(g->woff), o_boff = sb.put_vec(g->boff);
+    const size_t o_tc = sb.put_vec(tile_chr), o_tf = sb.put_vec(tile_first);
+    const std::vector<uint64_t> zeros(PSX_ID_CLASSES, 0);
+    const size_t o_cnt = sb.put_vec(zeros);
+    const size_t o_cur = sb.put_vec(zeros);
+    const std::vector<uint32_t> zl(PSX_ID_CLASSES, 0);
+    const size_t o_listed = sb.put_vec(zl);
+    uint8_t* base = (uint8_t*)psx_scratch_upload(ctx, sb);
+    id_args a{};
+    a.packed = g->packed.as<uint32_t>();
+    a.woff = (const uint64_t*)(base + o_woff);
+    a.len = (const uint32_t*)(base + o_len);
+    a.boff = (const uint64_t*)(base + o_boff);
+    a.tile_chr = (const uint32_t*)(base + o_tc);
+    a.tile_first = (const uint32_t*)(base + o_tf);
+    a.counts = (unsigned long long*)(base + o_cnt);
+    a.cursor = (unsigned long long*)(base + o_cur);
+    a.listed = (const uint32_t*)(base + o_listed);
+    const uint32_t n_tiles = (uint32_t)tile_chr.size();
+    PSX_HIP(hipEventRecord(ctx->ev0, s));
+    hipLaunchKernelGGL(k_id_scan<false>, dim3(n_tiles), dim3(256), 0, s, a);
+    PSX_HIP(hipGetLastError());
+    PSX_HIP(hipMemcpyAsync(ix->count, a.counts, sizeof ix->count, hipMemcpyDeviceToHost, s));
+    PSX_HIP(hipStreamSynchronize(s));
+    // a class is listed when a rejection search over (position, payload) would average more than DENSE_TRIES tries
+    const uint64_t G = g->boff[g->n_chr];
+    uint64_t total = 0;
+    bool zero_listed = false;
+    std::vector<uint64_t> cursor(PSX_ID_CLASSES);
+    for (uint32_t k = 0; k < PSX_ID_CLASSES; ++k) {
+        ix->off[k] = total;
+        const unsigned __int128 need = (unsigned __int128)G * psxid::payloads(k);
+        ix->listed[k] = ix->count[k] > 0 && (unsigned __int128)ix->count[k] * psxid::DENSE_TRIES < need;
+        cursor[k] = total;
+        if (ix->listed[k]) {
+            total += ix->count[k];
+            zero_listed |= psxid::decode(k).zero != 0;
+        }
+    }
+    ix->off[PSX_ID_CLASSES] = total;
+    ix->d_list.alloc(total * 8);
+    if (total) {
+        PSX_HIP(hipMemcpyAsync((void*)a.listed, ix->listed, sizeof ix->listed, hipMemcpyHostToDevice, s));
+        PSX_HIP(hipMemcpyAsync(a.cursor, cursor.data(), 8 * cursor.size(), hipMemcpyHostToDevice, s));
+        dbuf tagged;
+        tagged.alloc(total * 8);
+        a.list = tagged.as<uint64_t>();
+        a.zero_listed = zero_listed ? 1u : 0u;
+        hipLaunchKernelGGL(k_id_scan<true>, dim3(n_tiles), dim3(256), 0, s, a);
+        PSX_HIP(hipGetLastError());
+        size_t tmp_bytes = 0;
+        PSX_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tmp_bytes, tagged.as<uint64_t>(), ix->d_list.as<uint64_t>(),
+                                                  (int)total, 0, 51, s));
+        dbuf tmp;
+        tmp.alloc(tmp_bytes);
+        PSX_HIP(hipcub::DeviceRadixSort::SortKeys(tmp.p, tmp_bytes, tagged.as<uint64_t>(), ix->d_list.as<uint64_t>(),
+                                                  (int)total, 0, 51, s));
+        hipLaunchKernelGGL(k_id_untag, dim3(grid_of(total)), dim3(256), 0, s, ix->d_list.as<uint64_t>(), total);
+        PSX_HIP(hipGetLastError());
+        std::vector<uint64_t> end(PSX_ID_CLASSES);
+        PSX_HIP(hipMemcpyAsync(end.data(), a.cursor, 8 * end.size(), hipMemcpyDeviceToHost, s));
+        PSX_HIP(hipStreamSynchronize(s));
+        for (uint32_t k = 0; k < PSX_ID_CLASSES; ++k)
+            if (end[k] != ix->off[k + 1] && ix->listed[k])
+                psx_throw(PSX_EHIP, "indel index: class %u listed %llu sites, counted %llu", k,
+                          (unsigned long long)(end[k] - ix->off[k]), (unsigned long long)ix->count[k]);
+    }
+    PSX_HIP(hipEventRecord(ctx->ev1, s));
+    PSX_HIP(hipStreamSynchronize(s));
+    float ms = 0.f;
+    PSX_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    ix->build_ms = ms;
+    g->id_index = std::move(ix);
+    return *g->id_index;
+}
+
 extern "C" psx_status psx_place_mutations(psx_ctx* ctx, const psx_genome* g, const psx_forest* f,
                                           const psx_engine_params* p, uint64_t seed, psx_mutations** out) {
     if (!ctx || !g || !f || !p || !out) return PSX_EINVAL;
@@ -337,7 +577,21 @@ extern "C" psx_status psx_place_mutations(psx_ctx* ctx, const psx_genome* g, con
             psx_throw(PSX_EINVAL, "SNVs need a signature and an exposure");
         if (p->n_preneoplastic_snv > 0 && p->preneoplastic_sbs >= p->n_sbs)
             psx_throw(PSX_EINVAL, "unknown preneoplastic signature");
-        if (p->n_id) psx_throw(PSX_EUNSUPPORTED, "indel placement from ID signatures is not built yet");
+        // indels from ID signatures: the genome's indel index (built once per genome) and the class tables
+        bool use_id = false;
+        const psx_id_index* idx = nullptr;
+        if (p->n_id) {
+            bool need_id = p->n_preneoplastic_indel > 0;
+            for (uint32_t sp = 0; sp < p->n_species; ++sp) need_id |= p->indel_rate[sp] > 0.0;
+            if (!p->id || (need_id && (!p->id_exposure || p->n_epochs == 0)))
+                psx_throw(PSX_EINVAL, "indels need an ID signature and an exposure");
+            if (p->n_preneoplastic_indel > 0 && p->preneoplastic_id >= p->n_id)
+                psx_throw(PSX_EINVAL, "unknown preneoplastic ID signature");
+            if (need_id) {
+                use_id = true;
+                idx = &psx_genome_id_index(ctx, const_cast<psx_genome*>(g));
+            }
+        }
         const uint64_t G = g->boff[g->n_chr];
         // ---- integer tables (host, from the double pmfs) -> one u64 array
         std::vector<uint64_t> T;
@@ -361,6 +615,28 @@ extern "C" psx_status psx_place_mutations(psx_ctx* ctx, const psx_genome* g, con
         tmp.assign(p->indel_max_len, 0);
         psx_cdf_table(lp.data(), p->indel_max_len, tmp.data());
         const size_t len_off = append(tmp.data(), tmp.size());
+        size_t id_expo_off = 0, id_cls_off = 0;
+        if (use_id) {
+            id_expo_off = T.size();
+            for (uint32_t e = 0; e < p->n_epochs; ++e) {
+                tmp.assign(p->n_id, 0);
+                psx_cdf_table(p->id_exposure + (size_t)e * p->n_id, p->n_id, tmp.data());
+                append(tmp.data(), tmp.size());
+            }
+            id_cls_off = T.size();
+            for (uint32_t sg = 0; sg < p->n_id; ++sg) {
+                double pmf[PSX_ID_CLASSES];
+                double mass = 0.0;
+                for (uint32_t k = 0; k < PSX_ID_CLASSES; ++k) {
+                    pmf[k] = idx->count[k] ? p->id[(size_t)sg * PSX_ID_CLASSES + k] : 0.0;
+                    mass += pmf[k];
+                }
+                if (!(mass > 0.0)) psx_throw(PSX_EINVAL, "ID signature %u has no class with a site in the reference", sg);
+                tmp.assign(PSX_ID_CLASSES, 0);
+                psx_cdf_table(pmf, PSX_ID_CLASSES, tmp.data());
+                append(tmp.data(), tmp.size());
+            }
+        }
         const double allelic = (double)((uint64_t)p->ploidy * G);
         std::vector<uint32_t> sk0(p->n_species), soff(p->n_species), sn(p->n_species);
         std::vector<uint32_t> ik0(p->n_species), ioff(p->n_species), in_(p->n_species);
@@ -491,6 +767,12 @@ extern "C" psx_status psx_place_mutations(psx_ctx* ctx, const psx_genome* g, con
         const size_t o_germ = sb.put(p->germline, sizeof(psx_germline) * p->n_germline);
         const uint32_t zero = 0;
         const size_t o_status = sb.put(&zero, 4);
+        size_t o_idc = 0, o_idl = 0, o_ido = 0;
+        if (use_id) {
+            o_idc = sb.put(idx->count, sizeof idx->count);
+            o_idl = sb.put(idx->listed, sizeof idx->listed);
+            o_ido = sb.put(idx->off, sizeof idx->off);
+        }
         const size_t o_nlose = sb.put(&zero, 4);
         uint8_t* base = (uint8_t*)psx_scratch_upload(ctx, sb);
         auto at = [&](size_t off) { return (void*)(base + off); };
@@ -547,6 +829,12 @@ extern "C" psx_status psx_place_mutations(psx_ctx* ctx, const psx_genome* g, con
         pa.cnt_snv = d_cnt_snv.as<uint32_t>(); pa.node_off = d_off.as<uint64_t>();
         pa.M = M; pa.recs = d_recs.as<psx_mutation>(); pa.attempt = d_attempt.as<uint32_t>();
         pa.redraw = nullptr; pa.status = (uint32_t*)at(o_status);
+        if (use_id) {
+            pa.use_id = 1; pa.n_id = p->n_id; pa.pre_id = p->preneoplastic_id;
+            pa.id_expo_T = (uint64_t*)at(o_T) + id_expo_off; pa.id_cls_T = (uint64_t*)at(o_T) + id_cls_off;
+            pa.id_count = (const uint64_t*)at(o_idc); pa.id_listed = (const uint32_t*)at(o_idl);
+            pa.id_off = (const uint64_t*)at(o_ido); pa.id_list = idx->d_list.as<uint64_t>();
+        }
         if (M) {
             hipLaunchKernelGGL(k_place, dim3(grid_of(M)), dim3(256), 0, s, pa);
             PSX_HIP(hipGetLastError());
@@ -555,7 +843,7 @@ extern "C" psx_status psx_place_mutations(psx_ctx* ctx, const psx_genome* g, con
             uint32_t stv = 0;
             PSX_HIP(hipMemcpyAsync(&stv, pa.status, 4, hipMemcpyDeviceToHost, s));
             PSX_HIP(hipStreamSynchronize(s));
-            if (stv & ST_NO_SITE) psx_throw(PSX_EINVAL, "no reference site matches a drawn SBS class");
+            if (stv & ST_NO_SITE) psx_throw(PSX_EINVAL, "no reference site matches a drawn SBS or ID class");
             if (stv & ST_NO_INDEL_ROOM) psx_throw(PSX_EINVAL, "no room for an indel");
         };
         check_status();

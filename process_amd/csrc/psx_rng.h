@@ -29,6 +29,7 @@ enum psx_purpose : uint32_t {
     PSX_P_REF = 0x01u,       // synthetic reference bases
     PSX_P_TEMPLATE = 0x10u,  // read template: start, tumour/normal, source copy, insert
     PSX_P_ERROR = 0x11u,     // sequencing errors of a template (both mates)
+    PSX_P_QUALITY = 0x12u,   // quality scores of a written read (random quality model)
     PSX_P_EDGE = 0x20u,      // per-edge passenger counts
     PSX_P_SNV = 0x21u,       // SNV class and allele
     PSX_P_SITE = 0x22u,      // SNV site search (rejection on context)

@@ -1,0 +1,155 @@
+"""SAM output (write_SAM = TRUE; src/seq_simulation.cpp:536-564,
+vignettes/sequencing.Rmd:137-180, 272-309) from the library's streaming
+writer: every file, line for line, against SAM lines formatted here from the
+oracle's materialised reads (paired and single ends, sample read groups,
+template names); update mode writes "<prefix><chr>_<n>.sam" for files that
+exist and leaves them untouched; the random quality model stays within its
+range and repeats with the seed; a reference_genome given to simulate_seq
+changes the bases written and nothing else."""
+from __future__ import annotations
+
+import math
+import os
+
+import numpy as np
+import pytest
+
+import cases
+import process_amd as pa
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+NAMES = {"21": 40_000, "22": 23_457}
+
+
+def build(seed=4):
+    sig = cases.signatures(2, 11)
+    ref = pa.ReferenceGenome(NAMES, seed=31)
+    eng = pa.MutationEngine(ref, {"SBS1": sig[0], "SBS5": sig[1]}, germline=[("22", 1001, "T", 1)])
+    eng.add_mutant("A", {"SNV": 6e-6, "indel": 5e-7})
+    eng.add_mutant("B", {"SNV": 9e-6, "indel": 5e-7})
+    eng.add_exposure({"SBS1": 0.7, "SBS5": 0.3})
+    sf = pa.SamplesForest.synthetic(mutants={"A": (0.2, 0.01), "B": (0.3, 0.01)}, introductions={"B": 100},
+                                    n_cells=400, samples={"S1": 60, "S2": 60}, seed=seed)
+    return eng, sf
+
+
+def expected_lines(rows, names, chr_names, lens, L, pe, coverage, qual, prefix="r"):
+    """SAM lines per chromosome from materialised read rows in (sample,
+    chromosome, stratum, template, mate) order."""
+    rpt = 2 if pe else 1
+    T = [int(math.floor(coverage * ln / (rpt * L) + 0.5)) if ln >= L else 0 for ln in lens]
+    per_sample = sum(T)
+    out = {c: [] for c in range(len(lens))}
+    idx = 0
+    for so, name in enumerate(names):
+        base = so * per_sample
+        for c in range(len(lens)):
+            block = rows[idx: idx + T[c] * rpt]
+            for k, r in enumerate(block):
+                x = int(r[0])
+                bases = "".join("ACGT"[(int(r[2 + i // 16]) >> (2 * (i % 16))) & 3] for i in range(L))
+                q = qual(so, c, k) if callable(qual) else qual * L
+                t = base + k // rpt
+                if pe:
+                    mate = k % 2
+                    other = int(block[k + 1 - 2 * mate][0])
+                    lo, hi = min(x, other), max(x, other) + L
+                    tlen = hi - lo if mate == 0 else lo - hi
+                    out[c].append(f"{prefix}{t}\t{99 if mate == 0 else 147}\t{chr_names[c]}\t{x + 1}\t60\t{L}M\t=\t"
+                                  f"{other + 1}\t{tlen}\t{bases}\t{q}\tRG:Z:{name}")
+                else:
+                    out[c].append(f"{prefix}{t}\t0\t{chr_names[c]}\t{x + 1}\t60\t{L}M\t*\t0\t0\t{bases}\t{q}\t"
+                                  f"RG:Z:{name}")
+            idx += T[c] * rpt
+            base += T[c]
+    return out
+
+
+def read_sam(path):
+    with open(path) as fh:
+        lines = fh.read().splitlines()
+    return [l for l in lines if l.startswith("@")], [l for l in lines if not l.startswith("@")]
+
+
+@pytest.mark.parametrize("pe", [True, False])
+def test_sam_files_match_oracle_reads(ctx, tmp_path, pe):
+    eng, sf = build()
+    pf = eng.place_mutations(sf, 4, 1, seed=8)
+    seqer = pa.BasicIlluminaSequencer(2e-3, random_quality_scores=False)
+    kw = dict(insert_size_mean=350, insert_size_stddev=10) if pe else {}
+    out = tmp_path / "sam"
+    res = pa.simulate_seq(pf, sequencer=seqer, coverage=3, write_SAM=True, output_dir=str(out), seed=17,
+                          template_name_prefix="tmpl", **kw)
+    assert res["parameters"]["write_SAM"]
+    lens = eng.reference.lengths
+    of = O.Forest(sf.cell_id, sf.ancestor, sf.species_index(), sf.sample_index(sf.sample_names), sf.birth_time, 2)
+    seq = dict(coverage=3.0, read_size=150, error_rate=2e-3, **kw)
+    n = pa.api.core.num_reads(eng.genome(), pf.forest, **seq)
+    rows = O.simulate_reads(lens, eng._packed, of, pf.records(), 17, n, **seq)
+    names = ["S1", "S2", "normal_sample"]
+    q = chr(33 + round(-10 * math.log10(2e-3)))
+    exp = expected_lines(rows, names, eng.reference.names, [int(x) for x in lens], 150, pe, 3.0, q, prefix="tmpl")
+    assert sorted(os.listdir(out)) == ["chr_21.sam", "chr_22.sam"]
+    for c, cname in enumerate(eng.reference.names):
+        head, body = read_sam(out / f"chr_{cname}.sam")
+        assert head[0] == "@HD\tVN:1.6\tSO:unsorted" and head[1] == f"@SQ\tSN:{cname}\tLN:{int(lens[c])}"
+        assert head[2:] == [f"@RG\tID:{s}\tSM:{s}" for s in names]
+        assert body == exp[c], cname
+
+
+def test_update_mode_and_qualities(ctx, tmp_path):
+    eng, sf = build(seed=6)
+    pf = eng.place_mutations(sf, 2, 0, seed=3)
+    out = tmp_path / "sam"
+    seqer = pa.BasicIlluminaSequencer(1e-3)  # random quality scores (the default)
+    pa.simulate_seq(pf, sequencer=seqer, coverage=2, write_SAM=True, output_dir=str(out), seed=5,
+                    chromosomes=["22"])
+    first = (out / "chr_22.sam").read_text()
+    with pytest.raises(ValueError, match="already exists"):
+        pa.simulate_seq(pf, sequencer=seqer, coverage=2, write_SAM=True, output_dir=str(out), seed=5)
+    pa.simulate_seq(pf, sequencer=seqer, coverage=2, write_SAM=True, update_SAM=True, output_dir=str(out), seed=5,
+                    chromosomes=["22"])
+    pa.simulate_seq(pf, sequencer=seqer, coverage=2, write_SAM=True, update_SAM=True, output_dir=str(out), seed=9)
+    assert sorted(os.listdir(out)) == ["chr_21.sam", "chr_22.sam", "chr_22_1.sam", "chr_22_2.sam"]
+    assert (out / "chr_22.sam").read_text() == first
+    # the same seed gives the same file, qualities included; another seed does not
+    assert (out / "chr_22_1.sam").read_text() == first
+    assert (out / "chr_22_2.sam").read_text() != first
+    _, body = read_sam(out / "chr_22.sam")
+    quals = np.frombuffer("".join(l.split("\t")[10] for l in body).encode(), np.uint8).astype(int) - 33
+    assert quals.min() >= 25 and quals.max() <= 35 and len(set(quals.tolist())) == 11
+    # constant qualities when the model is off
+    out2 = tmp_path / "const"
+    pa.simulate_seq(pf, sequencer=pa.BasicIlluminaSequencer(1e-3, random_quality_scores=False), coverage=2,
+                    write_SAM=True, output_dir=str(out2), seed=5, chromosomes=["22"])
+    _, body2 = read_sam(out2 / "chr_22.sam")
+    assert {l.split("\t")[10] for l in body2} == {chr(33 + 30) * 150}
+    assert [l.split("\t")[9] for l in body2] == [l.split("\t")[9] for l in body]
+
+
+def test_reference_genome_override(ctx, tmp_path):
+    """Reads written from another reference with the engine's chromosomes:
+    the counts are unchanged; the bases are the oracle's reads made from that
+    reference."""
+    eng, sf = build(seed=9)
+    pf = eng.place_mutations(sf, 2, 0, seed=4)
+    other = pa.ReferenceGenome(NAMES, seed=77)
+    kw = dict(coverage=2, insert_size_mean=350, insert_size_stddev=10, seed=12)
+    a = pa.simulate_seq(pf, write_SAM=True, output_dir=str(tmp_path / "a"), **kw)
+    b = pa.simulate_seq(pf, reference_genome=other, write_SAM=True, output_dir=str(tmp_path / "b"), **kw)
+    assert a["mutations"].equals(b["mutations"])
+    lens = eng.reference.lengths
+    of = O.Forest(sf.cell_id, sf.ancestor, sf.species_index(), sf.sample_index(sf.sample_names), sf.birth_time, 2)
+    seq = dict(coverage=2.0, read_size=150, insert_size_mean=350, insert_size_stddev=10)
+    n = pa.api.core.num_reads(eng.genome(), pf.forest, **seq)
+    rows = O.simulate_reads(lens, O.genome_synth(lens, 77), of, pf.records(), 12, n, **seq)
+    exp = expected_lines(rows, ["S1", "S2", "normal_sample"], eng.reference.names, [int(x) for x in lens], 150, True,
+                         2.0, "I")
+    for c, cname in enumerate(eng.reference.names):
+        _, body = read_sam(tmp_path / "b" / f"chr_{cname}.sam")
+        assert body == exp[c]
+    with pytest.raises(ValueError, match="chromosome names and lengths"):
+        pa.simulate_seq(pf, reference_genome=pa.ReferenceGenome({"21": 40_000}, seed=1), write_SAM=True,
+                        output_dir=str(tmp_path / "c"), **kw)
