@@ -108,9 +108,13 @@ def place_leg(ctx, w, built, reps: int = 5) -> dict:
         walls.append((time.perf_counter() - t0) * 1e3)
         kms.append(ctx.last_place_ms())
         m.close()
+    ix = built.genome.context_index(w.engine.get("context_sampling", 1))
     return dict(mutations=built.muts.n, nodes=len(w.forest["cell_id"]), wall_ms=float(np.median(walls)),
                 kernel_ms=float(np.median(kms)),
-                value=built.muts.n / (np.median(walls) / 1e3), unit="mutations/s")
+                value=built.muts.n / (np.median(walls) / 1e3), unit="mutations/s",
+                context_index=dict(sampling=int(w.engine.get("context_sampling", 1)),
+                                   sites=int(ix["kept"].sum()), build_ms=ix["build_ms"],
+                                   note="built once per genome and sampling, before the timed calls"))
 
 
 def log(msg: str) -> None:

@@ -108,6 +108,16 @@ uint64_t psx_genome_words(uint32_t n_chr, const uint32_t* chr_len);
 psx_status psx_genome_download(psx_ctx* ctx, const psx_genome* g, uint32_t* packed,
                                uint64_t n_words);
 void psx_genome_free(psx_genome* g);
+/* The genome's SBS context index at a sampling (built on the GPU at the first
+ * placement or call with that sampling, kept with the genome; replaces
+ * build_contex_index, src/mutation_engine.cpp:213-261): counts[32] sites and
+ * kept[32] indexed sites per pyrimidine-centred context (5' base << 2 | 3'
+ * base, + 16 for a T centre), and, if list is not NULL, the indexed global
+ * positions, ascending per context in context order (list_cap entries at
+ * least: the sum of kept). build_ms: GPU time of the build. Any pointer but
+ * ctx and g may be NULL. */
+psx_status psx_genome_context_index(psx_ctx* ctx, psx_genome* g, uint32_t sampling, uint64_t* counts,
+                                    uint64_t* kept, uint32_t* list, uint64_t list_cap, double* build_ms);
 
 /* ------------------------------------------------------------------- forest */
 
@@ -251,6 +261,15 @@ typedef struct psx_engine_params {
     const double* id;           /* [n_id][PSX_ID_CLASSES] */
     const double* id_exposure;  /* [n_epochs][n_id] */
     uint32_t preneoplastic_id;  /* signature index of the preneoplastic indels */
+    /* SBS sites: the genome's context index keeps every context_sampling-th
+     * site (in genome order, from the first) of each of the 32
+     * pyrimidine-centred trinucleotide contexts (MutationEngine's
+     * context_sampling, src/mutations.cpp:818-819; 0 or 1 = every site); an
+     * SNV's site is uniform over its class's indexed sites other than the
+     * driver loci, and classes without an indexed site are left out of the
+     * signatures and the rest renormalised. The reference must be shorter
+     * than 2^32 bases. */
+    uint32_t context_sampling;
 } psx_engine_params;
 
 /* One record per placed mutation; the table is sorted by SID

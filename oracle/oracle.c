@@ -23,8 +23,9 @@ enum {
 };
 
 #define SITE_TRIES (1u << 20)
+#define LIST_TRIES 4096u  /* draws of a context-index site before the class is given up as driver loci only */
 #define INDEL_TRIES (1u << 20)
-#define IS_ROUNDS 64
+#define IS_ROUNDS 256  /* checks of the infinite-sites model before placement gives up */
 #define TABLE_CAP (1u << 16)
 #define TWO32 4294967296.0
 
@@ -414,6 +415,73 @@ int orc_sort_table(psx_mutation* m, uint64_t n) {
     return 0;
 }
 
+/* ------------------------------------------------------- SBS context index */
+
+/* The context index of the reference engine (build_contex_index,
+ * src/mutation_engine.cpp:213-261, a RACES ContextIndex built with
+ * context_sampling and the driver loci left out), restated for this model:
+ * a site is a position q with 1 <= q < len - 1; its context is the
+ * pyrimidine-centred trinucleotide, read on the strand where b[q] is C or T,
+ *     ctx = (mid == T) << 4 | five << 2 | three        (32 contexts),
+ * so SBS class cls (sub = cls / 16, five, three) samples context
+ * (sub >= 3) << 4 | (cls & 15). The index keeps, per context, every
+ * sampling-th site in genome order from the first (ranks 0, s, 2s, ...) as a
+ * 32-bit global position; the driver loci are skipped when a site is drawn
+ * (same law as leaving them out). RACES's own sampling rule is not in the
+ * reference: this one is a restatement, not a replay. */
+
+typedef struct sbs_index {
+    uint64_t count[32];   /* sites per context */
+    uint64_t kept[32];    /* ceil(count / sampling) */
+    uint64_t off[33];     /* list ranges per context */
+    uint32_t* list;       /* global positions, ascending per context */
+} sbs_index;
+
+static uint32_t sbs_context(uint32_t b5, uint32_t m, uint32_t b3) {
+    if (m == 1u || m == 3u) return ((m == 3u ? 1u : 0u) << 4) | (b5 << 2) | b3;
+    return ((m == 0u ? 1u : 0u) << 4) | ((3u - b3) << 2) | (3u - b5);
+}
+
+static int sbs_index_build(const genome_view* g, uint32_t sampling, sbs_index* ix) {
+    memset(ix, 0, sizeof *ix);
+    if (sampling == 0) sampling = 1;
+    if (g->boff[g->n_chr] > 0xFFFFFFFFull)
+        return fail("the reference is too long for the context index (at most 2^32 - 1 bases)");
+    for (uint32_t c = 0; c < g->n_chr; ++c)
+        for (uint32_t q = 1; q + 1 < g->len[c]; ++q)
+            ix->count[sbs_context(base_at(g, c, q - 1), base_at(g, c, q), base_at(g, c, q + 1))]++;
+    uint64_t total = 0;
+    for (uint32_t k = 0; k < 32; ++k) {
+        ix->kept[k] = (ix->count[k] + sampling - 1) / sampling;
+        ix->off[k] = total;
+        total += ix->kept[k];
+    }
+    ix->off[32] = total;
+    ix->list = (uint32_t*)malloc(sizeof(uint32_t) * (total ? total : 1));
+    uint64_t rank[32] = {0};
+    for (uint32_t c = 0; c < g->n_chr; ++c)
+        for (uint32_t q = 1; q + 1 < g->len[c]; ++q) {
+            const uint32_t k = sbs_context(base_at(g, c, q - 1), base_at(g, c, q), base_at(g, c, q + 1));
+            if (rank[k] % sampling == 0) ix->list[ix->off[k] + rank[k] / sampling] = (uint32_t)(g->boff[c] + q);
+            rank[k]++;
+        }
+    return 0;
+}
+
+int orc_sbs_index(uint32_t n_chr, const uint32_t* len, const uint32_t* packed, uint32_t sampling,
+                  uint64_t* counts, uint64_t* kept, uint32_t** list) {
+    if (n_chr == 0 || n_chr > PSX_MAX_CHROMOSOMES) return fail("invalid number of chromosomes");
+    genome_view g;
+    gv_init(&g, n_chr, len, packed);
+    sbs_index ix;
+    int rc = sbs_index_build(&g, sampling, &ix);
+    if (rc) { free(ix.list); return rc; }
+    memcpy(counts, ix.count, sizeof ix.count);
+    memcpy(kept, ix.kept, sizeof ix.kept);
+    *list = ix.list;
+    return 0;
+}
+
 /* ------------------------------------------------------ indel (ID83) classes */
 
 /* COSMIC ID83 classes over the reference (the repeated-sequence index of the
@@ -695,7 +763,10 @@ typedef struct place_ctx {
     const psx_engine_params* p;
     uint64_t seed, G;
     table* expo;     /* per epoch, over signatures */
-    table* cls;      /* per signature, over 96 classes */
+    table* cls;      /* per signature, over the 96 classes whose context has an indexed site */
+    sbs_index sbs;   /* SBS context index at the engine's sampling */
+    uint64_t* drv_pos;  /* driver loci (global positions), ascending, unique */
+    uint64_t n_drv_pos;
     table len_tab;   /* indel lengths, value = 1 + index */
     int use_id;      /* indels from ID signatures */
     id_index idx;    /* ID classes of the reference */
@@ -714,8 +785,17 @@ static uint32_t epoch_of(const psx_engine_params* p, double t) {
 
 /* SBS passenger: signature from the timed exposure at the cell's birth
  * (MutationEngine$add_exposure, src/mutation_engine.cpp:734-739), class from
- * the signature, site uniform over positions whose pyrimidine-centred
- * trinucleotide matches the class (rejection on the reference). */
+ * the signature, site uniform over the context index's sites of the class's
+ * context other than the driver loci (src/mutation_engine.cpp:244-261). */
+static int driver_locus(const place_ctx* x, uint64_t gp) {
+    uint64_t lo = 0, hi = x->n_drv_pos;
+    while (lo < hi) {
+        uint64_t mid = (lo + hi) / 2;
+        if (x->drv_pos[mid] < gp) lo = mid + 1; else hi = mid;
+    }
+    return lo < x->n_drv_pos && x->drv_pos[lo] == gp;
+}
+
 static int draw_snv(const place_ctx* x, uint32_t v, uint32_t k, uint32_t a, int fixed_sig,
                     psx_mutation* r) {
     uint32_t w[4];
@@ -729,29 +809,23 @@ static int draw_snv(const place_ctx* x, uint32_t v, uint32_t k, uint32_t a, int 
     const table* ct = &x->cls[sig];
     uint32_t cls = pick(ct->T, ct->n, w[1]);
     uint32_t allele = mulhi32(w[2], x->p->ploidy);
-    uint32_t sub = cls / 16, five = (cls >> 2) & 3u, three = cls & 3u;
-    uint32_t mid = sub < 3 ? 1u : 3u, tgt = SUB_TARGET[sub];
-    uint32_t ws[4] = {0, 0, 0, 0};
-    for (uint32_t t = 0; t < SITE_TRIES; ++t) {
-        if ((t & 1u) == 0) philox4(x->seed, k, v, a, ((uint32_t)P_SITE << 24) | (t >> 1), ws);
-        uint32_t hi = (t & 1u) ? ws[2] : ws[0], lo = (t & 1u) ? ws[3] : ws[1];
+    uint32_t sub = cls / 16, tgt = SUB_TARGET[sub];
+    uint32_t ctx = ((sub >= 3 ? 1u : 0u) << 4) | (cls & 15u);
+    const uint64_t n = x->sbs.kept[ctx];
+    for (uint32_t t = 0; t < LIST_TRIES && n; ++t) {
+        uint32_t ws[4];
+        philox4(x->seed, k, v, a, ((uint32_t)P_SITE << 24) | t, ws);
+        const uint32_t gp = x->sbs.list[x->sbs.off[ctx] + mulhi64(ws[0], ws[1], n)];
+        if (driver_locus(x, gp)) continue;
         uint32_t c, pos;
-        locate(&x->g, mulhi64(hi, lo, x->G), &c, &pos);
-        if (pos < 1 || pos + 1 >= x->g.len[c]) continue;
-        uint32_t b5 = base_at(&x->g, c, pos - 1), m = base_at(&x->g, c, pos), b3 = base_at(&x->g, c, pos + 1);
-        uint32_t alt;
-        if (m == 1u || m == 3u) {
-            if (!(b5 == five && m == mid && b3 == three)) continue;
-            alt = tgt;
-        } else {
-            if (!(3u - b3 == five && 3u - m == mid && 3u - b5 == three)) continue;
-            alt = 3u - tgt;
-        }
-        r->chr = c; r->pos = pos; r->kind = PSX_SNV; r->ref = (uint8_t)m; r->alt = (uint8_t)alt;
+        locate(&x->g, gp, &c, &pos);
+        uint32_t m = base_at(&x->g, c, pos);
+        r->chr = c; r->pos = pos; r->kind = PSX_SNV; r->ref = (uint8_t)m;
+        r->alt = (uint8_t)((m == 1u || m == 3u) ? tgt : 3u - tgt);
         r->len = 0; r->seq = 0; r->allele = (uint8_t)allele; r->cause = (uint16_t)sig;
         return 0;
     }
-    return fail("no reference site matches SBS class %u", cls);
+    return fail("no reference site matches a drawn SBS or ID class");
 }
 
 /* Indel: deletion or insertion (top bit), geometric length, uniform position
@@ -900,6 +974,7 @@ static void free_tables(place_ctx* x) {
     if (x->id_expo) for (uint32_t e = 0; e < x->p->n_epochs; ++e) free(x->id_expo[e].T);
     if (x->id_cls) for (uint32_t s = 0; s < x->p->n_id; ++s) free(x->id_cls[s].T);
     free(x->expo); free(x->cls); free(x->len_tab.T); free(x->id_expo); free(x->id_cls); free(x->idx.list);
+    free(x->sbs.list); free(x->drv_pos);
 }
 
 /* ID signature tables: per epoch over the signatures, and per signature over
@@ -961,10 +1036,30 @@ int orc_place_mutations(uint32_t n_chr, const uint32_t* len, const uint32_t* pac
         x.expo[e].n = p->n_sbs;
         rc = orc_cdf_table(p->exposure + (size_t)e * p->n_sbs, p->n_sbs, x.expo[e].T);
     }
-    for (uint32_t s = 0; s < p->n_sbs && !rc; ++s) {
+    /* SBS classes: the context index at the engine's sampling, every class whose context has no indexed site
+     * given probability 0 (a signature left without mass is an error) */
+    if (need_snv) rc = sbs_index_build(&x.g, p->context_sampling, &x.sbs);
+    for (uint32_t s = 0; s < p->n_sbs && !rc && need_snv; ++s) {
+        double pmf[96];
+        double mass = 0.0;
+        for (uint32_t k = 0; k < 96; ++k) {
+            pmf[k] = x.sbs.kept[((k >= 48 ? 1u : 0u) << 4) | (k & 15u)] ? p->sbs[(size_t)s * 96 + k] : 0.0;
+            mass += pmf[k];
+        }
+        if (!(mass > 0.0)) rc = fail("SBS signature %u has no class with a site in the context index", s);
         x.cls[s].T = (uint64_t*)malloc(sizeof(uint64_t) * 96);
         x.cls[s].n = 96;
-        rc = orc_cdf_table(p->sbs + (size_t)s * 96, 96, x.cls[s].T);
+        if (!rc) rc = orc_cdf_table(pmf, 96, x.cls[s].T);
+    }
+    x.drv_pos = (uint64_t*)malloc(sizeof(uint64_t) * (p->n_drivers ? p->n_drivers : 1));
+    for (uint32_t d = 0; d < p->n_drivers && !rc; ++d)
+        if (p->drivers[d].chr < n_chr) x.drv_pos[x.n_drv_pos++] = x.g.boff[p->drivers[d].chr] + p->drivers[d].pos;
+    qsort(x.drv_pos, x.n_drv_pos, sizeof(uint64_t), cmp_u64);
+    {
+        uint64_t u = 0;
+        for (uint64_t i = 0; i < x.n_drv_pos; ++i)
+            if (u == 0 || x.drv_pos[u - 1] != x.drv_pos[i]) x.drv_pos[u++] = x.drv_pos[i];
+        x.n_drv_pos = u;
     }
     if (p->n_id) {
         int need_id = p->n_preneoplastic_indel > 0;

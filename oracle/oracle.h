@@ -62,6 +62,13 @@ int orc_place_mutations(uint32_t n_chr, const uint32_t* len, const uint32_t* pac
 int orc_place_cna(uint32_t n_chr, const uint32_t* len, const orc_forest* f, const psx_engine_params* p,
                   uint64_t seed, psx_cna_event** out, uint32_t* n_out);
 
+/* SBS context index of a reference at a sampling (0 counts as 1): sites per
+ * pyrimidine-centred context (5' << 2 | 3', + 16 for a T centre), the kept
+ * sites (every sampling-th in genome order, from the first) and their list
+ * (32-bit global positions, ascending per context, contexts in order; free
+ * with orc_free). */
+int orc_sbs_index(uint32_t n_chr, const uint32_t* len, const uint32_t* packed, uint32_t sampling,
+                  uint64_t* counts, uint64_t* kept, uint32_t** list);
 /* Indel (ID83) classes of a reference: sites per class, the classes whose
  * sites are listed (sparse: rejection would need more than 1024 tries on
  * average), list ranges off[84] and the lists (global position << 10 |

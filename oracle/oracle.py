@@ -45,6 +45,7 @@ def lib():
                                          C.POINTER(A.EngineParams), C.c_uint64, C.POINTER(C.c_void_p), A.u64p]),
             "orc_sort_table": (st, [vp, C.c_uint64]),
             "orc_id_index": (st, [C.c_uint32, A.u32p, A.u32p, A.u64p, A.u8p, A.u64p, C.POINTER(C.c_void_p)]),
+            "orc_sbs_index": (st, [C.c_uint32, A.u32p, A.u32p, C.c_uint32, A.u64p, A.u64p, C.POINTER(C.c_void_p)]),
             "orc_id_site": (C.c_int, [C.c_uint32, A.u32p, A.u32p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]),
             "orc_place_cna": (st, [C.c_uint32, A.u32p, C.POINTER(OrcForest), C.POINTER(A.EngineParams), C.c_uint64,
                                    C.POINTER(C.c_void_p), A.u32p]),
@@ -180,6 +181,22 @@ def place_cna(lens, forest: Forest, seed: int = 0, **engine) -> np.ndarray:
     lib().orc_free(out)
     del keep
     return arr
+
+
+def sbs_index(lens, packed, sampling: int = 1) -> dict:
+    """SBS context index of a reference: counts[32] sites and kept[32]
+    indexed sites per pyrimidine-centred context, and the list (global
+    positions, ascending per context, contexts in order)."""
+    L = np.ascontiguousarray(lens, np.uint32)
+    pk = np.ascontiguousarray(packed, np.uint32)
+    counts, kept = np.zeros(32, np.uint64), np.zeros(32, np.uint64)
+    out = C.c_void_p()
+    _check(lib().orc_sbs_index(len(L), A.ptr(L, C.c_uint32), A.ptr(pk, C.c_uint32), int(sampling),
+                               A.ptr(counts, C.c_uint64), A.ptr(kept, C.c_uint64), C.byref(out)))
+    n = int(kept.sum())
+    lst = np.frombuffer(C.string_at(out, 4 * n), np.uint32).copy() if n else np.zeros(0, np.uint32)
+    lib().orc_free(out)
+    return {"counts": counts, "kept": kept, "list": lst}
 
 
 def id_index(lens, packed) -> dict:

@@ -73,7 +73,8 @@ class EngineParams(C.Structure):
                 ("n_germline", C.c_uint32), ("germline", C.POINTER(Germline)),
                 ("n_cna", C.c_uint32), ("cna", C.POINTER(Cna)),
                 ("cna_rate", f64p), ("n_cna_catalogue", C.c_uint32), ("cna_catalogue", C.POINTER(CnaRegion)),
-                ("n_id", C.c_uint32), ("id", f64p), ("id_exposure", f64p), ("preneoplastic_id", C.c_uint32)]
+                ("n_id", C.c_uint32), ("id", f64p), ("id_exposure", f64p), ("preneoplastic_id", C.c_uint32),
+                ("context_sampling", C.c_uint32)]
 
 
 class SeqParams(C.Structure):
@@ -157,10 +158,11 @@ def forest_desc(cell_id, parent, species, sample, birth, n_samples) -> tuple[For
 def engine_params(*, snv_rate, indel_rate, sbs, epoch_start, exposure, n_pre_snv=0, n_pre_indel=0,
                   pre_sbs=0, indel_max_len=10, indel_len_p=0.5, infinite_sites=True, ploidy=2,
                   drivers=(), germline=(), cna=(), cna_rate=None, cna_catalogue=(), id_sig=None,
-                  id_exposure=None, pre_id=0) -> tuple[EngineParams, list]:
+                  id_exposure=None, pre_id=0, context_sampling=1) -> tuple[EngineParams, list]:
     """cna: (species, chr, start, len, allele, kind, copies) tuples (kind PSX_CNA_*); cna_rate: passenger
     CNAs per base per division per species (None: none); cna_catalogue: (chr, start, len, kind) regions;
-    id_sig: [n_id][83] indel signatures; id_exposure: [n_epochs][n_id]."""
+    id_sig: [n_id][83] indel signatures; id_exposure: [n_epochs][n_id]; context_sampling: every how many
+    sites of a trinucleotide context the SBS context index keeps (1: all)."""
     snv = np.ascontiguousarray(snv_rate, np.float64)
     ind = np.ascontiguousarray(indel_rate, np.float64)
     sbs_a = np.ascontiguousarray(sbs, np.float64).reshape(-1, 96)
@@ -188,7 +190,7 @@ def engine_params(*, snv_rate, indel_rate, sbs, epoch_start, exposure, n_pre_snv
                      int(ploidy), len(drivers), C.cast(drv, C.POINTER(Driver)), len(germline),
                      C.cast(grm, C.POINTER(Germline)), len(cna), C.cast(cn, C.POINTER(Cna)),
                      ptr(crate, C.c_double), len(cna_catalogue), C.cast(cat, C.POINTER(CnaRegion)),
-                     n_id, ptr(ids, C.c_double), ptr(idx, C.c_double), int(pre_id))
+                     n_id, ptr(ids, C.c_double), ptr(idx, C.c_double), int(pre_id), int(context_sampling))
     return p, [snv, ind, sbs_a, es, ex, drv, grm, cn, crate, cat, ids, idx]
 
 

@@ -46,6 +46,8 @@ def _declare(lib):
         "psx_genome_upload": (st, [vp, C.c_uint32, A.u32p, A.u32p, pp]),
         "psx_genome_words": (C.c_uint64, [C.c_uint32, A.u32p]),
         "psx_genome_download": (st, [vp, vp, A.u32p, C.c_uint64]),
+        "psx_genome_context_index": (st, [vp, vp, C.c_uint32, A.u64p, A.u64p, A.u32p, C.c_uint64,
+                                          C.POINTER(C.c_double)]),
         "psx_genome_free": (None, [vp]),
         "psx_forest_upload": (st, [vp, C.POINTER(A.ForestDesc), pp]),
         "psx_forest_info": (st, [vp, A.u32p, A.u32p, A.u32p]),

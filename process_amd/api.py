@@ -235,9 +235,12 @@ class MutationEngine:
                  germline: Iterable[tuple[str, int, str, int]] = (), ctx: core.Context | None = None,
                  indel_max_len: int = 10, indel_len_p: float = 0.5,
                  ID_signatures: Mapping[str, Sequence[float]] | None = None, passenger_CNAs=None,
-                 tumour_type: str = "", tumour_study: str = ""):
+                 tumour_type: str = "", tumour_study: str = "", context_sampling: int = 100):
         """ID_signatures: COSMIC ID83 signatures (83 classes each); without
         them indels have geometric lengths and uniform positions.
+        context_sampling: the SBS context index keeps every
+        context_sampling-th site of each trinucleotide context
+        (src/mutations.cpp:818-819, default 100; 1 keeps every site).
         passenger_CNAs: the passenger CNA catalogue, a tab-separated file
         (load_passenger_CNAs, filtered by tumour_type and tumour_study) or
         (chr, begin, end, "A"/"D") tuples with 1-based inclusive regions."""
@@ -264,6 +267,9 @@ class MutationEngine:
         self.germline = list(germline)
         self.indel_max_len = indel_max_len
         self.indel_len_p = indel_len_p
+        if int(context_sampling) != context_sampling or context_sampling < 1:
+            raise ValueError("context_sampling must be a positive integer")
+        self.context_sampling = int(context_sampling)
         self._genome: core.Genome | None = None
 
     @property
@@ -410,7 +416,7 @@ class MutationEngine:
                    pre_sbs=self.signature_names.index(pre_sig) if pre_sig in self.signature_names else 0,
                    indel_max_len=self.indel_max_len, indel_len_p=self.indel_len_p,
                    infinite_sites=bool(self.infinite_sites_model), ploidy=2, drivers=drivers, germline=germ,
-                   cna=cna_drivers)
+                   cna=cna_drivers, context_sampling=self.context_sampling)
         if any(cna):
             cat = [(self.reference.chr_index(c), b - 1, e + 1 - b, A.PSX_CNA_AMP if t == "A" else A.PSX_CNA_DEL)
                    for c, b, e, t in self.passenger_CNAs]

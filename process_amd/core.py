@@ -155,6 +155,22 @@ class Genome(_Handle):
         check(lib().psx_genome_download(self.ctx.h, self.h, A.ptr(out, C.c_uint32), n), self.ctx.h)
         return out
 
+    def context_index(self, sampling: int = 1, with_list: bool = False) -> dict:
+        """The SBS context index at a sampling (built on the GPU on first
+        use): counts[32] and kept[32] sites per context, build_ms, and with
+        with_list the indexed global positions (ascending per context)."""
+        counts, kept = np.zeros(32, np.uint64), np.zeros(32, np.uint64)
+        ms = C.c_double()
+        check(lib().psx_genome_context_index(self.ctx.h, self.h, int(sampling), A.ptr(counts, C.c_uint64),
+                                             A.ptr(kept, C.c_uint64), None, 0, C.byref(ms)), self.ctx.h)
+        out = {"counts": counts, "kept": kept, "build_ms": ms.value}
+        if with_list:
+            lst = np.zeros(max(1, int(kept.sum())), np.uint32)
+            check(lib().psx_genome_context_index(self.ctx.h, self.h, int(sampling), None, None,
+                                                 A.ptr(lst, C.c_uint32), len(lst), None), self.ctx.h)
+            out["list"] = lst[:int(kept.sum())]
+        return out
+
 
 class Forest(_Handle):
     """Flattened, device-resident samples forest (psx_forest)."""

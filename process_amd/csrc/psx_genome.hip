@@ -167,20 +167,36 @@ extern "C" void psx_genome_free(psx_genome* g) {
     delete g;
 }
 
-// d_recs must hold m->n records in SID order with sid_head set; fills the SoA
-// arrays, the chromosome ranges and the per-stratum index.
+static size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+size_t psx_mutations_bytes(uint64_t n) {
+    return align256(n * sizeof(psx_mutation)) + 4 * align256(n * 4u);
+}
+
+void psx_mutations_carve(psx_mutations* m, const std::shared_ptr<dbuf>& block, size_t off) {
+    uint8_t* b = (uint8_t*)block->p + off;
+    const uint64_t n = m->n;
+    m->store = block;
+    m->recs = (psx_mutation*)b;
+    b += align256(n * sizeof(psx_mutation));
+    m->pos = (uint32_t*)b;
+    b += align256(n * 4u);
+    m->lo = (uint32_t*)b;
+    b += align256(n * 4u);
+    m->hi = (uint32_t*)b;
+    b += align256(n * 4u);
+    m->info = (uint32_t*)b;
+}
+
+// m->recs must hold m->n records in SID order with sid_head set; fills the
+// SoA arrays, the chromosome ranges and the per-stratum index.
 void psx_mutations_finish(psx_ctx* ctx, psx_mutations* m, const std::vector<uint64_t>& chr_begin) {
     const uint64_t n = m->n;
     m->chr_begin = chr_begin;
     m->chr_begin.resize(PSX_MAX_CHROMOSOMES + 1, n);
-    m->d_pos.alloc(n * 4u);
-    m->d_lo.alloc(n * 4u);
-    m->d_hi.alloc(n * 4u);
-    m->d_info.alloc(n * 4u);
     if (n) {
-        hipLaunchKernelGGL(k_mut_soa, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, ctx->stream,
-                           m->d_recs.as<psx_mutation>(), n, m->d_pos.as<uint32_t>(),
-                           m->d_lo.as<uint32_t>(), m->d_hi.as<uint32_t>(), m->d_info.as<uint32_t>());
+        hipLaunchKernelGGL(k_mut_soa, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, ctx->stream, m->recs, n,
+                           m->pos, m->lo, m->hi, m->info);
         PSX_HIP(hipGetLastError());
     }
     psx_mutations_index(ctx, m, m->chr_len);
@@ -202,7 +218,7 @@ const psx_mut_index& psx_mutations_index(psx_ctx* ctx, psx_mutations* m, const s
     auto ix = std::make_unique<psx_mut_index>();
     ix->chr_len = chr_len;
     bucket_args a{};
-    a.pos = m->d_pos.as<uint32_t>();
+    a.pos = m->pos;
     a.n_chr = n_chr;
     ix->bucket_off.assign(n_chr + 1, 0);
     for (uint32_t c = 0; c <= PSX_MAX_CHROMOSOMES; ++c) a.chr_begin[c] = m->chr_begin[c];
@@ -244,9 +260,11 @@ void psx_mutations_build(psx_ctx* ctx, psx_mutations* m, std::vector<psx_mutatio
     }
     m->n = n;
     m->chr_len = chr_len;
-    m->d_recs.alloc(n * sizeof(psx_mutation));
+    auto block = std::make_shared<dbuf>();
+    block->alloc(psx_mutations_bytes(n));
+    psx_mutations_carve(m, block, 0);
     if (n)
-        PSX_HIP(hipMemcpyAsync(m->d_recs.p, recs.data(), n * sizeof(psx_mutation), hipMemcpyHostToDevice,
+        PSX_HIP(hipMemcpyAsync(m->recs, recs.data(), n * sizeof(psx_mutation), hipMemcpyHostToDevice,
                                ctx->stream));
     psx_mutations_finish(ctx, m, chr_begin);
     PSX_HIP(hipStreamSynchronize(ctx->stream));
@@ -330,7 +348,7 @@ extern "C" psx_status psx_mutations_download(psx_ctx* ctx, const psx_mutations* 
                                 (unsigned long long)m->n);
         PSX_HIP(hipSetDevice(ctx->device));
         if (m->n)
-            PSX_HIP(hipMemcpyAsync(recs, m->d_recs.p, m->n * sizeof(psx_mutation), hipMemcpyDeviceToHost,
+            PSX_HIP(hipMemcpyAsync(recs, m->recs, m->n * sizeof(psx_mutation), hipMemcpyDeviceToHost,
                                    ctx->stream));
         PSX_HIP(hipStreamSynchronize(ctx->stream));
     });

@@ -56,6 +56,8 @@ extern "C" void psx_ctx_destroy(psx_ctx* ctx) {
         (void)hipStreamSynchronize(ctx->stream);
         ctx->scratch.release();
         ctx->queue.release();
+        ctx->place_nodes.release();
+        ctx->place_muts.release();
         (void)hipEventDestroy(ctx->ev0);
         (void)hipEventDestroy(ctx->ev1);
         (void)hipStreamDestroy(ctx->stream);

@@ -5,6 +5,7 @@
 
 #include <cstdarg>
 #include <cstdint>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -79,6 +80,8 @@ struct psx_ctx {
     double last_place_ms = 0.0;
     dbuf scratch;       // per-call tables and descriptors
     dbuf queue;         // persistent-kernel work counters
+    dbuf place_nodes;   // placement temporaries per node and per mutation (grown, kept for later calls)
+    dbuf place_muts;
     uint8_t* noalloc_scratch = nullptr;  // device address of the device's no-allocation scratch
     bool tuned = false;
     psx_launch_config tuned_cfg{};
@@ -99,6 +102,21 @@ struct psx_id_index {
     double build_ms = 0.0;
 };
 
+// SBS context index of a genome's reference at one sampling s: per
+// pyrimidine-centred trinucleotide context (32), the sites (positions q with
+// 1 <= q < len - 1) counted, and every s-th one in genome order from the
+// first kept as a 32-bit global position, ascending per context. d_meta holds
+// kept[32] then off[33] (uint64) on the device. Built on first use.
+struct psx_sbs_index {
+    uint32_t sampling = 1;
+    uint64_t count[32] = {};
+    uint64_t kept[32] = {};
+    uint64_t off[33] = {};
+    dbuf d_list;
+    dbuf d_meta;
+    double build_ms = 0.0;
+};
+
 struct psx_genome {
     psx_ctx* ctx = nullptr;
     uint32_t n_chr = 0;
@@ -109,10 +127,15 @@ struct psx_genome {
     dbuf packed;
     std::mutex id_mu;
     std::unique_ptr<psx_id_index> id_index;
+    std::mutex sbs_mu;
+    std::map<uint32_t, std::unique_ptr<psx_sbs_index>> sbs_index;  // by sampling
 };
 
 // The genome's indel index, built on ctx's stream at the first call.
 const psx_id_index& psx_genome_id_index(psx_ctx* ctx, psx_genome* g);
+// The genome's SBS context index at a sampling (0 counts as 1), built on ctx's
+// stream at the first call with that sampling.
+const psx_sbs_index& psx_genome_sbs_index(psx_ctx* ctx, psx_genome* g, uint32_t sampling);
 
 struct psx_forest {
     psx_ctx* ctx = nullptr;
@@ -145,14 +168,22 @@ struct psx_mutations {
     uint32_t n_leaves = 0;
     std::vector<uint64_t> chr_begin;    // PSX_MAX_CHROMOSOMES + 1
     std::vector<uint32_t> chr_len;      // chromosome lengths at placement (uploads: largest position + 1)
-    dbuf d_recs;                        // psx_mutation[n]
-    dbuf d_pos, d_lo, d_hi, d_info;     // uint32[n]
+    // device storage: the records (psx_mutation[n]) and the kernels' SoA (uint32[n] each), carved out of one
+    // block that the tables of one batched placement share
+    std::shared_ptr<dbuf> store;
+    psx_mutation* recs = nullptr;
+    uint32_t *pos = nullptr, *lo = nullptr, *hi = nullptr, *info = nullptr;
     // every stratum index built so far, one per set of chromosome lengths;
     // kept until the table is freed, since launches in flight may read any
     std::mutex idx_mu;
     std::vector<std::unique_ptr<psx_mut_index>> index;
     std::vector<psx_cna_event> cna;     // copy-number events in DFS order of their node
 };
+
+// Device bytes of a table of n records (records and SoA, 256-byte aligned parts).
+size_t psx_mutations_bytes(uint64_t n);
+// Points m's records and SoA into block at byte offset off (256-aligned).
+void psx_mutations_carve(psx_mutations* m, const std::shared_ptr<dbuf>& block, size_t off);
 
 // Validates copy-number events against a forest and the chromosome lengths.
 void psx_cna_check(const std::vector<psx_cna_event>& ev, uint32_t n_leaves, uint32_t ploidy,
@@ -198,7 +229,7 @@ uint64_t psx_sid_key(const psx_mutation& r);
 // Sorts recs into SID order, marks sid_head and builds the device table.
 void psx_mutations_build(psx_ctx* ctx, psx_mutations* m, std::vector<psx_mutation>& recs,
                          const std::vector<uint32_t>& chr_len);
-// d_recs holds m->n records in SID order: builds the SoA arrays and the
+// m->recs holds m->n records in SID order: builds the SoA arrays and the
 // per-stratum index for the chromosome lengths in m->chr_len.
 void psx_mutations_finish(psx_ctx* ctx, psx_mutations* m, const std::vector<uint64_t>& chr_begin);
 // The per-stratum index of m for chr_len. The first request for a set of

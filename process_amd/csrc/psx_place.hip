@@ -3,8 +3,9 @@
 //   k_edge_counts   one thread per forest node: Poisson SNV / indel counts
 //   (scan)          exclusive sum of counts -> mutation ids per node
 //   k_place         one thread per mutation: signature, SBS class, allele and
-//                   a site whose trinucleotide matches the class (rejection on
-//                   the 2-bit reference in HBM), or an indel
+//                   a site drawn uniformly from the genome's context index
+//                   list of the class's trinucleotide (one list read, driver
+//                   loci redrawn), or an indel (ID classes: list or rejection)
 //   k_conflicts     infinite-sites model: after a footprint sort, a mutation
 //                   touching an earlier-created one is redrawn (next attempt)
 //   (sort + gather) SID order, sid_head flags, chromosome ranges
@@ -22,8 +23,9 @@
 namespace {
 
 constexpr uint32_t SITE_TRIES = 1u << 20;
+constexpr uint32_t LIST_TRIES = 4096;  // draws of a context-index site before the class is given up as driver loci only
 constexpr uint32_t INDEL_TRIES = 1u << 20;
-constexpr int IS_ROUNDS = 64;
+constexpr int IS_ROUNDS = 256;  // checks of the infinite-sites model before placement gives up
 
 enum : uint32_t { ST_OK = 0, ST_NO_SITE = 1, ST_NO_INDEL_ROOM = 2 };
 
@@ -173,7 +175,8 @@ struct place_args {
     uint64_t M;
     psx_mutation* recs;
     uint32_t* attempt;
-    const uint8_t* redraw;  // nullptr = every id
+    const uint32_t* redraw;  // nullptr = every id
+    uint32_t* moved;         // set for every redrawn id
     uint32_t* status;
     // indels from ID signatures (use_id): per-epoch exposure and per-signature class tables (classes without a
     // site in the reference have probability 0), the reference's sites per class and the listed classes' sites
@@ -186,10 +189,29 @@ struct place_args {
     const uint32_t* id_listed;  // [PSX_ID_CLASSES]
     const uint64_t* id_off;     // [PSX_ID_CLASSES + 1]
     const uint64_t* id_list;
+    // SNV sites: the context index at the engine's sampling and the driver loci (global, ascending, unique)
+    const uint32_t* sbs_list;
+    const uint64_t* sbs_kept;   // [32]
+    const uint64_t* sbs_off;    // [33]
+    const uint64_t* drv_pos;
+    uint32_t n_drv_pos;
 };
 
 __device__ const uint32_t SUB_TARGET[6] = {0, 2, 3, 0, 1, 2};  // C>A C>G C>T T>A T>C T>G
 
+__device__ __forceinline__ bool driver_locus(const place_args& a, uint64_t gp) {
+    uint32_t lo = 0, hi = a.n_drv_pos;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (a.drv_pos[mid] < gp) lo = mid + 1; else hi = mid;
+    }
+    return lo < a.n_drv_pos && a.drv_pos[lo] == gp;
+}
+
+// SNV: signature from the node's epoch (or the preneoplastic one), class from
+// the signature (classes whose context has no indexed site have probability
+// 0), allele uniform, site uniform over the class's context-index sites other
+// than the driver loci; the strand of the site's base gives the alternative.
 __device__ uint32_t draw_snv(const place_args& a, uint32_t v, uint32_t k, uint32_t att, int fixed_sig,
                              psx_mutation& r) {
     const psx_u4 w = psx_draw(a.seed, k, v, att, PSX_P_SNV << 24);
@@ -197,26 +219,19 @@ __device__ uint32_t draw_snv(const place_args& a, uint32_t v, uint32_t k, uint32
                                         : psx_cdf_pick(a.expo_T + (size_t)a.node_epoch[v] * a.n_sbs, a.n_sbs, w.x);
     const uint32_t cls = psx_cdf_pick(a.cls_T + (size_t)sig * 96u, 96u, w.y);
     const uint32_t allele = psx_uniform32(w.z, a.ploidy);
-    const uint32_t sub = cls >> 4, five = (cls >> 2) & 3u, three = cls & 3u;
-    const uint32_t mid = sub < 3 ? 1u : 3u, tgt = SUB_TARGET[sub];
-    psx_u4 ws{0, 0, 0, 0};
-    for (uint32_t t = 0; t < SITE_TRIES; ++t) {
-        if ((t & 1u) == 0) ws = psx_draw(a.seed, k, v, att, (PSX_P_SITE << 24) | (t >> 1));
-        const uint32_t hi = (t & 1u) ? ws.z : ws.x, lo = (t & 1u) ? ws.w : ws.y;
+    const uint32_t sub = cls >> 4, tgt = SUB_TARGET[sub];
+    const uint32_t ctx = (sub >= 3u ? 16u : 0u) | (cls & 15u);
+    const uint64_t n = a.sbs_kept[ctx];
+    const uint32_t* list = a.sbs_list + a.sbs_off[ctx];
+    for (uint32_t t = 0; t < LIST_TRIES && n; ++t) {
+        const psx_u4 ws = psx_draw(a.seed, k, v, att, (PSX_P_SITE << 24) | t);
+        const uint32_t gp = list[psx_uniform64(ws.x, ws.y, n)];
+        if (a.n_drv_pos && driver_locus(a, gp)) continue;
         uint32_t c, pos;
-        glocate(a.g, psx_uniform64(hi, lo, a.g.G), c, pos);
-        if (pos < 1 || pos + 1 >= a.g.len[c]) continue;
+        glocate(a.g, gp, c, pos);
         const uint32_t m = gbase(a.g, c, pos);
-        const uint32_t b5 = gbase(a.g, c, pos - 1), b3 = gbase(a.g, c, pos + 1);
-        uint32_t alt;
-        if (m == 1u || m == 3u) {
-            if (!(b5 == five && m == mid && b3 == three)) continue;
-            alt = tgt;
-        } else {
-            if (!(3u - b3 == five && 3u - m == mid && 3u - b5 == three)) continue;
-            alt = 3u - tgt;
-        }
-        r.chr = c; r.pos = pos; r.kind = PSX_SNV; r.ref = (uint8_t)m; r.alt = (uint8_t)alt;
+        r.chr = c; r.pos = pos; r.kind = PSX_SNV; r.ref = (uint8_t)m;
+        r.alt = (uint8_t)((m == 1u || m == 3u) ? tgt : 3u - tgt);
         r.len = 0; r.seq = 0; r.allele = (uint8_t)allele; r.cause = (uint16_t)sig;
         return ST_OK;
     }
@@ -344,6 +359,7 @@ __global__ void __launch_bounds__(256) k_place(const place_args a) {
     if (a.redraw) {
         att = a.attempt[id] + 1;
         a.attempt[id] = att;
+        a.moved[id] = 1u;
     }
     psx_mutation r;
     const uint32_t st = place_one(a, id, att, r);
@@ -384,89 +400,224 @@ __global__ void __launch_bounds__(256) k_edge_counts(const edge_args a) {
     a.cnt_tot[v] = (uint64_t)ns + ni;
 }
 
+// Padded position of a record: pstart[c] = boff[c] + 16 c, so positions of
+// different chromosomes are at least 16 apart and footprints (at most 11
+// bases) on different chromosomes never touch.
+__device__ __forceinline__ uint64_t padded_pos(const uint64_t* pstart, const psx_mutation& r) {
+    return pstart[r.chr] + r.pos;
+}
+
+// footprint key: padded position << 4 | deleted length (0 for SNVs and insertions)
+__device__ __forceinline__ uint64_t fp_key(const uint64_t* pstart, const psx_mutation& r) {
+    return (padded_pos(pstart, r) << 4) | (r.kind == PSX_DEL ? (uint64_t)(r.len & 15u) : 0ull);
+}
+
+__device__ __forceinline__ bool fp_touch(uint64_t ka, uint64_t kb) {
+    const uint64_t sa = ka >> 4, sb = kb >> 4;
+    return sb <= sa + (ka & 15u) + 1u && sa <= sb + (kb & 15u) + 1u;
+}
+
 __global__ void __launch_bounds__(256) k_footprint(const psx_mutation* __restrict__ recs, uint64_t M,
-                                                   uint64_t* __restrict__ fkey, uint32_t* __restrict__ ids) {
+                                                   const uint64_t* __restrict__ pstart, uint64_t* __restrict__ fkey,
+                                                   uint32_t* __restrict__ ids) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= M) return;
-    fkey[i] = ((uint64_t)recs[i].chr << 32) | recs[i].pos;
+    fkey[i] = fp_key(pstart, recs[i]);
     ids[i] = (uint32_t)i;
 }
 
-__device__ __forceinline__ uint32_t fp_end(const psx_mutation& r) {
-    return r.pos + (r.kind == PSX_DEL ? (uint32_t)r.len : 0u);
+__device__ __forceinline__ void mark_loser(uint32_t* lose, uint32_t* n_lose, uint32_t id) {
+    if (atomicExch(&lose[id], 1u) == 0u) atomicAdd(n_lose, 1u);
 }
 
-// Footprints [pos, pos + deleted length] touching (gap < 2) an earlier-created
-// mutation's footprint lose; drivers and germline mutations never lose.
-__global__ void __launch_bounds__(256) k_conflicts(const psx_mutation* __restrict__ recs,
-                                                   const uint64_t* __restrict__ fkey,
-                                                   const uint32_t* __restrict__ ids, uint64_t M,
-                                                   uint64_t n_fixed, uint8_t* __restrict__ loser,
-                                                   uint32_t* __restrict__ n_lose) {
+// First round of the infinite-sites model over footprints sorted by key:
+// footprints touching (gap < 2) an earlier-created mutation's footprint lose;
+// drivers and germline mutations (ids below n_fixed) never lose. Every id's
+// flag is written (ids is a permutation).
+__global__ void __launch_bounds__(256) k_conflicts(const uint64_t* __restrict__ fkey,
+                                                   const uint32_t* __restrict__ ids, uint64_t M, uint64_t n_fixed,
+                                                   uint32_t* __restrict__ lose, uint32_t* __restrict__ n_lose) {
     const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= M) return;
     const uint32_t idj = ids[j];
-    loser[idj] = 0;
-    if (idj < n_fixed) return;
-    const uint64_t kj = fkey[j];
-    const uint32_t chr = (uint32_t)(kj >> 32), sj = (uint32_t)kj, ej = fp_end(recs[idj]);
-    bool lose = false;
-    for (uint64_t i = j; i-- > 0 && !lose;) {
-        const uint64_t ki = fkey[i];
-        const uint32_t si = (uint32_t)ki;
-        if ((uint32_t)(ki >> 32) != chr || si + 12u < sj) break;
-        const uint32_t idi = ids[i];
-        if (idi < idj && (uint64_t)sj <= (uint64_t)fp_end(recs[idi]) + 1u && (uint64_t)si <= (uint64_t)ej + 1u)
-            lose = true;
+    bool l = false;
+    if (idj >= n_fixed) {
+        const uint64_t kj = fkey[j], sj = kj >> 4, ej = sj + (kj & 15u);
+        for (uint64_t i = j; i-- > 0 && !l;) {
+            const uint64_t ki = fkey[i];
+            if ((ki >> 4) + 12u < sj) break;
+            l = ids[i] < idj && fp_touch(ki, kj);
+        }
+        for (uint64_t i = j + 1; i < M && !l; ++i) {
+            const uint64_t ki = fkey[i];
+            if ((ki >> 4) > ej + 1u) break;
+            l = ids[i] < idj && fp_touch(ki, kj);
+        }
     }
-    for (uint64_t i = j + 1; i < M && !lose; ++i) {
-        const uint64_t ki = fkey[i];
-        const uint32_t si = (uint32_t)ki;
-        if ((uint32_t)(ki >> 32) != chr || (uint64_t)si > (uint64_t)ej + 1u) break;
-        const uint32_t idi = ids[i];
-        if (idi < idj && (uint64_t)sj <= (uint64_t)fp_end(recs[idi]) + 1u) lose = true;
-    }
-    if (lose) {
-        loser[idj] = 1;
-        atomicAdd(n_lose, 1u);
-    }
+    lose[idj] = l ? 1u : 0u;
+    if (l) atomicAdd(n_lose, 1u);
 }
 
-__global__ void __launch_bounds__(256) k_sid_key(const psx_mutation* __restrict__ recs, uint64_t M,
-                                                 uint64_t* __restrict__ key, uint32_t* __restrict__ ids) {
+// Mutations redrawn since the first round, with their current footprints:
+// slots [0, cap) of b are sentinels (key all ones, id ~0) until filled.
+__global__ void __launch_bounds__(256) k_collect_moved(const psx_mutation* __restrict__ recs, uint64_t M,
+                                                       const uint64_t* __restrict__ pstart,
+                                                       const uint32_t* __restrict__ moved, uint64_t* __restrict__ bkey,
+                                                       uint32_t* __restrict__ bids, uint32_t* __restrict__ nb) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M || !moved[i]) return;
+    const uint32_t slot = atomicAdd(nb, 1u);
+    bkey[slot] = fp_key(pstart, recs[i]);
+    bids[slot] = (uint32_t)i;
+}
+
+__global__ void __launch_bounds__(256) k_fill_sentinel(uint64_t* __restrict__ key, uint32_t* __restrict__ ids,
+                                                       uint64_t n, uint64_t sentinel) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    key[i] = sentinel;
+    ids[i] = 0xFFFFFFFFu;
+}
+
+__device__ __forceinline__ uint64_t lower_bound_u64(const uint64_t* a, uint64_t n, uint64_t x) {
+    uint64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (a[mid] < x) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+struct inc_args {
+    const uint64_t* akey;   // first-round footprints, sorted (entries of moved mutations are stale)
+    const uint32_t* aids;
+    uint64_t M;
+    const uint64_t* bkey;   // moved mutations' current footprints, sorted, sentinels last
+    const uint32_t* bids;
+    uint64_t nb;
+    const uint32_t* moved;
+    const uint32_t* changed;  // redrawn in the last redraw
+    uint32_t* lose;
+    uint32_t* n_lose;
+};
+
+// Later rounds: a pair of touching footprints has a member redrawn in the
+// last redraw (a touching pair left unchanged would have made its later
+// member lose and be redrawn), so only those are checked: each against the
+// first-round footprints of unmoved mutations and the moved ones. The later
+// member of every touching pair loses, as in a full pass.
+__global__ void __launch_bounds__(256) k_conflicts_inc(const inc_args a) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.nb) return;
+    const uint32_t idc = a.bids[t];
+    if (idc == 0xFFFFFFFFu || !a.changed[idc]) return;
+    const uint64_t kc = a.bkey[t], sc = kc >> 4, ec = sc + (kc & 15u);
+    const uint64_t from = sc >= 11u ? (sc - 11u) << 4 : 0u;
+    bool l = false;
+    for (uint64_t i = lower_bound_u64(a.akey, a.M, from); i < a.M; ++i) {
+        const uint64_t ki = a.akey[i];
+        if ((ki >> 4) > ec + 1u) break;
+        const uint32_t idi = a.aids[i];
+        if (a.moved[idi] || !fp_touch(ki, kc)) continue;
+        if (idi < idc) l = true;
+        else mark_loser(a.lose, a.n_lose, idi);
+    }
+    for (uint64_t i = lower_bound_u64(a.bkey, a.nb, from); i < a.nb; ++i) {
+        const uint64_t ki = a.bkey[i];
+        const uint32_t idi = a.bids[i];
+        if (idi == 0xFFFFFFFFu || (ki >> 4) > ec + 1u) break;
+        if (idi == idc || !fp_touch(ki, kc)) continue;
+        if (idi < idc) l = true;
+        else mark_loser(a.lose, a.n_lose, idi);
+    }
+    if (l) mark_loser(a.lose, a.n_lose, idc);
+}
+
+__global__ void __launch_bounds__(256) k_pos_key(const psx_mutation* __restrict__ recs, uint64_t M,
+                                                 const uint64_t* __restrict__ pstart, uint64_t* __restrict__ key,
+                                                 uint32_t* __restrict__ ids) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= M) return;
-    const psx_mutation r = recs[i];
-    const uint64_t payload = r.kind == PSX_SNV ? r.alt : (r.kind == PSX_INS ? r.seq : 0u);
-    key[i] = ((uint64_t)r.chr << 58) | ((uint64_t)r.pos << 26) | ((uint64_t)r.kind << 24) |
-             ((uint64_t)(r.len & 15u) << 20) | (payload & 0xFFFFFu);
+    key[i] = padded_pos(pstart, recs[i]);
     ids[i] = (uint32_t)i;
 }
 
+// SID order within runs of one position: after a stable sort by position,
+// the thread at the start of each run sorts the run by (kind, length,
+// alternative sequence, creation order) and flags the first record of each
+// SID (runs are short: several records share a position only without the
+// infinite-sites model or among drivers and germline mutations).
+__device__ __forceinline__ uint32_t sid_rest(const psx_mutation& r) {
+    const uint32_t payload = r.kind == PSX_SNV ? r.alt : (r.kind == PSX_INS ? r.seq : 0u);
+    return ((uint32_t)r.kind << 24) | ((uint32_t)(r.len & 15u) << 20) | (payload & 0xFFFFFu);
+}
+
+__global__ void __launch_bounds__(256) k_sid_runs(const psx_mutation* __restrict__ recs,
+                                                  const uint64_t* __restrict__ key, uint32_t* __restrict__ ids,
+                                                  uint64_t M, uint8_t* __restrict__ head) {
+    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= M || (j > 0 && key[j - 1] == key[j])) return;
+    uint64_t e = j + 1;
+    while (e < M && key[e] == key[j]) ++e;
+    if (e - j == 1) {
+        head[j] = 1;
+        return;
+    }
+    for (uint64_t i = j + 1; i < e; ++i) {  // insertion sort by (rest, id)
+        const uint32_t id = ids[i], ri = sid_rest(recs[id]);
+        uint64_t k = i;
+        while (k > j) {
+            const uint32_t idp = ids[k - 1], rp = sid_rest(recs[idp]);
+            if (rp < ri || (rp == ri && idp < id)) break;
+            ids[k] = idp;
+            --k;
+        }
+        ids[k] = id;
+    }
+    uint32_t prev = 0;
+    for (uint64_t i = j; i < e; ++i) {
+        const uint32_t r = sid_rest(recs[ids[i]]);
+        head[i] = (i == j || r != prev) ? 1 : 0;
+        prev = r;
+    }
+}
+
 __global__ void __launch_bounds__(256) k_gather(const psx_mutation* __restrict__ src,
-                                                const uint64_t* __restrict__ key,
-                                                const uint32_t* __restrict__ ids, uint64_t M,
-                                                psx_mutation* __restrict__ dst) {
+                                                const uint32_t* __restrict__ ids, const uint8_t* __restrict__ head,
+                                                uint64_t M, psx_mutation* __restrict__ dst) {
     const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= M) return;
     psx_mutation r = src[ids[j]];
-    r.sid_head = (j == 0 || key[j] != key[j - 1]) ? 1 : 0;
+    r.sid_head = head[j];
     dst[j] = r;
 }
 
-__global__ void k_chr_begin(const uint64_t* __restrict__ key, uint64_t M, uint64_t* __restrict__ out) {
+// out[c] = first record of chromosome c in position order (c < n_chr), M beyond
+__global__ void k_chr_begin(const uint64_t* __restrict__ key, uint64_t M, const uint64_t* __restrict__ pstart,
+                            uint32_t n_chr, uint64_t* __restrict__ out) {
     const uint32_t c = threadIdx.x;
     if (c > PSX_MAX_CHROMOSOMES) return;
-    const uint64_t target = c >= 64 ? ~0ull : ((uint64_t)c << 58);
-    uint64_t lo = 0, hi = M;
-    while (lo < hi) {
-        const uint64_t mid = (lo + hi) >> 1;
-        if (key[mid] < target) lo = mid + 1; else hi = mid;
-    }
-    out[c] = c >= 64 ? M : lo;
+    out[c] = c < n_chr ? lower_bound_u64(key, M, pstart[c]) : M;
+}
+
+// Bits that represent x (at least 1).
+inline int bits_for(uint64_t x) {
+    int b = 1;
+    while (b < 64 && (x >> b) != 0) ++b;
+    return b;
 }
 
 inline uint32_t grid_of(uint64_t n) { return (uint32_t)((n + 255) / 256); }
+
+// Byte offsets of 256-aligned parts of one buffer.
+struct carver {
+    size_t bytes = 0;
+    size_t take(size_t n) {
+        const size_t off = bytes;
+        bytes += (std::max<size_t>(n, 1) + 255) & ~size_t(255);
+        return off;
+    }
+};
 
 }  // namespace
 
@@ -603,10 +754,21 @@ extern "C" psx_status psx_place_mutations(psx_ctx* ctx, const psx_genome* g, con
             psx_cdf_table(p->exposure + (size_t)e * p->n_sbs, p->n_sbs, tmp.data());
             append(tmp.data(), tmp.size());
         }
+        // SBS classes: the genome's context index at the engine's sampling (built once per genome and sampling),
+        // every class whose context has no indexed site given probability 0
+        const psx_sbs_index* sidx =
+            need_snv ? &psx_genome_sbs_index(ctx, const_cast<psx_genome*>(g), p->context_sampling) : nullptr;
         const size_t cls_off = T.size();
-        for (uint32_t sg = 0; sg < p->n_sbs; ++sg) {
+        for (uint32_t sg = 0; sg < p->n_sbs && sidx; ++sg) {
+            double pmf[96];
+            double mass = 0.0;
+            for (uint32_t k = 0; k < 96; ++k) {
+                pmf[k] = sidx->kept[(k >= 48 ? 16u : 0u) | (k & 15u)] ? p->sbs[(size_t)sg * 96 + k] : 0.0;
+                mass += pmf[k];
+            }
+            if (!(mass > 0.0)) psx_throw(PSX_EINVAL, "SBS signature %u has no class with a site in the context index", sg);
             tmp.assign(96, 0);
-            psx_cdf_table(p->sbs + (size_t)sg * 96, 96, tmp.data());
+            psx_cdf_table(pmf, 96, tmp.data());
             append(tmp.data(), tmp.size());
         }
         std::vector<double> lp(p->indel_max_len);
@@ -666,6 +828,11 @@ extern "C" psx_status psx_place_mutations(psx_ctx* ctx, const psx_genome* g, con
                 }
             }
         }
+        // driver loci: no SNV site is drawn there (src/mutation_engine.cpp:244-254)
+        std::vector<uint64_t> drv_pos;
+        for (uint32_t d = 0; d < p->n_drivers; ++d) drv_pos.push_back(g->boff[p->drivers[d].chr] + p->drivers[d].pos);
+        std::sort(drv_pos.begin(), drv_pos.end());
+        drv_pos.erase(std::unique(drv_pos.begin(), drv_pos.end()), drv_pos.end());
         for (uint32_t k = 0; k < p->n_germline; ++k) {
             const psx_germline& gm = p->germline[k];
             if (gm.chr >= g->n_chr || gm.pos >= g->len[gm.chr] || gm.alt > 3 || gm.allele >= p->ploidy ||
@@ -774,44 +941,72 @@ extern "C" psx_status psx_place_mutations(psx_ctx* ctx, const psx_genome* g, con
             o_ido = sb.put(idx->off, sizeof idx->off);
         }
         const size_t o_nlose = sb.put(&zero, 4);
+        const size_t o_drvpos = sb.put_vec(drv_pos);
+        std::vector<uint64_t> pstart(g->n_chr);  // padded chromosome starts: boff[c] + 16 c
+        for (uint32_t c = 0; c < g->n_chr; ++c) pstart[c] = g->boff[c] + 16ull * c;
+        const size_t o_pstart = sb.put_vec(pstart);
         uint8_t* base = (uint8_t*)psx_scratch_upload(ctx, sb);
         auto at = [&](size_t off) { return (void*)(base + off); };
 
         PSX_HIP(hipEventRecord(ctx->ev0, s));
         const uint32_t n = f->n_nodes;
-        dbuf d_cnt_snv, d_cnt_tot, d_off, d_tmp;
-        d_cnt_snv.alloc((size_t)n * 4);
-        d_cnt_tot.alloc((size_t)n * 8);
-        d_off.alloc((size_t)n * 8);
+        // per-node temporaries from the context's placement pool (grown, never shrunk)
+        size_t scan_bytes = 0;
+        PSX_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, (uint64_t*)nullptr, (uint64_t*)nullptr, (int)n,
+                                                 s));
+        carver cn;
+        const size_t c_cnt_snv = cn.take((size_t)n * 4), c_cnt_tot = cn.take((size_t)n * 8);
+        const size_t c_off = cn.take((size_t)n * 8), c_scan = cn.take(scan_bytes);
+        ctx->place_nodes.reserve(cn.bytes);
+        uint8_t* pn = (uint8_t*)ctx->place_nodes.p;
+        uint32_t* d_cnt_snv = (uint32_t*)(pn + c_cnt_snv);
+        uint64_t* d_cnt_tot = (uint64_t*)(pn + c_cnt_tot);
+        uint64_t* d_off = (uint64_t*)(pn + c_off);
         edge_args ea{};
         ea.seed = seed; ea.n_nodes = n; ea.parent = f->d_parent.as<uint32_t>(); ea.species = f->d_species.as<uint32_t>();
         ea.snv_k0 = (uint32_t*)at(o_sk0); ea.snv_off = (uint32_t*)at(o_soff); ea.snv_n = (uint32_t*)at(o_sn);
         ea.ind_k0 = (uint32_t*)at(o_ik0); ea.ind_off = (uint32_t*)at(o_ioff); ea.ind_n = (uint32_t*)at(o_in);
-        ea.T = (uint64_t*)at(o_T); ea.cnt_snv = d_cnt_snv.as<uint32_t>(); ea.cnt_tot = d_cnt_tot.as<uint64_t>();
+        ea.T = (uint64_t*)at(o_T); ea.cnt_snv = d_cnt_snv; ea.cnt_tot = d_cnt_tot;
         hipLaunchKernelGGL(k_edge_counts, dim3(grid_of(n)), dim3(256), 0, s, ea);
         PSX_HIP(hipGetLastError());
-        size_t tmp_bytes = 0;
-        PSX_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, d_cnt_tot.as<uint64_t>(), d_off.as<uint64_t>(),
-                                                 (int)n, s));
-        d_tmp.alloc(tmp_bytes);
-        PSX_HIP(hipcub::DeviceScan::ExclusiveSum(d_tmp.p, tmp_bytes, d_cnt_tot.as<uint64_t>(), d_off.as<uint64_t>(),
-                                                 (int)n, s));
-        uint64_t last_off = 0, last_cnt = 0;
-        PSX_HIP(hipMemcpyAsync(&last_off, d_off.as<uint64_t>() + (n - 1), 8, hipMemcpyDeviceToHost, s));
-        PSX_HIP(hipMemcpyAsync(&last_cnt, d_cnt_tot.as<uint64_t>() + (n - 1), 8, hipMemcpyDeviceToHost, s));
+        PSX_HIP(hipcub::DeviceScan::ExclusiveSum(pn + c_scan, scan_bytes, d_cnt_tot, d_off, (int)n, s));
+        uint64_t last[2] = {0, 0};
+        PSX_HIP(hipMemcpyAsync(&last[0], d_off + (n - 1), 8, hipMemcpyDeviceToHost, s));
+        PSX_HIP(hipMemcpyAsync(&last[1], d_cnt_tot + (n - 1), 8, hipMemcpyDeviceToHost, s));
         PSX_HIP(hipStreamSynchronize(s));
+        const uint64_t last_off = last[0], last_cnt = last[1];
         const uint64_t n_pass = last_off + last_cnt;
         const uint64_t n_fixed = drv_d.size() + p->n_germline;
         const uint64_t M = n_fixed + p->n_preneoplastic_snv + p->n_preneoplastic_indel + n_pass;
         if (M >= 0xFFFFFFFFull) psx_throw(PSX_EINVAL, "too many mutations (%llu)", (unsigned long long)M);
 
-        dbuf d_recs, d_attempt, d_redraw, d_key, d_key2, d_ids, d_ids2;
-        d_recs.alloc(M * sizeof(psx_mutation));
-        d_attempt.alloc(M * 4);
-        d_redraw.alloc(M);
-        d_key.alloc(M * 8); d_key2.alloc(M * 8);
-        d_ids.alloc(M * 4); d_ids2.alloc(M * 4);
-        PSX_HIP(hipMemsetAsync(d_attempt.p, 0, M * 4, s));
+        // per-mutation temporaries from the context's placement pool, the table in one block of its own;
+        // sort keys: padded positions (chromosomes 16 apart) << 4 | deleted length
+        const uint64_t p_max = pstart[g->n_chr - 1] + g->len[g->n_chr - 1];
+        const int pbits = bits_for(p_max), kbits = bits_for((p_max << 4) | 15u);
+        size_t sort_bytes = 0;
+        PSX_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (uint64_t*)nullptr, (uint64_t*)nullptr,
+                                                   (uint32_t*)nullptr, (uint32_t*)nullptr, (int)M, 0, kbits, s));
+        carver cm;
+        const size_t c_recs = cm.take(M * sizeof(psx_mutation)), c_att = cm.take(M * 4), c_moved = cm.take(M * 4);
+        const size_t c_lose0 = cm.take(M * 4), c_lose1 = cm.take(M * 4), c_head = cm.take(M);
+        const size_t c_key = cm.take(M * 8), c_key2 = cm.take(M * 8), c_key3 = cm.take(M * 8);
+        const size_t c_ids = cm.take(M * 4), c_ids2 = cm.take(M * 4), c_ids3 = cm.take(M * 4);
+        const size_t c_sort = cm.take(sort_bytes), c_cb = cm.take(8 * (PSX_MAX_CHROMOSOMES + 1));
+        ctx->place_muts.reserve(cm.bytes);
+        uint8_t* pm = (uint8_t*)ctx->place_muts.p;
+        uint64_t* d_key = (uint64_t*)(pm + c_key);
+        uint64_t* d_key2 = (uint64_t*)(pm + c_key2);
+        uint64_t* d_key3 = (uint64_t*)(pm + c_key3);
+        uint32_t* d_ids = (uint32_t*)(pm + c_ids);
+        uint32_t* d_ids2 = (uint32_t*)(pm + c_ids2);
+        uint32_t* d_ids3 = (uint32_t*)(pm + c_ids3);
+        uint32_t* d_lose[2] = {(uint32_t*)(pm + c_lose0), (uint32_t*)(pm + c_lose1)};
+        uint32_t* d_moved = (uint32_t*)(pm + c_moved);
+        const uint64_t* d_pstart = (const uint64_t*)at(o_pstart);
+        void* d_sort = pm + c_sort;
+        PSX_HIP(hipMemsetAsync(pm + c_att, 0, M * 4, s));
+        PSX_HIP(hipMemsetAsync(d_moved, 0, M * 4, s));
         place_args pa{};
         pa.g = gdev{g->packed.as<uint32_t>(), (uint32_t*)at(o_len), (uint64_t*)at(o_woff), (uint64_t*)at(o_boff), G,
                     g->n_chr};
@@ -826,9 +1021,16 @@ extern "C" psx_status psx_place_mutations(psx_ctx* ctx, const psx_genome* g, con
         pa.drv_d = (uint32_t*)at(o_drvd); pa.drv_v = (uint32_t*)at(o_drvv);
         pa.drivers = (psx_driver*)at(o_drivers); pa.germ = (psx_germline*)at(o_germ);
         pa.pre_sig = p->preneoplastic_sbs;
-        pa.cnt_snv = d_cnt_snv.as<uint32_t>(); pa.node_off = d_off.as<uint64_t>();
-        pa.M = M; pa.recs = d_recs.as<psx_mutation>(); pa.attempt = d_attempt.as<uint32_t>();
-        pa.redraw = nullptr; pa.status = (uint32_t*)at(o_status);
+        pa.cnt_snv = d_cnt_snv; pa.node_off = d_off;
+        pa.M = M; pa.recs = (psx_mutation*)(pm + c_recs); pa.attempt = (uint32_t*)(pm + c_att);
+        pa.redraw = nullptr; pa.moved = d_moved; pa.status = (uint32_t*)at(o_status);
+        if (sidx) {
+            pa.sbs_list = sidx->d_list.as<uint32_t>();
+            pa.sbs_kept = sidx->d_meta.as<uint64_t>();
+            pa.sbs_off = sidx->d_meta.as<uint64_t>() + 32;
+        }
+        pa.drv_pos = (const uint64_t*)at(o_drvpos);
+        pa.n_drv_pos = (uint32_t)drv_pos.size();
         if (use_id) {
             pa.use_id = 1; pa.n_id = p->n_id; pa.pre_id = p->preneoplastic_id;
             pa.id_expo_T = (uint64_t*)at(o_T) + id_expo_off; pa.id_cls_T = (uint64_t*)at(o_T) + id_cls_off;
@@ -839,72 +1041,85 @@ extern "C" psx_status psx_place_mutations(psx_ctx* ctx, const psx_genome* g, con
             hipLaunchKernelGGL(k_place, dim3(grid_of(M)), dim3(256), 0, s, pa);
             PSX_HIP(hipGetLastError());
         }
-        auto check_status = [&] {
-            uint32_t stv = 0;
+        // status of the last placement launch and the losers of the last check, read in one synchronisation
+        uint32_t* d_nlose = (uint32_t*)at(o_nlose);
+        auto read_back = [&](bool with_losers) -> uint32_t {
+            uint32_t stv = 0, n_lose = 0;
             PSX_HIP(hipMemcpyAsync(&stv, pa.status, 4, hipMemcpyDeviceToHost, s));
+            if (with_losers) PSX_HIP(hipMemcpyAsync(&n_lose, d_nlose, 4, hipMemcpyDeviceToHost, s));
             PSX_HIP(hipStreamSynchronize(s));
             if (stv & ST_NO_SITE) psx_throw(PSX_EINVAL, "no reference site matches a drawn SBS or ID class");
             if (stv & ST_NO_INDEL_ROOM) psx_throw(PSX_EINVAL, "no room for an indel");
+            return n_lose;
         };
-        check_status();
-        size_t sort_bytes = 0, sort_bytes38 = 0;
-        PSX_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, d_key.as<uint64_t>(), d_key2.as<uint64_t>(),
-                                                   d_ids.as<uint32_t>(), d_ids2.as<uint32_t>(), (int)M, 0, 64, s));
-        PSX_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes38, d_key.as<uint64_t>(), d_key2.as<uint64_t>(),
-                                                   d_ids.as<uint32_t>(), d_ids2.as<uint32_t>(), (int)M, 0, 38, s));
-        sort_bytes = std::max(sort_bytes, sort_bytes38);
-        dbuf d_sort;
-        d_sort.alloc(sort_bytes);
         if (p->infinite_sites && M > 1) {
-            int round = 0;
-            for (; round < IS_ROUNDS; ++round) {
-                hipLaunchKernelGGL(k_footprint, dim3(grid_of(M)), dim3(256), 0, s, pa.recs, M, d_key.as<uint64_t>(),
-                                   d_ids.as<uint32_t>());
-                PSX_HIP(hipGetLastError());
-                PSX_HIP(hipcub::DeviceRadixSort::SortPairs(d_sort.p, sort_bytes, d_key.as<uint64_t>(),
-                                                           d_key2.as<uint64_t>(), d_ids.as<uint32_t>(),
-                                                           d_ids2.as<uint32_t>(), (int)M, 0, 38, s));
-                PSX_HIP(hipMemsetAsync(at(o_nlose), 0, 4, s));
-                hipLaunchKernelGGL(k_conflicts, dim3(grid_of(M)), dim3(256), 0, s, pa.recs,
-                                   (const uint64_t*)d_key2.p, (const uint32_t*)d_ids2.p, M, n_fixed,
-                                   d_redraw.as<uint8_t>(), (uint32_t*)at(o_nlose));
-                PSX_HIP(hipGetLastError());
-                uint32_t n_lose = 0;
-                PSX_HIP(hipMemcpyAsync(&n_lose, at(o_nlose), 4, hipMemcpyDeviceToHost, s));
-                PSX_HIP(hipStreamSynchronize(s));
-                if (n_lose == 0) break;
+            // first round over every footprint; the sorted footprints stay as the reference of later rounds
+            hipLaunchKernelGGL(k_footprint, dim3(grid_of(M)), dim3(256), 0, s, pa.recs, M, d_pstart, d_key, d_ids);
+            PSX_HIP(hipGetLastError());
+            PSX_HIP(hipcub::DeviceRadixSort::SortPairs(d_sort, sort_bytes, d_key, d_key2, d_ids, d_ids2, (int)M, 0,
+                                                       kbits, s));
+            PSX_HIP(hipMemsetAsync(d_nlose, 0, 4, s));
+            hipLaunchKernelGGL(k_conflicts, dim3(grid_of(M)), dim3(256), 0, s, d_key2, d_ids2, M, n_fixed, d_lose[0],
+                               d_nlose);
+            PSX_HIP(hipGetLastError());
+            uint32_t n_lose = read_back(true);
+            uint64_t moved_cap = n_lose;  // redrawn mutations so far, counted with repeats
+            int round = 1, cur = 0;
+            for (; n_lose && round < IS_ROUNDS; ++round) {
                 place_args pr = pa;
-                pr.redraw = d_redraw.as<uint8_t>();
+                pr.redraw = d_lose[cur];
                 hipLaunchKernelGGL(k_place, dim3(grid_of(M)), dim3(256), 0, s, pr);
                 PSX_HIP(hipGetLastError());
-                check_status();
+                // the moved mutations' current footprints, sorted; sentinels fill the unused slots
+                const uint64_t nb = std::min<uint64_t>(moved_cap, M);
+                hipLaunchKernelGGL(k_fill_sentinel, dim3(grid_of(nb)), dim3(256), 0, s, d_key, d_ids, nb,
+                                   (kbits >= 64 ? ~0ull : (1ull << kbits) - 1ull));
+                PSX_HIP(hipGetLastError());
+                PSX_HIP(hipMemsetAsync(d_nlose, 0, 4, s));
+                hipLaunchKernelGGL(k_collect_moved, dim3(grid_of(M)), dim3(256), 0, s, pa.recs, M, d_pstart, d_moved,
+                                   d_key, d_ids, d_nlose);
+                PSX_HIP(hipGetLastError());
+                PSX_HIP(hipcub::DeviceRadixSort::SortPairs(d_sort, sort_bytes, d_key, d_key3, d_ids, d_ids3, (int)nb,
+                                                           0, kbits, s));
+                PSX_HIP(hipMemsetAsync(d_lose[cur ^ 1], 0, M * 4, s));
+                PSX_HIP(hipMemsetAsync(d_nlose, 0, 4, s));
+                inc_args ia{};
+                ia.akey = d_key2; ia.aids = d_ids2; ia.M = M;
+                ia.bkey = d_key3; ia.bids = d_ids3; ia.nb = nb;
+                ia.moved = d_moved; ia.changed = d_lose[cur]; ia.lose = d_lose[cur ^ 1]; ia.n_lose = d_nlose;
+                hipLaunchKernelGGL(k_conflicts_inc, dim3(grid_of(nb)), dim3(256), 0, s, ia);
+                PSX_HIP(hipGetLastError());
+                n_lose = read_back(true);
+                moved_cap += n_lose;
+                cur ^= 1;
             }
-            if (round == IS_ROUNDS) psx_throw(PSX_EINVAL, "infinite-sites placement did not converge");
+            if (n_lose) psx_throw(PSX_EINVAL, "infinite-sites placement did not converge");
+        } else {
+            read_back(false);
         }
-        // ---- SID order
+        // ---- SID order: a stable sort by padded position, each run of one position sorted by the rest of the SID
         mt->n = M;
         mt->ploidy = p->ploidy;
         mt->n_leaves = f->n_leaves;
         mt->chr_len = g->len;
-        mt->d_recs.alloc(M * sizeof(psx_mutation));
+        auto block = std::make_shared<dbuf>();
+        block->alloc(psx_mutations_bytes(M));
+        psx_mutations_carve(mt, block, 0);
         std::vector<uint64_t> chr_begin(PSX_MAX_CHROMOSOMES + 1, M);
         if (M) {
-            hipLaunchKernelGGL(k_sid_key, dim3(grid_of(M)), dim3(256), 0, s, pa.recs, M, d_key.as<uint64_t>(),
-                               d_ids.as<uint32_t>());
+            hipLaunchKernelGGL(k_pos_key, dim3(grid_of(M)), dim3(256), 0, s, pa.recs, M, d_pstart, d_key, d_ids);
             PSX_HIP(hipGetLastError());
-            PSX_HIP(hipcub::DeviceRadixSort::SortPairs(d_sort.p, sort_bytes, d_key.as<uint64_t>(),
-                                                       d_key2.as<uint64_t>(), d_ids.as<uint32_t>(),
-                                                       d_ids2.as<uint32_t>(), (int)M, 0, 64, s));
-            hipLaunchKernelGGL(k_gather, dim3(grid_of(M)), dim3(256), 0, s, pa.recs, (const uint64_t*)d_key2.p,
-                               (const uint32_t*)d_ids2.p, M, mt->d_recs.as<psx_mutation>());
+            PSX_HIP(hipcub::DeviceRadixSort::SortPairs(d_sort, sort_bytes, d_key, d_key2, d_ids, d_ids2, (int)M, 0,
+                                                       pbits, s));
+            uint8_t* d_head = pm + c_head;
+            hipLaunchKernelGGL(k_sid_runs, dim3(grid_of(M)), dim3(256), 0, s, pa.recs, d_key2, d_ids2, M, d_head);
             PSX_HIP(hipGetLastError());
-            dbuf d_cb;
-            d_cb.alloc(8 * (PSX_MAX_CHROMOSOMES + 1));
-            hipLaunchKernelGGL(k_chr_begin, dim3(1), dim3(128), 0, s, (const uint64_t*)d_key2.p, M,
-                               d_cb.as<uint64_t>());
+            hipLaunchKernelGGL(k_gather, dim3(grid_of(M)), dim3(256), 0, s, pa.recs, d_ids2, d_head, M, mt->recs);
             PSX_HIP(hipGetLastError());
-            PSX_HIP(hipMemcpyAsync(chr_begin.data(), d_cb.p, 8 * (PSX_MAX_CHROMOSOMES + 1), hipMemcpyDeviceToHost, s));
-            PSX_HIP(hipStreamSynchronize(s));
+            uint64_t* d_cb = (uint64_t*)(pm + c_cb);
+            hipLaunchKernelGGL(k_chr_begin, dim3(1), dim3(128), 0, s, d_key2, M, d_pstart, g->n_chr, d_cb);
+            PSX_HIP(hipGetLastError());
+            PSX_HIP(hipMemcpyAsync(chr_begin.data(), d_cb, 8 * (PSX_MAX_CHROMOSOMES + 1), hipMemcpyDeviceToHost, s));
         }
         PSX_HIP(hipEventRecord(ctx->ev1, s));
         PSX_HIP(hipStreamSynchronize(s));

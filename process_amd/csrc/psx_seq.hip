@@ -162,8 +162,8 @@ void prepare_job(psx_ctx* ctx, seq_batch& P, const job_in& in, uint32_t spi) {
     const uint32_t job = (uint32_t)P.jobs.size();
     job_dev J{};
     J.seed = in.seed;
-    J.pos = m->d_pos.as<uint32_t>(); J.lo = m->d_lo.as<uint32_t>(); J.hi = m->d_hi.as<uint32_t>();
-    J.info = m->d_info.as<uint32_t>(); J.bucket = ix->d_bucket.as<uint32_t>();
+    J.pos = m->pos; J.lo = m->lo; J.hi = m->hi;
+    J.info = m->info; J.bucket = ix->d_bucket.as<uint32_t>();
     J.sample_leaves = f->d_sample_leaves.as<uint32_t>();
     J.leaf_count = f->d_leaf_count.as<uint4>();
     J.leaf_blocks = f->leaf_blocks;
@@ -547,7 +547,7 @@ extern "C" psx_status psx_seq_vaf_f32(psx_ctx* ctx, const psx_mutations* m, uint
         if (!total) return;
         PSX_HIP(hipSetDevice(ctx->device));
         hipLaunchKernelGGL(k_seq_vaf, dim3((uint32_t)((total + 255) / 256)), dim3(256), 0, ctx->stream,
-                           m->d_info.as<uint32_t>(), m->n, total, d_alt, d_depth, d_present, d_vaf);
+                           m->info, m->n, total, d_alt, d_depth, d_present, d_vaf);
         PSX_HIP(hipGetLastError());
         PSX_HIP(hipStreamSynchronize(ctx->stream));
     });

@@ -135,7 +135,8 @@ def test_read_buffer_shape_is_checked():
 def build(seed=3):
     sig = cases.signatures(2, 11)
     ref = pa.ReferenceGenome({"21": 60_000, "22": 45_000}, seed=17)
-    eng = pa.MutationEngine(ref, {"SBS1": sig[0], "SBS5": sig[1]}, germline=[("22", 1001, "T", 1)])
+    eng = pa.MutationEngine(ref, {"SBS1": sig[0], "SBS5": sig[1]}, germline=[("22", 1001, "T", 1)],
+                            context_sampling=1)
     eng.add_mutant("A", {"SNV": 5e-6, "indel": 5e-7})
     eng.add_mutant("B", {"SNV": 8e-6, "indel": 5e-7}, drivers=[pa.SNV("21", 30_001, "A")])
     eng.add_exposure({"SBS1": 0.7, "SBS5": 0.3})

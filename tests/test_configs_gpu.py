@@ -91,7 +91,8 @@ def test_ploidy_against_oracle(ctx, ploidy, with_cna):
 def api_build(seed=3):
     sig = cases.signatures(2, 11)
     ref = pa.ReferenceGenome({"21": 60_000, "22": 45_000}, seed=17)
-    eng = pa.MutationEngine(ref, {"SBS1": sig[0], "SBS5": sig[1]}, germline=[("22", 1001, "T", 1), ("21", 501, "G", 0)])
+    eng = pa.MutationEngine(ref, {"SBS1": sig[0], "SBS5": sig[1]}, germline=[("22", 1001, "T", 1), ("21", 501, "G", 0)],
+                            context_sampling=1)
     eng.add_mutant("A", {"SNV": 5e-6, "indel": 5e-7})
     eng.add_mutant("B", {"SNV": 8e-6, "indel": 5e-7})
     eng.add_exposure({"SBS1": 0.7, "SBS5": 0.3})
@@ -334,7 +335,7 @@ def test_cna_drivers_through_add_mutant(ctx):
         with open(path, "w") as fh:
             fh.write("chr\tbegin\tend\tmajor\tminor\ttype\tstudy\n1\t1001\t9000\t2\t1\tT\tS\n"
                      "2\t20001\t26000\t1\t0\tT\tS\n1\t40001\t41000\t3\t1\tU\tS\n")
-        eng = pa.MutationEngine(ref, {"SBS1": sig[0]}, passenger_CNAs=path, tumour_type="T")
+        eng = pa.MutationEngine(ref, {"SBS1": sig[0]}, passenger_CNAs=path, tumour_type="T", context_sampling=1)
     assert eng.passenger_CNAs == [("1", 1001, 9000, "A"), ("2", 20001, 26000, "D")]
     eng.add_mutant("A", {"SNV": 5e-6, "indel": 2e-7, "CNA": 5e-8})
     eng.add_mutant("B", {"SNV": 8e-6, "CNA": 1e-7},

@@ -1,0 +1,124 @@
+"""The SBS context index on the GPU against the oracle: counts, kept sites
+and the whole list at several samplings, over chromosomes that end inside,
+at and just past a tile of 4096 positions and the shortest ones (3-5 bases);
+placement through the index with drivers, the infinite-sites model and a
+coarse sampling byte for byte equal to the oracle's; a driver locus never
+drawn on a crafted reference; and the engine API with the reference's
+default sampling of 100 on a 3 Mb reference."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import cases
+import process_amd as pa
+from oracle import oracle as O
+from process_amd import _abi as A
+from process_amd import core
+
+pytestmark = pytest.mark.gpu
+
+LENS = [4096 * 3 + 5, 3, 4, 5, 4096, 4097, 50_000, 4095]
+
+
+@pytest.mark.parametrize("sampling", [1, 5, 100])
+def test_index_matches_oracle(ctx, sampling):
+    g = core.Genome(ctx, LENS, seed=41)
+    packed = g.download()
+    got = g.context_index(sampling, with_list=True)
+    exp = O.sbs_index(LENS, packed, sampling)
+    assert got["counts"].tolist() == exp["counts"].tolist()
+    assert got["kept"].tolist() == exp["kept"].tolist()
+    assert np.array_equal(got["list"], exp["list"])
+    assert int(got["counts"].sum()) == sum(max(0, n - 2) for n in LENS)
+    # cached with the genome: the same index again
+    assert g.context_index(sampling)["build_ms"] == got["build_ms"]
+
+
+@pytest.mark.parametrize("sampling,infinite_sites", [(1, True), (50, True), (13, False)])
+def test_placement_through_the_index_matches_oracle(ctx, sampling, infinite_sites):
+    lens = [300_000, 120_000, 75_000]
+    g = core.Genome(ctx, lens, seed=8)
+    packed = g.download()
+    fd = cases.forest(n_cells=1500, n_samples=2, sample_size=200, seed=12)
+    f = core.Forest(ctx, fd["cell_id"], fd["parent"], fd["species"], fd["sample"], fd["birth"], fd["n_samples"])
+    of = O.Forest(fd["cell_id"], fd["parent"], fd["species"], fd["sample"], fd["birth"], fd["n_samples"])
+    eng = cases.engine(packed, lens, snv=1e-6, indel=1e-7, pre_snv=25, infinite_sites=infinite_sites)
+    eng["context_sampling"] = sampling
+    m = core.Mutations.place(ctx, g, f, 77, **eng)
+    om = O.place_mutations(lens, packed, of, 77, **eng)
+    assert m.download().tobytes() == om.tobytes()
+    assert (om["kind"] == A.PSX_SNV).sum() > 300
+
+
+def test_dense_infinite_sites_rounds(ctx):
+    """About one mutation per 30 bases: many redraw rounds, each checked only
+    around the redrawn footprints; the table equals the oracle's (which
+    checks every footprint every round) and no two footprints touch."""
+    lens = [30_000, 15_000]
+    g = core.Genome(ctx, lens, seed=8)
+    packed = g.download()
+    fd = cases.forest(n_cells=1500, n_samples=2, sample_size=200, seed=12)
+    f = core.Forest(ctx, fd["cell_id"], fd["parent"], fd["species"], fd["sample"], fd["birth"], fd["n_samples"])
+    of = O.Forest(fd["cell_id"], fd["parent"], fd["species"], fd["sample"], fd["birth"], fd["n_samples"])
+    eng = cases.engine(packed, lens, snv=1e-5, indel=2e-6, pre_snv=25)
+    m = core.Mutations.place(ctx, g, f, 77, **eng).download()
+    om = O.place_mutations(lens, packed, of, 77, **eng)
+    assert m.tobytes() == om.tobytes() and len(m) > 1500
+    for c in range(len(lens)):
+        r = m[m["chr"] == c]
+        start = r["pos"].astype(np.int64)
+        end = start + np.where(r["kind"] == A.PSX_DEL, r["len"], 0)
+        order = np.argsort(start, kind="stable")
+        start, end = start[order], end[order]
+        assert (start[1:] > end[:-1] + 1).all()
+
+
+def test_driver_locus_never_drawn(ctx):
+    seq = ["A"] * 400
+    sites = [40, 250]
+    for q in sites:
+        seq[q] = "C"
+    ref = "".join(seq)
+    b = np.array(["ACGT".index(ch) for ch in ref], np.uint32)
+    b = np.concatenate([b, np.zeros((-len(b)) % 16, np.uint32)]).reshape(-1, 16)
+    packed = (b << (2 * np.arange(16, dtype=np.uint32))).sum(axis=1, dtype=np.uint64).astype(np.uint32)
+    g = core.Genome(ctx, [len(ref)], packed=packed)
+    fd = cases.forest(n_cells=600, n_samples=1, sample_size=200, seed=7, two_species=False)
+    f = core.Forest(ctx, fd["cell_id"], fd["parent"], fd["species"], fd["sample"], fd["birth"], fd["n_samples"])
+    of = O.Forest(fd["cell_id"], fd["parent"], fd["species"], fd["sample"], fd["birth"], fd["n_samples"])
+    sig = np.zeros(96)
+    sig[32] = 1.0  # A[C>T]A
+    eng = dict(snv_rate=[5e-4], indel_rate=[0.0], sbs=sig[None, :], epoch_start=[0.0], exposure=[[1.0]],
+               n_pre_snv=5, n_pre_indel=0, pre_sbs=0, indel_max_len=10, indel_len_p=0.5, infinite_sites=False,
+               ploidy=2, drivers=[(0, 0, sites[0], 2, -1)], germline=[])
+    m = core.Mutations.place(ctx, g, f, 3, **eng).download()
+    assert m.tobytes() == O.place_mutations([len(ref)], packed, of, 3, **eng).tobytes()
+    pas = m[m["nature"] != A.PSX_DRIVER]
+    assert len(pas) > 100 and set(pas["pos"].tolist()) == {sites[1]}
+
+
+def test_engine_default_sampling(ctx):
+    ref = pa.ReferenceGenome({"1": 2_000_000, "2": 1_000_000}, seed=5)
+    sig = cases.signatures(2, 21)
+    eng = pa.MutationEngine(ref, {"SBS1": sig[0], "SBS5": sig[1]})
+    assert eng.context_sampling == 100
+    eng.add_mutant("A", {"SNV": 2e-8, "indel": 1e-9})
+    eng.add_mutant("B", {"SNV": 3e-8, "indel": 1e-9}, drivers=[pa.SNV("1", 1_000_001, "A")])
+    eng.add_exposure({"SBS1": 0.5, "SBS5": 0.5})
+    sf = pa.SamplesForest.synthetic(mutants={"A": (0.2, 0.01), "B": (0.3, 0.01)}, introductions={"B": 200},
+                                    n_cells=2000, samples={"S1": 300}, seed=4)
+    pf = eng.place_mutations(sf, 10, 2, seed=19)
+    params = eng._params(sf, 10, "SBS1", 2)
+    assert params["context_sampling"] == 100
+    of = O.Forest(sf.cell_id, sf.ancestor, sf.species_index(), sf.sample_index(sf.sample_names), sf.birth_time, 1)
+    lens, packed = eng.reference.lengths, eng._packed
+    om = O.place_mutations(lens, packed, of, 19, **params)
+    assert pf.records().tobytes() == om.tobytes()
+    ix = O.sbs_index(lens, packed, 100)
+    snv = om[(om["kind"] == A.PSX_SNV) & (om["nature"] == A.PSX_PASSENGER)]
+    boff = np.concatenate([[0], np.cumsum(np.asarray(lens, np.int64))])
+    assert len(snv) > 50
+    assert np.isin(boff[snv["chr"].astype(np.int64)] + snv["pos"], ix["list"].astype(np.int64)).all()
+    with pytest.raises(ValueError, match="context_sampling must be a positive integer"):
+        pa.MutationEngine(ref, {"SBS1": sig[0]}, context_sampling=0)

@@ -26,7 +26,8 @@ pytestmark = pytest.mark.gpu
 def engine(length=60_000, snv=5e-6, infinite_sites=True):
     sig = cases.signatures(2, 5)
     ref = pa.ReferenceGenome({"1": length, "2": length // 2}, seed=9)
-    eng = pa.MutationEngine(ref, {"SBS1": sig[0], "SBS5": sig[1]}, germline=[("2", 101, "C", 0)])
+    eng = pa.MutationEngine(ref, {"SBS1": sig[0], "SBS5": sig[1]}, germline=[("2", 101, "C", 0)],
+                            context_sampling=1)
     eng.add_mutant("A", {"SNV": snv, "indel": snv / 10})
     eng.add_mutant("B", {"SNV": 2 * snv, "indel": snv / 10})
     eng.add_exposure({"SBS1": 0.5, "SBS5": 0.5})

@@ -87,7 +87,8 @@ def test_id_signatures_through_the_engine(ctx):
     ix = O.id_index(ref.lengths, ref.packed)
     sigs, _ = signatures(ix, rng)
     sbs = cases.signatures(2, 4)
-    eng = pa.MutationEngine(ref, {"SBS1": sbs[0], "SBS5": sbs[1]}, ID_signatures={"ID1": sigs[0], "ID8": sigs[1]})
+    eng = pa.MutationEngine(ref, {"SBS1": sbs[0], "SBS5": sbs[1]}, ID_signatures={"ID1": sigs[0], "ID8": sigs[1]},
+                            context_sampling=1)
     eng.add_mutant("A", {"SNV": 2e-6, "indel": 1e-6})
     eng.add_mutant("B", {"SNV": 3e-6, "indel": 2e-6})
     eng.add_exposure({"SBS1": 0.6, "SBS5": 0.4, "ID1": 0.7, "ID8": 0.3})

@@ -26,7 +26,8 @@ NAMES = {"21": 40_000, "22": 23_457}
 def build(seed=4):
     sig = cases.signatures(2, 11)
     ref = pa.ReferenceGenome(NAMES, seed=31)
-    eng = pa.MutationEngine(ref, {"SBS1": sig[0], "SBS5": sig[1]}, germline=[("22", 1001, "T", 1)])
+    eng = pa.MutationEngine(ref, {"SBS1": sig[0], "SBS5": sig[1]}, germline=[("22", 1001, "T", 1)],
+                            context_sampling=1)
     eng.add_mutant("A", {"SNV": 6e-6, "indel": 5e-7})
     eng.add_mutant("B", {"SNV": 9e-6, "indel": 5e-7})
     eng.add_exposure({"SBS1": 0.7, "SBS5": 0.3})
@@ -88,7 +89,7 @@ def test_sam_files_match_oracle_reads(ctx, tmp_path, pe):
     seq = dict(coverage=3.0, read_size=150, error_rate=2e-3, **kw)
     n = pa.api.core.num_reads(eng.genome(), pf.forest, **seq)
     rows = O.simulate_reads(lens, eng._packed, of, pf.records(), 17, n, **seq)
-    names = ["S1", "S2", "normal_sample"]
+    names = list(sf.sample_names) + ["normal_sample"]
     q = chr(33 + round(-10 * math.log10(2e-3)))
     exp = expected_lines(rows, names, eng.reference.names, [int(x) for x in lens], 150, pe, 3.0, q, prefix="tmpl")
     assert sorted(os.listdir(out)) == ["chr_21.sam", "chr_22.sam"]
@@ -145,8 +146,8 @@ def test_reference_genome_override(ctx, tmp_path):
     seq = dict(coverage=2.0, read_size=150, insert_size_mean=350, insert_size_stddev=10)
     n = pa.api.core.num_reads(eng.genome(), pf.forest, **seq)
     rows = O.simulate_reads(lens, O.genome_synth(lens, 77), of, pf.records(), 12, n, **seq)
-    exp = expected_lines(rows, ["S1", "S2", "normal_sample"], eng.reference.names, [int(x) for x in lens], 150, True,
-                         2.0, "I")
+    exp = expected_lines(rows, list(sf.sample_names) + ["normal_sample"], eng.reference.names, [int(x) for x in lens],
+                         150, True, 2.0, "I")
     for c, cname in enumerate(eng.reference.names):
         _, body = read_sam(tmp_path / "b" / f"chr_{cname}.sam")
         assert body == exp[c]
