@@ -26,6 +26,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# serving lanes start a launch beside a running one only once it has run this long (microseconds)
+SERVE_OVERLAP_US = 0
 
 # HBM bytes per launch of the dominant kernels on the plain_large workload, from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE runs calibrated against a device copy, and vector instructions per launch from SQ_INSTS_VALU (the
@@ -187,7 +189,15 @@ def batched_leg(ctx, n_jobs: int, reps: int, tune: bool) -> dict:
     B = workloads.build_batched(ctx, n_jobs=n_jobs)
     leaves = [s["leaves"] for s in B.specs]
     log(f"batched: {n_jobs} jobs built in {time.perf_counter() - t:.1f} s, {B.reads:.3e} reads, "
-        f"leaves {min(leaves)}..{max(leaves)}, placement {B.place_ms:.1f} ms")
+        f"leaves {min(leaves)}..{max(leaves)}, batched placement {B.place_wall_ms:.1f} ms "
+        f"(kernels and copies {B.place_ms:.1f} ms)")
+    # the same placements one call per job
+    eng = B.engine
+    t0 = time.perf_counter()
+    for s, f in zip(B.specs, B.forests):
+        core.Mutations.place(ctx, B.genome, f, s["place_seed"], **eng).close()
+    place_one_ms = (time.perf_counter() - t0) * 1e3
+    log(f"batched: placement one call per job {place_one_ms:.1f} ms")
 
     def timed_batch(cfg=None):
         core.simulate_seq_batch(ctx, B.jobs, cfg)  # warm-up
@@ -234,7 +244,11 @@ def batched_leg(ctx, n_jobs: int, reps: int, tune: bool) -> dict:
                sequential=dict(value=n / seq_wall, unit="reads/s", wall_ms=seq_wall * 1e3,
                                job_p50_ms=float(np.percentile(lat, 50) * 1e3),
                                job_p99_ms=float(np.percentile(lat, 99) * 1e3)),
-               batched_over_sequential=seq_wall / wall, results_identical=bool(same))
+               batched_over_sequential=seq_wall / wall, results_identical=bool(same),
+               placement=dict(batched_wall_ms=B.place_wall_ms, batched_event_ms=B.place_ms,
+                              one_call_per_job_wall_ms=place_one_ms,
+                              mutations=int(sum(m.n for m in B.muts)),
+                              note="every job's place_mutations in one psx_place_mutations_batch call"))
     log(f"batched: {n / wall:.3e} reads/s ({wall * 1e3:.1f} ms), replayed plan {n / plan_wall:.3e} reads/s "
         f"({plan_wall * 1e3:.1f} ms), one by one {n / seq_wall:.3e} reads/s ({seq_wall * 1e3:.1f} ms), "
         f"identical {same}")
@@ -309,7 +323,8 @@ def batched_reads_leg(ctx, B, reps: int) -> dict:
     return {}
 
 
-def serving_leg(ctx, rate: float, duration: float, saturated: int) -> dict:
+def serving_leg(ctx, rate: float, duration: float, saturated: int, overlap_us: int = 0,
+                compare_overlap_us=()) -> dict:
     """The serving workload: Poisson arrivals of requests over device-resident
     forests (1e3-1e6 cells) through the continuous-batching server, against
     the same stream served one request at a time; then the same number of
@@ -326,8 +341,8 @@ def serving_leg(ctx, rate: float, duration: float, saturated: int) -> dict:
         f"{time.perf_counter() - t:.1f} s, {(f0 - f1) / 2**20:.0f} MiB")
     budget = 2 << 30
 
-    def run(stream, lanes, max_batch, paced, label):
-        srv = core.Server(ctx, budget, max_inflight=lanes, max_batch=max_batch)
+    def run(stream, lanes, max_batch, paced, label, overlap=0):
+        srv = core.Server(ctx, budget, max_inflight=lanes, max_batch=max_batch, overlap_after_us=overlap)
         tickets = []
         t0 = time.perf_counter()
 
@@ -371,12 +386,16 @@ def serving_leg(ctx, rate: float, duration: float, saturated: int) -> dict:
                         "forests of 1e3-1e6 cells (2-3 samples of <= 1e4 cells), shared 50 Mb reference, "
                         "coverage log-uniform 30-500x, PE 150 bp, error 1e-3, random sample subsets + normal",
                resident_input_bytes=int(f0 - f1), hbm_budget_bytes=budget,
-               continuous=run(stream, 2, 16, True, "continuous"),
+               overlap_after_us=overlap_us,
+               continuous=run(stream, 2, 16, True, "continuous", overlap_us),
                one_at_a_time=run(stream, 1, 1, True, "one at a time"))
+    for us in compare_overlap_us:
+        out.setdefault("continuous_by_overlap_us", {})[str(us)] = run(stream, 2, 16, True,
+                                                                       f"continuous, overlap after {us} us", us)
     if saturated:
         burst = workloads.serving_requests(pool, saturated, 0.0, seed=10)
         out["saturated"] = dict(requests=saturated,
-                                continuous=run(burst, 2, 16, False, "saturated continuous"),
+                                continuous=run(burst, 2, 16, False, "saturated continuous", overlap_us),
                                 one_at_a_time=run(burst, 1, 1, False, "saturated one at a time"))
     return out
 
@@ -437,6 +456,11 @@ def main() -> int:
                     help="only build the plain-large workload and repeat its placement (for profiling)")
     ap.add_argument("--probe-only", action="store_true",
                     help="only run the vector-ALU probe kernels (for profiling their instruction counts)")
+    ap.add_argument("--serve-overlap-us", type=int, default=SERVE_OVERLAP_US,
+                    help="serving lanes start a launch beside a running one only after it ran this long")
+    ap.add_argument("--serve-compare-overlap-us", type=str, default="",
+                    help="comma-separated overlap settings also run on the paced stream (comparison)")
+    ap.add_argument("--serving-only", action="store_true", help="only run the serving workload")
     ap.add_argument("--batched-reads-only", action="store_true",
                     help="only run the batched read materialisation (one launch per repetition; for profiling)")
     args = ap.parse_args()
@@ -458,6 +482,11 @@ def main() -> int:
         for _ in range(3):
             check(lib().psx_dev_copy(ctx.h, C.c_void_p(dst.ptr), C.c_void_p(src.ptr), args.calibrate_copy), ctx.h)
         log(f"copied {args.calibrate_copy} bytes 3 times")
+        return 0
+    if args.serving_only:
+        cmp = [int(x) for x in args.serve_compare_overlap_us.split(",") if x]
+        print(json.dumps(serving_leg(ctx, args.serve_rate, args.serve_seconds, args.serve_burst,
+                                     args.serve_overlap_us, cmp)), flush=True)
         return 0
     if args.probe_only:
         print(json.dumps(valu_ceiling(ctx)), flush=True)
@@ -614,7 +643,8 @@ def main() -> int:
         out["batched"] = batched_leg(ctx, args.batched_jobs, 3, not args.no_tune)
 
     if not args.no_serving:
-        out["serving"] = serving_leg(ctx, args.serve_rate, args.serve_seconds, args.serve_burst)
+        out["serving"] = serving_leg(ctx, args.serve_rate, args.serve_seconds, args.serve_burst,
+                                     args.serve_overlap_us)
 
     if not args.no_cpu:
         recs = built.muts.download()

@@ -294,6 +294,16 @@ typedef struct psx_mutation {
 
 psx_status psx_place_mutations(psx_ctx* ctx, const psx_genome* g, const psx_forest* f,
                                const psx_engine_params* p, uint64_t seed, psx_mutations** out);
+/* Placement of n_jobs forests over one genome in one call, their device
+ * phases run together with one synchronisation per phase. No counterpart in
+ * the reference (which places one forest per MutationEngine call,
+ * src/mutation_engine.cpp:1085-1113): the batched form for workloads of many
+ * independent forests. Table k is identical to psx_place_mutations(ctx, g,
+ * f[k], p[k], seeds[k]); the tables share one device block, freed with the
+ * last of them. On an error (its message names the job) no table is made. */
+psx_status psx_place_mutations_batch(psx_ctx* ctx, const psx_genome* g, uint32_t n_jobs,
+                                     const psx_forest* const* f, const psx_engine_params* const* p,
+                                     const uint64_t* seeds, psx_mutations** out);
 /* Replay: a caller-provided (e.g. exported) table; it is sorted by SID here. */
 psx_status psx_mutations_upload(psx_ctx* ctx, const psx_forest* f, const psx_mutation* recs,
                                 uint64_t n, uint32_t ploidy, psx_mutations** out);
@@ -516,6 +526,11 @@ typedef struct psx_server_config {
     uint32_t max_inflight;     /* concurrent launches, 1..8 */
     uint32_t max_batch;        /* requests per launch, >= 1 */
     psx_launch_config launch;  /* kernel configuration of every launch (use_graph ignored) */
+    /* a free lane starts a launch while another lane's launch runs only once
+     * that launch has run overlap_after_us microseconds, or when max_batch
+     * requests are queued (0: at once). Short launches then finish alone
+     * instead of sharing the compute units with the next request's. */
+    uint32_t overlap_after_us;
 } psx_server_config;
 
 typedef struct psx_request {

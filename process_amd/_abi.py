@@ -104,7 +104,7 @@ class LaunchConfig(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget", C.c_uint64), ("max_inflight", C.c_uint32), ("max_batch", C.c_uint32),
-                ("launch", LaunchConfig)]
+                ("launch", LaunchConfig), ("overlap_after_us", C.c_uint32)]
 
 
 class Request(C.Structure):

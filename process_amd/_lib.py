@@ -54,6 +54,8 @@ def _declare(lib):
         "psx_forest_flat": (st, [vp, A.i64p, A.u32p, A.u32p, A.u32p, A.u32p, A.u32p]),
         "psx_forest_free": (None, [vp]),
         "psx_place_mutations": (st, [vp, vp, vp, C.POINTER(A.EngineParams), C.c_uint64, pp]),
+        "psx_place_mutations_batch": (st, [vp, vp, C.c_uint32, C.POINTER(vp), C.POINTER(C.POINTER(A.EngineParams)),
+                                           A.u64p, C.POINTER(vp)]),
         "psx_mutations_upload": (st, [vp, vp, vp, C.c_uint64, C.c_uint32, pp]),
         "psx_mutations_cna": (st, [vp, vp, C.c_uint32, u32p_]),
         "psx_mutations_set_cna": (st, [vp, vp, vp, C.c_uint32]),

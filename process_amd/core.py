@@ -221,6 +221,24 @@ class Mutations(_Handle):
         return cls(ctx, h, int(engine.get("ploidy", 2)))
 
     @classmethod
+    def place_batch(cls, ctx: Context, genome: Genome, forests: list[Forest], seeds, engines: list[dict]) \
+            -> list["Mutations"]:
+        """Placement of many forests in one call (psx_place_mutations_batch):
+        table k equals Mutations.place(ctx, genome, forests[k], seeds[k],
+        **engines[k])."""
+        n = len(forests)
+        if not (n == len(engines) == len(seeds)) or n == 0:
+            raise ValueError("one seed and one engine per forest, at least one forest")
+        built = [A.engine_params(**e) for e in engines]
+        fs = (C.c_void_p * n)(*[f.h for f in forests])
+        ps = (C.POINTER(A.EngineParams) * n)(*[C.pointer(b[0]) for b in built])
+        sd = np.ascontiguousarray(seeds, np.uint64)
+        hs = (C.c_void_p * n)()
+        check(lib().psx_place_mutations_batch(ctx.h, genome.h, n, fs, ps, A.ptr(sd, C.c_uint64), hs), ctx.h)
+        del built
+        return [cls(ctx, C.c_void_p(hs[k]), int(engines[k].get("ploidy", 2))) for k in range(n)]
+
+    @classmethod
     def upload(cls, ctx: Context, forest: Forest, recs: np.ndarray, ploidy: int = 2) -> "Mutations":
         r = np.ascontiguousarray(recs, A.MUTATION_DTYPE)
         h = C.c_void_p()
@@ -484,13 +502,13 @@ class Server:
     sequencing requests within an HBM budget, results copied to host arrays."""
 
     def __init__(self, ctx: Context, hbm_budget: int, max_inflight: int = 2, max_batch: int = 16,
-                 launch: A.LaunchConfig | None = None):
+                 launch: A.LaunchConfig | None = None, overlap_after_us: int = 0):
         self.ctx = ctx
         if launch is None:
             # every lane launches a full persistent grid: a launch that starts
             # while another runs takes its compute units as they free up
             launch = A.LaunchConfig(256, 512, 1, 8, 0)
-        cfg = A.ServerConfig(int(hbm_budget), int(max_inflight), int(max_batch), launch)
+        cfg = A.ServerConfig(int(hbm_budget), int(max_inflight), int(max_batch), launch, int(overlap_after_us))
         h = C.c_void_p()
         check(lib().psx_server_create(ctx.h, C.byref(cfg), C.byref(h)), ctx.h)
         self.h = h

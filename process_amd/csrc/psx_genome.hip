@@ -57,7 +57,7 @@ struct bucket_args {
     const uint32_t* pos;
     uint32_t* bucket;
     uint32_t n_chr;
-    uint64_t chr_begin[PSX_MAX_CHROMOSOMES + 1];
+    const uint64_t* chr_begin;                 // [PSX_MAX_CHROMOSOMES + 1] on the device
     uint64_t boff[PSX_MAX_CHROMOSOMES + 1];  // first bucket entry of each chromosome
 };
 
@@ -170,7 +170,7 @@ extern "C" void psx_genome_free(psx_genome* g) {
 static size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
 size_t psx_mutations_bytes(uint64_t n) {
-    return align256(n * sizeof(psx_mutation)) + 4 * align256(n * 4u);
+    return align256(n * sizeof(psx_mutation)) + 4 * align256(n * 4u) + align256(8 * (PSX_MAX_CHROMOSOMES + 1));
 }
 
 void psx_mutations_carve(psx_mutations* m, const std::shared_ptr<dbuf>& block, size_t off) {
@@ -186,20 +186,28 @@ void psx_mutations_carve(psx_mutations* m, const std::shared_ptr<dbuf>& block, s
     m->hi = (uint32_t*)b;
     b += align256(n * 4u);
     m->info = (uint32_t*)b;
+    b += align256(n * 4u);
+    m->d_chr_begin = (uint64_t*)b;
 }
 
 // m->recs must hold m->n records in SID order with sid_head set; fills the
 // SoA arrays, the chromosome ranges and the per-stratum index.
 void psx_mutations_finish(psx_ctx* ctx, psx_mutations* m, const std::vector<uint64_t>& chr_begin) {
-    const uint64_t n = m->n;
     m->chr_begin = chr_begin;
-    m->chr_begin.resize(PSX_MAX_CHROMOSOMES + 1, n);
+    m->chr_begin.resize(PSX_MAX_CHROMOSOMES + 1, m->n);
+    PSX_HIP(hipMemcpyAsync(m->d_chr_begin, m->chr_begin.data(), 8 * (PSX_MAX_CHROMOSOMES + 1),
+                           hipMemcpyHostToDevice, ctx->stream));
+    psx_mutations_finish_device(ctx, m, true);
+}
+
+void psx_mutations_finish_device(psx_ctx* ctx, psx_mutations* m, bool sync) {
+    const uint64_t n = m->n;
     if (n) {
         hipLaunchKernelGGL(k_mut_soa, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, ctx->stream, m->recs, n,
                            m->pos, m->lo, m->hi, m->info);
         PSX_HIP(hipGetLastError());
     }
-    psx_mutations_index(ctx, m, m->chr_len);
+    psx_mutations_index(ctx, m, m->chr_len, sync);
 }
 
 const psx_mut_index* psx_mutations_find_index(psx_mutations* m, const std::vector<uint32_t>& chr_len) {
@@ -209,7 +217,8 @@ const psx_mut_index* psx_mutations_find_index(psx_mutations* m, const std::vecto
     return nullptr;
 }
 
-const psx_mut_index& psx_mutations_index(psx_ctx* ctx, psx_mutations* m, const std::vector<uint32_t>& chr_len) {
+const psx_mut_index& psx_mutations_index(psx_ctx* ctx, psx_mutations* m, const std::vector<uint32_t>& chr_len,
+                                         bool sync) {
     const uint32_t n_chr = (uint32_t)chr_len.size();
     if (n_chr == 0 || n_chr > PSX_MAX_CHROMOSOMES) psx_throw(PSX_EINVAL, "invalid number of chromosomes");
     std::lock_guard<std::mutex> g(m->idx_mu);
@@ -221,7 +230,7 @@ const psx_mut_index& psx_mutations_index(psx_ctx* ctx, psx_mutations* m, const s
     a.pos = m->pos;
     a.n_chr = n_chr;
     ix->bucket_off.assign(n_chr + 1, 0);
-    for (uint32_t c = 0; c <= PSX_MAX_CHROMOSOMES; ++c) a.chr_begin[c] = m->chr_begin[c];
+    a.chr_begin = m->d_chr_begin;
     for (uint32_t c = 0; c < n_chr; ++c)
         ix->bucket_off[c + 1] = ix->bucket_off[c] + (chr_len[c] + PSX_STRATUM - 1) / PSX_STRATUM + 1;
     for (uint32_t c = 0; c <= n_chr; ++c) a.boff[c] = ix->bucket_off[c];
@@ -230,8 +239,9 @@ const psx_mut_index& psx_mutations_index(psx_ctx* ctx, psx_mutations* m, const s
     hipLaunchKernelGGL(k_mut_bucket, dim3((uint32_t)((ix->bucket_off[n_chr] + 255) / 256)), dim3(256), 0,
                        ctx->stream, a);
     PSX_HIP(hipGetLastError());
-    // callers on other streams (serving lanes) read the index without waiting on ctx's stream
-    PSX_HIP(hipStreamSynchronize(ctx->stream));
+    // callers on other streams (serving lanes) read the index without waiting on ctx's stream; a caller
+    // passing sync = false synchronises ctx's stream itself before the table is used
+    if (sync) PSX_HIP(hipStreamSynchronize(ctx->stream));
     m->index.push_back(std::move(ix));
     return *m->index.back();
 }

@@ -58,6 +58,7 @@ extern "C" void psx_ctx_destroy(psx_ctx* ctx) {
         ctx->queue.release();
         ctx->place_nodes.release();
         ctx->place_muts.release();
+        if (ctx->host_io) (void)hipHostFree(ctx->host_io);
         (void)hipEventDestroy(ctx->ev0);
         (void)hipEventDestroy(ctx->ev1);
         (void)hipStreamDestroy(ctx->stream);
@@ -159,6 +160,18 @@ size_t scratch_builder::put(const void* src, size_t bytes) {
     host.resize(off + bytes);
     if (bytes) std::memcpy(host.data() + off, src, bytes);
     return off;
+}
+
+uint8_t* psx_host_io(psx_ctx* ctx, size_t bytes) {
+    if (bytes <= ctx->host_io_bytes) return ctx->host_io;
+    PSX_HIP(hipStreamSynchronize(ctx->stream));
+    if (ctx->host_io) PSX_HIP(hipHostFree(ctx->host_io));
+    ctx->host_io = nullptr;
+    ctx->host_io_bytes = 0;
+    const size_t n = std::max<size_t>(bytes, 64 << 10);
+    PSX_HIP(hipHostMalloc((void**)&ctx->host_io, n, hipHostMallocDefault));
+    ctx->host_io_bytes = n;
+    return ctx->host_io;
 }
 
 void* psx_scratch_upload(psx_ctx* ctx, const scratch_builder& b) {

@@ -83,9 +83,14 @@ struct psx_ctx {
     dbuf place_nodes;   // placement temporaries per node and per mutation (grown, kept for later calls)
     dbuf place_muts;
     uint8_t* noalloc_scratch = nullptr;  // device address of the device's no-allocation scratch
+    uint8_t* host_io = nullptr;          // pinned host memory for small device-to-host readbacks
+    size_t host_io_bytes = 0;
     bool tuned = false;
     psx_launch_config tuned_cfg{};
 };
+
+// ctx's pinned readback area, at least bytes long (grown after a stream synchronisation, never shrunk).
+uint8_t* psx_host_io(psx_ctx* ctx, size_t bytes);
 
 // Device address of the no-allocation entry point's scratch array on
 // `device`, which must be the current device; looked up once per device.
@@ -173,6 +178,7 @@ struct psx_mutations {
     std::shared_ptr<dbuf> store;
     psx_mutation* recs = nullptr;
     uint32_t *pos = nullptr, *lo = nullptr, *hi = nullptr, *info = nullptr;
+    uint64_t* d_chr_begin = nullptr;    // chr_begin on the device
     // every stratum index built so far, one per set of chromosome lengths;
     // kept until the table is freed, since launches in flight may read any
     std::mutex idx_mu;
@@ -232,10 +238,14 @@ void psx_mutations_build(psx_ctx* ctx, psx_mutations* m, std::vector<psx_mutatio
 // m->recs holds m->n records in SID order: builds the SoA arrays and the
 // per-stratum index for the chromosome lengths in m->chr_len.
 void psx_mutations_finish(psx_ctx* ctx, psx_mutations* m, const std::vector<uint64_t>& chr_begin);
+// The same with m->d_chr_begin already written on ctx's stream; with sync = false the caller synchronises ctx's
+// stream before the table is used (and sets the host chr_begin).
+void psx_mutations_finish_device(psx_ctx* ctx, psx_mutations* m, bool sync);
 // The per-stratum index of m for chr_len. The first request for a set of
 // lengths builds it on ctx's stream and synchronises that stream, so launches
 // on any stream may read it at once; it stays valid until m is freed.
-const psx_mut_index& psx_mutations_index(psx_ctx* ctx, psx_mutations* m, const std::vector<uint32_t>& chr_len);
+const psx_mut_index& psx_mutations_index(psx_ctx* ctx, psx_mutations* m, const std::vector<uint32_t>& chr_len,
+                                         bool sync = true);
 // The index of m for chr_len if one was built, else nullptr (allocates nothing).
 const psx_mut_index* psx_mutations_find_index(psx_mutations* m, const std::vector<uint32_t>& chr_len);
 

@@ -2,13 +2,22 @@
 //
 //   k_edge_counts   one thread per forest node: Poisson SNV / indel counts
 //   (scan)          exclusive sum of counts -> mutation ids per node
+//   k_node_expand   the node of every passenger id
 //   k_place         one thread per mutation: signature, SBS class, allele and
 //                   a site drawn uniformly from the genome's context index
 //                   list of the class's trinucleotide (one list read, driver
 //                   loci redrawn), or an indel (ID classes: list or rejection)
-//   k_conflicts     infinite-sites model: after a footprint sort, a mutation
-//                   touching an earlier-created one is redrawn (next attempt)
-//   (sort + gather) SID order, sid_head flags, chromosome ranges
+//   k_conflicts     infinite-sites model, first round: after a sort by padded
+//                   position, a mutation touching an earlier-created one
+//                   loses; the losers are listed and redrawn (next attempt)
+//   k_conflicts_inc later rounds: only the redrawn footprints are checked,
+//                   against the first round's sorted footprints and the
+//                   moved ones (sorted again each round, a few thousand)
+//   (sort, k_sid_runs, k_gather) SID order, sid_head flags, chromosome ranges
+//
+// A call places a batch of forests (one for psx_place_mutations): each phase
+// (node counts; placement and the first check; each later round; SID order)
+// is launched for every job and ends in one synchronisation.
 //
 // Every draw is Philox(seed, counter) with the counter layout of the oracle,
 // so the table is identical to the CPU restatement's.
@@ -175,8 +184,10 @@ struct place_args {
     uint64_t M;
     psx_mutation* recs;
     uint32_t* attempt;
-    const uint32_t* redraw;  // nullptr = every id
+    const uint32_t* redraw;  // ids to redraw (n_redraw of them); nullptr = every id
+    uint32_t n_redraw;
     uint32_t* moved;         // set for every redrawn id
+    const uint32_t* node_of; // node of each passenger
     uint32_t* status;
     // indels from ID signatures (use_id): per-epoch exposure and per-signature class tables (classes without a
     // site in the reference have probability 0), the reference's sites per class and the listed classes' sites
@@ -336,13 +347,7 @@ __device__ uint32_t place_one(const place_args& a, uint64_t id, uint32_t att, ps
         return st;
     }
     i -= a.n_pre_indel;
-    // node of passenger i: last v with node_off[v] <= i
-    uint32_t lo = 0, hi = a.n_nodes;
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (a.node_off[mid] <= i) lo = mid + 1; else hi = mid;
-    }
-    const uint32_t v = lo - 1;
+    const uint32_t v = a.node_of[i];
     const uint32_t q = (uint32_t)(i - a.node_off[v]);
     if (q < a.cnt_snv[v]) st = draw_snv(a, v, q, att, -1, r);
     else if (a.use_id) st = draw_indel_id(a, v, q - a.cnt_snv[v], att, -1, r);
@@ -351,10 +356,11 @@ __device__ uint32_t place_one(const place_args& a, uint64_t id, uint32_t att, ps
     return st;
 }
 
+// One thread per mutation, or per id of the redraw list (next attempt).
 __global__ void __launch_bounds__(256) k_place(const place_args a) {
-    const uint64_t id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= a.M) return;
-    if (a.redraw && !a.redraw[id]) return;
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (a.redraw ? (uint64_t)a.n_redraw : a.M)) return;
+    const uint64_t id = a.redraw ? a.redraw[t] : t;
     uint32_t att = 0;
     if (a.redraw) {
         att = a.attempt[id] + 1;
@@ -407,76 +413,102 @@ __device__ __forceinline__ uint64_t padded_pos(const uint64_t* pstart, const psx
     return pstart[r.chr] + r.pos;
 }
 
-// footprint key: padded position << 4 | deleted length (0 for SNVs and insertions)
-__device__ __forceinline__ uint64_t fp_key(const uint64_t* pstart, const psx_mutation& r) {
-    return (padded_pos(pstart, r) << 4) | (r.kind == PSX_DEL ? (uint64_t)(r.len & 15u) : 0ull);
+// node_of[i] = v for the passengers of node v (ids node_off[v] .. + count)
+__global__ void __launch_bounds__(256) k_node_expand(const uint64_t* __restrict__ node_off,
+                                                     const uint64_t* __restrict__ cnt, uint32_t n_nodes,
+                                                     uint32_t* __restrict__ node_of) {
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n_nodes) return;
+    const uint64_t o = node_off[v], n = cnt[v];
+    for (uint64_t k = 0; k < n; ++k) node_of[o + k] = v;
 }
 
-__device__ __forceinline__ bool fp_touch(uint64_t ka, uint64_t kb) {
-    const uint64_t sa = ka >> 4, sb = kb >> 4;
-    return sb <= sa + (ka & 15u) + 1u && sa <= sb + (kb & 15u) + 1u;
+// Sort entries of the infinite-sites check: key = padded position, value =
+// id | deleted length << 28 (ids are below 2^28).
+constexpr uint32_t ID_BITS = 28, ID_MASK = (1u << ID_BITS) - 1u, NO_ENTRY = 0xFFFFFFFFu;
+
+__device__ __forceinline__ uint32_t fp_val(uint32_t id, const psx_mutation& r) {
+    return id | ((r.kind == PSX_DEL ? (uint32_t)(r.len & 15u) : 0u) << ID_BITS);
+}
+
+// footprints [sa, sa + la] and [sb, sb + lb] touch: gap below 2 bases
+__device__ __forceinline__ bool fp_touch(uint64_t sa, uint32_t la, uint64_t sb, uint32_t lb) {
+    return sb <= sa + la + 1u && sa <= sb + lb + 1u;
 }
 
 __global__ void __launch_bounds__(256) k_footprint(const psx_mutation* __restrict__ recs, uint64_t M,
                                                    const uint64_t* __restrict__ pstart, uint64_t* __restrict__ fkey,
-                                                   uint32_t* __restrict__ ids) {
+                                                   uint32_t* __restrict__ fval) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= M) return;
-    fkey[i] = fp_key(pstart, recs[i]);
-    ids[i] = (uint32_t)i;
+    const psx_mutation r = recs[i];
+    fkey[i] = padded_pos(pstart, r);
+    fval[i] = fp_val((uint32_t)i, r);
 }
 
-__device__ __forceinline__ void mark_loser(uint32_t* lose, uint32_t* n_lose, uint32_t id) {
-    if (atomicExch(&lose[id], 1u) == 0u) atomicAdd(n_lose, 1u);
+// a new loser: its flag, and its id appended to the redraw list
+__device__ __forceinline__ void mark_loser(uint32_t* lose, uint32_t* list, uint32_t* n_lose, uint32_t id) {
+    if (atomicExch(&lose[id], 1u) == 0u) list[atomicAdd(n_lose, 1u)] = id;
 }
 
-// First round of the infinite-sites model over footprints sorted by key:
-// footprints touching (gap < 2) an earlier-created mutation's footprint lose;
-// drivers and germline mutations (ids below n_fixed) never lose. Every id's
-// flag is written (ids is a permutation).
+// First round of the infinite-sites model over footprints sorted by padded
+// position: footprints touching (gap < 2) an earlier-created mutation's
+// footprint lose; drivers and germline mutations (ids below n_fixed) never
+// lose. Every id's flag is written (the entries are a permutation); the
+// losers are appended to the redraw list with one atomic per wave.
 __global__ void __launch_bounds__(256) k_conflicts(const uint64_t* __restrict__ fkey,
-                                                   const uint32_t* __restrict__ ids, uint64_t M, uint64_t n_fixed,
-                                                   uint32_t* __restrict__ lose, uint32_t* __restrict__ n_lose) {
+                                                   const uint32_t* __restrict__ fval, uint64_t M, uint64_t n_fixed,
+                                                   uint32_t* __restrict__ lose, uint32_t* __restrict__ list,
+                                                   uint32_t* __restrict__ n_lose) {
     const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= M) return;
-    const uint32_t idj = ids[j];
+    const uint32_t vj = fval[j], idj = vj & ID_MASK, lj = vj >> ID_BITS;
     bool l = false;
     if (idj >= n_fixed) {
-        const uint64_t kj = fkey[j], sj = kj >> 4, ej = sj + (kj & 15u);
+        const uint64_t sj = fkey[j], ej = sj + lj;
         for (uint64_t i = j; i-- > 0 && !l;) {
-            const uint64_t ki = fkey[i];
-            if ((ki >> 4) + 12u < sj) break;
-            l = ids[i] < idj && fp_touch(ki, kj);
+            const uint64_t si = fkey[i];
+            if (si + 12u < sj) break;
+            const uint32_t vi = fval[i];
+            l = (vi & ID_MASK) < idj && fp_touch(si, vi >> ID_BITS, sj, lj);
         }
         for (uint64_t i = j + 1; i < M && !l; ++i) {
-            const uint64_t ki = fkey[i];
-            if ((ki >> 4) > ej + 1u) break;
-            l = ids[i] < idj && fp_touch(ki, kj);
+            const uint64_t si = fkey[i];
+            if (si > ej + 1u) break;
+            const uint32_t vi = fval[i];
+            l = (vi & ID_MASK) < idj && fp_touch(si, vi >> ID_BITS, sj, lj);
         }
     }
     lose[idj] = l ? 1u : 0u;
-    if (l) atomicAdd(n_lose, 1u);
+    const uint64_t bal = __ballot(l);
+    if (!bal) return;
+    const uint32_t lane = threadIdx.x & 63u, leader = (uint32_t)__builtin_ctzll(bal);
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(n_lose, (uint32_t)__popcll(bal));
+    base = __shfl(base, (int)leader);
+    if (l) list[base + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull))] = idj;
 }
 
 // Mutations redrawn since the first round, with their current footprints:
-// slots [0, cap) of b are sentinels (key all ones, id ~0) until filled.
+// slots [0, cap) of b are sentinels (key all ones, NO_ENTRY) until filled.
 __global__ void __launch_bounds__(256) k_collect_moved(const psx_mutation* __restrict__ recs, uint64_t M,
                                                        const uint64_t* __restrict__ pstart,
                                                        const uint32_t* __restrict__ moved, uint64_t* __restrict__ bkey,
-                                                       uint32_t* __restrict__ bids, uint32_t* __restrict__ nb) {
+                                                       uint32_t* __restrict__ bval, uint32_t* __restrict__ nb) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= M || !moved[i]) return;
+    const psx_mutation r = recs[i];
     const uint32_t slot = atomicAdd(nb, 1u);
-    bkey[slot] = fp_key(pstart, recs[i]);
-    bids[slot] = (uint32_t)i;
+    bkey[slot] = padded_pos(pstart, r);
+    bval[slot] = fp_val((uint32_t)i, r);
 }
 
-__global__ void __launch_bounds__(256) k_fill_sentinel(uint64_t* __restrict__ key, uint32_t* __restrict__ ids,
+__global__ void __launch_bounds__(256) k_fill_sentinel(uint64_t* __restrict__ key, uint32_t* __restrict__ val,
                                                        uint64_t n, uint64_t sentinel) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     key[i] = sentinel;
-    ids[i] = 0xFFFFFFFFu;
+    val[i] = NO_ENTRY;
 }
 
 __device__ __forceinline__ uint64_t lower_bound_u64(const uint64_t* a, uint64_t n, uint64_t x) {
@@ -490,14 +522,15 @@ __device__ __forceinline__ uint64_t lower_bound_u64(const uint64_t* a, uint64_t 
 
 struct inc_args {
     const uint64_t* akey;   // first-round footprints, sorted (entries of moved mutations are stale)
-    const uint32_t* aids;
+    const uint32_t* aval;
     uint64_t M;
     const uint64_t* bkey;   // moved mutations' current footprints, sorted, sentinels last
-    const uint32_t* bids;
+    const uint32_t* bval;
     uint64_t nb;
     const uint32_t* moved;
     const uint32_t* changed;  // redrawn in the last redraw
     uint32_t* lose;
+    uint32_t* list;
     uint32_t* n_lose;
 };
 
@@ -509,28 +542,31 @@ struct inc_args {
 __global__ void __launch_bounds__(256) k_conflicts_inc(const inc_args a) {
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= a.nb) return;
-    const uint32_t idc = a.bids[t];
-    if (idc == 0xFFFFFFFFu || !a.changed[idc]) return;
-    const uint64_t kc = a.bkey[t], sc = kc >> 4, ec = sc + (kc & 15u);
-    const uint64_t from = sc >= 11u ? (sc - 11u) << 4 : 0u;
+    const uint32_t vc = a.bval[t];
+    if (vc == NO_ENTRY) return;
+    const uint32_t idc = vc & ID_MASK, lc = vc >> ID_BITS;
+    if (!a.changed[idc]) return;
+    const uint64_t sc = a.bkey[t], ec = sc + lc;
+    const uint64_t from = sc >= 11u ? sc - 11u : 0u;
     bool l = false;
     for (uint64_t i = lower_bound_u64(a.akey, a.M, from); i < a.M; ++i) {
-        const uint64_t ki = a.akey[i];
-        if ((ki >> 4) > ec + 1u) break;
-        const uint32_t idi = a.aids[i];
-        if (a.moved[idi] || !fp_touch(ki, kc)) continue;
+        const uint64_t si = a.akey[i];
+        if (si > ec + 1u) break;
+        const uint32_t vi = a.aval[i], idi = vi & ID_MASK;
+        if (a.moved[idi] || !fp_touch(si, vi >> ID_BITS, sc, lc)) continue;
         if (idi < idc) l = true;
-        else mark_loser(a.lose, a.n_lose, idi);
+        else mark_loser(a.lose, a.list, a.n_lose, idi);
     }
     for (uint64_t i = lower_bound_u64(a.bkey, a.nb, from); i < a.nb; ++i) {
-        const uint64_t ki = a.bkey[i];
-        const uint32_t idi = a.bids[i];
-        if (idi == 0xFFFFFFFFu || (ki >> 4) > ec + 1u) break;
-        if (idi == idc || !fp_touch(ki, kc)) continue;
+        const uint64_t si = a.bkey[i];
+        const uint32_t vi = a.bval[i];
+        if (vi == NO_ENTRY || si > ec + 1u) break;
+        const uint32_t idi = vi & ID_MASK;
+        if (idi == idc || !fp_touch(si, vi >> ID_BITS, sc, lc)) continue;
         if (idi < idc) l = true;
-        else mark_loser(a.lose, a.n_lose, idi);
+        else mark_loser(a.lose, a.list, a.n_lose, idi);
     }
-    if (l) mark_loser(a.lose, a.n_lose, idc);
+    if (l) mark_loser(a.lose, a.list, a.n_lose, idc);
 }
 
 __global__ void __launch_bounds__(256) k_pos_key(const psx_mutation* __restrict__ recs, uint64_t M,
@@ -709,427 +745,625 @@ const psx_id_index& psx_genome_id_index(psx_ctx* ctx, psx_genome* g) {
     return *g->id_index;
 }
 
-extern "C" psx_status psx_place_mutations(psx_ctx* ctx, const psx_genome* g, const psx_forest* f,
-                                          const psx_engine_params* p, uint64_t seed, psx_mutations** out) {
-    if (!ctx || !g || !f || !p || !out) return PSX_EINVAL;
-    psx_mutations* mt = new psx_mutations();
-    mt->ctx = ctx;
-    psx_status st = psx_guard(ctx, [&] {
-        PSX_HIP(hipSetDevice(ctx->device));
-        hipStream_t s = ctx->stream;
-        if (p->ploidy < 1 || p->ploidy > 255) psx_throw(PSX_EINVAL, "invalid ploidy");
-        if (p->indel_max_len < 1 || p->indel_max_len > 10) psx_throw(PSX_EINVAL, "indel_max_len must be in [1, 10]");
-        if (!(p->indel_len_p > 0.0 && p->indel_len_p <= 1.0)) psx_throw(PSX_EINVAL, "invalid indel length parameter");
-        for (uint32_t v = 0; v < f->n_nodes; ++v)
-            if (f->species[v] >= p->n_species) psx_throw(PSX_EINVAL, "node species %u has no rates", f->species[v]);
-        bool need_snv = p->n_preneoplastic_snv > 0;
-        for (uint32_t sp = 0; sp < p->n_species; ++sp) need_snv |= p->snv_rate[sp] > 0.0;
-        if (need_snv && (p->n_sbs == 0 || p->n_epochs == 0))
-            psx_throw(PSX_EINVAL, "SNVs need a signature and an exposure");
-        if (p->n_preneoplastic_snv > 0 && p->preneoplastic_sbs >= p->n_sbs)
-            psx_throw(PSX_EINVAL, "unknown preneoplastic signature");
-        // indels from ID signatures: the genome's indel index (built once per genome) and the class tables
-        bool use_id = false;
-        const psx_id_index* idx = nullptr;
-        if (p->n_id) {
-            bool need_id = p->n_preneoplastic_indel > 0;
-            for (uint32_t sp = 0; sp < p->n_species; ++sp) need_id |= p->indel_rate[sp] > 0.0;
-            if (!p->id || (need_id && (!p->id_exposure || p->n_epochs == 0)))
-                psx_throw(PSX_EINVAL, "indels need an ID signature and an exposure");
-            if (p->n_preneoplastic_indel > 0 && p->preneoplastic_id >= p->n_id)
-                psx_throw(PSX_EINVAL, "unknown preneoplastic ID signature");
-            if (need_id) {
-                use_id = true;
-                idx = &psx_genome_id_index(ctx, const_cast<psx_genome*>(g));
-            }
+namespace {
+
+// One forest's placement in a batch: host preparation, then the device phases
+// that every job of the batch goes through together, one synchronisation per
+// phase (node counts; placement and the first infinite-sites check; each
+// later round; SID order).
+struct place_job {
+    uint32_t k = 0;  // index in the batch
+    const psx_forest* f = nullptr;
+    const psx_engine_params* p = nullptr;
+    uint64_t seed = 0;
+    psx_mutations* mt = nullptr;
+    // host preparation
+    std::vector<uint64_t> T;
+    size_t expo_off = 0, cls_off = 0, len_off = 0, id_expo_off = 0, id_cls_off = 0;
+    std::vector<uint32_t> sk0, soff, sn, ik0, ioff, in_, epoch, drv_d, drv_v;
+    std::vector<uint64_t> drv_pos;
+    bool use_id = false;
+    const psx_id_index* idx = nullptr;
+    const psx_sbs_index* sidx = nullptr;
+    std::vector<uint64_t> check_pos;  // global positions of the drivers, then the germline mutations
+    // scratch offsets
+    size_t o_T = 0, o_sk0 = 0, o_soff = 0, o_sn = 0, o_ik0 = 0, o_ioff = 0, o_in = 0, o_epoch = 0, o_drvd = 0,
+           o_drvv = 0, o_drivers = 0, o_germ = 0, o_io = 0, o_idc = 0, o_idl = 0, o_ido = 0, o_drvpos = 0;
+    // sizes
+    uint64_t n_pass = 0, n_fixed = 0, M = 0;
+    // device state
+    place_args pa{};
+    uint32_t* d_cnt_snv = nullptr;
+    uint64_t *d_cnt_tot = nullptr, *d_off = nullptr;
+    uint64_t *d_key = nullptr, *d_key2 = nullptr, *d_key3 = nullptr, *d_cb = nullptr;
+    uint32_t *d_ids = nullptr, *d_ids2 = nullptr, *d_ids3 = nullptr, *d_moved = nullptr, *d_list = nullptr;
+    uint32_t *d_node_of = nullptr, *d_att = nullptr;
+    uint32_t* d_lose[2] = {nullptr, nullptr};
+    uint32_t* d_io = nullptr;  // [0] status of the last placement launch, [1] losers of the last check
+    uint8_t* d_head = nullptr;
+    // infinite-sites rounds
+    uint32_t n_lose = 0;
+    uint64_t moved_cap = 0;
+    int cur = 0;
+};
+
+[[noreturn]] void job_throw(const place_job& J, uint32_t n_jobs, psx_status st, const std::string& msg) {
+    if (n_jobs > 1) psx_throw(st, "job %u: %s", J.k, msg.c_str());
+    psx_throw(st, "%s", msg.c_str());
+}
+
+// Validation and the host-side tables of one job (the copy-number events go
+// straight into its table); the genome's context and indel indexes are built
+// here on first use.
+void prepare_job(psx_ctx* ctx, const psx_genome* g, place_job& J) {
+    const psx_engine_params* p = J.p;
+    const psx_forest* f = J.f;
+    const uint64_t seed = J.seed;
+    if (p->ploidy < 1 || p->ploidy > 255) psx_throw(PSX_EINVAL, "invalid ploidy");
+    if (p->indel_max_len < 1 || p->indel_max_len > 10) psx_throw(PSX_EINVAL, "indel_max_len must be in [1, 10]");
+    if (!(p->indel_len_p > 0.0 && p->indel_len_p <= 1.0)) psx_throw(PSX_EINVAL, "invalid indel length parameter");
+    for (uint32_t v = 0; v < f->n_nodes; ++v)
+        if (f->species[v] >= p->n_species) psx_throw(PSX_EINVAL, "node species %u has no rates", f->species[v]);
+    bool need_snv = p->n_preneoplastic_snv > 0;
+    for (uint32_t sp = 0; sp < p->n_species; ++sp) need_snv |= p->snv_rate[sp] > 0.0;
+    if (need_snv && (p->n_sbs == 0 || p->n_epochs == 0)) psx_throw(PSX_EINVAL, "SNVs need a signature and an exposure");
+    if (p->n_preneoplastic_snv > 0 && p->preneoplastic_sbs >= p->n_sbs)
+        psx_throw(PSX_EINVAL, "unknown preneoplastic signature");
+    // indels from ID signatures: the genome's indel index (built once per genome) and the class tables
+    if (p->n_id) {
+        bool need_id = p->n_preneoplastic_indel > 0;
+        for (uint32_t sp = 0; sp < p->n_species; ++sp) need_id |= p->indel_rate[sp] > 0.0;
+        if (!p->id || (need_id && (!p->id_exposure || p->n_epochs == 0)))
+            psx_throw(PSX_EINVAL, "indels need an ID signature and an exposure");
+        if (p->n_preneoplastic_indel > 0 && p->preneoplastic_id >= p->n_id)
+            psx_throw(PSX_EINVAL, "unknown preneoplastic ID signature");
+        if (need_id) {
+            J.use_id = true;
+            J.idx = &psx_genome_id_index(ctx, const_cast<psx_genome*>(g));
         }
-        const uint64_t G = g->boff[g->n_chr];
-        // ---- integer tables (host, from the double pmfs) -> one u64 array
-        std::vector<uint64_t> T;
-        auto append = [&](const uint64_t* src, size_t n) { size_t off = T.size(); T.insert(T.end(), src, src + n); return off; };
-        std::vector<uint64_t> tmp;
-        const size_t expo_off = T.size();
+    }
+    const uint64_t G = g->boff[g->n_chr];
+    // ---- integer tables (host, from the double pmfs) -> one u64 array
+    std::vector<uint64_t>& T = J.T;
+    auto append = [&](const uint64_t* src, size_t n) {
+        const size_t off = T.size();
+        T.insert(T.end(), src, src + n);
+        return off;
+    };
+    std::vector<uint64_t> tmp;
+    J.expo_off = T.size();
+    for (uint32_t e = 0; e < p->n_epochs; ++e) {
+        tmp.assign(p->n_sbs, 0);
+        psx_cdf_table(p->exposure + (size_t)e * p->n_sbs, p->n_sbs, tmp.data());
+        append(tmp.data(), tmp.size());
+    }
+    // SBS classes: the genome's context index at the engine's sampling (built once per genome and sampling),
+    // every class whose context has no indexed site given probability 0
+    J.sidx = need_snv ? &psx_genome_sbs_index(ctx, const_cast<psx_genome*>(g), p->context_sampling) : nullptr;
+    J.cls_off = T.size();
+    for (uint32_t sg = 0; sg < p->n_sbs && J.sidx; ++sg) {
+        double pmf[96];
+        double mass = 0.0;
+        for (uint32_t k = 0; k < 96; ++k) {
+            pmf[k] = J.sidx->kept[(k >= 48 ? 16u : 0u) | (k & 15u)] ? p->sbs[(size_t)sg * 96 + k] : 0.0;
+            mass += pmf[k];
+        }
+        if (!(mass > 0.0)) psx_throw(PSX_EINVAL, "SBS signature %u has no class with a site in the context index", sg);
+        tmp.assign(96, 0);
+        psx_cdf_table(pmf, 96, tmp.data());
+        append(tmp.data(), tmp.size());
+    }
+    std::vector<double> lp(p->indel_max_len);
+    for (uint32_t l = 1; l <= p->indel_max_len; ++l)
+        lp[l - 1] = std::pow(1.0 - p->indel_len_p, (double)(l - 1)) * p->indel_len_p;
+    tmp.assign(p->indel_max_len, 0);
+    psx_cdf_table(lp.data(), p->indel_max_len, tmp.data());
+    J.len_off = append(tmp.data(), tmp.size());
+    if (J.use_id) {
+        J.id_expo_off = T.size();
         for (uint32_t e = 0; e < p->n_epochs; ++e) {
-            tmp.assign(p->n_sbs, 0);
-            psx_cdf_table(p->exposure + (size_t)e * p->n_sbs, p->n_sbs, tmp.data());
+            tmp.assign(p->n_id, 0);
+            psx_cdf_table(p->id_exposure + (size_t)e * p->n_id, p->n_id, tmp.data());
             append(tmp.data(), tmp.size());
         }
-        // SBS classes: the genome's context index at the engine's sampling (built once per genome and sampling),
-        // every class whose context has no indexed site given probability 0
-        const psx_sbs_index* sidx =
-            need_snv ? &psx_genome_sbs_index(ctx, const_cast<psx_genome*>(g), p->context_sampling) : nullptr;
-        const size_t cls_off = T.size();
-        for (uint32_t sg = 0; sg < p->n_sbs && sidx; ++sg) {
-            double pmf[96];
+        J.id_cls_off = T.size();
+        for (uint32_t sg = 0; sg < p->n_id; ++sg) {
+            double pmf[PSX_ID_CLASSES];
             double mass = 0.0;
-            for (uint32_t k = 0; k < 96; ++k) {
-                pmf[k] = sidx->kept[(k >= 48 ? 16u : 0u) | (k & 15u)] ? p->sbs[(size_t)sg * 96 + k] : 0.0;
+            for (uint32_t k = 0; k < PSX_ID_CLASSES; ++k) {
+                pmf[k] = J.idx->count[k] ? p->id[(size_t)sg * PSX_ID_CLASSES + k] : 0.0;
                 mass += pmf[k];
             }
-            if (!(mass > 0.0)) psx_throw(PSX_EINVAL, "SBS signature %u has no class with a site in the context index", sg);
-            tmp.assign(96, 0);
-            psx_cdf_table(pmf, 96, tmp.data());
+            if (!(mass > 0.0)) psx_throw(PSX_EINVAL, "ID signature %u has no class with a site in the reference", sg);
+            tmp.assign(PSX_ID_CLASSES, 0);
+            psx_cdf_table(pmf, PSX_ID_CLASSES, tmp.data());
             append(tmp.data(), tmp.size());
         }
-        std::vector<double> lp(p->indel_max_len);
-        for (uint32_t l = 1; l <= p->indel_max_len; ++l)
-            lp[l - 1] = std::pow(1.0 - p->indel_len_p, (double)(l - 1)) * p->indel_len_p;
-        tmp.assign(p->indel_max_len, 0);
-        psx_cdf_table(lp.data(), p->indel_max_len, tmp.data());
-        const size_t len_off = append(tmp.data(), tmp.size());
-        size_t id_expo_off = 0, id_cls_off = 0;
-        if (use_id) {
-            id_expo_off = T.size();
-            for (uint32_t e = 0; e < p->n_epochs; ++e) {
-                tmp.assign(p->n_id, 0);
-                psx_cdf_table(p->id_exposure + (size_t)e * p->n_id, p->n_id, tmp.data());
-                append(tmp.data(), tmp.size());
-            }
-            id_cls_off = T.size();
-            for (uint32_t sg = 0; sg < p->n_id; ++sg) {
-                double pmf[PSX_ID_CLASSES];
-                double mass = 0.0;
-                for (uint32_t k = 0; k < PSX_ID_CLASSES; ++k) {
-                    pmf[k] = idx->count[k] ? p->id[(size_t)sg * PSX_ID_CLASSES + k] : 0.0;
-                    mass += pmf[k];
-                }
-                if (!(mass > 0.0)) psx_throw(PSX_EINVAL, "ID signature %u has no class with a site in the reference", sg);
-                tmp.assign(PSX_ID_CLASSES, 0);
-                psx_cdf_table(pmf, PSX_ID_CLASSES, tmp.data());
-                append(tmp.data(), tmp.size());
+    }
+    const double allelic = (double)((uint64_t)p->ploidy * G);
+    J.sk0.resize(p->n_species); J.soff.resize(p->n_species); J.sn.resize(p->n_species);
+    J.ik0.resize(p->n_species); J.ioff.resize(p->n_species); J.in_.resize(p->n_species);
+    for (uint32_t sp = 0; sp < p->n_species; ++sp) {
+        psx_itable a = psx_poisson_table(p->snv_rate[sp] * allelic);
+        J.sk0[sp] = a.k0; J.sn[sp] = (uint32_t)a.T.size(); J.soff[sp] = (uint32_t)append(a.T.data(), a.T.size());
+        psx_itable b = psx_poisson_table(p->indel_rate[sp] * allelic);
+        J.ik0[sp] = b.k0; J.in_[sp] = (uint32_t)b.T.size(); J.ioff[sp] = (uint32_t)append(b.T.data(), b.T.size());
+    }
+    // ---- drivers and germline (their reference bases are checked after the first device phase), node epochs
+    for (uint32_t d = 0; d < p->n_drivers; ++d) {
+        const psx_driver& dr = p->drivers[d];
+        if (dr.chr >= g->n_chr || dr.pos >= g->len[dr.chr] || dr.alt > 3 ||
+            (dr.allele >= 0 && (uint32_t)dr.allele >= p->ploidy))
+            psx_throw(PSX_EINVAL, "invalid driver %u", d);
+        J.check_pos.push_back(g->boff[dr.chr] + dr.pos);
+        for (uint32_t v = 0; v < f->n_nodes; ++v) {
+            const uint32_t par = f->parent[v];
+            if (f->species[v] == dr.species && (par == PSX_NO_NODE || f->species[par] != dr.species)) {
+                J.drv_d.push_back(d);
+                J.drv_v.push_back(v);
             }
         }
-        const double allelic = (double)((uint64_t)p->ploidy * G);
-        std::vector<uint32_t> sk0(p->n_species), soff(p->n_species), sn(p->n_species);
-        std::vector<uint32_t> ik0(p->n_species), ioff(p->n_species), in_(p->n_species);
-        for (uint32_t sp = 0; sp < p->n_species; ++sp) {
-            psx_itable a = psx_poisson_table(p->snv_rate[sp] * allelic);
-            sk0[sp] = a.k0; sn[sp] = (uint32_t)a.T.size(); soff[sp] = (uint32_t)append(a.T.data(), a.T.size());
-            psx_itable b = psx_poisson_table(p->indel_rate[sp] * allelic);
-            ik0[sp] = b.k0; in_[sp] = (uint32_t)b.T.size(); ioff[sp] = (uint32_t)append(b.T.data(), b.T.size());
+    }
+    // driver loci: no SNV site is drawn there (src/mutation_engine.cpp:244-254)
+    for (uint32_t d = 0; d < p->n_drivers; ++d) J.drv_pos.push_back(g->boff[p->drivers[d].chr] + p->drivers[d].pos);
+    std::sort(J.drv_pos.begin(), J.drv_pos.end());
+    J.drv_pos.erase(std::unique(J.drv_pos.begin(), J.drv_pos.end()), J.drv_pos.end());
+    for (uint32_t k = 0; k < p->n_germline; ++k) {
+        const psx_germline& gm = p->germline[k];
+        if (gm.chr >= g->n_chr || gm.pos >= g->len[gm.chr] || gm.alt > 3 || gm.allele >= p->ploidy)
+            psx_throw(PSX_EINVAL, "invalid germline mutation %u", k);
+        J.check_pos.push_back(g->boff[gm.chr] + gm.pos);
+    }
+    // copy-number drivers: on the edge into every node where the species first appears
+    std::vector<psx_cna_event> cna;
+    for (uint32_t k = 0; k < p->n_cna; ++k) {
+        const psx_cna& c = p->cna[k];
+        const bool wgd = c.kind == PSX_CNA_WGD;
+        if (c.kind > PSX_CNA_WGD || c.species >= p->n_species ||
+            (!wgd && (c.chr >= g->n_chr || c.len == 0 || c.start >= g->len[c.chr] ||
+                      c.len > g->len[c.chr] - c.start ||
+                      (c.allele >= p->ploidy && c.allele != PSX_CNA_RANDOM_ALLELE) ||
+                      (c.kind == PSX_CNA_AMP && c.copies == 0))))
+            psx_throw(PSX_EINVAL, "invalid copy-number driver %u", k);
+        for (uint32_t v = 0; v < f->n_nodes; ++v) {
+            const uint32_t par = f->parent[v];
+            if (f->species[v] != c.species || (par != PSX_NO_NODE && f->species[par] == c.species)) continue;
+            psx_cna_event e{};
+            e.node = v;
+            e.leaf_lo = f->leaf_lo[v];
+            e.leaf_hi = f->leaf_hi[v];
+            e.chr = wgd ? PSX_ALL_CHROMOSOMES : c.chr;
+            e.start = wgd ? 0u : c.start;
+            e.end = wgd ? 0u : c.start + c.len;
+            e.allele = wgd ? 0 : c.allele;
+            if (!wgd && c.allele == PSX_CNA_RANDOM_ALLELE)  // drawn per node: (driver, node, 1, DRIVER)
+                e.allele = (uint8_t)psx_uniform32(psx_draw(seed, k, v, 1u, PSX_P_DRIVER << 24).x, p->ploidy);
+            e.kind = c.kind;
+            e.copies = c.kind == PSX_CNA_AMP ? c.copies : 0;
+            cna.push_back(e);
         }
-        // ---- drivers, germline, node epochs (host)
-        auto base_at = [&](uint32_t c, uint32_t i) -> uint32_t {
-            uint32_t w = 0;
-            PSX_HIP(hipMemcpy(&w, g->packed.as<uint32_t>() + g->woff[c] + i / 16, 4, hipMemcpyDeviceToHost));
-            return (w >> (2 * (i % 16))) & 3u;
-        };
-        std::vector<uint32_t> drv_d, drv_v;
-        for (uint32_t d = 0; d < p->n_drivers; ++d) {
-            const psx_driver& dr = p->drivers[d];
-            if (dr.chr >= g->n_chr || dr.pos >= g->len[dr.chr] || dr.alt > 3 ||
-                (dr.allele >= 0 && (uint32_t)dr.allele >= p->ploidy) || dr.alt == base_at(dr.chr, dr.pos))
-                psx_throw(PSX_EINVAL, "invalid driver %u", d);
-            for (uint32_t v = 0; v < f->n_nodes; ++v) {
-                const uint32_t par = f->parent[v];
-                if (f->species[v] == dr.species && (par == PSX_NO_NODE || f->species[par] != dr.species)) {
-                    drv_d.push_back(d);
-                    drv_v.push_back(v);
-                }
+    }
+    // passenger CNAs (src/mutation_engine.cpp:441-487, 901-927): on the edge into every non-root node,
+    // Poisson(cna_rate(species) x ploidy x G) events counted from word z of the node's edge draw (words x
+    // and y count its SNVs and indels), each a uniform catalogue region on a uniform allele
+    bool any_pcna = false;
+    for (uint32_t sp = 0; sp < p->n_species && p->cna_rate; ++sp) {
+        if (!(p->cna_rate[sp] >= 0.0) || !std::isfinite(p->cna_rate[sp]))
+            psx_throw(PSX_EINVAL, "CNA rates must be non-negative");
+        any_pcna |= p->cna_rate[sp] > 0.0;
+    }
+    if (any_pcna) {
+        if (p->n_cna_catalogue == 0 || !p->cna_catalogue) psx_throw(PSX_EINVAL, "passenger CNAs need a CNA catalogue");
+        for (uint32_t k = 0; k < p->n_cna_catalogue; ++k) {
+            const psx_cna_region& r = p->cna_catalogue[k];
+            if (r.chr >= g->n_chr || r.len == 0 || r.start >= g->len[r.chr] || r.len > g->len[r.chr] - r.start ||
+                (r.kind != PSX_CNA_AMP && r.kind != PSX_CNA_DEL))
+                psx_throw(PSX_EINVAL, "invalid passenger CNA catalogue region %u", k);
+        }
+        std::vector<psx_itable> ct(p->n_species);
+        for (uint32_t sp = 0; sp < p->n_species; ++sp) ct[sp] = psx_poisson_table(p->cna_rate[sp] * allelic);
+        for (uint32_t v = 0; v < f->n_nodes; ++v) {
+            if (f->parent[v] == PSX_NO_NODE) continue;
+            const psx_itable& t = ct[f->species[v]];
+            if (t.T.size() == 1 && t.k0 == 0) continue;  // rate 0
+            const psx_u4 w = psx_draw(seed, v, 0u, 0u, PSX_P_EDGE << 24);
+            const uint32_t n_ev = t.k0 + psx_cdf_pick(t.T.data(), (uint32_t)t.T.size(), w.z);
+            for (uint32_t k = 0; k < n_ev; ++k) {
+                const psx_u4 e = psx_draw(seed, k, v, 0u, PSX_P_PCNA << 24);
+                const psx_cna_region& r = p->cna_catalogue[psx_uniform32(e.x, p->n_cna_catalogue)];
+                psx_cna_event ev{};
+                ev.node = v;
+                ev.leaf_lo = f->leaf_lo[v];
+                ev.leaf_hi = f->leaf_hi[v];
+                ev.chr = r.chr;
+                ev.start = r.start;
+                ev.end = r.start + r.len;
+                ev.allele = (uint8_t)psx_uniform32(e.y, p->ploidy);
+                ev.kind = r.kind;
+                ev.copies = r.kind == PSX_CNA_AMP ? 1 : 0;
+                cna.push_back(ev);
             }
         }
-        // driver loci: no SNV site is drawn there (src/mutation_engine.cpp:244-254)
-        std::vector<uint64_t> drv_pos;
-        for (uint32_t d = 0; d < p->n_drivers; ++d) drv_pos.push_back(g->boff[p->drivers[d].chr] + p->drivers[d].pos);
-        std::sort(drv_pos.begin(), drv_pos.end());
-        drv_pos.erase(std::unique(drv_pos.begin(), drv_pos.end()), drv_pos.end());
-        for (uint32_t k = 0; k < p->n_germline; ++k) {
-            const psx_germline& gm = p->germline[k];
-            if (gm.chr >= g->n_chr || gm.pos >= g->len[gm.chr] || gm.alt > 3 || gm.allele >= p->ploidy ||
-                gm.alt == base_at(gm.chr, gm.pos))
-                psx_throw(PSX_EINVAL, "invalid germline mutation %u", k);
-        }
-        // copy-number drivers: on the edge into every node where the species first appears
-        std::vector<psx_cna_event> cna;
-        for (uint32_t k = 0; k < p->n_cna; ++k) {
-            const psx_cna& c = p->cna[k];
-            const bool wgd = c.kind == PSX_CNA_WGD;
-            if (c.kind > PSX_CNA_WGD || c.species >= p->n_species ||
-                (!wgd && (c.chr >= g->n_chr || c.len == 0 || c.start >= g->len[c.chr] ||
-                          c.len > g->len[c.chr] - c.start ||
-                          (c.allele >= p->ploidy && c.allele != PSX_CNA_RANDOM_ALLELE) ||
-                          (c.kind == PSX_CNA_AMP && c.copies == 0))))
-                psx_throw(PSX_EINVAL, "invalid copy-number driver %u", k);
-            for (uint32_t v = 0; v < f->n_nodes; ++v) {
-                const uint32_t par = f->parent[v];
-                if (f->species[v] != c.species || (par != PSX_NO_NODE && f->species[par] == c.species)) continue;
-                psx_cna_event e{};
-                e.node = v;
-                e.leaf_lo = f->leaf_lo[v];
-                e.leaf_hi = f->leaf_hi[v];
-                e.chr = wgd ? PSX_ALL_CHROMOSOMES : c.chr;
-                e.start = wgd ? 0u : c.start;
-                e.end = wgd ? 0u : c.start + c.len;
-                e.allele = wgd ? 0 : c.allele;
-                if (!wgd && c.allele == PSX_CNA_RANDOM_ALLELE)  // drawn per node: (driver, node, 1, DRIVER)
-                    e.allele = (uint8_t)psx_uniform32(psx_draw(seed, k, v, 1u, PSX_P_DRIVER << 24).x, p->ploidy);
-                e.kind = c.kind;
-                e.copies = c.kind == PSX_CNA_AMP ? c.copies : 0;
-                cna.push_back(e);
-            }
-        }
-        // passenger CNAs (src/mutation_engine.cpp:441-487, 901-927): on the edge into every non-root node,
-        // Poisson(cna_rate(species) x ploidy x G) events counted from word z of the node's edge draw (words x
-        // and y count its SNVs and indels), each a uniform catalogue region on a uniform allele
-        bool any_pcna = false;
-        for (uint32_t sp = 0; sp < p->n_species && p->cna_rate; ++sp) {
-            if (!(p->cna_rate[sp] >= 0.0) || !std::isfinite(p->cna_rate[sp]))
-                psx_throw(PSX_EINVAL, "CNA rates must be non-negative");
-            any_pcna |= p->cna_rate[sp] > 0.0;
-        }
-        if (any_pcna) {
-            if (p->n_cna_catalogue == 0 || !p->cna_catalogue)
-                psx_throw(PSX_EINVAL, "passenger CNAs need a CNA catalogue");
-            for (uint32_t k = 0; k < p->n_cna_catalogue; ++k) {
-                const psx_cna_region& r = p->cna_catalogue[k];
-                if (r.chr >= g->n_chr || r.len == 0 || r.start >= g->len[r.chr] || r.len > g->len[r.chr] - r.start ||
-                    (r.kind != PSX_CNA_AMP && r.kind != PSX_CNA_DEL))
-                    psx_throw(PSX_EINVAL, "invalid passenger CNA catalogue region %u", k);
-            }
-            std::vector<psx_itable> ct(p->n_species);
-            for (uint32_t sp = 0; sp < p->n_species; ++sp) ct[sp] = psx_poisson_table(p->cna_rate[sp] * allelic);
-            for (uint32_t v = 0; v < f->n_nodes; ++v) {
-                if (f->parent[v] == PSX_NO_NODE) continue;
-                const psx_itable& t = ct[f->species[v]];
-                if (t.T.size() == 1 && t.k0 == 0) continue;  // rate 0
-                const psx_u4 w = psx_draw(seed, v, 0u, 0u, PSX_P_EDGE << 24);
-                const uint32_t n_ev = t.k0 + psx_cdf_pick(t.T.data(), (uint32_t)t.T.size(), w.z);
-                for (uint32_t k = 0; k < n_ev; ++k) {
-                    const psx_u4 e = psx_draw(seed, k, v, 0u, PSX_P_PCNA << 24);
-                    const psx_cna_region& r = p->cna_catalogue[psx_uniform32(e.x, p->n_cna_catalogue)];
-                    psx_cna_event ev{};
-                    ev.node = v;
-                    ev.leaf_lo = f->leaf_lo[v];
-                    ev.leaf_hi = f->leaf_hi[v];
-                    ev.chr = r.chr;
-                    ev.start = r.start;
-                    ev.end = r.start + r.len;
-                    ev.allele = (uint8_t)psx_uniform32(e.y, p->ploidy);
-                    ev.kind = r.kind;
-                    ev.copies = r.kind == PSX_CNA_AMP ? 1 : 0;
-                    cna.push_back(ev);
-                }
-            }
-        }
-        // DFS order of the nodes; at one node the drivers in their order, then the passengers
-        std::stable_sort(cna.begin(), cna.end(),
-                         [](const psx_cna_event& a, const psx_cna_event& b) { return a.node < b.node; });
-        psx_cna_check(cna, f->n_leaves, p->ploidy, g->len);
-        mt->cna = std::move(cna);
-        std::vector<uint32_t> epoch(f->n_nodes, 0);
-        for (uint32_t v = 0; v < f->n_nodes; ++v)
-            for (uint32_t e = 0; e < p->n_epochs; ++e)
-                if (p->epoch_start[e] <= f->birth[v]) epoch[v] = e;
-        // ---- descriptors into the scratch arena
-        scratch_builder sb;
-        const size_t o_T = sb.put_vec(T);
-        const size_t o_len = sb.put_vec(g->len);
-        const size_t o_woff = sb.put_vec(g->woff);
-        const size_t o_boff = sb.put_vec(g->boff);
-        const size_t o_sk0 = sb.put_vec(sk0), o_soff = sb.put_vec(soff), o_sn = sb.put_vec(sn);
-        const size_t o_ik0 = sb.put_vec(ik0), o_ioff = sb.put_vec(ioff), o_in = sb.put_vec(in_);
-        const size_t o_epoch = sb.put_vec(epoch);
-        const size_t o_drvd = sb.put_vec(drv_d), o_drvv = sb.put_vec(drv_v);
-        const size_t o_drivers = sb.put(p->drivers, sizeof(psx_driver) * p->n_drivers);
-        const size_t o_germ = sb.put(p->germline, sizeof(psx_germline) * p->n_germline);
-        const uint32_t zero = 0;
-        const size_t o_status = sb.put(&zero, 4);
-        size_t o_idc = 0, o_idl = 0, o_ido = 0;
-        if (use_id) {
-            o_idc = sb.put(idx->count, sizeof idx->count);
-            o_idl = sb.put(idx->listed, sizeof idx->listed);
-            o_ido = sb.put(idx->off, sizeof idx->off);
-        }
-        const size_t o_nlose = sb.put(&zero, 4);
-        const size_t o_drvpos = sb.put_vec(drv_pos);
-        std::vector<uint64_t> pstart(g->n_chr);  // padded chromosome starts: boff[c] + 16 c
-        for (uint32_t c = 0; c < g->n_chr; ++c) pstart[c] = g->boff[c] + 16ull * c;
-        const size_t o_pstart = sb.put_vec(pstart);
-        uint8_t* base = (uint8_t*)psx_scratch_upload(ctx, sb);
-        auto at = [&](size_t off) { return (void*)(base + off); };
+    }
+    // DFS order of the nodes; at one node the drivers in their order, then the passengers
+    std::stable_sort(cna.begin(), cna.end(),
+                     [](const psx_cna_event& a, const psx_cna_event& b) { return a.node < b.node; });
+    psx_cna_check(cna, f->n_leaves, p->ploidy, g->len);
+    J.mt->cna = std::move(cna);
+    J.epoch.assign(f->n_nodes, 0);
+    for (uint32_t v = 0; v < f->n_nodes; ++v)
+        for (uint32_t e = 0; e < p->n_epochs; ++e)
+            if (p->epoch_start[e] <= f->birth[v]) J.epoch[v] = e;
+}
 
-        PSX_HIP(hipEventRecord(ctx->ev0, s));
-        const uint32_t n = f->n_nodes;
-        // per-node temporaries from the context's placement pool (grown, never shrunk)
-        size_t scan_bytes = 0;
-        PSX_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, (uint64_t*)nullptr, (uint64_t*)nullptr, (int)n,
-                                                 s));
-        carver cn;
-        const size_t c_cnt_snv = cn.take((size_t)n * 4), c_cnt_tot = cn.take((size_t)n * 8);
-        const size_t c_off = cn.take((size_t)n * 8), c_scan = cn.take(scan_bytes);
-        ctx->place_nodes.reserve(cn.bytes);
-        uint8_t* pn = (uint8_t*)ctx->place_nodes.p;
-        uint32_t* d_cnt_snv = (uint32_t*)(pn + c_cnt_snv);
-        uint64_t* d_cnt_tot = (uint64_t*)(pn + c_cnt_tot);
-        uint64_t* d_off = (uint64_t*)(pn + c_off);
+void put_job(scratch_builder& sb, place_job& J) {
+    const psx_engine_params* p = J.p;
+    J.o_T = sb.put_vec(J.T);
+    J.o_sk0 = sb.put_vec(J.sk0); J.o_soff = sb.put_vec(J.soff); J.o_sn = sb.put_vec(J.sn);
+    J.o_ik0 = sb.put_vec(J.ik0); J.o_ioff = sb.put_vec(J.ioff); J.o_in = sb.put_vec(J.in_);
+    J.o_epoch = sb.put_vec(J.epoch);
+    J.o_drvd = sb.put_vec(J.drv_d); J.o_drvv = sb.put_vec(J.drv_v);
+    J.o_drivers = sb.put(p->drivers, sizeof(psx_driver) * p->n_drivers);
+    J.o_germ = sb.put(p->germline, sizeof(psx_germline) * p->n_germline);
+    const uint32_t zero[2] = {0, 0};
+    J.o_io = sb.put(zero, sizeof zero);
+    if (J.use_id) {
+        J.o_idc = sb.put(J.idx->count, sizeof J.idx->count);
+        J.o_idl = sb.put(J.idx->listed, sizeof J.idx->listed);
+        J.o_ido = sb.put(J.idx->off, sizeof J.idx->off);
+    }
+    J.o_drvpos = sb.put_vec(J.drv_pos);
+}
+
+// reference bases of a list of global positions (driver and germline checks)
+__global__ void k_bases_at(const uint32_t* __restrict__ packed, const uint64_t* __restrict__ woff,
+                           const uint64_t* __restrict__ boff, uint32_t n_chr, const uint64_t* __restrict__ gpos,
+                           uint32_t n, uint8_t* __restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t x = gpos[i];
+    uint32_t c = 0;
+    while (c + 1 < n_chr && boff[c + 1] <= x) ++c;
+    const uint64_t q = x - boff[c];
+    out[i] = (uint8_t)((packed[woff[c] + (q >> 4)] >> (2u * (uint32_t)(q & 15u))) & 3u);
+}
+
+void place_batch(psx_ctx* ctx, const psx_genome* g, std::vector<place_job>& jobs) {
+    hipStream_t s = ctx->stream;
+    const uint32_t nj = (uint32_t)jobs.size();
+    for (place_job& J : jobs) {
+        try {
+            prepare_job(ctx, g, J);
+        } catch (const psx_error& e) {
+            job_throw(J, nj, e.code, e.msg);
+        }
+    }
+    const uint64_t G = g->boff[g->n_chr];
+    // ---- descriptors of every job in one scratch upload
+    scratch_builder sb;
+    const size_t o_len = sb.put_vec(g->len), o_woff = sb.put_vec(g->woff), o_boff = sb.put_vec(g->boff);
+    std::vector<uint64_t> pstart(g->n_chr);  // padded chromosome starts: boff[c] + 16 c
+    for (uint32_t c = 0; c < g->n_chr; ++c) pstart[c] = g->boff[c] + 16ull * c;
+    const size_t o_pstart = sb.put_vec(pstart);
+    std::vector<uint64_t> check_pos;
+    for (place_job& J : jobs) {
+        put_job(sb, J);
+        check_pos.insert(check_pos.end(), J.check_pos.begin(), J.check_pos.end());
+    }
+    const size_t o_check = sb.put_vec(check_pos);
+    uint8_t* base = (uint8_t*)psx_scratch_upload(ctx, sb);
+    auto at = [&](size_t off) { return (void*)(base + off); };
+    const uint64_t* d_pstart = (const uint64_t*)at(o_pstart);
+
+    // pinned readback: per job {last offset, last count, status | losers << 32, chromosome ranges}, then the
+    // checked reference bases
+    const size_t io_job = 3 + (PSX_MAX_CHROMOSOMES + 1);
+    uint64_t* io = (uint64_t*)psx_host_io(ctx, 8 * io_job * nj + check_pos.size() + 8);
+    uint8_t* io_bases = (uint8_t*)(io + io_job * nj);
+
+    PSX_HIP(hipEventRecord(ctx->ev0, s));
+    // ---- phase 1: passenger counts per node, the driver and germline reference bases
+    size_t scan_bytes = 0;
+    uint32_t max_nodes = 1;
+    for (const place_job& J : jobs) max_nodes = std::max(max_nodes, J.f->n_nodes);
+    PSX_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, (uint64_t*)nullptr, (uint64_t*)nullptr,
+                                             (int)max_nodes, s));
+    carver cn;
+    const size_t c_scan = cn.take(scan_bytes), c_bases = cn.take(check_pos.size());
+    std::vector<size_t> c_node(nj);
+    for (uint32_t k = 0; k < nj; ++k) {
+        const size_t n = jobs[k].f->n_nodes;
+        c_node[k] = cn.take(n * 4);
+        cn.take(n * 8);
+        cn.take(n * 8);
+    }
+    ctx->place_nodes.reserve(cn.bytes);
+    uint8_t* pn = (uint8_t*)ctx->place_nodes.p;
+    if (!check_pos.empty()) {
+        hipLaunchKernelGGL(k_bases_at, dim3(grid_of(check_pos.size())), dim3(256), 0, s, g->packed.as<uint32_t>(),
+                           (const uint64_t*)at(o_woff), (const uint64_t*)at(o_boff), g->n_chr,
+                           (const uint64_t*)at(o_check), (uint32_t)check_pos.size(), pn + c_bases);
+        PSX_HIP(hipGetLastError());
+        PSX_HIP(hipMemcpyAsync(io_bases, pn + c_bases, check_pos.size(), hipMemcpyDeviceToHost, s));
+    }
+    for (uint32_t k = 0; k < nj; ++k) {
+        place_job& J = jobs[k];
+        const uint32_t n = J.f->n_nodes;
+        uint8_t* b = pn + c_node[k];
+        J.d_cnt_snv = (uint32_t*)b;
+        J.d_cnt_tot = (uint64_t*)(b + ((n * 4 + 255) & ~size_t(255)));
+        J.d_off = (uint64_t*)((uint8_t*)J.d_cnt_tot + ((n * 8 + 255) & ~size_t(255)));
         edge_args ea{};
-        ea.seed = seed; ea.n_nodes = n; ea.parent = f->d_parent.as<uint32_t>(); ea.species = f->d_species.as<uint32_t>();
-        ea.snv_k0 = (uint32_t*)at(o_sk0); ea.snv_off = (uint32_t*)at(o_soff); ea.snv_n = (uint32_t*)at(o_sn);
-        ea.ind_k0 = (uint32_t*)at(o_ik0); ea.ind_off = (uint32_t*)at(o_ioff); ea.ind_n = (uint32_t*)at(o_in);
-        ea.T = (uint64_t*)at(o_T); ea.cnt_snv = d_cnt_snv; ea.cnt_tot = d_cnt_tot;
+        ea.seed = J.seed; ea.n_nodes = n; ea.parent = J.f->d_parent.as<uint32_t>();
+        ea.species = J.f->d_species.as<uint32_t>();
+        ea.snv_k0 = (uint32_t*)at(J.o_sk0); ea.snv_off = (uint32_t*)at(J.o_soff); ea.snv_n = (uint32_t*)at(J.o_sn);
+        ea.ind_k0 = (uint32_t*)at(J.o_ik0); ea.ind_off = (uint32_t*)at(J.o_ioff); ea.ind_n = (uint32_t*)at(J.o_in);
+        ea.T = (uint64_t*)at(J.o_T); ea.cnt_snv = J.d_cnt_snv; ea.cnt_tot = J.d_cnt_tot;
         hipLaunchKernelGGL(k_edge_counts, dim3(grid_of(n)), dim3(256), 0, s, ea);
         PSX_HIP(hipGetLastError());
-        PSX_HIP(hipcub::DeviceScan::ExclusiveSum(pn + c_scan, scan_bytes, d_cnt_tot, d_off, (int)n, s));
-        uint64_t last[2] = {0, 0};
-        PSX_HIP(hipMemcpyAsync(&last[0], d_off + (n - 1), 8, hipMemcpyDeviceToHost, s));
-        PSX_HIP(hipMemcpyAsync(&last[1], d_cnt_tot + (n - 1), 8, hipMemcpyDeviceToHost, s));
-        PSX_HIP(hipStreamSynchronize(s));
-        const uint64_t last_off = last[0], last_cnt = last[1];
-        const uint64_t n_pass = last_off + last_cnt;
-        const uint64_t n_fixed = drv_d.size() + p->n_germline;
-        const uint64_t M = n_fixed + p->n_preneoplastic_snv + p->n_preneoplastic_indel + n_pass;
-        if (M >= 0xFFFFFFFFull) psx_throw(PSX_EINVAL, "too many mutations (%llu)", (unsigned long long)M);
-
-        // per-mutation temporaries from the context's placement pool, the table in one block of its own;
-        // sort keys: padded positions (chromosomes 16 apart) << 4 | deleted length
-        const uint64_t p_max = pstart[g->n_chr - 1] + g->len[g->n_chr - 1];
-        const int pbits = bits_for(p_max), kbits = bits_for((p_max << 4) | 15u);
-        size_t sort_bytes = 0;
-        PSX_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (uint64_t*)nullptr, (uint64_t*)nullptr,
-                                                   (uint32_t*)nullptr, (uint32_t*)nullptr, (int)M, 0, kbits, s));
-        carver cm;
-        const size_t c_recs = cm.take(M * sizeof(psx_mutation)), c_att = cm.take(M * 4), c_moved = cm.take(M * 4);
-        const size_t c_lose0 = cm.take(M * 4), c_lose1 = cm.take(M * 4), c_head = cm.take(M);
-        const size_t c_key = cm.take(M * 8), c_key2 = cm.take(M * 8), c_key3 = cm.take(M * 8);
-        const size_t c_ids = cm.take(M * 4), c_ids2 = cm.take(M * 4), c_ids3 = cm.take(M * 4);
-        const size_t c_sort = cm.take(sort_bytes), c_cb = cm.take(8 * (PSX_MAX_CHROMOSOMES + 1));
-        ctx->place_muts.reserve(cm.bytes);
-        uint8_t* pm = (uint8_t*)ctx->place_muts.p;
-        uint64_t* d_key = (uint64_t*)(pm + c_key);
-        uint64_t* d_key2 = (uint64_t*)(pm + c_key2);
-        uint64_t* d_key3 = (uint64_t*)(pm + c_key3);
-        uint32_t* d_ids = (uint32_t*)(pm + c_ids);
-        uint32_t* d_ids2 = (uint32_t*)(pm + c_ids2);
-        uint32_t* d_ids3 = (uint32_t*)(pm + c_ids3);
-        uint32_t* d_lose[2] = {(uint32_t*)(pm + c_lose0), (uint32_t*)(pm + c_lose1)};
-        uint32_t* d_moved = (uint32_t*)(pm + c_moved);
-        const uint64_t* d_pstart = (const uint64_t*)at(o_pstart);
-        void* d_sort = pm + c_sort;
-        PSX_HIP(hipMemsetAsync(pm + c_att, 0, M * 4, s));
-        PSX_HIP(hipMemsetAsync(d_moved, 0, M * 4, s));
-        place_args pa{};
+        PSX_HIP(hipcub::DeviceScan::ExclusiveSum(pn + c_scan, scan_bytes, J.d_cnt_tot, J.d_off, (int)n, s));
+        PSX_HIP(hipMemcpyAsync(io + io_job * k, J.d_off + (n - 1), 8, hipMemcpyDeviceToHost, s));
+        PSX_HIP(hipMemcpyAsync(io + io_job * k + 1, J.d_cnt_tot + (n - 1), 8, hipMemcpyDeviceToHost, s));
+    }
+    PSX_HIP(hipStreamSynchronize(s));
+    {
+        size_t ci = 0;
+        for (place_job& J : jobs) {
+            for (uint32_t d = 0; d < J.p->n_drivers; ++d, ++ci)
+                if (J.p->drivers[d].alt == io_bases[ci]) job_throw(J, nj, PSX_EINVAL, "invalid driver " + std::to_string(d));
+            for (uint32_t q = 0; q < J.p->n_germline; ++q, ++ci)
+                if (J.p->germline[q].alt == io_bases[ci])
+                    job_throw(J, nj, PSX_EINVAL, "invalid germline mutation " + std::to_string(q));
+        }
+    }
+    // ---- phase 2: mutation ids, placement, the first infinite-sites check
+    const uint64_t p_max = pstart[g->n_chr - 1] + g->len[g->n_chr - 1];
+    const int pbits = bits_for(p_max);
+    uint64_t max_m = 1, sum_m = 0;
+    for (uint32_t k = 0; k < nj; ++k) {
+        place_job& J = jobs[k];
+        J.n_pass = io[io_job * k] + io[io_job * k + 1];
+        J.n_fixed = J.drv_d.size() + J.p->n_germline;
+        J.M = J.n_fixed + J.p->n_preneoplastic_snv + J.p->n_preneoplastic_indel + J.n_pass;
+        if (J.M > ID_MASK)
+            job_throw(J, nj, PSX_EINVAL, "too many mutations (" + std::to_string(J.M) + ", at most " +
+                                             std::to_string(ID_MASK) + ")");
+        max_m = std::max(max_m, J.M);
+        sum_m += J.M;
+    }
+    size_t sort_bytes = 0;
+    PSX_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (uint64_t*)nullptr, (uint64_t*)nullptr,
+                                               (uint32_t*)nullptr, (uint32_t*)nullptr, (int)max_m, 0, pbits, s));
+    // per-mutation temporaries from the context's placement pool: the attempt counters and moved flags of every
+    // job first (cleared together), then each job's own
+    carver cm;
+    const size_t c_zero = cm.take(sum_m * 8), c_sort = cm.take(sort_bytes);
+    std::vector<size_t> c_mut(nj);
+    for (uint32_t k = 0; k < nj; ++k) {
+        const uint64_t M = jobs[k].M;
+        c_mut[k] = cm.bytes;
+        cm.take(M * sizeof(psx_mutation));
+        for (int q = 0; q < 5; ++q) cm.take(M * 4);  // lose[2], list, ids, ids2
+        cm.take(M * 4);                             // ids3
+        cm.take(jobs[k].n_pass * 4);                // node_of
+        for (int q = 0; q < 3; ++q) cm.take(M * 8);  // key, key2, key3
+        cm.take(M);                                 // head
+    }
+    ctx->place_muts.reserve(cm.bytes);
+    uint8_t* pm = (uint8_t*)ctx->place_muts.p;
+    void* d_sort = pm + c_sort;
+    PSX_HIP(hipMemsetAsync(pm + c_zero, 0, sum_m * 8, s));
+    uint64_t zero_at = 0;
+    for (uint32_t k = 0; k < nj; ++k) {
+        place_job& J = jobs[k];
+        const uint64_t M = J.M;
+        J.d_att = (uint32_t*)(pm + c_zero) + zero_at;
+        J.d_moved = (uint32_t*)(pm + c_zero) + sum_m + zero_at;
+        zero_at += M;
+        carver c;
+        c.bytes = c_mut[k];
+        psx_mutation* recs = (psx_mutation*)(pm + c.take(M * sizeof(psx_mutation)));
+        J.d_lose[0] = (uint32_t*)(pm + c.take(M * 4));
+        J.d_lose[1] = (uint32_t*)(pm + c.take(M * 4));
+        J.d_list = (uint32_t*)(pm + c.take(M * 4));
+        J.d_ids = (uint32_t*)(pm + c.take(M * 4));
+        J.d_ids2 = (uint32_t*)(pm + c.take(M * 4));
+        J.d_ids3 = (uint32_t*)(pm + c.take(M * 4));
+        J.d_node_of = (uint32_t*)(pm + c.take(J.n_pass * 4));
+        J.d_key = (uint64_t*)(pm + c.take(M * 8));
+        J.d_key2 = (uint64_t*)(pm + c.take(M * 8));
+        J.d_key3 = (uint64_t*)(pm + c.take(M * 8));
+        J.d_head = pm + c.take(M);
+        J.d_io = (uint32_t*)at(J.o_io);
+        const psx_engine_params* p = J.p;
+        place_args& pa = J.pa;
         pa.g = gdev{g->packed.as<uint32_t>(), (uint32_t*)at(o_len), (uint64_t*)at(o_woff), (uint64_t*)at(o_boff), G,
                     g->n_chr};
-        pa.seed = seed; pa.ploidy = p->ploidy; pa.n_sbs = p->n_sbs;
-        pa.expo_T = (uint64_t*)at(o_T) + expo_off; pa.cls_T = (uint64_t*)at(o_T) + cls_off;
-        pa.len_T = (uint64_t*)at(o_T) + len_off; pa.len_n = p->indel_max_len;
-        pa.node_epoch = (uint32_t*)at(o_epoch);
-        pa.leaf_lo = f->d_leaf_lo.as<uint32_t>(); pa.leaf_hi = f->d_leaf_hi.as<uint32_t>();
-        pa.n_leaves = f->n_leaves; pa.n_nodes = n;
-        pa.n_drv = drv_d.size(); pa.n_germ = p->n_germline;
+        pa.seed = J.seed; pa.ploidy = p->ploidy; pa.n_sbs = p->n_sbs;
+        pa.expo_T = (uint64_t*)at(J.o_T) + J.expo_off; pa.cls_T = (uint64_t*)at(J.o_T) + J.cls_off;
+        pa.len_T = (uint64_t*)at(J.o_T) + J.len_off; pa.len_n = p->indel_max_len;
+        pa.node_epoch = (uint32_t*)at(J.o_epoch);
+        pa.leaf_lo = J.f->d_leaf_lo.as<uint32_t>(); pa.leaf_hi = J.f->d_leaf_hi.as<uint32_t>();
+        pa.n_leaves = J.f->n_leaves; pa.n_nodes = J.f->n_nodes;
+        pa.n_drv = J.drv_d.size(); pa.n_germ = p->n_germline;
         pa.n_pre_snv = p->n_preneoplastic_snv; pa.n_pre_indel = p->n_preneoplastic_indel;
-        pa.drv_d = (uint32_t*)at(o_drvd); pa.drv_v = (uint32_t*)at(o_drvv);
-        pa.drivers = (psx_driver*)at(o_drivers); pa.germ = (psx_germline*)at(o_germ);
+        pa.drv_d = (uint32_t*)at(J.o_drvd); pa.drv_v = (uint32_t*)at(J.o_drvv);
+        pa.drivers = (psx_driver*)at(J.o_drivers); pa.germ = (psx_germline*)at(J.o_germ);
         pa.pre_sig = p->preneoplastic_sbs;
-        pa.cnt_snv = d_cnt_snv; pa.node_off = d_off;
-        pa.M = M; pa.recs = (psx_mutation*)(pm + c_recs); pa.attempt = (uint32_t*)(pm + c_att);
-        pa.redraw = nullptr; pa.moved = d_moved; pa.status = (uint32_t*)at(o_status);
-        if (sidx) {
-            pa.sbs_list = sidx->d_list.as<uint32_t>();
-            pa.sbs_kept = sidx->d_meta.as<uint64_t>();
-            pa.sbs_off = sidx->d_meta.as<uint64_t>() + 32;
+        pa.cnt_snv = J.d_cnt_snv; pa.node_off = J.d_off; pa.node_of = J.d_node_of;
+        pa.M = M; pa.recs = recs; pa.attempt = J.d_att;
+        pa.redraw = nullptr; pa.moved = J.d_moved; pa.status = J.d_io;
+        if (J.sidx) {
+            pa.sbs_list = J.sidx->d_list.as<uint32_t>();
+            pa.sbs_kept = J.sidx->d_meta.as<uint64_t>();
+            pa.sbs_off = J.sidx->d_meta.as<uint64_t>() + 32;
         }
-        pa.drv_pos = (const uint64_t*)at(o_drvpos);
-        pa.n_drv_pos = (uint32_t)drv_pos.size();
-        if (use_id) {
+        pa.drv_pos = (const uint64_t*)at(J.o_drvpos);
+        pa.n_drv_pos = (uint32_t)J.drv_pos.size();
+        if (J.use_id) {
             pa.use_id = 1; pa.n_id = p->n_id; pa.pre_id = p->preneoplastic_id;
-            pa.id_expo_T = (uint64_t*)at(o_T) + id_expo_off; pa.id_cls_T = (uint64_t*)at(o_T) + id_cls_off;
-            pa.id_count = (const uint64_t*)at(o_idc); pa.id_listed = (const uint32_t*)at(o_idl);
-            pa.id_off = (const uint64_t*)at(o_ido); pa.id_list = idx->d_list.as<uint64_t>();
+            pa.id_expo_T = (uint64_t*)at(J.o_T) + J.id_expo_off; pa.id_cls_T = (uint64_t*)at(J.o_T) + J.id_cls_off;
+            pa.id_count = (const uint64_t*)at(J.o_idc); pa.id_listed = (const uint32_t*)at(J.o_idl);
+            pa.id_off = (const uint64_t*)at(J.o_ido); pa.id_list = J.idx->d_list.as<uint64_t>();
+        }
+        if (J.n_pass) {
+            hipLaunchKernelGGL(k_node_expand, dim3(grid_of(J.f->n_nodes)), dim3(256), 0, s, J.d_off, J.d_cnt_tot,
+                               J.f->n_nodes, J.d_node_of);
+            PSX_HIP(hipGetLastError());
         }
         if (M) {
             hipLaunchKernelGGL(k_place, dim3(grid_of(M)), dim3(256), 0, s, pa);
             PSX_HIP(hipGetLastError());
         }
-        // status of the last placement launch and the losers of the last check, read in one synchronisation
-        uint32_t* d_nlose = (uint32_t*)at(o_nlose);
-        auto read_back = [&](bool with_losers) -> uint32_t {
-            uint32_t stv = 0, n_lose = 0;
-            PSX_HIP(hipMemcpyAsync(&stv, pa.status, 4, hipMemcpyDeviceToHost, s));
-            if (with_losers) PSX_HIP(hipMemcpyAsync(&n_lose, d_nlose, 4, hipMemcpyDeviceToHost, s));
-            PSX_HIP(hipStreamSynchronize(s));
-            if (stv & ST_NO_SITE) psx_throw(PSX_EINVAL, "no reference site matches a drawn SBS or ID class");
-            if (stv & ST_NO_INDEL_ROOM) psx_throw(PSX_EINVAL, "no room for an indel");
-            return n_lose;
-        };
         if (p->infinite_sites && M > 1) {
             // first round over every footprint; the sorted footprints stay as the reference of later rounds
-            hipLaunchKernelGGL(k_footprint, dim3(grid_of(M)), dim3(256), 0, s, pa.recs, M, d_pstart, d_key, d_ids);
+            hipLaunchKernelGGL(k_footprint, dim3(grid_of(M)), dim3(256), 0, s, pa.recs, M, d_pstart, J.d_key,
+                               J.d_ids);
             PSX_HIP(hipGetLastError());
-            PSX_HIP(hipcub::DeviceRadixSort::SortPairs(d_sort, sort_bytes, d_key, d_key2, d_ids, d_ids2, (int)M, 0,
-                                                       kbits, s));
-            PSX_HIP(hipMemsetAsync(d_nlose, 0, 4, s));
-            hipLaunchKernelGGL(k_conflicts, dim3(grid_of(M)), dim3(256), 0, s, d_key2, d_ids2, M, n_fixed, d_lose[0],
-                               d_nlose);
+            PSX_HIP(hipcub::DeviceRadixSort::SortPairs(d_sort, sort_bytes, J.d_key, J.d_key2, J.d_ids, J.d_ids2,
+                                                       (int)M, 0, pbits, s));
+            hipLaunchKernelGGL(k_conflicts, dim3(grid_of(M)), dim3(256), 0, s, J.d_key2, J.d_ids2, M, J.n_fixed,
+                               J.d_lose[0], J.d_list, J.d_io + 1);
             PSX_HIP(hipGetLastError());
-            uint32_t n_lose = read_back(true);
-            uint64_t moved_cap = n_lose;  // redrawn mutations so far, counted with repeats
-            int round = 1, cur = 0;
-            for (; n_lose && round < IS_ROUNDS; ++round) {
-                place_args pr = pa;
-                pr.redraw = d_lose[cur];
-                hipLaunchKernelGGL(k_place, dim3(grid_of(M)), dim3(256), 0, s, pr);
-                PSX_HIP(hipGetLastError());
-                // the moved mutations' current footprints, sorted; sentinels fill the unused slots
-                const uint64_t nb = std::min<uint64_t>(moved_cap, M);
-                hipLaunchKernelGGL(k_fill_sentinel, dim3(grid_of(nb)), dim3(256), 0, s, d_key, d_ids, nb,
-                                   (kbits >= 64 ? ~0ull : (1ull << kbits) - 1ull));
-                PSX_HIP(hipGetLastError());
-                PSX_HIP(hipMemsetAsync(d_nlose, 0, 4, s));
-                hipLaunchKernelGGL(k_collect_moved, dim3(grid_of(M)), dim3(256), 0, s, pa.recs, M, d_pstart, d_moved,
-                                   d_key, d_ids, d_nlose);
-                PSX_HIP(hipGetLastError());
-                PSX_HIP(hipcub::DeviceRadixSort::SortPairs(d_sort, sort_bytes, d_key, d_key3, d_ids, d_ids3, (int)nb,
-                                                           0, kbits, s));
-                PSX_HIP(hipMemsetAsync(d_lose[cur ^ 1], 0, M * 4, s));
-                PSX_HIP(hipMemsetAsync(d_nlose, 0, 4, s));
-                inc_args ia{};
-                ia.akey = d_key2; ia.aids = d_ids2; ia.M = M;
-                ia.bkey = d_key3; ia.bids = d_ids3; ia.nb = nb;
-                ia.moved = d_moved; ia.changed = d_lose[cur]; ia.lose = d_lose[cur ^ 1]; ia.n_lose = d_nlose;
-                hipLaunchKernelGGL(k_conflicts_inc, dim3(grid_of(nb)), dim3(256), 0, s, ia);
-                PSX_HIP(hipGetLastError());
-                n_lose = read_back(true);
-                moved_cap += n_lose;
-                cur ^= 1;
-            }
-            if (n_lose) psx_throw(PSX_EINVAL, "infinite-sites placement did not converge");
-        } else {
-            read_back(false);
         }
-        // ---- SID order: a stable sort by padded position, each run of one position sorted by the rest of the SID
+        PSX_HIP(hipMemcpyAsync(io + io_job * k + 2, J.d_io, 8, hipMemcpyDeviceToHost, s));
+    }
+    PSX_HIP(hipStreamSynchronize(s));
+    // status of the last placement launch and the losers of the last check of every job
+    auto take_status = [&](place_job& J) {
+        const uint64_t v = io[io_job * J.k + 2];
+        const uint32_t stv = (uint32_t)v;
+        if (stv & ST_NO_SITE) job_throw(J, nj, PSX_EINVAL, "no reference site matches a drawn SBS or ID class");
+        if (stv & ST_NO_INDEL_ROOM) job_throw(J, nj, PSX_EINVAL, "no room for an indel");
+        J.n_lose = (uint32_t)(v >> 32);
+    };
+    for (place_job& J : jobs) {
+        take_status(J);
+        J.moved_cap = J.n_lose;
+    }
+    // ---- phase 3: later infinite-sites rounds, every job with losers in each
+    for (int round = 1;; ++round) {
+        bool any = false;
+        for (const place_job& J : jobs) any |= J.n_lose != 0;
+        if (!any) break;
+        if (round >= IS_ROUNDS) {
+            for (const place_job& J : jobs)
+                if (J.n_lose) job_throw(J, nj, PSX_EINVAL, "infinite-sites placement did not converge");
+        }
+        for (place_job& J : jobs) {
+            if (!J.n_lose) continue;
+            const uint64_t M = J.M;
+            place_args pr = J.pa;
+            pr.redraw = J.d_list;
+            pr.n_redraw = J.n_lose;
+            hipLaunchKernelGGL(k_place, dim3(grid_of(J.n_lose)), dim3(256), 0, s, pr);
+            PSX_HIP(hipGetLastError());
+            // the moved mutations' current footprints, sorted; sentinels fill the unused slots
+            const uint64_t nb = std::min<uint64_t>(J.moved_cap, M);
+            hipLaunchKernelGGL(k_fill_sentinel, dim3(grid_of(nb)), dim3(256), 0, s, J.d_key, J.d_ids, nb,
+                               (pbits >= 64 ? ~0ull : (1ull << pbits) - 1ull));
+            PSX_HIP(hipGetLastError());
+            PSX_HIP(hipMemsetAsync(J.d_io + 1, 0, 4, s));
+            hipLaunchKernelGGL(k_collect_moved, dim3(grid_of(M)), dim3(256), 0, s, J.pa.recs, M, d_pstart, J.d_moved,
+                               J.d_key, J.d_ids, J.d_io + 1);
+            PSX_HIP(hipGetLastError());
+            PSX_HIP(hipcub::DeviceRadixSort::SortPairs(d_sort, sort_bytes, J.d_key, J.d_key3, J.d_ids, J.d_ids3,
+                                                       (int)nb, 0, pbits, s));
+            PSX_HIP(hipMemsetAsync(J.d_lose[J.cur ^ 1], 0, M * 4, s));
+            PSX_HIP(hipMemsetAsync(J.d_io + 1, 0, 4, s));
+            inc_args ia{};
+            ia.akey = J.d_key2; ia.aval = J.d_ids2; ia.M = M;
+            ia.bkey = J.d_key3; ia.bval = J.d_ids3; ia.nb = nb;
+            ia.moved = J.d_moved; ia.changed = J.d_lose[J.cur]; ia.lose = J.d_lose[J.cur ^ 1]; ia.list = J.d_list;
+            ia.n_lose = J.d_io + 1;
+            hipLaunchKernelGGL(k_conflicts_inc, dim3(grid_of(nb)), dim3(256), 0, s, ia);
+            PSX_HIP(hipGetLastError());
+            PSX_HIP(hipMemcpyAsync(io + io_job * J.k + 2, J.d_io, 8, hipMemcpyDeviceToHost, s));
+        }
+        PSX_HIP(hipStreamSynchronize(s));
+        for (place_job& J : jobs) {
+            if (!J.n_lose) continue;
+            take_status(J);
+            J.moved_cap += J.n_lose;
+            J.cur ^= 1;
+        }
+    }
+    // ---- phase 4: SID order (a stable sort by padded position, each run of one position sorted by the rest of
+    // the SID) into the tables, which share one block
+    size_t block_bytes = 0;
+    std::vector<size_t> t_off(nj);
+    for (uint32_t k = 0; k < nj; ++k) {
+        t_off[k] = block_bytes;
+        block_bytes += psx_mutations_bytes(jobs[k].M);
+    }
+    auto block = std::make_shared<dbuf>();
+    block->alloc(block_bytes);
+    for (uint32_t k = 0; k < nj; ++k) {
+        place_job& J = jobs[k];
+        psx_mutations* mt = J.mt;
+        const uint64_t M = J.M;
         mt->n = M;
-        mt->ploidy = p->ploidy;
-        mt->n_leaves = f->n_leaves;
+        mt->ploidy = J.p->ploidy;
+        mt->n_leaves = J.f->n_leaves;
         mt->chr_len = g->len;
-        auto block = std::make_shared<dbuf>();
-        block->alloc(psx_mutations_bytes(M));
-        psx_mutations_carve(mt, block, 0);
-        std::vector<uint64_t> chr_begin(PSX_MAX_CHROMOSOMES + 1, M);
+        psx_mutations_carve(mt, block, t_off[k]);
         if (M) {
-            hipLaunchKernelGGL(k_pos_key, dim3(grid_of(M)), dim3(256), 0, s, pa.recs, M, d_pstart, d_key, d_ids);
+            hipLaunchKernelGGL(k_pos_key, dim3(grid_of(M)), dim3(256), 0, s, J.pa.recs, M, d_pstart, J.d_key, J.d_ids);
             PSX_HIP(hipGetLastError());
-            PSX_HIP(hipcub::DeviceRadixSort::SortPairs(d_sort, sort_bytes, d_key, d_key2, d_ids, d_ids2, (int)M, 0,
-                                                       pbits, s));
-            uint8_t* d_head = pm + c_head;
-            hipLaunchKernelGGL(k_sid_runs, dim3(grid_of(M)), dim3(256), 0, s, pa.recs, d_key2, d_ids2, M, d_head);
+            PSX_HIP(hipcub::DeviceRadixSort::SortPairs(d_sort, sort_bytes, J.d_key, J.d_key2, J.d_ids, J.d_ids2,
+                                                       (int)M, 0, pbits, s));
+            hipLaunchKernelGGL(k_sid_runs, dim3(grid_of(M)), dim3(256), 0, s, J.pa.recs, J.d_key2, J.d_ids2, M,
+                               J.d_head);
             PSX_HIP(hipGetLastError());
-            hipLaunchKernelGGL(k_gather, dim3(grid_of(M)), dim3(256), 0, s, pa.recs, d_ids2, d_head, M, mt->recs);
+            hipLaunchKernelGGL(k_gather, dim3(grid_of(M)), dim3(256), 0, s, J.pa.recs, J.d_ids2, J.d_head, M, mt->recs);
             PSX_HIP(hipGetLastError());
-            uint64_t* d_cb = (uint64_t*)(pm + c_cb);
-            hipLaunchKernelGGL(k_chr_begin, dim3(1), dim3(128), 0, s, d_key2, M, d_pstart, g->n_chr, d_cb);
-            PSX_HIP(hipGetLastError());
-            PSX_HIP(hipMemcpyAsync(chr_begin.data(), d_cb, 8 * (PSX_MAX_CHROMOSOMES + 1), hipMemcpyDeviceToHost, s));
         }
-        PSX_HIP(hipEventRecord(ctx->ev1, s));
-        PSX_HIP(hipStreamSynchronize(s));
-        psx_mutations_finish(ctx, mt, chr_begin);
-        PSX_HIP(hipStreamSynchronize(s));
-        float ms = 0.f;
-        PSX_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-        ctx->last_place_ms = ms;
+        hipLaunchKernelGGL(k_chr_begin, dim3(1), dim3(128), 0, s, J.d_key2, M, d_pstart, g->n_chr, mt->d_chr_begin);
+        PSX_HIP(hipGetLastError());
+        PSX_HIP(hipMemcpyAsync(io + io_job * k + 3, mt->d_chr_begin, 8 * (PSX_MAX_CHROMOSOMES + 1),
+                               hipMemcpyDeviceToHost, s));
+        psx_mutations_finish_device(ctx, mt, false);
+    }
+    PSX_HIP(hipEventRecord(ctx->ev1, s));
+    PSX_HIP(hipStreamSynchronize(s));
+    for (uint32_t k = 0; k < nj; ++k)
+        jobs[k].mt->chr_begin.assign(io + io_job * k + 3, io + io_job * k + 3 + (PSX_MAX_CHROMOSOMES + 1));
+    float ms = 0.f;
+    PSX_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    ctx->last_place_ms = ms;
+}
+
+}  // namespace
+
+extern "C" psx_status psx_place_mutations_batch(psx_ctx* ctx, const psx_genome* g, uint32_t n_jobs,
+                                                const psx_forest* const* f, const psx_engine_params* const* p,
+                                                const uint64_t* seeds, psx_mutations** out) {
+    if (!ctx || !g || !f || !p || !seeds || !out || n_jobs == 0) return PSX_EINVAL;
+    for (uint32_t k = 0; k < n_jobs; ++k)
+        if (!f[k] || !p[k]) return PSX_EINVAL;
+    std::vector<place_job> jobs(n_jobs);
+    for (uint32_t k = 0; k < n_jobs; ++k) {
+        jobs[k].k = k;
+        jobs[k].f = f[k];
+        jobs[k].p = p[k];
+        jobs[k].seed = seeds[k];
+        jobs[k].mt = new psx_mutations();
+        jobs[k].mt->ctx = ctx;
+    }
+    const psx_status st = psx_guard(ctx, [&] {
+        PSX_HIP(hipSetDevice(ctx->device));
+        place_batch(ctx, g, jobs);
     });
-    if (st != PSX_OK) { delete mt; return st; }
-    *out = mt;
+    if (st != PSX_OK) {
+        for (place_job& J : jobs) delete J.mt;
+        return st;
+    }
+    for (uint32_t k = 0; k < n_jobs; ++k) out[k] = jobs[k].mt;
     return PSX_OK;
+}
+
+extern "C" psx_status psx_place_mutations(psx_ctx* ctx, const psx_genome* g, const psx_forest* f,
+                                          const psx_engine_params* p, uint64_t seed, psx_mutations** out) {
+    if (!ctx || !g || !f || !p || !out) return PSX_EINVAL;
+    return psx_place_mutations_batch(ctx, g, 1, &f, &p, &seed, out);
 }

@@ -32,7 +32,7 @@ enum psx_purpose : uint32_t {
     PSX_P_QUALITY = 0x12u,   // quality scores of a written read (random quality model)
     PSX_P_EDGE = 0x20u,      // per-edge passenger counts
     PSX_P_SNV = 0x21u,       // SNV class and allele
-    PSX_P_SITE = 0x22u,      // SNV site search (rejection on context)
+    PSX_P_SITE = 0x22u,      // SNV site: index into the context index list of the class
     PSX_P_INDEL = 0x23u,     // indel type, length, allele, position
     PSX_P_IDCLS = 0x24u,     // indel signature, ID class and allele (ID signatures)
     PSX_P_DRIVER = 0x25u,    // driver allele

@@ -76,6 +76,8 @@ struct psx_server {
     std::unordered_map<uint64_t, std::unique_ptr<srv_req>> reqs;
     std::vector<std::unique_ptr<lane_state>> lanes;
     std::mutex prep_mu;          // descriptor preparation may re-index a shared mutation table
+    std::vector<clk::time_point> lane_start;  // start of each lane's running launch
+    std::vector<char> lane_busy;
     bool stop = false;
     uint64_t next_ticket = 1;
     psx_server_totals tot{};
@@ -131,9 +133,24 @@ void worker(psx_server* s, uint32_t k) {
     lane_state& L = *s->lanes[k];
     const bool dev_ok = hipSetDevice(s->ctx->device) == hipSuccess;
     std::unique_lock<std::mutex> lk(s->mu);
+    const auto overlap = std::chrono::microseconds(s->cfg.overlap_after_us);
     for (;;) {
         s->cv.wait(lk, [&] { return s->stop || !s->pending.empty(); });
         if (s->stop) break;
+        if (s->cfg.overlap_after_us && s->pending.size() < s->cfg.max_batch) {
+            // another lane's young launch: wait until it has run overlap_after_us (or ends)
+            bool young = false;
+            clk::time_point until{};
+            for (size_t j = 0; j < s->lanes.size(); ++j)
+                if (s->lane_busy[j] && (!young || s->lane_start[j] + overlap < until)) {
+                    young = true;
+                    until = s->lane_start[j] + overlap;
+                }
+            if (young && clk::now() < until) {
+                s->cv.wait_until(lk, until);
+                continue;
+            }
+        }
         std::vector<srv_req*> batch;
         admit_fifo(s->pending, s->ar, s->cfg.max_batch, batch, [](srv_req* q) { return q->out_bytes; },
                    [](srv_req* q, size_t off) { q->off = off; });
@@ -143,6 +160,8 @@ void worker(psx_server* s, uint32_t k) {
             continue;
         }
         s->tot.launches++;
+        s->lane_busy[k] = 1;
+        s->lane_start[k] = clk::now();
         lk.unlock();
         psx_status st = PSX_OK;
         std::string msg;
@@ -158,6 +177,7 @@ void worker(psx_server* s, uint32_t k) {
         }
         const clk::time_point t_done = clk::now();
         lk.lock();
+        s->lane_busy[k] = 0;
         for (srv_req* q : batch) {
             s->ar.give(q->off);
             q->off = arena::NONE;
@@ -199,6 +219,8 @@ extern "C" psx_status psx_server_create(psx_ctx* ctx, const psx_server_config* c
         s->pool.alloc(cfg->hbm_budget);
         s->ar.reset(cfg->hbm_budget);
         s->tot.budget_bytes = s->ar.total();
+        s->lane_start.assign(cfg->max_inflight, clk::time_point{});
+        s->lane_busy.assign(cfg->max_inflight, 0);
         for (uint32_t k = 0; k < cfg->max_inflight; ++k) {
             auto L = std::make_unique<lane_state>();
             L->lane.n_cu = ctx->n_cu;

@@ -6,6 +6,8 @@ from __future__ import annotations
 
 from dataclasses import dataclass, field
 
+import time
+
 import numpy as np
 
 from . import core
@@ -133,8 +135,10 @@ class BatchedSet:
     specs: list         # dicts of batched_jobs
     forests: list
     muts: list
-    place_ms: float
+    place_ms: float       # HIP-event span of the batched placement
     reads: int
+    place_wall_ms: float = 0.0  # wall clock of the batched placement call
+    engine: dict = field(default_factory=dict)  # the engine parameters every job was placed with
 
 
 def seq_job_params(coverage: float) -> dict:
@@ -153,20 +157,23 @@ def build_batched(ctx: core.Context, seed: int = 99, n_jobs: int = 256, **kw) ->
     engine = dict(snv_rate=[1e-8], indel_rate=[1e-9], sbs=sig[None, :], epoch_start=[0.0], exposure=[[1.0]],
                   n_pre_snv=0, n_pre_indel=0, pre_sbs=0, indel_max_len=10, indel_len_p=0.5, infinite_sites=True,
                   ploidy=2, drivers=[], germline=[])
-    jobs, forests, muts = [], [], []
-    place_ms, reads = 0.0, 0
+    forests = []
     for s in specs:
         fd = core.synth_forest(growth=[0.2], death=[0.01], intro_at=[0], n_cells=2 * s["leaves"], n_samples=1,
                                sample_size=s["leaves"], seed=s["forest_seed"])
-        f = core.Forest(ctx, fd["cell_id"], fd["parent"], fd["species"], fd["sample"], fd["birth"], fd["n_samples"])
-        m = core.Mutations.place(ctx, g, f, s["place_seed"], **engine)
-        place_ms += ctx.last_place_ms()
+        forests.append(core.Forest(ctx, fd["cell_id"], fd["parent"], fd["species"], fd["sample"], fd["birth"],
+                                   fd["n_samples"]))
+    # every job's placement in one batched call
+    t0 = time.perf_counter()
+    muts = core.Mutations.place_batch(ctx, g, forests, [s["place_seed"] for s in specs], [engine] * len(specs))
+    place_wall = (time.perf_counter() - t0) * 1e3
+    place_ms = ctx.last_place_ms()
+    jobs, reads = [], 0
+    for s, f, m in zip(specs, forests, muts):
         seq = seq_job_params(s["coverage"])
         jobs.append(core.SeqJob(ctx, g, f, m, s["seq_seed"], **seq))
         reads += core.num_reads(g, f, **seq)
-        forests.append(f)
-        muts.append(m)
-    return BatchedSet(g, jobs, specs, forests, muts, place_ms, reads)
+    return BatchedSet(g, jobs, specs, forests, muts, place_ms, reads, place_wall, engine)
 
 
 @dataclass
