@@ -26,8 +26,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# serving lanes start a launch beside a running one only once it has run this long (microseconds)
-SERVE_OVERLAP_US = 0
+# serving lanes start a launch beside a running one only once it has run this long (microseconds): at
+# 20 req/s on this workload 250 us keeps the p99 of one request at a time (2.8 ms against 4.2 ms with no
+# delay) and the saturated throughput of two lanes
+SERVE_OVERLAP_US = 250
 
 # HBM bytes per launch of the dominant kernels on the plain_large workload, from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE runs calibrated against a device copy, and vector instructions per launch from SQ_INSTS_VALU (the
@@ -382,6 +384,10 @@ def serving_leg(ctx, rate: float, duration: float, saturated: int, overlap_us: i
         return r
 
     stream = workloads.serving_requests(pool, int(rate * duration), rate)
+    # warm-up outside the timed runs: the first request of each kind loads its kernels and builds its tables
+    warm = workloads.serving_requests(pool, 4 * len(pool.forests), 0.0, seed=99)
+    run(warm, 2, 16, False, "warm-up", overlap_us)
+    run(warm, 1, 1, False, "warm-up one at a time")
     out = dict(workload=f"Poisson arrivals at {rate:g} req/s for {duration:g} s over {len(pool.forests)} resident "
                         "forests of 1e3-1e6 cells (2-3 samples of <= 1e4 cells), shared 50 Mb reference, "
                         "coverage log-uniform 30-500x, PE 150 bp, error 1e-3, random sample subsets + normal",

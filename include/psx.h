@@ -346,6 +346,12 @@ uint32_t psx_seq_num_samples(const psx_forest* f, const psx_seq_params* p);
  * sequencing calls report it). */
 psx_status psx_seq_num_reads(const psx_genome* g, const psx_forest* f, const psx_seq_params* p,
                              uint64_t* n_reads);
+/* Reads a sequencing call of this table and seed simulates, copy-number
+ * events included (their kept templates are counted on the GPU with the
+ * call's own draws): the capacity psx_simulate_seq_reads needs. */
+psx_status psx_seq_num_reads_table(psx_ctx* ctx, const psx_genome* g, const psx_forest* f,
+                                   const psx_mutations* m, const psx_seq_params* p, uint64_t seed,
+                                   uint64_t* n_reads);
 
 /* Copy numbers. With copy-number events on a chromosome, a template draws
  * its source cell and one of kmax copy slots (kmax = the chromosome's largest
@@ -394,7 +400,9 @@ psx_status psx_simulate_seq_device_noalloc(psx_ctx* ctx, const psx_genome* g, co
  * header (start position; source leaf | allele << 26 | mate << 30 | has-error
  * << 31) plus ceil(read_size/16) words of 2-bit bases with the source's SNVs
  * and the sequencing errors applied. d_reads holds psx_read_words(p) uint32
- * per read, reads ordered by (sample, chromosome, stratum, template, mate). */
+ * per read, reads ordered by (sample, chromosome, stratum, template, mate);
+ * on a chromosome with copy-number events only the kept templates have reads
+ * (psx_seq_num_reads_table counts them). */
 uint32_t psx_read_words(const psx_seq_params* p);
 psx_status psx_simulate_seq_reads(psx_ctx* ctx, const psx_genome* g, const psx_forest* f,
                                   const psx_mutations* m, const psx_seq_params* p,

@@ -1319,7 +1319,6 @@ typedef struct seq_ctx {
     uint64_t chr_woff[PSX_MAX_CHROMOSOMES];
     uint32_t* rows;
     uint32_t rw;
-    uint64_t* seg_base;
     /* a consumer of every read in place of the counting (orc_seq_walk) */
     orc_read_fn hook;
     void* hook_user;
@@ -1515,6 +1514,45 @@ int orc_simulate_seq(uint32_t n_chr, const uint32_t* len, const orc_forest* f,
 
 /* simulate_seq; with rows != NULL also every read, materialised into rows
  * (capacity reads of psx_read_words words; no copy-number events). */
+/* A template's source from its draw w (x its start): the normal genome with
+ * probability 1 - purity (always for the normal sample), else a uniform
+ * (cell, copy) of the sample. With copy-number events on the chromosome one
+ * of K copy slots per cell is drawn and the template is kept only when the
+ * slot is one of the source's copies at x (returns 0 otherwise). */
+static int template_source(const seq_ctx* s, uint32_t c, int has_ev, uint32_t K, int is_normal, uint64_t thr_normal,
+                           const uint32_t* sl, uint32_t ns, const uint32_t w[4], uint32_t x, uint32_t* leaf,
+                           uint32_t* allele) {
+    const uint32_t ploidy = s->ploidy;
+    if (!has_ev) {
+        if (is_normal || (uint64_t)w[1] < thr_normal) {
+            *leaf = s->f->n_leaves;
+            *allele = mulhi32(w[2], ploidy);
+        } else {
+            uint32_t X = mulhi32(w[2], ns * ploidy);
+            *leaf = sl[X / ploidy];
+            *allele = X % ploidy;
+        }
+        return 1;
+    }
+    if (is_normal || (uint64_t)w[1] < thr_normal) {
+        uint32_t k = mulhi32(w[2], K);
+        if (k >= ploidy) return 0;  /* not a copy of the normal genome */
+        *leaf = s->f->n_leaves;
+        *allele = k;
+        return 1;
+    }
+    uint32_t X = mulhi32(w[2], ns * K);
+    uint32_t k = X % K;
+    *leaf = sl[X / K];
+    uint32_t cn[PSX_MAX_CNA_PLOIDY];
+    uint32_t tot = copies(s, c, x, *leaf, cn);
+    if (k >= tot) return 0;  /* the cell has fewer copies at x */
+    uint32_t a = 0, cum = cn[0];
+    while (a + 1 < ploidy && k >= cum) cum += cn[++a];
+    *allele = a;
+    return 1;
+}
+
 static int seq_run(uint32_t n_chr, const uint32_t* len, const uint32_t* packed, const orc_forest* f,
                    const psx_mutation* m, uint64_t n_mut, uint32_t ploidy, const psx_cna_event* ev, uint32_t n_ev,
                    const psx_seq_params* p, uint64_t seed, int32_t* alt, int32_t* depth, uint8_t* present,
@@ -1537,7 +1575,6 @@ static int seq_run(uint32_t n_chr, const uint32_t* len, const uint32_t* packed, 
         w += (len[c] + 15u) / 16u;
     }
     int rc = 0;
-    if (rows && n_ev) rc = fail("read materialisation does not support copy-number events");
     if (n_ev && ploidy > PSX_MAX_CNA_PLOIDY) rc = fail("copy-number events need a small ploidy");
     if (s->pe) rc = insert_table(p->insert_size_mean, p->insert_size_stddev, &s->ins_k0, &s->ins_n, s->ins_T);
     if (!rc && s->e > 0.0) {
@@ -1605,10 +1642,8 @@ static int seq_run(uint32_t n_chr, const uint32_t* len, const uint32_t* packed, 
             return fail("too many cell copies");
         }
     }
-    /* strata; the first read of each segment, in (output sample, chromosome) order */
+    /* strata in (output sample, chromosome, stratum) order */
     uint64_t n_strata = 0, reads = 0;
-    uint64_t* seg_base = (uint64_t*)calloc((size_t)n_out * n_chr + 1, sizeof(uint64_t));
-    s->seg_base = seg_base;
     for (uint32_t so = 0; so < n_out; ++so)
         for (uint32_t c = 0; c < n_chr; ++c)
             if (sel[c] && len[c] >= s->L) n_strata += (len[c] - s->L + 1 + PSX_STRATUM - 1) / PSX_STRATUM;
@@ -1621,13 +1656,58 @@ static int seq_run(uint32_t n_chr, const uint32_t* len, const uint32_t* packed, 
             uint32_t J = (W + PSX_STRATUM - 1) / PSX_STRATUM;
             for (uint32_t j = 0; j < J; ++j) st[q++] = (stratum_ref){so, c, j};
             uint64_t T = (uint64_t)llround(p->coverage * (double)len[c] / (double)(s->pe ? 2 * s->L : s->L));
-            seg_base[(uint64_t)so * n_chr + c] = reads;
             if (!has_ev[c]) reads += T * (s->pe ? 2u : 1u);
         }
-    if (rows && reads > capacity) {
-        free(st); free(seg_base); free(req); free(s);
-        return fail("the read buffer holds %llu reads, %llu needed", (unsigned long long)capacity,
-                    (unsigned long long)reads);
+    /* rows: segments in (output sample, chromosome) order; a segment without copy-number events holds its
+     * templates' reads in template order, one with events its accepted templates' reads in template order,
+     * stratum after stratum (a first pass counts each stratum's accepted templates) */
+    uint64_t* st_row = (uint64_t*)calloc(n_strata ? n_strata : 1, sizeof(uint64_t));
+    uint64_t row_total = 0;
+    if (rows) {
+        uint64_t* acc = (uint64_t*)calloc(n_strata ? n_strata : 1, sizeof(uint64_t));
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 8)
+#endif
+        for (int64_t qi = 0; qi < (int64_t)n_strata; ++qi) {
+            uint32_t so = st[qi].so, c = st[qi].c, j = st[qi].j;
+            if (!has_ev[c]) continue;
+            int is_normal = so == n_req;
+            const uint32_t* sl = is_normal ? NULL : f->sample_leaves + f->sample_begin[req[so]];
+            uint32_t ns = is_normal ? 0 : f->sample_begin[req[so] + 1] - f->sample_begin[req[so]];
+            uint32_t W = len[c] - s->L + 1, K = kmax[c];
+            uint64_t T = (uint64_t)llround(p->coverage * (double)len[c] / (double)(s->pe ? 2 * s->L : s->L) *
+                                           ((double)K / (double)ploidy));
+            uint32_t b = j * PSX_STRATUM, e = b + PSX_STRATUM < W ? b + PSX_STRATUM : W;
+            uint64_t nj = T * e / W - T * b / W;
+            uint32_t c2 = (c << 16) | so;
+            for (uint32_t r = 0; r < (uint32_t)nj; ++r) {
+                uint32_t w[4], leaf, allele;
+                philox4(seed, r, j, c2, (uint32_t)P_TEMPLATE << 24, w);
+                uint32_t x = b + mulhi32(w[0], e - b);
+                acc[qi] += (uint64_t)template_source(s, c, 1, K, is_normal, thr_normal, sl, ns, w, x, &leaf, &allele);
+            }
+        }
+        uint32_t rpt = s->pe ? 2u : 1u;
+        for (uint64_t qi = 0; qi < n_strata; ++qi) {
+            uint32_t so = st[qi].so, c = st[qi].c, j = st[qi].j;
+            if (j == 0) {  /* a segment's first stratum */
+                st_row[qi] = row_total;
+                if (!has_ev[c]) {
+                    uint64_t T = (uint64_t)llround(p->coverage * (double)len[c] / (double)(s->pe ? 2 * s->L : s->L));
+                    row_total += T * rpt;
+                }
+                (void)so;
+            } else {
+                st_row[qi] = has_ev[c] ? row_total : st_row[qi - 1];
+            }
+            if (has_ev[c]) row_total += acc[qi] * rpt;
+        }
+        free(acc);
+        if (row_total > capacity) {
+            free(st); free(st_row); free(req); free(s);
+            return fail("the read buffer holds %llu reads, %llu needed", (unsigned long long)capacity,
+                        (unsigned long long)row_total);
+        }
     }
     uint64_t accepted = 0;
 #ifdef _OPENMP
@@ -1651,38 +1731,14 @@ static int seq_run(uint32_t n_chr, const uint32_t* len, const uint32_t* packed, 
         int32_t* d_out = depth + (uint64_t)so * n_mut;
         uint32_t rpt = s->pe ? 2u : 1u;
         uint32_t eb[PSX_ERR_KCAP], es[PSX_ERR_KCAP], eoff[2][PSX_ERR_KCAP], esub[2][PSX_ERR_KCAP];
+        uint64_t kept = 0;  /* accepted templates of the stratum so far (copy-number events) */
         for (uint32_t r = 0; r < (uint32_t)nj; ++r) {
             uint32_t w[4];
             philox4(seed, r, j, c2, (uint32_t)P_TEMPLATE << 24, w);
             uint32_t x = b + mulhi32(w[0], e - b);
             uint32_t leaf, allele;
-            if (!has_ev[c]) {
-                if (is_normal || (uint64_t)w[1] < thr_normal) {
-                    leaf = f->n_leaves;
-                    allele = mulhi32(w[2], ploidy);
-                } else {
-                    uint32_t X = mulhi32(w[2], ns * ploidy);
-                    leaf = sl[X / ploidy];
-                    allele = X % ploidy;
-                }
-            } else if (is_normal || (uint64_t)w[1] < thr_normal) {
-                uint32_t k = mulhi32(w[2], K);
-                if (k >= ploidy) continue;  /* not a copy of the normal genome */
-                leaf = f->n_leaves;
-                allele = k;
-                accepted += s->pe ? 2u : 1u;
-            } else {
-                uint32_t X = mulhi32(w[2], ns * K);
-                uint32_t k = X % K;
-                leaf = sl[X / K];
-                uint32_t cn[PSX_MAX_CNA_PLOIDY];
-                uint32_t tot = copies(s, c, x, leaf, cn);
-                if (k >= tot) continue;  /* the cell has fewer copies at x */
-                allele = 0;
-                uint32_t cum = cn[0];
-                while (allele + 1 < ploidy && k >= cum) cum += cn[++allele];
-                accepted += s->pe ? 2u : 1u;
-            }
+            if (!template_source(s, c, has_ev[c], K, is_normal, thr_normal, sl, ns, w, x, &leaf, &allele)) continue;
+            if (has_ev[c]) accepted += s->pe ? 2u : 1u;
             /* the template's errors per mate, in draw order */
             uint32_t ne[2] = {0, 0};
             if (s->e > 0.0) {
@@ -1694,7 +1750,9 @@ static int seq_run(uint32_t n_chr, const uint32_t* len, const uint32_t* packed, 
                     ++ne[m];
                 }
             }
-            uint64_t ri = seg_base[(uint64_t)so * n_chr + c] + (T * b / W + r) * rpt;  /* read index (rows) */
+            /* read index (rows) */
+            uint64_t ri = st_row[qi] + (has_ev[c] ? kept : T * b / W + r) * rpt;
+            ++kept;
             if (s->hook) s->hook(s->hook_user, so, c, x, leaf, allele, 0, eoff[0], esub[0], ne[0]);
             else seq_read(s, c, x, leaf, allele, eoff[0], esub[0], ne[0], a_out, d_out);
             if (rows) read_row(s, c, x, leaf, allele, 0, eoff[0], esub[0], ne[0], rows + ri * s->rw);
@@ -1709,7 +1767,7 @@ static int seq_run(uint32_t n_chr, const uint32_t* len, const uint32_t* packed, 
             }
         }
     }
-    free(st); free(seg_base); free(req); free(s);
+    free(st); free(st_row); free(req); free(s);
     *n_reads = reads + accepted;
     return 0;
 }
@@ -1723,18 +1781,25 @@ int orc_simulate_seq_cna(uint32_t n_chr, const uint32_t* len, const orc_forest* 
                    n_reads);
 }
 
-int orc_simulate_reads(uint32_t n_chr, const uint32_t* len, const uint32_t* packed, const orc_forest* f,
-                       const psx_mutation* m, uint64_t n_mut, uint32_t ploidy, const psx_seq_params* p,
-                       uint64_t seed, uint32_t* rows, uint64_t capacity, uint64_t* n_reads) {
+int orc_simulate_reads_cna(uint32_t n_chr, const uint32_t* len, const uint32_t* packed, const orc_forest* f,
+                           const psx_mutation* m, uint64_t n_mut, uint32_t ploidy, const psx_cna_event* ev,
+                           uint32_t n_ev, const psx_seq_params* p, uint64_t seed, uint32_t* rows, uint64_t capacity,
+                           uint64_t* n_reads) {
     uint32_t n_out = (p->samples ? p->n_samples : f->n_samples) + (p->with_normal_sample ? 1u : 0u);
     size_t n = (size_t)n_out * (n_mut ? n_mut : 1);
     int32_t* alt = (int32_t*)malloc(n * sizeof(int32_t));
     int32_t* depth = (int32_t*)malloc(n * sizeof(int32_t));
     uint8_t* present = (uint8_t*)malloc(n);
-    int rc = seq_run(n_chr, len, packed, f, m, n_mut, ploidy, NULL, 0, p, seed, alt, depth, present, rows, capacity,
+    int rc = seq_run(n_chr, len, packed, f, m, n_mut, ploidy, ev, n_ev, p, seed, alt, depth, present, rows, capacity,
                      NULL, NULL, n_reads);
     free(alt); free(depth); free(present);
     return rc;
+}
+
+int orc_simulate_reads(uint32_t n_chr, const uint32_t* len, const uint32_t* packed, const orc_forest* f,
+                       const psx_mutation* m, uint64_t n_mut, uint32_t ploidy, const psx_seq_params* p,
+                       uint64_t seed, uint32_t* rows, uint64_t capacity, uint64_t* n_reads) {
+    return orc_simulate_reads_cna(n_chr, len, packed, f, m, n_mut, ploidy, NULL, 0, p, seed, rows, capacity, n_reads);
 }
 
 int orc_seq_walk(uint32_t n_chr, const uint32_t* len, const orc_forest* f, const psx_mutation* m, uint64_t n_mut,

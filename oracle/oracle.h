@@ -97,11 +97,16 @@ int orc_simulate_seq_cna(uint32_t n_chr, const uint32_t* len, const orc_forest* 
 
 /* simulate_seq in read-materialising form (psx_simulate_seq_reads): every
  * read into rows (capacity reads of 2 + ceil(L / 16) words each, in output
- * order: output sample, chromosome, stratum, template, mate); no copy-number
- * events. */
+ * order: output sample, chromosome, stratum, template, mate; on a chromosome
+ * with copy-number events only the accepted templates have reads, in template
+ * order). */
 int orc_simulate_reads(uint32_t n_chr, const uint32_t* len, const uint32_t* packed, const orc_forest* f,
                        const psx_mutation* m, uint64_t n_mut, uint32_t ploidy, const psx_seq_params* p,
                        uint64_t seed, uint32_t* rows, uint64_t capacity, uint64_t* n_reads);
+int orc_simulate_reads_cna(uint32_t n_chr, const uint32_t* len, const uint32_t* packed, const orc_forest* f,
+                           const psx_mutation* m, uint64_t n_mut, uint32_t ploidy, const psx_cna_event* ev,
+                           uint32_t n_ev, const psx_seq_params* p, uint64_t seed, uint32_t* rows, uint64_t capacity,
+                           uint64_t* n_reads);
 
 /* Every read of simulate_seq (no copy-number events) handed to fn instead of
  * being counted: output sample, chromosome, start, source leaf (n_leaves for

@@ -57,6 +57,9 @@ def lib():
             "orc_simulate_reads": (st, [C.c_uint32, A.u32p, A.u32p, C.POINTER(OrcForest), vp, C.c_uint64,
                                         C.c_uint32, C.POINTER(A.SeqParams), C.c_uint64, A.u32p, C.c_uint64,
                                         A.u64p]),
+            "orc_simulate_reads_cna": (st, [C.c_uint32, A.u32p, A.u32p, C.POINTER(OrcForest), vp, C.c_uint64,
+                                            C.c_uint32, vp, C.c_uint32, C.POINTER(A.SeqParams), C.c_uint64, A.u32p,
+                                            C.c_uint64, A.u64p]),
             "orc_refshape_seq": (st, [C.c_uint32, A.u32p, A.u32p, C.POINTER(OrcForest), vp, C.c_uint64, C.c_uint32,
                                       C.POINTER(A.SeqParams), C.c_uint64, C.c_int, A.i32p, A.i32p, A.u64p]),
             "orc_seq_table": (st, [vp, C.c_uint64, C.c_uint32, A.i32p, A.i32p, A.u8p, C.c_int32, A.u64p, A.u64p,
@@ -250,19 +253,23 @@ def simulate_seq(lens, forest: Forest, recs: np.ndarray, seed: int, ploidy: int 
 
 
 def simulate_reads(lens, packed, forest: Forest, recs: np.ndarray, seed: int, capacity: int, ploidy: int = 2,
-                   **seq) -> np.ndarray:
+                   cna=None, **seq) -> np.ndarray:
     """Every read of simulate_seq, materialised: (n_reads, 2 + ceil(L / 16))
-    uint32 rows in the layout and order of psx_simulate_seq_reads."""
+    uint32 rows in the layout and order of psx_simulate_seq_reads; cna:
+    copy-number events (CNA_EVENT_DTYPE, DFS order) or None."""
     L = np.ascontiguousarray(lens, np.uint32)
     P = np.ascontiguousarray(packed, np.uint32)
     r = np.ascontiguousarray(recs, A.MUTATION_DTYPE)
+    ev = np.ascontiguousarray(cna if cna is not None else np.zeros(0, A.CNA_EVENT_DTYPE), A.CNA_EVENT_DTYPE)
     params, keep = A.seq_params(**seq)
     rw = 2 + (int(seq["read_size"]) + 15) // 16
     rows = np.zeros(max(capacity, 1) * rw, np.uint32)
     nr = C.c_uint64()
-    _check(lib().orc_simulate_reads(len(L), A.ptr(L, C.c_uint32), A.ptr(P, C.c_uint32), forest.p,
-                                    r.ctypes.data_as(C.c_void_p), len(r), ploidy, C.byref(params), C.c_uint64(seed),
-                                    A.ptr(rows, C.c_uint32), capacity, C.byref(nr)))
+    _check(lib().orc_simulate_reads_cna(len(L), A.ptr(L, C.c_uint32), A.ptr(P, C.c_uint32), forest.p,
+                                        r.ctypes.data_as(C.c_void_p), len(r), ploidy,
+                                        ev.ctypes.data_as(C.c_void_p) if len(ev) else None, len(ev),
+                                        C.byref(params), C.c_uint64(seed), A.ptr(rows, C.c_uint32), capacity,
+                                        C.byref(nr)))
     del keep
     return rows[: nr.value * rw].reshape(nr.value, rw)
 

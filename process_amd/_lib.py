@@ -72,6 +72,7 @@ def _declare(lib):
                                                  A.u64p]),
         "psx_seq_vaf_f32": (st, [vp, vp, C.c_uint32, vp, vp, vp, vp]),
         "psx_read_words": (C.c_uint32, [C.POINTER(A.SeqParams)]),
+        "psx_seq_num_reads_table": (st, [vp, vp, vp, vp, C.POINTER(A.SeqParams), C.c_uint64, A.u64p]),
         "psx_simulate_seq_reads": (st, [vp, vp, vp, vp, C.POINTER(A.SeqParams), C.c_uint64, vp, C.c_uint64,
                                         A.u64p]),
         "psx_write_sam": (st, [vp, vp, vp, vp, C.POINTER(A.SeqParams), C.c_uint64, C.POINTER(A.SamParams), A.u64p]),

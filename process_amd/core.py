@@ -295,6 +295,16 @@ def num_reads(genome: Genome, forest: Forest, **seq) -> int:
     return n.value
 
 
+def num_reads_table(ctx: Context, genome: Genome, forest: Forest, muts: Mutations, seed: int, **seq) -> int:
+    """Reads a sequencing call of this table and seed simulates, copy-number
+    events included (psx_seq_num_reads_table)."""
+    p, keep = A.seq_params(**seq)
+    n = C.c_uint64()
+    check(lib().psx_seq_num_reads_table(ctx.h, genome.h, forest.h, muts.h, C.byref(p), C.c_uint64(seed),
+                                        C.byref(n)), ctx.h)
+    return n.value
+
+
 class SeqJob:
     """One job of a batched launch, with device-resident outputs."""
 
@@ -415,7 +425,7 @@ def _check_read_buffer(out: DeviceArray, rw: int) -> None:
 def simulate_seq_reads(ctx: Context, genome: Genome, forest: Forest, muts: Mutations, seed: int,
                        out: DeviceArray | None = None, **seq):
     """Materialised reads into a device buffer of shape [n_reads, read_words]."""
-    n = num_reads(genome, forest, **seq)
+    n = num_reads_table(ctx, genome, forest, muts, seed, **seq)
     rw = read_words(**seq)
     if out is None:
         out = DeviceArray(ctx, (max(n, 1), rw), np.uint32)
@@ -431,7 +441,7 @@ def simulate_seq_reads_batch(ctx: Context, jobs: list, out: DeviceArray | None =
     """Materialised reads of every job in one launch: (buffer [n_reads,
     read_words], job_first with len(jobs) + 1 entries, n_reads); job k's reads
     are rows job_first[k]:job_first[k + 1]."""
-    n = sum(num_reads(j.genome, j.forest, **j.seq) for j in jobs)
+    n = sum(num_reads_table(ctx, j.genome, j.forest, j.muts, j.seed, **j.seq) for j in jobs)
     rw = read_words(**jobs[0].seq) if jobs else 1
     if out is None:
         out = DeviceArray(ctx, (max(n, 1), rw), np.uint32)
@@ -502,7 +512,7 @@ class Server:
     sequencing requests within an HBM budget, results copied to host arrays."""
 
     def __init__(self, ctx: Context, hbm_budget: int, max_inflight: int = 2, max_batch: int = 16,
-                 launch: A.LaunchConfig | None = None, overlap_after_us: int = 0):
+                 launch: A.LaunchConfig | None = None, overlap_after_us: int = 250):
         self.ctx = ctx
         if launch is None:
             # every lane launches a full persistent grid: a launch that starts

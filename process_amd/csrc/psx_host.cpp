@@ -400,6 +400,7 @@ static void flatten_forest(const psx_forest_desc* d, psx_forest* f) {
         f->cell_id[v] = d->cell_id[i];
         f->parent[v] = prow[i] == PSX_NO_NODE ? PSX_NO_NODE : dfs[prow[i]];
         f->species[v] = d->species[i];
+        f->max_species = std::max(f->max_species, d->species[i]);
         f->birth[v] = d->birth_time[i];
         f->leaf_lo[v] = leaves_before[v];
         f->leaf_hi[v] = leaves_before[v + size[v]];

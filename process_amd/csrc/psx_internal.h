@@ -148,6 +148,7 @@ struct psx_forest {
     // host copies of the flattened forest (DFS pre-order)
     std::vector<int64_t> cell_id;
     std::vector<uint32_t> parent, species, leaf_lo, leaf_hi;
+    uint32_t max_species = 0;
     std::vector<double> birth;
     std::vector<uint32_t> sample_begin, sample_leaves;
     // device copies

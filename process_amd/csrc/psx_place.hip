@@ -210,6 +210,10 @@ struct place_args {
 
 __device__ const uint32_t SUB_TARGET[6] = {0, 2, 3, 0, 1, 2};  // C>A C>G C>T T>A T>C T>G
 
+__device__ __forceinline__ uint32_t node_epoch(const place_args& a, uint32_t v) {
+    return a.node_epoch ? a.node_epoch[v] : 0u;
+}
+
 __device__ __forceinline__ bool driver_locus(const place_args& a, uint64_t gp) {
     uint32_t lo = 0, hi = a.n_drv_pos;
     while (lo < hi) {
@@ -227,7 +231,7 @@ __device__ uint32_t draw_snv(const place_args& a, uint32_t v, uint32_t k, uint32
                              psx_mutation& r) {
     const psx_u4 w = psx_draw(a.seed, k, v, att, PSX_P_SNV << 24);
     const uint32_t sig = fixed_sig >= 0 ? (uint32_t)fixed_sig
-                                        : psx_cdf_pick(a.expo_T + (size_t)a.node_epoch[v] * a.n_sbs, a.n_sbs, w.x);
+                                        : psx_cdf_pick(a.expo_T + (size_t)node_epoch(a, v) * a.n_sbs, a.n_sbs, w.x);
     const uint32_t cls = psx_cdf_pick(a.cls_T + (size_t)sig * 96u, 96u, w.y);
     const uint32_t allele = psx_uniform32(w.z, a.ploidy);
     const uint32_t sub = cls >> 4, tgt = SUB_TARGET[sub];
@@ -280,7 +284,7 @@ __device__ uint32_t draw_indel_id(const place_args& a, uint32_t v, uint32_t k, u
                                   psx_mutation& r) {
     const psx_u4 w = psx_draw(a.seed, k, v, att, PSX_P_IDCLS << 24);
     const uint32_t sig = fixed_sig >= 0 ? (uint32_t)fixed_sig
-                                        : psx_cdf_pick(a.id_expo_T + (size_t)a.node_epoch[v] * a.n_id, a.n_id, w.x);
+                                        : psx_cdf_pick(a.id_expo_T + (size_t)node_epoch(a, v) * a.n_id, a.n_id, w.x);
     const uint32_t cls = psx_cdf_pick(a.id_cls_T + (size_t)sig * PSX_ID_CLASSES, PSX_ID_CLASSES, w.y);
     const uint32_t allele = psx_uniform32(w.z, a.ploidy);
     uint32_t c = 0, q = 0, payload = 0;
@@ -802,8 +806,8 @@ void prepare_job(psx_ctx* ctx, const psx_genome* g, place_job& J) {
     if (p->ploidy < 1 || p->ploidy > 255) psx_throw(PSX_EINVAL, "invalid ploidy");
     if (p->indel_max_len < 1 || p->indel_max_len > 10) psx_throw(PSX_EINVAL, "indel_max_len must be in [1, 10]");
     if (!(p->indel_len_p > 0.0 && p->indel_len_p <= 1.0)) psx_throw(PSX_EINVAL, "invalid indel length parameter");
-    for (uint32_t v = 0; v < f->n_nodes; ++v)
-        if (f->species[v] >= p->n_species) psx_throw(PSX_EINVAL, "node species %u has no rates", f->species[v]);
+    if (f->n_nodes && f->max_species >= p->n_species)
+        psx_throw(PSX_EINVAL, "node species %u has no rates", f->max_species);
     bool need_snv = p->n_preneoplastic_snv > 0;
     for (uint32_t sp = 0; sp < p->n_species; ++sp) need_snv |= p->snv_rate[sp] > 0.0;
     if (need_snv && (p->n_sbs == 0 || p->n_epochs == 0)) psx_throw(PSX_EINVAL, "SNVs need a signature and an exposure");
@@ -990,10 +994,13 @@ void prepare_job(psx_ctx* ctx, const psx_genome* g, place_job& J) {
                      [](const psx_cna_event& a, const psx_cna_event& b) { return a.node < b.node; });
     psx_cna_check(cna, f->n_leaves, p->ploidy, g->len);
     J.mt->cna = std::move(cna);
-    J.epoch.assign(f->n_nodes, 0);
-    for (uint32_t v = 0; v < f->n_nodes; ++v)
-        for (uint32_t e = 0; e < p->n_epochs; ++e)
-            if (p->epoch_start[e] <= f->birth[v]) J.epoch[v] = e;
+    // epoch of each node's birth (none needed with a single epoch: every node is in epoch 0)
+    if (p->n_epochs > 1) {
+        J.epoch.assign(f->n_nodes, 0);
+        for (uint32_t v = 0; v < f->n_nodes; ++v)
+            for (uint32_t e = 0; e < p->n_epochs; ++e)
+                if (p->epoch_start[e] <= f->birth[v]) J.epoch[v] = e;
+    }
 }
 
 void put_job(scratch_builder& sb, place_job& J) {
@@ -1182,7 +1189,7 @@ void place_batch(psx_ctx* ctx, const psx_genome* g, std::vector<place_job>& jobs
         pa.seed = J.seed; pa.ploidy = p->ploidy; pa.n_sbs = p->n_sbs;
         pa.expo_T = (uint64_t*)at(J.o_T) + J.expo_off; pa.cls_T = (uint64_t*)at(J.o_T) + J.cls_off;
         pa.len_T = (uint64_t*)at(J.o_T) + J.len_off; pa.len_n = p->indel_max_len;
-        pa.node_epoch = (uint32_t*)at(J.o_epoch);
+        pa.node_epoch = J.epoch.empty() ? nullptr : (uint32_t*)at(J.o_epoch);
         pa.leaf_lo = J.f->d_leaf_lo.as<uint32_t>(); pa.leaf_hi = J.f->d_leaf_hi.as<uint32_t>();
         pa.n_leaves = J.f->n_leaves; pa.n_nodes = J.f->n_nodes;
         pa.n_drv = J.drv_d.size(); pa.n_germ = p->n_germline;

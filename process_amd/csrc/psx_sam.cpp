@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "psx_internal.h"
+#include "psx_seq_host.h"
 
 namespace {
 
@@ -120,14 +121,26 @@ extern "C" psx_status psx_write_sam(psx_ctx* ctx, const psx_genome* g, const psx
             sel[p->chromosomes[k]] = 1;
         }
         std::vector<uint32_t> chrs;
-        std::vector<uint64_t> T(g->n_chr, 0);
-        uint64_t per_sample = 0, max_reads = 0;
-        for (uint32_t c = 0; c < g->n_chr; ++c) {
-            if (!sel[c] || g->len[c] < L) continue;
-            chrs.push_back(c);
-            T[c] = (uint64_t)std::llround(p->coverage * (double)g->len[c] / (double)(rpt * L));
-            per_sample += T[c];
-            max_reads = std::max<uint64_t>(max_reads, T[c] * rpt * S);
+        for (uint32_t c = 0; c < g->n_chr; ++c)
+            if (sel[c] && g->len[c] >= L) chrs.push_back(c);
+        // reads of every (output sample, chromosome) segment (kept templates with copy-number events), the
+        // first read of each in the order of one call over every selected chromosome, and the largest call
+        const std::vector<uint64_t> seg = psxh::segment_reads(ctx, psxh::job_in{g, f, m, p, seed, nullptr, nullptr,
+                                                                                nullptr});
+        std::vector<uint64_t> seg_first(seg.size(), 0);
+        uint64_t max_reads = 0;
+        {
+            uint64_t at = 0;
+            for (uint32_t so = 0; so < S; ++so)
+                for (uint32_t c : chrs) {
+                    seg_first[(size_t)so * g->n_chr + c] = at;
+                    at += seg[(size_t)so * g->n_chr + c];
+                }
+            for (uint32_t c : chrs) {
+                uint64_t n = 0;
+                for (uint32_t so = 0; so < S; ++so) n += seg[(size_t)so * g->n_chr + c];
+                max_reads = std::max(max_reads, n);
+            }
         }
         // file names first: nothing is written when one is taken (create mode)
         const std::string dir = sam->output_dir;
@@ -155,7 +168,7 @@ extern "C" psx_status psx_write_sam(psx_ctx* ctx, const psx_genome* g, const psx
         PSX_HIP(hipHostMalloc(&staging, 2 * chunk_bytes, hipHostMallocDefault));
         if (max_reads) PSX_HIP(hipMalloc((void**)&d_reads, (size_t)max_reads * rw * 4u));
         const uint32_t n_threads = std::max(1u, std::min(MAX_THREADS, std::thread::hardware_concurrency()));
-        uint64_t total = 0, before_chr = 0;
+        uint64_t total = 0;
         for (size_t ci = 0; ci < chrs.size(); ++ci) {
             const uint32_t c = chrs[ci];
             psx_seq_params pc = *p;
@@ -164,8 +177,11 @@ extern "C" psx_status psx_write_sam(psx_ctx* ctx, const psx_genome* g, const psx
             uint64_t n = 0;
             const psx_status rs = psx_simulate_seq_reads(ctx, g, f, m, &pc, seed, d_reads, max_reads, &n);
             if (rs != PSX_OK) psx_throw(rs, "%s", ctx->err.c_str());
-            if (n != T[c] * rpt * S) psx_throw(PSX_EHIP, "chromosome %u: %llu reads, expected %llu", c,
-                                               (unsigned long long)n, (unsigned long long)(T[c] * rpt * S));
+            // this call's reads: the chromosome's segments, output sample after output sample
+            std::vector<uint64_t> call_first(S + 1, 0);
+            for (uint32_t so = 0; so < S; ++so) call_first[so + 1] = call_first[so] + seg[(size_t)so * g->n_chr + c];
+            if (n != call_first[S]) psx_throw(PSX_EHIP, "chromosome %u: %llu reads, expected %llu", c,
+                                              (unsigned long long)n, (unsigned long long)call_first[S]);
             sam_file out;
             out.fh = std::fopen(paths[ci].c_str(), "w");
             if (!out.fh) psx_throw(PSX_EINVAL, "cannot open \"%s\" for writing", paths[ci].c_str());
@@ -205,12 +221,14 @@ extern "C" psx_status psx_write_sam(psx_ctx* ctx, const psx_genome* g, const psx
                         s.clear();
                         const uint64_t a = std::min<uint64_t>(cnt, t * per), b = std::min<uint64_t>(cnt, a + per);
                         s.reserve((size_t)(b - a) * (2 * L + 96));
+                        uint32_t so = 0;
                         for (uint64_t i = a; i < b; ++i) {
-                            const uint64_t r = r0 + i;              // read of this chromosome's call
-                            const uint64_t so = r / (T[c] * rpt);   // output sample
-                            const uint64_t kin = r - so * T[c] * rpt;
-                            const uint64_t tmpl = so * per_sample + before_chr + kin / rpt;
-                            const uint64_t full_row = (so * per_sample + before_chr) * rpt + kin;
+                            const uint64_t r = r0 + i;  // read of this chromosome's call
+                            while (r >= call_first[so + 1]) ++so;
+                            const uint64_t kin = r - call_first[so];
+                            // the read's index in a call over every selected chromosome, and its template's
+                            const uint64_t full_row = seg_first[(size_t)so * g->n_chr + c] + kin;
+                            const uint64_t tmpl = full_row / rpt;
                             const uint32_t* row = rows + i * rw;
                             const uint32_t* mrow = rpt == 2 ? rows + (i ^ 1u) * rw : row;
                             format_read(x, row, mrow, tmpl, full_row, sam->sample_names[so], s);
@@ -228,7 +246,6 @@ extern "C" psx_status psx_write_sam(psx_ctx* ctx, const psx_genome* g, const psx
             }
             out.fh = nullptr;
             total += n;
-            before_chr += T[c];
         }
         if (n_reads) *n_reads = total;
     });

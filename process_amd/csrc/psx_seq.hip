@@ -4,6 +4,7 @@
 // batched form. Kernels are in psx_seq_dev.h; this file builds descriptors
 // on the host, launches, and times the dominant kernel with HIP events.
 #include <algorithm>
+#include <numeric>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -580,14 +581,15 @@ extern "C" psx_status psx_simulate_seq(psx_ctx* ctx, const psx_genome* g, const 
 
 // One persistent read-materialising launch: as many blocks as fit on every CU
 // at once (capped by the work items).
-template <int BLOCK, int CAP, bool STAGE, int NW>
+template <int BLOCK, int CAP, bool STAGE, int NW, bool CNA = false>
 void launch_reads(psx_ctx* ctx, const seq_launch& L, size_t items, size_t stage) {
     int per_cu = 0;
-    PSX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_seq_reads<BLOCK, CAP, STAGE, NW>, BLOCK, stage));
+    PSX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_seq_reads<BLOCK, CAP, STAGE, NW, CNA>, BLOCK,
+                                                         stage));
     const uint64_t want = (uint64_t)ctx->n_cu * (uint64_t)std::max(per_cu, 1);
     const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, items));
     PSX_HIP(hipEventRecord(ctx->ev0, ctx->stream));
-    hipLaunchKernelGGL((k_seq_reads<BLOCK, CAP, STAGE, NW>), dim3(grid), dim3(BLOCK), stage, ctx->stream, L);
+    hipLaunchKernelGGL((k_seq_reads<BLOCK, CAP, STAGE, NW, CNA>), dim3(grid), dim3(BLOCK), stage, ctx->stream, L);
 }
 
 extern "C" uint32_t psx_read_words(const psx_seq_params* p) {
@@ -595,6 +597,84 @@ extern "C" uint32_t psx_read_words(const psx_seq_params* p) {
 }
 
 namespace {
+
+// Reads of a prepared batch in output order when some segments have
+// copy-number events: there only the kept templates have reads, so their
+// count comes from the device. Before the upload (prepare) every segment with
+// events gets the index of its first stratum and every job the device array
+// of those strata's first reads; after it (finish) the kept templates of each
+// stratum are counted (k_cn_accept), every segment's first read and each such
+// stratum's first read are set in output order, and the segments are
+// uploaded again. Without events nothing changes.
+struct cn_rows {
+    dbuf buf;                    // kept | first read (uint64 per stratum) | segment list | first strata
+    std::vector<uint32_t> seg, first;
+    uint64_t n = 0;
+    std::vector<uint64_t> seg_reads;  // reads of each segment (finish)
+    bool prepare(seq_batch& P) {
+        seg_dev* hs = (seg_dev*)(P.sb.host.data() + P.o_segs);
+        first.assign(1, 0);
+        for (uint32_t k = 0; k < (uint32_t)P.segs.size(); ++k)
+            if (hs[k].n_cna && hs[k].T) {
+                hs[k].st_off = (uint32_t)n;
+                seg.push_back(k);
+                n += hs[k].J;
+                first.push_back((uint32_t)n);
+            }
+        if (!n) return false;
+        if (n >= (uint64_t(1) << 32)) psx_throw(PSX_EINVAL, "too many strata with copy-number events");
+        buf.alloc(16 * n + 4 * (seg.size() + first.size()) + 256);
+        job_dev* hj = (job_dev*)(P.sb.host.data() + P.o_jobs);
+        for (size_t k = 0; k < P.jobs.size(); ++k) hj[k].cn_row = buf.as<uint64_t>() + n;
+        return true;
+    }
+    void finish(psx_ctx* ctx, seq_batch& P, const seq_lane& lane) {
+        uint64_t* d_kept = buf.as<uint64_t>();
+        uint32_t* d_seg = (uint32_t*)(d_kept + 2 * n);
+        uint32_t* d_first = d_seg + seg.size();
+        PSX_HIP(hipMemcpyAsync(d_seg, seg.data(), 4 * seg.size(), hipMemcpyHostToDevice, lane.stream));
+        PSX_HIP(hipMemcpyAsync(d_first, first.data(), 4 * first.size(), hipMemcpyHostToDevice, lane.stream));
+        psxk::accept_args a{(const job_dev*)(lane.desc + P.o_jobs), (const seg_dev*)(lane.desc + P.o_segs), d_seg,
+                            d_first, (uint32_t)seg.size(), d_kept};
+        hipLaunchKernelGGL(psxk::k_cn_accept, dim3((uint32_t)((n + 3) / 4)), dim3(256), 0, lane.stream, a);
+        PSX_HIP(hipGetLastError());
+        std::vector<uint64_t> kept(n), row(n);
+        PSX_HIP(hipMemcpyAsync(kept.data(), d_kept, 8 * n, hipMemcpyDeviceToHost, lane.stream));
+        PSX_HIP(hipStreamSynchronize(lane.stream));
+        // output order = the order of the first reads the segments got at preparation (job, sample, chromosome)
+        seg_dev* hs = (seg_dev*)(P.sb.host.data() + P.o_segs);
+        const job_dev* hj = (const job_dev*)(P.sb.host.data() + P.o_jobs);
+        std::vector<uint32_t> order(P.segs.size());
+        std::iota(order.begin(), order.end(), 0u);
+        std::stable_sort(order.begin(), order.end(),
+                         [&](uint32_t a2, uint32_t b2) { return hs[a2].read_base < hs[b2].read_base; });
+        std::fill(P.job_reads.begin(), P.job_reads.end(), 0);
+        seg_reads.assign(P.segs.size(), 0);
+        uint64_t at = 0;
+        for (uint32_t k : order) {
+            seg_dev& S = hs[k];
+            const uint64_t rpt = hj[S.job].pe ? 2u : 1u;
+            S.read_base = at;
+            uint64_t reads = S.T * rpt;
+            if (S.n_cna && S.T) {
+                uint64_t acc = 0;
+                for (uint32_t j = 0; j < S.J; ++j) {
+                    row[S.st_off + j] = at + acc * rpt;
+                    acc += kept[S.st_off + j];
+                }
+                reads = acc * rpt;
+            }
+            at += reads;
+            seg_reads[k] = reads;
+            P.job_reads[S.job] += reads;
+        }
+        P.n_reads = at;
+        PSX_HIP(hipMemcpyAsync(d_kept + n, row.data(), 8 * n, hipMemcpyHostToDevice, lane.stream));
+        PSX_HIP(hipMemcpyAsync(lane.desc + P.o_segs, hs, sizeof(seg_dev) * P.segs.size(), hipMemcpyHostToDevice,
+                               lane.stream));
+        PSX_HIP(hipStreamSynchronize(lane.stream));  // the host arrays above go out of scope
+    }
+};
 
 // Read materialisation of jobs in one persistent launch: job k's reads start
 // at read index sum of the earlier jobs' reads (segments keep their job-order
@@ -613,21 +693,14 @@ void run_reads(psx_ctx* ctx, const std::vector<job_in>& in, uint32_t* d_reads, u
         if (f->n_leaves >= (1u << 26))
             psx_throw(PSX_EINVAL, "read records hold leaf ranks below 2^26 (job %zu: the forest has %u leaves)", k,
                       f->n_leaves);
-        if (!m->cna.empty())
-            psx_throw(PSX_EUNSUPPORTED, "read materialisation of tables with copy-number events is not supported");
         if (psx_read_words(in[k].p) != rw)
             psx_throw(PSX_EINVAL, "jobs of one read buffer need one read size (job %zu: %u words per read, job 0: %u)",
                       k, psx_read_words(in[k].p), rw);
     }
     std::unique_ptr<seq_batch, void (*)(seq_batch*)> B(batch_prepare(ctx, in.data(), in.size(), 1), batch_free);
     seq_batch& P = *B;
-    if (P.n_reads > capacity_reads)
-        psx_throw(PSX_EINVAL, "the read buffer holds %llu reads, the run simulates %llu",
-                  (unsigned long long)capacity_reads, (unsigned long long)P.n_reads);
-    if (job_first) {
-        job_first[0] = 0;
-        for (size_t k = 0; k < P.job_reads.size(); ++k) job_first[k + 1] = job_first[k] + P.job_reads[k];
-    }
+    cn_rows cn;
+    const bool with_cn = cn.prepare(P);
     // presence flags per (output sample, mutation): the read kernel stages
     // only the SNVs a read of the sample can carry. Kernels index a job's
     // flags from its own base (so * n_mut + m), so each job gets the slice
@@ -641,6 +714,14 @@ void run_reads(psx_ctx* ctx, const std::vector<job_in>& in, uint32_t* d_reads, u
     for (size_t k = 0; k < P.jobs.size(); ++k) hj[k].present = d_present.as<uint8_t>() + pres_base[k];
     const seq_lane lane = ctx_lane(ctx, batch_desc_bytes(&P));
     batch_upload(&P, lane);
+    if (with_cn) cn.finish(ctx, P, lane);
+    if (P.n_reads > capacity_reads)
+        psx_throw(PSX_EINVAL, "the read buffer holds %llu reads, the run simulates %llu",
+                  (unsigned long long)capacity_reads, (unsigned long long)P.n_reads);
+    if (job_first) {
+        job_first[0] = 0;
+        for (size_t k = 0; k < P.job_reads.size(); ++k) job_first[k + 1] = job_first[k] + P.job_reads[k];
+    }
     seq_launch L = launch_desc(P, lane.desc, lane.queue);
     PSX_HIP(hipMemsetAsync(lane.queue, 0, 8, ctx->stream));
     if (P.pres_total) {
@@ -660,14 +741,22 @@ void run_reads(psx_ctx* ctx, const std::vector<job_in>& in, uint32_t* d_reads, u
     if (P.n_items == 0) PSX_HIP(hipEventRecord(ctx->ev0, ctx->stream));
     if (P.n_items) {
         const size_t items = P.n_items;
-        if (L.out_vec4 && L.read_words == 12)  // 145-160 bp reads
+        if (with_cn) {  // one instantiation with the copy-number path (any read size)
+            if (8 * wave_stage <= STAGE_MAX)
+                launch_reads<512, CAP, true, 0, true>(ctx, L, items, 8 * wave_stage);
+            else if (4 * wave_stage <= STAGE_MAX)
+                launch_reads<256, CAP, true, 0, true>(ctx, L, items, 4 * wave_stage);
+            else
+                launch_reads<256, CAP, false, 0, true>(ctx, L, items, 0);
+        } else if (L.out_vec4 && L.read_words == 12) {  // 145-160 bp reads
             launch_reads<512, CAP, true, 10>(ctx, L, items, 8 * wave_stage);
-        else if (8 * wave_stage <= STAGE_MAX)
+        } else if (8 * wave_stage <= STAGE_MAX) {
             launch_reads<512, CAP, true, 0>(ctx, L, items, 8 * wave_stage);
-        else if (4 * wave_stage <= STAGE_MAX)
+        } else if (4 * wave_stage <= STAGE_MAX) {
             launch_reads<256, CAP, true, 0>(ctx, L, items, 4 * wave_stage);
-        else
+        } else {
             launch_reads<256, CAP, false, 0>(ctx, L, items, 0);
+        }
     }
     PSX_HIP(hipGetLastError());
     PSX_HIP(hipEventRecord(ctx->ev1, ctx->stream));
@@ -679,6 +768,42 @@ void run_reads(psx_ctx* ctx, const std::vector<job_in>& in, uint32_t* d_reads, u
 }
 
 }  // namespace
+
+// Reads of each (output sample, chromosome) segment of one job, [n_out][n_chr]
+// (0 for chromosomes left out or shorter than a read): the closed form, or
+// with copy-number events the device count of kept templates.
+std::vector<uint64_t> psxh::segment_reads(psx_ctx* ctx, const job_in& in) {
+    std::unique_ptr<seq_batch, void (*)(seq_batch*)> B(batch_prepare(ctx, &in, 1, 1), batch_free);
+    seq_batch& P = *B;
+    cn_rows cn;
+    if (cn.prepare(P)) {
+        const seq_lane lane = ctx_lane(ctx, batch_desc_bytes(&P));
+        batch_upload(&P, lane);
+        cn.finish(ctx, P, lane);
+    }
+    const seg_dev* hs = (const seg_dev*)(P.sb.host.data() + P.o_segs);
+    const job_dev* hj = (const job_dev*)(P.sb.host.data() + P.o_jobs);
+    const uint32_t n_chr = in.g->n_chr;
+    std::vector<uint64_t> out((size_t)hj[0].n_out * n_chr, 0);
+    for (size_t k = 0; k < P.segs.size(); ++k) {
+        const seg_dev& S = hs[k];
+        out[(size_t)S.so * n_chr + S.chr] = cn.seg_reads.empty() ? S.T * (hj[0].pe ? 2u : 1u) : cn.seg_reads[k];
+    }
+    return out;
+}
+
+extern "C" psx_status psx_seq_num_reads_table(psx_ctx* ctx, const psx_genome* g, const psx_forest* f,
+                                              const psx_mutations* m, const psx_seq_params* p, uint64_t seed,
+                                              uint64_t* n_reads) {
+    if (!ctx || !g || !f || !m || !p || !n_reads) return PSX_EINVAL;
+    return psx_guard(ctx, [&] {
+        PSX_HIP(hipSetDevice(ctx->device));
+        const std::vector<uint64_t> r = psxh::segment_reads(ctx, job_in{g, f, m, p, seed, nullptr, nullptr, nullptr});
+        uint64_t n = 0;
+        for (uint64_t x : r) n += x;
+        *n_reads = n;
+    });
+}
 
 extern "C" psx_status psx_simulate_seq_reads(psx_ctx* ctx, const psx_genome* g, const psx_forest* f,
                                              const psx_mutations* m, const psx_seq_params* p, uint64_t seed,

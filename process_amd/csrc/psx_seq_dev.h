@@ -66,6 +66,7 @@ struct job_dev {
     uint32_t leaf_blocks;        // leaf_count entries per sample
     const psx_cna_dev* cna;      // copy-number events of every chromosome, per segment ranges
     unsigned long long* acc;     // reads of accepted templates (jobs with copy-number events)
+    const uint64_t* cn_row;      // read materialisation: first read of each copy-number stratum (seg st_off + j)
 };
 
 struct seg_dev {
@@ -79,6 +80,7 @@ struct seg_dev {
     uint32_t cna_off, n_cna;     // the chromosome's copy-number events (J.cna + cna_off)
     uint32_t kmax;               // copies drawn per template: the largest copy number, >= ploidy
     uint32_t first_item;         // its work items: first_item .. first_item + ceil(J / strata per item) - 1
+    uint32_t st_off;             // segments with copy-number events: index of its first stratum in J.cn_row
 };
 
 struct item_dev {
@@ -287,12 +289,14 @@ __device__ __forceinline__ uint32_t guided_pick_u(const uint32_t* T32, const uin
     return lo;
 }
 
-// Template r of stratum j (oracle: the template loop of orc_simulate_seq).
-template <bool CNA = false, class TAB>
-__device__ __forceinline__ tmpl make_template(const job_dev& J, const seg_dev& S, const TAB& tab, uint32_t r,
-                                              uint32_t j, uint32_t b, uint32_t e) {
+// Template r of stratum j without its mate (oracle: template_source): start,
+// source and, with copy-number events, whether the template is kept. w gets
+// the template draw (its word w is the insert size).
+template <bool CNA = false>
+__device__ __forceinline__ tmpl template_source(const job_dev& J, const seg_dev& S, uint32_t r, uint32_t j,
+                                                uint32_t b, uint32_t e, psx_u4& w) {
     const uint32_t c2 = (S.chr << 16) | S.so;
-    const psx_u4 w = psx_draw_u(J.seed, r, j, c2, PSX_P_TEMPLATE << 24);
+    w = psx_draw_u(J.seed, r, j, c2, PSX_P_TEMPLATE << 24);
     tmpl t;
     t.x = b + psx_uniform32(w.x, e - b);
     t.ok = 1u;
@@ -343,6 +347,15 @@ __device__ __forceinline__ tmpl make_template(const job_dev& J, const seg_dev& S
         }
     }
     t.x2 = t.x;
+    return t;
+}
+
+// Template r of stratum j (oracle: the template loop of orc_simulate_seq).
+template <bool CNA = false, class TAB>
+__device__ __forceinline__ tmpl make_template(const job_dev& J, const seg_dev& S, const TAB& tab, uint32_t r,
+                                              uint32_t j, uint32_t b, uint32_t e) {
+    psx_u4 w;
+    tmpl t = template_source<CNA>(J, S, r, j, b, e, w);
     if (J.pe) {
         uint32_t I = J.ins_k0 + guided_pick_u<TAB::GUIDE_BITS>(tab.ins, tab.guide, J.ins_n, w.w);
         if (I < J.L) I = J.L;
@@ -1270,8 +1283,10 @@ __device__ __forceinline__ void apply_error(uint32_t* row, uint32_t o, uint32_t 
 // STAGE: reads are built in LDS and copied out per pass (read_words small
 // enough for 64 rows per wave); otherwise each lane builds its read in place
 // in the output. NW > 0: reads of NW base words (rows of whole 16-byte
-// vectors, built in registers); NW = 0: any read size.
-template <int BLOCK, int CAP, bool STAGE, int NW>
+// vectors, built in registers); NW = 0: any read size. CNA: segments with
+// copy-number events keep only the templates whose copy slot is a copy of
+// their source (their reads from J.cn_row on); the others as without.
+template <int BLOCK, int CAP, bool STAGE, int NW, bool CNA = false>
 __global__ void __launch_bounds__(BLOCK) k_seq_reads(const seq_launch L) {
     constexpr uint32_t WAVES = BLOCK / 64;
     __shared__ uint32_t s_pos[CAP], s_lo[CAP], s_hi[CAP], s_info[CAP];
@@ -1281,6 +1296,7 @@ __global__ void __launch_bounds__(BLOCK) k_seq_reads(const seq_launch L) {
     __shared__ uint32_t s_item;
     __shared__ uint32_t s_cnt[WAVES];
     __shared__ uint32_t s_elist[WAVES][ELIST];  // per wave: errors of a slow template
+    __shared__ uint32_t s_kept[WAVES];          // CNA: kept templates per wave in a round
     extern __shared__ uint4 s_stage4[];  // STAGE: per wave 64 rows of read_words words
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63u, wave = tid >> 6;
@@ -1316,16 +1332,13 @@ __global__ void __launch_bounds__(BLOCK) k_seq_reads(const seq_launch L) {
             const lds_view lv{s_pos, s_lo, s_hi, s_info, nullptr, nullptr, s_bucket, n_snv, s.b};
             const glb_view gv{J.pos + s.k0, J.lo + s.k0, J.hi + s.k0, J.info + s.k0, nullptr, nullptr, s.n_win};
             const uint32_t nj = (uint32_t)s.nj;
-            for (uint32_t c0 = wave * 64u; c0 < nj; c0 += WAVES * 64u) {
-                const uint32_t rt = c0 + lane;
-                uint32_t tx = 0, tx2 = 0, tla = 0;
-                if (rt < nj) {
-                    tmpl t = make_template(J, S, s_tab, rt, j, s.b, s.e);
-                    tx = t.x;
-                    tx2 = t.x2;
-                    tla = leaf_of(J, S, t) | (t.allele << 26);
-                }
-                const terr te = J.err_n ? template_errors(J, s_tab, rt < nj ? rt : c0, j, c2, rt < nj) : terr{0u, 0u};
+            // The reads of the n_t templates held by lanes [0, n_t) (start tx, mate start tx2, leaf |
+            // allele << 26, template index tr), written in passes of 64 reads from read index first0 on.
+            auto write_group = [&](uint32_t tx, uint32_t tx2, uint32_t tla, uint32_t tr, uint32_t n_t,
+                                   uint64_t first0) {
+                const bool valid = lane < n_t;
+                const uint32_t tr0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)tr);  // lane 0 holds a template
+                const terr te = J.err_n ? template_errors(J, s_tab, valid ? tr : tr0, j, c2, valid) : terr{0u, 0u};
                 for (uint32_t p = 0; p < rpt; ++p) {
                     const uint32_t src = p * tpp + lane / rpt;
                     const uint32_t mate = lane & (rpt - 1u);
@@ -1333,27 +1346,27 @@ __global__ void __launch_bounds__(BLOCK) k_seq_reads(const seq_launch L) {
                     const uint32_t x2 = (uint32_t)__shfl((int)tx2, (int)src);
                     const uint32_t la = (uint32_t)__shfl((int)tla, (int)src);
                     const terr e = shfl_terr(te, src);
-                    const uint32_t r = c0 + src;
-                    const uint32_t t0 = c0 + p * tpp;  // first template of the pass
-                    const uint32_t nt = t0 < nj ? min(tpp, nj - t0) : 0u;
-                    const uint64_t first = S.read_base + (s.before + t0) * rpt;
+                    const uint32_t t0 = p * tpp;  // first template of the pass
+                    const uint32_t nt = t0 < n_t ? min(tpp, n_t - t0) : 0u;
+                    const uint64_t first = first0 + (uint64_t)t0 * rpt;
                     uint32_t* row = STAGE ? stage + lane * RW : L.reads_out + (first + lane) * RW;
-                    if (r < nj) {
+                    if (src < n_t) {
                         const uint32_t x = mate ? x2 : x1;
                         if (fits)
                             write_read<NW>(J, lv, x, la & 0x3FFFFFFu, la >> 26, mate, e, s_ref, ref_w0 << 4, row);
                         else
                             write_read<NW>(J, gv, x, la & 0x3FFFFFFu, la >> 26, mate, e, s_ref, ref_w0 << 4, row);
                     }
-                    // slow templates of the pass (template lanes [p tpp, p tpp + nt)): the wave
-                    // generates each one's errors and patches its reads
-                    unsigned long long sm = __ballot(te.slow() && lane >= p * tpp && lane < p * tpp + nt);
+                    // slow templates of the pass (template lanes [t0, t0 + nt)): the wave generates each
+                    // one's errors and patches its reads
+                    unsigned long long sm = __ballot(te.slow() && lane >= t0 && lane < t0 + nt);
                     while (sm) {  // wave-uniform, rare
                         const uint32_t sl = (uint32_t)__ffsll((long long)sm) - 1u;
                         sm &= sm - 1ull;
+                        const uint32_t rs = (uint32_t)__builtin_amdgcn_readlane((int)tr, (int)sl);
                         wave_sync();
                         const uint32_t n_err = wave_template_errors(J.seed, J.err_n, J.err_k0, J.err_N, s_tab.err,
-                                                                    c0 + sl, j, c2, list, lane);
+                                                                    rs, j, c2, list, lane);
                         wave_sync();  // list entries and (STAGE) every row of the pass written
                         for (uint32_t k0 = 0; k0 < n_err; k0 += 64u) {
                             const uint32_t k = k0 + lane;
@@ -1361,7 +1374,7 @@ __global__ void __launch_bounds__(BLOCK) k_seq_reads(const seq_launch L) {
                             const uint32_t b = w & 0xFFFFFu, sub = w >> 20;
                             const uint32_t m = b >= J.L ? 1u : 0u;
                             const uint32_t o = b - m * J.L;
-                            const uint32_t rl = (sl - p * tpp) * rpt + m;  // the read's lane in the pass
+                            const uint32_t rl = (sl - t0) * rpt + m;  // the read's lane in the pass
                             const bool on = k < n_err;
                             if (STAGE) {
                                 if (on) apply_error<true>(stage + rl * RW, o, sub);
@@ -1390,11 +1403,99 @@ __global__ void __launch_bounds__(BLOCK) k_seq_reads(const seq_launch L) {
                         wave_sync();
                     }
                 }
+            };
+            if (CNA && S.n_cna) {
+                // copy-number events: only the kept templates have reads, in template order. Rounds of
+                // WAVES x 64 templates: each wave counts its kept ones, the block orders the waves, and each
+                // wave moves its kept templates to its first lanes (forward lane permutation) and writes them.
+                const uint64_t row0 = uniform_at(J.cn_row, S.st_off + j);
+                uint32_t kept = 0;  // kept templates of the stratum's earlier rounds
+                for (uint32_t base = 0; base < nj; base += WAVES * 64u) {
+                    const uint32_t rt = base + wave * 64u + lane;
+                    tmpl t{};
+                    bool ok = false;
+                    if (rt < nj) {
+                        t = make_template<true>(J, S, s_tab, rt, j, s.b, s.e);
+                        ok = t.ok != 0u;
+                    }
+                    const unsigned long long bal = __ballot(ok);
+                    if (lane == 0) s_kept[wave] = (uint32_t)__popcll(bal);
+                    __syncthreads();
+                    uint32_t before = kept, total = 0;
+                    for (uint32_t w2 = 0; w2 < WAVES; ++w2) {
+                        const uint32_t c = s_kept[w2];
+                        before += w2 < wave ? c : 0u;
+                        total += c;
+                    }
+                    __syncthreads();  // every wave has read s_kept before the next round writes it
+                    kept += total;
+                    const uint32_t n_acc = (uint32_t)__popcll(bal);
+                    if (n_acc == 0) continue;
+                    const uint32_t mine = (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+                    const int to = (int)(ok ? mine : n_acc + (lane - mine)) << 2;
+                    const uint32_t tla = leaf_of(J, S, t) | (t.allele << 26);
+                    const uint32_t px = (uint32_t)__builtin_amdgcn_ds_permute(to, (int)t.x);
+                    const uint32_t px2 = (uint32_t)__builtin_amdgcn_ds_permute(to, (int)t.x2);
+                    const uint32_t pla = (uint32_t)__builtin_amdgcn_ds_permute(to, (int)tla);
+                    const uint32_t pr = (uint32_t)__builtin_amdgcn_ds_permute(to, (int)rt);
+                    write_group(px, px2, pla, pr, n_acc, row0 + (uint64_t)before * rpt);
+                }
+            } else {
+                for (uint32_t c0 = wave * 64u; c0 < nj; c0 += WAVES * 64u) {
+                    const uint32_t rt = c0 + lane;
+                    uint32_t tx = 0, tx2 = 0, tla = 0;
+                    if (rt < nj) {
+                        tmpl t = make_template(J, S, s_tab, rt, j, s.b, s.e);
+                        tx = t.x;
+                        tx2 = t.x2;
+                        tla = leaf_of(J, S, t) | (t.allele << 26);
+                    }
+                    write_group(tx, tx2, tla, rt, min(64u, nj - c0), S.read_base + (s.before + c0) * rpt);
+                }
             }
             __syncthreads();
         }
         __syncthreads();
     }
+}
+
+// Kept templates of every stratum of the segments with copy-number events
+// (the read kernel's draws, so its row order follows): one wave per stratum,
+// 64 templates at a time.
+struct accept_args {
+    const job_dev* jobs;
+    const seg_dev* segs;
+    const uint32_t* cn_seg;    // segment index of each segment with events
+    const uint32_t* cn_first;  // its first stratum (index into kept), n_seg + 1 entries
+    uint32_t n_seg;
+    uint64_t* kept;            // per stratum
+};
+
+__global__ void __launch_bounds__(256) k_cn_accept(const accept_args a) {
+    const uint32_t g = (uint32_t)__builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4u + (threadIdx.x >> 6)));
+    const uint32_t lane = threadIdx.x & 63u;
+    if (g >= uniform_at(a.cn_first, a.n_seg)) return;  // the whole wave
+    uint32_t lo = 0, hi = a.n_seg;  // the segment: the last k with cn_first[k] <= g
+    while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (uniform_at(a.cn_first, mid) <= g) lo = mid; else hi = mid;
+    }
+    const seg_dev S = uniform_at(a.segs, uniform_at(a.cn_seg, lo));
+    const job_dev J = uniform_at(a.jobs, S.job);
+    const uint32_t j = g - uniform_at(a.cn_first, lo);
+    const uint32_t b = j * PSX_STRATUM, e = min(b + PSX_STRATUM, S.W);
+    const uint64_t nj = S.T * e / S.W - S.T * b / S.W;
+    uint64_t cnt = 0;
+    for (uint64_t r0 = 0; r0 < nj; r0 += 64u) {
+        const uint64_t r = r0 + lane;
+        bool ok = false;
+        if (r < nj) {
+            psx_u4 w;
+            ok = template_source<true>(J, S, (uint32_t)r, j, b, e, w).ok != 0u;
+        }
+        cnt += (uint64_t)__popcll(__ballot(ok));
+    }
+    if (lane == 0) a.kept[g] = cnt;
 }
 
 }  // namespace psxk

@@ -51,6 +51,12 @@ void batch_upload(seq_batch* b, const seq_lane& lane);
 // timed != 0 the count kernel is bracketed by lane.ev0 / lane.ev1.
 void batch_launch(const seq_batch* b, const seq_lane& lane, const psx_launch_config& cfg, bool timed);
 
+// Reads of each (output sample, chromosome) segment of one job, [n_out][n_chr]
+// in output order (0 where a chromosome is left out or shorter than a read):
+// the closed form, or with copy-number events the kept templates counted on
+// ctx's stream (synchronises it).
+std::vector<uint64_t> segment_reads(psx_ctx* ctx, const job_in& in);
+
 struct run_result {
     uint64_t n_reads;
     double kernel_ms;

@@ -392,7 +392,7 @@ def test_cna_seq_bit_exact(ctx, cna_world, case):
 
 def test_cna_replay_and_batch(ctx, cna_world):
     """A replayed table with its events, batched with a plain job, matches the
-    oracle job by job; read materialisation refuses copy-number tables."""
+    oracle job by job; its reads are the oracle's reads."""
     say("[parity] copy-number replay + batch")
     w = cna_world
     rm = core.Mutations.upload(ctx, w.pf, w.om)
@@ -409,8 +409,44 @@ def test_cna_replay_and_batch(ctx, cna_world):
     a, d, p = jobs[1].results()
     assert np.array_equal(a, pa) and np.array_equal(d, pd)
     assert n == on + pn
-    with pytest.raises(PsxError, match="copy-number"):
-        core.simulate_seq_reads(ctx, w.g, w.pf, rm, 1, **kw)
+    rows, nr = core.simulate_seq_reads(ctx, w.g, w.pf, rm, 1, **kw)
+    assert nr == core.num_reads_table(ctx, w.g, w.pf, rm, 1, **kw)
+    assert np.array_equal(rows.numpy()[:nr], O.simulate_reads(w.lens, w.packed, w.of, w.om, 1, nr, cna=w.oev, **kw))
+
+
+@pytest.mark.parametrize("case", sorted(CNA_SEQ))
+def test_cna_reads_bit_exact(ctx, cna_world, case):
+    """Read materialisation with copy-number events: only kept templates have
+    reads, in template order; every read equals the oracle's, and their number
+    is the count-only call's."""
+    say(f"[parity] copy-number read materialisation {case}")
+    w = cna_world
+    kw = CNA_SEQ[case]
+    n = core.num_reads_table(ctx, w.g, w.pf, w.pm, 23, **kw)
+    rows, got = core.simulate_seq_reads(ctx, w.g, w.pf, w.pm, 23, **kw)
+    assert got == n
+    _, _, _, on = O.simulate_seq(w.lens, w.of, w.om, 23, cna=w.oev, **kw)
+    assert n == on
+    orows = O.simulate_reads(w.lens, w.packed, w.of, w.om, 23, n, cna=w.oev, **kw)
+    assert np.array_equal(rows.numpy()[:n], orows)
+
+
+def test_cna_reads_batched_with_plain_jobs(ctx, world, cna_world):
+    """A batched read launch of copy-number and plain jobs: each job's rows are
+    its rows alone (and the oracle's)."""
+    say("[parity] copy-number read materialisation, batched with plain jobs")
+    kw = dict(coverage=20, read_size=150, insert_size_mean=350, insert_size_stddev=10, error_rate=2e-3, purity=0.7)
+    spec = [(cna_world, 3), (world, 4), (cna_world, 5)]
+    jobs = [core.SeqJob(ctx, w.g, w.pf, w.pm, seed, **kw) for w, seed in spec]
+    out, first, n = core.simulate_seq_reads_batch(ctx, jobs)
+    got = out.numpy()[:n]
+    for k, (w, seed) in enumerate(spec):
+        alone, na = core.simulate_seq_reads(ctx, w.g, w.pf, w.pm, seed, **kw)
+        assert first[k + 1] - first[k] == na
+        assert np.array_equal(got[first[k]:first[k + 1]], alone.numpy()[:na]), k
+    w = cna_world
+    assert np.array_equal(got[first[0]:first[1]],
+                          O.simulate_reads(w.lens, w.packed, w.of, w.om, 3, first[1], cna=w.oev, **kw))
 
 
 def test_batch_mixes_sparse_and_dense_jobs(ctx, world, cna_world):

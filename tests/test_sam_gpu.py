@@ -154,3 +154,43 @@ def test_reference_genome_override(ctx, tmp_path):
     with pytest.raises(ValueError, match="chromosome names and lengths"):
         pa.simulate_seq(pf, reference_genome=pa.ReferenceGenome({"21": 40_000}, seed=1), write_SAM=True,
                         output_dir=str(tmp_path / "c"), **kw)
+
+
+def test_sam_with_copy_number_events(ctx, tmp_path):
+    """A table with copy-number drivers (amplification, deletion, doubling):
+    each chromosome's file holds the reads of the kept templates, line for
+    line (position and bases) the oracle's reads of that chromosome, with
+    templates numbered over every earlier segment's kept templates."""
+    sig = cases.signatures(1, 3)
+    ref = pa.ReferenceGenome({"1": 50_000, "2": 30_000}, seed=23)
+    eng = pa.MutationEngine(ref, {"SBS1": sig[0]}, context_sampling=1)
+    eng.add_mutant("A", {"SNV": 6e-6, "indel": 3e-7})
+    eng.add_mutant("B", {"SNV": 8e-6},
+                   drivers=[pa.Amplification("1", 20_001, 10_000, src_allele=1), pa.WGD,
+                            pa.Deletion("2", 5_001, 7_000, allele=0)])
+    eng.add_exposure({"SBS1": 1.0})
+    sf = pa.SamplesForest.synthetic(mutants={"A": (0.2, 0.01), "B": (0.4, 0.01)}, introductions={"B": 60},
+                                    n_cells=500, samples={"S1": 80, "S2": 80}, seed=12)
+    pf = eng.place_mutations(sf, 3, 0, seed=41)
+    ev = pf.mutations.cna()
+    assert len(ev) >= 3
+    out = tmp_path / "cn"
+    seqer = pa.BasicIlluminaSequencer(2e-3, random_quality_scores=False)
+    kw = dict(insert_size_mean=350, insert_size_stddev=10)
+    pa.simulate_seq(pf, sequencer=seqer, coverage=4, purity=0.8, write_SAM=True, output_dir=str(out), seed=7, **kw)
+    of = O.Forest(sf.cell_id, sf.ancestor, sf.species_index(), sf.sample_index(sf.sample_names), sf.birth_time, 2)
+    lens = eng.reference.lengths
+    seq = dict(coverage=4.0, read_size=150, error_rate=2e-3, purity=0.8, **kw)
+    n_before = 0
+    for c, cname in enumerate(eng.reference.names):
+        rows = O.simulate_reads(lens, eng._packed, of, pf.records(), 7, 10**6, cna=ev, **dict(seq, chromosomes=[c]))
+        _, body = read_sam(out / f"chr_{cname}.sam")
+        assert len(body) == len(rows) > 0
+        got = [(int(l.split("\t")[3]), l.split("\t")[9]) for l in body]
+        exp = [(int(r[0]) + 1, "".join("ACGT"[(int(r[2 + i // 16]) >> (2 * (i % 16))) & 3] for i in range(150)))
+               for r in rows]
+        assert got == exp, cname
+        n_before += len(rows)
+    # the reads of a whole-genome call, all chromosomes at once
+    _, _, _, n = O.simulate_seq(lens, of, pf.records(), 7, cna=ev, **seq)
+    assert n_before == n
