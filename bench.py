@@ -467,6 +467,7 @@ def main() -> int:
     ap.add_argument("--serve-compare-overlap-us", type=str, default="",
                     help="comma-separated overlap settings also run on the paced stream (comparison)")
     ap.add_argument("--serving-only", action="store_true", help="only run the serving workload")
+    ap.add_argument("--cna-only", action="store_true", help="only run the copy-number workload (for profiling)")
     ap.add_argument("--batched-reads-only", action="store_true",
                     help="only run the batched read materialisation (one launch per repetition; for profiling)")
     args = ap.parse_args()
@@ -488,6 +489,9 @@ def main() -> int:
         for _ in range(3):
             check(lib().psx_dev_copy(ctx.h, C.c_void_p(dst.ptr), C.c_void_p(src.ptr), args.calibrate_copy), ctx.h)
         log(f"copied {args.calibrate_copy} bytes 3 times")
+        return 0
+    if args.cna_only:
+        print(json.dumps(cna_leg(ctx, args.steps)), flush=True)
         return 0
     if args.serving_only:
         cmp = [int(x) for x in args.serve_compare_overlap_us.split(",") if x]

@@ -458,8 +458,8 @@ __device__ __forceinline__ void mark_loser(uint32_t* lose, uint32_t* list, uint3
 // First round of the infinite-sites model over footprints sorted by padded
 // position: footprints touching (gap < 2) an earlier-created mutation's
 // footprint lose; drivers and germline mutations (ids below n_fixed) never
-// lose. Every id's flag is written (the entries are a permutation); the
-// losers are appended to the redraw list with one atomic per wave.
+// lose. The losers' flags are set (the caller clears them all first) and
+// their ids appended to the redraw list with one atomic per wave.
 __global__ void __launch_bounds__(256) k_conflicts(const uint64_t* __restrict__ fkey,
                                                    const uint32_t* __restrict__ fval, uint64_t M, uint64_t n_fixed,
                                                    uint32_t* __restrict__ lose, uint32_t* __restrict__ list,
@@ -483,26 +483,37 @@ __global__ void __launch_bounds__(256) k_conflicts(const uint64_t* __restrict__ 
             l = (vi & ID_MASK) < idj && fp_touch(si, vi >> ID_BITS, sj, lj);
         }
     }
-    lose[idj] = l ? 1u : 0u;
     const uint64_t bal = __ballot(l);
     if (!bal) return;
     const uint32_t lane = threadIdx.x & 63u, leader = (uint32_t)__builtin_ctzll(bal);
     uint32_t base = 0;
     if (lane == leader) base = atomicAdd(n_lose, (uint32_t)__popcll(bal));
     base = __shfl(base, (int)leader);
-    if (l) list[base + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull))] = idj;
+    if (l) {
+        lose[idj] = 1u;
+        list[base + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull))] = idj;
+    }
 }
 
 // Mutations redrawn since the first round, with their current footprints:
-// slots [0, cap) of b are sentinels (key all ones, NO_ENTRY) until filled.
+// slots [0, cap) of b are sentinels (key all ones, NO_ENTRY) until filled;
+// one atomic per wave reserves a wave's slots.
 __global__ void __launch_bounds__(256) k_collect_moved(const psx_mutation* __restrict__ recs, uint64_t M,
                                                        const uint64_t* __restrict__ pstart,
                                                        const uint32_t* __restrict__ moved, uint64_t* __restrict__ bkey,
                                                        uint32_t* __restrict__ bval, uint32_t* __restrict__ nb) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= M || !moved[i]) return;
+    if (i >= M) return;
+    const bool mv = moved[i] != 0u;
+    const uint64_t bal = __ballot(mv);
+    if (!bal) return;
+    const uint32_t lane = threadIdx.x & 63u, leader = (uint32_t)__builtin_ctzll(bal);
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(nb, (uint32_t)__popcll(bal));
+    base = __shfl(base, (int)leader);
+    if (!mv) return;
     const psx_mutation r = recs[i];
-    const uint32_t slot = atomicAdd(nb, 1u);
+    const uint32_t slot = base + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
     bkey[slot] = padded_pos(pstart, r);
     bval[slot] = fp_val((uint32_t)i, r);
 }
@@ -1141,16 +1152,16 @@ void place_batch(psx_ctx* ctx, const psx_genome* g, std::vector<place_job>& jobs
     size_t sort_bytes = 0;
     PSX_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (uint64_t*)nullptr, (uint64_t*)nullptr,
                                                (uint32_t*)nullptr, (uint32_t*)nullptr, (int)max_m, 0, pbits, s));
-    // per-mutation temporaries from the context's placement pool: the attempt counters and moved flags of every
-    // job first (cleared together), then each job's own
+    // per-mutation temporaries from the context's placement pool: the attempt counters, moved flags and
+    // first-round loser flags of every job first (cleared together), then each job's own
     carver cm;
-    const size_t c_zero = cm.take(sum_m * 8), c_sort = cm.take(sort_bytes);
+    const size_t c_zero = cm.take(sum_m * 12), c_sort = cm.take(sort_bytes);
     std::vector<size_t> c_mut(nj);
     for (uint32_t k = 0; k < nj; ++k) {
         const uint64_t M = jobs[k].M;
         c_mut[k] = cm.bytes;
         cm.take(M * sizeof(psx_mutation));
-        for (int q = 0; q < 5; ++q) cm.take(M * 4);  // lose[2], list, ids, ids2
+        for (int q = 0; q < 4; ++q) cm.take(M * 4);  // lose[1], list, ids, ids2
         cm.take(M * 4);                             // ids3
         cm.take(jobs[k].n_pass * 4);                // node_of
         for (int q = 0; q < 3; ++q) cm.take(M * 8);  // key, key2, key3
@@ -1159,18 +1170,18 @@ void place_batch(psx_ctx* ctx, const psx_genome* g, std::vector<place_job>& jobs
     ctx->place_muts.reserve(cm.bytes);
     uint8_t* pm = (uint8_t*)ctx->place_muts.p;
     void* d_sort = pm + c_sort;
-    PSX_HIP(hipMemsetAsync(pm + c_zero, 0, sum_m * 8, s));
+    PSX_HIP(hipMemsetAsync(pm + c_zero, 0, sum_m * 12, s));
     uint64_t zero_at = 0;
     for (uint32_t k = 0; k < nj; ++k) {
         place_job& J = jobs[k];
         const uint64_t M = J.M;
         J.d_att = (uint32_t*)(pm + c_zero) + zero_at;
         J.d_moved = (uint32_t*)(pm + c_zero) + sum_m + zero_at;
+        J.d_lose[0] = (uint32_t*)(pm + c_zero) + 2 * sum_m + zero_at;
         zero_at += M;
         carver c;
         c.bytes = c_mut[k];
         psx_mutation* recs = (psx_mutation*)(pm + c.take(M * sizeof(psx_mutation)));
-        J.d_lose[0] = (uint32_t*)(pm + c.take(M * 4));
         J.d_lose[1] = (uint32_t*)(pm + c.take(M * 4));
         J.d_list = (uint32_t*)(pm + c.take(M * 4));
         J.d_ids = (uint32_t*)(pm + c.take(M * 4));

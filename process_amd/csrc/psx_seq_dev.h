@@ -1332,10 +1332,58 @@ __global__ void __launch_bounds__(BLOCK) k_seq_reads(const seq_launch L) {
             const lds_view lv{s_pos, s_lo, s_hi, s_info, nullptr, nullptr, s_bucket, n_snv, s.b};
             const glb_view gv{J.pos + s.k0, J.lo + s.k0, J.hi + s.k0, J.info + s.k0, nullptr, nullptr, s.n_win};
             const uint32_t nj = (uint32_t)s.nj;
-            // The reads of the n_t templates held by lanes [0, n_t) (start tx, mate start tx2, leaf |
-            // allele << 26, template index tr), written in passes of 64 reads from read index first0 on.
-            auto write_group = [&](uint32_t tx, uint32_t tx2, uint32_t tla, uint32_t tr, uint32_t n_t,
-                                   uint64_t first0) {
+            // Rounds of WAVES x 64 templates, 64 per wave. Without copy-number events wave w takes templates
+            // [base + 64 w, base + 64 w + 64). With them only the templates whose copy slot is a copy of their
+            // source have reads, in template order: each wave counts its kept templates, one block barrier
+            // orders the waves, and each wave moves its kept templates to its first lanes (forward lane
+            // permutation). Either way lanes [0, n_t) then hold the templates (start tx, mate start tx2,
+            // leaf | allele << 26, template index tr) whose reads start at read index first0.
+            const bool cn = CNA && S.n_cna != 0u;
+            const uint64_t row0 = cn ? uniform_at(J.cn_row, S.st_off + j) : 0ull;
+            uint32_t kept = 0;  // copy-number events: kept templates of the stratum's earlier rounds
+            for (uint32_t base = 0; base < nj; base += WAVES * 64u) {
+                const uint32_t c0 = base + wave * 64u;
+                const uint32_t rt = c0 + lane;
+                uint32_t tx = 0, tx2 = 0, tla = 0, tr = rt, n_t;
+                uint64_t first0;
+                if (cn) {
+                    tmpl t{};
+                    bool ok = false;
+                    if (rt < nj) {
+                        t = make_template<true>(J, S, s_tab, rt, j, s.b, s.e);
+                        ok = t.ok != 0u;
+                    }
+                    const unsigned long long bal = __ballot(ok);
+                    if (lane == 0) s_kept[wave] = (uint32_t)__popcll(bal);
+                    __syncthreads();
+                    uint32_t before = kept, total = 0;
+                    for (uint32_t w2 = 0; w2 < WAVES; ++w2) {
+                        const uint32_t c = s_kept[w2];
+                        before += w2 < wave ? c : 0u;
+                        total += c;
+                    }
+                    __syncthreads();  // every wave has read s_kept before the next round writes it
+                    kept += total;
+                    n_t = (uint32_t)__popcll(bal);
+                    const uint32_t mine = (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+                    const int to = (int)(ok ? mine : n_t + (lane - mine)) << 2;
+                    tla = leaf_of(J, S, t) | (t.allele << 26);
+                    tx = (uint32_t)__builtin_amdgcn_ds_permute(to, (int)t.x);
+                    tx2 = (uint32_t)__builtin_amdgcn_ds_permute(to, (int)t.x2);
+                    tla = (uint32_t)__builtin_amdgcn_ds_permute(to, (int)tla);
+                    tr = (uint32_t)__builtin_amdgcn_ds_permute(to, (int)rt);
+                    first0 = row0 + (uint64_t)before * rpt;
+                } else {
+                    n_t = c0 < nj ? min(64u, nj - c0) : 0u;
+                    if (rt < nj) {
+                        tmpl t = make_template(J, S, s_tab, rt, j, s.b, s.e);
+                        tx = t.x;
+                        tx2 = t.x2;
+                        tla = leaf_of(J, S, t) | (t.allele << 26);
+                    }
+                    first0 = S.read_base + (s.before + c0) * rpt;
+                }
+                if (n_t == 0) continue;
                 const bool valid = lane < n_t;
                 const uint32_t tr0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)tr);  // lane 0 holds a template
                 const terr te = J.err_n ? template_errors(J, s_tab, valid ? tr : tr0, j, c2, valid) : terr{0u, 0u};
@@ -1363,7 +1411,8 @@ __global__ void __launch_bounds__(BLOCK) k_seq_reads(const seq_launch L) {
                     while (sm) {  // wave-uniform, rare
                         const uint32_t sl = (uint32_t)__ffsll((long long)sm) - 1u;
                         sm &= sm - 1ull;
-                        const uint32_t rs = (uint32_t)__builtin_amdgcn_readlane((int)tr, (int)sl);
+                        // the slow template's index (contiguous without copy-number events)
+                        const uint32_t rs = cn ? (uint32_t)__builtin_amdgcn_readlane((int)tr, (int)sl) : tr0 + sl;
                         wave_sync();
                         const uint32_t n_err = wave_template_errors(J.seed, J.err_n, J.err_k0, J.err_N, s_tab.err,
                                                                     rs, j, c2, list, lane);
@@ -1402,55 +1451,6 @@ __global__ void __launch_bounds__(BLOCK) k_seq_reads(const seq_launch L) {
                         }
                         wave_sync();
                     }
-                }
-            };
-            if (CNA && S.n_cna) {
-                // copy-number events: only the kept templates have reads, in template order. Rounds of
-                // WAVES x 64 templates: each wave counts its kept ones, the block orders the waves, and each
-                // wave moves its kept templates to its first lanes (forward lane permutation) and writes them.
-                const uint64_t row0 = uniform_at(J.cn_row, S.st_off + j);
-                uint32_t kept = 0;  // kept templates of the stratum's earlier rounds
-                for (uint32_t base = 0; base < nj; base += WAVES * 64u) {
-                    const uint32_t rt = base + wave * 64u + lane;
-                    tmpl t{};
-                    bool ok = false;
-                    if (rt < nj) {
-                        t = make_template<true>(J, S, s_tab, rt, j, s.b, s.e);
-                        ok = t.ok != 0u;
-                    }
-                    const unsigned long long bal = __ballot(ok);
-                    if (lane == 0) s_kept[wave] = (uint32_t)__popcll(bal);
-                    __syncthreads();
-                    uint32_t before = kept, total = 0;
-                    for (uint32_t w2 = 0; w2 < WAVES; ++w2) {
-                        const uint32_t c = s_kept[w2];
-                        before += w2 < wave ? c : 0u;
-                        total += c;
-                    }
-                    __syncthreads();  // every wave has read s_kept before the next round writes it
-                    kept += total;
-                    const uint32_t n_acc = (uint32_t)__popcll(bal);
-                    if (n_acc == 0) continue;
-                    const uint32_t mine = (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
-                    const int to = (int)(ok ? mine : n_acc + (lane - mine)) << 2;
-                    const uint32_t tla = leaf_of(J, S, t) | (t.allele << 26);
-                    const uint32_t px = (uint32_t)__builtin_amdgcn_ds_permute(to, (int)t.x);
-                    const uint32_t px2 = (uint32_t)__builtin_amdgcn_ds_permute(to, (int)t.x2);
-                    const uint32_t pla = (uint32_t)__builtin_amdgcn_ds_permute(to, (int)tla);
-                    const uint32_t pr = (uint32_t)__builtin_amdgcn_ds_permute(to, (int)rt);
-                    write_group(px, px2, pla, pr, n_acc, row0 + (uint64_t)before * rpt);
-                }
-            } else {
-                for (uint32_t c0 = wave * 64u; c0 < nj; c0 += WAVES * 64u) {
-                    const uint32_t rt = c0 + lane;
-                    uint32_t tx = 0, tx2 = 0, tla = 0;
-                    if (rt < nj) {
-                        tmpl t = make_template(J, S, s_tab, rt, j, s.b, s.e);
-                        tx = t.x;
-                        tx2 = t.x2;
-                        tla = leaf_of(J, S, t) | (t.allele << 26);
-                    }
-                    write_group(tx, tx2, tla, rt, min(64u, nj - c0), S.read_base + (s.before + c0) * rpt);
                 }
             }
             __syncthreads();

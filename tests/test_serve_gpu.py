@@ -115,14 +115,17 @@ def test_admission_refusals(ctx, pool):
         srv.close()
 
 
-def test_arrivals_while_running(ctx, pool):
+@pytest.mark.parametrize("overlap_us", [0, 250, 100_000])
+def test_arrivals_while_running(ctx, pool, overlap_us):
     """A producer thread submits at intervals while the lanes run; every
-    request completes and matches its lone result."""
+    request completes and matches its lone result, whether a second lane
+    starts at once, after a running launch's first 250 us or (100 ms) only
+    when max_batch requests wait."""
     g, items = pool
     f, m, _ = items[1]
     seq = dict(coverage=25, read_size=150, insert_size_mean=350, insert_size_stddev=10, error_rate=1e-3)
     ref = {s: core.simulate_seq_arrays(ctx, g, f, m, s, **seq) for s in range(20, 30)}
-    srv = core.Server(ctx, 64 << 20, max_inflight=2, max_batch=8)
+    srv = core.Server(ctx, 64 << 20, max_inflight=2, max_batch=8, overlap_after_us=overlap_us)
     tickets = {}
     try:
         def produce():
