@@ -10,10 +10,13 @@
 //   k_conflicts     infinite-sites model, first round: after a sort by padded
 //                   position, a mutation touching an earlier-created one
 //                   loses; the losers are listed and redrawn (next attempt)
-//   k_conflicts_inc later rounds: only the redrawn footprints are checked,
-//                   against the first round's sorted footprints and the
-//                   moved ones (sorted again each round, a few thousand)
-//   (sort, k_sid_runs, k_gather) SID order, sid_head flags, chromosome ranges
+//   k_redraw        later rounds: the losers redrawn, each new footprint put
+//                   into a hash table of moved mutations (by 32-base bucket)
+//   k_check         only the redrawn footprints are checked, against the first
+//                   round's sorted footprints and the hash table
+//   (k_merge, k_sid_runs, k_gather) SID order: the first round's sorted
+//                   footprints without the moved ones merged with the moved
+//                   ones sorted; sid_head flags, chromosome ranges
 //
 // A call places a batch of forests (one for psx_place_mutations): each phase
 // (node counts; placement and the first check; each later round; SID order)
@@ -35,6 +38,7 @@ constexpr uint32_t SITE_TRIES = 1u << 20;
 constexpr uint32_t LIST_TRIES = 4096;  // draws of a context-index site before the class is given up as driver loci only
 constexpr uint32_t INDEL_TRIES = 1u << 20;
 constexpr int IS_ROUNDS = 256;  // checks of the infinite-sites model before placement gives up
+constexpr uint32_t SPEC_ROUNDS = 4;  // infinite-sites rounds launched between two synchronisations
 
 enum : uint32_t { ST_OK = 0, ST_NO_SITE = 1, ST_NO_INDEL_ROOM = 2 };
 
@@ -183,10 +187,7 @@ struct place_args {
     const uint64_t* node_off;
     uint64_t M;
     psx_mutation* recs;
-    uint32_t* attempt;
-    const uint32_t* redraw;  // ids to redraw (n_redraw of them); nullptr = every id
-    uint32_t n_redraw;
-    uint32_t* moved;         // set for every redrawn id
+    uint32_t* attempt;       // attempts of every id (the first is 0)
     const uint32_t* node_of; // node of each passenger
     uint32_t* status;
     // indels from ID signatures (use_id): per-epoch exposure and per-signature class tables (classes without a
@@ -360,19 +361,12 @@ __device__ uint32_t place_one(const place_args& a, uint64_t id, uint32_t att, ps
     return st;
 }
 
-// One thread per mutation, or per id of the redraw list (next attempt).
+// One thread per mutation, first attempt.
 __global__ void __launch_bounds__(256) k_place(const place_args a) {
-    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (a.redraw ? (uint64_t)a.n_redraw : a.M)) return;
-    const uint64_t id = a.redraw ? a.redraw[t] : t;
-    uint32_t att = 0;
-    if (a.redraw) {
-        att = a.attempt[id] + 1;
-        a.attempt[id] = att;
-        a.moved[id] = 1u;
-    }
+    const uint64_t id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= a.M) return;
     psx_mutation r;
-    const uint32_t st = place_one(a, id, att, r);
+    const uint32_t st = place_one(a, id, 0u, r);
     if (st != ST_OK) atomicOr(a.status, st);
     a.recs[id] = r;
 }
@@ -431,9 +425,12 @@ __global__ void __launch_bounds__(256) k_node_expand(const uint64_t* __restrict_
 // id | deleted length << 28 (ids are below 2^28).
 constexpr uint32_t ID_BITS = 28, ID_MASK = (1u << ID_BITS) - 1u, NO_ENTRY = 0xFFFFFFFFu;
 
-__device__ __forceinline__ uint32_t fp_val(uint32_t id, const psx_mutation& r) {
-    return id | ((r.kind == PSX_DEL ? (uint32_t)(r.len & 15u) : 0u) << ID_BITS);
+// deleted length: a footprint covers [start, start + length] (at most 11 bases)
+__device__ __forceinline__ uint32_t fp_len(const psx_mutation& r) {
+    return r.kind == PSX_DEL ? (uint32_t)(r.len & 15u) : 0u;
 }
+
+__device__ __forceinline__ uint32_t fp_val(uint32_t id, const psx_mutation& r) { return id | (fp_len(r) << ID_BITS); }
 
 // footprints [sa, sa + la] and [sb, sb + lb] touch: gap below 2 bases
 __device__ __forceinline__ bool fp_touch(uint64_t sa, uint32_t la, uint64_t sb, uint32_t lb) {
@@ -450,80 +447,80 @@ __global__ void __launch_bounds__(256) k_footprint(const psx_mutation* __restric
     fval[i] = fp_val((uint32_t)i, r);
 }
 
-// a new loser: its flag, and its id appended to the redraw list
-__device__ __forceinline__ void mark_loser(uint32_t* lose, uint32_t* list, uint32_t* n_lose, uint32_t id) {
-    if (atomicExch(&lose[id], 1u) == 0u) list[atomicAdd(n_lose, 1u)] = id;
+// A loser of check number `stamp` (1 for the first): lose[id] holds the
+// number of the last check it lost (cleared once per call), so every loser is
+// listed once per check without clearing flags between checks.
+__device__ __forceinline__ void mark_loser(uint32_t* lose, uint32_t stamp, uint32_t* list, uint32_t* n_lose,
+                                           uint32_t id) {
+    if (atomicExch(&lose[id], stamp) != stamp) list[atomicAdd(n_lose, 1u)] = id;
+}
+
+// Appending kernels take APPEND_ITEMS items per thread (item k of thread t of
+// block b: b * 256 * APPEND_ITEMS + t + 256 k) and reserve their output slots
+// with one atomic per block: atomics on one address serialise (about 12 ns
+// each), so one per item or per wave would cost more than the scan itself.
+constexpr uint32_t APPEND_ITEMS = 16;
+
+// The calling thread's first slot for n items (every thread of the block calls it).
+__device__ __forceinline__ uint32_t block_reserve(uint32_t n, uint32_t* counter) {
+    using scan = hipcub::BlockScan<uint32_t, 256>;
+    __shared__ typename scan::TempStorage tmp;
+    __shared__ uint32_t s_base;
+    uint32_t excl = 0, total = 0;
+    scan(tmp).ExclusiveSum(n, excl, total);
+    if (threadIdx.x == 0) s_base = total ? atomicAdd(counter, total) : 0u;
+    __syncthreads();
+    return s_base + excl;
 }
 
 // First round of the infinite-sites model over footprints sorted by padded
 // position: footprints touching (gap < 2) an earlier-created mutation's
 // footprint lose; drivers and germline mutations (ids below n_fixed) never
-// lose. The losers' flags are set (the caller clears them all first) and
-// their ids appended to the redraw list with one atomic per wave.
+// lose. The losers' stamps are set to 1 (the caller clears them all first)
+// and their ids appended to the redraw list.
+__device__ __forceinline__ bool loses(const uint64_t* __restrict__ fkey, const uint32_t* __restrict__ fval,
+                                      uint64_t M, uint64_t n_fixed, uint64_t j, uint32_t& idj) {
+    const uint32_t vj = fval[j], lj = vj >> ID_BITS;
+    idj = vj & ID_MASK;
+    if (idj < n_fixed) return false;
+    const uint64_t sj = fkey[j], ej = sj + lj;
+    for (uint64_t i = j; i-- > 0;) {
+        const uint64_t si = fkey[i];
+        if (si + 12u < sj) break;
+        const uint32_t vi = fval[i];
+        if ((vi & ID_MASK) < idj && fp_touch(si, vi >> ID_BITS, sj, lj)) return true;
+    }
+    for (uint64_t i = j + 1; i < M; ++i) {
+        const uint64_t si = fkey[i];
+        if (si > ej + 1u) break;
+        const uint32_t vi = fval[i];
+        if ((vi & ID_MASK) < idj && fp_touch(si, vi >> ID_BITS, sj, lj)) return true;
+    }
+    return false;
+}
+
 __global__ void __launch_bounds__(256) k_conflicts(const uint64_t* __restrict__ fkey,
                                                    const uint32_t* __restrict__ fval, uint64_t M, uint64_t n_fixed,
                                                    uint32_t* __restrict__ lose, uint32_t* __restrict__ list,
                                                    uint32_t* __restrict__ n_lose) {
-    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= M) return;
-    const uint32_t vj = fval[j], idj = vj & ID_MASK, lj = vj >> ID_BITS;
-    bool l = false;
-    if (idj >= n_fixed) {
-        const uint64_t sj = fkey[j], ej = sj + lj;
-        for (uint64_t i = j; i-- > 0 && !l;) {
-            const uint64_t si = fkey[i];
-            if (si + 12u < sj) break;
-            const uint32_t vi = fval[i];
-            l = (vi & ID_MASK) < idj && fp_touch(si, vi >> ID_BITS, sj, lj);
-        }
-        for (uint64_t i = j + 1; i < M && !l; ++i) {
-            const uint64_t si = fkey[i];
-            if (si > ej + 1u) break;
-            const uint32_t vi = fval[i];
-            l = (vi & ID_MASK) < idj && fp_touch(si, vi >> ID_BITS, sj, lj);
+    const uint64_t b = (uint64_t)blockIdx.x * 256u * APPEND_ITEMS + threadIdx.x;
+    uint32_t mask = 0, n = 0, ids[APPEND_ITEMS];
+#pragma unroll
+    for (uint32_t k = 0; k < APPEND_ITEMS; ++k) {
+        const uint64_t j = b + 256u * k;
+        ids[k] = 0;
+        if (j < M && loses(fkey, fval, M, n_fixed, j, ids[k])) {
+            mask |= 1u << k;
+            ++n;
         }
     }
-    const uint64_t bal = __ballot(l);
-    if (!bal) return;
-    const uint32_t lane = threadIdx.x & 63u, leader = (uint32_t)__builtin_ctzll(bal);
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(n_lose, (uint32_t)__popcll(bal));
-    base = __shfl(base, (int)leader);
-    if (l) {
-        lose[idj] = 1u;
-        list[base + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull))] = idj;
-    }
-}
-
-// Mutations redrawn since the first round, with their current footprints:
-// slots [0, cap) of b are sentinels (key all ones, NO_ENTRY) until filled;
-// one atomic per wave reserves a wave's slots.
-__global__ void __launch_bounds__(256) k_collect_moved(const psx_mutation* __restrict__ recs, uint64_t M,
-                                                       const uint64_t* __restrict__ pstart,
-                                                       const uint32_t* __restrict__ moved, uint64_t* __restrict__ bkey,
-                                                       uint32_t* __restrict__ bval, uint32_t* __restrict__ nb) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= M) return;
-    const bool mv = moved[i] != 0u;
-    const uint64_t bal = __ballot(mv);
-    if (!bal) return;
-    const uint32_t lane = threadIdx.x & 63u, leader = (uint32_t)__builtin_ctzll(bal);
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(nb, (uint32_t)__popcll(bal));
-    base = __shfl(base, (int)leader);
-    if (!mv) return;
-    const psx_mutation r = recs[i];
-    const uint32_t slot = base + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
-    bkey[slot] = padded_pos(pstart, r);
-    bval[slot] = fp_val((uint32_t)i, r);
-}
-
-__global__ void __launch_bounds__(256) k_fill_sentinel(uint64_t* __restrict__ key, uint32_t* __restrict__ val,
-                                                       uint64_t n, uint64_t sentinel) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    key[i] = sentinel;
-    val[i] = NO_ENTRY;
+    uint32_t slot = block_reserve(n, n_lose);
+#pragma unroll
+    for (uint32_t k = 0; k < APPEND_ITEMS; ++k)
+        if (mask & (1u << k)) {
+            lose[ids[k]] = 1u;
+            list[slot++] = ids[k];
+        }
 }
 
 __device__ __forceinline__ uint64_t lower_bound_u64(const uint64_t* a, uint64_t n, uint64_t x) {
@@ -535,53 +532,215 @@ __device__ __forceinline__ uint64_t lower_bound_u64(const uint64_t* a, uint64_t 
     return lo;
 }
 
-struct inc_args {
-    const uint64_t* akey;   // first-round footprints, sorted (entries of moved mutations are stale)
+__device__ __forceinline__ uint64_t upper_bound_u64(const uint64_t* a, uint64_t n, uint64_t x) {
+    uint64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (a[mid] <= x) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+// Moved mutations by the 32-base bucket of their padded start, in an
+// open-addressing table with linear probing (slots hold ids, NO_ENTRY when
+// empty; the host keeps it at most half full). Slots are never cleared while
+// rounds run: a mutation that moves again leaves its old entry behind, which
+// no longer touches anything because footprints are always read from the
+// mutation's current record.
+constexpr uint32_t BUCKET_SHIFT = 5;
+
+__device__ __forceinline__ uint32_t tab_home(uint64_t bucket, uint32_t mask) {
+    return (uint32_t)((bucket * 0x9E3779B97F4A7C15ull) >> 32) & mask;
+}
+
+__device__ __forceinline__ void tab_insert(uint32_t* tab, uint32_t mask, uint64_t start, uint32_t id) {
+    uint32_t h = tab_home(start >> BUCKET_SHIFT, mask);
+    for (uint32_t n = 0; n <= mask; ++n, h = (h + 1u) & mask)
+        if (atomicCAS(&tab[h], NO_ENTRY, id) == NO_ENTRY) return;
+}
+
+// Device state of the infinite-sites rounds of one job: rounds are launched
+// several at a time without the host in between, so each reads its loser
+// count here.
+struct round_io {
+    uint32_t status;    // placement status bits
+    uint32_t n_moved;   // mutations moved so far (entries of the moved list)
+    uint32_t tab_used;  // table entries since it was last cleared
+    uint32_t halt;      // first round that found the table too full to run (0: none)
+    uint32_t n_lose[IS_ROUNDS + 1];  // losers of check r (0: the first check)
+};
+
+struct redraw_args {
+    uint32_t round;        // r >= 1: redraws the losers of check r - 1
+    const uint32_t* list;  // losers of check r - 1
+    uint32_t* moved;       // set for every redrawn id
+    uint32_t* mlist;       // ids in the order they first moved
+    round_io* io;
+    uint32_t* tab;
+    uint32_t tab_mask;
+    const uint64_t* pstart;
+};
+
+// Round r: the losers' next attempts (grid-stride), each new footprint into
+// the table. The round does nothing after a halt, and halts when its entries
+// would fill the table beyond half.
+__global__ void __launch_bounds__(256) k_redraw(const place_args a, const redraw_args b) {
+    round_io* io = b.io;
+    const uint32_t n = io->n_lose[b.round - 1];
+    if (io->halt || n == 0) return;
+    if ((uint64_t)io->tab_used + n > (uint64_t)(b.tab_mask + 1u) / 2u) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) io->halt = b.round;
+        return;
+    }
+    for (uint32_t base = blockIdx.x * 256u; base < n; base += gridDim.x * 256u) {
+        const uint32_t t = base + threadIdx.x;
+        const bool on = t < n;
+        const uint32_t id = on ? b.list[t] : 0u;
+        const bool first = on && atomicExch(&b.moved[id], 1u) == 0u;
+        const uint32_t slot = block_reserve(first ? 1u : 0u, &io->n_moved);
+        if (first) b.mlist[slot] = id;
+        if (on) {
+            const uint32_t att = a.attempt[id] + 1u;
+            a.attempt[id] = att;
+            psx_mutation r;
+            const uint32_t st = place_one(a, id, att, r);
+            if (st != ST_OK) atomicOr(&io->status, st);
+            a.recs[id] = r;
+            tab_insert(b.tab, b.tab_mask, padded_pos(b.pstart, r), id);
+        }
+        __syncthreads();  // block_reserve's shared slot is reused by the next pass
+    }
+}
+
+// The table rebuilt (after the host cleared it) from the moved mutations'
+// current footprints; the halt, if any, is lifted.
+__global__ void __launch_bounds__(256) k_tab_fill(const uint32_t* __restrict__ mlist, uint32_t n,
+                                                  const psx_mutation* __restrict__ recs,
+                                                  const uint64_t* __restrict__ pstart, uint32_t* tab, uint32_t mask,
+                                                  round_io* io) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0) {
+        io->tab_used = n;
+        io->halt = 0;
+    }
+    if (t >= n) return;
+    const uint32_t id = mlist[t];
+    tab_insert(tab, mask, padded_pos(pstart, recs[id]), id);
+}
+
+struct check_args {
+    const uint64_t* akey;  // first-round footprints, sorted (entries of moved mutations are stale)
     const uint32_t* aval;
     uint64_t M;
-    const uint64_t* bkey;   // moved mutations' current footprints, sorted, sentinels last
-    const uint32_t* bval;
-    uint64_t nb;
+    const psx_mutation* recs;
+    const uint64_t* pstart;
+    const uint32_t* tab;
+    uint32_t tab_mask;
     const uint32_t* moved;
-    const uint32_t* changed;  // redrawn in the last redraw
+    uint32_t round;        // r >= 1: checks the footprints redrawn in round r (check r)
+    const uint32_t* list;  // losers of check r - 1, redrawn in round r
+    round_io* io;
     uint32_t* lose;
-    uint32_t* list;
-    uint32_t* n_lose;
+    uint32_t* out;         // losers of check r
 };
 
 // Later rounds: a pair of touching footprints has a member redrawn in the
 // last redraw (a touching pair left unchanged would have made its later
 // member lose and be redrawn), so only those are checked: each against the
-// first-round footprints of unmoved mutations and the moved ones. The later
-// member of every touching pair loses, as in a full pass.
-__global__ void __launch_bounds__(256) k_conflicts_inc(const inc_args a) {
-    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= a.nb) return;
-    const uint32_t vc = a.bval[t];
-    if (vc == NO_ENTRY) return;
-    const uint32_t idc = vc & ID_MASK, lc = vc >> ID_BITS;
-    if (!a.changed[idc]) return;
-    const uint64_t sc = a.bkey[t], ec = sc + lc;
-    const uint64_t from = sc >= 11u ? sc - 11u : 0u;
+// first-round footprints of unmoved mutations and the moved ones in the
+// table. The later member of every touching pair loses, as in a full pass.
+__device__ __forceinline__ void check_one(const check_args& a, uint32_t idc) {
+    const uint32_t stamp = a.round + 1u;
+    uint32_t* n_out = &a.io->n_lose[a.round];
+    const psx_mutation rc = a.recs[idc];
+    const uint64_t sc = padded_pos(a.pstart, rc);
+    const uint32_t lc = fp_len(rc);
+    const uint64_t from = sc >= 11u ? sc - 11u : 0u, to = sc + lc + 1u;  // starts of touching footprints
     bool l = false;
     for (uint64_t i = lower_bound_u64(a.akey, a.M, from); i < a.M; ++i) {
         const uint64_t si = a.akey[i];
-        if (si > ec + 1u) break;
+        if (si > to) break;
         const uint32_t vi = a.aval[i], idi = vi & ID_MASK;
         if (a.moved[idi] || !fp_touch(si, vi >> ID_BITS, sc, lc)) continue;
         if (idi < idc) l = true;
-        else mark_loser(a.lose, a.list, a.n_lose, idi);
+        else mark_loser(a.lose, stamp, a.out, n_out, idi);
     }
-    for (uint64_t i = lower_bound_u64(a.bkey, a.nb, from); i < a.nb; ++i) {
-        const uint64_t si = a.bkey[i];
-        const uint32_t vi = a.bval[i];
-        if (vi == NO_ENTRY || si > ec + 1u) break;
-        const uint32_t idi = vi & ID_MASK;
-        if (idi == idc || !fp_touch(si, vi >> ID_BITS, sc, lc)) continue;
-        if (idi < idc) l = true;
-        else mark_loser(a.lose, a.list, a.n_lose, idi);
+    for (uint64_t bk = from >> BUCKET_SHIFT; bk <= to >> BUCKET_SHIFT; ++bk) {
+        uint32_t h = tab_home(bk, a.tab_mask);
+        for (uint32_t n = 0; n <= a.tab_mask; ++n, h = (h + 1u) & a.tab_mask) {
+            const uint32_t idi = a.tab[h];
+            if (idi == NO_ENTRY) break;
+            if (idi == idc) continue;
+            const psx_mutation ri = a.recs[idi];
+            if (!fp_touch(padded_pos(a.pstart, ri), fp_len(ri), sc, lc)) continue;
+            if (idi < idc) l = true;
+            else mark_loser(a.lose, stamp, a.out, n_out, idi);
+        }
     }
-    if (l) mark_loser(a.lose, a.list, a.n_lose, idc);
+    if (l) mark_loser(a.lose, stamp, a.out, n_out, idc);
+}
+
+// Check r (grid-stride over the footprints redrawn in round r); nothing after a halt.
+__global__ void __launch_bounds__(256) k_check(const check_args a) {
+    round_io* io = a.io;
+    const uint32_t n = io->n_lose[a.round - 1];
+    if (io->halt || n == 0) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) io->tab_used += n;  // k_redraw's entries
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x)
+        check_one(a, a.list[t]);
+}
+
+// SID order after infinite-sites rounds: the moved mutations' current
+// positions (sorted next) ...
+__global__ void __launch_bounds__(256) k_moved_keys(const uint32_t* __restrict__ mlist, uint32_t n,
+                                                    const psx_mutation* __restrict__ recs,
+                                                    const uint64_t* __restrict__ pstart, uint64_t* __restrict__ key,
+                                                    uint32_t* __restrict__ ids) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const uint32_t id = mlist[t];
+    key[t] = padded_pos(pstart, recs[id]);
+    ids[t] = id;
+}
+
+// ... the first round's entries still current (a scan of these flags ranks them) ...
+struct kept_flag {
+    const uint32_t* moved;
+    __device__ __forceinline__ uint32_t operator()(uint32_t v) const { return moved[v & ID_MASK] ? 0u : 1u; }
+};
+
+struct merge_args {
+    const uint64_t* akey;  // first-round footprints, sorted
+    const uint32_t* aval;
+    uint64_t M;
+    const uint32_t* moved;
+    const uint32_t* rank;  // kept entries of akey before each one
+    const uint64_t* bkey;  // moved mutations' current positions, sorted
+    const uint32_t* bval;
+    uint64_t nb;
+    uint64_t* okey;
+    uint32_t* oid;
+};
+
+// ... and both merged into position order (one thread per entry of either,
+// ties: first-round entries first; runs of one position are put in SID
+// order afterwards). okey/oid receive M entries.
+__global__ void __launch_bounds__(256) k_merge(const merge_args a) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < a.M) {
+        const uint32_t id = a.aval[t] & ID_MASK;
+        if (a.nb && a.moved[id]) return;
+        const uint64_t k = a.akey[t];
+        const uint64_t o = (a.nb ? a.rank[t] : t) + lower_bound_u64(a.bkey, a.nb, k);
+        a.okey[o] = k;
+        a.oid[o] = id;
+    } else if (t < a.M + a.nb) {
+        const uint64_t i = t - a.M, k = a.bkey[i];
+        const uint64_t u = upper_bound_u64(a.akey, a.M, k);
+        const uint64_t o = i + (u < a.M ? a.rank[u] : a.M - a.nb);
+        a.okey[o] = k;
+        a.oid[o] = a.bval[i];
+    }
 }
 
 __global__ void __launch_bounds__(256) k_pos_key(const psx_mutation* __restrict__ recs, uint64_t M,
@@ -659,6 +818,19 @@ inline int bits_for(uint64_t x) {
 }
 
 inline uint32_t grid_of(uint64_t n) { return (uint32_t)((n + 255) / 256); }
+inline uint32_t grid_append(uint64_t n) { return (uint32_t)((n + 256 * APPEND_ITEMS - 1) / (256 * APPEND_ITEMS)); }
+
+inline uint64_t next_pow2(uint64_t x) {
+    uint64_t p = 1;
+    while (p < x) p <<= 1;
+    return p;
+}
+
+// Slots of the moved mutations' table at its largest: at most half full with
+// every mutation moved and as many entries left behind (then cleared and
+// refilled), and room for the sorted moved positions (12 bytes each) after
+// the rounds.
+inline uint64_t tab_max(uint64_t M) { return next_pow2(std::max<uint64_t>(1024, 4 * M)); }
 
 // Byte offsets of 256-aligned parts of one buffer.
 struct carver {
@@ -790,16 +962,17 @@ struct place_job {
     place_args pa{};
     uint32_t* d_cnt_snv = nullptr;
     uint64_t *d_cnt_tot = nullptr, *d_off = nullptr;
-    uint64_t *d_key = nullptr, *d_key2 = nullptr, *d_key3 = nullptr, *d_cb = nullptr;
-    uint32_t *d_ids = nullptr, *d_ids2 = nullptr, *d_ids3 = nullptr, *d_moved = nullptr, *d_list = nullptr;
-    uint32_t *d_node_of = nullptr, *d_att = nullptr;
-    uint32_t* d_lose[2] = {nullptr, nullptr};
-    uint32_t* d_io = nullptr;  // [0] status of the last placement launch, [1] losers of the last check
+    uint64_t *d_key = nullptr, *d_key2 = nullptr, *d_cb = nullptr;
+    uint32_t *d_ids = nullptr, *d_ids2 = nullptr, *d_moved = nullptr, *d_mlist = nullptr, *d_rank = nullptr;
+    uint32_t *d_node_of = nullptr, *d_att = nullptr, *d_lose = nullptr, *d_tab = nullptr;
+    uint32_t* d_list[2] = {nullptr, nullptr};
+    round_io* d_io = nullptr;
     uint8_t* d_head = nullptr;
-    // infinite-sites rounds
-    uint32_t n_lose = 0;
-    uint64_t moved_cap = 0;
-    int cur = 0;
+    // infinite-sites rounds: the next round to run (its losers, of check round - 1, are in
+    // d_list[(round - 1) & 1]), the end of the last launched ones, moved mutations, the table's slots
+    uint32_t round = 1, spec_end = 1, n_lose = 0, n_moved = 0;
+    uint64_t tab_slots = 0;
+    bool halted = false;
 };
 
 [[noreturn]] void job_throw(const place_job& J, uint32_t n_jobs, psx_status st, const std::string& msg) {
@@ -1023,8 +1196,8 @@ void put_job(scratch_builder& sb, place_job& J) {
     J.o_drvd = sb.put_vec(J.drv_d); J.o_drvv = sb.put_vec(J.drv_v);
     J.o_drivers = sb.put(p->drivers, sizeof(psx_driver) * p->n_drivers);
     J.o_germ = sb.put(p->germline, sizeof(psx_germline) * p->n_germline);
-    const uint32_t zero[2] = {0, 0};
-    J.o_io = sb.put(zero, sizeof zero);
+    const round_io zero{};
+    J.o_io = sb.put(&zero, sizeof zero);
     if (J.use_id) {
         J.o_idc = sb.put(J.idx->count, sizeof J.idx->count);
         J.o_idl = sb.put(J.idx->listed, sizeof J.idx->listed);
@@ -1073,9 +1246,10 @@ void place_batch(psx_ctx* ctx, const psx_genome* g, std::vector<place_job>& jobs
     auto at = [&](size_t off) { return (void*)(base + off); };
     const uint64_t* d_pstart = (const uint64_t*)at(o_pstart);
 
-    // pinned readback: per job {last offset, last count, status | losers << 32, chromosome ranges}, then the
-    // checked reference bases
-    const size_t io_job = 3 + (PSX_MAX_CHROMOSOMES + 1);
+    // pinned readback: per job {last offset, last count, round state, chromosome ranges}, then the checked
+    // reference bases
+    constexpr size_t RIO = (sizeof(round_io) + 7) / 8;
+    const size_t io_job = 2 + RIO + (PSX_MAX_CHROMOSOMES + 1);
     uint64_t* io = (uint64_t*)psx_host_io(ctx, 8 * io_job * nj + check_pos.size() + 8);
     uint8_t* io_bases = (uint8_t*)(io + io_job * nj);
 
@@ -1149,11 +1323,15 @@ void place_batch(psx_ctx* ctx, const psx_genome* g, std::vector<place_job>& jobs
         max_m = std::max(max_m, J.M);
         sum_m += J.M;
     }
-    size_t sort_bytes = 0;
+    size_t sort_bytes = 0, rank_bytes = 0;
     PSX_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (uint64_t*)nullptr, (uint64_t*)nullptr,
                                                (uint32_t*)nullptr, (uint32_t*)nullptr, (int)max_m, 0, pbits, s));
+    PSX_HIP(hipcub::DeviceScan::ExclusiveSum(
+        nullptr, rank_bytes, hipcub::TransformInputIterator<uint32_t, kept_flag, const uint32_t*>(nullptr, kept_flag{}),
+        (uint32_t*)nullptr, (int)max_m, s));
+    sort_bytes = std::max(sort_bytes, rank_bytes);
     // per-mutation temporaries from the context's placement pool: the attempt counters, moved flags and
-    // first-round loser flags of every job first (cleared together), then each job's own
+    // loser stamps of every job first (cleared together), then each job's own
     carver cm;
     const size_t c_zero = cm.take(sum_m * 12), c_sort = cm.take(sort_bytes);
     std::vector<size_t> c_mut(nj);
@@ -1161,10 +1339,10 @@ void place_batch(psx_ctx* ctx, const psx_genome* g, std::vector<place_job>& jobs
         const uint64_t M = jobs[k].M;
         c_mut[k] = cm.bytes;
         cm.take(M * sizeof(psx_mutation));
-        for (int q = 0; q < 4; ++q) cm.take(M * 4);  // lose[1], list, ids, ids2
-        cm.take(M * 4);                             // ids3
+        for (int q = 0; q < 6; ++q) cm.take(M * 4);  // list[2], ids, ids2, mlist, rank
         cm.take(jobs[k].n_pass * 4);                // node_of
-        for (int q = 0; q < 3; ++q) cm.take(M * 8);  // key, key2, key3
+        for (int q = 0; q < 2; ++q) cm.take(M * 8);  // key, key2
+        cm.take(tab_max(M) * 4);                    // table
         cm.take(M);                                 // head
     }
     ctx->place_muts.reserve(cm.bytes);
@@ -1177,22 +1355,23 @@ void place_batch(psx_ctx* ctx, const psx_genome* g, std::vector<place_job>& jobs
         const uint64_t M = J.M;
         J.d_att = (uint32_t*)(pm + c_zero) + zero_at;
         J.d_moved = (uint32_t*)(pm + c_zero) + sum_m + zero_at;
-        J.d_lose[0] = (uint32_t*)(pm + c_zero) + 2 * sum_m + zero_at;
+        J.d_lose = (uint32_t*)(pm + c_zero) + 2 * sum_m + zero_at;
         zero_at += M;
         carver c;
         c.bytes = c_mut[k];
         psx_mutation* recs = (psx_mutation*)(pm + c.take(M * sizeof(psx_mutation)));
-        J.d_lose[1] = (uint32_t*)(pm + c.take(M * 4));
-        J.d_list = (uint32_t*)(pm + c.take(M * 4));
+        J.d_list[0] = (uint32_t*)(pm + c.take(M * 4));
+        J.d_list[1] = (uint32_t*)(pm + c.take(M * 4));
         J.d_ids = (uint32_t*)(pm + c.take(M * 4));
         J.d_ids2 = (uint32_t*)(pm + c.take(M * 4));
-        J.d_ids3 = (uint32_t*)(pm + c.take(M * 4));
+        J.d_mlist = (uint32_t*)(pm + c.take(M * 4));
+        J.d_rank = (uint32_t*)(pm + c.take(M * 4));
         J.d_node_of = (uint32_t*)(pm + c.take(J.n_pass * 4));
         J.d_key = (uint64_t*)(pm + c.take(M * 8));
         J.d_key2 = (uint64_t*)(pm + c.take(M * 8));
-        J.d_key3 = (uint64_t*)(pm + c.take(M * 8));
+        J.d_tab = (uint32_t*)(pm + c.take(tab_max(M) * 4));
         J.d_head = pm + c.take(M);
-        J.d_io = (uint32_t*)at(J.o_io);
+        J.d_io = (round_io*)at(J.o_io);
         const psx_engine_params* p = J.p;
         place_args& pa = J.pa;
         pa.g = gdev{g->packed.as<uint32_t>(), (uint32_t*)at(o_len), (uint64_t*)at(o_woff), (uint64_t*)at(o_boff), G,
@@ -1209,8 +1388,7 @@ void place_batch(psx_ctx* ctx, const psx_genome* g, std::vector<place_job>& jobs
         pa.drivers = (psx_driver*)at(J.o_drivers); pa.germ = (psx_germline*)at(J.o_germ);
         pa.pre_sig = p->preneoplastic_sbs;
         pa.cnt_snv = J.d_cnt_snv; pa.node_off = J.d_off; pa.node_of = J.d_node_of;
-        pa.M = M; pa.recs = recs; pa.attempt = J.d_att;
-        pa.redraw = nullptr; pa.moved = J.d_moved; pa.status = J.d_io;
+        pa.M = M; pa.recs = recs; pa.attempt = J.d_att; pa.status = &J.d_io->status;
         if (J.sidx) {
             pa.sbs_list = J.sidx->d_list.as<uint32_t>();
             pa.sbs_kept = J.sidx->d_meta.as<uint64_t>();
@@ -1240,74 +1418,13 @@ void place_batch(psx_ctx* ctx, const psx_genome* g, std::vector<place_job>& jobs
             PSX_HIP(hipGetLastError());
             PSX_HIP(hipcub::DeviceRadixSort::SortPairs(d_sort, sort_bytes, J.d_key, J.d_key2, J.d_ids, J.d_ids2,
                                                        (int)M, 0, pbits, s));
-            hipLaunchKernelGGL(k_conflicts, dim3(grid_of(M)), dim3(256), 0, s, J.d_key2, J.d_ids2, M, J.n_fixed,
-                               J.d_lose[0], J.d_list, J.d_io + 1);
+            hipLaunchKernelGGL(k_conflicts, dim3(grid_append(M)), dim3(256), 0, s, J.d_key2, J.d_ids2, M, J.n_fixed,
+                               J.d_lose, J.d_list[0], &J.d_io->n_lose[0]);
             PSX_HIP(hipGetLastError());
         }
-        PSX_HIP(hipMemcpyAsync(io + io_job * k + 2, J.d_io, 8, hipMemcpyDeviceToHost, s));
+        PSX_HIP(hipMemcpyAsync(io + io_job * k + 2, J.d_io, sizeof(round_io), hipMemcpyDeviceToHost, s));
     }
-    PSX_HIP(hipStreamSynchronize(s));
-    // status of the last placement launch and the losers of the last check of every job
-    auto take_status = [&](place_job& J) {
-        const uint64_t v = io[io_job * J.k + 2];
-        const uint32_t stv = (uint32_t)v;
-        if (stv & ST_NO_SITE) job_throw(J, nj, PSX_EINVAL, "no reference site matches a drawn SBS or ID class");
-        if (stv & ST_NO_INDEL_ROOM) job_throw(J, nj, PSX_EINVAL, "no room for an indel");
-        J.n_lose = (uint32_t)(v >> 32);
-    };
-    for (place_job& J : jobs) {
-        take_status(J);
-        J.moved_cap = J.n_lose;
-    }
-    // ---- phase 3: later infinite-sites rounds, every job with losers in each
-    for (int round = 1;; ++round) {
-        bool any = false;
-        for (const place_job& J : jobs) any |= J.n_lose != 0;
-        if (!any) break;
-        if (round >= IS_ROUNDS) {
-            for (const place_job& J : jobs)
-                if (J.n_lose) job_throw(J, nj, PSX_EINVAL, "infinite-sites placement did not converge");
-        }
-        for (place_job& J : jobs) {
-            if (!J.n_lose) continue;
-            const uint64_t M = J.M;
-            place_args pr = J.pa;
-            pr.redraw = J.d_list;
-            pr.n_redraw = J.n_lose;
-            hipLaunchKernelGGL(k_place, dim3(grid_of(J.n_lose)), dim3(256), 0, s, pr);
-            PSX_HIP(hipGetLastError());
-            // the moved mutations' current footprints, sorted; sentinels fill the unused slots
-            const uint64_t nb = std::min<uint64_t>(J.moved_cap, M);
-            hipLaunchKernelGGL(k_fill_sentinel, dim3(grid_of(nb)), dim3(256), 0, s, J.d_key, J.d_ids, nb,
-                               (pbits >= 64 ? ~0ull : (1ull << pbits) - 1ull));
-            PSX_HIP(hipGetLastError());
-            PSX_HIP(hipMemsetAsync(J.d_io + 1, 0, 4, s));
-            hipLaunchKernelGGL(k_collect_moved, dim3(grid_of(M)), dim3(256), 0, s, J.pa.recs, M, d_pstart, J.d_moved,
-                               J.d_key, J.d_ids, J.d_io + 1);
-            PSX_HIP(hipGetLastError());
-            PSX_HIP(hipcub::DeviceRadixSort::SortPairs(d_sort, sort_bytes, J.d_key, J.d_key3, J.d_ids, J.d_ids3,
-                                                       (int)nb, 0, pbits, s));
-            PSX_HIP(hipMemsetAsync(J.d_lose[J.cur ^ 1], 0, M * 4, s));
-            PSX_HIP(hipMemsetAsync(J.d_io + 1, 0, 4, s));
-            inc_args ia{};
-            ia.akey = J.d_key2; ia.aval = J.d_ids2; ia.M = M;
-            ia.bkey = J.d_key3; ia.bval = J.d_ids3; ia.nb = nb;
-            ia.moved = J.d_moved; ia.changed = J.d_lose[J.cur]; ia.lose = J.d_lose[J.cur ^ 1]; ia.list = J.d_list;
-            ia.n_lose = J.d_io + 1;
-            hipLaunchKernelGGL(k_conflicts_inc, dim3(grid_of(nb)), dim3(256), 0, s, ia);
-            PSX_HIP(hipGetLastError());
-            PSX_HIP(hipMemcpyAsync(io + io_job * J.k + 2, J.d_io, 8, hipMemcpyDeviceToHost, s));
-        }
-        PSX_HIP(hipStreamSynchronize(s));
-        for (place_job& J : jobs) {
-            if (!J.n_lose) continue;
-            take_status(J);
-            J.moved_cap += J.n_lose;
-            J.cur ^= 1;
-        }
-    }
-    // ---- phase 4: SID order (a stable sort by padded position, each run of one position sorted by the rest of
-    // the SID) into the tables, which share one block
+    // the tables' block, allocated while the GPU runs the placement
     size_t block_bytes = 0;
     std::vector<size_t> t_off(nj);
     for (uint32_t k = 0; k < nj; ++k) {
@@ -1316,6 +1433,69 @@ void place_batch(psx_ctx* ctx, const psx_genome* g, std::vector<place_job>& jobs
     }
     auto block = std::make_shared<dbuf>();
     block->alloc(block_bytes);
+    PSX_HIP(hipStreamSynchronize(s));
+    // status of the placement launches, the moved mutations and the losers of the last check run
+    auto rio = [&](const place_job& J) { return (const round_io*)(io + io_job * J.k + 2); };
+    auto take_status = [&](place_job& J) {
+        const round_io* x = rio(J);
+        if (x->status & ST_NO_SITE) job_throw(J, nj, PSX_EINVAL, "no reference site matches a drawn SBS or ID class");
+        if (x->status & ST_NO_INDEL_ROOM) job_throw(J, nj, PSX_EINVAL, "no room for an indel");
+        J.n_moved = x->n_moved;
+        if (x->halt) {  // resumed at the halted round once the table is rebuilt
+            J.round = x->halt;
+            J.halted = true;
+        } else {
+            J.round = J.spec_end;
+            J.halted = false;
+        }
+        J.n_lose = x->n_lose[J.round - 1];
+    };
+    for (place_job& J : jobs) take_status(J);
+    // ---- phase 3: later infinite-sites rounds, SPEC_ROUNDS of them launched for every job with losers
+    // between synchronisations (rounds after the last loser do nothing)
+    for (;;) {
+        bool any = false;
+        for (const place_job& J : jobs) any |= J.n_lose != 0;
+        if (!any) break;
+        for (place_job& J : jobs) {
+            if (!J.n_lose) continue;
+            if (J.round >= (uint32_t)IS_ROUNDS) job_throw(J, nj, PSX_EINVAL, "infinite-sites placement did not converge");
+            if (!J.tab_slots || J.halted) {
+                // cleared, sized for a few rounds of this many losers, refilled with the moved footprints
+                J.tab_slots = std::min(tab_max(J.M),
+                                       next_pow2(std::max<uint64_t>(1024, 8 * ((uint64_t)J.n_moved + J.n_lose))));
+                PSX_HIP(hipMemsetAsync(J.d_tab, 0xFF, J.tab_slots * 4, s));
+                hipLaunchKernelGGL(k_tab_fill, dim3(std::max<uint32_t>(1, grid_of(J.n_moved))), dim3(256), 0, s,
+                                   J.d_mlist, J.n_moved, J.pa.recs, d_pstart, J.d_tab, (uint32_t)(J.tab_slots - 1),
+                                   J.d_io);
+                PSX_HIP(hipGetLastError());
+            }
+            const uint32_t grid = std::min<uint32_t>(1024, grid_of(J.n_lose));
+            J.spec_end = std::min<uint32_t>(J.round + SPEC_ROUNDS, IS_ROUNDS);
+            for (uint32_t r = J.round; r < J.spec_end; ++r) {
+                redraw_args rd{};
+                rd.round = r; rd.list = J.d_list[(r - 1) & 1];
+                rd.moved = J.d_moved; rd.mlist = J.d_mlist; rd.io = J.d_io;
+                rd.tab = J.d_tab; rd.tab_mask = (uint32_t)(J.tab_slots - 1); rd.pstart = d_pstart;
+                hipLaunchKernelGGL(k_redraw, dim3(grid), dim3(256), 0, s, J.pa, rd);
+                PSX_HIP(hipGetLastError());
+                check_args ca{};
+                ca.akey = J.d_key2; ca.aval = J.d_ids2; ca.M = J.M;
+                ca.recs = J.pa.recs; ca.pstart = d_pstart; ca.tab = J.d_tab; ca.tab_mask = rd.tab_mask;
+                ca.moved = J.d_moved; ca.round = r; ca.list = J.d_list[(r - 1) & 1]; ca.io = J.d_io;
+                ca.lose = J.d_lose; ca.out = J.d_list[r & 1];
+                hipLaunchKernelGGL(k_check, dim3(grid), dim3(256), 0, s, ca);
+                PSX_HIP(hipGetLastError());
+            }
+            PSX_HIP(hipMemcpyAsync((void*)rio(J), J.d_io, sizeof(round_io), hipMemcpyDeviceToHost, s));
+        }
+        PSX_HIP(hipStreamSynchronize(s));
+        for (place_job& J : jobs)
+            if (J.n_lose) take_status(J);
+    }
+    // ---- phase 4: SID order (position order, each run of one position sorted by the rest of the SID) into the
+    // tables, which share one block. With the infinite-sites model the first check's sorted footprints are in
+    // position order but for the moved mutations: those are sorted alone and merged in; else a stable sort.
     for (uint32_t k = 0; k < nj; ++k) {
         place_job& J = jobs[k];
         psx_mutations* mt = J.mt;
@@ -1325,27 +1505,53 @@ void place_batch(psx_ctx* ctx, const psx_genome* g, std::vector<place_job>& jobs
         mt->n_leaves = J.f->n_leaves;
         mt->chr_len = g->len;
         psx_mutations_carve(mt, block, t_off[k]);
-        if (M) {
+        uint64_t* key = J.d_key2;  // positions in SID order
+        uint32_t* ids = J.d_ids2;
+        if (J.p->infinite_sites && M > 1) {
+            merge_args ma{};
+            ma.akey = J.d_key2; ma.aval = J.d_ids2; ma.M = M; ma.moved = J.d_moved; ma.nb = J.n_moved;
+            ma.okey = J.d_key; ma.oid = J.d_ids;
+            if (J.n_moved) {
+                // the sorted moved positions go into the table's memory, free after the rounds
+                uint64_t* bkey = (uint64_t*)J.d_tab;
+                uint32_t* bval = (uint32_t*)(bkey + J.n_moved);
+                hipLaunchKernelGGL(k_moved_keys, dim3(grid_of(J.n_moved)), dim3(256), 0, s, J.d_mlist, J.n_moved,
+                                   J.pa.recs, d_pstart, J.d_key, J.d_ids);
+                PSX_HIP(hipGetLastError());
+                PSX_HIP(hipcub::DeviceRadixSort::SortPairs(d_sort, sort_bytes, J.d_key, bkey, J.d_ids, bval,
+                                                           (int)J.n_moved, 0, pbits, s));
+                PSX_HIP(hipcub::DeviceScan::ExclusiveSum(
+                    d_sort, sort_bytes,
+                    hipcub::TransformInputIterator<uint32_t, kept_flag, const uint32_t*>(J.d_ids2, kept_flag{J.d_moved}),
+                    J.d_rank, (int)M, s));
+                ma.rank = J.d_rank; ma.bkey = bkey; ma.bval = bval;
+            }
+            hipLaunchKernelGGL(k_merge, dim3(grid_of(M + J.n_moved)), dim3(256), 0, s, ma);
+            PSX_HIP(hipGetLastError());
+            key = J.d_key;
+            ids = J.d_ids;
+        } else if (M) {
             hipLaunchKernelGGL(k_pos_key, dim3(grid_of(M)), dim3(256), 0, s, J.pa.recs, M, d_pstart, J.d_key, J.d_ids);
             PSX_HIP(hipGetLastError());
             PSX_HIP(hipcub::DeviceRadixSort::SortPairs(d_sort, sort_bytes, J.d_key, J.d_key2, J.d_ids, J.d_ids2,
                                                        (int)M, 0, pbits, s));
-            hipLaunchKernelGGL(k_sid_runs, dim3(grid_of(M)), dim3(256), 0, s, J.pa.recs, J.d_key2, J.d_ids2, M,
-                               J.d_head);
+        }
+        if (M) {
+            hipLaunchKernelGGL(k_sid_runs, dim3(grid_of(M)), dim3(256), 0, s, J.pa.recs, key, ids, M, J.d_head);
             PSX_HIP(hipGetLastError());
-            hipLaunchKernelGGL(k_gather, dim3(grid_of(M)), dim3(256), 0, s, J.pa.recs, J.d_ids2, J.d_head, M, mt->recs);
+            hipLaunchKernelGGL(k_gather, dim3(grid_of(M)), dim3(256), 0, s, J.pa.recs, ids, J.d_head, M, mt->recs);
             PSX_HIP(hipGetLastError());
         }
-        hipLaunchKernelGGL(k_chr_begin, dim3(1), dim3(128), 0, s, J.d_key2, M, d_pstart, g->n_chr, mt->d_chr_begin);
+        hipLaunchKernelGGL(k_chr_begin, dim3(1), dim3(128), 0, s, key, M, d_pstart, g->n_chr, mt->d_chr_begin);
         PSX_HIP(hipGetLastError());
-        PSX_HIP(hipMemcpyAsync(io + io_job * k + 3, mt->d_chr_begin, 8 * (PSX_MAX_CHROMOSOMES + 1),
+        PSX_HIP(hipMemcpyAsync(io + io_job * k + 2 + RIO, mt->d_chr_begin, 8 * (PSX_MAX_CHROMOSOMES + 1),
                                hipMemcpyDeviceToHost, s));
         psx_mutations_finish_device(ctx, mt, false);
     }
     PSX_HIP(hipEventRecord(ctx->ev1, s));
     PSX_HIP(hipStreamSynchronize(s));
     for (uint32_t k = 0; k < nj; ++k)
-        jobs[k].mt->chr_begin.assign(io + io_job * k + 3, io + io_job * k + 3 + (PSX_MAX_CHROMOSOMES + 1));
+        jobs[k].mt->chr_begin.assign(io + io_job * k + 2 + RIO, io + io_job * k + 2 + RIO + (PSX_MAX_CHROMOSOMES + 1));
     float ms = 0.f;
     PSX_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     ctx->last_place_ms = ms;

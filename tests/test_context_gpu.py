@@ -15,6 +15,7 @@ import process_amd as pa
 from oracle import oracle as O
 from process_amd import _abi as A
 from process_amd import core
+from process_amd._lib import PsxError
 
 pytestmark = pytest.mark.gpu
 
@@ -72,6 +73,35 @@ def test_dense_infinite_sites_rounds(ctx):
         order = np.argsort(start, kind="stable")
         start, end = start[order], end[order]
         assert (start[1:] > end[:-1] + 1).all()
+
+
+@pytest.mark.parametrize("n_snv", [1150, 1200])
+def test_saturated_genome_rounds(ctx, n_snv):
+    """SNVs on 3000 bases near the packing limit (positions 2 apart): the
+    losers of one check stay many for many rounds, so the moved mutations
+    fill the table repeatedly and it is rebuilt between launched rounds.
+    1150 converges (the table equals the oracle's); 1200 does not within the
+    checks allowed, on both sides."""
+    lens = [3000]
+    g = core.Genome(ctx, lens, seed=4)
+    packed = g.download()
+    fd = cases.forest(n_cells=200, n_samples=1, sample_size=50, seed=3, two_species=False)
+    f = core.Forest(ctx, fd["cell_id"], fd["parent"], fd["species"], fd["sample"], fd["birth"], fd["n_samples"])
+    of = O.Forest(fd["cell_id"], fd["parent"], fd["species"], fd["sample"], fd["birth"], fd["n_samples"])
+    eng = cases.engine(packed, lens, snv=1e-9, indel=1e-10, pre_snv=n_snv, pre_indel=0, drivers=False,
+                       germline=False)
+    eng["context_sampling"] = 1
+    try:
+        om = O.place_mutations(lens, packed, of, 5, **eng)
+    except O.OracleError as e:
+        assert "did not converge" in str(e)
+        with pytest.raises(PsxError, match="infinite-sites placement did not converge"):
+            core.Mutations.place(ctx, g, f, 5, **eng)
+        return
+    m = core.Mutations.place(ctx, g, f, 5, **eng).download()
+    assert m.tobytes() == om.tobytes() and len(m) == n_snv
+    start = np.sort(m["pos"].astype(np.int64))
+    assert (np.diff(start) >= 2).all()
 
 
 def test_driver_locus_never_drawn(ctx):
