@@ -449,10 +449,16 @@ __global__ void __launch_bounds__(256) k_footprint(const psx_mutation* __restric
 
 // A loser of check number `stamp` (1 for the first): lose[id] holds the
 // number of the last check it lost (cleared once per call), so every loser is
-// listed once per check without clearing flags between checks.
-__device__ __forceinline__ void mark_loser(uint32_t* lose, uint32_t stamp, uint32_t* list, uint32_t* n_lose,
-                                           uint32_t id) {
-    if (atomicExch(&lose[id], stamp) != stamp) list[atomicAdd(n_lose, 1u)] = id;
+// listed once per check without clearing flags between checks. Beside each
+// listed id: the index of its first-check entry when it has not moved yet
+// (found through that entry), else NO_ENTRY.
+__device__ __forceinline__ void mark_loser(uint32_t* lose, uint32_t stamp, uint32_t* list, uint32_t* apos,
+                                           uint32_t* n_lose, uint32_t id, uint32_t ap) {
+    if (atomicExch(&lose[id], stamp) != stamp) {
+        const uint32_t slot = atomicAdd(n_lose, 1u);
+        list[slot] = id;
+        apos[slot] = ap;
+    }
 }
 
 // Appending kernels take APPEND_ITEMS items per thread (item k of thread t of
@@ -502,7 +508,7 @@ __device__ __forceinline__ bool loses(const uint64_t* __restrict__ fkey, const u
 __global__ void __launch_bounds__(256) k_conflicts(const uint64_t* __restrict__ fkey,
                                                    const uint32_t* __restrict__ fval, uint64_t M, uint64_t n_fixed,
                                                    uint32_t* __restrict__ lose, uint32_t* __restrict__ list,
-                                                   uint32_t* __restrict__ n_lose) {
+                                                   uint32_t* __restrict__ apos, uint32_t* __restrict__ n_lose) {
     const uint64_t b = (uint64_t)blockIdx.x * 256u * APPEND_ITEMS + threadIdx.x;
     uint32_t mask = 0, n = 0, ids[APPEND_ITEMS];
 #pragma unroll
@@ -519,7 +525,9 @@ __global__ void __launch_bounds__(256) k_conflicts(const uint64_t* __restrict__ 
     for (uint32_t k = 0; k < APPEND_ITEMS; ++k)
         if (mask & (1u << k)) {
             lose[ids[k]] = 1u;
-            list[slot++] = ids[k];
+            list[slot] = ids[k];
+            apos[slot] = (uint32_t)(b + 256u * k);
+            ++slot;
         }
 }
 
@@ -573,8 +581,13 @@ struct round_io {
 struct redraw_args {
     uint32_t round;        // r >= 1: redraws the losers of check r - 1
     const uint32_t* list;  // losers of check r - 1
+    const uint32_t* lapos; // and their first-check entries (NO_ENTRY once moved)
     uint32_t* moved;       // set for every redrawn id
     uint32_t* mlist;       // ids in the order they first moved
+    uint32_t* mapos;       // and the index of each one's first-check entry
+    const uint64_t* akey;  // the first check's sorted footprints: a mutation's entry is made NO_ENTRY
+    uint32_t* aval;        // when it first moves
+    uint64_t M;
     round_io* io;
     uint32_t* tab;
     uint32_t tab_mask;
@@ -598,7 +611,16 @@ __global__ void __launch_bounds__(256) k_redraw(const place_args a, const redraw
         const uint32_t id = on ? b.list[t] : 0u;
         const bool first = on && atomicExch(&b.moved[id], 1u) == 0u;
         const uint32_t slot = block_reserve(first ? 1u : 0u, &io->n_moved);
-        if (first) b.mlist[slot] = id;
+        if (first) {
+            uint64_t j = b.lapos[t];
+            if (j == NO_ENTRY) {  // not listed with it: found from the first-check footprint, still in its record
+                j = lower_bound_u64(b.akey, b.M, padded_pos(b.pstart, a.recs[id]));
+                while (j + 1 < b.M && (b.aval[j] & ID_MASK) != id) ++j;  // among the entries at that position
+            }
+            b.aval[j] = NO_ENTRY;
+            b.mlist[slot] = id;
+            b.mapos[slot] = (uint32_t)j;
+        }
         if (on) {
             const uint32_t att = a.attempt[id] + 1u;
             a.attempt[id] = att;
@@ -629,19 +651,19 @@ __global__ void __launch_bounds__(256) k_tab_fill(const uint32_t* __restrict__ m
 }
 
 struct check_args {
-    const uint64_t* akey;  // first-round footprints, sorted (entries of moved mutations are stale)
+    const uint64_t* akey;  // first-round footprints, sorted (entries of moved mutations are NO_ENTRY)
     const uint32_t* aval;
     uint64_t M;
     const psx_mutation* recs;
     const uint64_t* pstart;
     const uint32_t* tab;
     uint32_t tab_mask;
-    const uint32_t* moved;
     uint32_t round;        // r >= 1: checks the footprints redrawn in round r (check r)
     const uint32_t* list;  // losers of check r - 1, redrawn in round r
     round_io* io;
     uint32_t* lose;
     uint32_t* out;         // losers of check r
+    uint32_t* out_apos;    // and their first-check entries
 };
 
 // Later rounds: a pair of touching footprints has a member redrawn in the
@@ -661,9 +683,9 @@ __device__ __forceinline__ void check_one(const check_args& a, uint32_t idc) {
         const uint64_t si = a.akey[i];
         if (si > to) break;
         const uint32_t vi = a.aval[i], idi = vi & ID_MASK;
-        if (a.moved[idi] || !fp_touch(si, vi >> ID_BITS, sc, lc)) continue;
+        if (vi == NO_ENTRY || !fp_touch(si, vi >> ID_BITS, sc, lc)) continue;
         if (idi < idc) l = true;
-        else mark_loser(a.lose, stamp, a.out, n_out, idi);
+        else mark_loser(a.lose, stamp, a.out, a.out_apos, n_out, idi, (uint32_t)i);
     }
     for (uint64_t bk = from >> BUCKET_SHIFT; bk <= to >> BUCKET_SHIFT; ++bk) {
         uint32_t h = tab_home(bk, a.tab_mask);
@@ -674,10 +696,10 @@ __device__ __forceinline__ void check_one(const check_args& a, uint32_t idc) {
             const psx_mutation ri = a.recs[idi];
             if (!fp_touch(padded_pos(a.pstart, ri), fp_len(ri), sc, lc)) continue;
             if (idi < idc) l = true;
-            else mark_loser(a.lose, stamp, a.out, n_out, idi);
+            else mark_loser(a.lose, stamp, a.out, a.out_apos, n_out, idi, NO_ENTRY);
         }
     }
-    if (l) mark_loser(a.lose, stamp, a.out, n_out, idc);
+    if (l) mark_loser(a.lose, stamp, a.out, a.out_apos, n_out, idc, NO_ENTRY);
 }
 
 // Check r (grid-stride over the footprints redrawn in round r); nothing after a halt.
@@ -703,44 +725,64 @@ __global__ void __launch_bounds__(256) k_moved_keys(const uint32_t* __restrict__
     ids[t] = id;
 }
 
-// ... the first round's entries still current (a scan of these flags ranks them) ...
-struct kept_flag {
-    const uint32_t* moved;
-    __device__ __forceinline__ uint32_t operator()(uint32_t v) const { return moved[v & ID_MASK] ? 0u : 1u; }
-};
+// ... then the output slots by one scan over w[0..M]: w[j] = 1 for a
+// first-check entry still current (0 for the moved ones') plus the number of
+// moved positions whose upper bound among the first-check keys is j. With S
+// the exclusive sum of w, a current entry j goes to S[j + 1] - 1 and the
+// i-th moved position (upper bound u, the first of its bound at i0) to
+// S[u] - i0 + i: both count the current entries and moved positions before
+// them, ties put first-check entries first.
+__global__ void __launch_bounds__(256) k_merge_marks(const uint32_t* __restrict__ mapos,
+                                                     const uint64_t* __restrict__ akey, uint64_t M,
+                                                     const uint64_t* __restrict__ bkey, uint32_t nb, uint32_t* w,
+                                                     uint32_t* __restrict__ ub) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nb) return;
+    atomicSub(&w[mapos[i]], 1u);
+    const uint64_t u = upper_bound_u64(akey, M, bkey[i]);
+    ub[i] = (uint32_t)u;
+    atomicAdd(&w[u], 1u);
+}
 
 struct merge_args {
-    const uint64_t* akey;  // first-round footprints, sorted
+    const uint64_t* akey;  // first-round footprints, sorted (moved ones NO_ENTRY)
     const uint32_t* aval;
     uint64_t M;
-    const uint32_t* moved;
-    const uint32_t* rank;  // kept entries of akey before each one
+    const uint32_t* S;     // exclusive sum of w (M + 1 entries)
     const uint64_t* bkey;  // moved mutations' current positions, sorted
     const uint32_t* bval;
-    uint64_t nb;
+    const uint32_t* ub;    // their upper bounds among akey
+    uint32_t nb;
     uint64_t* okey;
     uint32_t* oid;
 };
 
-// ... and both merged into position order (one thread per entry of either,
-// ties: first-round entries first; runs of one position are put in SID
-// order afterwards). okey/oid receive M entries.
+// One thread per entry of either; okey/oid receive M entries. Without moved
+// mutations the first check's order is the output.
 __global__ void __launch_bounds__(256) k_merge(const merge_args a) {
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < a.M) {
-        const uint32_t id = a.aval[t] & ID_MASK;
-        if (a.nb && a.moved[id]) return;
-        const uint64_t k = a.akey[t];
-        const uint64_t o = (a.nb ? a.rank[t] : t) + lower_bound_u64(a.bkey, a.nb, k);
-        a.okey[o] = k;
-        a.oid[o] = id;
+        const uint32_t v = a.aval[t];
+        if (v == NO_ENTRY) return;
+        const uint64_t o = a.nb ? a.S[t + 1] - 1u : t;
+        a.okey[o] = a.akey[t];
+        a.oid[o] = v & ID_MASK;
     } else if (t < a.M + a.nb) {
-        const uint64_t i = t - a.M, k = a.bkey[i];
-        const uint64_t u = upper_bound_u64(a.akey, a.M, k);
-        const uint64_t o = i + (u < a.M ? a.rank[u] : a.M - a.nb);
-        a.okey[o] = k;
+        const uint32_t i = (uint32_t)(t - a.M), u = a.ub[i];
+        uint32_t lo = 0, hi = i;  // first moved position with the same bound
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (a.ub[mid] < u) lo = mid + 1; else hi = mid;
+        }
+        const uint64_t o = (uint64_t)a.S[u] - lo + i;
+        a.okey[o] = a.bkey[i];
         a.oid[o] = a.bval[i];
     }
+}
+
+__global__ void __launch_bounds__(256) k_fill_u32(uint32_t* __restrict__ p, uint64_t n, uint32_t v, uint32_t last) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = i + 1 == n ? last : v;
 }
 
 __global__ void __launch_bounds__(256) k_pos_key(const psx_mutation* __restrict__ recs, uint64_t M,
@@ -963,9 +1005,11 @@ struct place_job {
     uint32_t* d_cnt_snv = nullptr;
     uint64_t *d_cnt_tot = nullptr, *d_off = nullptr;
     uint64_t *d_key = nullptr, *d_key2 = nullptr, *d_cb = nullptr;
-    uint32_t *d_ids = nullptr, *d_ids2 = nullptr, *d_moved = nullptr, *d_mlist = nullptr, *d_rank = nullptr;
+    uint32_t *d_ids = nullptr, *d_ids2 = nullptr, *d_moved = nullptr, *d_mlist = nullptr, *d_mapos = nullptr;
+    uint32_t *d_w = nullptr, *d_S = nullptr;
     uint32_t *d_node_of = nullptr, *d_att = nullptr, *d_lose = nullptr, *d_tab = nullptr;
     uint32_t* d_list[2] = {nullptr, nullptr};
+    uint32_t* d_lapos[2] = {nullptr, nullptr};  // first-check entries of the listed losers
     round_io* d_io = nullptr;
     uint8_t* d_head = nullptr;
     // infinite-sites rounds: the next round to run (its losers, of check round - 1, are in
@@ -1326,9 +1370,8 @@ void place_batch(psx_ctx* ctx, const psx_genome* g, std::vector<place_job>& jobs
     size_t sort_bytes = 0, rank_bytes = 0;
     PSX_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (uint64_t*)nullptr, (uint64_t*)nullptr,
                                                (uint32_t*)nullptr, (uint32_t*)nullptr, (int)max_m, 0, pbits, s));
-    PSX_HIP(hipcub::DeviceScan::ExclusiveSum(
-        nullptr, rank_bytes, hipcub::TransformInputIterator<uint32_t, kept_flag, const uint32_t*>(nullptr, kept_flag{}),
-        (uint32_t*)nullptr, (int)max_m, s));
+    PSX_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, rank_bytes, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                                             (int)max_m + 1, s));
     sort_bytes = std::max(sort_bytes, rank_bytes);
     // per-mutation temporaries from the context's placement pool: the attempt counters, moved flags and
     // loser stamps of every job first (cleared together), then each job's own
@@ -1339,7 +1382,8 @@ void place_batch(psx_ctx* ctx, const psx_genome* g, std::vector<place_job>& jobs
         const uint64_t M = jobs[k].M;
         c_mut[k] = cm.bytes;
         cm.take(M * sizeof(psx_mutation));
-        for (int q = 0; q < 6; ++q) cm.take(M * 4);  // list[2], ids, ids2, mlist, rank
+        for (int q = 0; q < 8; ++q) cm.take(M * 4);  // list[2], lapos[2], ids, ids2, mlist, mapos
+        for (int q = 0; q < 2; ++q) cm.take((M + 1) * 4);  // w, S
         cm.take(jobs[k].n_pass * 4);                // node_of
         for (int q = 0; q < 2; ++q) cm.take(M * 8);  // key, key2
         cm.take(tab_max(M) * 4);                    // table
@@ -1362,10 +1406,14 @@ void place_batch(psx_ctx* ctx, const psx_genome* g, std::vector<place_job>& jobs
         psx_mutation* recs = (psx_mutation*)(pm + c.take(M * sizeof(psx_mutation)));
         J.d_list[0] = (uint32_t*)(pm + c.take(M * 4));
         J.d_list[1] = (uint32_t*)(pm + c.take(M * 4));
+        J.d_lapos[0] = (uint32_t*)(pm + c.take(M * 4));
+        J.d_lapos[1] = (uint32_t*)(pm + c.take(M * 4));
         J.d_ids = (uint32_t*)(pm + c.take(M * 4));
         J.d_ids2 = (uint32_t*)(pm + c.take(M * 4));
         J.d_mlist = (uint32_t*)(pm + c.take(M * 4));
-        J.d_rank = (uint32_t*)(pm + c.take(M * 4));
+        J.d_mapos = (uint32_t*)(pm + c.take(M * 4));
+        J.d_w = (uint32_t*)(pm + c.take((M + 1) * 4));
+        J.d_S = (uint32_t*)(pm + c.take((M + 1) * 4));
         J.d_node_of = (uint32_t*)(pm + c.take(J.n_pass * 4));
         J.d_key = (uint64_t*)(pm + c.take(M * 8));
         J.d_key2 = (uint64_t*)(pm + c.take(M * 8));
@@ -1419,7 +1467,7 @@ void place_batch(psx_ctx* ctx, const psx_genome* g, std::vector<place_job>& jobs
             PSX_HIP(hipcub::DeviceRadixSort::SortPairs(d_sort, sort_bytes, J.d_key, J.d_key2, J.d_ids, J.d_ids2,
                                                        (int)M, 0, pbits, s));
             hipLaunchKernelGGL(k_conflicts, dim3(grid_append(M)), dim3(256), 0, s, J.d_key2, J.d_ids2, M, J.n_fixed,
-                               J.d_lose, J.d_list[0], &J.d_io->n_lose[0]);
+                               J.d_lose, J.d_list[0], J.d_lapos[0], &J.d_io->n_lose[0]);
             PSX_HIP(hipGetLastError());
         }
         PSX_HIP(hipMemcpyAsync(io + io_job * k + 2, J.d_io, sizeof(round_io), hipMemcpyDeviceToHost, s));
@@ -1474,16 +1522,17 @@ void place_batch(psx_ctx* ctx, const psx_genome* g, std::vector<place_job>& jobs
             J.spec_end = std::min<uint32_t>(J.round + SPEC_ROUNDS, IS_ROUNDS);
             for (uint32_t r = J.round; r < J.spec_end; ++r) {
                 redraw_args rd{};
-                rd.round = r; rd.list = J.d_list[(r - 1) & 1];
-                rd.moved = J.d_moved; rd.mlist = J.d_mlist; rd.io = J.d_io;
+                rd.round = r; rd.list = J.d_list[(r - 1) & 1]; rd.lapos = J.d_lapos[(r - 1) & 1];
+                rd.moved = J.d_moved; rd.mlist = J.d_mlist; rd.mapos = J.d_mapos; rd.io = J.d_io;
+                rd.akey = J.d_key2; rd.aval = J.d_ids2; rd.M = J.M;
                 rd.tab = J.d_tab; rd.tab_mask = (uint32_t)(J.tab_slots - 1); rd.pstart = d_pstart;
                 hipLaunchKernelGGL(k_redraw, dim3(grid), dim3(256), 0, s, J.pa, rd);
                 PSX_HIP(hipGetLastError());
                 check_args ca{};
                 ca.akey = J.d_key2; ca.aval = J.d_ids2; ca.M = J.M;
                 ca.recs = J.pa.recs; ca.pstart = d_pstart; ca.tab = J.d_tab; ca.tab_mask = rd.tab_mask;
-                ca.moved = J.d_moved; ca.round = r; ca.list = J.d_list[(r - 1) & 1]; ca.io = J.d_io;
-                ca.lose = J.d_lose; ca.out = J.d_list[r & 1];
+                ca.round = r; ca.list = J.d_list[(r - 1) & 1]; ca.io = J.d_io;
+                ca.lose = J.d_lose; ca.out = J.d_list[r & 1]; ca.out_apos = J.d_lapos[r & 1];
                 hipLaunchKernelGGL(k_check, dim3(grid), dim3(256), 0, s, ca);
                 PSX_HIP(hipGetLastError());
             }
@@ -1509,10 +1558,11 @@ void place_batch(psx_ctx* ctx, const psx_genome* g, std::vector<place_job>& jobs
         uint32_t* ids = J.d_ids2;
         if (J.p->infinite_sites && M > 1) {
             merge_args ma{};
-            ma.akey = J.d_key2; ma.aval = J.d_ids2; ma.M = M; ma.moved = J.d_moved; ma.nb = J.n_moved;
+            ma.akey = J.d_key2; ma.aval = J.d_ids2; ma.M = M; ma.nb = J.n_moved;
             ma.okey = J.d_key; ma.oid = J.d_ids;
             if (J.n_moved) {
-                // the sorted moved positions go into the table's memory, free after the rounds
+                // the sorted moved positions go into the table's memory, free after the rounds, their
+                // upper bounds into the first loser list's
                 uint64_t* bkey = (uint64_t*)J.d_tab;
                 uint32_t* bval = (uint32_t*)(bkey + J.n_moved);
                 hipLaunchKernelGGL(k_moved_keys, dim3(grid_of(J.n_moved)), dim3(256), 0, s, J.d_mlist, J.n_moved,
@@ -1520,11 +1570,13 @@ void place_batch(psx_ctx* ctx, const psx_genome* g, std::vector<place_job>& jobs
                 PSX_HIP(hipGetLastError());
                 PSX_HIP(hipcub::DeviceRadixSort::SortPairs(d_sort, sort_bytes, J.d_key, bkey, J.d_ids, bval,
                                                            (int)J.n_moved, 0, pbits, s));
-                PSX_HIP(hipcub::DeviceScan::ExclusiveSum(
-                    d_sort, sort_bytes,
-                    hipcub::TransformInputIterator<uint32_t, kept_flag, const uint32_t*>(J.d_ids2, kept_flag{J.d_moved}),
-                    J.d_rank, (int)M, s));
-                ma.rank = J.d_rank; ma.bkey = bkey; ma.bval = bval;
+                hipLaunchKernelGGL(k_fill_u32, dim3(grid_of(M + 1)), dim3(256), 0, s, J.d_w, M + 1, 1u, 0u);
+                PSX_HIP(hipGetLastError());
+                hipLaunchKernelGGL(k_merge_marks, dim3(grid_of(J.n_moved)), dim3(256), 0, s, J.d_mapos, J.d_key2, M,
+                                   bkey, J.n_moved, J.d_w, J.d_list[0]);
+                PSX_HIP(hipGetLastError());
+                PSX_HIP(hipcub::DeviceScan::ExclusiveSum(d_sort, sort_bytes, J.d_w, J.d_S, (int)M + 1, s));
+                ma.S = J.d_S; ma.bkey = bkey; ma.bval = bval; ma.ub = J.d_list[0];
             }
             hipLaunchKernelGGL(k_merge, dim3(grid_of(M + J.n_moved)), dim3(256), 0, s, ma);
             PSX_HIP(hipGetLastError());
