@@ -326,11 +326,14 @@ def batched_reads_leg(ctx, B, reps: int) -> dict:
 
 
 def serving_leg(ctx, rate: float, duration: float, saturated: int, overlap_us: int = 0,
-                compare_overlap_us=()) -> dict:
+                compare_overlap_us=(), forms=("continuous", "one_at_a_time")) -> dict:
     """The serving workload: Poisson arrivals of requests over device-resident
     forests (1e3-1e6 cells) through the continuous-batching server, against
-    the same stream served one request at a time; then the same number of
-    requests submitted at once (saturated) in both forms."""
+    the same stream served one request at a time (forms picks either or both);
+    then the same number of requests submitted at once (saturated) in both
+    forms. Peak HBM is the device's whole use (free memory sampled while
+    waiting on each request): resident inputs, the server's output arena and
+    the context's pools."""
     import threading
 
     from process_amd import core, workloads
@@ -360,9 +363,11 @@ def serving_leg(ctx, rate: float, duration: float, saturated: int, overlap_us: i
         th.start()
         lat, queued, batch, reads = [], [], [], 0
         last = time.perf_counter()
+        min_free = ctx.mem_info()[0]
         for i in range(len(stream)):
             while len(tickets) <= i:
                 time.sleep(0.0005)
+            min_free = min(min_free, ctx.mem_info()[0])
             _, _, _, st = srv.wait(tickets[i])
             lat.append(st["latency_ms"])
             queued.append(st["queued_ms"])
@@ -378,7 +383,8 @@ def serving_leg(ctx, rate: float, duration: float, saturated: int, overlap_us: i
         r = dict(requests=len(stream), reads=reads, value=reads / wall, unit="reads/s", wall_s=wall,
                  p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)),
                  queued_p99_ms=float(np.percentile(queued, 99)), mean_batch=float(np.mean(batch)),
-                 launches=tot["launches"], peak_output_bytes=tot["peak_bytes"])
+                 launches=tot["launches"], peak_output_bytes=tot["peak_bytes"],
+                 peak_device_bytes=int(ctx.mem_info()[1] - min_free))
         log(f"serving ({label}): {r['value']:.3e} reads/s, p50 {r['p50_ms']:.1f} ms, p99 {r['p99_ms']:.1f} ms, "
             f"mean batch {r['mean_batch']:.1f}")
         return r
@@ -392,9 +398,11 @@ def serving_leg(ctx, rate: float, duration: float, saturated: int, overlap_us: i
                         "forests of 1e3-1e6 cells (2-3 samples of <= 1e4 cells), shared 50 Mb reference, "
                         "coverage log-uniform 30-500x, PE 150 bp, error 1e-3, random sample subsets + normal",
                resident_input_bytes=int(f0 - f1), hbm_budget_bytes=budget,
-               overlap_after_us=overlap_us,
-               continuous=run(stream, 2, 16, True, "continuous", overlap_us),
-               one_at_a_time=run(stream, 1, 1, True, "one at a time"))
+               overlap_after_us=overlap_us)
+    if "continuous" in forms:
+        out["continuous"] = run(stream, 2, 16, True, "continuous", overlap_us)
+    if "one_at_a_time" in forms:
+        out["one_at_a_time"] = run(stream, 1, 1, True, "one at a time")
     for us in compare_overlap_us:
         out.setdefault("continuous_by_overlap_us", {})[str(us)] = run(stream, 2, 16, True,
                                                                        f"continuous, overlap after {us} us", us)
@@ -467,6 +475,8 @@ def main() -> int:
     ap.add_argument("--serve-compare-overlap-us", type=str, default="",
                     help="comma-separated overlap settings also run on the paced stream (comparison)")
     ap.add_argument("--serving-only", action="store_true", help="only run the serving workload")
+    ap.add_argument("--serve-forms", type=str, default="continuous,one_at_a_time",
+                    help="forms run on the paced stream (comma-separated: continuous, one_at_a_time)")
     ap.add_argument("--cna-only", action="store_true", help="only run the copy-number workload (for profiling)")
     ap.add_argument("--batched-reads-only", action="store_true",
                     help="only run the batched read materialisation (one launch per repetition; for profiling)")
@@ -496,7 +506,7 @@ def main() -> int:
     if args.serving_only:
         cmp = [int(x) for x in args.serve_compare_overlap_us.split(",") if x]
         print(json.dumps(serving_leg(ctx, args.serve_rate, args.serve_seconds, args.serve_burst,
-                                     args.serve_overlap_us, cmp)), flush=True)
+                                     args.serve_overlap_us, cmp, args.serve_forms.split(","))), flush=True)
         return 0
     if args.probe_only:
         print(json.dumps(valu_ceiling(ctx)), flush=True)
