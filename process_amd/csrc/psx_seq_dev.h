@@ -1340,9 +1340,28 @@ __global__ void __launch_bounds__(BLOCK) k_seq_reads(const seq_launch L) {
             // leaf | allele << 26, template index tr) whose reads start at read index first0.
             const bool cn = CNA && S.n_cna != 0u;
             const uint64_t row0 = cn ? uniform_at(J.cn_row, S.st_off + j) : 0ull;
+            // Without copy-number events the chunks of 64 templates are shifted by o so that every pass after
+            // the stratum's first chunk starts on a 1024-byte boundary of the output (else 512, 256, 128): the
+            // first chunk holds the templates before the first aligned row. A 16-byte offset of such 3 KiB
+            // passes costs about 10 % of the write rate (partly written lines at every pass boundary).
+            uint32_t o = 0;
+            if (!cn && STAGE && vec4) {
+                const uint64_t a0 = (uint64_t)(uintptr_t)(L.reads_out + (S.read_base + s.before * rpt) * RW);
+                const uint64_t step = (uint64_t)rpt * RW * 4u;  // bytes per template
+                for (uint32_t A = 1024u; A >= 128u; A >>= 1) {
+                    const unsigned long long m = __ballot(((a0 + lane * step) & (A - 1u)) == 0u);
+                    if (m) {
+                        o = (64u - ((uint32_t)__ffsll((long long)m) - 1u)) & 63u;
+                        break;
+                    }
+                }
+            }
             uint32_t kept = 0;  // copy-number events: kept templates of the stratum's earlier rounds
-            for (uint32_t base = 0; base < nj; base += WAVES * 64u) {
-                const uint32_t c0 = base + wave * 64u;
+            for (uint32_t base = 0; base < nj + o; base += WAVES * 64u) {
+                // the wave's chunk: templates [c0, cend) (virtual templates [vb, vb + 64) less o)
+                const uint32_t vb = base + wave * 64u;
+                const uint32_t c0 = vb == 0u ? 0u : vb - o;
+                const uint32_t cend = min(vb + 64u - o, nj);
                 const uint32_t rt = c0 + lane;
                 uint32_t tx = 0, tx2 = 0, tla = 0, tr = rt, n_t;
                 uint64_t first0;
@@ -1374,8 +1393,8 @@ __global__ void __launch_bounds__(BLOCK) k_seq_reads(const seq_launch L) {
                     tr = (uint32_t)__builtin_amdgcn_ds_permute(to, (int)rt);
                     first0 = row0 + (uint64_t)before * rpt;
                 } else {
-                    n_t = c0 < nj ? min(64u, nj - c0) : 0u;
-                    if (rt < nj) {
+                    n_t = cend > c0 ? cend - c0 : 0u;
+                    if (lane < n_t) {
                         tmpl t = make_template(J, S, s_tab, rt, j, s.b, s.e);
                         tx = t.x;
                         tx2 = t.x2;
