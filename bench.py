@@ -37,9 +37,9 @@ SERVE_OVERLAP_US = 250
 # vector instruction issue, not HBM; the probe kernels' instruction counts per lane-iteration come from the
 # same counter (profiles/k_probe_valu.md).
 PROFILED = {"k_seq_count": dict(traffic=302.1e6, valu_insts=6.206e9),
-            "k_seq_reads": dict(traffic=77.67e9),
+            "k_seq_reads": dict(traffic=77.42e9),
             # batched read materialisation of the 256-job batched workload's leading 19 jobs
-            "k_seq_reads_batched": dict(traffic=72.74e9),
+            "k_seq_reads_batched": dict(traffic=72.54e9),
             # wave64 VALU instructions per wave per loop iteration of the probe kernels
             "k_probe_valu": dict(philox_per_iter=43.0, int_per_iter=24.0)}
 
@@ -654,6 +654,20 @@ def main() -> int:
                                                kernel_ms=rk, algo_bytes_per_read=algo, row_bytes=4 * rw,
                                                row_write_rate_gbs=n_all * 4 * rw / (rk / 1e3) / 1e9))
         log(f"reads mode: {n_all} reads, kernel {rk:.2f} ms, {ach:.0f} GB/s of read bytes ({ach / HBM_PEAK_GBS:.1%})")
+        # the write ceiling on this GPU: stores only, into the same buffer (psx_probe_store), best of 3 each
+        ceil = {}
+        for mode in ("fill", "chunks", "chunks_shifted"):
+            core.probe_store(ctx, buf, mode)
+            ms, nb = min(core.probe_store(ctx, buf, mode) for _ in range(3))
+            ceil[mode] = nb / (ms / 1e3) / 1e9
+        rate = n_all * 4 * rw / (rk / 1e3) / 1e9
+        out["reads_mode"]["roofline"]["store_ceiling"] = dict(
+            gbs=ceil, row_write_frac_of_chunks=rate / ceil["chunks"],
+            note="stores only into the same buffer: grid-stride fill; the kernel's shape (items of 170 x 3 KiB "
+                 "chunks, 1024-byte aligned); the same shape 16 bytes off alignment")
+        log(f"store ceiling: fill {ceil['fill']:.0f} GB/s, chunks {ceil['chunks']:.0f} GB/s, shifted chunks "
+            f"{ceil['chunks_shifted']:.0f} GB/s; the kernel's row writes {rate:.0f} GB/s = "
+            f"{rate / ceil['chunks']:.1%} of the aligned chunks")
         buf.close()
 
     if not args.no_cna:

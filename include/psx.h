@@ -592,6 +592,18 @@ void psx_server_destroy(psx_server* s);
 enum { PSX_PROBE_PHILOX = 0, PSX_PROBE_INT = 1 };
 psx_status psx_probe_valu(psx_ctx* ctx, int mode, uint32_t iters, double* ms, uint64_t* lane_iters);
 
+/* HBM write ceiling of the context's GPU (no counterpart in the reference;
+ * the read-materialising kernel is bound by its output stores and is
+ * reported against this). One launch of stores only into `out` (device
+ * memory of `bytes`, 16-byte aligned, contents overwritten), timed with HIP
+ * events; *written gets the bytes stored. PSX_PROBE_STORE_FILL: grid-stride,
+ * consecutive lanes on consecutive 16-byte words. PSX_PROBE_STORE_CHUNKS: the
+ * read kernel's write shape, items of 170 chunks of 3 KiB taken by 512-thread
+ * blocks from a counter, 2 chunks per wave per round, chunks on 1024-byte
+ * boundaries; _SHIFTED: the same 16 bytes off them. */
+enum { PSX_PROBE_STORE_FILL = 0, PSX_PROBE_STORE_CHUNKS = 1, PSX_PROBE_STORE_CHUNKS_SHIFTED = 2 };
+psx_status psx_probe_store(psx_ctx* ctx, void* out, uint64_t bytes, int mode, double* ms, uint64_t* written);
+
 /* ------------------------------------------- synthetic forest (host input) */
 
 /* Birth-death genealogy standing in for the CPU spatial simulation, which

@@ -44,6 +44,7 @@ PSX_CNA_AMP, PSX_CNA_DEL, PSX_CNA_WGD = 0, 1, 2
 PSX_ALL_CHROMOSOMES = 0xFFFFFFFF
 PSX_CNA_RANDOM_ALLELE = 0xFF
 PSX_PROBE_PHILOX, PSX_PROBE_INT = 0, 1
+PSX_PROBE_STORE_FILL, PSX_PROBE_STORE_CHUNKS, PSX_PROBE_STORE_CHUNKS_SHIFTED = 0, 1, 2
 
 
 class Cna(C.Structure):

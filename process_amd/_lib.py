@@ -93,6 +93,7 @@ def _declare(lib):
         "psx_server_last_error": (C.c_char_p, [vp]),
         "psx_server_destroy": (None, [vp]),
         "psx_probe_valu": (st, [vp, C.c_int, C.c_uint32, C.POINTER(C.c_double), A.u64p]),
+        "psx_probe_store": (st, [vp, vp, C.c_uint64, C.c_int, C.POINTER(C.c_double), A.u64p]),
         "psx_synth_forest_run": (st, [C.POINTER(A.SynthParams), C.c_uint64, pp]),
         "psx_synth_forest_nodes": (C.c_uint32, [vp]),
         "psx_synth_forest_get": (st, [vp, A.i64p, A.i64p, A.u32p, A.i32p, A.f64p]),

@@ -485,6 +485,17 @@ def probe_valu(ctx: Context, mode: str, iters: int) -> tuple[float, int]:
     return ms.value, n.value
 
 
+def probe_store(ctx: Context, out: "DeviceArray", mode: str) -> tuple[float, int]:
+    """The HBM write-ceiling kernels (psx_probe_store) over out's memory
+    (overwritten): mode "fill", "chunks" or "chunks_shifted"; returns
+    (HIP-event ms of one launch, bytes written)."""
+    ms, n = C.c_double(), C.c_uint64()
+    m = {"fill": A.PSX_PROBE_STORE_FILL, "chunks": A.PSX_PROBE_STORE_CHUNKS,
+         "chunks_shifted": A.PSX_PROBE_STORE_CHUNKS_SHIFTED}[mode]
+    check(lib().psx_probe_store(ctx.h, C.c_void_p(out.ptr), out.nbytes, m, C.byref(ms), C.byref(n)), ctx.h)
+    return ms.value, n.value
+
+
 def synth_forest(*, growth, death, intro_at, n_cells: int, n_samples: int, sample_size: int, seed: int) -> dict:
     """Synthetic birth-death samples forest (host generator in libpsx)."""
     g = np.ascontiguousarray(growth, np.float64)
