@@ -321,6 +321,42 @@ def test_passenger_cnas_against_oracle(ctx):
     same(got, O.simulate_seq(lens, of, om, 8, cna=oev, **seq), "passenger CNAs")
 
 
+def test_many_events_per_chromosome(ctx):
+    """Short passenger CNAs at a high rate: 70 events on chromosome 0 (more
+    than the 64 a stratum's event mask covers, so every template scans the
+    whole list) and 60 on chromosome 1 (masked per stratum). Events, count
+    tables and every materialised read equal the oracle's."""
+    lens = np.array([120_000, 64_000], np.uint32)
+    packed = O.genome_synth(lens, 5)
+    g = core.Genome(ctx, lens, seed=5)
+    fd = core.synth_forest(growth=[0.2, 0.45], death=[0.01, 0.01], intro_at=[0, 25], n_cells=800, n_samples=2,
+                           sample_size=120, seed=19)
+    f = core.Forest(ctx, fd["cell_id"], fd["parent"], fd["species"], fd["sample"], fd["birth"], fd["n_samples"])
+    of = oforest(fd)
+    eng = cases.engine(packed, lens, snv=3e-6, indel=3e-7, pre_snv=5, pre_indel=1)
+    eng["cna"] = [(0, 0, 2_000, 30_000, A.PSX_CNA_RANDOM_ALLELE, A.PSX_CNA_AMP, 1), (1, 0, 0, 0, 0, A.PSX_CNA_WGD, 0)]
+    eng["cna_rate"] = [4e-7, 8e-7]
+    eng["cna_catalogue"] = [(0, 50_000, 4_000, A.PSX_CNA_AMP), (0, 5_000, 800, A.PSX_CNA_DEL),
+                            (0, 80_000, 30_000, A.PSX_CNA_DEL), (1, 30_000, 2_500, A.PSX_CNA_AMP),
+                            (1, 1_000, 3_000, A.PSX_CNA_DEL)]
+    m = core.Mutations.place(ctx, g, f, 314, **eng)
+    ev = m.cna()
+    oev = O.place_cna(lens, of, seed=314, **eng)
+    assert ev.tobytes() == oev.tobytes()
+    every = (ev["chr"] == A.PSX_ALL_CHROMOSOMES).sum()
+    assert (ev["chr"] == 0).sum() + every > 64 and (ev["chr"] == 1).sum() + every <= 64
+    om = O.place_mutations(lens, packed, of, 314, **eng)
+    assert m.download().tobytes() == om.tobytes()
+    seq = dict(coverage=40, read_size=150, insert_size_mean=350, insert_size_stddev=10, error_rate=1e-3, purity=0.8)
+    say("[configs] 130 copy-number events: counts")
+    exp = O.simulate_seq(lens, of, om, 8, cna=oev, **seq)
+    same(core.simulate_seq_arrays(ctx, g, f, m, 8, **seq), exp, "many events")
+    say("[configs] 130 copy-number events: reads")
+    rows, n = core.simulate_seq_reads(ctx, g, f, m, 8, **seq)
+    assert n == exp[3]
+    assert np.array_equal(rows.numpy()[:n], O.simulate_reads(lens, packed, of, om, 8, n, cna=oev, **seq))
+
+
 def test_cna_drivers_through_add_mutant(ctx):
     """MutationEngine$add_mutant with an amplification, a deletion and a
     whole-genome doubling among the drivers, and a passenger CNA rate over a
