@@ -292,7 +292,11 @@ int orc_forest_flatten(const psx_forest_desc* d, orc_forest** out) {
     for (uint32_t i = 0; i < n; ++i) { ix[i].id = d->cell_id[i]; ix[i].row = i; }
     qsort(ix, n, sizeof(id_row), cmp_id_row);
     for (uint32_t i = 1; i < n; ++i)
-        if (ix[i].id == ix[i - 1].id) { free(ix); return fail("duplicated cell id %lld", (long long)ix[i].id); }
+        if (ix[i].id == ix[i - 1].id) {
+            const long long dup = (long long)ix[i].id;
+            free(ix);
+            return fail("duplicated cell id %lld", dup);
+        }
     uint32_t* prow = (uint32_t*)malloc(sizeof(uint32_t) * n);
     uint32_t* nchild = (uint32_t*)calloc(n + 1, sizeof(uint32_t));
     for (uint32_t i = 0; i < n; ++i) {

@@ -7,6 +7,7 @@ never by the product package.
 from __future__ import annotations
 
 import ctypes as C
+import os
 import subprocess
 from pathlib import Path
 
@@ -15,7 +16,9 @@ import numpy as np
 from process_amd import _abi as A
 
 HERE = Path(__file__).resolve().parent
-LIB_PATH = HERE / "liboracle.so"
+# PSX_ORACLE_LIB names another build of the same sources (the sanitizer build of
+# tests/test_oracle_sanitized.py); the default is the plain one
+LIB_PATH = Path(os.environ["PSX_ORACLE_LIB"]) if os.environ.get("PSX_ORACLE_LIB") else HERE / "liboracle.so"
 _lib = None
 
 
