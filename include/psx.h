@@ -315,6 +315,19 @@ psx_status psx_mutations_set_cna(psx_ctx* ctx, psx_mutations* m, const psx_cna_e
 psx_status psx_mutations_download(psx_ctx* ctx, const psx_mutations* m, psx_mutation* recs,
                                   uint64_t n);
 void psx_mutations_free(psx_mutations* m);
+/* Checkpoint and resume of a table: psx_mutations_save writes the records (SID
+ * order), the copy-number events, the chromosome lengths, the ploidy and the
+ * leaf count to a file in this library's binary format (header, version,
+ * checksum); psx_mutations_load reads one back into device memory, ready for
+ * sequencing. Replaces PhylogeneticForest::save / load_phylogenetic_forest
+ * (src/phylogenetic_forest.cpp:940-992, src/mutations.cpp:1453-1478) for the
+ * mutation table, whose RACES archive format is not this library's; a missing
+ * or non-regular file, a wrong format (magic, size, checksum, a record out of
+ * range) and another format version are refused with the reference loader's
+ * messages (src/phylogenetic_forest.cpp:958-986, src/utility.cpp:141-161). */
+#define PSX_TABLE_FORMAT_VERSION 1u
+psx_status psx_mutations_save(psx_ctx* ctx, const psx_mutations* m, const char* path);
+psx_status psx_mutations_load(psx_ctx* ctx, const char* path, psx_mutations** out);
 /* Milliseconds of the placement kernels of the last psx_place_mutations. */
 psx_status psx_ctx_last_place_ms(const psx_ctx* ctx, double* ms);
 

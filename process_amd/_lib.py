@@ -62,6 +62,8 @@ def _declare(lib):
         "psx_mutations_count": (st, [vp, A.u64p]),
         "psx_mutations_download": (st, [vp, vp, vp, C.c_uint64]),
         "psx_mutations_free": (None, [vp]),
+        "psx_mutations_save": (st, [vp, vp, C.c_char_p]),
+        "psx_mutations_load": (st, [vp, C.c_char_p, pp]),
         "psx_insert_size_check": (st, [vp, C.c_uint32, C.c_uint32]),
         "psx_seq_num_samples": (C.c_uint32, [vp, C.POINTER(A.SeqParams)]),
         "psx_seq_num_reads": (st, [vp, vp, C.POINTER(A.SeqParams), A.u64p]),

@@ -558,6 +558,101 @@ class PhylogeneticForest:
     def get_samples_info(self) -> pd.DataFrame:
         return self.samples_forest.get_samples_info()
 
+    def save(self, filename, quiet: bool = False) -> None:
+        """PhylogeneticForest$save (src/mutations.cpp:1446-1460,
+        src/phylogenetic_forest.cpp:940-956): the forest in one file, a zip
+        archive holding the mutation table in the library's binary format
+        (psx_mutations_save), the samples forest, the reference (names,
+        lengths, seed or explicit bases), the signatures and the seed; read
+        back with load_phylogenetic_forest. quiet is accepted for the
+        reference's signature (there is no progress bar)."""
+        import io
+        import json
+        import tempfile
+        import zipfile
+
+        sf, ref, eng = self.samples_forest, self.engine.reference, self.engine
+
+        def npz(**arrays) -> bytes:
+            b = io.BytesIO()
+            np.savez(b, **arrays)
+            return b.getvalue()
+
+        text = lambda a: np.array(["" if v is None or v != v else str(v) for v in a], dtype=np.str_)
+        with tempfile.TemporaryDirectory() as d:
+            table = os.path.join(d, "mutations.psx")
+            self.mutations.save(table)
+            with zipfile.ZipFile(filename, "w", zipfile.ZIP_STORED) as z:
+                z.writestr("meta.json", json.dumps(dict(format=_FOREST_FORMAT, version=_FOREST_VERSION,
+                                                        seed=int(self.seed), ploidy=int(self.mutations.ploidy))))
+                z.writestr("forest.npz", npz(cell_id=sf.cell_id, ancestor=sf.ancestor, birth_time=sf.birth_time,
+                                             mutant=text(sf.mutant), epistate=text(sf.epistate),
+                                             sample=text(sf.sample), has_sample=np.array(
+                                                 [v is not None and v == v for v in sf.sample], bool)))
+                extra = {} if ref.packed is None else dict(packed=ref.packed)
+                z.writestr("reference.npz", npz(names=np.array(ref.names, dtype=np.str_), lengths=ref.lengths,
+                                                seed=np.array([ref.seed], np.int64), **extra))
+                z.writestr("signatures.npz", npz(sbs_names=np.array(eng.signature_names, dtype=np.str_),
+                                                 sbs=eng.signatures,
+                                                 id_names=np.array(eng.id_signature_names, dtype=np.str_),
+                                                 id=eng.id_signatures))
+                z.write(table, "mutations.psx")
+
+
+_FOREST_FORMAT = "psx phylogenetic forest"
+_FOREST_VERSION = 1
+
+
+def load_phylogenetic_forest(filename, quiet: bool = False, ctx: core.Context | None = None) -> PhylogeneticForest:
+    """load_phylogenetic_forest (src/mutations.cpp:1466-1478,
+    src/phylogenetic_forest.cpp:958-992): a forest written by
+    PhylogeneticForest.save, its mutation table back in device memory. The
+    engine it carries holds the reference and the signatures (what sequencing
+    and the result table use), as the reference's loaded forest holds its
+    reference path and exposures. Errors follow the reference's loader."""
+    import io
+    import json
+    import tempfile
+    import zipfile
+
+    path = str(filename)
+    if not os.path.exists(path):
+        raise ValueError(f"The file \"{path}\" does not exist.")
+    if not os.path.isfile(path):
+        raise ValueError(f"The file \"{path}\" is not a regular file.")
+    wrong = ValueError(f"Wrong file format for the phylogenetic forest file \"{path}\". Remove it and retry.")
+    try:
+        z = zipfile.ZipFile(path, "r")
+    except zipfile.BadZipFile:
+        raise wrong from None
+    with z, tempfile.TemporaryDirectory() as d:
+        try:
+            meta = json.loads(z.read("meta.json"))
+            if meta.get("format") != _FOREST_FORMAT:
+                raise wrong
+            if meta.get("version") != _FOREST_VERSION:
+                raise ValueError(f"The phylogenetic forest file \"{path}\" is in the file format version "
+                                 f"{meta.get('version')}, expected version {_FOREST_VERSION}. Remove it and retry.")
+            load = lambda name: np.load(io.BytesIO(z.read(name)), allow_pickle=False)
+            fz, rz, sz = load("forest.npz"), load("reference.npz"), load("signatures.npz")
+            z.extract("mutations.psx", d)
+        except (KeyError, ValueError, json.JSONDecodeError) as e:
+            if isinstance(e, ValueError) and str(e).startswith(("The phylogenetic", "Wrong file format")):
+                raise
+            raise wrong from None
+        ref = ReferenceGenome(dict(zip(rz["names"].tolist(), rz["lengths"].tolist())), seed=int(rz["seed"][0]))
+        if "packed" in rz.files:
+            ref.packed = rz["packed"]
+        sample = [s if h else None for s, h in zip(fz["sample"].tolist(), fz["has_sample"].tolist())]
+        sf = SamplesForest(fz["cell_id"], fz["ancestor"], fz["mutant"].tolist(), sample, fz["birth_time"],
+                           epistate=fz["epistate"].tolist())
+        eng = MutationEngine(ref, dict(zip(sz["sbs_names"].tolist(), sz["sbs"])), ctx=ctx,
+                             ID_signatures=dict(zip(sz["id_names"].tolist(), sz["id"])) or None)
+        f = core.Forest(eng.ctx, sf.cell_id, sf.ancestor, sf.species_index(), sf.sample_index(sf.sample_names),
+                        sf.birth_time, len(sf.sample_names))
+        m = core.Mutations.load(eng.ctx, os.path.join(d, "mutations.psx"))
+    return PhylogeneticForest(eng, sf, f, m, seed=int(meta["seed"]))
+
 
 # --------------------------------------------------------------- sequencers
 

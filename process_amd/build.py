@@ -17,7 +17,7 @@ OBJ = PKG / "_build"
 LIB = PKG / "libpsx.so"
 
 SOURCES = ["psx_host.cpp", "psx_genome.hip", "psx_place.hip", "psx_seq.hip", "psx_serve.cpp", "psx_noalloc.hip", "psx_probe.hip",
-           "psx_sam.cpp", "psx_context.hip"]
+           "psx_sam.cpp", "psx_context.hip", "psx_store.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

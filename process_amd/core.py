@@ -4,6 +4,7 @@ process_amd.api is built on these."""
 from __future__ import annotations
 
 import ctypes as C
+import os
 import threading
 
 import numpy as np
@@ -244,6 +245,20 @@ class Mutations(_Handle):
         h = C.c_void_p()
         check(lib().psx_mutations_upload(ctx.h, forest.h, r.ctypes.data_as(C.c_void_p), len(r), ploidy,
                                          C.byref(h)), ctx.h)
+        return cls(ctx, h, ploidy)
+
+    def save(self, path) -> None:
+        """Writes the table (records, copy-number events, chromosome lengths,
+        ploidy, leaves) to a file (psx_mutations_save)."""
+        check(lib().psx_mutations_save(self.ctx.h, self.h, os.fsencode(path)), self.ctx.h)
+
+    @classmethod
+    def load(cls, ctx: Context, path) -> "Mutations":
+        """A table written by save, back in device memory (psx_mutations_load)."""
+        h = C.c_void_p()
+        check(lib().psx_mutations_load(ctx.h, os.fsencode(path), C.byref(h)), ctx.h)
+        with open(path, "rb") as f:  # the header's ploidy (bytes 12..15; the library has checked the file)
+            ploidy = int(np.frombuffer(f.read(16)[12:16], np.uint32)[0])
         return cls(ctx, h, ploidy)
 
     def cna(self) -> np.ndarray:
