@@ -328,6 +328,15 @@ void psx_mutations_free(psx_mutations* m);
 #define PSX_TABLE_FORMAT_VERSION 1u
 psx_status psx_mutations_save(psx_ctx* ctx, const psx_mutations* m, const char* path);
 psx_status psx_mutations_load(psx_ctx* ctx, const char* path, psx_mutations** out);
+/* Asks the call running on ctx (from another thread) to stop at its next
+ * work-chunk boundary with PSX_ECANCELLED ("cancelled"): psx_write_sam checks
+ * between chromosomes and between chunks of reads, psx_autotune_seq between
+ * trials. The call that observes the request consumes it; a request made
+ * while none of those calls runs stops the next one at its first boundary.
+ * No counterpart in the reference, whose only interruption point is a user
+ * interrupt checked every 10,000 events of the spatial simulation
+ * (src/simulation.cpp:28-50), outside this path. */
+psx_status psx_ctx_cancel(psx_ctx* ctx);
 /* Milliseconds of the placement kernels of the last psx_place_mutations. */
 psx_status psx_ctx_last_place_ms(const psx_ctx* ctx, double* ms);
 

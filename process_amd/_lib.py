@@ -35,6 +35,7 @@ def _declare(lib):
         "psx_ctx_synchronize": (st, [vp]),
         "psx_ctx_last_kernel_ms": (st, [vp, C.POINTER(C.c_double)]),
         "psx_ctx_last_place_ms": (st, [vp, C.POINTER(C.c_double)]),
+        "psx_ctx_cancel": (st, [vp]),
         "psx_ctx_mem_info": (st, [vp, A.u64p, A.u64p]),
         "psx_dev_alloc": (st, [vp, C.c_uint64, pp]),
         "psx_dev_free": (None, [vp, vp]),

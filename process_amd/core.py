@@ -55,6 +55,11 @@ class Context:
         check(lib().psx_ctx_last_place_ms(self.h, C.byref(v)), self.h)
         return v.value
 
+    def cancel(self) -> None:
+        """Stops the call running on this context (from another thread) at its
+        next work-chunk boundary with PsxError(PSX_ECANCELLED) (psx_ctx_cancel)."""
+        check(lib().psx_ctx_cancel(self.h), self.h)
+
     def mem_info(self) -> tuple[int, int]:
         f, t = C.c_uint64(), C.c_uint64()
         check(lib().psx_ctx_mem_info(self.h, C.byref(f), C.byref(t)), self.h)

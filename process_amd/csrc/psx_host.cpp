@@ -83,6 +83,12 @@ extern "C" psx_status psx_ctx_last_kernel_ms(const psx_ctx* ctx, double* ms) {
     return PSX_OK;
 }
 
+extern "C" psx_status psx_ctx_cancel(psx_ctx* ctx) {
+    if (!ctx) return PSX_EINVAL;
+    ctx->cancel.store(true);
+    return PSX_OK;
+}
+
 extern "C" psx_status psx_ctx_last_place_ms(const psx_ctx* ctx, double* ms) {
     if (!ctx || !ms) return PSX_EINVAL;
     *ms = ctx->last_place_ms;

@@ -5,6 +5,7 @@
 
 #include <cstdarg>
 #include <cstdint>
+#include <atomic>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -87,7 +88,13 @@ struct psx_ctx {
     size_t host_io_bytes = 0;
     bool tuned = false;
     psx_launch_config tuned_cfg{};
+    std::atomic<bool> cancel{false};  // psx_ctx_cancel: the running call stops at its next work-chunk boundary
 };
+
+// Throws PSX_ECANCELLED when a cancellation was requested (and consumes the request).
+inline void psx_check_cancel(psx_ctx* ctx) {
+    if (ctx->cancel.exchange(false)) psx_throw(PSX_ECANCELLED, "cancelled");
+}
 
 // ctx's pinned readback area, at least bytes long (grown after a stream synchronisation, never shrunk).
 uint8_t* psx_host_io(psx_ctx* ctx, size_t bytes);

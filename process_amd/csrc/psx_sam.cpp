@@ -170,6 +170,7 @@ extern "C" psx_status psx_write_sam(psx_ctx* ctx, const psx_genome* g, const psx
         const uint32_t n_threads = std::max(1u, std::min(MAX_THREADS, std::thread::hardware_concurrency()));
         uint64_t total = 0;
         for (size_t ci = 0; ci < chrs.size(); ++ci) {
+            psx_check_cancel(ctx);
             const uint32_t c = chrs[ci];
             psx_seq_params pc = *p;
             pc.n_chromosomes = 1;
@@ -209,6 +210,7 @@ extern "C" psx_status psx_write_sam(psx_ctx* ctx, const psx_genome* g, const psx
             if (n_chunks) issue(0);
             std::vector<std::string> parts(n_threads);
             for (uint64_t k = 0; k < n_chunks; ++k) {
+                psx_check_cancel(ctx);
                 PSX_HIP(hipEventSynchronize((k & 1) ? ctx->ev1 : ctx->ev0));
                 if (k + 1 < n_chunks) issue(k + 1);
                 const uint64_t r0 = k * chunk_t, cnt = std::min(chunk_t, n - r0);
@@ -249,6 +251,7 @@ extern "C" psx_status psx_write_sam(psx_ctx* ctx, const psx_genome* g, const psx
         }
         if (n_reads) *n_reads = total;
     });
+    if (st != PSX_OK) (void)hipStreamSynchronize(ctx->stream);  // a chunk's copy may still be in flight
     if (d_reads) (void)hipFree(d_reads);
     if (staging) (void)hipHostFree(staging);
     return st;

@@ -949,6 +949,7 @@ extern "C" psx_status psx_autotune_seq(psx_ctx* ctx, const psx_seq_job* jobs, ui
                     for (uint32_t bpc : bpcs) {
                         if (b == 512 && bpc > 8) continue;
                         const psx_launch_config c{b, cap, spi, bpc, 0};
+                        psx_check_cancel(ctx);
                         double t = 0.0;
                         for (int rep = 0; rep < 2; ++rep) t += run_count(ctx, in, c).kernel_ms;
                         if (t < best_t) { best_t = t; bc = c; }

@@ -194,3 +194,33 @@ def test_sam_with_copy_number_events(ctx, tmp_path):
     # the reads of a whole-genome call, all chromosomes at once
     _, _, _, n = O.simulate_seq(lens, of, pf.records(), 7, cna=ev, **seq)
     assert n_before == n
+
+
+def test_cancel_between_chunks(ctx, tmp_path):
+    """A cancellation requested on the context (psx_ctx_cancel) stops the
+    writer at its first chunk boundary with PSX_ECANCELLED and no file
+    written; the request is consumed, so the next call writes every file,
+    identical to a call that was never cancelled. The autotuner stops the
+    same way between its trials."""
+    from process_amd._lib import PsxError
+
+    eng, sf = build()
+    pf = eng.place_mutations(sf, 4, 1, seed=8)
+    kw = dict(sequencer=pa.BasicIlluminaSequencer(2e-3, random_quality_scores=False), coverage=2,
+              insert_size_mean=350, insert_size_stddev=10, write_SAM=True, seed=17)
+    eng.ctx.cancel()
+    with pytest.raises(PsxError, match="cancelled") as e:
+        pa.simulate_seq(pf, output_dir=str(tmp_path / "a"), **kw)
+    assert e.value.status == 4  # PSX_ECANCELLED
+    assert not os.path.exists(tmp_path / "a") or os.listdir(tmp_path / "a") == []
+    pa.simulate_seq(pf, output_dir=str(tmp_path / "a"), **kw)
+    pa.simulate_seq(pf, output_dir=str(tmp_path / "b"), **kw)
+    assert sorted(os.listdir(tmp_path / "a")) == ["chr_21.sam", "chr_22.sam"]
+    for name in os.listdir(tmp_path / "a"):
+        assert (tmp_path / "a" / name).read_bytes() == (tmp_path / "b" / name).read_bytes()
+    core = pa.api.core
+    job = core.SeqJob(eng.ctx, eng.genome(), pf.forest, pf.mutations, 3, coverage=5.0, read_size=150,
+                      error_rate=1e-3)
+    eng.ctx.cancel()
+    with pytest.raises(PsxError, match="cancelled"):
+        core.autotune(eng.ctx, [job])
