@@ -217,6 +217,12 @@ const psx_mut_index* psx_mutations_find_index(psx_mutations* m, const std::vecto
     return nullptr;
 }
 
+std::vector<uint64_t> psx_index_layout(const std::vector<uint32_t>& chr_len) {
+    std::vector<uint64_t> off(chr_len.size() + 1, 0);
+    for (size_t c = 0; c < chr_len.size(); ++c) off[c + 1] = off[c] + (chr_len[c] + PSX_STRATUM - 1) / PSX_STRATUM + 1;
+    return off;
+}
+
 const psx_mut_index& psx_mutations_index(psx_ctx* ctx, psx_mutations* m, const std::vector<uint32_t>& chr_len,
                                          bool sync) {
     const uint32_t n_chr = (uint32_t)chr_len.size();
@@ -229,13 +235,13 @@ const psx_mut_index& psx_mutations_index(psx_ctx* ctx, psx_mutations* m, const s
     bucket_args a{};
     a.pos = m->pos;
     a.n_chr = n_chr;
-    ix->bucket_off.assign(n_chr + 1, 0);
+    ix->bucket_off = psx_index_layout(chr_len);
     a.chr_begin = m->d_chr_begin;
-    for (uint32_t c = 0; c < n_chr; ++c)
-        ix->bucket_off[c + 1] = ix->bucket_off[c] + (chr_len[c] + PSX_STRATUM - 1) / PSX_STRATUM + 1;
     for (uint32_t c = 0; c <= n_chr; ++c) a.boff[c] = ix->bucket_off[c];
-    ix->d_bucket.alloc(ix->bucket_off[n_chr] * 4u);
-    a.bucket = ix->d_bucket.as<uint32_t>();
+    ix->store = std::make_shared<dbuf>();
+    ix->store->alloc(ix->bucket_off[n_chr] * 4u);
+    ix->bucket = ix->store->as<uint32_t>();
+    a.bucket = ix->bucket;
     hipLaunchKernelGGL(k_mut_bucket, dim3((uint32_t)((ix->bucket_off[n_chr] + 255) / 256)), dim3(256), 0,
                        ctx->stream, a);
     PSX_HIP(hipGetLastError());

@@ -83,6 +83,7 @@ struct psx_ctx {
     dbuf queue;         // persistent-kernel work counters
     dbuf place_nodes;   // placement temporaries per node and per mutation (grown, kept for later calls)
     dbuf place_muts;
+    dbuf place_tab;     // the job table and round state of the last placement call
     uint8_t* noalloc_scratch = nullptr;  // device address of the device's no-allocation scratch
     uint8_t* host_io = nullptr;          // pinned host memory for small device-to-host readbacks
     size_t host_io_bytes = 0;
@@ -169,9 +170,13 @@ struct psx_forest {
 // Per-stratum index of a mutation table for one set of chromosome lengths.
 struct psx_mut_index {
     std::vector<uint32_t> chr_len;
-    std::vector<uint64_t> bucket_off;   // per chromosome offset into d_bucket
-    dbuf d_bucket;                      // per chromosome: first record at or after k * PSX_STRATUM
+    std::vector<uint64_t> bucket_off;   // per chromosome offset into bucket
+    std::shared_ptr<dbuf> store;        // holds bucket (the indexes of a batched placement share one block)
+    uint32_t* bucket = nullptr;         // per chromosome: first record at or after k * PSX_STRATUM
 };
+
+// Bucket offsets of a stratum index for the given chromosome lengths.
+std::vector<uint64_t> psx_index_layout(const std::vector<uint32_t>& chr_len);
 
 // Mutation table in SID order: device SoA for the kernels, AoS for download.
 struct psx_mutations {

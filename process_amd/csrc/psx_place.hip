@@ -28,6 +28,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 
 #include "psx_id.h"
 #include "psx_internal.h"
@@ -361,15 +362,17 @@ __device__ uint32_t place_one(const place_args& a, uint64_t id, uint32_t att, ps
     return st;
 }
 
-// One thread per mutation, first attempt.
-__global__ void __launch_bounds__(256) k_place(const place_args a) {
-    const uint64_t id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= a.M) return;
-    psx_mutation r;
-    const uint32_t st = place_one(a, id, 0u, r);
-    if (st != ST_OK) atomicOr(a.status, st);
-    a.recs[id] = r;
-}
+// ------------------------------------------------------------- batched phases
+//
+// A call places a batch of jobs (forests over one genome) together. Every
+// per-node and per-mutation array is one global array holding each job's
+// range at its base, and each kernel runs once over every job: a thread finds
+// its job by a binary search over the bases. Mutations are numbered globally
+// (gid = the job's base + its id in the job). Sort keys carry the job index
+// above the padded position (bit pbits up), so one radix sort orders every
+// job's footprints (job-major, then by position), and since 2^pbits exceeds
+// the padded genome by at least 16, footprints of different jobs are at
+// least 16 apart and never touch.
 
 struct edge_args {
     uint64_t seed;
@@ -383,14 +386,58 @@ struct edge_args {
     const uint32_t* ind_off;
     const uint32_t* ind_n;
     const uint64_t* T;
-    uint32_t* cnt_snv;
+    uint32_t* cnt_snv;  // at the job's node base
     uint64_t* cnt_tot;
 };
 
-// Passenger counts of the edge into node v (none for roots).
-__global__ void __launch_bounds__(256) k_edge_counts(const edge_args a) {
-    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= a.n_nodes) return;
+// One job of the batch on the device.
+struct fjob {
+    place_args pa;        // recs, attempt, node_of, cnt_snv, node_off and status at the job's bases
+    edge_args ea;         // its passenger counts
+    uint64_t mbase;       // gid of its first mutation
+    uint64_t n_fixed;     // drivers and germline mutations (never lose)
+    uint32_t is;          // infinite-sites model
+    uint32_t n_chr;
+    psx_mutation* out;    // its table's records
+    uint32_t *pos, *lo, *hi, *info;  // and the kernels' arrays of them
+    uint64_t* chr_begin;  // its table's chromosome ranges (PSX_MAX_CHROMOSOMES + 1)
+    uint32_t* bucket;     // its table's stratum index
+};
+
+struct batch {
+    const fjob* jobs;
+    uint32_t nj;
+    uint32_t pbits;
+    const uint64_t* nbase;     // node bases (nj + 1)
+    const uint64_t* mbase;     // mutation bases (nj + 1)
+    const uint64_t* pstart;    // padded chromosome starts: boff[c] + 16 c
+    psx_mutation* recs;        // every job's records, at the mutation bases
+};
+
+// The job whose range holds g: the last j with base[j] <= g (base[nj] = total > g).
+__device__ __forceinline__ uint32_t job_of(const uint64_t* __restrict__ base, uint32_t nj, uint64_t g) {
+    uint32_t lo = 0, hi = nj;
+    while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (base[mid] <= g) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// Sort key of a record of job j: the job above the padded position (positions
+// of different chromosomes at least 16 apart, so footprints of at most 11
+// bases on different chromosomes never touch).
+__device__ __forceinline__ uint64_t jkey(const batch& B, uint32_t j, const psx_mutation& r) {
+    return ((uint64_t)j << B.pbits) + B.pstart[r.chr] + r.pos;
+}
+
+// Passenger counts of the edge into every node of every job (none for roots).
+__global__ void __launch_bounds__(256) k_edge_counts(const batch B, uint64_t n) {
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const uint32_t j = job_of(B.nbase, B.nj, g);
+    const edge_args& a = B.jobs[j].ea;
+    const uint32_t v = (uint32_t)(g - B.nbase[j]);
     if (a.parent[v] == PSX_NO_NODE) {
         a.cnt_snv[v] = 0;
         a.cnt_tot[v] = 0;
@@ -404,25 +451,29 @@ __global__ void __launch_bounds__(256) k_edge_counts(const edge_args a) {
     a.cnt_tot[v] = (uint64_t)ns + ni;
 }
 
-// Padded position of a record: pstart[c] = boff[c] + 16 c, so positions of
-// different chromosomes are at least 16 apart and footprints (at most 11
-// bases) on different chromosomes never touch.
-__device__ __forceinline__ uint64_t padded_pos(const uint64_t* pstart, const psx_mutation& r) {
-    return pstart[r.chr] + r.pos;
+// After an exclusive scan of every node's count (off, n + 1 entries): each
+// node's first passenger within its job, and each job's passenger count.
+__global__ void __launch_bounds__(256) k_node_bases(const batch B, uint64_t n, const uint64_t* __restrict__ off,
+                                                    uint64_t* __restrict__ off_job, uint64_t* __restrict__ pcount) {
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < n) off_job[g] = off[g] - off[B.nbase[job_of(B.nbase, B.nj, g)]];
+    if (g < B.nj) pcount[g] = off[B.nbase[g + 1]] - off[B.nbase[g]];
 }
 
-// node_of[i] = v for the passengers of node v (ids node_off[v] .. + count)
-__global__ void __launch_bounds__(256) k_node_expand(const uint64_t* __restrict__ node_off,
-                                                     const uint64_t* __restrict__ cnt, uint32_t n_nodes,
-                                                     uint32_t* __restrict__ node_of) {
-    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= n_nodes) return;
-    const uint64_t o = node_off[v], n = cnt[v];
-    for (uint64_t k = 0; k < n; ++k) node_of[o + k] = v;
+// node_of[i] = v for the passengers of node v of each job (ids off_job[v] .. + count)
+__global__ void __launch_bounds__(256) k_node_expand(const batch B, uint64_t n, const uint64_t* __restrict__ off_job,
+                                                     const uint64_t* __restrict__ cnt) {
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const uint32_t j = job_of(B.nbase, B.nj, g);
+    uint32_t* node_of = const_cast<uint32_t*>(B.jobs[j].pa.node_of);
+    const uint32_t v = (uint32_t)(g - B.nbase[j]);
+    const uint64_t o = off_job[g], c = cnt[g];
+    for (uint64_t k = 0; k < c; ++k) node_of[o + k] = v;
 }
 
-// Sort entries of the infinite-sites check: key = padded position, value =
-// id | deleted length << 28 (ids are below 2^28).
+// Sort entries of the infinite-sites check: key = jkey, value = id in the job
+// | deleted length << 28 (ids are below 2^28).
 constexpr uint32_t ID_BITS = 28, ID_MASK = (1u << ID_BITS) - 1u, NO_ENTRY = 0xFFFFFFFFu;
 
 // deleted length: a footprint covers [start, start + length] (at most 11 bases)
@@ -437,26 +488,32 @@ __device__ __forceinline__ bool fp_touch(uint64_t sa, uint32_t la, uint64_t sb, 
     return sb <= sa + la + 1u && sa <= sb + lb + 1u;
 }
 
-__global__ void __launch_bounds__(256) k_footprint(const psx_mutation* __restrict__ recs, uint64_t M,
-                                                   const uint64_t* __restrict__ pstart, uint64_t* __restrict__ fkey,
-                                                   uint32_t* __restrict__ fval) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= M) return;
-    const psx_mutation r = recs[i];
-    fkey[i] = padded_pos(pstart, r);
-    fval[i] = fp_val((uint32_t)i, r);
+// One thread per mutation of every job, first attempt, and its sort entry.
+__global__ void __launch_bounds__(256) k_place(const batch B, uint64_t n, uint64_t* __restrict__ fkey,
+                                               uint32_t* __restrict__ fval) {
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const uint32_t j = job_of(B.mbase, B.nj, g);
+    const place_args& a = B.jobs[j].pa;
+    const uint64_t id = g - B.mbase[j];
+    psx_mutation r;
+    const uint32_t st = place_one(a, id, 0u, r);
+    if (st != ST_OK) atomicOr(a.status, st);
+    a.recs[id] = r;
+    fkey[g] = jkey(B, j, r);
+    fval[g] = fp_val((uint32_t)id, r);
 }
 
-// A loser of check number `stamp` (1 for the first): lose[id] holds the
+// A loser of check number `stamp` (1 for the first): lose[gid] holds the
 // number of the last check it lost (cleared once per call), so every loser is
 // listed once per check without clearing flags between checks. Beside each
-// listed id: the index of its first-check entry when it has not moved yet
+// listed gid: the index of its first-check entry when it has not moved yet
 // (found through that entry), else NO_ENTRY.
 __device__ __forceinline__ void mark_loser(uint32_t* lose, uint32_t stamp, uint32_t* list, uint32_t* apos,
-                                           uint32_t* n_lose, uint32_t id, uint32_t ap) {
-    if (atomicExch(&lose[id], stamp) != stamp) {
+                                           uint32_t* n_lose, uint32_t gid, uint32_t ap) {
+    if (atomicExch(&lose[gid], stamp) != stamp) {
         const uint32_t slot = atomicAdd(n_lose, 1u);
-        list[slot] = id;
+        list[slot] = gid;
         apos[slot] = ap;
     }
 }
@@ -479,24 +536,28 @@ __device__ __forceinline__ uint32_t block_reserve(uint32_t n, uint32_t* counter)
     return s_base + excl;
 }
 
-// First round of the infinite-sites model over footprints sorted by padded
-// position: footprints touching (gap < 2) an earlier-created mutation's
-// footprint lose; drivers and germline mutations (ids below n_fixed) never
-// lose. The losers' stamps are set to 1 (the caller clears them all first)
-// and their ids appended to the redraw list.
-__device__ __forceinline__ bool loses(const uint64_t* __restrict__ fkey, const uint32_t* __restrict__ fval,
-                                      uint64_t M, uint64_t n_fixed, uint64_t j, uint32_t& idj) {
-    const uint32_t vj = fval[j], lj = vj >> ID_BITS;
-    idj = vj & ID_MASK;
-    if (idj < n_fixed) return false;
-    const uint64_t sj = fkey[j], ej = sj + lj;
-    for (uint64_t i = j; i-- > 0;) {
+// First round of the infinite-sites model over every job's footprints sorted
+// by key: footprints touching (gap < 2) an earlier-created mutation's
+// footprint of the same job lose (neighbours of other jobs are too far to
+// touch); drivers and germline mutations (ids below n_fixed) never lose, nor
+// does anything of a job without the model. The losers' stamps are set to 1
+// (the caller clears them all first) and their gids appended to the redraw
+// list with their entry index.
+__device__ __forceinline__ bool loses(const batch& B, const uint64_t* __restrict__ fkey,
+                                      const uint32_t* __restrict__ fval, uint64_t n, uint64_t t, uint32_t& gid) {
+    const uint64_t sj = fkey[t];
+    const fjob& J = B.jobs[(uint32_t)(sj >> B.pbits)];
+    const uint32_t vj = fval[t], lj = vj >> ID_BITS, idj = vj & ID_MASK;
+    gid = (uint32_t)(J.mbase + idj);
+    if (!J.is || idj < J.n_fixed) return false;
+    const uint64_t ej = sj + lj;
+    for (uint64_t i = t; i-- > 0;) {
         const uint64_t si = fkey[i];
         if (si + 12u < sj) break;
         const uint32_t vi = fval[i];
         if ((vi & ID_MASK) < idj && fp_touch(si, vi >> ID_BITS, sj, lj)) return true;
     }
-    for (uint64_t i = j + 1; i < M; ++i) {
+    for (uint64_t i = t + 1; i < n; ++i) {
         const uint64_t si = fkey[i];
         if (si > ej + 1u) break;
         const uint32_t vi = fval[i];
@@ -505,27 +566,27 @@ __device__ __forceinline__ bool loses(const uint64_t* __restrict__ fkey, const u
     return false;
 }
 
-__global__ void __launch_bounds__(256) k_conflicts(const uint64_t* __restrict__ fkey,
-                                                   const uint32_t* __restrict__ fval, uint64_t M, uint64_t n_fixed,
+__global__ void __launch_bounds__(256) k_conflicts(const batch B, const uint64_t* __restrict__ fkey,
+                                                   const uint32_t* __restrict__ fval, uint64_t n,
                                                    uint32_t* __restrict__ lose, uint32_t* __restrict__ list,
                                                    uint32_t* __restrict__ apos, uint32_t* __restrict__ n_lose) {
     const uint64_t b = (uint64_t)blockIdx.x * 256u * APPEND_ITEMS + threadIdx.x;
-    uint32_t mask = 0, n = 0, ids[APPEND_ITEMS];
+    uint32_t mask = 0, c = 0, gids[APPEND_ITEMS];
 #pragma unroll
     for (uint32_t k = 0; k < APPEND_ITEMS; ++k) {
-        const uint64_t j = b + 256u * k;
-        ids[k] = 0;
-        if (j < M && loses(fkey, fval, M, n_fixed, j, ids[k])) {
+        const uint64_t t = b + 256u * k;
+        gids[k] = 0;
+        if (t < n && loses(B, fkey, fval, n, t, gids[k])) {
             mask |= 1u << k;
-            ++n;
+            ++c;
         }
     }
-    uint32_t slot = block_reserve(n, n_lose);
+    uint32_t slot = block_reserve(c, n_lose);
 #pragma unroll
     for (uint32_t k = 0; k < APPEND_ITEMS; ++k)
         if (mask & (1u << k)) {
-            lose[ids[k]] = 1u;
-            list[slot] = ids[k];
+            lose[gids[k]] = 1u;
+            list[slot] = gids[k];
             apos[slot] = (uint32_t)(b + 256u * k);
             ++slot;
         }
@@ -549,55 +610,54 @@ __device__ __forceinline__ uint64_t upper_bound_u64(const uint64_t* a, uint64_t 
     return lo;
 }
 
-// Moved mutations by the 32-base bucket of their padded start, in an
-// open-addressing table with linear probing (slots hold ids, NO_ENTRY when
-// empty; the host keeps it at most half full). Slots are never cleared while
-// rounds run: a mutation that moves again leaves its old entry behind, which
-// no longer touches anything because footprints are always read from the
-// mutation's current record.
+// Moved mutations by the 32-base bucket of their key, in an open-addressing
+// table with linear probing (slots hold gids, NO_ENTRY when empty; the host
+// keeps it at most half full). Slots are never cleared while rounds run: a
+// mutation that moves again leaves its old entry behind, which no longer
+// touches anything because footprints are always read from the mutation's
+// current record.
 constexpr uint32_t BUCKET_SHIFT = 5;
 
 __device__ __forceinline__ uint32_t tab_home(uint64_t bucket, uint32_t mask) {
     return (uint32_t)((bucket * 0x9E3779B97F4A7C15ull) >> 32) & mask;
 }
 
-__device__ __forceinline__ void tab_insert(uint32_t* tab, uint32_t mask, uint64_t start, uint32_t id) {
-    uint32_t h = tab_home(start >> BUCKET_SHIFT, mask);
-    for (uint32_t n = 0; n <= mask; ++n, h = (h + 1u) & mask)
-        if (atomicCAS(&tab[h], NO_ENTRY, id) == NO_ENTRY) return;
+__device__ __forceinline__ void tab_insert(uint32_t* tab, uint32_t mask, uint64_t key, uint32_t gid) {
+    uint32_t h = tab_home(key >> BUCKET_SHIFT, mask);
+    for (uint32_t k = 0; k <= mask; ++k, h = (h + 1u) & mask)
+        if (atomicCAS(&tab[h], NO_ENTRY, gid) == NO_ENTRY) return;
 }
 
-// Device state of the infinite-sites rounds of one job: rounds are launched
+// Device state of the infinite-sites rounds of a batch: rounds are launched
 // several at a time without the host in between, so each reads its loser
 // count here.
 struct round_io {
-    uint32_t status;    // placement status bits
     uint32_t n_moved;   // mutations moved so far (entries of the moved list)
     uint32_t tab_used;  // table entries since it was last cleared
     uint32_t halt;      // first round that found the table too full to run (0: none)
+    uint32_t pad;
     uint32_t n_lose[IS_ROUNDS + 1];  // losers of check r (0: the first check)
 };
 
 struct redraw_args {
-    uint32_t round;        // r >= 1: redraws the losers of check r - 1
-    const uint32_t* list;  // losers of check r - 1
-    const uint32_t* lapos; // and their first-check entries (NO_ENTRY once moved)
-    uint32_t* moved;       // set for every redrawn id
-    uint32_t* mlist;       // ids in the order they first moved
-    uint32_t* mapos;       // and the index of each one's first-check entry
-    const uint64_t* akey;  // the first check's sorted footprints: a mutation's entry is made NO_ENTRY
-    uint32_t* aval;        // when it first moves
-    uint64_t M;
+    uint32_t round;          // r >= 1: redraws the losers of check r - 1
+    const uint32_t* list;    // losers of check r - 1 (gids)
+    const uint32_t* lapos;   // and their first-check entries (NO_ENTRY once moved)
+    uint32_t* moved;         // set for every redrawn gid
+    uint32_t* mlist;         // gids in the order they first moved
+    uint32_t* mapos;         // and the index of each one's first-check entry
+    const uint64_t* akey;    // the first check's sorted entries: a mutation's entry is made NO_ENTRY
+    uint32_t* aval;          // when it first moves
+    uint64_t n;
     round_io* io;
     uint32_t* tab;
     uint32_t tab_mask;
-    const uint64_t* pstart;
 };
 
 // Round r: the losers' next attempts (grid-stride), each new footprint into
 // the table. The round does nothing after a halt, and halts when its entries
 // would fill the table beyond half.
-__global__ void __launch_bounds__(256) k_redraw(const place_args a, const redraw_args b) {
+__global__ void __launch_bounds__(256) k_redraw(const batch B, const redraw_args b) {
     round_io* io = b.io;
     const uint32_t n = io->n_lose[b.round - 1];
     if (io->halt || n == 0) return;
@@ -608,27 +668,31 @@ __global__ void __launch_bounds__(256) k_redraw(const place_args a, const redraw
     for (uint32_t base = blockIdx.x * 256u; base < n; base += gridDim.x * 256u) {
         const uint32_t t = base + threadIdx.x;
         const bool on = t < n;
-        const uint32_t id = on ? b.list[t] : 0u;
-        const bool first = on && atomicExch(&b.moved[id], 1u) == 0u;
+        const uint32_t gid = on ? b.list[t] : 0u;
+        const bool first = on && atomicExch(&b.moved[gid], 1u) == 0u;
         const uint32_t slot = block_reserve(first ? 1u : 0u, &io->n_moved);
+        const uint32_t j = job_of(B.mbase, B.nj, gid);
         if (first) {
-            uint64_t j = b.lapos[t];
-            if (j == NO_ENTRY) {  // not listed with it: found from the first-check footprint, still in its record
-                j = lower_bound_u64(b.akey, b.M, padded_pos(b.pstart, a.recs[id]));
-                while (j + 1 < b.M && (b.aval[j] & ID_MASK) != id) ++j;  // among the entries at that position
+            uint64_t e = b.lapos[t];
+            if (e == NO_ENTRY) {  // not listed with it: found from the first-check footprint, still in its record
+                e = lower_bound_u64(b.akey, b.n, jkey(B, j, B.recs[gid]));
+                const uint32_t id = (uint32_t)(gid - B.mbase[j]);
+                while (e + 1 < b.n && (b.aval[e] & ID_MASK) != id) ++e;  // among the entries at that key
             }
-            b.aval[j] = NO_ENTRY;
-            b.mlist[slot] = id;
-            b.mapos[slot] = (uint32_t)j;
+            b.aval[e] = NO_ENTRY;
+            b.mlist[slot] = gid;
+            b.mapos[slot] = (uint32_t)e;
         }
         if (on) {
+            const place_args& a = B.jobs[j].pa;
+            const uint64_t id = gid - B.mbase[j];
             const uint32_t att = a.attempt[id] + 1u;
             a.attempt[id] = att;
             psx_mutation r;
             const uint32_t st = place_one(a, id, att, r);
-            if (st != ST_OK) atomicOr(&io->status, st);
+            if (st != ST_OK) atomicOr(a.status, st);
             a.recs[id] = r;
-            tab_insert(b.tab, b.tab_mask, padded_pos(b.pstart, r), id);
+            tab_insert(b.tab, b.tab_mask, jkey(B, j, r), gid);
         }
         __syncthreads();  // block_reserve's shared slot is reused by the next pass
     }
@@ -636,26 +700,22 @@ __global__ void __launch_bounds__(256) k_redraw(const place_args a, const redraw
 
 // The table rebuilt (after the host cleared it) from the moved mutations'
 // current footprints; the halt, if any, is lifted.
-__global__ void __launch_bounds__(256) k_tab_fill(const uint32_t* __restrict__ mlist, uint32_t n,
-                                                  const psx_mutation* __restrict__ recs,
-                                                  const uint64_t* __restrict__ pstart, uint32_t* tab, uint32_t mask,
-                                                  round_io* io) {
+__global__ void __launch_bounds__(256) k_tab_fill(const batch B, const uint32_t* __restrict__ mlist, uint32_t n,
+                                                  uint32_t* tab, uint32_t mask, round_io* io) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t == 0) {
         io->tab_used = n;
         io->halt = 0;
     }
     if (t >= n) return;
-    const uint32_t id = mlist[t];
-    tab_insert(tab, mask, padded_pos(pstart, recs[id]), id);
+    const uint32_t gid = mlist[t];
+    tab_insert(tab, mask, jkey(B, job_of(B.mbase, B.nj, gid), B.recs[gid]), gid);
 }
 
 struct check_args {
-    const uint64_t* akey;  // first-round footprints, sorted (entries of moved mutations are NO_ENTRY)
+    const uint64_t* akey;  // first-round entries, sorted (entries of moved mutations are NO_ENTRY)
     const uint32_t* aval;
-    uint64_t M;
-    const psx_mutation* recs;
-    const uint64_t* pstart;
+    uint64_t n;
     const uint32_t* tab;
     uint32_t tab_mask;
     uint32_t round;        // r >= 1: checks the footprints redrawn in round r (check r)
@@ -669,114 +729,117 @@ struct check_args {
 // Later rounds: a pair of touching footprints has a member redrawn in the
 // last redraw (a touching pair left unchanged would have made its later
 // member lose and be redrawn), so only those are checked: each against the
-// first-round footprints of unmoved mutations and the moved ones in the
-// table. The later member of every touching pair loses, as in a full pass.
-__device__ __forceinline__ void check_one(const check_args& a, uint32_t idc) {
+// first-round entries of unmoved mutations and the moved ones in the table
+// (touching entries are of the same job, whose ids compare as gids do). The
+// later member of every touching pair loses, as in a full pass.
+__device__ __forceinline__ void check_one(const batch& B, const check_args& a, uint32_t gc) {
     const uint32_t stamp = a.round + 1u;
     uint32_t* n_out = &a.io->n_lose[a.round];
-    const psx_mutation rc = a.recs[idc];
-    const uint64_t sc = padded_pos(a.pstart, rc);
+    const uint32_t j = job_of(B.mbase, B.nj, gc);
+    const uint64_t mb = B.mbase[j];
+    const uint32_t idc = (uint32_t)(gc - mb);
+    const psx_mutation rc = B.recs[gc];
+    const uint64_t sc = jkey(B, j, rc);
     const uint32_t lc = fp_len(rc);
     const uint64_t from = sc >= 11u ? sc - 11u : 0u, to = sc + lc + 1u;  // starts of touching footprints
     bool l = false;
-    for (uint64_t i = lower_bound_u64(a.akey, a.M, from); i < a.M; ++i) {
+    for (uint64_t i = lower_bound_u64(a.akey, a.n, from); i < a.n; ++i) {
         const uint64_t si = a.akey[i];
         if (si > to) break;
         const uint32_t vi = a.aval[i], idi = vi & ID_MASK;
         if (vi == NO_ENTRY || !fp_touch(si, vi >> ID_BITS, sc, lc)) continue;
         if (idi < idc) l = true;
-        else mark_loser(a.lose, stamp, a.out, a.out_apos, n_out, idi, (uint32_t)i);
+        else mark_loser(a.lose, stamp, a.out, a.out_apos, n_out, (uint32_t)(mb + idi), (uint32_t)i);
     }
     for (uint64_t bk = from >> BUCKET_SHIFT; bk <= to >> BUCKET_SHIFT; ++bk) {
         uint32_t h = tab_home(bk, a.tab_mask);
-        for (uint32_t n = 0; n <= a.tab_mask; ++n, h = (h + 1u) & a.tab_mask) {
-            const uint32_t idi = a.tab[h];
-            if (idi == NO_ENTRY) break;
-            if (idi == idc) continue;
-            const psx_mutation ri = a.recs[idi];
-            if (!fp_touch(padded_pos(a.pstart, ri), fp_len(ri), sc, lc)) continue;
-            if (idi < idc) l = true;
-            else mark_loser(a.lose, stamp, a.out, a.out_apos, n_out, idi, NO_ENTRY);
+        for (uint32_t k = 0; k <= a.tab_mask; ++k, h = (h + 1u) & a.tab_mask) {
+            const uint32_t gi = a.tab[h];
+            if (gi == NO_ENTRY) break;
+            if (gi == gc) continue;
+            const psx_mutation ri = B.recs[gi];
+            if (!fp_touch(jkey(B, job_of(B.mbase, B.nj, gi), ri), fp_len(ri), sc, lc)) continue;
+            if (gi < gc) l = true;
+            else mark_loser(a.lose, stamp, a.out, a.out_apos, n_out, gi, NO_ENTRY);
         }
     }
-    if (l) mark_loser(a.lose, stamp, a.out, a.out_apos, n_out, idc, NO_ENTRY);
+    if (l) mark_loser(a.lose, stamp, a.out, a.out_apos, n_out, gc, NO_ENTRY);
 }
 
 // Check r (grid-stride over the footprints redrawn in round r); nothing after a halt.
-__global__ void __launch_bounds__(256) k_check(const check_args a) {
+__global__ void __launch_bounds__(256) k_check(const batch B, const check_args a) {
     round_io* io = a.io;
     const uint32_t n = io->n_lose[a.round - 1];
     if (io->halt || n == 0) return;
     if (blockIdx.x == 0 && threadIdx.x == 0) io->tab_used += n;  // k_redraw's entries
     for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x)
-        check_one(a, a.list[t]);
+        check_one(B, a, a.list[t]);
 }
 
-// SID order after infinite-sites rounds: the moved mutations' current
-// positions (sorted next) ...
-__global__ void __launch_bounds__(256) k_moved_keys(const uint32_t* __restrict__ mlist, uint32_t n,
-                                                    const psx_mutation* __restrict__ recs,
-                                                    const uint64_t* __restrict__ pstart, uint64_t* __restrict__ key,
-                                                    uint32_t* __restrict__ ids) {
+// SID order: the moved mutations' current keys (sorted next) ...
+__global__ void __launch_bounds__(256) k_moved_keys(const batch B, const uint32_t* __restrict__ mlist, uint32_t n,
+                                                    uint64_t* __restrict__ key, uint32_t* __restrict__ gids) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
-    const uint32_t id = mlist[t];
-    key[t] = padded_pos(pstart, recs[id]);
-    ids[t] = id;
+    const uint32_t gid = mlist[t];
+    key[t] = jkey(B, job_of(B.mbase, B.nj, gid), B.recs[gid]);
+    gids[t] = gid;
 }
 
-// ... then the output slots by one scan over w[0..M]: w[j] = 1 for a
+// ... then the output slots by one scan over w[0..n]: w[i] = 1 for a
 // first-check entry still current (0 for the moved ones') plus the number of
-// moved positions whose upper bound among the first-check keys is j. With S
-// the exclusive sum of w, a current entry j goes to S[j + 1] - 1 and the
-// i-th moved position (upper bound u, the first of its bound at i0) to
-// S[u] - i0 + i: both count the current entries and moved positions before
-// them, ties put first-check entries first.
+// moved keys whose upper bound among the first-check keys is i. With S the
+// exclusive sum of w, a current entry i goes to S[i + 1] - 1 and the k-th
+// moved key (upper bound u, the first of its bound at k0) to S[u] - k0 + k:
+// both count the current entries and moved keys before them, ties put
+// first-check entries first. The output is job-major, each job's range at its
+// mutation base.
 __global__ void __launch_bounds__(256) k_merge_marks(const uint32_t* __restrict__ mapos,
-                                                     const uint64_t* __restrict__ akey, uint64_t M,
+                                                     const uint64_t* __restrict__ akey, uint64_t n,
                                                      const uint64_t* __restrict__ bkey, uint32_t nb, uint32_t* w,
                                                      uint32_t* __restrict__ ub) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nb) return;
-    atomicSub(&w[mapos[i]], 1u);
-    const uint64_t u = upper_bound_u64(akey, M, bkey[i]);
-    ub[i] = (uint32_t)u;
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nb) return;
+    atomicSub(&w[mapos[k]], 1u);
+    const uint64_t u = upper_bound_u64(akey, n, bkey[k]);
+    ub[k] = (uint32_t)u;
     atomicAdd(&w[u], 1u);
 }
 
 struct merge_args {
-    const uint64_t* akey;  // first-round footprints, sorted (moved ones NO_ENTRY)
+    const uint64_t* akey;  // first-check entries, sorted (moved ones NO_ENTRY)
     const uint32_t* aval;
-    uint64_t M;
-    const uint32_t* S;     // exclusive sum of w (M + 1 entries)
-    const uint64_t* bkey;  // moved mutations' current positions, sorted
-    const uint32_t* bval;
+    uint64_t n;
+    const uint32_t* S;     // exclusive sum of w (n + 1 entries)
+    const uint64_t* bkey;  // moved mutations' current keys, sorted
+    const uint32_t* bval;  // and gids
     const uint32_t* ub;    // their upper bounds among akey
     uint32_t nb;
     uint64_t* okey;
-    uint32_t* oid;
+    uint32_t* ogid;
 };
 
-// One thread per entry of either; okey/oid receive M entries. Without moved
+// One thread per entry of either; okey/ogid receive n entries. Without moved
 // mutations the first check's order is the output.
-__global__ void __launch_bounds__(256) k_merge(const merge_args a) {
+__global__ void __launch_bounds__(256) k_merge(const batch B, const merge_args a) {
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < a.M) {
+    if (t < a.n) {
         const uint32_t v = a.aval[t];
         if (v == NO_ENTRY) return;
+        const uint64_t k = a.akey[t];
         const uint64_t o = a.nb ? a.S[t + 1] - 1u : t;
-        a.okey[o] = a.akey[t];
-        a.oid[o] = v & ID_MASK;
-    } else if (t < a.M + a.nb) {
-        const uint32_t i = (uint32_t)(t - a.M), u = a.ub[i];
-        uint32_t lo = 0, hi = i;  // first moved position with the same bound
+        a.okey[o] = k;
+        a.ogid[o] = (uint32_t)(B.jobs[(uint32_t)(k >> B.pbits)].mbase + (v & ID_MASK));
+    } else if (t < a.n + a.nb) {
+        const uint32_t k = (uint32_t)(t - a.n), u = a.ub[k];
+        uint32_t lo = 0, hi = k;  // first moved key with the same bound
         while (lo < hi) {
             const uint32_t mid = (lo + hi) >> 1;
             if (a.ub[mid] < u) lo = mid + 1; else hi = mid;
         }
-        const uint64_t o = (uint64_t)a.S[u] - lo + i;
-        a.okey[o] = a.bkey[i];
-        a.oid[o] = a.bval[i];
+        const uint64_t o = (uint64_t)a.S[u] - lo + k;
+        a.okey[o] = a.bkey[k];
+        a.ogid[o] = a.bval[k];
     }
 }
 
@@ -785,20 +848,11 @@ __global__ void __launch_bounds__(256) k_fill_u32(uint32_t* __restrict__ p, uint
     if (i < n) p[i] = i + 1 == n ? last : v;
 }
 
-__global__ void __launch_bounds__(256) k_pos_key(const psx_mutation* __restrict__ recs, uint64_t M,
-                                                 const uint64_t* __restrict__ pstart, uint64_t* __restrict__ key,
-                                                 uint32_t* __restrict__ ids) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= M) return;
-    key[i] = padded_pos(pstart, recs[i]);
-    ids[i] = (uint32_t)i;
-}
-
-// SID order within runs of one position: after a stable sort by position,
-// the thread at the start of each run sorts the run by (kind, length,
-// alternative sequence, creation order) and flags the first record of each
-// SID (runs are short: several records share a position only without the
-// infinite-sites model or among drivers and germline mutations).
+// SID order within runs of one key: after the position order (stable: ties
+// in creation order), the thread at the start of each run sorts the run by
+// (kind, length, alternative sequence, creation order) and flags the first
+// record of each SID (runs are short: several records share a position only
+// without the infinite-sites model or among drivers and germline mutations).
 __device__ __forceinline__ uint32_t sid_rest(const psx_mutation& r) {
     const uint32_t payload = r.kind == PSX_SNV ? r.alt : (r.kind == PSX_INS ? r.seq : 0u);
     return ((uint32_t)r.kind << 24) | ((uint32_t)(r.len & 15u) << 20) | (payload & 0xFFFFFu);
@@ -834,22 +888,68 @@ __global__ void __launch_bounds__(256) k_sid_runs(const psx_mutation* __restrict
     }
 }
 
-__global__ void __launch_bounds__(256) k_gather(const psx_mutation* __restrict__ src,
-                                                const uint32_t* __restrict__ ids, const uint8_t* __restrict__ head,
-                                                uint64_t M, psx_mutation* __restrict__ dst) {
-    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= M) return;
-    psx_mutation r = src[ids[j]];
-    r.sid_head = head[j];
-    dst[j] = r;
+// Every job's records in SID order into its table, and the kernels' arrays
+// of them (the info word as the sequencing kernels read it: allele | ref << 8
+// | alt << 10 | kind << 12 | nature << 14 | SID head << 16 | alone at its
+// position << 17).
+__global__ void __launch_bounds__(256) k_gather(const batch B, const uint64_t* __restrict__ key,
+                                                const uint32_t* __restrict__ gids, const uint8_t* __restrict__ head,
+                                                uint64_t n) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const uint32_t j = job_of(B.mbase, B.nj, t);
+    const uint64_t lo = B.mbase[j], hi = B.mbase[j + 1], i = t - lo;
+    const fjob& J = B.jobs[j];
+    psx_mutation r = B.recs[gids[t]];
+    r.sid_head = head[t];
+    J.out[i] = r;
+    J.pos[i] = r.pos;
+    J.lo[i] = r.leaf_lo;
+    J.hi[i] = r.leaf_hi;
+    const bool solo = (t == lo || key[t - 1] != key[t]) && (t + 1 == hi || key[t + 1] != key[t]);
+    J.info[i] = uint32_t(r.allele) | (uint32_t(r.ref & 3u) << 8) | (uint32_t(r.alt & 3u) << 10) |
+                (uint32_t(r.kind & 3u) << 12) | (uint32_t(r.nature & 3u) << 14) | (uint32_t(r.sid_head & 1u) << 16) |
+                (uint32_t(solo) << 17);
 }
 
-// out[c] = first record of chromosome c in position order (c < n_chr), M beyond
-__global__ void k_chr_begin(const uint64_t* __restrict__ key, uint64_t M, const uint64_t* __restrict__ pstart,
-                            uint32_t n_chr, uint64_t* __restrict__ out) {
-    const uint32_t c = threadIdx.x;
-    if (c > PSX_MAX_CHROMOSOMES) return;
-    out[c] = c < n_chr ? lower_bound_u64(key, M, pstart[c]) : M;
+struct index_args {
+    uint32_t n_chr;
+    uint64_t nb;                              // entries per table
+    uint64_t boff[PSX_MAX_CHROMOSOMES + 1];  // first entry of each chromosome
+};
+
+// Every table's stratum index: bucket[boff[c] + k] = first record of
+// chromosome c at or after position k * PSX_STRATUM (a record index).
+__global__ void __launch_bounds__(256) k_index(const batch B, const index_args a) {
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= a.nb * B.nj) return;
+    const uint32_t j = (uint32_t)(g / a.nb);
+    const uint64_t q = g - (uint64_t)j * a.nb;
+    const fjob& J = B.jobs[j];
+    uint32_t c = 0;
+    while (q >= a.boff[c + 1]) ++c;
+    const uint64_t target = (q - a.boff[c]) * PSX_STRATUM;
+    uint64_t lo = J.chr_begin[c], hi = J.chr_begin[c + 1];
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if ((uint64_t)J.pos[mid] < target) lo = mid + 1; else hi = mid;
+    }
+    J.bucket[q] = (uint32_t)lo;
+}
+
+// Chromosome ranges of every job's table: first record of chromosome c in its
+// position order (c < n_chr), its size beyond; also into a readback array.
+__global__ void __launch_bounds__(256) k_chr_begin(const batch B, const uint64_t* __restrict__ key,
+                                                   uint64_t* __restrict__ readback) {
+    constexpr uint32_t W = PSX_MAX_CHROMOSOMES + 1;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= B.nj * W) return;
+    const uint32_t j = t / W, c = t - j * W;
+    const uint64_t lo = B.mbase[j], M = B.mbase[j + 1] - lo;
+    const fjob& J = B.jobs[j];
+    const uint64_t v = c < J.n_chr ? lower_bound_u64(key + lo, M, ((uint64_t)j << B.pbits) + B.pstart[c]) : M;
+    J.chr_begin[c] = v;
+    readback[t] = v;
 }
 
 // Bits that represent x (at least 1).
@@ -997,26 +1097,10 @@ struct place_job {
     std::vector<uint64_t> check_pos;  // global positions of the drivers, then the germline mutations
     // scratch offsets
     size_t o_T = 0, o_sk0 = 0, o_soff = 0, o_sn = 0, o_ik0 = 0, o_ioff = 0, o_in = 0, o_epoch = 0, o_drvd = 0,
-           o_drvv = 0, o_drivers = 0, o_germ = 0, o_io = 0, o_idc = 0, o_idl = 0, o_ido = 0, o_drvpos = 0;
-    // sizes
-    uint64_t n_pass = 0, n_fixed = 0, M = 0;
-    // device state
+           o_drvv = 0, o_drivers = 0, o_germ = 0, o_idc = 0, o_idl = 0, o_ido = 0, o_drvpos = 0;
+    // sizes and bases in the batch's global arrays
+    uint64_t n_pass = 0, n_fixed = 0, M = 0, mbase = 0, pbase = 0;
     place_args pa{};
-    uint32_t* d_cnt_snv = nullptr;
-    uint64_t *d_cnt_tot = nullptr, *d_off = nullptr;
-    uint64_t *d_key = nullptr, *d_key2 = nullptr, *d_cb = nullptr;
-    uint32_t *d_ids = nullptr, *d_ids2 = nullptr, *d_moved = nullptr, *d_mlist = nullptr, *d_mapos = nullptr;
-    uint32_t *d_w = nullptr, *d_S = nullptr;
-    uint32_t *d_node_of = nullptr, *d_att = nullptr, *d_lose = nullptr, *d_tab = nullptr;
-    uint32_t* d_list[2] = {nullptr, nullptr};
-    uint32_t* d_lapos[2] = {nullptr, nullptr};  // first-check entries of the listed losers
-    round_io* d_io = nullptr;
-    uint8_t* d_head = nullptr;
-    // infinite-sites rounds: the next round to run (its losers, of check round - 1, are in
-    // d_list[(round - 1) & 1]), the end of the last launched ones, moved mutations, the table's slots
-    uint32_t round = 1, spec_end = 1, n_lose = 0, n_moved = 0;
-    uint64_t tab_slots = 0;
-    bool halted = false;
 };
 
 [[noreturn]] void job_throw(const place_job& J, uint32_t n_jobs, psx_status st, const std::string& msg) {
@@ -1240,8 +1324,6 @@ void put_job(scratch_builder& sb, place_job& J) {
     J.o_drvd = sb.put_vec(J.drv_d); J.o_drvv = sb.put_vec(J.drv_v);
     J.o_drivers = sb.put(p->drivers, sizeof(psx_driver) * p->n_drivers);
     J.o_germ = sb.put(p->germline, sizeof(psx_germline) * p->n_germline);
-    const round_io zero{};
-    J.o_io = sb.put(&zero, sizeof zero);
     if (J.use_id) {
         J.o_idc = sb.put(J.idx->count, sizeof J.idx->count);
         J.o_idl = sb.put(J.idx->listed, sizeof J.idx->listed);
@@ -1288,33 +1370,65 @@ void place_batch(psx_ctx* ctx, const psx_genome* g, std::vector<place_job>& jobs
     const size_t o_check = sb.put_vec(check_pos);
     uint8_t* base = (uint8_t*)psx_scratch_upload(ctx, sb);
     auto at = [&](size_t off) { return (void*)(base + off); };
-    const uint64_t* d_pstart = (const uint64_t*)at(o_pstart);
+    // sort keys: the padded position below bit pbits (at least 16 to spare), the job above
+    const uint64_t p_max = pstart[g->n_chr - 1] + g->len[g->n_chr - 1];
+    const uint32_t pbits = (uint32_t)bits_for(p_max + 16), kbits = pbits + (uint32_t)bits_for(nj - 1);
+    if (kbits > 64) psx_throw(PSX_EINVAL, "too many jobs for one call (%u)", nj);
 
-    // pinned readback: per job {last offset, last count, round state, chromosome ranges}, then the checked
-    // reference bases
-    constexpr size_t RIO = (sizeof(round_io) + 7) / 8;
-    const size_t io_job = 2 + RIO + (PSX_MAX_CHROMOSOMES + 1);
-    uint64_t* io = (uint64_t*)psx_host_io(ctx, 8 * io_job * nj + check_pos.size() + 8);
-    uint8_t* io_bases = (uint8_t*)(io + io_job * nj);
+    // node bases, the job table and the batch's round state (one device buffer)
+    std::vector<uint64_t> nbase(nj + 1, 0), mbase(nj + 1, 0);
+    for (uint32_t k = 0; k < nj; ++k) nbase[k + 1] = nbase[k] + jobs[k].f->n_nodes;
+    const uint64_t N = nbase[nj];
+    carver ct;
+    const size_t t_jobs = ct.take(sizeof(fjob) * nj), t_nbase = ct.take(8 * (nj + 1)),
+                 t_mbase = ct.take(8 * (nj + 1)), t_state = ct.take(sizeof(round_io) + 4 * nj);
+    ctx->place_tab.reserve(ct.bytes);
+    uint8_t* dt = (uint8_t*)ctx->place_tab.p;
+    round_io* d_io = (round_io*)(dt + t_state);
+    uint32_t* d_status = (uint32_t*)(d_io + 1);
+    batch B{};
+    B.jobs = (const fjob*)(dt + t_jobs);
+    B.nj = nj;
+    B.pbits = pbits;
+    B.nbase = (const uint64_t*)(dt + t_nbase);
+    B.mbase = (const uint64_t*)(dt + t_mbase);
+    B.pstart = (const uint64_t*)at(o_pstart);
+
+    // pinned host area: the readbacks (the round state and every job's status, the jobs' passenger counts,
+    // their chromosome ranges, the checked reference bases), then the job table staged in the device layout
+    constexpr uint32_t W = PSX_MAX_CHROMOSOMES + 1;
+    const size_t io_state = sizeof(round_io) + 4 * nj;
+    const size_t r_pcount = (io_state + 7) / 8, r_chr = r_pcount + nj, r_bases = r_chr + (size_t)nj * W;
+    const size_t r_stage = r_bases + (check_pos.size() + 8 + 255) / 8 / 32 * 32 + 32;  // past the bases
+    uint64_t* io = (uint64_t*)psx_host_io(ctx, 8 * r_stage + t_state);
+    const round_io* h_io = (const round_io*)io;
+    const uint32_t* h_status = (const uint32_t*)(h_io + 1);
+    uint8_t* io_bases = (uint8_t*)(io + r_bases);
+    uint8_t* stage = (uint8_t*)(io + r_stage);
+    fjob* tab = (fjob*)(stage + t_jobs);
+    for (uint32_t k = 0; k < nj; ++k) tab[k] = fjob{};
+    // the job table, bases included, in one copy (the stream is idle or in order before each use)
+    auto upload_table = [&] {
+        std::memcpy(stage + t_nbase, nbase.data(), 8 * (nj + 1));
+        std::memcpy(stage + t_mbase, mbase.data(), 8 * (nj + 1));
+        PSX_HIP(hipMemcpyAsync(dt, stage, t_state, hipMemcpyHostToDevice, s));
+    };
 
     PSX_HIP(hipEventRecord(ctx->ev0, s));
-    // ---- phase 1: passenger counts per node, the driver and germline reference bases
+    // ---- phase 1: passenger counts per node of every job, the driver and germline reference bases
     size_t scan_bytes = 0;
-    uint32_t max_nodes = 1;
-    for (const place_job& J : jobs) max_nodes = std::max(max_nodes, J.f->n_nodes);
     PSX_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, (uint64_t*)nullptr, (uint64_t*)nullptr,
-                                             (int)max_nodes, s));
+                                             (int)(N + 1), s));
     carver cn;
-    const size_t c_scan = cn.take(scan_bytes), c_bases = cn.take(check_pos.size());
-    std::vector<size_t> c_node(nj);
-    for (uint32_t k = 0; k < nj; ++k) {
-        const size_t n = jobs[k].f->n_nodes;
-        c_node[k] = cn.take(n * 4);
-        cn.take(n * 8);
-        cn.take(n * 8);
-    }
+    const size_t c_scan = cn.take(scan_bytes), c_bases = cn.take(check_pos.size()), c_cs = cn.take(N * 4),
+                 c_ct = cn.take((N + 1) * 8), c_off = cn.take((N + 1) * 8), c_offj = cn.take(N * 8),
+                 c_pc = cn.take(nj * 8), c_cb = cn.take((size_t)nj * W * 8);
     ctx->place_nodes.reserve(cn.bytes);
     uint8_t* pn = (uint8_t*)ctx->place_nodes.p;
+    uint32_t* d_cnt_snv = (uint32_t*)(pn + c_cs);
+    uint64_t *d_cnt_tot = (uint64_t*)(pn + c_ct), *d_off = (uint64_t*)(pn + c_off),
+             *d_off_job = (uint64_t*)(pn + c_offj), *d_pcount = (uint64_t*)(pn + c_pc),
+             *d_cb = (uint64_t*)(pn + c_cb);
     if (!check_pos.empty()) {
         hipLaunchKernelGGL(k_bases_at, dim3(grid_of(check_pos.size())), dim3(256), 0, s, g->packed.as<uint32_t>(),
                            (const uint64_t*)at(o_woff), (const uint64_t*)at(o_boff), g->n_chr,
@@ -1324,23 +1438,24 @@ void place_batch(psx_ctx* ctx, const psx_genome* g, std::vector<place_job>& jobs
     }
     for (uint32_t k = 0; k < nj; ++k) {
         place_job& J = jobs[k];
-        const uint32_t n = J.f->n_nodes;
-        uint8_t* b = pn + c_node[k];
-        J.d_cnt_snv = (uint32_t*)b;
-        J.d_cnt_tot = (uint64_t*)(b + ((n * 4 + 255) & ~size_t(255)));
-        J.d_off = (uint64_t*)((uint8_t*)J.d_cnt_tot + ((n * 8 + 255) & ~size_t(255)));
-        edge_args ea{};
-        ea.seed = J.seed; ea.n_nodes = n; ea.parent = J.f->d_parent.as<uint32_t>();
+        edge_args& ea = tab[k].ea;
+        ea.seed = J.seed; ea.n_nodes = J.f->n_nodes; ea.parent = J.f->d_parent.as<uint32_t>();
         ea.species = J.f->d_species.as<uint32_t>();
         ea.snv_k0 = (uint32_t*)at(J.o_sk0); ea.snv_off = (uint32_t*)at(J.o_soff); ea.snv_n = (uint32_t*)at(J.o_sn);
         ea.ind_k0 = (uint32_t*)at(J.o_ik0); ea.ind_off = (uint32_t*)at(J.o_ioff); ea.ind_n = (uint32_t*)at(J.o_in);
-        ea.T = (uint64_t*)at(J.o_T); ea.cnt_snv = J.d_cnt_snv; ea.cnt_tot = J.d_cnt_tot;
-        hipLaunchKernelGGL(k_edge_counts, dim3(grid_of(n)), dim3(256), 0, s, ea);
-        PSX_HIP(hipGetLastError());
-        PSX_HIP(hipcub::DeviceScan::ExclusiveSum(pn + c_scan, scan_bytes, J.d_cnt_tot, J.d_off, (int)n, s));
-        PSX_HIP(hipMemcpyAsync(io + io_job * k, J.d_off + (n - 1), 8, hipMemcpyDeviceToHost, s));
-        PSX_HIP(hipMemcpyAsync(io + io_job * k + 1, J.d_cnt_tot + (n - 1), 8, hipMemcpyDeviceToHost, s));
+        ea.T = (uint64_t*)at(J.o_T); ea.cnt_snv = d_cnt_snv + nbase[k]; ea.cnt_tot = d_cnt_tot + nbase[k];
     }
+    upload_table();
+    PSX_HIP(hipMemsetAsync(d_cnt_tot + N, 0, 8, s));
+    if (N) {
+        hipLaunchKernelGGL(k_edge_counts, dim3(grid_of(N)), dim3(256), 0, s, B, N);
+        PSX_HIP(hipGetLastError());
+    }
+    PSX_HIP(hipcub::DeviceScan::ExclusiveSum(pn + c_scan, scan_bytes, d_cnt_tot, d_off, (int)(N + 1), s));
+    hipLaunchKernelGGL(k_node_bases, dim3(grid_of(std::max<uint64_t>(N, nj))), dim3(256), 0, s, B, N, d_off,
+                       d_off_job, d_pcount);
+    PSX_HIP(hipGetLastError());
+    PSX_HIP(hipMemcpyAsync(io + r_pcount, d_pcount, 8 * nj, hipMemcpyDeviceToHost, s));
     PSX_HIP(hipStreamSynchronize(s));
     {
         size_t ci = 0;
@@ -1352,76 +1467,59 @@ void place_batch(psx_ctx* ctx, const psx_genome* g, std::vector<place_job>& jobs
                     job_throw(J, nj, PSX_EINVAL, "invalid germline mutation " + std::to_string(q));
         }
     }
-    // ---- phase 2: mutation ids, placement, the first infinite-sites check
-    const uint64_t p_max = pstart[g->n_chr - 1] + g->len[g->n_chr - 1];
-    const int pbits = bits_for(p_max);
-    uint64_t max_m = 1, sum_m = 0;
+    // ---- phase 2: mutation ids, placement and the first infinite-sites check of every job, one sort
+    uint64_t n_pass_all = 0;
     for (uint32_t k = 0; k < nj; ++k) {
         place_job& J = jobs[k];
-        J.n_pass = io[io_job * k] + io[io_job * k + 1];
+        J.n_pass = io[r_pcount + k];
         J.n_fixed = J.drv_d.size() + J.p->n_germline;
         J.M = J.n_fixed + J.p->n_preneoplastic_snv + J.p->n_preneoplastic_indel + J.n_pass;
         if (J.M > ID_MASK)
             job_throw(J, nj, PSX_EINVAL, "too many mutations (" + std::to_string(J.M) + ", at most " +
                                              std::to_string(ID_MASK) + ")");
-        max_m = std::max(max_m, J.M);
-        sum_m += J.M;
+        J.mbase = mbase[k];
+        J.pbase = n_pass_all;
+        mbase[k + 1] = mbase[k] + J.M;
+        n_pass_all += J.n_pass;
     }
+    const uint64_t Mt = mbase[nj];
+    if (Mt >= NO_ENTRY) psx_throw(PSX_EINVAL, "too many mutations in one call (%llu)", (unsigned long long)Mt);
     size_t sort_bytes = 0, rank_bytes = 0;
     PSX_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (uint64_t*)nullptr, (uint64_t*)nullptr,
-                                               (uint32_t*)nullptr, (uint32_t*)nullptr, (int)max_m, 0, pbits, s));
+                                               (uint32_t*)nullptr, (uint32_t*)nullptr, (int)std::max<uint64_t>(Mt, 1),
+                                               0, (int)kbits, s));
     PSX_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, rank_bytes, (uint32_t*)nullptr, (uint32_t*)nullptr,
-                                             (int)max_m + 1, s));
+                                             (int)Mt + 1, s));
     sort_bytes = std::max(sort_bytes, rank_bytes);
-    // per-mutation temporaries from the context's placement pool: the attempt counters, moved flags and
-    // loser stamps of every job first (cleared together), then each job's own
+    // per-mutation arrays of the whole batch from the context's placement pool: attempt counters, moved flags
+    // and loser stamps first (cleared together)
+    const uint64_t tab_slots_max = tab_max(Mt);
     carver cm;
-    const size_t c_zero = cm.take(sum_m * 12), c_sort = cm.take(sort_bytes);
-    std::vector<size_t> c_mut(nj);
-    for (uint32_t k = 0; k < nj; ++k) {
-        const uint64_t M = jobs[k].M;
-        c_mut[k] = cm.bytes;
-        cm.take(M * sizeof(psx_mutation));
-        for (int q = 0; q < 8; ++q) cm.take(M * 4);  // list[2], lapos[2], ids, ids2, mlist, mapos
-        for (int q = 0; q < 2; ++q) cm.take((M + 1) * 4);  // w, S
-        cm.take(jobs[k].n_pass * 4);                // node_of
-        for (int q = 0; q < 2; ++q) cm.take(M * 8);  // key, key2
-        cm.take(tab_max(M) * 4);                    // table
-        cm.take(M);                                 // head
-    }
+    const size_t c_zero = cm.take(Mt * 12), c_sort = cm.take(sort_bytes), c_recs = cm.take(Mt * sizeof(psx_mutation)),
+                 c_list0 = cm.take(Mt * 4), c_list1 = cm.take(Mt * 4), c_lap0 = cm.take(Mt * 4),
+                 c_lap1 = cm.take(Mt * 4), c_ids = cm.take(Mt * 4), c_ids2 = cm.take(Mt * 4), c_ml = cm.take(Mt * 4),
+                 c_map = cm.take(Mt * 4), c_w = cm.take((Mt + 1) * 4), c_S = cm.take((Mt + 1) * 4),
+                 c_nof = cm.take(n_pass_all * 4), c_key = cm.take(Mt * 8), c_key2 = cm.take(Mt * 8),
+                 c_tab = cm.take(tab_slots_max * 4), c_head = cm.take(Mt);
     ctx->place_muts.reserve(cm.bytes);
     uint8_t* pm = (uint8_t*)ctx->place_muts.p;
     void* d_sort = pm + c_sort;
-    PSX_HIP(hipMemsetAsync(pm + c_zero, 0, sum_m * 12, s));
-    uint64_t zero_at = 0;
+    uint32_t* d_att = (uint32_t*)(pm + c_zero);
+    uint32_t* d_moved = d_att + Mt;
+    uint32_t* d_lose = d_att + 2 * Mt;
+    psx_mutation* d_recs = (psx_mutation*)(pm + c_recs);
+    uint32_t* d_list[2] = {(uint32_t*)(pm + c_list0), (uint32_t*)(pm + c_list1)};
+    uint32_t* d_lapos[2] = {(uint32_t*)(pm + c_lap0), (uint32_t*)(pm + c_lap1)};
+    uint32_t *d_ids = (uint32_t*)(pm + c_ids), *d_ids2 = (uint32_t*)(pm + c_ids2), *d_mlist = (uint32_t*)(pm + c_ml),
+             *d_mapos = (uint32_t*)(pm + c_map), *d_w = (uint32_t*)(pm + c_w), *d_S = (uint32_t*)(pm + c_S),
+             *d_node_of = (uint32_t*)(pm + c_nof), *d_tab = (uint32_t*)(pm + c_tab);
+    uint64_t *d_key = (uint64_t*)(pm + c_key), *d_key2 = (uint64_t*)(pm + c_key2);
+    uint8_t* d_head = pm + c_head;
+    B.recs = d_recs;
     for (uint32_t k = 0; k < nj; ++k) {
         place_job& J = jobs[k];
-        const uint64_t M = J.M;
-        J.d_att = (uint32_t*)(pm + c_zero) + zero_at;
-        J.d_moved = (uint32_t*)(pm + c_zero) + sum_m + zero_at;
-        J.d_lose = (uint32_t*)(pm + c_zero) + 2 * sum_m + zero_at;
-        zero_at += M;
-        carver c;
-        c.bytes = c_mut[k];
-        psx_mutation* recs = (psx_mutation*)(pm + c.take(M * sizeof(psx_mutation)));
-        J.d_list[0] = (uint32_t*)(pm + c.take(M * 4));
-        J.d_list[1] = (uint32_t*)(pm + c.take(M * 4));
-        J.d_lapos[0] = (uint32_t*)(pm + c.take(M * 4));
-        J.d_lapos[1] = (uint32_t*)(pm + c.take(M * 4));
-        J.d_ids = (uint32_t*)(pm + c.take(M * 4));
-        J.d_ids2 = (uint32_t*)(pm + c.take(M * 4));
-        J.d_mlist = (uint32_t*)(pm + c.take(M * 4));
-        J.d_mapos = (uint32_t*)(pm + c.take(M * 4));
-        J.d_w = (uint32_t*)(pm + c.take((M + 1) * 4));
-        J.d_S = (uint32_t*)(pm + c.take((M + 1) * 4));
-        J.d_node_of = (uint32_t*)(pm + c.take(J.n_pass * 4));
-        J.d_key = (uint64_t*)(pm + c.take(M * 8));
-        J.d_key2 = (uint64_t*)(pm + c.take(M * 8));
-        J.d_tab = (uint32_t*)(pm + c.take(tab_max(M) * 4));
-        J.d_head = pm + c.take(M);
-        J.d_io = (round_io*)at(J.o_io);
         const psx_engine_params* p = J.p;
-        place_args& pa = J.pa;
+        place_args& pa = tab[k].pa;
         pa.g = gdev{g->packed.as<uint32_t>(), (uint32_t*)at(o_len), (uint64_t*)at(o_woff), (uint64_t*)at(o_boff), G,
                     g->n_chr};
         pa.seed = J.seed; pa.ploidy = p->ploidy; pa.n_sbs = p->n_sbs;
@@ -1435,8 +1533,8 @@ void place_batch(psx_ctx* ctx, const psx_genome* g, std::vector<place_job>& jobs
         pa.drv_d = (uint32_t*)at(J.o_drvd); pa.drv_v = (uint32_t*)at(J.o_drvv);
         pa.drivers = (psx_driver*)at(J.o_drivers); pa.germ = (psx_germline*)at(J.o_germ);
         pa.pre_sig = p->preneoplastic_sbs;
-        pa.cnt_snv = J.d_cnt_snv; pa.node_off = J.d_off; pa.node_of = J.d_node_of;
-        pa.M = M; pa.recs = recs; pa.attempt = J.d_att; pa.status = &J.d_io->status;
+        pa.cnt_snv = d_cnt_snv + nbase[k]; pa.node_off = d_off_job + nbase[k]; pa.node_of = d_node_of + J.pbase;
+        pa.M = J.M; pa.recs = d_recs + J.mbase; pa.attempt = d_att + J.mbase; pa.status = d_status + k;
         if (J.sidx) {
             pa.sbs_list = J.sidx->d_list.as<uint32_t>();
             pa.sbs_kept = J.sidx->d_meta.as<uint64_t>();
@@ -1450,28 +1548,30 @@ void place_batch(psx_ctx* ctx, const psx_genome* g, std::vector<place_job>& jobs
             pa.id_count = (const uint64_t*)at(J.o_idc); pa.id_listed = (const uint32_t*)at(J.o_idl);
             pa.id_off = (const uint64_t*)at(J.o_ido); pa.id_list = J.idx->d_list.as<uint64_t>();
         }
-        if (J.n_pass) {
-            hipLaunchKernelGGL(k_node_expand, dim3(grid_of(J.f->n_nodes)), dim3(256), 0, s, J.d_off, J.d_cnt_tot,
-                               J.f->n_nodes, J.d_node_of);
-            PSX_HIP(hipGetLastError());
-        }
-        if (M) {
-            hipLaunchKernelGGL(k_place, dim3(grid_of(M)), dim3(256), 0, s, pa);
-            PSX_HIP(hipGetLastError());
-        }
-        if (p->infinite_sites && M > 1) {
-            // first round over every footprint; the sorted footprints stay as the reference of later rounds
-            hipLaunchKernelGGL(k_footprint, dim3(grid_of(M)), dim3(256), 0, s, pa.recs, M, d_pstart, J.d_key,
-                               J.d_ids);
-            PSX_HIP(hipGetLastError());
-            PSX_HIP(hipcub::DeviceRadixSort::SortPairs(d_sort, sort_bytes, J.d_key, J.d_key2, J.d_ids, J.d_ids2,
-                                                       (int)M, 0, pbits, s));
-            hipLaunchKernelGGL(k_conflicts, dim3(grid_append(M)), dim3(256), 0, s, J.d_key2, J.d_ids2, M, J.n_fixed,
-                               J.d_lose, J.d_list[0], J.d_lapos[0], &J.d_io->n_lose[0]);
-            PSX_HIP(hipGetLastError());
-        }
-        PSX_HIP(hipMemcpyAsync(io + io_job * k + 2, J.d_io, sizeof(round_io), hipMemcpyDeviceToHost, s));
+        tab[k].mbase = J.mbase;
+        tab[k].n_fixed = J.n_fixed;
+        tab[k].is = p->infinite_sites ? 1u : 0u;
+        tab[k].n_chr = g->n_chr;
     }
+    upload_table();
+    PSX_HIP(hipMemsetAsync(dt + t_state, 0, sizeof(round_io) + 4 * nj, s));
+    PSX_HIP(hipMemsetAsync(pm + c_zero, 0, Mt * 12, s));
+    if (N) {
+        hipLaunchKernelGGL(k_node_expand, dim3(grid_of(N)), dim3(256), 0, s, B, N, d_off_job, d_cnt_tot);
+        PSX_HIP(hipGetLastError());
+    }
+    if (Mt) {
+        // first attempts and sort entries; one sort of every job's footprints (job-major, the sorted entries stay
+        // as the reference of later rounds and give the position order); the first check
+        hipLaunchKernelGGL(k_place, dim3(grid_of(Mt)), dim3(256), 0, s, B, Mt, d_key, d_ids);
+        PSX_HIP(hipGetLastError());
+        PSX_HIP(hipcub::DeviceRadixSort::SortPairs(d_sort, sort_bytes, d_key, d_key2, d_ids, d_ids2, (int)Mt, 0,
+                                                   (int)kbits, s));
+        hipLaunchKernelGGL(k_conflicts, dim3(grid_append(Mt)), dim3(256), 0, s, B, d_key2, d_ids2, Mt, d_lose,
+                           d_list[0], d_lapos[0], &d_io->n_lose[0]);
+        PSX_HIP(hipGetLastError());
+    }
+    PSX_HIP(hipMemcpyAsync(io, dt + t_state, io_state, hipMemcpyDeviceToHost, s));
     // the tables' block, allocated while the GPU runs the placement
     size_t block_bytes = 0;
     std::vector<size_t> t_off(nj);
@@ -1482,128 +1582,139 @@ void place_batch(psx_ctx* ctx, const psx_genome* g, std::vector<place_job>& jobs
     auto block = std::make_shared<dbuf>();
     block->alloc(block_bytes);
     PSX_HIP(hipStreamSynchronize(s));
-    // status of the placement launches, the moved mutations and the losers of the last check run
-    auto rio = [&](const place_job& J) { return (const round_io*)(io + io_job * J.k + 2); };
-    auto take_status = [&](place_job& J) {
-        const round_io* x = rio(J);
-        if (x->status & ST_NO_SITE) job_throw(J, nj, PSX_EINVAL, "no reference site matches a drawn SBS or ID class");
-        if (x->status & ST_NO_INDEL_ROOM) job_throw(J, nj, PSX_EINVAL, "no room for an indel");
-        J.n_moved = x->n_moved;
-        if (x->halt) {  // resumed at the halted round once the table is rebuilt
-            J.round = x->halt;
-            J.halted = true;
-        } else {
-            J.round = J.spec_end;
-            J.halted = false;
-        }
-        J.n_lose = x->n_lose[J.round - 1];
-    };
-    for (place_job& J : jobs) take_status(J);
-    // ---- phase 3: later infinite-sites rounds, SPEC_ROUNDS of them launched for every job with losers
-    // between synchronisations (rounds after the last loser do nothing)
-    for (;;) {
-        bool any = false;
-        for (const place_job& J : jobs) any |= J.n_lose != 0;
-        if (!any) break;
+    // status of every job's placement launches, the moved mutations and the losers of the last check run
+    uint32_t round = 1, spec_end = 1, n_lose = 0, n_moved = 0;
+    bool halted = false;
+    auto take_status = [&] {
         for (place_job& J : jobs) {
-            if (!J.n_lose) continue;
-            if (J.round >= (uint32_t)IS_ROUNDS) job_throw(J, nj, PSX_EINVAL, "infinite-sites placement did not converge");
-            if (!J.tab_slots || J.halted) {
-                // cleared, sized for a few rounds of this many losers, refilled with the moved footprints
-                J.tab_slots = std::min(tab_max(J.M),
-                                       next_pow2(std::max<uint64_t>(1024, 8 * ((uint64_t)J.n_moved + J.n_lose))));
-                PSX_HIP(hipMemsetAsync(J.d_tab, 0xFF, J.tab_slots * 4, s));
-                hipLaunchKernelGGL(k_tab_fill, dim3(std::max<uint32_t>(1, grid_of(J.n_moved))), dim3(256), 0, s,
-                                   J.d_mlist, J.n_moved, J.pa.recs, d_pstart, J.d_tab, (uint32_t)(J.tab_slots - 1),
-                                   J.d_io);
-                PSX_HIP(hipGetLastError());
-            }
-            const uint32_t grid = std::min<uint32_t>(1024, grid_of(J.n_lose));
-            J.spec_end = std::min<uint32_t>(J.round + SPEC_ROUNDS, IS_ROUNDS);
-            for (uint32_t r = J.round; r < J.spec_end; ++r) {
-                redraw_args rd{};
-                rd.round = r; rd.list = J.d_list[(r - 1) & 1]; rd.lapos = J.d_lapos[(r - 1) & 1];
-                rd.moved = J.d_moved; rd.mlist = J.d_mlist; rd.mapos = J.d_mapos; rd.io = J.d_io;
-                rd.akey = J.d_key2; rd.aval = J.d_ids2; rd.M = J.M;
-                rd.tab = J.d_tab; rd.tab_mask = (uint32_t)(J.tab_slots - 1); rd.pstart = d_pstart;
-                hipLaunchKernelGGL(k_redraw, dim3(grid), dim3(256), 0, s, J.pa, rd);
-                PSX_HIP(hipGetLastError());
-                check_args ca{};
-                ca.akey = J.d_key2; ca.aval = J.d_ids2; ca.M = J.M;
-                ca.recs = J.pa.recs; ca.pstart = d_pstart; ca.tab = J.d_tab; ca.tab_mask = rd.tab_mask;
-                ca.round = r; ca.list = J.d_list[(r - 1) & 1]; ca.io = J.d_io;
-                ca.lose = J.d_lose; ca.out = J.d_list[r & 1]; ca.out_apos = J.d_lapos[r & 1];
-                hipLaunchKernelGGL(k_check, dim3(grid), dim3(256), 0, s, ca);
-                PSX_HIP(hipGetLastError());
-            }
-            PSX_HIP(hipMemcpyAsync((void*)rio(J), J.d_io, sizeof(round_io), hipMemcpyDeviceToHost, s));
+            const uint32_t st = h_status[J.k];
+            if (st & ST_NO_SITE) job_throw(J, nj, PSX_EINVAL, "no reference site matches a drawn SBS or ID class");
+            if (st & ST_NO_INDEL_ROOM) job_throw(J, nj, PSX_EINVAL, "no room for an indel");
         }
+        n_moved = h_io->n_moved;
+        if (h_io->halt) {  // resumed at the halted round once the table is rebuilt
+            round = h_io->halt;
+            halted = true;
+        } else {
+            round = spec_end;
+            halted = false;
+        }
+        n_lose = h_io->n_lose[round - 1];
+    };
+    take_status();
+    // ---- phase 3: later infinite-sites rounds of every job together, SPEC_ROUNDS of them launched between
+    // synchronisations (rounds after the last loser do nothing)
+    uint64_t tab_slots = 0;
+    while (n_lose) {
+        if (round >= (uint32_t)IS_ROUNDS) {
+            // the job of a remaining loser
+            uint32_t gid = 0;
+            PSX_HIP(hipMemcpy(&gid, d_list[(round - 1) & 1], 4, hipMemcpyDeviceToHost));
+            uint32_t k = 0;
+            while (k + 1 < nj && mbase[k + 1] <= gid) ++k;
+            job_throw(jobs[k], nj, PSX_EINVAL, "infinite-sites placement did not converge");
+        }
+        if (!tab_slots || halted) {
+            // cleared, sized for a few rounds of this many losers, refilled with the moved footprints
+            tab_slots = std::min(tab_slots_max, next_pow2(std::max<uint64_t>(1024, 8 * ((uint64_t)n_moved + n_lose))));
+            PSX_HIP(hipMemsetAsync(d_tab, 0xFF, tab_slots * 4, s));
+            hipLaunchKernelGGL(k_tab_fill, dim3(std::max<uint32_t>(1, grid_of(n_moved))), dim3(256), 0, s, B, d_mlist,
+                               n_moved, d_tab, (uint32_t)(tab_slots - 1), d_io);
+            PSX_HIP(hipGetLastError());
+        }
+        const uint32_t grid = std::min<uint32_t>(1024, grid_of(n_lose));
+        spec_end = std::min<uint32_t>(round + SPEC_ROUNDS, IS_ROUNDS);
+        for (uint32_t r = round; r < spec_end; ++r) {
+            redraw_args rd{};
+            rd.round = r; rd.list = d_list[(r - 1) & 1]; rd.lapos = d_lapos[(r - 1) & 1];
+            rd.moved = d_moved; rd.mlist = d_mlist; rd.mapos = d_mapos;
+            rd.akey = d_key2; rd.aval = d_ids2; rd.n = Mt; rd.io = d_io;
+            rd.tab = d_tab; rd.tab_mask = (uint32_t)(tab_slots - 1);
+            hipLaunchKernelGGL(k_redraw, dim3(grid), dim3(256), 0, s, B, rd);
+            PSX_HIP(hipGetLastError());
+            check_args ca{};
+            ca.akey = d_key2; ca.aval = d_ids2; ca.n = Mt; ca.tab = d_tab; ca.tab_mask = rd.tab_mask;
+            ca.round = r; ca.list = d_list[(r - 1) & 1]; ca.io = d_io;
+            ca.lose = d_lose; ca.out = d_list[r & 1]; ca.out_apos = d_lapos[r & 1];
+            hipLaunchKernelGGL(k_check, dim3(grid), dim3(256), 0, s, B, ca);
+            PSX_HIP(hipGetLastError());
+        }
+        PSX_HIP(hipMemcpyAsync(io, dt + t_state, io_state, hipMemcpyDeviceToHost, s));
         PSX_HIP(hipStreamSynchronize(s));
-        for (place_job& J : jobs)
-            if (J.n_lose) take_status(J);
+        take_status();
     }
-    // ---- phase 4: SID order (position order, each run of one position sorted by the rest of the SID) into the
-    // tables, which share one block. With the infinite-sites model the first check's sorted footprints are in
-    // position order but for the moved mutations: those are sorted alone and merged in; else a stable sort.
+    // ---- phase 4: SID order into the tables, which share one block: the first check's sorted entries are in
+    // position order but for the moved mutations, which are sorted alone and merged in; then each run of one
+    // position is sorted by the rest of the SID
     for (uint32_t k = 0; k < nj; ++k) {
         place_job& J = jobs[k];
         psx_mutations* mt = J.mt;
-        const uint64_t M = J.M;
-        mt->n = M;
+        mt->n = J.M;
         mt->ploidy = J.p->ploidy;
         mt->n_leaves = J.f->n_leaves;
         mt->chr_len = g->len;
         psx_mutations_carve(mt, block, t_off[k]);
-        uint64_t* key = J.d_key2;  // positions in SID order
-        uint32_t* ids = J.d_ids2;
-        if (J.p->infinite_sites && M > 1) {
-            merge_args ma{};
-            ma.akey = J.d_key2; ma.aval = J.d_ids2; ma.M = M; ma.nb = J.n_moved;
-            ma.okey = J.d_key; ma.oid = J.d_ids;
-            if (J.n_moved) {
-                // the sorted moved positions go into the table's memory, free after the rounds, their
-                // upper bounds into the first loser list's
-                uint64_t* bkey = (uint64_t*)J.d_tab;
-                uint32_t* bval = (uint32_t*)(bkey + J.n_moved);
-                hipLaunchKernelGGL(k_moved_keys, dim3(grid_of(J.n_moved)), dim3(256), 0, s, J.d_mlist, J.n_moved,
-                                   J.pa.recs, d_pstart, J.d_key, J.d_ids);
-                PSX_HIP(hipGetLastError());
-                PSX_HIP(hipcub::DeviceRadixSort::SortPairs(d_sort, sort_bytes, J.d_key, bkey, J.d_ids, bval,
-                                                           (int)J.n_moved, 0, pbits, s));
-                hipLaunchKernelGGL(k_fill_u32, dim3(grid_of(M + 1)), dim3(256), 0, s, J.d_w, M + 1, 1u, 0u);
-                PSX_HIP(hipGetLastError());
-                hipLaunchKernelGGL(k_merge_marks, dim3(grid_of(J.n_moved)), dim3(256), 0, s, J.d_mapos, J.d_key2, M,
-                                   bkey, J.n_moved, J.d_w, J.d_list[0]);
-                PSX_HIP(hipGetLastError());
-                PSX_HIP(hipcub::DeviceScan::ExclusiveSum(d_sort, sort_bytes, J.d_w, J.d_S, (int)M + 1, s));
-                ma.S = J.d_S; ma.bkey = bkey; ma.bval = bval; ma.ub = J.d_list[0];
-            }
-            hipLaunchKernelGGL(k_merge, dim3(grid_of(M + J.n_moved)), dim3(256), 0, s, ma);
-            PSX_HIP(hipGetLastError());
-            key = J.d_key;
-            ids = J.d_ids;
-        } else if (M) {
-            hipLaunchKernelGGL(k_pos_key, dim3(grid_of(M)), dim3(256), 0, s, J.pa.recs, M, d_pstart, J.d_key, J.d_ids);
-            PSX_HIP(hipGetLastError());
-            PSX_HIP(hipcub::DeviceRadixSort::SortPairs(d_sort, sort_bytes, J.d_key, J.d_key2, J.d_ids, J.d_ids2,
-                                                       (int)M, 0, pbits, s));
-        }
-        if (M) {
-            hipLaunchKernelGGL(k_sid_runs, dim3(grid_of(M)), dim3(256), 0, s, J.pa.recs, key, ids, M, J.d_head);
-            PSX_HIP(hipGetLastError());
-            hipLaunchKernelGGL(k_gather, dim3(grid_of(M)), dim3(256), 0, s, J.pa.recs, ids, J.d_head, M, mt->recs);
-            PSX_HIP(hipGetLastError());
-        }
-        hipLaunchKernelGGL(k_chr_begin, dim3(1), dim3(128), 0, s, key, M, d_pstart, g->n_chr, mt->d_chr_begin);
-        PSX_HIP(hipGetLastError());
-        PSX_HIP(hipMemcpyAsync(io + io_job * k + 2 + RIO, mt->d_chr_begin, 8 * (PSX_MAX_CHROMOSOMES + 1),
-                               hipMemcpyDeviceToHost, s));
-        psx_mutations_finish_device(ctx, mt, false);
+        tab[k].out = mt->recs;
+        tab[k].pos = mt->pos; tab[k].lo = mt->lo; tab[k].hi = mt->hi; tab[k].info = mt->info;
+        tab[k].chr_begin = mt->d_chr_begin;
     }
+    // the tables' stratum indexes for the genome's chromosome lengths, in one block
+    const std::vector<uint64_t> ix_off = psx_index_layout(g->len);
+    index_args ia{};
+    ia.n_chr = g->n_chr;
+    ia.nb = ix_off[g->n_chr];
+    for (uint32_t c = 0; c <= g->n_chr; ++c) ia.boff[c] = ix_off[c];
+    auto ix_block = std::make_shared<dbuf>();
+    ix_block->alloc(ia.nb * nj * 4u);
+    for (uint32_t k = 0; k < nj; ++k) tab[k].bucket = ix_block->as<uint32_t>() + ia.nb * k;
+    upload_table();
+    if (Mt) {
+        merge_args ma{};
+        ma.akey = d_key2; ma.aval = d_ids2; ma.n = Mt; ma.nb = n_moved; ma.okey = d_key; ma.ogid = d_ids;
+        if (n_moved) {
+            // the sorted moved keys go into the table's memory, free after the rounds, their upper bounds into
+            // the first loser list's
+            uint64_t* bkey = (uint64_t*)d_tab;
+            uint32_t* bval = (uint32_t*)(bkey + n_moved);
+            hipLaunchKernelGGL(k_moved_keys, dim3(grid_of(n_moved)), dim3(256), 0, s, B, d_mlist, n_moved, d_key,
+                               d_ids);
+            PSX_HIP(hipGetLastError());
+            PSX_HIP(hipcub::DeviceRadixSort::SortPairs(d_sort, sort_bytes, d_key, bkey, d_ids, bval, (int)n_moved, 0,
+                                                       (int)kbits, s));
+            hipLaunchKernelGGL(k_fill_u32, dim3(grid_of(Mt + 1)), dim3(256), 0, s, d_w, Mt + 1, 1u, 0u);
+            PSX_HIP(hipGetLastError());
+            hipLaunchKernelGGL(k_merge_marks, dim3(grid_of(n_moved)), dim3(256), 0, s, d_mapos, d_key2, Mt, bkey,
+                               n_moved, d_w, d_list[0]);
+            PSX_HIP(hipGetLastError());
+            PSX_HIP(hipcub::DeviceScan::ExclusiveSum(d_sort, sort_bytes, d_w, d_S, (int)Mt + 1, s));
+            ma.S = d_S; ma.bkey = bkey; ma.bval = bval; ma.ub = d_list[0];
+        }
+        hipLaunchKernelGGL(k_merge, dim3(grid_of(Mt + n_moved)), dim3(256), 0, s, B, ma);
+        PSX_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_sid_runs, dim3(grid_of(Mt)), dim3(256), 0, s, d_recs, d_key, d_ids, Mt, d_head);
+        PSX_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_gather, dim3(grid_of(Mt)), dim3(256), 0, s, B, d_key, d_ids, d_head, Mt);
+        PSX_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_chr_begin, dim3(grid_of((uint64_t)nj * W)), dim3(256), 0, s, B, d_key, d_cb);
+    PSX_HIP(hipGetLastError());
+    PSX_HIP(hipMemcpyAsync(io + r_chr, d_cb, (size_t)nj * W * 8, hipMemcpyDeviceToHost, s));
+    hipLaunchKernelGGL(k_index, dim3(grid_of(ia.nb * nj)), dim3(256), 0, s, B, ia);
+    PSX_HIP(hipGetLastError());
     PSX_HIP(hipEventRecord(ctx->ev1, s));
     PSX_HIP(hipStreamSynchronize(s));
-    for (uint32_t k = 0; k < nj; ++k)
-        jobs[k].mt->chr_begin.assign(io + io_job * k + 2 + RIO, io + io_job * k + 2 + RIO + (PSX_MAX_CHROMOSOMES + 1));
+    for (uint32_t k = 0; k < nj; ++k) {
+        psx_mutations* mt = jobs[k].mt;
+        const uint64_t* cb = io + r_chr + (size_t)k * W;
+        mt->chr_begin.assign(cb, cb + W);
+        auto ix = std::make_unique<psx_mut_index>();
+        ix->chr_len = g->len;
+        ix->bucket_off = ix_off;
+        ix->store = ix_block;
+        ix->bucket = tab[k].bucket;
+        std::lock_guard<std::mutex> lk(mt->idx_mu);
+        mt->index.push_back(std::move(ix));
+    }
     float ms = 0.f;
     PSX_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     ctx->last_place_ms = ms;

@@ -164,7 +164,7 @@ void prepare_job(psx_ctx* ctx, seq_batch& P, const job_in& in, uint32_t spi) {
     job_dev J{};
     J.seed = in.seed;
     J.pos = m->pos; J.lo = m->lo; J.hi = m->hi;
-    J.info = m->info; J.bucket = ix->d_bucket.as<uint32_t>();
+    J.info = m->info; J.bucket = ix->bucket;
     J.sample_leaves = f->d_sample_leaves.as<uint32_t>();
     J.leaf_count = f->d_leaf_count.as<uint4>();
     J.leaf_blocks = f->leaf_blocks;
