@@ -180,6 +180,35 @@ def cpu_reference_shaped(w, packed, recs, threads: int, target_reads: float):
                        f"chromosomes {sorted(chrs)} of {len(w.lengths)}, {n} reads in {dt:.2f} s")
 
 
+def batched_place_leg(ctx, B, reps: int = 3) -> dict:
+    """Every job's place_mutations of the batched workload: one batched call
+    (the first, in the workload's construction, and again with the context's
+    buffers grown) against one call per job."""
+    from process_amd import core
+
+    eng = B.engine
+    t0 = time.perf_counter()
+    for s, f in zip(B.specs, B.forests):
+        core.Mutations.place(ctx, B.genome, f, s["place_seed"], **eng).close()
+    place_one_ms = (time.perf_counter() - t0) * 1e3
+    seeds = [s["place_seed"] for s in B.specs]
+    warm_wall, warm_ev = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        for m in core.Mutations.place_batch(ctx, B.genome, B.forests, seeds, [eng] * len(B.specs)):
+            m.close()
+        warm_wall.append((time.perf_counter() - t0) * 1e3)
+        warm_ev.append(ctx.last_place_ms())
+    warm_wall_ms, warm_ev_ms = float(np.median(warm_wall)), float(np.median(warm_ev))
+    log(f"batched: placement one call per job {place_one_ms:.1f} ms, batched again {warm_wall_ms:.1f} ms "
+        f"(kernels and copies {warm_ev_ms:.1f} ms)")
+    return dict(batched_wall_ms=B.place_wall_ms, batched_event_ms=B.place_ms, warm_wall_ms=warm_wall_ms,
+                warm_event_ms=warm_ev_ms, one_call_per_job_wall_ms=place_one_ms,
+                mutations=int(sum(m.n for m in B.muts)),
+                note="every job's place_mutations in one psx_place_mutations_batch call; the first call "
+                     "grows the context's buffers")
+
+
 def batched_leg(ctx, n_jobs: int, reps: int, tune: bool) -> dict:
     """The batched workload: every job in one persistent hipGraph launch
     against the same jobs launched one by one (each call waits for its
@@ -193,13 +222,7 @@ def batched_leg(ctx, n_jobs: int, reps: int, tune: bool) -> dict:
     log(f"batched: {n_jobs} jobs built in {time.perf_counter() - t:.1f} s, {B.reads:.3e} reads, "
         f"leaves {min(leaves)}..{max(leaves)}, batched placement {B.place_wall_ms:.1f} ms "
         f"(kernels and copies {B.place_ms:.1f} ms)")
-    # the same placements one call per job
-    eng = B.engine
-    t0 = time.perf_counter()
-    for s, f in zip(B.specs, B.forests):
-        core.Mutations.place(ctx, B.genome, f, s["place_seed"], **eng).close()
-    place_one_ms = (time.perf_counter() - t0) * 1e3
-    log(f"batched: placement one call per job {place_one_ms:.1f} ms")
+    place = batched_place_leg(ctx, B)
 
     def timed_batch(cfg=None):
         core.simulate_seq_batch(ctx, B.jobs, cfg)  # warm-up
@@ -247,10 +270,7 @@ def batched_leg(ctx, n_jobs: int, reps: int, tune: bool) -> dict:
                                job_p50_ms=float(np.percentile(lat, 50) * 1e3),
                                job_p99_ms=float(np.percentile(lat, 99) * 1e3)),
                batched_over_sequential=seq_wall / wall, results_identical=bool(same),
-               placement=dict(batched_wall_ms=B.place_wall_ms, batched_event_ms=B.place_ms,
-                              one_call_per_job_wall_ms=place_one_ms,
-                              mutations=int(sum(m.n for m in B.muts)),
-                              note="every job's place_mutations in one psx_place_mutations_batch call"))
+               placement=place)
     log(f"batched: {n / wall:.3e} reads/s ({wall * 1e3:.1f} ms), replayed plan {n / plan_wall:.3e} reads/s "
         f"({plan_wall * 1e3:.1f} ms), one by one {n / seq_wall:.3e} reads/s ({seq_wall * 1e3:.1f} ms), "
         f"identical {same}")
@@ -478,6 +498,8 @@ def main() -> int:
     ap.add_argument("--serve-forms", type=str, default="continuous,one_at_a_time",
                     help="forms run on the paced stream (comma-separated: continuous, one_at_a_time)")
     ap.add_argument("--cna-only", action="store_true", help="only run the copy-number workload (for profiling)")
+    ap.add_argument("--batched-place-only", action="store_true",
+                    help="only run the batched workload's placements (--steps batched calls; for profiling)")
     ap.add_argument("--batched-reads-only", action="store_true",
                     help="only run the batched read materialisation (one launch per repetition; for profiling)")
     args = ap.parse_args()
@@ -510,6 +532,10 @@ def main() -> int:
         return 0
     if args.probe_only:
         print(json.dumps(valu_ceiling(ctx)), flush=True)
+        return 0
+    if args.batched_place_only:
+        B = workloads.build_batched(ctx, n_jobs=args.batched_jobs)
+        print(json.dumps(batched_place_leg(ctx, B, max(1, args.steps))), flush=True)
         return 0
     if args.batched_reads_only:
         B = workloads.build_batched(ctx, n_jobs=args.batched_jobs)

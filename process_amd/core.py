@@ -235,7 +235,8 @@ class Mutations(_Handle):
         n = len(forests)
         if not (n == len(engines) == len(seeds)) or n == 0:
             raise ValueError("one seed and one engine per forest, at least one forest")
-        built = [A.engine_params(**e) for e in engines]
+        by_id: dict[int, tuple] = {}  # one parameter block per distinct engine object
+        built = [by_id[id(e)] if id(e) in by_id else by_id.setdefault(id(e), A.engine_params(**e)) for e in engines]
         fs = (C.c_void_p * n)(*[f.h for f in forests])
         ps = (C.POINTER(A.EngineParams) * n)(*[C.pointer(b[0]) for b in built])
         sd = np.ascontiguousarray(seeds, np.uint64)

@@ -424,6 +424,49 @@ __device__ __forceinline__ uint32_t job_of(const uint64_t* __restrict__ base, ui
     return lo;
 }
 
+// A job's descriptor read through the constant address space: with j
+// wave-uniform its fields come by scalar loads into SGPRs.
+__device__ __forceinline__ const fjob& cjob(const batch& B, uint32_t j) {
+    return *(const fjob*)((const __attribute__((address_space(4))) fjob*)B.jobs + j);
+}
+
+__device__ __forceinline__ uint64_t first_lane(uint64_t x) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)x);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(x >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// job_of for lanes holding increasing g (consecutive ids of a wave): the
+// first active lane's job by a scalar search, lanes past that job's end (a
+// wave at a job boundary) by their own.
+__device__ __forceinline__ uint32_t job_of_seq(const uint64_t* __restrict__ base, uint32_t nj, uint64_t g) {
+    const __attribute__((address_space(4))) uint64_t* cb = (const __attribute__((address_space(4))) uint64_t*)base;
+    const uint64_t g0 = first_lane(g);
+    uint32_t lo = 0, hi = nj;
+    while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (cb[mid] <= g0) lo = mid; else hi = mid;
+    }
+    return g < cb[lo + 1] ? lo : job_of(base, nj, g);
+}
+
+// f(j) with j wave-uniform: one pass per distinct job among the wave's
+// active lanes (one unless the wave straddles a job boundary), each lane in
+// the pass of its own job. The loop runs on a wave-uniform mask, so the
+// passes stay in the loop with the job index in an SGPR.
+template <class F>
+__device__ __forceinline__ void per_job(uint32_t j, F&& f) {
+    uint64_t left = __ballot(1);
+    while (left) {
+        const uint32_t ju = (uint32_t)__builtin_amdgcn_readlane((int)j, (int)__builtin_ctzll(left));
+        uint32_t jv = j;
+        asm volatile("" : "+v"(jv));  // an opaque copy made after ju: where the two are equal the
+        const bool mine = ju == jv;   // compiler substitutes ju for it, not the divergent j for ju
+        if (mine) f(ju);
+        left &= ~__ballot(mine);
+    }
+}
+
 // Sort key of a record of job j: the job above the padded position (positions
 // of different chromosomes at least 16 apart, so footprints of at most 11
 // bases on different chromosomes never touch).
@@ -435,20 +478,21 @@ __device__ __forceinline__ uint64_t jkey(const batch& B, uint32_t j, const psx_m
 __global__ void __launch_bounds__(256) k_edge_counts(const batch B, uint64_t n) {
     const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n) return;
-    const uint32_t j = job_of(B.nbase, B.nj, g);
-    const edge_args& a = B.jobs[j].ea;
-    const uint32_t v = (uint32_t)(g - B.nbase[j]);
-    if (a.parent[v] == PSX_NO_NODE) {
-        a.cnt_snv[v] = 0;
-        a.cnt_tot[v] = 0;
-        return;
-    }
-    const psx_u4 w = psx_draw(a.seed, v, 0u, 0u, PSX_P_EDGE << 24);
-    const uint32_t s = a.species[v];
-    const uint32_t ns = a.snv_k0[s] + psx_cdf_pick(a.T + a.snv_off[s], a.snv_n[s], w.x);
-    const uint32_t ni = a.ind_k0[s] + psx_cdf_pick(a.T + a.ind_off[s], a.ind_n[s], w.y);
-    a.cnt_snv[v] = ns;
-    a.cnt_tot[v] = (uint64_t)ns + ni;
+    per_job(job_of_seq(B.nbase, B.nj, g), [&](uint32_t j) {
+        const edge_args& a = cjob(B, j).ea;
+        const uint32_t v = (uint32_t)(g - B.nbase[j]);
+        if (a.parent[v] == PSX_NO_NODE) {
+            a.cnt_snv[v] = 0;
+            a.cnt_tot[v] = 0;
+            return;
+        }
+        const psx_u4 w = psx_draw(a.seed, v, 0u, 0u, PSX_P_EDGE << 24);
+        const uint32_t s = a.species[v];
+        const uint32_t ns = a.snv_k0[s] + psx_cdf_pick(a.T + a.snv_off[s], a.snv_n[s], w.x);
+        const uint32_t ni = a.ind_k0[s] + psx_cdf_pick(a.T + a.ind_off[s], a.ind_n[s], w.y);
+        a.cnt_snv[v] = ns;
+        a.cnt_tot[v] = (uint64_t)ns + ni;
+    });
 }
 
 // After an exclusive scan of every node's count (off, n + 1 entries): each
@@ -456,20 +500,38 @@ __global__ void __launch_bounds__(256) k_edge_counts(const batch B, uint64_t n) 
 __global__ void __launch_bounds__(256) k_node_bases(const batch B, uint64_t n, const uint64_t* __restrict__ off,
                                                     uint64_t* __restrict__ off_job, uint64_t* __restrict__ pcount) {
     const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g < n) off_job[g] = off[g] - off[B.nbase[job_of(B.nbase, B.nj, g)]];
+    if (g < n) off_job[g] = off[g] - off[B.nbase[job_of_seq(B.nbase, B.nj, g)]];
     if (g < B.nj) pcount[g] = off[B.nbase[g + 1]] - off[B.nbase[g]];
 }
 
-// node_of[i] = v for the passengers of node v of each job (ids off_job[v] .. + count)
-__global__ void __launch_bounds__(256) k_node_expand(const batch B, uint64_t n, const uint64_t* __restrict__ off_job,
-                                                     const uint64_t* __restrict__ cnt) {
-    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= n) return;
-    const uint32_t j = job_of(B.nbase, B.nj, g);
-    uint32_t* node_of = const_cast<uint32_t*>(B.jobs[j].pa.node_of);
-    const uint32_t v = (uint32_t)(g - B.nbase[j]);
-    const uint64_t o = off_job[g], c = cnt[g];
-    for (uint64_t k = 0; k < c; ++k) node_of[o + k] = v;
+// node_of[off[g] + k] = v (k < cnt[g]) for node v of job j (g = nbase[j] + v),
+// node_of being every job's passenger array (each at its passenger base
+// off[nbase[j]]). A wave takes 64 nodes and writes their passengers'
+// entries together, 64 consecutive entries a pass: each lane finds its
+// entry's node among the 64 first offsets in LDS, so nodes with many
+// passengers need no long loop of one lane.
+__global__ void __launch_bounds__(256) k_node_expand(const batch B, uint64_t n, const uint64_t* __restrict__ off,
+                                                     uint32_t* __restrict__ node_of) {
+    __shared__ uint64_t s_off[4][64];
+    __shared__ uint32_t s_v[4][64];
+    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+    const uint64_t g0 = ((uint64_t)blockIdx.x * blockDim.x) + wv * 64u;
+    if (g0 >= n) return;
+    const uint64_t g = g0 + lane;
+    const uint64_t gl = g < n ? g : n - 1u;
+    s_off[wv][lane] = g < n ? off[g] : off[n];
+    s_v[wv][lane] = (uint32_t)(gl - B.nbase[job_of_seq(B.nbase, B.nj, gl)]);
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    const uint64_t start = s_off[wv][0];
+    const uint64_t end = g0 + 64u < n ? off[g0 + 64u] : off[n];
+    for (uint64_t i = start + lane; i < end; i += 64u) {
+        uint32_t lo = 0;  // last node of the 64 whose first entry is at or before i
+#pragma unroll
+        for (uint32_t step = 32; step; step >>= 1)
+            if (s_off[wv][lo + step] <= i) lo += step;
+        node_of[i] = s_v[wv][lo];
+    }
 }
 
 // Sort entries of the infinite-sites check: key = jkey, value = id in the job
@@ -493,15 +555,16 @@ __global__ void __launch_bounds__(256) k_place(const batch B, uint64_t n, uint64
                                                uint32_t* __restrict__ fval) {
     const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n) return;
-    const uint32_t j = job_of(B.mbase, B.nj, g);
-    const place_args& a = B.jobs[j].pa;
-    const uint64_t id = g - B.mbase[j];
-    psx_mutation r;
-    const uint32_t st = place_one(a, id, 0u, r);
-    if (st != ST_OK) atomicOr(a.status, st);
-    a.recs[id] = r;
-    fkey[g] = jkey(B, j, r);
-    fval[g] = fp_val((uint32_t)id, r);
+    per_job(job_of_seq(B.mbase, B.nj, g), [&](uint32_t j) {
+        const place_args& a = cjob(B, j).pa;
+        const uint64_t id = g - B.mbase[j];
+        psx_mutation r;
+        const uint32_t st = place_one(a, id, 0u, r);
+        if (st != ST_OK) atomicOr(a.status, st);
+        a.recs[id] = r;
+        fkey[g] = jkey(B, j, r);
+        fval[g] = fp_val((uint32_t)id, r);
+    });
 }
 
 // A loser of check number `stamp` (1 for the first): lose[gid] holds the
@@ -897,19 +960,21 @@ __global__ void __launch_bounds__(256) k_gather(const batch B, const uint64_t* _
                                                 uint64_t n) {
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
-    const uint32_t j = job_of(B.mbase, B.nj, t);
-    const uint64_t lo = B.mbase[j], hi = B.mbase[j + 1], i = t - lo;
-    const fjob& J = B.jobs[j];
     psx_mutation r = B.recs[gids[t]];
     r.sid_head = head[t];
-    J.out[i] = r;
-    J.pos[i] = r.pos;
-    J.lo[i] = r.leaf_lo;
-    J.hi[i] = r.leaf_hi;
-    const bool solo = (t == lo || key[t - 1] != key[t]) && (t + 1 == hi || key[t + 1] != key[t]);
-    J.info[i] = uint32_t(r.allele) | (uint32_t(r.ref & 3u) << 8) | (uint32_t(r.alt & 3u) << 10) |
-                (uint32_t(r.kind & 3u) << 12) | (uint32_t(r.nature & 3u) << 14) | (uint32_t(r.sid_head & 1u) << 16) |
-                (uint32_t(solo) << 17);
+    const uint64_t kt = key[t];
+    per_job(job_of_seq(B.mbase, B.nj, t), [&](uint32_t j) {
+        const uint64_t lo = B.mbase[j], hi = B.mbase[j + 1], i = t - lo;
+        const fjob& J = cjob(B, j);
+        J.out[i] = r;
+        J.pos[i] = r.pos;
+        J.lo[i] = r.leaf_lo;
+        J.hi[i] = r.leaf_hi;
+        const bool solo = (t == lo || key[t - 1] != kt) && (t + 1 == hi || key[t + 1] != kt);
+        J.info[i] = uint32_t(r.allele) | (uint32_t(r.ref & 3u) << 8) | (uint32_t(r.alt & 3u) << 10) |
+                    (uint32_t(r.kind & 3u) << 12) | (uint32_t(r.nature & 3u) << 14) |
+                    (uint32_t(r.sid_head & 1u) << 16) | (uint32_t(solo) << 17);
+    });
 }
 
 struct index_args {
@@ -1557,7 +1622,7 @@ void place_batch(psx_ctx* ctx, const psx_genome* g, std::vector<place_job>& jobs
     PSX_HIP(hipMemsetAsync(dt + t_state, 0, sizeof(round_io) + 4 * nj, s));
     PSX_HIP(hipMemsetAsync(pm + c_zero, 0, Mt * 12, s));
     if (N) {
-        hipLaunchKernelGGL(k_node_expand, dim3(grid_of(N)), dim3(256), 0, s, B, N, d_off_job, d_cnt_tot);
+        hipLaunchKernelGGL(k_node_expand, dim3(grid_of(N)), dim3(256), 0, s, B, N, d_off, d_node_of);
         PSX_HIP(hipGetLastError());
     }
     if (Mt) {
