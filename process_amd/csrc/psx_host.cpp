@@ -4,7 +4,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <exception>
 #include <numeric>
+#include <thread>
 
 #include "psx_internal.h"
 
@@ -59,6 +61,7 @@ extern "C" void psx_ctx_destroy(psx_ctx* ctx) {
         ctx->place_nodes.release();
         ctx->place_muts.release();
         if (ctx->host_io) (void)hipHostFree(ctx->host_io);
+        if (ctx->host_stage) (void)hipHostFree(ctx->host_stage);
         (void)hipEventDestroy(ctx->ev0);
         (void)hipEventDestroy(ctx->ev1);
         (void)hipStreamDestroy(ctx->stream);
@@ -136,6 +139,11 @@ extern "C" psx_status psx_dev_download(psx_ctx* ctx, void* dst, const void* src,
     if (!ctx || (bytes && (!dst || !src))) return PSX_EINVAL;
     return psx_guard(ctx, [&] {
         PSX_HIP(hipSetDevice(ctx->device));
+        if (bytes >= (uint64_t(8) << 20)) {  // large: through pinned chunks on several threads
+            PSX_HIP(hipStreamSynchronize(ctx->stream));
+            psx_copy_out(ctx, {psx_copy_range{src, dst, (size_t)bytes}});
+            return;
+        }
         if (bytes) PSX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
         PSX_HIP(hipStreamSynchronize(ctx->stream));
     });
@@ -178,6 +186,59 @@ uint8_t* psx_host_io(psx_ctx* ctx, size_t bytes) {
     PSX_HIP(hipHostMalloc((void**)&ctx->host_io, n, hipHostMallocDefault));
     ctx->host_io_bytes = n;
     return ctx->host_io;
+}
+
+void psx_copy_out(psx_ctx* ctx, const std::vector<psx_copy_range>& ranges) {
+    constexpr size_t CHUNK = size_t(2) << 20;
+    struct piece {
+        const uint8_t* src;
+        uint8_t* dst;
+        size_t bytes;
+    };
+    std::vector<piece> pieces;
+    for (const psx_copy_range& r : ranges)
+        for (size_t o = 0; o < r.bytes; o += CHUNK)
+            pieces.push_back({(const uint8_t*)r.src + o, (uint8_t*)r.dst + o, std::min(CHUNK, r.bytes - o)});
+    if (pieces.empty()) return;
+    const unsigned T = (unsigned)std::min<size_t>(8, pieces.size());
+    if (ctx->host_stage_bytes < T * CHUNK) {
+        PSX_HIP(hipStreamSynchronize(ctx->stream));
+        if (ctx->host_stage) PSX_HIP(hipHostFree(ctx->host_stage));
+        ctx->host_stage = nullptr;
+        ctx->host_stage_bytes = 0;
+        PSX_HIP(hipHostMalloc((void**)&ctx->host_stage, 8 * CHUNK, hipHostMallocDefault));
+        ctx->host_stage_bytes = 8 * CHUNK;
+    }
+    std::vector<hipEvent_t> ev(T, nullptr);
+    std::vector<std::exception_ptr> err(T);
+    auto work = [&](unsigned t) {
+        try {
+            PSX_HIP(hipSetDevice(ctx->device));
+            uint8_t* buf = ctx->host_stage + t * CHUNK;
+            for (size_t k = t; k < pieces.size(); k += T) {
+                PSX_HIP(hipMemcpyAsync(buf, pieces[k].src, pieces[k].bytes, hipMemcpyDeviceToHost, ctx->stream));
+                PSX_HIP(hipEventRecord(ev[t], ctx->stream));
+                PSX_HIP(hipEventSynchronize(ev[t]));
+                std::memcpy(pieces[k].dst, buf, pieces[k].bytes);
+            }
+        } catch (...) {
+            err[t] = std::current_exception();
+        }
+    };
+    try {
+        for (unsigned t = 0; t < T; ++t) PSX_HIP(hipEventCreateWithFlags(&ev[t], hipEventDisableTiming));
+        std::vector<std::thread> th;
+        for (unsigned t = 1; t < T; ++t) th.emplace_back(work, t);
+        work(0);
+        for (std::thread& x : th) x.join();
+    } catch (...) {
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+        throw;
+    }
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    for (const std::exception_ptr& e : err)
+        if (e) std::rethrow_exception(e);
 }
 
 void* psx_scratch_upload(psx_ctx* ctx, const scratch_builder& b) {

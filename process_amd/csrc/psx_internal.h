@@ -84,9 +84,12 @@ struct psx_ctx {
     dbuf place_nodes;   // placement temporaries per node and per mutation (grown, kept for later calls)
     dbuf place_muts;
     dbuf place_tab;     // the job table and round state of the last placement call
+    dbuf seq_out;       // psx_simulate_seq's device tables before their copy to the caller (grown, kept)
     uint8_t* noalloc_scratch = nullptr;  // device address of the device's no-allocation scratch
     uint8_t* host_io = nullptr;          // pinned host memory for small device-to-host readbacks
     size_t host_io_bytes = 0;
+    uint8_t* host_stage = nullptr;       // pinned chunks of large copies to pageable memory
+    size_t host_stage_bytes = 0;
     bool tuned = false;
     psx_launch_config tuned_cfg{};
     std::atomic<bool> cancel{false};  // psx_ctx_cancel: the running call stops at its next work-chunk boundary
@@ -99,6 +102,19 @@ inline void psx_check_cancel(psx_ctx* ctx) {
 
 // ctx's pinned readback area, at least bytes long (grown after a stream synchronisation, never shrunk).
 uint8_t* psx_host_io(psx_ctx* ctx, size_t bytes);
+
+// Device-to-host copies into pageable memory through pinned chunks: the
+// ranges are cut into 2 MiB pieces dealt to up to 8 host threads, each moving
+// its pieces one by one (a DMA on ctx's stream into its pinned chunk, then a
+// memcpy out), so one thread's DMA overlaps the others' memcpys. Returns when
+// every byte is in place; the stream must hold nothing the copies should not
+// wait for.
+struct psx_copy_range {
+    const void* src;  // device
+    void* dst;        // host
+    size_t bytes;
+};
+void psx_copy_out(psx_ctx* ctx, const std::vector<psx_copy_range>& ranges);
 
 // Device address of the no-allocation entry point's scratch array on
 // `device`, which must be the current device; looked up once per device.

@@ -7,6 +7,7 @@
 #include <numeric>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -407,7 +408,8 @@ void check_config(const psx_launch_config& c) {
                   c.block_threads, c.lds_records, c.strata_per_item, c.blocks_per_cu);
 }
 
-run_result run_prepared(psx_ctx* ctx, seq_batch* batch, const seq_lane& lane, const psx_launch_config& cfg) {
+run_result run_prepared(psx_ctx* ctx, seq_batch* batch, const seq_lane& lane, const psx_launch_config& cfg,
+                        const std::function<void()>& during) {
     std::unique_ptr<seq_batch, void (*)(seq_batch*)> P(batch, batch_free);
     batch_upload(P.get(), lane);
     if (cfg.use_graph) {
@@ -420,11 +422,13 @@ run_result run_prepared(psx_ctx* ctx, seq_batch* batch, const seq_lane& lane, co
         PSX_HIP(hipEventRecord(lane.ev0, lane.stream));
         PSX_HIP(hipGraphLaunch(exec, lane.stream));
         PSX_HIP(hipEventRecord(lane.ev1, lane.stream));
+        if (during) during();
         PSX_HIP(hipStreamSynchronize(lane.stream));
         (void)hipGraphExecDestroy(exec);
         (void)hipGraphDestroy(graph);
     } else {
         batch_launch(P.get(), lane, cfg, true);
+        if (during) during();
         PSX_HIP(hipStreamSynchronize(lane.stream));
     }
     float ms = 0.f;
@@ -456,7 +460,8 @@ seq_lane ctx_lane(psx_ctx* ctx, size_t desc_bytes) {
 }
 
 // The jobs in one launch with descriptors in the context's scratch buffer.
-run_result run_count(psx_ctx* ctx, const std::vector<job_in>& jobs, const psx_launch_config& cfg) {
+run_result run_count(psx_ctx* ctx, const std::vector<job_in>& jobs, const psx_launch_config& cfg,
+                     const std::function<void()>& during = {}) {
     check_config(cfg);
     PSX_HIP(hipSetDevice(ctx->device));
     seq_batch* P = batch_prepare(ctx, jobs.data(), jobs.size(), cfg.strata_per_item);
@@ -467,7 +472,16 @@ run_result run_count(psx_ctx* ctx, const std::vector<job_in>& jobs, const psx_la
         batch_free(P);
         throw;
     }
-    return run_prepared(ctx, P, lane, cfg);
+    return run_prepared(ctx, P, lane, cfg, during);
+}
+
+// Writes one byte per 4 KiB page of a host buffer, so its pages are mapped
+// before a device copy fills it (fresh pageable memory otherwise faults page
+// by page inside the copy).
+void touch_pages(void* p, size_t bytes) {
+    volatile uint8_t* b = (volatile uint8_t*)p;
+    for (size_t i = 0; i < bytes; i += 4096) b[i] = 0;
+    if (bytes) b[bytes - 1] = 0;
 }
 
 std::vector<job_in> jobs_of(const psx_seq_job* jobs, uint32_t n_jobs) {
@@ -561,20 +575,25 @@ extern "C" psx_status psx_simulate_seq(psx_ctx* ctx, const psx_genome* g, const 
     return psx_guard(ctx, [&] {
         PSX_HIP(hipSetDevice(ctx->device));
         const uint64_t n = (uint64_t)out_samples(f, p) * m->n;
-        dbuf d_alt, d_depth, d_present;
-        d_alt.alloc(n * 4);
-        d_depth.alloc(n * 4);
-        d_present.alloc(n);
+        // the device tables in the context's output buffer (grown, kept for later calls): alt | depth | present
+        const size_t o_depth = (n * 4 + 255) / 256 * 256, o_present = o_depth + (n * 4 + 255) / 256 * 256;
+        ctx->seq_out.reserve(o_present + n);
+        uint8_t* d = (uint8_t*)ctx->seq_out.p;
         psx_launch_config c = ctx->tuned ? ctx->tuned_cfg : default_config();
         c.use_graph = 0;
-        run_result r = run_count(ctx, {job_in{g, f, m, p, seed, d_alt.as<int32_t>(), d_depth.as<int32_t>(),
-                                              d_present.as<uint8_t>()}}, c);
-        if (n) {
-            if (alt) PSX_HIP(hipMemcpyAsync(alt, d_alt.p, n * 4, hipMemcpyDeviceToHost, ctx->stream));
-            if (depth) PSX_HIP(hipMemcpyAsync(depth, d_depth.p, n * 4, hipMemcpyDeviceToHost, ctx->stream));
-            if (present) PSX_HIP(hipMemcpyAsync(present, d_present.p, n, hipMemcpyDeviceToHost, ctx->stream));
-        }
-        PSX_HIP(hipStreamSynchronize(ctx->stream));
+        // while the kernels run, the host maps the output pages the copies will fill
+        auto prefault = [&] {
+            if (alt) touch_pages(alt, n * 4);
+            if (depth) touch_pages(depth, n * 4);
+            if (present) touch_pages(present, n);
+        };
+        run_result r = run_count(ctx, {job_in{g, f, m, p, seed, (int32_t*)d, (int32_t*)(d + o_depth),
+                                              d + o_present}}, c, prefault);
+        std::vector<psx_copy_range> out;
+        if (alt) out.push_back({d, alt, n * 4});
+        if (depth) out.push_back({d + o_depth, depth, n * 4});
+        if (present) out.push_back({d + o_present, present, n});
+        psx_copy_out(ctx, out);
         if (n_reads) *n_reads = r.n_reads;
     });
 }

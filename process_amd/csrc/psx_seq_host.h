@@ -2,6 +2,8 @@
 // entry points (psx_seq.hip) and the serving runtime (psx_serve.cpp).
 // Not part of the C ABI.
 #pragma once
+#include <functional>
+
 #include "psx_internal.h"
 
 namespace psxh {
@@ -65,8 +67,10 @@ struct run_result {
 // runs the init and count kernels (captured in a hipGraph and replayed once
 // with cfg.use_graph), waits, and returns the reads simulated and the
 // HIP-event time of the count kernel (of the graph launch when captured),
-// also kept in ctx->last_kernel_ms.
-run_result run_prepared(psx_ctx* ctx, seq_batch* batch, const seq_lane& lane, const psx_launch_config& cfg);
+// also kept in ctx->last_kernel_ms. `during`, when given, runs on the host
+// while the kernels run (after the launch, before the wait).
+run_result run_prepared(psx_ctx* ctx, seq_batch* batch, const seq_lane& lane, const psx_launch_config& cfg,
+                        const std::function<void()>& during = {});
 
 uint32_t out_samples(const psx_forest* f, const psx_seq_params* p);
 psx_launch_config default_config();
