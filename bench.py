@@ -606,7 +606,8 @@ def main() -> int:
         host_ms.append((time.perf_counter() - t2) * 1e3)
     host_out = dict(value=reads_per_step / (np.median(host_ms) / 1e3), unit="reads/s",
                     ms_per_step=float(np.median(host_ms)), table_bytes=9 * M * S,
-                    note="psx_simulate_seq: device outputs allocated per call and copied to host arrays")
+                    note="psx_simulate_seq into fresh host arrays: the step plus the copy of alt, depth and presence "
+                         "(pinned chunks on several host threads)")
     log(f"host outputs: {host_out['ms_per_step']:.2f} ms/step, {host_out['value']:.3e} reads/s")
 
     # algorithmic bytes of one count-only launch: mutation records read once
