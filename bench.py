@@ -304,6 +304,48 @@ def batched_leg(ctx, n_jobs: int, reps: int, tune: bool) -> dict:
     return out
 
 
+def sam_leg(ctx, w, built, coverage: float = 20.0) -> dict:
+    """psx_write_sam (the write_SAM = TRUE path: reads made on the GPU one
+    chromosome at a time, formatted on host threads from pinned chunks) on a
+    bounded sample of the workload: its shortest chromosome at a reduced
+    coverage, every output sample, written to a temporary directory that is
+    removed afterwards."""
+    import ctypes as C
+    import shutil
+    import tempfile
+
+    from process_amd import _abi as A
+    from process_amd import core
+    from process_amd._lib import check, lib
+
+    c = int(np.argmin(w.lengths))
+    seq = dict(w.seq, coverage=coverage, chromosomes=[c])
+    S = core.num_out_samples(built.forest, **seq)
+    p, keep = A.seq_params(**seq)
+    names = [f"S{k}".encode() for k in range(S)]
+    chrs = [f"chr{k + 1}".encode() for k in range(len(w.lengths))]
+    sn = (C.c_char_p * S)(*names)
+    cn = (C.c_char_p * len(chrs))(*chrs)
+    out_dir = tempfile.mkdtemp(prefix="psx_sam_")
+    try:
+        sp = A.SamParams(out_dir.encode(), b"chr_", b"r", 0, 0, C.cast(sn, C.POINTER(C.c_char_p)),
+                         C.cast(cn, C.POINTER(C.c_char_p)))
+        n = C.c_uint64()
+        t0 = time.perf_counter()
+        check(lib().psx_write_sam(ctx.h, built.genome.h, built.forest.h, built.muts.h, C.byref(p),
+                                  C.c_uint64(w.seq_seed), C.byref(sp), C.byref(n)), ctx.h)
+        dt = time.perf_counter() - t0
+        size = sum(os.path.getsize(os.path.join(out_dir, f)) for f in os.listdir(out_dir))
+    finally:
+        shutil.rmtree(out_dir, ignore_errors=True)
+    del keep
+    r = dict(value=n.value / dt, unit="reads/s", seconds=dt, reads=n.value, bytes=size, mb_per_s=size / dt / 1e6,
+             sample=f"chromosome {c} ({int(w.lengths[c])} bases) at {coverage:g}x, {S} output samples, one SAM file",
+             note="bound by host text formatting and the file system, not the GPU")
+    log(f"sam: {n.value} reads, {size / 1e6:.0f} MB in {dt:.2f} s: {r['value']:.3e} reads/s, {r['mb_per_s']:.0f} MB/s")
+    return r
+
+
 def batched_reads_leg(ctx, B, reps: int) -> dict:
     """Read materialisation of the batched workload's leading jobs whose
     reads fit 1.6e9 rows (76.8 GB), all in one launch."""
@@ -498,6 +540,7 @@ def main() -> int:
     ap.add_argument("--serve-forms", type=str, default="continuous,one_at_a_time",
                     help="forms run on the paced stream (comma-separated: continuous, one_at_a_time)")
     ap.add_argument("--cna-only", action="store_true", help="only run the copy-number workload (for profiling)")
+    ap.add_argument("--no-sam", action="store_true", help="skip the SAM writer leg")
     ap.add_argument("--batched-place-only", action="store_true",
                     help="only run the batched workload's placements (--steps batched calls; for profiling)")
     ap.add_argument("--batched-reads-only", action="store_true",
@@ -696,6 +739,9 @@ def main() -> int:
             f"{ceil['chunks_shifted']:.0f} GB/s; the kernel's row writes {rate:.0f} GB/s = "
             f"{rate / ceil['chunks']:.1%} of the aligned chunks")
         buf.close()
+
+    if not args.no_sam:
+        out["sam"] = sam_leg(ctx, w, built)
 
     if not args.no_cna:
         out["cna_purity"] = cna_leg(ctx, 5)
