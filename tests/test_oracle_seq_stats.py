@@ -70,18 +70,23 @@ def chi2_discrete(obs_values, pmf, min_expected=5.0):
     return stats.chisquare(bins_o, bins_e).pvalue
 
 
+# The single-end case: a 20 kb genome, 100 bp reads at 30x, a star forest of
+# 10 leaves, three passengers at positions inside stratum 0, across the
+# stratum 0/1 boundary and inside stratum 1.
+SE_CASE = dict(G=20_000, L=100, C=30.0, n_leaves=10,
+               rows=[(0, 5000, 0, 3, 0, 0, 1, A.PSX_PASSENGER),    # f = 3/10 * 1/2
+                     (0, 8150, 0, 10, 1, 1, 2, A.PSX_PASSENGER),   # clonal, allele 1: f = 1/2
+                     (0, 12000, 9, 10, 0, 2, 3, A.PSX_PASSENGER)],  # one cell: f = 1/20
+               f=[0.15, 0.5, 0.05])
+
+
 @pytest.fixture(scope="module")
 def se_runs():
     """1,000 seeds of single-end 30x sequencing of a 20 kb genome."""
-    G, L, C = 20_000, 100, 30.0
+    G, L, C = SE_CASE["G"], SE_CASE["L"], SE_CASE["C"]
     lens = np.array([G], np.uint32)
-    f = star_forest(10)
-    # positions: inside stratum 0, across the stratum 0/1 boundary, inside stratum 1
-    pos = [5000, 8150, 12000]
-    rows = [(0, pos[0], 0, 3, 0, 0, 1, A.PSX_PASSENGER),   # f = 3/10 * 1/2
-            (0, pos[1], 0, 10, 1, 1, 2, A.PSX_PASSENGER),  # clonal, allele 1: f = 1/2
-            (0, pos[2], 9, 10, 0, 2, 3, A.PSX_PASSENGER)]  # one cell: f = 1/20
-    m = table(rows)
+    f = star_forest(SE_CASE["n_leaves"])
+    m = table(SE_CASE["rows"])
     out = {}
     for e in (0.0, 0.01):
         alts, deps = [], []
@@ -91,7 +96,7 @@ def se_runs():
             alts.append(a[0])
             deps.append(d[0])
         out[e] = (np.array(alts), np.array(deps), n)
-    return dict(G=G, L=L, C=C, m=m, out=out, f=[0.15, 0.5, 0.05])
+    return dict(G=G, L=L, C=C, m=m, out=out, f=SE_CASE["f"])
 
 
 def test_read_count_exact(se_runs):
@@ -100,6 +105,10 @@ def test_read_count_exact(se_runs):
 
 
 def test_depth_distribution_matches_closed_form(se_runs):
+    check_depth(se_runs)
+
+
+def check_depth(se_runs):
     G, L, C = se_runs["G"], se_runs["L"], se_runs["C"]
     T, W = round(C * G / L), G - L + 1
     alts, deps, _ = se_runs["out"][0.0]
@@ -114,6 +123,10 @@ def test_depth_distribution_matches_closed_form(se_runs):
 
 @pytest.mark.parametrize("e", [0.0, 0.01])
 def test_alt_given_depth_is_binomial(se_runs, e):
+    check_alt_given_depth(se_runs, e)
+
+
+def check_alt_given_depth(se_runs, e):
     alts, deps, _ = se_runs["out"][e]
     m = se_runs["m"]
     fs = dict(zip(m["pos"].tolist(), se_runs["f"]))
@@ -132,6 +145,10 @@ def test_alt_given_depth_is_binomial(se_runs, e):
 
 
 def test_vaf_distribution_matches_mixture(se_runs):
+    check_vaf(se_runs)
+
+
+def check_vaf(se_runs):
     G, L, C = se_runs["G"], se_runs["L"], se_runs["C"]
     T, W = round(C * G / L), G - L + 1
     alts, deps, _ = se_runs["out"][0.01]
