@@ -39,6 +39,7 @@ struct ctx_args {
     const uint32_t* scan;   // exclusive sum of cnt (list pass)
     const uint64_t* off;    // [33] list ranges (list pass)
     uint32_t* list;
+    uint32_t strand_bit;    // list entries carry the site's strand in bit 31
 };
 
 __device__ __forceinline__ uint32_t wbase(const uint32_t* words, uint32_t i) {
@@ -69,8 +70,13 @@ __global__ void __launch_bounds__(256) k_ctx_scan(const ctx_args a) {
     if (LIST && lane < 32u) run = a.scan[lane * a.n_tiles + tile] - a.scan[lane * a.n_tiles];
     for (uint32_t q0 = first; q0 < end; q0 += 64u) {
         const uint32_t q = q0 + lane;
-        uint32_t k = 32u;
-        if (q >= 1u && q + 1u < len && q < end) k = ctx_of(wbase(words, q - 1u), wbase(words, q), wbase(words, q + 1u));
+        uint32_t k = 32u, purine = 0u;
+        if (q >= 1u && q + 1u < len && q < end) {
+            const uint32_t m = wbase(words, q);
+            k = ctx_of(wbase(words, q - 1u), m, wbase(words, q + 1u));
+            purine = (m & 1u) ? 0u : 1u;
+        }
+        const uint32_t entry = (g0 + q) | ((purine & a.strand_bit) << 31);
         uint64_t pending = __ballot(k < 32u);
         while (pending) {
             const uint32_t kk = (uint32_t)__builtin_amdgcn_readlane((int)k, (int)__builtin_ctzll(pending));
@@ -79,8 +85,8 @@ __global__ void __launch_bounds__(256) k_ctx_scan(const ctx_args a) {
                 const uint32_t r0 = (uint32_t)__builtin_amdgcn_readlane((int)run, (int)kk);
                 if (k == kk) {
                     const uint32_t r = r0 + (uint32_t)__popcll(same & lt);
-                    if (a.sampling == 1u) a.list[a.off[kk] + r] = g0 + q;
-                    else if (r % a.sampling == 0u) a.list[a.off[kk] + r / a.sampling] = g0 + q;
+                    if (a.sampling == 1u) a.list[a.off[kk] + r] = entry;
+                    else if (r % a.sampling == 0u) a.list[a.off[kk] + r / a.sampling] = entry;
                 }
             }
             if (lane == kk) run += (uint32_t)__popcll(same);
@@ -172,6 +178,8 @@ const psx_sbs_index& psx_genome_sbs_index(psx_ctx* ctx, psx_genome* g, uint32_t 
     PSX_HIP(hipMemcpyAsync(ix->d_meta.p, meta, sizeof meta, hipMemcpyHostToDevice, s));
     a.off = ix->d_meta.as<uint64_t>() + 32;
     a.list = ix->d_list.as<uint32_t>();
+    ix->strand_bit = g->boff[g->n_chr] <= (uint64_t(1) << 31);
+    a.strand_bit = ix->strand_bit ? 1u : 0u;
     if (total) {
         hipLaunchKernelGGL(k_ctx_scan<true>, grid, dim3(256), 0, s, a);
         PSX_HIP(hipGetLastError());
@@ -200,6 +208,8 @@ extern "C" psx_status psx_genome_context_index(psx_ctx* ctx, psx_genome* g, uint
                                                  (unsigned long long)ix.off[32], (unsigned long long)list_cap);
             if (ix.off[32])
                 PSX_HIP(hipMemcpy(list, ix.d_list.p, ix.off[32] * 4, hipMemcpyDeviceToHost));
+            if (ix.strand_bit)  // positions only
+                for (uint64_t i = 0; i < ix.off[32]; ++i) list[i] &= 0x7FFFFFFFu;
         }
     });
 }

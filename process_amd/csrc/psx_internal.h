@@ -144,6 +144,10 @@ struct psx_sbs_index {
     dbuf d_list;
     dbuf d_meta;
     double build_ms = 0.0;
+    // with genomes below 2^31 bases, bit 31 of each list entry holds the
+    // site's strand (set: the reference base there is G or A, the purine), so
+    // a draw needs no read of the reference
+    bool strand_bit = false;
 };
 
 struct psx_genome {

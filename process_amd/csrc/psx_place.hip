@@ -204,6 +204,7 @@ struct place_args {
     const uint64_t* id_list;
     // SNV sites: the context index at the engine's sampling and the driver loci (global, ascending, unique)
     const uint32_t* sbs_list;
+    uint32_t sbs_strand;      // list entries carry the site's strand in bit 31
     const uint64_t* sbs_kept;   // [32]
     const uint64_t* sbs_off;    // [33]
     const uint64_t* drv_pos;
@@ -242,11 +243,14 @@ __device__ uint32_t draw_snv(const place_args& a, uint32_t v, uint32_t k, uint32
     const uint32_t* list = a.sbs_list + a.sbs_off[ctx];
     for (uint32_t t = 0; t < LIST_TRIES && n; ++t) {
         const psx_u4 ws = psx_draw(a.seed, k, v, att, (PSX_P_SITE << 24) | t);
-        const uint32_t gp = list[psx_uniform64(ws.x, ws.y, n)];
+        const uint32_t en = list[psx_uniform64(ws.x, ws.y, n)];
+        const uint32_t gp = a.sbs_strand ? en & 0x7FFFFFFFu : en;
         if (a.n_drv_pos && driver_locus(a, gp)) continue;
         uint32_t c, pos;
         glocate(a.g, gp, c, pos);
-        const uint32_t m = gbase(a.g, c, pos);
+        // the site's base: C or G for the C contexts, T or A for the T ones (the strand bit tells which)
+        const uint32_t m = a.sbs_strand ? (ctx < 16u ? ((en >> 31) ? 2u : 1u) : ((en >> 31) ? 0u : 3u))
+                                        : gbase(a.g, c, pos);
         r.chr = c; r.pos = pos; r.kind = PSX_SNV; r.ref = (uint8_t)m;
         r.alt = (uint8_t)((m == 1u || m == 3u) ? tgt : 3u - tgt);
         r.len = 0; r.seq = 0; r.allele = (uint8_t)allele; r.cause = (uint16_t)sig;
@@ -1602,6 +1606,7 @@ void place_batch(psx_ctx* ctx, const psx_genome* g, std::vector<place_job>& jobs
         pa.M = J.M; pa.recs = d_recs + J.mbase; pa.attempt = d_att + J.mbase; pa.status = d_status + k;
         if (J.sidx) {
             pa.sbs_list = J.sidx->d_list.as<uint32_t>();
+            pa.sbs_strand = J.sidx->strand_bit ? 1u : 0u;
             pa.sbs_kept = J.sidx->d_meta.as<uint64_t>();
             pa.sbs_off = J.sidx->d_meta.as<uint64_t>() + 32;
         }
