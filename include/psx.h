@@ -80,6 +80,12 @@ psx_status psx_ctx_synchronize(psx_ctx* ctx);
 psx_status psx_ctx_last_kernel_ms(const psx_ctx* ctx, double* ms);
 /* Free and total HBM of the context device, in bytes. */
 psx_status psx_ctx_mem_info(psx_ctx* ctx, uint64_t* free_bytes, uint64_t* total_bytes);
+/* Frees the buffers a context keeps grown between calls (placement
+ * temporaries, psx_simulate_seq's device tables, launch descriptors, pinned
+ * staging) after waiting for its stream; later calls grow them again. No
+ * counterpart in the reference: for long-lived processes (an R session) that
+ * ran one large call and want the memory back. */
+psx_status psx_ctx_trim(psx_ctx* ctx);
 
 /* Device buffers on the context device for callers that have no allocator of
  * their own in this process (no reference counterpart; plumbing for the

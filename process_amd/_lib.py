@@ -37,6 +37,7 @@ def _declare(lib):
         "psx_ctx_last_place_ms": (st, [vp, C.POINTER(C.c_double)]),
         "psx_ctx_cancel": (st, [vp]),
         "psx_ctx_mem_info": (st, [vp, A.u64p, A.u64p]),
+        "psx_ctx_trim": (st, [vp]),
         "psx_dev_alloc": (st, [vp, C.c_uint64, pp]),
         "psx_dev_free": (None, [vp, vp]),
         "psx_dev_upload": (st, [vp, vp, vp, C.c_uint64]),

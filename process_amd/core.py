@@ -65,6 +65,11 @@ class Context:
         check(lib().psx_ctx_mem_info(self.h, C.byref(f), C.byref(t)), self.h)
         return f.value, t.value
 
+    def trim(self) -> None:
+        """Frees the buffers the context keeps grown between calls; later calls
+        grow them again (psx_ctx_trim)."""
+        check(lib().psx_ctx_trim(self.h), self.h)
+
 
 class DeviceArray:
     """A device buffer owned through libpsx (psx_dev_alloc), with numpy copies.

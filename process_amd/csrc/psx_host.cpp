@@ -51,6 +51,25 @@ extern "C" psx_status psx_ctx_create(int device, psx_ctx** out) {
     return PSX_OK;
 }
 
+extern "C" psx_status psx_ctx_trim(psx_ctx* ctx) {
+    if (!ctx) return PSX_EINVAL;
+    return psx_guard(ctx, [&] {
+        PSX_HIP(hipSetDevice(ctx->device));
+        PSX_HIP(hipStreamSynchronize(ctx->stream));
+        ctx->scratch.release();
+        ctx->place_nodes.release();
+        ctx->place_muts.release();
+        ctx->place_tab.release();
+        ctx->seq_out.release();
+        if (ctx->host_io) PSX_HIP(hipHostFree(ctx->host_io));
+        ctx->host_io = nullptr;
+        ctx->host_io_bytes = 0;
+        if (ctx->host_stage) PSX_HIP(hipHostFree(ctx->host_stage));
+        ctx->host_stage = nullptr;
+        ctx->host_stage_bytes = 0;
+    });
+}
+
 extern "C" void psx_ctx_destroy(psx_ctx* ctx) {
     if (!ctx) return;
     if (ctx->stream) {

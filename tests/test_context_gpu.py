@@ -152,3 +152,23 @@ def test_engine_default_sampling(ctx):
     assert np.isin(boff[snv["chr"].astype(np.int64)] + snv["pos"], ix["list"].astype(np.int64)).all()
     with pytest.raises(ValueError, match="context_sampling must be a positive integer"):
         pa.MutationEngine(ref, {"SBS1": sig[0]}, context_sampling=0)
+
+
+def test_trim_keeps_results(ctx):
+    """psx_ctx_trim frees the context's grown buffers; placement and
+    sequencing after it give the same tables as before it."""
+    lens = [30_000, 12_000]
+    g = core.Genome(ctx, lens, seed=8)
+    fd = cases.forest(n_cells=300, n_samples=2, sample_size=50, seed=6)
+    f = core.Forest(ctx, fd["cell_id"], fd["parent"], fd["species"], fd["sample"], fd["birth"], fd["n_samples"])
+    eng = cases.engine(g.download(), lens, snv=2e-5, indel=2e-6)
+    kw = dict(coverage=30, read_size=150, insert_size_mean=350, insert_size_stddev=10, error_rate=1e-3)
+    m1 = core.Mutations.place(ctx, g, f, 4, **eng)
+    a1 = core.simulate_seq_arrays(ctx, g, f, m1, 9, **kw)
+    ctx.trim()
+    m2 = core.Mutations.place(ctx, g, f, 4, **eng)
+    a2 = core.simulate_seq_arrays(ctx, g, f, m2, 9, **kw)
+    assert m1.download().tobytes() == m2.download().tobytes() and m1.n > 20
+    assert all(np.array_equal(x, y) for x, y in zip(a1[:3], a2[:3])) and a1[3] == a2[3]
+    ctx.trim()
+    ctx.trim()  # nothing left to free
