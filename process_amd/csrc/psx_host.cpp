@@ -244,13 +244,15 @@ void psx_copy_out(psx_ctx* ctx, const std::vector<psx_copy_range>& ranges) {
             err[t] = std::current_exception();
         }
     };
+    std::vector<std::thread> th;
     try {
         for (unsigned t = 0; t < T; ++t) PSX_HIP(hipEventCreateWithFlags(&ev[t], hipEventDisableTiming));
-        std::vector<std::thread> th;
         for (unsigned t = 1; t < T; ++t) th.emplace_back(work, t);
         work(0);
         for (std::thread& x : th) x.join();
-    } catch (...) {
+    } catch (...) {  // a thread that could not start: the started ones finish first
+        for (std::thread& x : th)
+            if (x.joinable()) x.join();
         for (hipEvent_t e : ev)
             if (e) (void)hipEventDestroy(e);
         throw;
