@@ -1,9 +1,9 @@
 """Batched placement (psx_place_mutations_batch): many forests over one
 genome in one call, each table byte for byte the single-forest placement's
 and the oracle's, with the infinite-sites model on and off, ID signatures,
-copy-number drivers, a coarse context sampling, an empty table and jobs whose
-redraw rounds outlast the others'; a bad job fails the batch with its index
-in the message."""
+copy-number drivers, a coarse context sampling, an empty table, jobs whose
+redraw rounds outlast the others' and many tiny jobs sharing waves; a bad job
+fails the batch with its index in the message."""
 from __future__ import annotations
 
 import numpy as np
@@ -94,3 +94,32 @@ def test_batch_error_names_the_job(ctx):
     es[0]["ploidy"] = 0
     with pytest.raises(PsxError, match="job 0: invalid ploidy"):
         core.Mutations.place_batch(ctx, g, [f for f, _, _ in fs], [1, 2, 3], es)
+
+
+def test_many_tiny_jobs_share_waves(ctx):
+    """48 forests of 12-60 cells: each job has a few to tens of mutations and
+    nodes, so most waves of every per-node and per-mutation kernel hold
+    several jobs (one pass per job in the wave); every table equals the
+    single-forest placement, and every eighth the oracle's."""
+    g = core.Genome(ctx, LENS, seed=21)
+    packed = g.download()
+    rng = np.random.default_rng(9)
+    fs, es, seeds = [], [], []
+    for k in range(48):
+        n = int(rng.integers(12, 61))
+        fd = cases.forest(n_cells=n, n_samples=1, sample_size=max(2, n // 4), seed=100 + k)
+        f = core.Forest(ctx, fd["cell_id"], fd["parent"], fd["species"], fd["sample"], fd["birth"], fd["n_samples"])
+        of = O.Forest(fd["cell_id"], fd["parent"], fd["species"], fd["sample"], fd["birth"], fd["n_samples"])
+        fs.append((f, of))
+        es.append(cases.engine(packed, LENS, snv=float(rng.uniform(2e-6, 2e-5)), indel=2e-7, drivers=k % 3 == 0,
+                               germline=k % 4 == 0, infinite_sites=k % 5 != 0))
+        seeds.append(500 + k)
+    batch = core.Mutations.place_batch(ctx, g, [f for f, _ in fs], seeds, es)
+    sizes = []
+    for k, ((f, of), eng, sd) in enumerate(zip(fs, es, seeds)):
+        got = batch[k].download()
+        sizes.append(len(got))
+        assert got.tobytes() == core.Mutations.place(ctx, g, f, sd, **eng).download().tobytes(), k
+        if k % 8 == 0:
+            assert got.tobytes() == O.place_mutations(LENS, packed, of, sd, **eng).tobytes(), k
+    assert sum(sizes) > 48 * 5 and min(sizes) < 64
